@@ -1,0 +1,161 @@
+"""Benchmark of the plain perceptual-analysis path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 4032x3024]
+
+A step is one pass of the path over one synthetic 8-bit sRGB image whose
+packed pixels are already resident in HBM when the timed region starts.
+Prints one JSON line (rank 0).  Per-stage kernel time is measured with HIP
+events on the library's own stream; `roofline` describes the dominant stage.
+`cpu_baseline` times the oracle (the CPU restatement) on the host on a bounded
+crop of the same image.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def stage_bytes(name: str, w: int, h: int) -> int:
+    """Algorithmic HBM bytes of one launch: declared inputs read once and
+    declared outputs written once."""
+    n = w * h
+    pn = ((w + 7) // 8 * 8) * ((h + 7) // 8 * 8)
+    if name == "srgb_to_linear_and_opsin":
+        return 3 * n + 12 * n + 12 * pn
+    raise KeyError(name)
+
+
+def run_step(dev, bufs, w, h):
+    """Enqueues one pass; records event slot i+1 after stage i."""
+    import gjxl_amd
+
+    dev.record(0)
+    dev.srgb8_to_linear_and_opsin(bufs["srgb"], w, h, linear=bufs["linear"],
+                                  opsin=bufs["opsin"])
+    dev.record(1)
+    return ["srgb_to_linear_and_opsin"]
+
+
+def cpu_baseline(srgb: np.ndarray, budget_s: float = 10.0) -> dict:
+    """Times the oracle's stages on the host on a crop of the image."""
+    from oracle import binding
+
+    h, w = srgb.shape[:2]
+    ch, cw = min(h, 1024), min(w, 1024)
+    crop = np.ascontiguousarray(srgb[:ch, :cw])
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        lin = binding.srgb8_to_linear(crop)
+        binding.linear_to_padded_opsin(lin)
+        done += 1
+        if time.perf_counter() - t0 > budget_s or done >= 50:
+            break
+    elapsed = time.perf_counter() - t0
+    return {
+        "value": round(done * cw * ch / 1e6 / elapsed, 4),
+        "unit": "MP/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} passes over a {cw}x{ch} crop of the same image, "
+                  "same stages as the GPU step, single thread",
+    }
+
+
+def main() -> int:
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=50)
+    parser.add_argument("--warmup", type=int, default=5)
+    parser.add_argument("--workload", default="4032x3024")
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    args = parser.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU "
+              "path (images and tiles are independent; see DESIGN.md for how "
+              "the path would shard).", file=sys.stderr)
+        return 2
+
+    import gjxl_amd
+    from gjxl_amd import synthetic
+
+    spec = synthetic.WORKLOADS[args.workload]
+    w, h = spec["width"], spec["height"]
+    srgb = synthetic.generate(**spec)
+    dev = gjxl_amd.Device(0)
+    bufs = {
+        "srgb": dev.array(srgb),
+        "linear": dev.empty((3, h, w)),
+        "opsin": dev.empty((3, gjxl_amd.pad8(h), gjxl_amd.pad8(w))),
+    }
+
+    for _ in range(args.warmup):
+        run_step(dev, bufs, w, h)
+    dev.synchronize()
+
+    stage_ms = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        names = run_step(dev, bufs, w, h)
+        dev.synchronize()
+        for i, name in enumerate(names):
+            stage_ms.setdefault(name, []).append(dev.elapsed_ms(i, i + 1))
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    mp = w * h / 1e6
+    avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
+    dominant = max(avg, key=avg.get)
+    nbytes = stage_bytes(dominant, w, h)
+    achieved = nbytes / (avg[dominant] * 1e-3) / 1e9
+    result = {
+        "metric": "MP/s at d1.0 e7, plain GPU path (stages built so far), 1 GPU",
+        "value": round(mp / (ms_per_step * 1e-3), 3),
+        "unit": "MP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w}x{h} 8-bit sRGB, seeded synthetic "
+                        f"({spec['recipe']}), distance 1.0 effort 7",
+            "stages": list(avg),
+            "stage_ms": {k: round(v, 5) for k, v in avg.items()},
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(srgb)
+    dev.close()
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

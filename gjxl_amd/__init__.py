@@ -1,0 +1,223 @@
+"""gjxl_amd: MI355X-native perceptual-analysis path of the gjxl encoder.
+
+Python host mirror over the C ABI in include/gjxl_amd.h.  The library
+(libgjxl_amd.so, built by gjxl_amd.build) holds every kernel and the C++
+host orchestration; this module only binds it with ctypes and moves numpy
+arrays to and from device memory.  There is no CPU fallback: if the library
+is missing or no HIP device is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+LIBRARY_PATH = PACKAGE_DIR / "libgjxl_amd.so"
+
+GJXL_OK = 0
+_ERROR_NAMES = {
+    1: "INVALID_ARGUMENT", 2: "UNSUPPORTED", 3: "UNAVAILABLE",
+    4: "OUT_OF_MEMORY", 5: "BACKEND", 6: "INTERNAL",
+    7: "RESOURCE_PLAN_EXCEEDED",
+}
+
+
+class GjxlError(RuntimeError):
+    """A non-OK GJXLResult, with the library's thread-local diagnostic."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libgjxl_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIBRARY_PATH.exists():
+            raise RuntimeError(
+                f"{LIBRARY_PATH} is missing: run `python -m gjxl_amd.build`")
+        _lib = ctypes.CDLL(str(LIBRARY_PATH))
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SZ = ctypes.c_size_t
+_F = ctypes.c_float
+_I = ctypes.c_int
+
+# name -> argument types (all return GJXLResult unless listed in _VOID/_STR)
+_SIGNATURES = {
+    "gjxl_amd_device_create": [_I, ctypes.POINTER(_P)],
+    "gjxl_amd_device_destroy": [_P],
+    "gjxl_amd_alloc": [_P, _SZ, ctypes.POINTER(_P)],
+    "gjxl_amd_free": [_P, _P],
+    "gjxl_amd_upload": [_P, _P, _P, _SZ],
+    "gjxl_amd_download": [_P, _P, _P, _SZ],
+    "gjxl_amd_synchronize": [_P],
+    "gjxl_amd_last_error": [],
+    "gjxl_amd_event_record": [_P, _I],
+    "gjxl_amd_event_elapsed": [_P, _I, _I, ctypes.POINTER(_F)],
+    "gjxl_amd_srgb8_to_linear_and_opsin":
+        [_P, _P, _SZ, _U32, _U32, _U32, _F, _P, _P],
+    "gjxl_amd_linear_rgb_to_padded_opsin": [_P, _P, _U32, _U32, _F, _P],
+}
+_VOID = {"gjxl_amd_device_destroy"}
+_STR = {"gjxl_amd_last_error"}
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        if name in _VOID:
+            fn.restype = None
+        elif name in _STR:
+            fn.restype = ctypes.c_char_p
+        else:
+            fn.restype = ctypes.c_int32
+
+
+def _check(code: int) -> None:
+    if code != GJXL_OK:
+        msg = library().gjxl_amd_last_error()
+        raise GjxlError(code, msg.decode() if msg else "")
+
+
+def pad8(v: int) -> int:
+    return (v + 7) // 8 * 8
+
+
+class DeviceArray:
+    """A device allocation with a numpy shape and dtype."""
+
+    def __init__(self, device: "Device", shape, dtype):
+        self.device = device
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = device._alloc(max(self.nbytes, 1))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(library().gjxl_amd_download(
+                self.device.handle, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, array: np.ndarray) -> "DeviceArray":
+        array = np.ascontiguousarray(array, dtype=self.dtype)
+        if array.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.nbytes:
+            _check(library().gjxl_amd_upload(
+                self.device.handle, self.ptr, array.ctypes.data, self.nbytes))
+        return self
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(library().gjxl_amd_free(self.device.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.device.handle:
+                library().gjxl_amd_free(self.device.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class Device:
+    """One HIP device and its stream (gjxl_amd_device_create)."""
+
+    def __init__(self, ordinal: int = 0):
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_device_create(ordinal, ctypes.byref(handle)))
+        self.handle = handle.value
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_device_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _alloc(self, nbytes: int) -> int:
+        ptr = ctypes.c_void_p()
+        _check(library().gjxl_amd_alloc(self.handle, nbytes, ctypes.byref(ptr)))
+        return ptr.value
+
+    def empty(self, shape, dtype=np.float32) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def array(self, array: np.ndarray, dtype=None) -> DeviceArray:
+        array = np.asarray(array)
+        d = DeviceArray(self, array.shape, dtype or array.dtype)
+        return d.copy_from(array)
+
+    def synchronize(self) -> None:
+        _check(library().gjxl_amd_synchronize(self.handle))
+
+    def record(self, slot: int) -> None:
+        _check(library().gjxl_amd_event_record(self.handle, slot))
+
+    def elapsed_ms(self, begin: int, end: int) -> float:
+        ms = ctypes.c_float()
+        _check(library().gjxl_amd_event_elapsed(
+            self.handle, begin, end, ctypes.byref(ms)))
+        return ms.value
+
+    # ---- a1 + a2 --------------------------------------------------------
+    def srgb8_to_linear_and_opsin(self, srgb: DeviceArray, width: int,
+                                  height: int, bytes_per_pixel: int = 3,
+                                  row_stride: int | None = None,
+                                  intensity_target: float = 255.0,
+                                  linear: DeviceArray | None = None,
+                                  opsin: DeviceArray | None = None):
+        row_stride = row_stride or width * bytes_per_pixel
+        linear = linear or self.empty((3, height, width))
+        opsin = opsin or self.empty((3, pad8(height), pad8(width)))
+        _check(library().gjxl_amd_srgb8_to_linear_and_opsin(
+            self.handle, srgb.ptr, row_stride, bytes_per_pixel, width, height,
+            intensity_target, linear.ptr, opsin.ptr))
+        return linear, opsin
+
+    def linear_rgb_to_padded_opsin(self, linear: DeviceArray, width: int,
+                                   height: int, intensity_target: float = 255.0,
+                                   opsin: DeviceArray | None = None):
+        opsin = opsin or self.empty((3, pad8(height), pad8(width)))
+        _check(library().gjxl_amd_linear_rgb_to_padded_opsin(
+            self.handle, linear.ptr, width, height, intensity_target, opsin.ptr))
+        return opsin
+
+
+def gpu_available() -> bool:
+    """True when a HIP device can be opened (does not fall back to CPU)."""
+    if os.environ.get("GJXL_AMD_NO_GPU"):
+        return False
+    try:
+        count = ctypes.c_int()
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        return hip.hipGetDeviceCount(ctypes.byref(count)) == 0 and count.value > 0
+    except OSError:
+        return False
+
+
+from . import pipeline  # noqa: E402,F401  (stage sequencing over this API)

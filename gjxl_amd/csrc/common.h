@@ -1,0 +1,114 @@
+// Shared host/device plumbing for the gjxl_amd HIP library (gfx950).
+//
+// Every translation unit is compiled with -ffp-contract=off: a*b+c is two
+// roundings unless the code says __builtin_fmaf, matching the reference's
+// CPU arithmetic (which has no FMA contraction on baseline x86-64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gjxl_amd.h"
+
+namespace gjxl_amd {
+
+// ---- errors ------------------------------------------------------------
+
+// Records this thread's diagnostic and returns `code`.
+GJXLResult Fail(GJXLResult code, const std::string& message);
+// Maps a HIP error to GJXL_ERROR_BACKEND with context; GJXL_OK on success.
+GJXLResult Check(hipError_t error, const char* what);
+// Checks the launch that was just enqueued.
+GJXLResult CheckLaunch(const char* kernel);
+
+#define GJXL_AMD_TRY(expr)                   \
+  do {                                       \
+    const GJXLResult gjxl_amd_result_ = (expr); \
+    if (gjxl_amd_result_ != GJXL_OK) return gjxl_amd_result_; \
+  } while (0)
+
+// ---- device handle ------------------------------------------------------
+
+}  // namespace gjxl_amd
+
+struct GJXLAmdDevice {
+  int ordinal = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[GJXL_AMD_EVENT_SLOTS] = {};
+};
+
+namespace gjxl_amd {
+
+GJXLResult ValidateDevice(const GJXLAmdDevice* device);
+
+inline uint32_t CeilDiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+inline uint32_t PadTo8(uint32_t v) { return CeilDiv(v, 8) * 8; }
+
+// ---- device math --------------------------------------------------------
+
+__device__ __forceinline__ uint32_t FloatBits(float v) {
+  return __float_as_uint(v);
+}
+__device__ __forceinline__ float BitsFloat(uint32_t u) {
+  return __uint_as_float(u);
+}
+
+// libjxl CubeRootAndAdd: exponent-bit seed for 1/cbrt, three Newton steps and
+// one refinement, then r^2 * v + add (src/codec/color_transform.cpp:54-79).
+__device__ __forceinline__ float FastCubeRootAndAdd(float value, float add) {
+  const float one_third = 1.0f / 3.0f;
+  const float four_thirds = 4.0f / 3.0f;
+  const uint32_t bits = FloatBits(value);
+  const uint32_t seed =
+    bits == 0 ? 0u : 0x54800000u - (bits >> 23) * 0x002AAAAAu;
+  float r = BitsFloat(seed);
+  const float divided = one_third * value;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float sq = r * r;
+    r = __builtin_fmaf(-divided, sq * sq, four_thirds * r);
+  }
+  float sq = r * r;
+  r = __builtin_fmaf(one_third, __builtin_fmaf(-value, sq * sq, r), r);
+  sq = r * r;
+  return __builtin_fmaf(sq, value, add);
+}
+
+// libjxl FastLog2f rational polynomial (src/util/fast_math.h:14-38); the
+// caller guarantees a finite positive input.
+__device__ __forceinline__ float FastLog2(float value) {
+  const uint32_t bits = FloatBits(value);
+  const int32_t e = static_cast<int32_t>(bits - 0x3f2aaaabu) >> 23;
+  const float x = BitsFloat(bits - (static_cast<uint32_t>(e) << 23)) - 1.0f;
+  float num = __builtin_fmaf(0.74245873327820566f, x, 1.4287160470083755f);
+  num = __builtin_fmaf(num, x, -1.8503833400518310e-06f);
+  float den = __builtin_fmaf(0.17409343003366853f, x, 1.0096718572241148f);
+  den = __builtin_fmaf(den, x, 0.99032814277590719f);
+  return num / den + static_cast<float>(e);
+}
+
+// libjxl FastPow2f rational polynomial (src/util/fast_math.h:40-70); the
+// caller guarantees an input in [-126, 127].
+__device__ __forceinline__ float FastPow2(float value) {
+  const float fl = floorf(value);
+  const int32_t e = static_cast<int32_t>(fl) + 127;
+  const float ev = BitsFloat(static_cast<uint32_t>(e) << 23);
+  const float f = value - fl;
+  float num = f + 1.01749063e+01f;
+  num = __builtin_fmaf(num, f, 4.88687798e+01f);
+  num = __builtin_fmaf(num, f, 9.85506591e+01f);
+  num *= ev;
+  float den = __builtin_fmaf(f, 2.10242958e-01f, -2.22328856e-02f);
+  den = __builtin_fmaf(den, f, -1.94414990e+01f);
+  den = __builtin_fmaf(den, f, 9.85506633e+01f);
+  return num / den;
+}
+
+__device__ __forceinline__ float FastPow(float base, float exponent) {
+  return FastPow2(FastLog2(base) * exponent);
+}
+
+}  // namespace gjxl_amd

@@ -1,0 +1,158 @@
+// Device handle, memory, copies, events and thread-local diagnostics.
+// Replaces the allocation/copy/submission part of GpuBackend
+// (src/gpu/backend.h:24-105) for the perceptual-analysis path.
+#include <cstdio>
+#include <new>
+#include <string>
+
+#include "common.h"
+
+namespace gjxl_amd {
+namespace {
+thread_local std::string g_last_error;
+}  // namespace
+
+GJXLResult Fail(GJXLResult code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+GJXLResult Check(hipError_t error, const char* what) {
+  if (error == hipSuccess) return GJXL_OK;
+  const GJXLResult code = error == hipErrorOutOfMemory
+    ? GJXL_ERROR_OUT_OF_MEMORY : GJXL_ERROR_BACKEND;
+  return Fail(code, std::string(what) + ": " + hipGetErrorString(error));
+}
+
+GJXLResult CheckLaunch(const char* kernel) {
+  return Check(hipGetLastError(), kernel);
+}
+
+GJXLResult ValidateDevice(const GJXLAmdDevice* device) {
+  if (device == nullptr || device->stream == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Device handle is null");
+  }
+  return Check(hipSetDevice(device->ordinal), "hipSetDevice");
+}
+
+}  // namespace gjxl_amd
+
+using gjxl_amd::Check;
+using gjxl_amd::Fail;
+
+extern "C" {
+
+GJXLResult gjxl_amd_device_create(int ordinal, GJXLAmdDevice** out) {
+  if (out == nullptr || *out != nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Device output must be non-null and empty");
+  }
+  int count = 0;
+  GJXL_AMD_TRY(Check(hipGetDeviceCount(&count), "hipGetDeviceCount"));
+  if (ordinal < 0 || ordinal >= count) {
+    return Fail(GJXL_ERROR_UNAVAILABLE, "HIP device ordinal is unavailable");
+  }
+  GJXL_AMD_TRY(Check(hipSetDevice(ordinal), "hipSetDevice"));
+  auto* device = new (std::nothrow) GJXLAmdDevice;
+  if (device == nullptr) {
+    return Fail(GJXL_ERROR_OUT_OF_MEMORY, "Unable to allocate device handle");
+  }
+  device->ordinal = ordinal;
+  GJXLResult result = Check(
+    hipStreamCreateWithFlags(&device->stream, hipStreamNonBlocking),
+    "hipStreamCreateWithFlags");
+  for (int i = 0; result == GJXL_OK && i < GJXL_AMD_EVENT_SLOTS; ++i) {
+    result = Check(hipEventCreate(&device->events[i]), "hipEventCreate");
+  }
+  if (result != GJXL_OK) {
+    gjxl_amd_device_destroy(device);
+    return result;
+  }
+  *out = device;
+  return GJXL_OK;
+}
+
+void gjxl_amd_device_destroy(GJXLAmdDevice* device) {
+  if (device == nullptr) return;
+  (void)hipSetDevice(device->ordinal);
+  if (device->stream != nullptr) (void)hipStreamSynchronize(device->stream);
+  for (hipEvent_t& event : device->events) {
+    if (event != nullptr) (void)hipEventDestroy(event);
+  }
+  if (device->stream != nullptr) (void)hipStreamDestroy(device->stream);
+  delete device;
+}
+
+GJXLResult gjxl_amd_alloc(GJXLAmdDevice* device, size_t bytes, void** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || bytes == 0) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Allocation size or output is invalid");
+  }
+  *out = nullptr;
+  return Check(hipMalloc(out, bytes), "hipMalloc");
+}
+
+GJXLResult gjxl_amd_free(GJXLAmdDevice* device, void* pointer) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (pointer == nullptr) return GJXL_OK;
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize"));
+  return Check(hipFree(pointer), "hipFree");
+}
+
+GJXLResult gjxl_amd_upload(GJXLAmdDevice* device, void* dst, const void* src,
+                           size_t bytes) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (bytes == 0) return GJXL_OK;
+  if (dst == nullptr || src == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Upload pointer is null");
+  }
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                    device->stream), "hipMemcpyAsync(H2D)"));
+  return Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+}
+
+GJXLResult gjxl_amd_download(GJXLAmdDevice* device, void* dst, const void* src,
+                             size_t bytes) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (bytes == 0) return GJXL_OK;
+  if (dst == nullptr || src == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Download pointer is null");
+  }
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                                    device->stream), "hipMemcpyAsync(D2H)"));
+  return Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+}
+
+GJXLResult gjxl_amd_synchronize(GJXLAmdDevice* device) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  return Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+}
+
+const char* gjxl_amd_last_error(void) {
+  return gjxl_amd::g_last_error.c_str();
+}
+
+GJXLResult gjxl_amd_event_record(GJXLAmdDevice* device, int slot) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (slot < 0 || slot >= GJXL_AMD_EVENT_SLOTS) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Event slot is out of range");
+  }
+  return Check(hipEventRecord(device->events[slot], device->stream),
+               "hipEventRecord");
+}
+
+GJXLResult gjxl_amd_event_elapsed(GJXLAmdDevice* device, int begin_slot,
+                                  int end_slot, float* milliseconds) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (begin_slot < 0 || begin_slot >= GJXL_AMD_EVENT_SLOTS || end_slot < 0 ||
+      end_slot >= GJXL_AMD_EVENT_SLOTS || milliseconds == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Event slot is out of range");
+  }
+  GJXL_AMD_TRY(Check(hipEventSynchronize(device->events[end_slot]),
+                     "hipEventSynchronize"));
+  return Check(hipEventElapsedTime(milliseconds, device->events[begin_slot],
+                                   device->events[end_slot]),
+               "hipEventElapsedTime");
+}
+
+}  // extern "C"

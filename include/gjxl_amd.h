@@ -1,0 +1,103 @@
+/*
+ * gjxl_amd: MI355X (gfx950) device layer for gjxl's perceptual-analysis path.
+ *
+ * A C ABI with plain pointers and sizes.  Device images are planar binary32:
+ * plane c of an image with row stride S and height H starts at
+ * base + c * S * H.  Padded ("coding") images have width and height rounded
+ * up to multiples of 8; block fields have one value per 8x8 block.
+ *
+ * Every call returns a GJXLResult (codes as in gjxl.h:34-45).  On failure
+ * gjxl_amd_last_error() returns this thread's diagnostic, valid until the next
+ * failing call on the thread (gjxl.h:247-248 contract).  Stage functions
+ * enqueue on the device handle's stream and return without waiting;
+ * gjxl_amd_synchronize() waits and reports asynchronous kernel errors.
+ *
+ * Each stage entry point names the reference interface it replaces
+ * (paths relative to the gjxl source tree).
+ */
+#ifndef GJXL_AMD_H_
+#define GJXL_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define GJXL_AMD_API __attribute__((visibility("default")))
+#else
+#define GJXL_AMD_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef GJXL_GJXL_H_
+typedef int32_t GJXLResult;
+enum {
+  GJXL_OK = 0,
+  GJXL_ERROR_INVALID_ARGUMENT = 1,
+  GJXL_ERROR_UNSUPPORTED = 2,
+  GJXL_ERROR_UNAVAILABLE = 3,
+  GJXL_ERROR_OUT_OF_MEMORY = 4,
+  GJXL_ERROR_BACKEND = 5,
+  GJXL_ERROR_INTERNAL = 6,
+  GJXL_ERROR_RESOURCE_PLAN_EXCEEDED = 7,
+};
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Device handle: one HIP device and one stream.                           */
+/* Replaces GpuBackend (src/gpu/backend.h:24-105) for this path: allocation, */
+/* copies and submission; CudaBackend binds one device_ordinal the same way */
+/* (src/gpu/cuda/cuda_backend.cpp:1043-1059).                                */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdDevice GJXLAmdDevice;
+
+GJXL_AMD_API GJXLResult gjxl_amd_device_create(int ordinal,
+                                               GJXLAmdDevice** out);
+GJXL_AMD_API void gjxl_amd_device_destroy(GJXLAmdDevice* device);
+GJXL_AMD_API GJXLResult gjxl_amd_alloc(GJXLAmdDevice* device, size_t bytes,
+                                       void** out);
+GJXL_AMD_API GJXLResult gjxl_amd_free(GJXLAmdDevice* device, void* pointer);
+/* Synchronous copies on the device stream. */
+GJXL_AMD_API GJXLResult gjxl_amd_upload(GJXLAmdDevice* device, void* dst,
+                                        const void* src, size_t bytes);
+GJXL_AMD_API GJXLResult gjxl_amd_download(GJXLAmdDevice* device, void* dst,
+                                          const void* src, size_t bytes);
+GJXL_AMD_API GJXLResult gjxl_amd_synchronize(GJXLAmdDevice* device);
+GJXL_AMD_API const char* gjxl_amd_last_error(void);
+
+/* Timing on the device stream (HIP events), for measurement.  Slots are
+ * 0..GJXL_AMD_EVENT_SLOTS-1; elapsed time is in milliseconds. */
+enum { GJXL_AMD_EVENT_SLOTS = 64 };
+GJXL_AMD_API GJXLResult gjxl_amd_event_record(GJXLAmdDevice* device,
+                                              int slot);
+GJXL_AMD_API GJXLResult gjxl_amd_event_elapsed(GJXLAmdDevice* device,
+                                               int begin_slot, int end_slot,
+                                               float* milliseconds);
+
+/* ---------------------------------------------------------------------- */
+/* a1 + a2: packed sRGB8 -> linear RGB planes and padded XYB planes.        */
+/* Replaces PrepareResidentInput (src/gpu/cuda/cuda_input_preparation.cpp:  */
+/* 299): ConvertValidatedPackedSrgbInto (src/c_api/image_conversion.cpp:    */
+/* 123-145) fused with LinearRgbToPaddedOpsin (src/codec/color_transform.cpp*/
+/* :395-431).  srgb: device copy of the packed pixels (3 or 4 bytes/pixel,   */
+/* alpha ignored after host validation).  linear: 3*width*height floats.    */
+/* opsin: 3*pw*ph floats, pw = ceil8(width), ph = ceil8(height).            */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_srgb8_to_linear_and_opsin(
+  GJXLAmdDevice* device, const uint8_t* srgb, size_t row_stride_bytes,
+  uint32_t bytes_per_pixel, uint32_t width, uint32_t height,
+  float intensity_target, float* linear, float* opsin);
+
+/* a2 alone: LinearRgbToPaddedOpsin (src/codec/color_transform_internal.h:17,
+ * src/codec/color_transform.cpp:395-431).  linear: 3*width*height floats. */
+GJXL_AMD_API GJXLResult gjxl_amd_linear_rgb_to_padded_opsin(
+  GJXLAmdDevice* device, const float* linear, uint32_t width,
+  uint32_t height, float intensity_target, float* opsin);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* GJXL_AMD_H_ */

@@ -1,0 +1,118 @@
+"""ctypes binding of the CPU checker.  TEST INFRASTRUCTURE ONLY.
+
+``oracle()`` loads liboracle.so (the plain-C restatement in this directory);
+``reference()`` loads _ref/libgjxlref.so (the reference encoder's own CPU
+sources compiled in place, plus ref_harness.cpp) and returns None when it has
+not been built.  Only tests/, the smoke check and the CPU-baseline leg of
+bench.py may use this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+REF_LIB = ORACLE_DIR / "_ref" / "libgjxlref.so"
+
+_P = ctypes.c_void_p
+_F = ctypes.c_float
+_U32 = ctypes.c_uint32
+_SZ = ctypes.c_size_t
+_I = ctypes.c_int
+
+
+def build(reference: bool = True) -> None:
+    target = "all" if reference else "oracle"
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "-j8", target],
+                   check=True)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle() -> ctypes.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_LIB.exists():
+            build(reference=False)
+        lib = ctypes.CDLL(str(ORACLE_LIB))
+        lib.oracle_srgb_to_linear_table.argtypes = [_P]
+        lib.oracle_srgb_to_linear_table.restype = None
+        lib.oracle_srgb8_to_linear.argtypes = [_P, _SZ, _U32, _U32, _U32, _P]
+        lib.oracle_srgb8_to_linear.restype = None
+        lib.oracle_fast_cube_root_and_add.argtypes = [_F, _F]
+        lib.oracle_fast_cube_root_and_add.restype = _F
+        lib.oracle_linear_to_padded_opsin.argtypes = [
+            _P, _U32, _U32, _F, _U32, _U32, _P]
+        lib.oracle_linear_to_padded_opsin.restype = None
+        lib.oracle_opsin_to_linear.argtypes = [_P, _SZ, _F, _P]
+        lib.oracle_opsin_to_linear.restype = None
+        for name in ("oracle_fast_log2", "oracle_fast_pow2"):
+            getattr(lib, name).argtypes = [_F]
+            getattr(lib, name).restype = _F
+        lib.oracle_fast_pow.argtypes = [_F, _F]
+        lib.oracle_fast_pow.restype = _F
+        _oracle = lib
+    return _oracle
+
+
+def reference():
+    global _ref
+    if _ref is None and REF_LIB.exists():
+        lib = ctypes.CDLL(str(REF_LIB))
+        lib.ref_srgb8_to_linear.argtypes = [_P, _SZ, _SZ, _U32, _U32, _U32, _P]
+        lib.ref_forward_dct.argtypes = [_I, _P, _P]
+        lib.ref_inverse_dct.argtypes = [_I, _P, _P]
+        lib.ref_gaborish_inverse.argtypes = [_P, _SZ, _SZ, _P, _P]
+        lib.ref_quant_matrix.argtypes = [_I, _I, _P, _P, ctypes.POINTER(_SZ)]
+        lib.ref_initial_cfl.argtypes = [_P, _SZ, _SZ, _P, _P]
+        lib.ref_ac_strategy_cost.argtypes = [
+            _I, _SZ, _SZ, _P, _SZ, _SZ, _P, _P, _F, _F, _F, _F,
+            ctypes.POINTER(_F)]
+        lib.ref_find_ac_strategy_grid.argtypes = [_P, _SZ, _SZ, _P, _P, _F, _P]
+        _ref = lib
+    return _ref
+
+
+def ptr(array: np.ndarray) -> int:
+    assert array.flags["C_CONTIGUOUS"]
+    return array.ctypes.data
+
+
+def pad8(v: int) -> int:
+    return (v + 7) // 8 * 8
+
+
+# ---- convenience wrappers (numpy in, numpy out) -------------------------
+
+def srgb8_to_linear(srgb: np.ndarray) -> np.ndarray:
+    h, w, bpp = srgb.shape
+    srgb = np.ascontiguousarray(srgb)
+    out = np.empty((3, h, w), np.float32)
+    oracle().oracle_srgb8_to_linear(ptr(srgb), w * bpp, bpp, w, h, ptr(out))
+    return out
+
+
+def linear_to_padded_opsin(linear: np.ndarray,
+                           intensity_target: float = 255.0) -> np.ndarray:
+    _, h, w = linear.shape
+    linear = np.ascontiguousarray(linear, np.float32)
+    out = np.empty((3, pad8(h), pad8(w)), np.float32)
+    oracle().oracle_linear_to_padded_opsin(
+        ptr(linear), w, h, intensity_target, pad8(w), pad8(h), ptr(out))
+    return out
+
+
+def opsin_to_linear(opsin: np.ndarray,
+                    intensity_target: float = 255.0) -> np.ndarray:
+    opsin = np.ascontiguousarray(opsin, np.float32)
+    count = opsin[0].size
+    out = np.empty_like(opsin)
+    oracle().oracle_opsin_to_linear(ptr(opsin), count, intensity_target,
+                                    ptr(out))
+    return out
