@@ -33,6 +33,11 @@ def stage_bytes(name: str, w: int, h: int) -> int:
     pn = ((w + 7) // 8 * 8) * ((h + 7) // 8 * 8)
     if name == "srgb_to_linear_and_opsin":
         return 3 * n + 12 * n + 12 * pn
+    if name == "gaborish_inverse":
+        return 12 * pn + 12 * pn
+    if name == "initial_quant_field":
+        # opsin in; pixel mask + quant field + strategy mask out
+        return 12 * pn + 4 * pn + 2 * 4 * (pn // 64)
     raise KeyError(name)
 
 
@@ -40,11 +45,22 @@ def run_step(dev, bufs, w, h):
     """Enqueues one pass; records event slot i+1 after stage i."""
     import gjxl_amd
 
+    names = []
+
+    def mark(name):
+        names.append(name)
+        dev.record(len(names))
+
     dev.record(0)
     dev.srgb8_to_linear_and_opsin(bufs["srgb"], w, h, linear=bufs["linear"],
                                   opsin=bufs["opsin"])
-    dev.record(1)
-    return ["srgb_to_linear_and_opsin"]
+    mark("srgb_to_linear_and_opsin")
+    dev.gaborish_inverse(bufs["opsin"], out=bufs["preprocessed"])
+    mark("gaborish_inverse")
+    dev.initial_quant_field(bufs["opsin"], 1.0, 1.0, out=bufs["aq"],
+                            scratch=bufs["aq_scratch"])
+    mark("initial_quant_field")
+    return names
 
 
 def cpu_baseline(srgb: np.ndarray, budget_s: float = 10.0) -> dict:
@@ -58,7 +74,9 @@ def cpu_baseline(srgb: np.ndarray, budget_s: float = 10.0) -> dict:
     t0 = time.perf_counter()
     while True:
         lin = binding.srgb8_to_linear(crop)
-        binding.linear_to_padded_opsin(lin)
+        ops = binding.linear_to_padded_opsin(lin)
+        binding.gaborish_inverse(ops)
+        binding.initial_quant_field(ops, 1.0, 1.0)
         done += 1
         if time.perf_counter() - t0 > budget_s or done >= 50:
             break
@@ -95,10 +113,16 @@ def main() -> int:
     w, h = spec["width"], spec["height"]
     srgb = synthetic.generate(**spec)
     dev = gjxl_amd.Device(0)
+    ph, pw = gjxl_amd.pad8(h), gjxl_amd.pad8(w)
     bufs = {
         "srgb": dev.array(srgb),
         "linear": dev.empty((3, h, w)),
-        "opsin": dev.empty((3, gjxl_amd.pad8(h), gjxl_amd.pad8(w))),
+        "opsin": dev.empty((3, ph, pw)),
+        "preprocessed": dev.empty((3, ph, pw)),
+        "aq": {},
+        "aq_scratch": dev.empty(
+            (gjxl_amd.library().gjxl_amd_initial_quant_field_scratch_bytes(pw, ph),),
+            np.uint8),
     }
 
     for _ in range(args.warmup):

@@ -69,9 +69,14 @@ _SIGNATURES = {
     "gjxl_amd_srgb8_to_linear_and_opsin":
         [_P, _P, _SZ, _U32, _U32, _U32, _F, _P, _P],
     "gjxl_amd_linear_rgb_to_padded_opsin": [_P, _P, _U32, _U32, _F, _P],
+    "gjxl_amd_gaborish_inverse": [_P, _P, _U32, _U32, _P, _P],
+    "gjxl_amd_initial_quant_field_scratch_bytes": [_U32, _U32],
+    "gjxl_amd_initial_quant_field":
+        [_P, _P, _U32, _U32, _F, _F, _P, _P, _P, _P],
 }
 _VOID = {"gjxl_amd_device_destroy"}
 _STR = {"gjxl_amd_last_error"}
+_SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
 
 def exported_symbols() -> list[str]:
@@ -86,6 +91,8 @@ def _declare(lib: ctypes.CDLL) -> None:
             fn.restype = None
         elif name in _STR:
             fn.restype = ctypes.c_char_p
+        elif name in _SIZE:
+            fn.restype = ctypes.c_size_t
         else:
             fn.restype = ctypes.c_int32
 
@@ -205,6 +212,36 @@ class Device:
         _check(library().gjxl_amd_linear_rgb_to_padded_opsin(
             self.handle, linear.ptr, width, height, intensity_target, opsin.ptr))
         return opsin
+
+
+    # ---- a4 ---------------------------------------------------------------
+    def gaborish_inverse(self, opsin: DeviceArray, multipliers=(1.0, 1.0, 1.0),
+                         out: DeviceArray | None = None) -> DeviceArray:
+        _, h, w = opsin.shape
+        out = out or self.empty(opsin.shape)
+        mult = (ctypes.c_float * 3)(*multipliers)
+        _check(library().gjxl_amd_gaborish_inverse(
+            self.handle, opsin.ptr, w, h, mult, out.ptr))
+        return out
+
+    # ---- a5 ---------------------------------------------------------------
+    def initial_quant_field(self, opsin: DeviceArray, target: float = 1.0,
+                            rescale: float = 1.0, out: dict | None = None,
+                            scratch: DeviceArray | None = None) -> dict:
+        _, h, w = opsin.shape
+        out = out if out is not None else {}
+        if "quant_field" not in out:
+            out["quant_field"] = self.empty((h // 8, w // 8))
+            out["strategy_mask"] = self.empty((h // 8, w // 8))
+            out["pixel_mask"] = self.empty((h, w))
+        if scratch is None:
+            nbytes = library().gjxl_amd_initial_quant_field_scratch_bytes(w, h)
+            scratch = self.empty((nbytes,), np.uint8)
+        _check(library().gjxl_amd_initial_quant_field(
+            self.handle, opsin.ptr, w, h, target, rescale,
+            out["quant_field"].ptr, out["strategy_mask"].ptr,
+            out["pixel_mask"].ptr, scratch.ptr))
+        return out
 
 
 def gpu_available() -> bool:

@@ -96,6 +96,32 @@ GJXL_AMD_API GJXLResult gjxl_amd_linear_rgb_to_padded_opsin(
   GJXLAmdDevice* device, const float* linear, uint32_t width,
   uint32_t height, float intensity_target, float* opsin);
 
+/* ---------------------------------------------------------------------- */
+/* a4: Gaborish inverse on a padded XYB image (3 planes of w*h floats).     */
+/* Replaces ApplyGaborishInverse (src/codec/gaborish.cpp:49-99) behind      */
+/* GaborishInverseProvider (src/codec/quantization_pipeline_internal.h:19-  */
+/* 73).  w, h: non-zero multiples of 8.  in and out must not alias.         */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_gaborish_inverse(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  const float multipliers[3], float* out);
+
+/* ---------------------------------------------------------------------- */
+/* a5: initial quant field, strategy mask and blurred pixel mask from the   */
+/* pre-Gaborish padded XYB image.  Replaces ComputeInitialQuantField        */
+/* (src/codec/adaptive_quantization.cpp:583-738) as driven by               */
+/* PreparedAqEvaluation::ComputeInitialQuantization                          */
+/* (src/gpu/ops/aq_evaluation.h:219-420).  quant_field, strategy_mask:      */
+/* (w/8)*(h/8) floats; pixel_mask: w*h floats; scratch: device memory of    */
+/* gjxl_amd_initial_quant_field_scratch_bytes(w, h) bytes.                  */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API size_t gjxl_amd_initial_quant_field_scratch_bytes(
+  uint32_t width, uint32_t height);
+GJXL_AMD_API GJXLResult gjxl_amd_initial_quant_field(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  float butteraugli_target, float rescale, float* quant_field,
+  float* strategy_mask, float* pixel_mask, float* scratch);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

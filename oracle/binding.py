@@ -57,6 +57,19 @@ def oracle() -> ctypes.CDLL:
             getattr(lib, name).restype = _F
         lib.oracle_fast_pow.argtypes = [_F, _F]
         lib.oracle_fast_pow.restype = _F
+        lib.oracle_mirror.argtypes = [ctypes.c_longlong, _SZ]
+        lib.oracle_mirror.restype = _SZ
+        lib.oracle_convolve_symmetric5.argtypes = [_P, _SZ, _SZ, _P, _P]
+        lib.oracle_convolve_symmetric5.restype = None
+        lib.oracle_gaborish_inverse_weights.argtypes = [_F, _P]
+        lib.oracle_gaborish_inverse_weights.restype = None
+        lib.oracle_gaborish_inverse.argtypes = [_P, _SZ, _SZ, _P, _P]
+        lib.oracle_gaborish_inverse.restype = None
+        lib.oracle_initial_quant_field.argtypes = [
+            _P, _SZ, _SZ, _F, _F, _P, _P, _P]
+        lib.oracle_initial_quant_field.restype = _I
+        lib.oracle_initial_quant_dc.argtypes = [_F]
+        lib.oracle_initial_quant_dc.restype = _F
         _oracle = lib
     return _oracle
 
@@ -116,3 +129,26 @@ def opsin_to_linear(opsin: np.ndarray,
     oracle().oracle_opsin_to_linear(ptr(opsin), count, intensity_target,
                                     ptr(out))
     return out
+
+
+def gaborish_inverse(opsin: np.ndarray, multipliers=(1.0, 1.0, 1.0)) -> np.ndarray:
+    opsin = np.ascontiguousarray(opsin, np.float32)
+    _, h, w = opsin.shape
+    out = np.empty_like(opsin)
+    mult = np.asarray(multipliers, np.float32)
+    oracle().oracle_gaborish_inverse(ptr(opsin), w, h, ptr(mult), ptr(out))
+    return out
+
+
+def initial_quant_field(opsin: np.ndarray, target: float = 1.0,
+                        rescale: float = 1.0) -> dict:
+    opsin = np.ascontiguousarray(opsin, np.float32)
+    _, h, w = opsin.shape
+    q = np.empty((h // 8, w // 8), np.float32)
+    s = np.empty_like(q)
+    m = np.empty((h, w), np.float32)
+    rc = oracle().oracle_initial_quant_field(ptr(opsin), w, h, target, rescale,
+                                             ptr(q), ptr(s), ptr(m))
+    if rc != 0:
+        raise ValueError(f"oracle initial quant field failed ({rc})")
+    return {"quant_field": q, "strategy_mask": s, "pixel_mask": m}

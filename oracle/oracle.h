@@ -44,6 +44,31 @@ void oracle_linear_to_padded_opsin(const float* linear, uint32_t width,
 void oracle_opsin_to_linear(const float* opsin, size_t count,
                             float intensity_target, float* linear);
 
+/* ---- a4: Gaborish inverse ------------------------------------------- */
+/* src/core/image_ops.h:20-47 */
+size_t oracle_mirror(long long coordinate, size_t size);
+/* src/codec/convolution.cpp:38-119; weights = d0, d1, d2, d4, d8, d5.
+ * Pinned: bit-exact against the reference's compiled convolution.cpp. */
+void oracle_convolve_symmetric5(const float* in, size_t w, size_t h,
+                                const float weights[6], float* out);
+/* src/codec/gaborish.cpp:22-47 */
+void oracle_gaborish_inverse_weights(float multiplier, float weights[6]);
+/* src/codec/gaborish.cpp:49-99.  Pinned: bit-exact against the reference's
+ * compiled gaborish.cpp, plus tests/gaborish_test.cpp pinned points. */
+void oracle_gaborish_inverse(const float* in, size_t w, size_t h,
+                             const float multipliers[3], float* out);
+
+/* ---- a5: initial quant field --------------------------------------- */
+/* src/codec/adaptive_quantization.cpp:583-738 (non-uniform branch).
+ * Pinned: tests/adaptive_quantization_test.cpp:184-250 libjxl goldens
+ * (block maps 3e-6, pixel mask 3e-4), closed-form properties.  The unit
+ * itself is behind the reference's toolchain gate (not compiled here). */
+int oracle_initial_quant_field(const float* opsin, size_t w, size_t h,
+                               float target, float rescale, float* quant_field,
+                               float* strategy_mask, float* pixel_mask);
+/* src/codec/adaptive_quantization.cpp:556-580 */
+float oracle_initial_quant_dc(float target);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);
