@@ -70,6 +70,27 @@ def oracle() -> ctypes.CDLL:
         lib.oracle_initial_quant_field.restype = _I
         lib.oracle_initial_quant_dc.argtypes = [_F]
         lib.oracle_initial_quant_dc.restype = _F
+        lib.oracle_quant_matrix.argtypes = [_I, _I, _P, _P]
+        lib.oracle_quant_matrix.restype = _I
+        lib.oracle_forward_dct.argtypes = [_I, _P, _P]
+        lib.oracle_forward_dct.restype = _I
+        lib.oracle_inverse_dct.argtypes = [_I, _P, _P]
+        lib.oracle_inverse_dct.restype = _I
+        lib.oracle_find_best_multiplier.argtypes = [_P, _P, _SZ, _F, _I, _U32]
+        lib.oracle_find_best_multiplier.restype = ctypes.c_int8
+        lib.oracle_initial_cfl.argtypes = [_P, _SZ, _SZ, _P, _P]
+        lib.oracle_initial_cfl.restype = _I
+        lib.oracle_ac_strategy_cost.argtypes = [
+            _I, _SZ, _SZ, _P, _SZ, _SZ, _P, _P, _F, _F, _F, _F,
+            ctypes.POINTER(_F)]
+        lib.oracle_ac_strategy_cost.restype = _I
+        lib.oracle_stage_index.argtypes = [_I]
+        lib.oracle_stage_index.restype = _I
+        lib.oracle_candidate_costs.argtypes = [_P, _SZ, _SZ, _P, _P, _P, _P,
+                                               _F, _P]
+        lib.oracle_candidate_costs.restype = _I
+        lib.oracle_search_from_costs.argtypes = [_P, _SZ, _SZ, _F, _P]
+        lib.oracle_search_from_costs.restype = None
         _oracle = lib
     return _oracle
 
@@ -152,3 +173,47 @@ def initial_quant_field(opsin: np.ndarray, target: float = 1.0,
     if rc != 0:
         raise ValueError(f"oracle initial quant field failed ({rc})")
     return {"quant_field": q, "strategy_mask": s, "pixel_mask": m}
+
+
+STAGE_STRATEGIES = (0, 6, 7, 4, 10, 11, 5)
+STAGE_ENTROPY = (1.0, 1.21, 1.21, 1.34, 1.49, 1.49, 1.48)
+COVERED = {0: (1, 1), 4: (2, 2), 5: (4, 4), 6: (1, 2), 7: (2, 1), 10: (2, 4),
+           11: (4, 2)}
+
+
+def quant_matrix(strategy: int, channel: int):
+    dq = np.empty(1024, np.float32)
+    inv = np.empty(1024, np.float32)
+    n = oracle().oracle_quant_matrix(strategy, channel, ptr(dq), ptr(inv))
+    return dq[:n].copy(), inv[:n].copy()
+
+
+def initial_cfl(opsin: np.ndarray):
+    opsin = np.ascontiguousarray(opsin, np.float32)
+    _, h, w = opsin.shape
+    tw, th = (w + 63) // 64, (h + 63) // 64
+    x = np.empty((th, tw), np.int8)
+    b = np.empty((th, tw), np.int8)
+    assert oracle().oracle_initial_cfl(ptr(opsin), w, h, ptr(x), ptr(b)) == 0
+    return x, b
+
+
+def candidate_costs(opsin, quant_field, pixel_mask, y_to_x, y_to_b,
+                    target: float = 1.0) -> np.ndarray:
+    opsin = np.ascontiguousarray(opsin, np.float32)
+    _, h, w = opsin.shape
+    costs = np.empty((7, h // 8, w // 8), np.float32)
+    args = [np.ascontiguousarray(a) for a in
+            (quant_field, pixel_mask, y_to_x, y_to_b)]
+    rc = oracle().oracle_candidate_costs(ptr(opsin), w, h, *map(ptr, args),
+                                         target, ptr(costs))
+    assert rc == 0
+    return costs
+
+
+def search_from_costs(costs: np.ndarray, target: float = 1.0) -> np.ndarray:
+    costs = np.ascontiguousarray(costs, np.float32)
+    _, bh, bw = costs.shape
+    cells = np.empty((bh, bw), np.uint8)
+    oracle().oracle_search_from_costs(ptr(costs), bw, bh, target, ptr(cells))
+    return cells

@@ -69,6 +69,44 @@ int oracle_initial_quant_field(const float* opsin, size_t w, size_t h,
 /* src/codec/adaptive_quantization.cpp:556-580 */
 float oracle_initial_quant_dc(float target);
 
+/* ---- a6-a10: quantization matrices, DCT, CfL, AC strategy ----------- */
+/* Strategies searched: 0 DCT8, 4 DCT16x16, 5 DCT32x32, 6 DCT16x8,
+ * 7 DCT8x16, 10 DCT32x16, 11 DCT16x32 (src/core/ac_strategy.h:25-148). */
+int oracle_strategy_covered(int strategy, int* blocks_w, int* blocks_h);
+/* src/codec/quantization_tables_generated.h via
+ * scripts/generate_quantization_tables.py; returns the coefficient count.
+ * Pinned: bit-exact against the reference's compiled quantization.cpp. */
+int oracle_quant_matrix(int strategy, int channel, float* dequant,
+                        float* inverse_dequant);
+/* src/codec/dct.cpp:135-233.  Pinned: bit-exact against compiled dct.cpp
+ * and the FP64 textbook DCT. */
+int oracle_forward_dct(int strategy, const float* pixels, float* coefficients);
+int oracle_inverse_dct(int strategy, const float* coefficients, float* pixels);
+/* src/codec/chroma_from_luma.cpp:112-165.  Pinned: compiled reference. */
+int8_t oracle_find_best_multiplier(const float* luma, const float* chroma,
+                                   size_t n, float base, int fast,
+                                   uint32_t iterations);
+/* src/codec/chroma_from_luma.cpp:434-529.  Pinned: compiled reference. */
+int oracle_initial_cfl(const float* opsin, size_t w, size_t h, int8_t* y_to_x,
+                       int8_t* y_to_b);
+/* src/codec/ac_strategy.cpp:823-994.  Pinned: compiled reference. */
+int oracle_ac_strategy_cost(int strategy, size_t bx, size_t by,
+                            const float* opsin, size_t w, size_t h,
+                            const float* quant_field, const float* pixel_mask,
+                            float target, float entropy_multiplier,
+                            float cfl_x, float cfl_b, float* cost);
+/* Candidate cost table (7 stages x blocks, stage order of
+ * src/codec/ac_strategy_search_policy.h:20-28) and the table-driven
+ * search (src/codec/ac_strategy.cpp:591-773, 996-1099).  Pinned: grids
+ * equal to the compiled reference's FindAcStrategyGrid. */
+int oracle_stage_index(int strategy);
+int oracle_candidate_costs(const float* opsin, size_t w, size_t h,
+                           const float* quant_field, const float* pixel_mask,
+                           const int8_t* y_to_x, const int8_t* y_to_b,
+                           float target, float* costs);
+void oracle_search_from_costs(const float* costs, size_t bw, size_t bh,
+                              float target, uint8_t* cells);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);
