@@ -24,6 +24,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def stage_bytes(name: str, w: int, h: int) -> int:
@@ -38,7 +39,42 @@ def stage_bytes(name: str, w: int, h: int) -> int:
     if name == "initial_quant_field":
         # opsin in; pixel mask + quant field + strategy mask out
         return 12 * pn + 4 * pn + 2 * 4 * (pn // 64)
+    if name == "initial_cfl":
+        return 12 * pn + 2 * (pn // 4096)
+    if name == "ac_strategy_costs":
+        # XYB + pixel mask + quant field in, 7 cost planes out
+        return 12 * pn + 4 * pn + 4 * (pn // 64) + 7 * 4 * (pn // 64)
+    if name == "ac_strategy_search":
+        return 7 * 4 * (pn // 64) + pn // 64
     raise KeyError(name)
+
+
+STAGE_SHAPES = ((1, 1, 1), (1, 2, 1), (2, 1, 1), (2, 2, 1), (2, 4, 2),
+                (4, 2, 2), (4, 4, 2))  # covered w, h, anchor step
+
+
+def candidate_flops(w: int, h: int) -> int:
+    """binary64 flops of all candidate DCTs (forward + inverse, 3 channels):
+    12 * W * H * (W + H) per candidate of W x H pixels."""
+    bw, bh = (w + 7) // 8, (h + 7) // 8
+    total = 0
+    for cw, ch, step in STAGE_SHAPES:
+        count = 0
+        for by in range(bh):
+            ly = by % 8
+            tbh = min(8, bh - (by - ly))
+            if ly + ch > tbh or (step == 2 and ly & 1):
+                continue
+            for lx in range(8):
+                nx = 0
+                for tx in range(0, bw, 8):
+                    tbw = min(8, bw - tx)
+                    if lx < tbw and lx + cw <= tbw and not (step == 2 and lx & 1):
+                        nx += 1
+                count += nx
+        W, H = cw * 8, ch * 8
+        total += count * 12 * W * H * (W + H)
+    return total
 
 
 def run_step(dev, bufs, w, h):
@@ -60,6 +96,14 @@ def run_step(dev, bufs, w, h):
     dev.initial_quant_field(bufs["opsin"], 1.0, 1.0, out=bufs["aq"],
                             scratch=bufs["aq_scratch"])
     mark("initial_quant_field")
+    dev.initial_cfl(bufs["preprocessed"], out=bufs["cfl"])
+    mark("initial_cfl")
+    dev.ac_strategy_costs(bufs["preprocessed"], bufs["aq"]["quant_field"],
+                          bufs["aq"]["pixel_mask"], bufs["cfl"]["y_to_x"],
+                          bufs["cfl"]["y_to_b"], 1.0, costs=bufs["costs"])
+    mark("ac_strategy_costs")
+    dev.ac_strategy_search(bufs["costs"], 1.0, cells=bufs["cells"])
+    mark("ac_strategy_search")
     return names
 
 
@@ -75,8 +119,12 @@ def cpu_baseline(srgb: np.ndarray, budget_s: float = 10.0) -> dict:
     while True:
         lin = binding.srgb8_to_linear(crop)
         ops = binding.linear_to_padded_opsin(lin)
-        binding.gaborish_inverse(ops)
-        binding.initial_quant_field(ops, 1.0, 1.0)
+        pre = binding.gaborish_inverse(ops)
+        aq = binding.initial_quant_field(ops, 1.0, 1.0)
+        cx, cb = binding.initial_cfl(pre)
+        costs = binding.candidate_costs(pre, aq["quant_field"], aq["pixel_mask"],
+                                        cx, cb)
+        binding.search_from_costs(costs)
         done += 1
         if time.perf_counter() - t0 > budget_s or done >= 50:
             break
@@ -120,6 +168,9 @@ def main() -> int:
         "opsin": dev.empty((3, ph, pw)),
         "preprocessed": dev.empty((3, ph, pw)),
         "aq": {},
+        "cfl": {},
+        "costs": dev.empty((7, ph // 8, pw // 8)),
+        "cells": dev.empty((ph // 8, pw // 8), np.uint8),
         "aq_scratch": dev.empty(
             (gjxl_amd.library().gjxl_amd_initial_quant_field_scratch_bytes(pw, ph),),
             np.uint8),
@@ -143,8 +194,21 @@ def main() -> int:
     mp = w * h / 1e6
     avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
     dominant = max(avg, key=avg.get)
-    nbytes = stage_bytes(dominant, w, h)
-    achieved = nbytes / (avg[dominant] * 1e-3) / 1e9
+    if dominant == "ac_strategy_costs":
+        flops = candidate_flops(w, h)
+        achieved = flops / (avg[dominant] * 1e-3) / 1e12
+        roofline = {"kernel": "CandidateCostKernel (7 strategy launches)",
+                    "bound": "fp64", "achieved": round(achieved, 3),
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                    "traffic": None}
+    else:
+        nbytes = stage_bytes(dominant, w, h)
+        achieved = nbytes / (avg[dominant] * 1e-3) / 1e9
+        roofline = {"kernel": dominant, "bound": "hbm",
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": None}
     result = {
         "metric": "MP/s at d1.0 e7, plain GPU path (stages built so far), 1 GPU",
         "value": round(mp / (ms_per_step * 1e-3), 3),
@@ -164,15 +228,7 @@ def main() -> int:
             "stages": list(avg),
             "stage_ms": {k: round(v, 5) for k, v in avg.items()},
         },
-        "roofline": {
-            "kernel": dominant,
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
-        },
+        "roofline": roofline,
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(srgb)

@@ -73,6 +73,10 @@ _SIGNATURES = {
     "gjxl_amd_initial_quant_field_scratch_bytes": [_U32, _U32],
     "gjxl_amd_initial_quant_field":
         [_P, _P, _U32, _U32, _F, _F, _P, _P, _P, _P],
+    "gjxl_amd_initial_cfl": [_P, _P, _U32, _U32, _P, _P],
+    "gjxl_amd_ac_strategy_costs":
+        [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P],
+    "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
 }
 _VOID = {"gjxl_amd_device_destroy"}
 _STR = {"gjxl_amd_last_error"}
@@ -242,6 +246,39 @@ class Device:
             out["quant_field"].ptr, out["strategy_mask"].ptr,
             out["pixel_mask"].ptr, scratch.ptr))
         return out
+
+    # ---- a6 ---------------------------------------------------------------
+    def initial_cfl(self, opsin: DeviceArray, out: dict | None = None) -> dict:
+        _, h, w = opsin.shape
+        out = out if out is not None else {}
+        if "y_to_x" not in out:
+            tiles = ((h + 63) // 64, (w + 63) // 64)
+            out["y_to_x"] = self.empty(tiles, np.int8)
+            out["y_to_b"] = self.empty(tiles, np.int8)
+        _check(library().gjxl_amd_initial_cfl(
+            self.handle, opsin.ptr, w, h, out["y_to_x"].ptr, out["y_to_b"].ptr))
+        return out
+
+    # ---- a7-a9 ------------------------------------------------------------
+    def ac_strategy_costs(self, opsin: DeviceArray, quant_field: DeviceArray,
+                          pixel_mask: DeviceArray, y_to_x: DeviceArray,
+                          y_to_b: DeviceArray, target: float = 1.0,
+                          costs: DeviceArray | None = None) -> DeviceArray:
+        _, h, w = opsin.shape
+        costs = costs or self.empty((7, h // 8, w // 8))
+        _check(library().gjxl_amd_ac_strategy_costs(
+            self.handle, opsin.ptr, w, h, quant_field.ptr, pixel_mask.ptr,
+            y_to_x.ptr, y_to_b.ptr, target, costs.ptr))
+        return costs
+
+    # ---- a10 --------------------------------------------------------------
+    def ac_strategy_search(self, costs: DeviceArray, target: float = 1.0,
+                           cells: DeviceArray | None = None) -> DeviceArray:
+        _, bh, bw = costs.shape
+        cells = cells or self.empty((bh, bw), np.uint8)
+        _check(library().gjxl_amd_ac_strategy_search(
+            self.handle, costs.ptr, bw * 8, bh * 8, target, cells.ptr))
+        return cells
 
 
 def gpu_available() -> bool:

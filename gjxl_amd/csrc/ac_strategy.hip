@@ -1,0 +1,759 @@
+// a6 initial chroma-from-luma, a7-a9 AC-strategy candidate costs, a10 the
+// greedy merge.
+//
+// Arithmetic follows the reference's CPU code term for term:
+//   * DCTs use the binary64 bases with one sequential multiply-add chain per
+//     output in the reference's index order (src/codec/dct.cpp:135-233), so
+//     every coefficient is the same binary32 value;
+//   * every floating-point reduction whose order the reference fixes (the
+//     candidate's sqrt-magnitude and 8th-power loss sums, CfL lane sums) is
+//     evaluated by one thread in that order, after all lanes of the work
+//     group have produced the terms in parallel.
+// CandidateCostKernel: one work group evaluates kCands candidates of one
+// strategy (3072 coefficient slots: 1x32x32, 2x32x16, 4x16x16, 8x16x8 or
+// 16x8x8), three channels each, entirely in LDS.
+#include <cmath>
+
+#include "common.h"
+#include "tables.h"
+
+namespace gjxl_amd {
+namespace {
+
+__device__ __forceinline__ float StdMax(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float StdMin(float a, float b) { return b < a ? b : a; }
+
+// src/core/ac_strategy.h:95-108
+template <int W, int H>
+__device__ __forceinline__ int CoefficientIndex(int v, int u) {
+  if constexpr (H < W) return v * W + u;
+  return u * H + v;
+}
+
+// ---- a6: initial chroma from luma --------------------------------------
+
+constexpr float kDistanceMultiplierAc = 1.0e-9f;
+
+// src/codec/chroma_from_luma.cpp:96-110
+__device__ __forceinline__ int8_t QuantizeMultiplier(float x) {
+  if (x >= 2.6f) x -= 2.6f;
+  else if (x <= -2.6f) x += 2.6f;
+  else x = 0.0f;
+  float r = roundf(x);
+  r = r < -128.0f ? -128.0f : (127.0f < r ? 127.0f : r);
+  return static_cast<int8_t>(r);
+}
+
+__device__ __forceinline__ float CflDerivative(float a, float residual) {
+  float v = (2.0f / 3.0f) * a * (fabsf(residual) + 1.0f);
+  if (residual < 0.0f) v = -v;
+  return v;
+}
+
+constexpr int kCflThreads = 256;
+
+// One work group per 64x64 colour tile (src/codec/chroma_from_luma.cpp:
+// 434-529): DCT8 of each block and channel, LLF cleared, scaled by the DCT8
+// X/B inverse-dequant tables into the four value arrays in block-raster then
+// coefficient order; then 20 damped Newton steps per map
+// (:40-94, :112-165), each lane of the reference's 4-lane sums on one thread.
+__global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
+  const float* __restrict__ opsin, int width, int height,
+  const float* __restrict__ tables, const double* __restrict__ bases,
+  int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
+  // DCT8 coefficients of the tile per channel (LLF cleared); the four value
+  // arrays of the reference are products with the DCT8 tables, formed on
+  // the fly (multiplying by the quant scale 1.0 is exact).
+  __shared__ float coefs[3][4096];
+  __shared__ float inv[2][64];
+  __shared__ float lane_sum[2][3][4];
+  __shared__ float state_x[2];
+  __shared__ int state_done[2];
+  const int tid = threadIdx.x;
+  const int tiles_w = (width + 63) / 64;
+  const int bx0 = blockIdx.x * 8, by0 = blockIdx.y * 8;
+  const int nbx = min(8, width / 8 - bx0), nby = min(8, height / 8 - by0);
+  const int nblocks = nbx * nby;
+  const size_t plane = static_cast<size_t>(width) * height;
+  const double* basis = bases + BasisOffset(8, true);
+  const float* inv_x = tables + InverseOffset(kDct8, 0);
+  const float* inv_b = tables + InverseOffset(kDct8, 2);
+
+  for (int job = tid; job < nblocks * 3; job += kCflThreads) {
+    const int k = job / 3, c = job % 3;
+    const int bx = bx0 + k % nbx, by = by0 + k / nbx;
+    const float* src = opsin + c * plane + static_cast<size_t>(by * 8) * width + bx * 8;
+    float p[64];
+#pragma unroll
+    for (int y = 0; y < 8; ++y)
+#pragma unroll
+      for (int x = 0; x < 8; ++x) p[y * 8 + x] = src[static_cast<size_t>(y) * width + x];
+    double hz[64];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double sum = 0.0;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) sum += static_cast<double>(p[y * 8 + x]) * basis[u * 8 + x];
+        hz[y * 8 + u] = sum;
+      }
+    }
+    float coef[64];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        double sum = 0.0;
+#pragma unroll
+        for (int y = 0; y < 8; ++y) sum += basis[v * 8 + y] * hz[y * 8 + u];
+        coef[CoefficientIndex<8, 8>(v, u)] = static_cast<float>(sum);
+      }
+    }
+    coef[0] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) coefs[c][k * 64 + i] = coef[i];
+  }
+  if (tid < 64) {
+    inv[0][tid] = inv_x[tid];
+    inv[1][tid] = inv_b[tid];
+  }
+  if (tid < 2) {
+    state_x[tid] = 0.0f;
+    state_done[tid] = 0;
+  }
+  __syncthreads();
+
+  const int n = nblocks * 64;
+  const float count = static_cast<float>(n);
+  constexpr float kEps = 100.0f;
+  for (int it = 0; it < 20; ++it) {
+    if (tid < 8) {
+      const int problem = tid / 4, lane = tid % 4;
+      if (!state_done[problem]) {
+        const float* luma_coef = coefs[1];
+        const float* chroma_coef = coefs[problem == 0 ? 0 : 2];
+        const float* table = inv[problem];
+        const float base = problem == 0 ? 0.0f : 1.0f;
+        const float x = state_x[problem];
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+        for (int i = lane; i < n; i += 4) {
+          const float luma = luma_coef[i] * table[i & 63];
+          const float chroma = chroma_coef[i] * table[i & 63];
+          const float a = luma / 84.0f;
+          const float b = base * luma - chroma;
+          const float cr = __builtin_fmaf(a, x, b);
+          if (fabsf(cr) >= 100.0f) continue;
+          s0 += CflDerivative(a, cr);
+          s1 += CflDerivative(a, __builtin_fmaf(a, x + kEps, b));
+          s2 += CflDerivative(a, __builtin_fmaf(a, x - kEps, b));
+        }
+        lane_sum[problem][0][lane] = s0;
+        lane_sum[problem][1][lane] = s1;
+        lane_sum[problem][2][lane] = s2;
+      }
+    }
+    __syncthreads();
+    if (tid < 2 && !state_done[tid]) {
+      const float x = state_x[tid];
+      float center = 2.0f * kDistanceMultiplierAc * count * x;
+      float plus = 2.0f * kDistanceMultiplierAc * count * (x + kEps);
+      float minus = 2.0f * kDistanceMultiplierAc * count * (x - kEps);
+      const float (*l)[4] = lane_sum[tid];
+      center += (l[0][0] + l[0][1]) + (l[0][2] + l[0][3]);
+      plus += (l[1][0] + l[1][1]) + (l[1][2] + l[1][3]);
+      minus += (l[2][0] + l[2][1]) + (l[2][2] + l[2][3]);
+      const float second = (plus - minus) / (2.0f * kEps);
+      const float step = center / (second + 0.85f);
+      const float clamped = step < -20.0f ? -20.0f : (20.0f < step ? 20.0f : step);
+      state_x[tid] = x - clamped;
+      if (fabsf(step) < 3.0e-3f) state_done[tid] = 1;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int t = blockIdx.y * tiles_w + blockIdx.x;
+    y_to_x[t] = n == 0 ? 0 : QuantizeMultiplier(state_x[0]);
+    y_to_b[t] = n == 0 ? 0 : QuantizeMultiplier(state_x[1]);
+  }
+}
+
+// ---- a7-a9: candidate cost ---------------------------------------------
+
+struct CostParams {
+  float info_loss_multiplier;
+  float zeros_multiplier;
+  float cost_delta;
+  float entropy_multiplier;
+};
+
+__device__ __forceinline__ int CeilLog2Nonzero(int v) {
+  return v <= 1 ? 0 : 32 - __clz(v - 1);
+}
+
+constexpr int kCostThreads = 256;
+constexpr int kCostSlots = 3072;
+
+template <int kStrategy>
+struct CostShape {
+  static constexpr int kCw = CoveredW(kStrategy);
+  static constexpr int kCh = CoveredH(kStrategy);
+  static constexpr int kW = kCw * 8;
+  static constexpr int kH = kCh * 8;
+  static constexpr int kN = kW * kH;
+  static constexpr int kCands = kCostSlots / (3 * kN);
+  static constexpr int kStep =
+    (kStrategy == kDct32x16 || kStrategy == kDct16x32 || kStrategy == kDct32x32) ? 2 : 1;
+};
+
+template <int kStrategy>
+__global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
+  const float* __restrict__ opsin, int width, int height,
+  const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
+  const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
+  const float* __restrict__ tables, const double* __restrict__ bases,
+  CostParams params, float* __restrict__ costs) {
+  using S = CostShape<kStrategy>;
+  constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
+  __shared__ float pix[K][3][N];     // pixels, then residual coefficients
+  __shared__ double tmp[K][3][N];    // horizontal pass (forward, inverse)
+  __shared__ float coef[K][3][N];    // coefficients, then weighted loss terms
+  __shared__ float sqa[K][3][N];     // sqrt(|rounded|)
+  __shared__ int nnz[K][3];
+  __shared__ float sums[K][3][2];
+  __shared__ float qn[K];
+  __shared__ float cfl[K][3];
+  __shared__ int valid[K];
+
+  const int tid = threadIdx.x;
+  const int bw = width / 8, bh = height / 8;
+  const int by = blockIdx.y;
+  const size_t plane = static_cast<size_t>(width) * height;
+
+  if (tid < K) {
+    const int bx = blockIdx.x * K + tid;
+    int ok = bx < bw && by < bh;
+    if (ok) {
+      const int lx = bx % 8, ly = by % 8;
+      const int tbw = min(8, bw - (bx - lx)), tbh = min(8, bh - (by - ly));
+      ok = lx + S::kCw <= tbw && ly + S::kCh <= tbh &&
+           (S::kStep == 1 || ((lx | ly) & 1) == 0);
+    }
+    valid[tid] = ok;
+    if (ok) {
+      // src/codec/ac_strategy.cpp:103-137 ComputeQuantNormUnchecked
+      const float* q = quant_field + static_cast<size_t>(by) * bw + bx;
+      float norm;
+      if constexpr (S::kCw * S::kCh == 1) {
+        norm = q[0];
+      } else if constexpr (S::kCw * S::kCh == 2) {
+        norm = S::kCh == 2 ? StdMax(q[0], q[bw]) : StdMax(q[0], q[1]);
+      } else {
+        float sum = 0.0f;
+        for (int dy = 0; dy < S::kCh; ++dy) {
+          for (int dx = 0; dx < S::kCw; ++dx) {
+            float v = q[static_cast<size_t>(dy) * bw + dx];
+            v *= v;
+            v *= v;
+            v *= v;
+            sum += v * v;
+          }
+        }
+        sum /= static_cast<float>(S::kCw * S::kCh);
+        norm = FastPow(sum, 1.0f / 16.0f);
+      }
+      qn[tid] = norm;
+      const int tile = (by / 8) * ((bw + 7) / 8) + bx / 8;
+      cfl[tid][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
+      cfl[tid][1] = 0.0f;
+      cfl[tid][2] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+    }
+    for (int c = 0; c < 3; ++c) nnz[tid][c] = 0;
+  }
+  __syncthreads();
+
+  // Load pixels (invalid candidates load zeros and are discarded).
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int y = r / W, x = r % W;
+    float v = 0.0f;
+    if (valid[k]) {
+      const int bx = blockIdx.x * K + k;
+      v = opsin[c * plane + static_cast<size_t>(by * 8 + y) * width + bx * 8 + x];
+    }
+    pix[k][c][r] = v;
+  }
+  __syncthreads();
+
+  const double* fh = bases + BasisOffset(W, true);
+  const double* fv = bases + BasisOffset(H, true);
+  const double* ih = bases + BasisOffset(W, false);
+  const double* iv = bases + BasisOffset(H, false);
+
+  // Forward horizontal: tmp[y][u] = sum_x pix[y][x] * fh[u][x].
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int y = r / W, u = r % W;
+    const float* row = pix[k][c] + y * W;
+    double sum = 0.0;
+#pragma unroll 8
+    for (int x = 0; x < W; ++x) sum += static_cast<double>(row[x]) * fh[u * W + x];
+    tmp[k][c][r] = sum;
+  }
+  __syncthreads();
+  // Forward vertical: coef[idx(v,u)] = sum_y fv[v][y] * tmp[y][u].
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int v = r / W, u = r % W;
+    double sum = 0.0;
+#pragma unroll 8
+    for (int y = 0; y < H; ++y) sum += fv[v * H + y] * tmp[k][c][y * W + u];
+    coef[k][c][CoefficientIndex<W, H>(v, u)] = static_cast<float>(sum);
+  }
+  __syncthreads();
+  // Quantise (src/codec/ac_strategy.cpp:914-926).
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+    const float* dq = tables + DequantOffset(kStrategy, c);
+    const float* inv = tables + InverseOffset(kStrategy, c);
+    const float dec = coef[k][c][i] - coef[k][1][i] * cfl[k][c];
+    const float scaled = dec * inv[i] * qn[k];
+    const float rounded = roundf(scaled);
+    const float diff = scaled - rounded;
+    pix[k][c][i] = dq[i] * diff;
+    sqa[k][c][i] = sqrtf(fabsf(rounded));
+    if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
+  }
+  __syncthreads();
+  // Inverse horizontal: tmp[v][x] = sum_u res[idx(v,u)] * ih[u][x].
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int v = r / W, x = r % W;
+    double sum = 0.0;
+#pragma unroll 8
+    for (int u = 0; u < W; ++u) {
+      sum += static_cast<double>(pix[k][c][CoefficientIndex<W, H>(v, u)]) * ih[u * W + x];
+    }
+    tmp[k][c][r] = sum;
+  }
+  __syncthreads();
+  // Inverse vertical and the masked 8th power (:935-952).
+  constexpr float kMaskOffset[3] = {12.0f, 0.0f, 4.0f};
+  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int y = r / W, x = r % W;
+    double sum = 0.0;
+#pragma unroll 8
+    for (int v = 0; v < H; ++v) sum += iv[v * H + y] * tmp[k][c][v * W + x];
+    float weighted = 0.0f;
+    if (valid[k]) {
+      const int bx = blockIdx.x * K + k;
+      const float m = pixel_mask[static_cast<size_t>(by * 8 + y) * width + bx * 8 + x];
+      weighted = (m + kMaskOffset[c]) * static_cast<float>(sum);
+      weighted *= weighted;
+      weighted *= weighted;
+      weighted *= weighted;
+    }
+    coef[k][c][r] = weighted;
+  }
+  __syncthreads();
+  // Ordered sums, one thread each.
+  if (tid < K * 6) {
+    const int k = tid / 6, c = (tid / 2) % 3, kind = tid % 2;
+    const float* src = kind == 0 ? sqa[k][c] : coef[k][c];
+    float s = 0.0f;
+    for (int i = 0; i < N; ++i) s += src[i];
+    sums[k][c][kind] = s;
+  }
+  __syncthreads();
+  if (tid < K) {
+    const int k = tid;
+    const int bx = blockIdx.x * K + k;
+    if (bx < bw) {
+      float result = __builtin_nanf("");
+      if (valid[k]) {
+        constexpr float kChannelMultiplier[3] = {2.0441408586549744e7f, 1.0f,
+                                                 1.266770081387616f};
+        float entropy = 0.0f, loss = 0.0f;
+        for (int c = 0; c < 3; ++c) {
+          loss += sums[k][c][1] * kChannelMultiplier[c];
+          entropy += params.cost_delta * sums[k][c][0];
+          const int nz_bits = CeilLog2Nonzero(nnz[k][c] + 1) + 1;
+          entropy += params.zeros_multiplier *
+                     static_cast<float>(CeilLog2Nonzero(nz_bits + 17) + nz_bits);
+          if (c == 0 && S::kCw * S::kCh >= 2) {
+            const float weight =
+              1.0f + StdMin(3.0f, static_cast<float>(S::kCw * S::kCh) / 8.0f);
+            entropy *= weight;
+            loss *= weight;
+          }
+        }
+        const float normalized = loss / static_cast<float>(N);
+        // x^(1/8) as three binary64 square roots, rounded once to binary32.
+        const float root8 = static_cast<float>(
+          sqrt(sqrt(sqrt(static_cast<double>(normalized)))));
+        const float loss_cost = root8 * static_cast<float>(N) / qn[k];
+        result = entropy * params.entropy_multiplier +
+                 params.info_loss_multiplier * loss_cost;
+      }
+      costs[static_cast<size_t>(by) * bw + bx] = result;
+    }
+  }
+}
+
+// ---- a10: greedy merge, one thread per 64x64 tile --------------------------
+
+constexpr int kStageStrategy[7] = {kDct8, kDct16x8, kDct8x16, kDct16x16,
+                                   kDct32x16, kDct16x32, kDct32x32};
+
+__device__ __forceinline__ int StageOf(int strategy) {
+  switch (strategy) {
+    case kDct8: return 0;
+    case kDct16x8: return 1;
+    case kDct8x16: return 2;
+    case kDct16x16: return 3;
+    case kDct32x16: return 4;
+    case kDct16x32: return 5;
+    default: return 6;
+  }
+}
+
+constexpr int kSearchThreads = 64;
+
+struct TileSearch {
+  const float* costs;
+  int bw, bh;
+  int tbx, tby;  // tile origin in blocks
+  int tw, th;
+  uint8_t* cells;     // [64][kSearchThreads] per-thread column
+  float* cost;
+  uint8_t* prio;
+  int lane;
+
+  __device__ uint8_t& Cell(int lx, int ly) { return cells[(ly * 8 + lx) * kSearchThreads + lane]; }
+  __device__ float& Cost(int lx, int ly) { return cost[(ly * 8 + lx) * kSearchThreads + lane]; }
+  __device__ uint8_t& Prio(int lx, int ly) { return prio[(ly * 8 + lx) * kSearchThreads + lane]; }
+
+  __device__ float Table(int strategy, int lx, int ly) const {
+    return costs[static_cast<size_t>(StageOf(strategy)) * bw * bh +
+                 static_cast<size_t>(tby + ly) * bw + tbx + lx];
+  }
+
+  __device__ void Set(int lx, int ly, int strategy) {
+    const int cw = CoveredW(strategy), ch = CoveredH(strategy);
+    for (int dy = 0; dy < ch; ++dy)
+      for (int dx = 0; dx < cw; ++dx)
+        Cell(lx + dx, ly + dy) = static_cast<uint8_t>((strategy << 1) | (dx == 0 && dy == 0));
+  }
+
+  // src/codec/ac_strategy.cpp:172-191 (global block coordinates)
+  __device__ bool CrossesHorizontal(int sx, int y, int ex) {
+    if (sx >= bw || y >= bh || y % 8 == 0) return false;
+    ex = min(ex, bw);
+    const int limit = sx & ~7;
+    while (sx != limit && !(Cell(sx - tbx, y - tby) & 1)) --sx;
+    for (int x = sx; x < ex;) {
+      const uint8_t c = Cell(x - tbx, y - tby);
+      if (!(c & 1)) return true;
+      x += CoveredW(c >> 1);
+    }
+    return false;
+  }
+
+  // :193-213
+  __device__ bool CrossesVertical(int x, int sy, int ey) {
+    if (x >= bw || sy >= bh || x % 8 == 0) return false;
+    ey = min(ey, bh);
+    const int limit = sy & ~7;
+    while (sy != limit && !(Cell(x - tbx, sy - tby) & 1)) --sy;
+    for (int y = sy; y < ey;) {
+      const uint8_t c = Cell(x - tbx, y - tby);
+      if (!(c & 1)) return true;
+      y += CoveredH(c >> 1);
+    }
+    return false;
+  }
+
+  __device__ void SetCostForTransform(int lx, int ly, int strategy, float c) {
+    const int cw = CoveredW(strategy), ch = CoveredH(strategy);
+    for (int dy = 0; dy < ch; ++dy)
+      for (int dx = 0; dx < cw; ++dx) Cost(lx + dx, ly + dy) = 0.0f;
+    Cost(lx, ly) = c;
+  }
+
+  // :340-384
+  __device__ void TryMerge(int strategy, int lx, int ly, uint8_t priority) {
+    const int cw = CoveredW(strategy), ch = CoveredH(strategy);
+    float current = 0.0f;
+    for (int dy = 0; dy < ch; ++dy) {
+      for (int dx = 0; dx < cw; ++dx) {
+        if (Prio(lx + dx, ly + dy) >= priority) return;
+        current += Cost(lx + dx, ly + dy);
+      }
+    }
+    const float candidate = Table(strategy, lx, ly);
+    if (candidate >= current) return;
+    for (int dy = 0; dy < ch; ++dy) {
+      for (int dx = 0; dx < cw; ++dx) {
+        Cost(lx + dx, ly + dy) = 0.0f;
+        Prio(lx + dx, ly + dy) = priority;
+      }
+    }
+    Set(lx, ly, strategy);
+    Cost(lx, ly) = candidate;
+  }
+
+  // :404-589
+  __device__ void FirstLevelDivision(int blocks, int lx, int ly) {
+    const int half = blocks / 2;
+    const int bx = tbx + lx, by = tby + ly;
+    if (CrossesHorizontal(bx, by, bx + blocks) ||
+        CrossesHorizontal(bx, by + blocks, bx + blocks) ||
+        CrossesVertical(bx, by, by + blocks) ||
+        CrossesVertical(bx + blocks, by, by + blocks)) {
+      return;
+    }
+    const bool allow_v = !CrossesVertical(bx + half, by, by + blocks);
+    const bool allow_h = !CrossesHorizontal(bx, by + half, bx + blocks);
+    float q00 = 0.0f, q01 = 0.0f, q10 = 0.0f, q11 = 0.0f;  // [row][col]
+    for (int dy = 0; dy < blocks; ++dy) {
+      for (int dx = 0; dx < blocks; ++dx) {
+        const float c = Cost(lx + dx, ly + dy);
+        if (dy < half) { if (dx < half) q00 += c; else q01 += c; }
+        else { if (dx < half) q10 += c; else q11 += c; }
+      }
+    }
+    constexpr float kUnavailable = 3.40282347e+38f;
+    float vl = kUnavailable, vr = kUnavailable, ht = kUnavailable, hb = kUnavailable;
+    const int vs = blocks == 2 ? kDct16x8 : kDct32x16;
+    const int hs = blocks == 2 ? kDct8x16 : kDct16x32;
+    const int ss = blocks == 2 ? kDct16x16 : kDct32x32;
+    if (allow_v) {
+      if ((Cell(lx, ly) >> 1) != vs) vl = Table(vs, lx, ly);
+      if ((Cell(lx + half, ly) >> 1) != vs) vr = Table(vs, lx + half, ly);
+    }
+    if (allow_h) {
+      if ((Cell(lx, ly) >> 1) != hs) ht = Table(hs, lx, ly);
+      if ((Cell(lx, ly + half) >> 1) != hs) hb = Table(hs, lx, ly + half);
+    }
+    const float sq = Table(ss, lx, ly);
+    const float vcost = StdMin(vl, q00 + q10) + StdMin(vr, q01 + q11);
+    const float hcost = StdMin(ht, q00 + q01) + StdMin(hb, q10 + q11);
+    if (sq < vcost && sq < hcost) {
+      Set(lx, ly, ss);
+      SetCostForTransform(lx, ly, ss, sq);
+    } else if (vcost < hcost) {
+      if (vl < q00 + q10) { Set(lx, ly, vs); SetCostForTransform(lx, ly, vs, vl); }
+      if (vr < q01 + q11) { Set(lx + half, ly, vs); SetCostForTransform(lx + half, ly, vs, vr); }
+    } else {
+      if (ht < q00 + q01) { Set(lx, ly, hs); SetCostForTransform(lx, ly, hs, ht); }
+      if (hb < q10 + q11) { Set(lx, ly + half, hs); SetCostForTransform(lx, ly + half, hs, hb); }
+    }
+  }
+};
+
+__global__ __launch_bounds__(kSearchThreads) void SearchKernel(
+  const float* __restrict__ costs, int bw, int bh, float target,
+  uint8_t* __restrict__ out_cells) {
+  __shared__ uint8_t cells[64 * kSearchThreads];
+  __shared__ float cost[64 * kSearchThreads];
+  __shared__ uint8_t prio[64 * kSearchThreads];
+  const int tiles_w = (bw + 7) / 8, tiles_h = (bh + 7) / 8;
+  const int tile = blockIdx.x * kSearchThreads + threadIdx.x;
+  if (tile >= tiles_w * tiles_h) return;
+  TileSearch s;
+  s.costs = costs;
+  s.bw = bw;
+  s.bh = bh;
+  s.tbx = (tile % tiles_w) * 8;
+  s.tby = (tile / tiles_w) * 8;
+  s.tw = min(8, bw - s.tbx);
+  s.th = min(8, bh - s.tby);
+  s.cells = cells;
+  s.cost = cost;
+  s.prio = prio;
+  s.lane = threadIdx.x;
+  for (int i = 0; i < 64; ++i) {
+    s.Cell(i % 8, i / 8) = static_cast<uint8_t>((kDct8 << 1) | 1);
+    s.Cost(i % 8, i / 8) = 0.0f;
+    s.Prio(i % 8, i / 8) = 0;
+  }
+  // src/codec/ac_strategy.cpp:591-773 SearchTile
+  const float baseline = 1.0f + -0.4f / (target + 1.4f);
+  for (int y = 0; y < s.th; ++y)
+    for (int x = 0; x < s.tw; ++x) s.Cost(x, y) = s.Table(kDct8, x, y) * baseline;
+  constexpr int kMergeStrategy[4] = {kDct16x8, kDct8x16, kDct16x32, kDct32x16};
+  constexpr uint8_t kMergePriority[4] = {2, 2, 4, 4};
+  for (int m = 0; m < 4; ++m) {
+    const int st = kMergeStrategy[m];
+    const int cw = CoveredW(st), ch = CoveredH(st);
+    for (int y = 0; y + ch <= s.th; y += ch) {
+      for (int x = 0; x + cw <= s.tw; x += cw) {
+        if (y + 3 < s.th && x + 3 < s.tw) {
+          if (st == kDct16x32) {
+            if ((y | x) % 4 == 0) s.FirstLevelDivision(4, x, y);
+            continue;
+          }
+          if (st == kDct32x16) continue;
+        }
+        if ((st == kDct16x32 && y % 4 != 0) || (st == kDct32x16 && x % 4 != 0)) continue;
+        if (y + 1 < s.th && x + 1 < s.tw) {
+          if (st == kDct8x16) {
+            if ((y | x) % 2 == 0) s.FirstLevelDivision(2, x, y);
+            continue;
+          }
+          if (st == kDct16x8) continue;
+        }
+        if ((st == kDct8x16 && y % 2 == 1) || (st == kDct16x8 && x % 2 == 1)) continue;
+        s.TryMerge(st, x, y, kMergePriority[m]);
+      }
+    }
+  }
+  for (int y = 0; y + 1 < s.th; ++y)
+    for (int x = 0; x + 1 < s.tw; ++x)
+      if ((y | x) % 2 != 0) s.FirstLevelDivision(2, x, y);
+  for (int y = 0; y + 3 < s.th; y += 2)
+    for (int x = 0; x + 3 < s.tw; x += 2)
+      if ((y | x) % 4 != 0) s.FirstLevelDivision(4, x, y);
+  for (int y = 0; y < s.th; ++y)
+    for (int x = 0; x < s.tw; ++x)
+      out_cells[static_cast<size_t>(s.tby + y) * bw + s.tbx + x] = s.Cell(x, y);
+}
+
+template <int kStrategy>
+GJXLResult LaunchCost(GJXLAmdDevice* device, const float* opsin, int width,
+                      int height, const float* qf, const float* mask,
+                      const int8_t* yx, const int8_t* yb,
+                      const CostParams& params, float* stage_costs) {
+  using S = CostShape<kStrategy>;
+  const int bw = width / 8, bh = height / 8;
+  const dim3 grid(CeilDiv(bw, S::kCands), bh);
+  hipLaunchKernelGGL(CandidateCostKernel<kStrategy>, grid, dim3(kCostThreads),
+                     0, device->stream, opsin, width, height, qf, mask, yx, yb,
+                     device->quant_tables, device->dct_bases, params,
+                     stage_costs);
+  return CheckLaunch("CandidateCostKernel");
+}
+
+}  // namespace
+
+GJXLResult LaunchInitialCfl(GJXLAmdDevice* device, const float* opsin,
+                            int width, int height, int8_t* y_to_x,
+                            int8_t* y_to_b) {
+  const dim3 grid(CeilDiv(width, 64), CeilDiv(height, 64));
+  hipLaunchKernelGGL(InitialCflKernel, grid, dim3(kCflThreads), 0,
+                     device->stream, opsin, width, height,
+                     device->quant_tables, device->dct_bases, y_to_x, y_to_b);
+  return CheckLaunch("InitialCflKernel");
+}
+
+GJXLResult LaunchCandidateCosts(GJXLAmdDevice* device, const float* opsin,
+                                int width, int height, const float* qf,
+                                const float* mask, const int8_t* yx,
+                                const int8_t* yb, float target, float* costs) {
+  // src/codec/ac_strategy.cpp:884-891 (host libm, once per call)
+  const float kBias = 0.13731742964354549f;
+  const float ratio = (target + kBias) / (1.0f + kBias);
+  CostParams p;
+  p.info_loss_multiplier = 1.2f * std::pow(ratio, 0.33677806662454718f);
+  p.zeros_multiplier = 9.3089059022677905f * std::pow(ratio, 0.50990926717963703f);
+  p.cost_delta = 10.833273317067883f * std::pow(ratio, 0.36702940662370243f);
+  const size_t stage = static_cast<size_t>(width / 8) * (height / 8);
+  constexpr float kEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
+  p.entropy_multiplier = kEntropy[0];
+  GJXL_AMD_TRY(LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 0 * stage));
+  p.entropy_multiplier = kEntropy[1];
+  GJXL_AMD_TRY(LaunchCost<kDct16x8>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 1 * stage));
+  p.entropy_multiplier = kEntropy[2];
+  GJXL_AMD_TRY(LaunchCost<kDct8x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 2 * stage));
+  p.entropy_multiplier = kEntropy[3];
+  GJXL_AMD_TRY(LaunchCost<kDct16x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 3 * stage));
+  p.entropy_multiplier = kEntropy[4];
+  GJXL_AMD_TRY(LaunchCost<kDct32x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 4 * stage));
+  p.entropy_multiplier = kEntropy[5];
+  GJXL_AMD_TRY(LaunchCost<kDct16x32>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 5 * stage));
+  p.entropy_multiplier = kEntropy[6];
+  return LaunchCost<kDct32x32>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 6 * stage);
+}
+
+GJXLResult LaunchSearch(GJXLAmdDevice* device, const float* costs, int width,
+                        int height, float target, uint8_t* cells) {
+  const int bw = width / 8, bh = height / 8;
+  const int tiles = CeilDiv(bw, 8) * CeilDiv(bh, 8);
+  hipLaunchKernelGGL(SearchKernel, dim3(CeilDiv(tiles, kSearchThreads)),
+                     dim3(kSearchThreads), 0, device->stream, costs, bw, bh,
+                     target, cells);
+  return CheckLaunch("SearchKernel");
+}
+
+}  // namespace gjxl_amd
+
+using gjxl_amd::Fail;
+
+namespace {
+GJXLResult ValidateGeometry(uint32_t width, uint32_t height) {
+  if (width == 0 || height == 0 || width % 8 != 0 || height % 8 != 0 ||
+      width > (1u << 28) || height > (1u << 28)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Padded image dimensions must be non-zero multiples of 8");
+  }
+  return GJXL_OK;
+}
+
+GJXLResult ValidateTarget(float target) {
+  if (!std::isfinite(target) || target <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Butteraugli target must be finite and positive");
+  }
+  return GJXL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+GJXLResult gjxl_amd_initial_cfl(GJXLAmdDevice* device, const float* opsin,
+                                uint32_t width, uint32_t height,
+                                int8_t* y_to_x, int8_t* y_to_b) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  GJXL_AMD_TRY(ValidateGeometry(width, height));
+  if (opsin == nullptr || y_to_x == nullptr || y_to_b == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "CfL buffers must be non-null");
+  }
+  return gjxl_amd::LaunchInitialCfl(device, opsin, width, height, y_to_x, y_to_b);
+}
+
+GJXLResult gjxl_amd_ac_strategy_costs(GJXLAmdDevice* device,
+                                      const float* opsin, uint32_t width,
+                                      uint32_t height, const float* quant_field,
+                                      const float* pixel_mask,
+                                      const int8_t* y_to_x,
+                                      const int8_t* y_to_b,
+                                      float butteraugli_target, float* costs) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  GJXL_AMD_TRY(ValidateGeometry(width, height));
+  GJXL_AMD_TRY(ValidateTarget(butteraugli_target));
+  if (opsin == nullptr || quant_field == nullptr || pixel_mask == nullptr ||
+      y_to_x == nullptr || y_to_b == nullptr || costs == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AC cost buffers must be non-null");
+  }
+  return gjxl_amd::LaunchCandidateCosts(device, opsin, width, height,
+                                        quant_field, pixel_mask, y_to_x,
+                                        y_to_b, butteraugli_target, costs);
+}
+
+GJXLResult gjxl_amd_ac_strategy_search(GJXLAmdDevice* device,
+                                       const float* costs, uint32_t width,
+                                       uint32_t height,
+                                       float butteraugli_target,
+                                       uint8_t* cells) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  GJXL_AMD_TRY(ValidateGeometry(width, height));
+  GJXL_AMD_TRY(ValidateTarget(butteraugli_target));
+  if (costs == nullptr || cells == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AC search buffers must be non-null");
+  }
+  return gjxl_amd::LaunchSearch(device, costs, width, height,
+                                butteraugli_target, cells);
+}
+
+}  // extern "C"

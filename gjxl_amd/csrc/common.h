@@ -38,6 +38,9 @@ struct GJXLAmdDevice {
   int ordinal = 0;
   hipStream_t stream = nullptr;
   hipEvent_t events[GJXL_AMD_EVENT_SLOTS] = {};
+  // Read-only tables uploaded at creation (see tables.h).
+  float* quant_tables = nullptr;
+  double* dct_bases = nullptr;
 };
 
 namespace gjxl_amd {

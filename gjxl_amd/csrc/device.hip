@@ -6,6 +6,7 @@
 #include <string>
 
 #include "common.h"
+#include "tables.h"
 
 namespace gjxl_amd {
 namespace {
@@ -64,6 +65,32 @@ GJXLResult gjxl_amd_device_create(int ordinal, GJXLAmdDevice** out) {
   for (int i = 0; result == GJXL_OK && i < GJXL_AMD_EVENT_SLOTS; ++i) {
     result = Check(hipEventCreate(&device->events[i]), "hipEventCreate");
   }
+  if (result == GJXL_OK) {
+    result = Check(hipMalloc(reinterpret_cast<void**>(&device->quant_tables),
+                             sizeof(float) * gjxl_amd::kQuantTableFloats),
+                   "hipMalloc(quant tables)");
+  }
+  if (result == GJXL_OK) {
+    result = Check(hipMalloc(reinterpret_cast<void**>(&device->dct_bases),
+                             sizeof(double) * gjxl_amd::kBasisDoubles),
+                   "hipMalloc(DCT bases)");
+  }
+  if (result == GJXL_OK) {
+    static float tables[gjxl_amd::kQuantTableFloats];
+    static double bases[gjxl_amd::kBasisDoubles];
+    static const bool built = [] {
+      gjxl_amd::BuildQuantTables(tables);
+      gjxl_amd::BuildDctBases(bases);
+      return true;
+    }();
+    (void)built;
+    result = Check(hipMemcpy(device->quant_tables, tables, sizeof(tables),
+                             hipMemcpyHostToDevice), "hipMemcpy(quant tables)");
+    if (result == GJXL_OK) {
+      result = Check(hipMemcpy(device->dct_bases, bases, sizeof(bases),
+                               hipMemcpyHostToDevice), "hipMemcpy(DCT bases)");
+    }
+  }
   if (result != GJXL_OK) {
     gjxl_amd_device_destroy(device);
     return result;
@@ -79,6 +106,8 @@ void gjxl_amd_device_destroy(GJXLAmdDevice* device) {
   for (hipEvent_t& event : device->events) {
     if (event != nullptr) (void)hipEventDestroy(event);
   }
+  if (device->quant_tables != nullptr) (void)hipFree(device->quant_tables);
+  if (device->dct_bases != nullptr) (void)hipFree(device->dct_bases);
   if (device->stream != nullptr) (void)hipStreamDestroy(device->stream);
   delete device;
 }

@@ -122,6 +122,43 @@ GJXL_AMD_API GJXLResult gjxl_amd_initial_quant_field(
   float butteraugli_target, float rescale, float* quant_field,
   float* strategy_mask, float* pixel_mask, float* scratch);
 
+/* ---------------------------------------------------------------------- */
+/* a6: initial chroma-from-luma map from the Gaborish-inverted XYB image.   */
+/* Replaces ComputeInitialColorCorrelationMap (src/codec/chroma_from_luma.  */
+/* cpp:434-529; DCT8 coefficients, 20 Newton steps per 64x64 tile).         */
+/* y_to_x, y_to_b: ceil(w/64)*ceil(h/64) int8 each.                         */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_initial_cfl(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  int8_t* y_to_x, int8_t* y_to_b);
+
+/* ---------------------------------------------------------------------- */
+/* a7-a9: AC-strategy candidate costs.  Replaces                             */
+/* GpuAcStrategyEvaluation::EvaluateAcStrategyCandidateBatches               */
+/* (src/gpu/ops/ac_strategy.h:118-164), each cost equal to                   */
+/* EstimateAcStrategyCost (src/codec/ac_strategy.cpp:823-994).  costs: 7     */
+/* planes of (w/8)*(h/8) floats in the stage order of kCandidateStages       */
+/* (src/codec/ac_strategy_search_policy.h:20-28: DCT8, 16x8, 8x16, 16x16,    */
+/* 32x16, 16x32, 32x32); every anchor whose footprint lies inside its 64x64  */
+/* tile is evaluated (the 32-pixel families on even tile positions); other   */
+/* entries are NaN.                                                          */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_costs(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  const float* quant_field, const float* pixel_mask, const int8_t* y_to_x,
+  const int8_t* y_to_b, float butteraugli_target, float* costs);
+
+/* ---------------------------------------------------------------------- */
+/* a10: greedy merge of the candidate costs into a strategy grid.  Replaces  */
+/* FindAcStrategyGridFromCandidateCosts (src/codec/ac_strategy_search_      */
+/* internal.h:24, src/codec/ac_strategy.cpp:591-773) on the device, one      */
+/* thread per 64x64 tile.  cells: (w/8)*(h/8) bytes, strategy << 1 | anchor  */
+/* (AcStrategyGrid::CreateFromEncodedCells, src/core/ac_strategy.h:196).     */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_search(
+  GJXLAmdDevice* device, const float* costs, uint32_t width, uint32_t height,
+  float butteraugli_target, uint8_t* cells);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif
