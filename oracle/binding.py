@@ -217,3 +217,170 @@ def search_from_costs(costs: np.ndarray, target: float = 1.0) -> np.ndarray:
     cells = np.empty((bh, bw), np.uint8)
     oracle().oracle_search_from_costs(ptr(costs), bw, bh, target, ptr(cells))
     return cells
+
+
+# ---- a11-a15: AQ evaluation without Butteraugli ---------------------------
+
+_AQ_DECLARED = False
+
+
+def _declare_aq():
+    global _AQ_DECLARED
+    if _AQ_DECLARED:
+        return
+    lib = oracle()
+    P, F_, SZ, I = _P, _F, _SZ, _I
+    sig = {
+        "oracle_matrix_scale_stats": ([P, SZ, SZ, SZ, SZ, P], None),
+        "oracle_select_matrix_scales": ([P, F_, P, P], None),
+        "oracle_matrix_multiplier": ([I], F_),
+        "oracle_adjust_quant_field": ([P, SZ, SZ, F_, P, P], None),
+        "oracle_policy_setup": ([P, SZ, F_, P, P, P], I),
+        "oracle_quantizer_from_field": ([F_, P, SZ, P, P], I),
+        "oracle_final_cfl_fast": ([P, P, SZ, SZ, P, P, F_, P, P], None),
+        "oracle_llf_to_dc": ([I, I, P, P], None),
+        "oracle_dc_to_llf": ([I, I, P, P], None),
+        "oracle_dequantize_ac": ([F_, ctypes.c_int32, F_, I, I, P, P, P], None),
+        "oracle_code_transform": ([I, P, ctypes.c_int32, F_, F_, F_, F_, P, P, P, P], None),
+        "oracle_quantize_dc": ([P, SZ, SZ, P, P, I, P, P], None),
+        "oracle_smooth_dc": ([P, SZ, SZ, P, P], None),
+        "oracle_anchor_offsets": ([P, SZ, SZ, P], None),
+        "oracle_anchor_coefficients": ([P, SZ, SZ, P, P, P], None),
+        "oracle_code_coefficients": ([P, P, SZ, SZ, P, P, P, P, P, F_, F_, P, P, P], None),
+        "oracle_epf_inverse_sigma": ([P, SZ, SZ, P, F_, P], None),
+        "oracle_reconstruct": ([P, P, SZ, SZ, P, P, P, P, P, F_, F_, P, P], None),
+        "oracle_gaborish": ([P, SZ, SZ, P], None),
+        "oracle_loop_filters": ([P, SZ, SZ, P, P], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    ref = reference()
+    if ref is not None:
+        U32 = _U32
+        rsig = {
+            "ref_quantizer_from_field": [F_, P, SZ, SZ, P, P],
+            "ref_final_cfl": [P, SZ, SZ, P, P, U32, U32, I, P, P],
+            "ref_code_transform": [I, U32, U32, ctypes.c_int32, F_, F_, F_, F_, P, P, P],
+            "ref_dequantize": [I, U32, U32, ctypes.c_int32, I, F_, P, P],
+            "ref_llf_to_dc": [I, P, P],
+            "ref_dc_to_llf": [I, P, P],
+            "ref_quantize_dc": [P, SZ, SZ, U32, U32, P, P],
+            "ref_smooth_dc": [P, SZ, SZ, U32, U32, P],
+            "ref_epf_inverse_sigma": [P, SZ, SZ, P, U32, U32, P],
+            "ref_loop_filters": [P, SZ, SZ, P, P],
+        }
+        for name, args in rsig.items():
+            fn = getattr(ref, name)
+            fn.argtypes = args
+            fn.restype = I
+    _AQ_DECLARED = True
+
+
+def store_size(bw: int, bh: int) -> int:
+    return ((bw + 31) // 32) * ((bh + 31) // 32) * 65536
+
+
+def quantizer_from_field(field: np.ndarray, quant_dc: float):
+    _declare_aq()
+    field = np.ascontiguousarray(field, np.float32)
+    raw = np.empty(field.shape, np.int32)
+    q = np.empty(10, np.float32)
+    oracle().oracle_quantizer_from_field(quant_dc, ptr(field), field.size,
+                                         ptr(raw), ptr(q))
+    return raw, q
+
+
+def frontend(srgb: np.ndarray, target: float = 1.0) -> dict:
+    """The oracle's plain frontend: XYB, Gaborish inverse, initial quant
+    field, initial CfL, candidate costs and the strategy grid."""
+    lin = srgb8_to_linear(srgb)
+    ops = linear_to_padded_opsin(lin)
+    pre = gaborish_inverse(ops)
+    aq = initial_quant_field(ops, target, 1.0)
+    cx, cb = initial_cfl(pre)
+    costs = candidate_costs(pre, aq["quant_field"], aq["pixel_mask"], cx, cb,
+                            target)
+    cells = search_from_costs(costs, target)
+    return {"linear": lin, "opsin": ops, "pre": pre, "cells": cells,
+            "costs": costs, **aq, "init_y_to_x": cx, "init_y_to_b": cb}
+
+
+def aq_setup(fe: dict, width: int, height: int, target: float = 1.0) -> dict:
+    """Per-encode AQ setup: matrix scales, adjusted field, bounds, anchor
+    layout and the anchors' forward coefficients."""
+    _declare_aq()
+    lib = oracle()
+    ops = fe["opsin"]
+    _, ph, pw = ops.shape
+    bh, bw = ph // 8, pw // 8
+    stats = np.empty(3, np.float32)
+    lib.oracle_matrix_scale_stats(ptr(ops), width, height, pw, pw * ph,
+                                  ptr(stats))
+    xs, bs = ctypes.c_int(), ctypes.c_int()
+    lib.oracle_select_matrix_scales(ptr(stats), target, ctypes.byref(xs),
+                                    ctypes.byref(bs))
+    cells = np.ascontiguousarray(fe["cells"])
+    adjusted = np.empty((bh, bw), np.float32)
+    lib.oracle_adjust_quant_field(ptr(cells), bw, bh, target,
+                                  ptr(np.ascontiguousarray(fe["quant_field"])),
+                                  ptr(adjusted))
+    lo, hi, qdc = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    lib.oracle_policy_setup(ptr(adjusted), adjusted.size, target,
+                            ctypes.byref(lo), ctypes.byref(hi),
+                            ctypes.byref(qdc))
+    offsets = np.empty((bh, bw), np.uint64)
+    lib.oracle_anchor_offsets(ptr(cells), bw, bh, ptr(offsets))
+    coefs = np.zeros((3, store_size(bw, bh)), np.float32)
+    lib.oracle_anchor_coefficients(ptr(np.ascontiguousarray(fe["pre"])), pw,
+                                   ph, ptr(cells), ptr(offsets), ptr(coefs))
+    return {"stats": stats, "x_scale": xs.value, "b_scale": bs.value,
+            "x_mult": lib.oracle_matrix_multiplier(xs.value),
+            "b_mult": lib.oracle_matrix_multiplier(bs.value),
+            "adjusted": adjusted, "lower": lo.value, "upper": hi.value,
+            "quant_dc": qdc.value, "offsets": offsets, "coefs": coefs}
+
+
+def aq_evaluate(fe: dict, setup: dict, field: np.ndarray, width: int,
+                height: int) -> dict:
+    """One evaluation up to the loop filters (no Butteraugli)."""
+    _declare_aq()
+    lib = oracle()
+    cells = np.ascontiguousarray(fe["cells"])
+    bh, bw = cells.shape
+    raw, q = quantizer_from_field(field, setup["quant_dc"])
+    coefs = setup["coefs"]
+    planes = (ctypes.c_void_p * 3)(*(coefs[k].ctypes.data for k in range(3)))
+    tiles = ((bh + 7) // 8, (bw + 7) // 8)
+    fx = np.empty(tiles, np.int8)
+    fb = np.empty(tiles, np.int8)
+    lib.oracle_final_cfl_fast(planes, ptr(cells), bw, bh,
+                              ptr(setup["offsets"]), ptr(raw), float(q[0]),
+                              ptr(fx), ptr(fb))
+    field_raw = raw.copy()
+    quantized = np.zeros((3, store_size(bw, bh)), np.int32)
+    dc_q = np.empty((3, bh, bw), np.int32)
+    dc_rec = np.empty((3, bh, bw), np.float32)
+    lib.oracle_code_coefficients(ptr(coefs), ptr(cells), bw, bh,
+                                 ptr(setup["offsets"]), ptr(q), ptr(raw),
+                                 ptr(fx), ptr(fb), setup["x_mult"],
+                                 setup["b_mult"], ptr(quantized), ptr(dc_q),
+                                 ptr(dc_rec))
+    inv_sigma = np.empty((bh, bw), np.float32)
+    lib.oracle_epf_inverse_sigma(ptr(cells), bw, bh, ptr(raw), float(q[0]),
+                                 ptr(inv_sigma))
+    rec = np.empty((3, bh * 8, bw * 8), np.float32)
+    lib.oracle_reconstruct(ptr(quantized), ptr(cells), bw * 8, bh * 8,
+                           ptr(setup["offsets"]), ptr(q), ptr(raw), ptr(fx),
+                           ptr(fb), setup["x_mult"], setup["b_mult"],
+                           ptr(dc_rec), ptr(rec))
+    cropped = np.ascontiguousarray(rec[:, :height, :width])
+    filtered = np.empty_like(cropped)
+    lib.oracle_loop_filters(ptr(cropped), width, height, ptr(inv_sigma),
+                            ptr(filtered))
+    linear = opsin_to_linear(filtered)
+    return {"q": q, "field_raw": field_raw, "raw": raw, "y_to_x": fx,
+            "y_to_b": fb, "quantized": quantized, "dc_q": dc_q,
+            "dc_rec": dc_rec, "inv_sigma": inv_sigma, "reconstructed": rec,
+            "filtered": filtered, "linear": linear}

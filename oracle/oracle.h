@@ -107,6 +107,79 @@ int oracle_candidate_costs(const float* opsin, size_t w, size_t h,
 void oracle_search_from_costs(const float* costs, size_t bw, size_t bh,
                               float target, uint8_t* cells);
 
+/* ---- a11-a15: one AQ evaluation without Butteraugli (oracle/aq.c) ----- */
+/* src/codestream/workflow.cpp:1318-1484.  stats: x_edge, b_edge, exposed
+ * blue over the unpadded image (stride/plane in floats). */
+void oracle_matrix_scale_stats(const float* opsin, size_t w, size_t h,
+                               size_t stride, size_t plane, float stats[3]);
+void oracle_select_matrix_scales(const float stats[3], float target,
+                                 int* x_scale, int* b_scale);
+float oracle_matrix_multiplier(int scale);
+/* src/codec/adaptive_quantization.cpp:740-835, :1178-1242 */
+void oracle_adjust_quant_field(const uint8_t* cells, size_t bw, size_t bh,
+                               float target, const float* in, float* out);
+int oracle_policy_setup(const float* adjusted, size_t n, float target,
+                        float* lower, float* upper, float* quant_dc);
+/* src/codec/quantization.cpp:536-639.  q: scale, inverse global scale,
+ * global scale, quant dc, dc steps (3), inverse dc steps (3). */
+int oracle_quantizer_from_field(float quant_dc, const float* field, size_t n,
+                                int32_t* raw, float q[10]);
+/* src/codec/chroma_from_luma.cpp:543-670 (fast) over per-anchor
+ * coefficients stored at offsets[] (oracle_anchor_offsets). */
+void oracle_final_cfl_fast(const float* const coefs[3], const uint8_t* cells,
+                           size_t bw, size_t bh, const size_t* offsets,
+                           const int32_t* raw, float scale, int8_t* y_to_x,
+                           int8_t* y_to_b);
+/* src/codec/dc_conversion.cpp */
+void oracle_llf_to_dc(int cw, int ch, const float* coef, float* dc);
+void oracle_dc_to_llf(int cw, int ch, const float* dc, float* coef);
+/* src/codec/quantization.cpp:808-835 */
+void oracle_dequantize_ac(float inverse_global_scale, int32_t raw, float mult,
+                          int channel, int n, const float* dequant,
+                          const int32_t* quantized, float* out);
+/* src/codec/reconstruction.cpp:565-700 for one transform; dc: 3 x 16. */
+void oracle_code_transform(int strategy, const float q[10], int32_t raw,
+                           float x_mult, float b_mult, float cfl_x,
+                           float cfl_b, float* coef, int32_t* quantized,
+                           float* dc, int32_t* decided_raw);
+/* src/codec/vardct/dc_quantization.cpp (prediction-aware, weighted). */
+void oracle_quantize_dc(const float* dc, size_t bw, size_t bh,
+                        const float dc_steps[3], const float inv_dc_steps[3],
+                        int extra_precision, int32_t* quantized,
+                        float* reconstructed);
+/* src/codec/dc_smoothing.cpp */
+void oracle_smooth_dc(const float* dc, size_t bw, size_t bh,
+                      const float dc_steps[3], float* out);
+/* Dense per-group coefficient layout (groups of 256x256 px, 65536 slots
+ * each, anchors of a group in raster order). */
+void oracle_anchor_offsets(const uint8_t* cells, size_t bw, size_t bh,
+                           size_t* offsets);
+void oracle_anchor_coefficients(const float* opsin, size_t w, size_t h,
+                                const uint8_t* cells, const size_t* offsets,
+                                float* coefs);
+/* src/codec/reconstruction.cpp:386-760 over the whole frame. */
+void oracle_code_coefficients(const float* coefs, const uint8_t* cells,
+                              size_t bw, size_t bh, const size_t* offsets,
+                              const float q[10], int32_t* raw,
+                              const int8_t* y_to_x, const int8_t* y_to_b,
+                              float x_mult, float b_mult, int32_t* quantized,
+                              int32_t* dc_q, float* dc_rec);
+/* src/codec/epf.cpp:207-290 (neutral sharpness) */
+void oracle_epf_inverse_sigma(const uint8_t* cells, size_t bw, size_t bh,
+                              const int32_t* raw, float scale, float* out);
+/* src/codec/reconstruction.cpp:790-936 with adaptive DC smoothing. */
+void oracle_reconstruct(const int32_t* quantized, const uint8_t* cells,
+                        size_t w, size_t h, const size_t* offsets,
+                        const float q[10], const int32_t* raw,
+                        const int8_t* y_to_x, const int8_t* y_to_b,
+                        float x_mult, float b_mult, const float* dc_rec,
+                        float* out);
+/* src/codec/gaborish.cpp:101-190, src/codec/loop_filter.cpp:14-65,
+ * src/codec/epf.cpp:61-205 on an unpadded image. */
+void oracle_gaborish(const float* in, size_t w, size_t h, float* out);
+void oracle_loop_filters(const float* in, size_t w, size_t h,
+                         const float* inverse_sigma, float* out);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);

@@ -14,12 +14,18 @@
 #include <cstdint>
 #include <cstring>
 #include <span>
+#include <vector>
 
 #include "c_api/image_conversion.h"
 #include "codec/ac_strategy.h"
 #include "codec/butteraugli.h"
 #include "codec/chroma_from_luma.h"
+#include "codec/dc_conversion.h"
+#include "codec/dc_quantization.h"
+#include "codec/dc_smoothing.h"
 #include "codec/dct.h"
+#include "codec/epf.h"
+#include "codec/loop_filter.h"
 #include "codec/gaborish.h"
 #include "codec/quantization.h"
 #include "core/ac_strategy.h"
@@ -177,6 +183,175 @@ int ref_find_ac_strategy_grid(const float* opsin, size_t width, size_t height,
     }
   }
   return 0;
+}
+
+namespace {
+gjxl::Quantizer MakeQuantizer(uint32_t global_scale, uint32_t quant_dc) {
+  gjxl::Quantizer q;
+  (void)gjxl::Quantizer::Create({global_scale, quant_dc}, &q);
+  return q;
+}
+
+gjxl::Status GridFromCells(const uint8_t* cells, size_t bw, size_t bh,
+                           gjxl::AcStrategyGrid* grid) {
+  return gjxl::AcStrategyGrid::CreateFromEncodedCells(
+    {bw, bh}, std::span<const uint8_t>(cells, bw * bh), grid);
+}
+}  // namespace
+
+// src/codec/quantization.cpp:536-639; params[0] = global scale, [1] = quant dc
+int ref_quantizer_from_field(float quant_dc, const float* field, size_t bw,
+                             size_t bh, int32_t* raw, uint32_t params[2]) {
+  gjxl::Quantizer q;
+  const gjxl::Status status = gjxl::CreateQuantizerFromField(
+    quant_dc, {field, {bw, bh}, bw}, {raw, {bw, bh}, bw}, &q);
+  if (!status.ok()) return Code(status);
+  params[0] = q.params().global_scale;
+  params[1] = q.params().quant_dc;
+  return 0;
+}
+
+// src/codec/chroma_from_luma.cpp:543-670 (fast or Newton final CfL)
+int ref_final_cfl(const float* opsin, size_t width, size_t height,
+                  const uint8_t* cells, const int32_t* raw,
+                  uint32_t global_scale, uint32_t quant_dc, int fast,
+                  int8_t* y_to_x, int8_t* y_to_b) {
+  const size_t bw = width / 8, bh = height / 8;
+  gjxl::AcStrategyGrid grid;
+  gjxl::Status status = GridFromCells(cells, bw, bh, &grid);
+  if (!status.ok()) return Code(status);
+  gjxl::ColorCorrelationMap map;
+  status = gjxl::ComputeFinalColorCorrelationMap(
+    ConstImage(opsin, width, height), grid, {raw, {bw, bh}, bw},
+    MakeQuantizer(global_scale, quant_dc), fast != 0, &map, 20);
+  if (!status.ok()) return Code(status);
+  const auto ext = map.tile_extent();
+  for (size_t y = 0; y < ext.height; ++y) {
+    std::memcpy(y_to_x + y * ext.width, map.y_to_x_map().Row(y), ext.width);
+    std::memcpy(y_to_b + y * ext.width, map.y_to_b_map().Row(y), ext.width);
+  }
+  return 0;
+}
+
+// src/codec/quantization.cpp:741-806 then the Y/X/B quantisation of one
+// transform as ComputeQuantizedCoefficients performs it
+// (src/codec/reconstruction.cpp:565-700, without the DC parts).
+int ref_code_transform(int strategy, uint32_t global_scale, uint32_t quant_dc,
+                       int32_t raw_quant, float x_mult, float b_mult,
+                       float cfl_x, float cfl_b, float* coefficients /*3*n*/,
+                       int32_t* quantized /*3*n*/, int32_t* decided_raw) {
+  const auto type = static_cast<gjxl::AcStrategyType>(strategy);
+  const size_t n = gjxl::GetAcStrategyInfo(type)->coefficient_count();
+  const gjxl::Quantizer q = MakeQuantizer(global_scale, quant_dc);
+  gjxl::AdjustedAcQuantization decision;
+  gjxl::Status status = gjxl::SelectAdjustedAcQuantization(
+    type, q, raw_quant, {x_mult, 1.0f, b_mult},
+    {std::span<const float>(coefficients, n),
+     std::span<const float>(coefficients + n, n),
+     std::span<const float>(coefficients + 2 * n, n)}, &decision);
+  if (!status.ok()) return Code(status);
+  *decided_raw = decision.raw_quant;
+  status = gjxl::QuantizeAdjustedYAcBlock(
+    type, q, decision, std::span<const float>(coefficients + n, n),
+    std::span<int32_t>(quantized + n, n));
+  if (!status.ok()) return Code(status);
+  status = gjxl::DequantizeAcBlock(
+    type, q, decision.raw_quant, {.channel = gjxl::XybChannel::kY},
+    std::span<const int32_t>(quantized + n, n),
+    std::span<float>(coefficients + n, n));
+  if (!status.ok()) return Code(status);
+  for (size_t i = 0; i < n; ++i) {
+    coefficients[i] -= cfl_x * coefficients[n + i];
+    coefficients[2 * n + i] -= cfl_b * coefficients[n + i];
+  }
+  for (int c : {0, 2}) {
+    status = gjxl::QuantizeAcBlock(
+      type, q, decision.raw_quant,
+      {.channel = static_cast<gjxl::XybChannel>(c),
+       .matrix_multiplier = c == 0 ? x_mult : b_mult},
+      std::span<const float>(coefficients + c * n, n),
+      std::span<int32_t>(quantized + c * n, n));
+    if (!status.ok()) return Code(status);
+  }
+  return 0;
+}
+
+// src/codec/quantization.cpp:808-835
+int ref_dequantize(int strategy, uint32_t global_scale, uint32_t quant_dc,
+                   int32_t raw_quant, int channel, float mult,
+                   const int32_t* quantized, float* out) {
+  const auto type = static_cast<gjxl::AcStrategyType>(strategy);
+  const size_t n = gjxl::GetAcStrategyInfo(type)->coefficient_count();
+  return Code(gjxl::DequantizeAcBlock(
+    type, MakeQuantizer(global_scale, quant_dc), raw_quant,
+    {.channel = static_cast<gjxl::XybChannel>(channel), .matrix_multiplier = mult},
+    std::span<const int32_t>(quantized, n), std::span<float>(out, n)));
+}
+
+// src/codec/dc_conversion.cpp
+int ref_llf_to_dc(int strategy, const float* coefficients, float* dc) {
+  const auto type = static_cast<gjxl::AcStrategyType>(strategy);
+  const auto* info = gjxl::GetAcStrategyInfo(type);
+  return Code(gjxl::ConvertLowFrequenciesToDc(
+    type, std::span<const float>(coefficients, info->coefficient_count()),
+    {dc, info->covered_blocks, info->covered_blocks.width}));
+}
+
+int ref_dc_to_llf(int strategy, const float* dc, float* coefficients) {
+  const auto type = static_cast<gjxl::AcStrategyType>(strategy);
+  const auto* info = gjxl::GetAcStrategyInfo(type);
+  return Code(gjxl::ConvertDcToLowFrequencies(
+    type, {dc, info->covered_blocks, info->covered_blocks.width},
+    std::span<float>(coefficients, info->coefficient_count())));
+}
+
+// src/codec/vardct/dc_quantization.cpp (prediction-aware, weighted, +1 bit)
+int ref_quantize_dc(const float* dc, size_t bw, size_t bh,
+                    uint32_t global_scale, uint32_t quant_dc,
+                    int32_t* quantized, float* reconstructed) {
+  const size_t n = bw * bh;
+  gjxl::ConstImage3FView in;
+  gjxl::Image3I32View q;
+  gjxl::Image3FView r;
+  for (size_t c = 0; c < 3; ++c) {
+    in.plane[c] = {dc + c * n, {bw, bh}, bw};
+    q.plane[c] = {quantized + c * n, {bw, bh}, bw};
+    r.plane[c] = {reconstructed + c * n, {bw, bh}, bw};
+  }
+  return Code(gjxl::QuantizeDcCoefficients(
+    in, MakeQuantizer(global_scale, quant_dc), {q, r},
+    {gjxl::DcQuantizationMode::kPredictionAware,
+     gjxl::VarDctDcPrediction::kWeighted, 1}));
+}
+
+// src/codec/dc_smoothing.cpp
+int ref_smooth_dc(const float* dc, size_t bw, size_t bh, uint32_t global_scale,
+                  uint32_t quant_dc, float* out) {
+  return Code(gjxl::SmoothDcCoefficients(
+    ConstImage(dc, bw, bh), MakeQuantizer(global_scale, quant_dc),
+    MutableImage(out, bw, bh)));
+}
+
+// src/codec/epf.cpp:207-290 with neutral sharpness
+int ref_epf_inverse_sigma(const uint8_t* cells, size_t bw, size_t bh,
+                          const int32_t* raw, uint32_t global_scale,
+                          uint32_t quant_dc, float* out) {
+  gjxl::AcStrategyGrid grid;
+  gjxl::Status status = GridFromCells(cells, bw, bh, &grid);
+  if (!status.ok()) return Code(status);
+  std::vector<uint8_t> sharp(bw * bh, 4);
+  return Code(gjxl::ComputeEpfInverseSigma(
+    grid, {raw, {bw, bh}, bw}, MakeQuantizer(global_scale, quant_dc),
+    {sharp.data(), {bw, bh}, bw}, {}, {out, {bw, bh}, bw}));
+}
+
+// src/codec/loop_filter.cpp:14-65 with the default options
+int ref_loop_filters(const float* in, size_t width, size_t height,
+                     const float* inverse_sigma, float* out) {
+  const size_t bw = (width + 7) / 8, bh = (height + 7) / 8;
+  return Code(gjxl::ApplyLoopFilters(
+    ConstImage(in, width, height), {inverse_sigma, {bw, bh}, bw}, {},
+    MutableImage(out, width, height)));
 }
 
 }  // extern "C"
