@@ -77,8 +77,17 @@ _SIGNATURES = {
     "gjxl_amd_ac_strategy_costs":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P],
     "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
+    "gjxl_amd_aq_session_create": [_P, _U32, _U32, ctypes.POINTER(_P)],
+    "gjxl_amd_aq_session_destroy": [_P],
+    "gjxl_amd_aq_prepare": [_P, _P, _P, _P, _P, _F],
+    "gjxl_amd_aq_policy": [_P, ctypes.POINTER(_F), ctypes.POINTER(_F),
+                           ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_int32),
+                           ctypes.POINTER(ctypes.c_int32)],
+    "gjxl_amd_aq_evaluate": [_P, _P],
+    "gjxl_amd_aq_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
+                           ctypes.POINTER(_SZ)],
 }
-_VOID = {"gjxl_amd_device_destroy"}
+_VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
@@ -114,12 +123,14 @@ def pad8(v: int) -> int:
 class DeviceArray:
     """A device allocation with a numpy shape and dtype."""
 
-    def __init__(self, device: "Device", shape, dtype):
+    def __init__(self, device: "Device", shape, dtype, pointer: int | None = None):
         self.device = device
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
-        self.ptr = device._alloc(max(self.nbytes, 1))
+        # a borrowed pointer (a session buffer) is never freed here
+        self.owned = pointer is None
+        self.ptr = device._alloc(max(self.nbytes, 1)) if pointer is None else pointer
 
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
@@ -138,13 +149,13 @@ class DeviceArray:
         return self
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.owned:
             _check(library().gjxl_amd_free(self.device.handle, self.ptr))
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
-            if self.ptr and self.device.handle:
+            if self.ptr and self.owned and self.device.handle:
                 library().gjxl_amd_free(self.device.handle, self.ptr)
         except Exception:
             pass
@@ -279,6 +290,74 @@ class Device:
         _check(library().gjxl_amd_ac_strategy_search(
             self.handle, costs.ptr, bw * 8, bh * 8, target, cells.ptr))
         return cells
+
+
+# Session buffer ids (include/gjxl_amd.h GJXL_AMD_AQ_*), in order.
+AQ_BUFFERS = (
+    "offsets", "anchor_list", "anchor_count", "coefficients", "adjusted_field",
+    "stats", "select", "quantizer", "raw_quant", "y_to_x", "y_to_b",
+    "quantized", "dc", "dc_quantized", "dc_reconstructed", "dc_scratch",
+    "dc_smoothed", "inverse_sigma", "reconstructed", "scratch_a", "scratch_b",
+    "filtered", "linear",
+)
+
+
+class AqSession:
+    """Adaptive-quantization evaluation session (gjxl_amd_aq_session_*)."""
+
+    def __init__(self, device: Device, width: int, height: int):
+        self.device = device
+        self.width, self.height = width, height
+        self.pw, self.ph = pad8(width), pad8(height)
+        self.bw, self.bh = self.pw // 8, self.ph // 8
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_aq_session_create(
+            device.handle, width, height, ctypes.byref(handle)))
+        self.handle = handle.value
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_aq_session_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, opsin: DeviceArray, preprocessed: DeviceArray,
+                cells: DeviceArray, initial_quant_field: DeviceArray,
+                target: float = 1.0) -> None:
+        _check(library().gjxl_amd_aq_prepare(
+            self.handle, opsin.ptr, preprocessed.ptr, cells.ptr,
+            initial_quant_field.ptr, target))
+
+    def policy(self) -> dict:
+        lo, hi, qdc = _F(), _F(), _F()
+        xs, bs = ctypes.c_int32(), ctypes.c_int32()
+        _check(library().gjxl_amd_aq_policy(
+            self.handle, ctypes.byref(lo), ctypes.byref(hi), ctypes.byref(qdc),
+            ctypes.byref(xs), ctypes.byref(bs)))
+        return {"lower": lo.value, "upper": hi.value, "quant_dc": qdc.value,
+                "x_scale": xs.value, "b_scale": bs.value}
+
+    def evaluate(self, quant_field: DeviceArray) -> None:
+        _check(library().gjxl_amd_aq_evaluate(self.handle, quant_field.ptr))
+
+    def buffer(self, name: str, shape=None, dtype=np.float32) -> DeviceArray:
+        """A borrowed view of one session buffer (whole buffer by default)."""
+        ptr, size = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(library().gjxl_amd_aq_buffer(
+            self.handle, AQ_BUFFERS.index(name), ctypes.byref(ptr),
+            ctypes.byref(size)))
+        dtype = np.dtype(dtype)
+        if shape is None:
+            shape = (size.value // dtype.itemsize,)
+        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value)
+        if view.nbytes > size.value:
+            raise ValueError(f"{name}: view of {view.nbytes} bytes exceeds {size.value}")
+        return view
 
 
 def gpu_available() -> bool:

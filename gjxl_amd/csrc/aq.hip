@@ -1,0 +1,1057 @@
+// a11-a13: adaptive-quantization setup and one evaluation up to the
+// reconstructed, Gaborish-inverted-domain XYB image.
+//
+//   AnchorLayoutKernel      per 256x256 AC group: anchors in raster order,
+//                           their coefficient offsets and the group lists
+//                           (src/codec/reconstruction.cpp:386-760 layout)
+//   AnchorDctKernel         forward DCT of every selected transform, once
+//   AdjustFieldKernel       AdjustQuantField (adaptive_quantization.cpp:740)
+//   MatrixStatsKernel       X/B matrix-scale statistics (workflow.cpp:1318)
+//   RangeKernel             min/max of the adjusted field
+//   Radix*Kernel            exact median and MAD of the quant field
+//   RawQuantKernel          CreateQuantizerFromField's raw quant
+//   FinalCflKernel          ComputeFinalColorCorrelationMap (fast LS)
+//   CodeTransformKernel     one transform of ComputeQuantizedCoefficients
+//   DcQuantKernel           prediction-aware DC, weighted predictor,
+//                           evaluated as a wavefront over each DC group
+//   EpfSigmaKernel          ComputeEpfInverseSigma
+//   SmoothDcKernel          SmoothDcCoefficients
+//   ReconstructKernel       ReconstructQuantizedCoefficients
+#include <cmath>
+#include <cstring>
+
+#include "aq_common.h"
+#include "common.h"
+#include "tables.h"
+
+namespace gjxl_amd {
+namespace {
+
+// Fills words with a value (a kernel, so that the sequence stays capturable
+// in a HIP graph without memset nodes).
+__global__ void FillWordsKernel(uint32_t* __restrict__ p, int n, uint32_t value) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = value;
+}
+
+// ---- anchor layout ---------------------------------------------------------
+
+__device__ __forceinline__ bool SearchedStrategy(int st) {
+  return st == kDct8 || st == kDct16x16 || st == kDct32x32 || st == kDct16x8 ||
+         st == kDct8x16 || st == kDct32x16 || st == kDct16x32;
+}
+
+// One thread per group walks its cells in raster order.  The grid is
+// validated on the way (every cell covered by exactly one transform of a
+// searched strategy, footprints inside the group): a group that fails gets
+// no anchors and raises *error, so that no later kernel indexes past the
+// group's 65536 coefficient slots.
+__global__ __launch_bounds__(64) void AnchorLayoutKernel(
+  const uint8_t* __restrict__ cells, int bw, int bh,
+  uint32_t* __restrict__ offsets, uint32_t* __restrict__ anchor_list,
+  uint32_t* __restrict__ anchor_count, uint32_t* __restrict__ error) {
+  __shared__ uint32_t covered[kGroupBlocks];  // one bit per cell of the group
+  const int gw = (bw + kGroupBlocks - 1) / kGroupBlocks;
+  const int g = blockIdx.x;
+  const int gx0 = (g % gw) * kGroupBlocks, gy0 = (g / gw) * kGroupBlocks;
+  const int gx1 = min(bw, gx0 + kGroupBlocks), gy1 = min(bh, gy0 + kGroupBlocks);
+  if (threadIdx.x < kGroupBlocks) covered[threadIdx.x] = 0;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  uint32_t used = 0, count = 0;
+  bool valid = true;
+  for (int y = gy0; y < gy1 && valid; ++y) {
+    for (int x = gx0; x < gx1; ++x) {
+      const uint8_t c = cells[static_cast<size_t>(y) * bw + x];
+      const uint32_t bit = 1u << (x - gx0);
+      if (!(c & 1)) {
+        offsets[static_cast<size_t>(y) * bw + x] = kNoAnchor;
+        if (!(covered[y - gy0] & bit)) { valid = false; break; }
+        continue;
+      }
+      const int st = c >> 1;
+      if (!SearchedStrategy(st)) { valid = false; break; }
+      const int cw = CoveredW(st), ch = CoveredH(st);
+      if (x + cw > gx1 || y + ch > gy1) { valid = false; break; }
+      const uint32_t span = ((1u << cw) - 1u) << (x - gx0);
+      for (int dy = 0; dy < ch; ++dy) {
+        if (covered[y - gy0 + dy] & span) valid = false;
+        covered[y - gy0 + dy] |= span;
+      }
+      if (!valid) break;
+      offsets[static_cast<size_t>(y) * bw + x] =
+        static_cast<uint32_t>(g) * kGroupCoefficients + used;
+      used += cw * ch * 64;
+      anchor_list[static_cast<size_t>(g) * kGroupAnchors + count] =
+        (static_cast<uint32_t>(y) << 16) | static_cast<uint32_t>(x);
+      ++count;
+    }
+  }
+  if (!valid) {
+    count = 0;
+    atomicOr(error, 1u);
+  }
+  anchor_count[g] = count;
+}
+
+struct AnchorRef {
+  int bx, by, strategy, cw, ch, W, H, N;
+  uint32_t offset;
+};
+
+__device__ __forceinline__ bool LoadAnchor(const uint32_t* list,
+                                           const uint32_t* count,
+                                           const uint8_t* cells,
+                                           const uint32_t* offsets, int bw,
+                                           AnchorRef* a) {
+  const int g = blockIdx.y, k = blockIdx.x;
+  if (static_cast<uint32_t>(k) >= count[g]) return false;
+  const uint32_t packed = list[static_cast<size_t>(g) * kGroupAnchors + k];
+  a->bx = packed & 0xffff;
+  a->by = packed >> 16;
+  a->strategy = cells[static_cast<size_t>(a->by) * bw + a->bx] >> 1;
+  a->cw = CoveredW(a->strategy);
+  a->ch = CoveredH(a->strategy);
+  a->W = a->cw * 8;
+  a->H = a->ch * 8;
+  a->N = a->W * a->H;
+  a->offset = offsets[static_cast<size_t>(a->by) * bw + a->bx];
+  return true;
+}
+
+constexpr int kAnchorThreads = 64;
+
+// Forward DCT of the three channels of every anchor into the coefficient
+// store (3 planes of groups * 65536 floats).
+__global__ __launch_bounds__(kAnchorThreads) void AnchorDctKernel(
+  const float* __restrict__ opsin, int width, int height,
+  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
+  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+  const double* __restrict__ bases, size_t store, float* __restrict__ coefs) {
+  __shared__ float pix[3 * 1024];
+  __shared__ double tmp[3 * 1024];
+  __shared__ float out[3 * 1024];
+  AnchorRef a;
+  if (!LoadAnchor(list, count, cells, offsets, width / 8, &a)) return;
+  const size_t plane = static_cast<size_t>(width) * height;
+  for (int e = threadIdx.x; e < 3 * a.N; e += kAnchorThreads) {
+    const int c = e / a.N, r = e % a.N, y = r / a.W, x = r % a.W;
+    pix[e] = opsin[c * plane + static_cast<size_t>(a.by * 8 + y) * width + a.bx * 8 + x];
+  }
+  __syncthreads();
+  ForwardDctLds(bases, a.W, a.H, 3, pix, tmp, out);
+  for (int e = threadIdx.x; e < 3 * a.N; e += kAnchorThreads) {
+    const int c = e / a.N, i = e % a.N;
+    coefs[c * store + a.offset + i] = out[e];
+  }
+}
+
+// AdjustQuantField (src/codec/adaptive_quantization.cpp:740-835), one
+// thread per anchor.
+__global__ __launch_bounds__(256) void AdjustFieldKernel(
+  const uint8_t* __restrict__ cells, int bw, int bh, float mixer,
+  const float* __restrict__ in, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= bw) return;
+  const uint8_t c = cells[static_cast<size_t>(y) * bw + x];
+  if (!(c & 1)) return;
+  const int cw = CoveredW(c >> 1), ch = CoveredH(c >> 1);
+  float maximum = in[static_cast<size_t>(y) * bw + x];
+  float mean = 0.0f;
+  for (int dy = 0; dy < ch; ++dy) {
+    for (int dx = 0; dx < cw; ++dx) {
+      const float v = in[static_cast<size_t>(y + dy) * bw + x + dx];
+      maximum = DStdMax(maximum, v);
+      mean += v;
+    }
+  }
+  const int n = cw * ch;
+  mean /= static_cast<float>(n);
+  float result = maximum;
+  if (n >= 4) result = maximum * mixer + mean * (1.0f - mixer);
+  for (int dy = 0; dy < ch; ++dy)
+    for (int dx = 0; dx < cw; ++dx) out[static_cast<size_t>(y + dy) * bw + x + dx] = result;
+}
+
+// Non-negative float maxima through their bit patterns (order-free).
+__global__ __launch_bounds__(256) void MatrixStatsKernel(
+  const float* __restrict__ opsin, int width, int height, int stride,
+  size_t plane, uint32_t* __restrict__ stats) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  float xe = 0.0f, be = 0.0f, ex = 0.0f;
+  if (x >= 1 && x < width && y >= 1) {
+    const float* X = opsin;
+    const float* Y = opsin + plane;
+    const float* B = opsin + 2 * plane;
+    const size_t i = static_cast<size_t>(y) * stride + x, l = i - 1, u = i - stride;
+    xe = DStdMax(fabsf(X[i] - X[l]), fabsf(X[i] - X[u]));
+    const float d = B[i] - Y[i];
+    be = DStdMax(fabsf(d - (B[l] - Y[l])), fabsf(d - (B[u] - Y[u])));
+    float e = B[i] - 1.2f * Y[i];
+    if (e >= 0.0f) {
+      e *= fabsf(B[i] - B[l]) + fabsf(B[i] - B[u]);
+      ex = e;
+    }
+  }
+  // wave-level max, then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    xe = DStdMax(xe, __shfl_xor(xe, off));
+    be = DStdMax(be, __shfl_xor(be, off));
+    ex = DStdMax(ex, __shfl_xor(ex, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&stats[0], __float_as_uint(xe));
+    atomicMax(&stats[1], __float_as_uint(be));
+    atomicMax(&stats[2], __float_as_uint(ex));
+  }
+}
+
+// min / max of a positive field via bit patterns.
+__global__ __launch_bounds__(256) void RangeKernel(const float* __restrict__ v,
+                                                   size_t n,
+                                                   uint32_t* __restrict__ range) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint32_t lo = 0xffffffffu, hi = 0;
+  if (i < n) {
+    lo = __float_as_uint(v[i]);
+    hi = lo;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), off)));
+    hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), off)));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&range[0], lo);
+    atomicMax(&range[1], hi);
+  }
+}
+
+// ---- exact median / MAD (radix select on non-negative float bits) ----------
+
+// state: [0] prefix, [1] k (rank within the prefix), [2] median bits,
+// [3..] unused; hist: 256 counters.
+__device__ __forceinline__ uint32_t SelectKey(const float* v, size_t i,
+                                              int pass_kind, uint32_t median) {
+  if (pass_kind == 0) return __float_as_uint(v[i]);
+  return __float_as_uint(fabsf(v[i] - __uint_as_float(median)));
+}
+
+__global__ __launch_bounds__(256) void RadixHistogramKernel(
+  const float* __restrict__ v, size_t n, int shift, int pass_kind,
+  const uint32_t* __restrict__ state, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t local[256];
+  local[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t prefix = state[0];
+  const uint32_t mask = shift == 24 ? 0u : (0xffffffffu << (shift + 8));
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const uint32_t key = SelectKey(v, i, pass_kind, state[2]);
+    if ((key & mask) == (prefix & mask)) atomicAdd(&local[(key >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  if (local[threadIdx.x]) atomicAdd(&hist[threadIdx.x], local[threadIdx.x]);
+}
+
+__global__ void RadixPickKernel(uint32_t* __restrict__ hist, int shift,
+                                int pass_kind, uint32_t* __restrict__ state) {
+  if (threadIdx.x != 0) return;
+  uint32_t k = state[1];
+  uint32_t bin = 0;
+  for (; bin < 256; ++bin) {
+    if (k < hist[bin]) break;
+    k -= hist[bin];
+  }
+  state[0] |= bin << shift;
+  state[1] = k;
+  for (int i = 0; i < 256; ++i) hist[i] = 0;
+  if (shift == 0 && pass_kind == 0) state[2] = state[0];  // median found
+  if (shift == 0 && pass_kind == 1) state[3] = state[0];  // MAD found
+}
+
+__global__ void RadixResetKernel(uint32_t* state, uint32_t k, int pass_kind) {
+  state[0] = 0;
+  state[1] = k;
+  (void)pass_kind;
+}
+
+// ComputeQuantizerParams + Quantizer::Create (src/codec/quantization.cpp:
+// 59-115, core/quantizer.h:33-82), one thread.
+__global__ void QuantizerParamsKernel(const uint32_t* __restrict__ state,
+                                      float quant_dc,
+                                      QuantizerState* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  const float median = __uint_as_float(state[2]);
+  const float mad = __uint_as_float(state[3]);
+  float scale = 65536.0f * (median - mad) / 5.0f;
+  scale = scale < 1.0f ? 1.0f : (32768.0f < scale ? 32768.0f : scale);
+  int32_t global_scale = static_cast<int32_t>(scale);
+  const int32_t scaled_dc =
+    static_cast<int32_t>(static_cast<double>(quant_dc * 4096.0f) * 1.6);
+  if (global_scale > scaled_dc) global_scale = scaled_dc > 1 ? scaled_dc : 1;
+  const float inv_gs = 65536.0f / static_cast<float>(global_scale);
+  const float qdv = DStdMin(65536.0f, quant_dc * inv_gs + 0.5f);
+  const uint32_t qdc = static_cast<uint32_t>(qdv);
+  QuantizerState q;
+  q.global_scale = static_cast<uint32_t>(global_scale);
+  q.quant_dc = qdc;
+  q.scale = static_cast<float>(q.global_scale) / 65536.0f;
+  q.inverse_global_scale = 65536.0f / static_cast<float>(q.global_scale);
+  const float inv_qdc = q.inverse_global_scale / static_cast<float>(qdc);
+  constexpr float kInvDc[3] = {4096.0f, 512.0f, 256.0f};
+  for (int c = 0; c < 3; ++c) {
+    q.dc_steps[c] = inv_qdc / kInvDc[c];
+    q.inverse_dc_steps[c] = kInvDc[c] * q.scale * static_cast<float>(qdc);
+  }
+  *out = q;
+}
+
+__global__ __launch_bounds__(256) void RawQuantKernel(
+  const float* __restrict__ field, size_t n, const QuantizerState* __restrict__ q,
+  int32_t* __restrict__ raw) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = field[i] * q->inverse_global_scale + 0.5f;
+  r = r < 1.0f ? 1.0f : (256.0f < r ? 256.0f : r);
+  raw[i] = static_cast<int32_t>(r);
+}
+
+// ---- final CfL, fast least squares (chroma_from_luma.cpp:543-670, :112-
+// 134), one work group per 64x64 colour tile.  Sixteen threads run the
+// reference's four-lane fma chains (2 maps x {quadratic, linear} x 4 lanes)
+// over the tile's anchors in raster order.
+
+__global__ __launch_bounds__(64) void FinalCflKernel(
+  const float* __restrict__ coefs, size_t store,
+  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
+  int bw, int bh, const int32_t* __restrict__ raw,
+  const QuantizerState* __restrict__ q, const float* __restrict__ tables,
+  int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
+  __shared__ float lanes[2][2][4];
+  __shared__ int total;
+  const int tx = blockIdx.x, ty = blockIdx.y;
+  const int t = threadIdx.x;
+  if (t == 0) total = 0;
+  __syncthreads();
+  if (t < 16) {
+    const int problem = t / 8, kind = (t / 4) % 2, lane = t % 4;
+    float acc = 0.0f;
+    int n_total = 0;
+    const float scale128 = q->scale * 128.0f;
+    for (int by = ty * 8; by < min(bh, ty * 8 + 8); ++by) {
+      for (int bx = tx * 8; bx < min(bw, tx * 8 + 8); ++bx) {
+        const uint8_t c = cells[static_cast<size_t>(by) * bw + bx];
+        if (!(c & 1)) continue;
+        const int st = c >> 1;
+        const int cw = CoveredW(st), ch = CoveredH(st);
+        const int W = cw * 8, H = ch * 8, N = W * H;
+        const int cols = W > H ? W : H;
+        const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
+        const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
+        const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
+        const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
+        for (int i = lane; i < N; i += 4) {
+          const int xx = i % cols, yy = i / cols;
+          const bool llf = xx < lw && yy < lh;
+          const float cy = llf ? 0.0f : coefs[store + off + i];
+          const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
+          const float luma = cy * inv[i] * qs;
+          const float chroma = cc * inv[i] * qs;
+          const float a = luma / 84.0f;
+          const float b = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
+          acc = kind == 0 ? __builtin_fmaf(a, a, acc) : __builtin_fmaf(a, b, acc);
+        }
+        n_total += N;
+      }
+    }
+    lanes[problem][kind][lane] = acc;
+    if (t == 0) total = n_total;
+  }
+  __syncthreads();
+  if (t < 2) {
+    const float (*l)[4] = lanes[t];
+    const float quad = (l[0][0] + l[0][1]) + (l[0][2] + l[0][3]);
+    const float lin = (l[1][0] + l[1][1]) + (l[1][2] + l[1][3]);
+    int8_t r = 0;
+    if (total > 0) {
+      const float x = -lin / (quad + static_cast<float>(total) * 1.0e-9f * 0.5f);
+      float v = x;
+      if (v >= 2.6f) v -= 2.6f;
+      else if (v <= -2.6f) v += 2.6f;
+      else v = 0.0f;
+      float rr = roundf(v);
+      rr = rr < -128.0f ? -128.0f : (127.0f < rr ? 127.0f : rr);
+      r = static_cast<int8_t>(rr);
+    }
+    const int tile = ty * ((bw + 7) / 8) + tx;
+    if (t == 0) y_to_x[tile] = r; else y_to_b[tile] = r;
+  }
+}
+
+// ---- coefficient coding, one wave per transform ---------------------------
+
+struct CodingParams {
+  float x_mult;
+  float b_mult;
+  DcBasisTable dc_basis;
+};
+
+// AdjustQuantForChannel (src/codec/quantization.cpp:292-533), one thread.
+__device__ void AdjustQuantForChannel(float qscale, int channel, float mult,
+                                      int strategy, int cols, int rows,
+                                      const float* inv, const float* coef,
+                                      float* thr, int32_t* raw_quant) {
+  const int xsize = cols / 8, ysize = rows / 8;
+  const float qac = qscale * static_cast<float>(*raw_quant);
+  if (xsize > 1 || ysize > 1) {
+    float reduction = 0.003f * static_cast<float>(xsize * ysize);
+    reduction = reduction < 0.0f ? 0.0f : (0.08f < reduction ? 0.08f : reduction);
+    for (int i = 0; i < 4; ++i) thr[i] = DStdMax(0.54f, thr[i] - reduction);
+  }
+  float border = 0.0f, err_sum = 0.0f, val_sum = 0.0f;
+  float nz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int y = 0; y < rows; ++y) {
+    for (int x = 0; x < cols; ++x) {
+      if (x < xsize && y < ysize) continue;
+      const int index = y * cols + x;
+      const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
+      const float value = coef[index] * (inv[index] * qac * mult);
+      const float quantized = fabsf(value) < thr[quadrant] ? 0.0f : rintf(value);
+      const float error = fabsf(value - quantized);
+      err_sum += error;
+      val_sum += fabsf(quantized);
+      if (channel == 1 && quantized == 0.0f) mx[quadrant] = DStdMax(mx[quadrant], error);
+      if (quantized != 0.0f) {
+        nz[quadrant] += fabsf(quantized);
+        const bool in_corner = y >= 7 * ysize && x >= 7 * xsize;
+        const bool on_border = y + 1 == rows || x + 1 == cols;
+        const bool in_larger = x >= 4 * xsize && y >= 4 * ysize;
+        if (in_corner || (on_border && in_larger)) border += fabsf(value);
+      }
+    }
+  }
+  if (channel == 1 && val_sum * 8.0f < static_cast<float>(xsize * ysize)) {
+    const double limit = 0.46, multiplier = 0.9999;
+    const int32_t original = *raw_quant;
+    int32_t new_quant = *raw_quant;
+    for (int qd = 1; qd < 4; ++qd) {
+      if (nz[qd] == 0.0f && mx[qd] > limit) {
+        new_quant = original + 1;
+        break;
+      }
+    }
+    *raw_quant = new_quant;
+    if (nz[3] == 0.0f && mx[3] > limit) {
+      thr[3] = static_cast<float>(multiplier * mx[3] * new_quant / original);
+    } else if ((nz[1] == 0.0f && mx[1] > limit) || (nz[2] == 0.0f && mx[2] > limit)) {
+      thr[1] = static_cast<float>(multiplier * DStdMax(mx[1], mx[2]) * new_quant / original);
+      thr[2] = thr[1];
+    } else if (nz[0] == 0.0f && mx[0] > limit) {
+      thr[0] = static_cast<float>(multiplier * mx[0] * new_quant / original);
+    }
+  }
+  const float all_nz = nz[0] + nz[1] + nz[2] + nz[3] + 1.0f;
+  constexpr float kBorder[3] = {70.0f, 30.0f, 60.0f};
+  if (kBorder[channel] * border >= all_nz) {
+    *raw_quant = static_cast<int32_t>(static_cast<float>(*raw_quant) +
+                                      kBorder[channel] * border / all_nz);
+    if (*raw_quant >= 256) *raw_quant = 255;
+  }
+  if (strategy == kDct8 && nz[0] + nz[1] + nz[2] + nz[3] < 11.0f) {
+    ++*raw_quant;
+    if (*raw_quant >= 256) *raw_quant = 255;
+  }
+  constexpr double kFirst[4][3] = {
+    {0.22080615753848404, 0.45797479824262011, 0.29859235095977965},
+    {0.70109486510286834, 0.16185281305512639, 0.14387691730035473},
+    {0.114985964456218638, 0.44656840441027695, 0.10587658215149048},
+    {0.46849665264409396, 0.41239077937781954, 0.088667407767185444},
+  };
+  constexpr double kSecond[4][3] = {
+    {0.27450281941822197, 1.1255766549984996, 0.98950459134128388},
+    {0.4652168675598285, 0.40945807983455818, 0.36581899811751367},
+    {0.28034972424715715, 0.9182653201929738, 1.5581531543057416},
+    {0.26873118114033728, 0.68863712390392484, 1.2082185408666786},
+  };
+  err_sum = static_cast<float>(static_cast<double>(err_sum) * 2.2942708343284721);
+  val_sum = static_cast<float>(static_cast<double>(val_sum) * 2.2942708343284721);
+  if (strategy >= kDct16x16) {
+    int cls = 3;
+    if (strategy == kDct32x16 || strategy == kDct16x32) cls = 1;
+    else if (strategy == kDct16x16) cls = 0;
+    else if (strategy == kDct32x32) cls = 2;
+    const double threshold =
+      kFirst[cls][channel] * xsize * ysize * 8 * 8 + kSecond[cls][channel] * val_sum;
+    int32_t step = static_cast<int32_t>(err_sum / threshold);
+    step = step < 0 ? 0 : (step > 2 ? 2 : step);
+    if (err_sum > threshold) {
+      *raw_quant += step;
+      if (*raw_quant >= 256) *raw_quant = 255;
+    }
+  }
+  const int32_t divisor = xsize * ysize;
+  float min_nz = nz[0];
+  for (int qd = 1; qd < 4; ++qd) if (nz[qd] < min_nz) min_nz = nz[qd];
+  int32_t activity = 15;
+  if (min_nz < 15.0f * divisor) activity = (static_cast<int32_t>(min_nz) + divisor / 2) / divisor;
+  int32_t adjusted = *raw_quant - activity;
+  if (channel == 1) for (int qd = 1; qd < 4; ++qd) thr[qd] += 0.01f * activity;
+  const int32_t floor_quant = max(4, *raw_quant / 2);
+  if (adjusted < floor_quant) adjusted = floor_quant;
+  *raw_quant = adjusted;
+}
+
+__global__ __launch_bounds__(kAnchorThreads) void CodeTransformKernel(
+  const float* __restrict__ coefs, size_t store,
+  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
+  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+  int bw, int32_t* __restrict__ raw, const int8_t* __restrict__ y_to_x,
+  const int8_t* __restrict__ y_to_b, const QuantizerState* __restrict__ qstate,
+  const float* __restrict__ tables, CodingParams params,
+  int32_t* __restrict__ quantized, float* __restrict__ dc, size_t dc_plane) {
+  __shared__ float coef[3 * 1024];
+  __shared__ float thr_y[4];
+  __shared__ int32_t cand[3];
+  __shared__ int32_t decided;
+  AnchorRef a;
+  if (!LoadAnchor(list, count, cells, offsets, bw, &a)) return;
+  const QuantizerState q = *qstate;
+  const int N = a.N;
+  const int cols = a.W > a.H ? a.W : a.H, rows = a.W > a.H ? a.H : a.W;
+  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
+    coef[e] = coefs[(e / N) * store + a.offset + e % N];
+  }
+  __syncthreads();
+  const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
+  const int32_t initial = raw[static_cast<size_t>(a.by) * bw + a.bx];
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    float thr[4] = {0.58f, 0.64f, 0.64f, 0.64f};
+    int32_t c = initial;
+    AdjustQuantForChannel(q.scale, k, mults[k], a.strategy, cols, rows,
+                          tables + InverseOffset(a.strategy, k), coef + k * N,
+                          thr, &c);
+    cand[k] = c;
+    if (k == 1) for (int i = 0; i < 4; ++i) thr_y[i] = thr[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // max over Y, X, B (starting from zero) as SelectAdjustedAcQuantization
+    int32_t d = 0;
+    const int order[3] = {1, 0, 2};
+    for (int i = 0; i < 3; ++i) d = cand[order[i]] > d ? cand[order[i]] : d;
+    decided = d;
+    raw[static_cast<size_t>(a.by) * bw + a.bx] = d;
+    // Y DC from the unmodified Y coefficients
+    float dcv[16];
+    LlfToDc(params.dc_basis, a.cw, a.ch, coef + N, dcv);
+    for (int dy = 0; dy < a.ch; ++dy)
+      for (int dx = 0; dx < a.cw; ++dx)
+        dc[dc_plane + static_cast<size_t>(a.by + dy) * bw + a.bx + dx] = dcv[dy * a.cw + dx];
+  }
+  __syncthreads();
+  const int32_t d = decided;
+  // Y: quantise with the adjusted thresholds, store, dequantise in place
+  {
+    const float* inv = tables + InverseOffset(a.strategy, 1);
+    const float* dq = tables + DequantOffset(a.strategy, 1);
+    const float qscale = q.scale * static_cast<float>(d) * 1.0f;
+    const float dscale = q.inverse_global_scale / static_cast<float>(d) / 1.0f;
+    for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
+      const int x = i % cols, y = i / cols;
+      const int t = (y >= rows / 2) * 2 + (x >= cols / 2);
+      const float value = inv[i] * qscale * coef[N + i];
+      int32_t qv = 0;
+      if (!(fabsf(value) < thr_y[t])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
+      quantized[store + a.offset + i] = qv;
+      coef[N + i] = QuantBias(qv, 1) * (dq[i] * dscale);
+    }
+  }
+  __syncthreads();
+  const int tile = (a.by / 8) * ((bw + 7) / 8) + a.bx / 8;
+  const float fx = static_cast<float>(y_to_x[tile]) / 84.0f;
+  const float fb = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+  for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
+    coef[i] -= fx * coef[N + i];
+    coef[2 * N + i] -= fb * coef[N + i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int k = threadIdx.x * 2;
+    float dcv[16];
+    LlfToDc(params.dc_basis, a.cw, a.ch, coef + k * N, dcv);
+    for (int dy = 0; dy < a.ch; ++dy)
+      for (int dx = 0; dx < a.cw; ++dx)
+        dc[k * dc_plane + static_cast<size_t>(a.by + dy) * bw + a.bx + dx] =
+          dcv[dy * a.cw + dx];
+  }
+  const int covered = a.cw * a.ch;
+  for (int k = 0; k < 3; k += 2) {
+    float thr[4] = {0.58f, 0.62f, 0.62f, 0.62f};
+    if (covered >= 4)
+      for (int t = 0; t < 4; ++t)
+        thr[t] = DStdMax(0.5f, thr[t] - 0.00744f * static_cast<float>(covered));
+    const float* inv = tables + InverseOffset(a.strategy, k);
+    const float qscale = q.scale * static_cast<float>(d) * mults[k];
+    for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
+      const int x = i % cols, y = i / cols;
+      const int t = (y >= rows / 2) * 2 + (x >= cols / 2);
+      const float value = inv[i] * qscale * coef[k * N + i];
+      int32_t qv = 0;
+      if (!(fabsf(value) < thr[t])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
+      quantized[k * store + a.offset + i] = qv;
+    }
+  }
+}
+
+// ---- prediction-aware DC with the weighted predictor ----------------------
+//
+// src/codec/vardct/dc_quantization.cpp:121-240 with
+// src/codec/modular/prediction.h:59-153.  The predictor's two-row buffers
+// are unrolled into full per-group planes: the "north" error it reads at
+// (x, y) is e(x, y-1) + e(x-1, y) because the update of (x-1, y) adds its
+// errors into the previous row's slot x; "north-west" is
+// e(x-1, y-1) + e(x-2, y); "north-east" e(x+1, y-1).  Every input of (x, y)
+// is therefore final at wavefront step t = x + 2y, and one work group walks
+// the wavefront of one DC group (<= 256 x 256 blocks) per channel.
+
+constexpr int kDcGroup = 256;
+constexpr int kDcThreads = 256;
+
+struct WpPlanes {
+  uint32_t* e[4];  // per-sample sub-predictor errors
+  int32_t* err;    // signed prediction error
+};
+
+__device__ __forceinline__ uint32_t Reciprocal(uint32_t i) {
+  return (1u << 24) / (i + 1);
+}
+
+__device__ __forceinline__ int BitWidth64(uint64_t v) {
+  return v == 0 ? 0 : 64 - __clzll(static_cast<long long>(v));
+}
+
+__global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
+  const float* __restrict__ dc, int bw, int bh,
+  const QuantizerState* __restrict__ qstate, int channel, int extra_precision,
+  int32_t* __restrict__ quantized, uint32_t* __restrict__ scratch) {
+  const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
+  const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
+  const size_t n = static_cast<size_t>(bw) * bh;
+  const size_t group = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
+  // scratch per group: 5 planes of 256*256
+  uint32_t* base = scratch + group * 5 * kDcGroup * kDcGroup;
+  WpPlanes p;
+  for (int i = 0; i < 4; ++i) p.e[i] = base + i * kDcGroup * kDcGroup;
+  p.err = reinterpret_cast<int32_t*>(base + 4 * kDcGroup * kDcGroup);
+  const QuantizerState q = *qstate;
+  const float precision = static_cast<float>(1u << extra_precision);
+  const float inv = q.inverse_dc_steps[channel] * precision;
+  const float step_y = q.dc_steps[1] / precision;
+  int32_t* plane = quantized + channel * n;
+  constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
+  constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
+  const int steps = w + 2 * h;
+  for (int t = 0; t < steps; ++t) {
+    // samples on this wavefront: x = t - 2y
+    for (int y = threadIdx.x; y < h; y += kDcThreads) {
+      const int x = t - 2 * y;
+      if (x < 0 || x >= w) continue;
+      const size_t index = static_cast<size_t>(gy0 + y) * bw + gx0 + x;
+      const int64_t W_ = x ? plane[index - 1] : y ? plane[index - bw] : 0;
+      const int64_t N_ = y ? plane[index - bw] : W_;
+      const int64_t NW_ = x && y ? plane[index - bw - 1] : W_;
+      const int64_t NE_ = y && x + 1 < w ? plane[index - bw + 1] : N_;
+      const int64_t NN_ = y > 1 ? plane[index - 2 * bw] : N_;
+      auto E = [&](int i, int xx, int yy) -> uint32_t {
+        if (xx < 0 || yy < 0) return 0u;
+        return p.e[i][yy * kDcGroup + xx];
+      };
+      auto R = [&](int xx, int yy) -> int64_t {
+        if (xx < 0 || yy < 0) return 0;
+        return p.err[yy * kDcGroup + xx];
+      };
+      // Predict (prediction.h:75-123)
+      uint32_t weights[4];
+      uint32_t weight_sum = 0;
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t north = E(i, x, y - 1) + E(i, x - 1, y);
+        const uint32_t ne = x + 1 < w ? E(i, x + 1, y - 1) : north;
+        const uint32_t nw = x ? E(i, x - 1, y - 1) + E(i, x - 2, y) : north;
+        const uint32_t e = north + ne + nw;
+        int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
+        if (shift < 0) shift = 0;
+        weights[i] = 4 + ((kMaxWeights[i] * Reciprocal(e >> shift)) >> shift);
+        weight_sum += weights[i];
+      }
+      const unsigned log_weight = BitWidth64(weight_sum) - 1;
+      weight_sum = 0;
+      for (int i = 0; i < 4; ++i) {
+        weights[i] >>= log_weight - 4;
+        weight_sum += weights[i];
+      }
+      const int64_t ew = x ? R(x - 1, y) : 0;
+      const int64_t en = R(x, y - 1);
+      const int64_t enw = x ? R(x - 1, y - 1) : en;
+      const int64_t ene = x + 1 < w ? R(x + 1, y - 1) : en;
+      const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
+      int64_t pred[4];
+      pred[0] = w8 + ne8 - n8;
+      pred[1] = n8 - (((en + ew + ene) * kCoeff[0]) >> 5);
+      pred[2] = w8 - (((en + ew + enw) * kCoeff[1]) >> 5);
+      pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
+                       (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
+      int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
+      for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
+      int64_t prediction = (sum * static_cast<int64_t>(Reciprocal(weight_sum - 1))) >> 24;
+      if (((en ^ ew) | (en ^ enw)) <= 0) {
+        int64_t lo = w8 < ne8 ? w8 : ne8; lo = lo < n8 ? lo : n8;
+        int64_t hi = w8 > ne8 ? w8 : ne8; hi = hi > n8 ? hi : n8;
+        prediction = prediction < lo ? lo : (prediction > hi ? hi : prediction);
+      }
+      const int64_t predicted = (prediction + 3) >> 3;
+      // quantise the residual (dc_quantization.cpp:176-203)
+      float value = dc[channel * n + index];
+      if (channel == 2) {
+        value = __builtin_fmaf(-static_cast<float>(quantized[n + index]), step_y, value);
+      }
+      float residual = value * inv;
+      residual -= static_cast<float>(predicted);
+      if (residual > -0.62f && residual < 0.62f) residual = 0;
+      float rounded = roundf(residual);
+      if (rounded > 2 || rounded < -2) rounded = roundf(residual * 0.5f) * 2;
+      const int32_t ir = static_cast<int32_t>(roundf(rounded));
+      const int32_t value_q = static_cast<int32_t>(predicted + ir);
+      plane[index] = value_q;
+      // Update (prediction.h:124-144)
+      const int64_t scaled = static_cast<int64_t>(value_q) * 8;
+      p.err[y * kDcGroup + x] = static_cast<int32_t>(prediction - scaled);
+      for (int i = 0; i < 4; ++i) {
+        int64_t dd = pred[i] - scaled;
+        if (dd < 0) dd = -dd;
+        p.e[i][y * kDcGroup + x] = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void DcReconstructKernel(
+  const int32_t* __restrict__ quantized, size_t n,
+  const QuantizerState* __restrict__ qstate, int extra_precision,
+  float* __restrict__ reconstructed) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float precision = static_cast<float>(1u << extra_precision);
+  const QuantizerState q = *qstate;
+  const float ry = static_cast<float>(quantized[n + i]) * (q.dc_steps[1] / precision);
+  for (int c = 0; c < 3; ++c) {
+    float r = static_cast<float>(quantized[c * n + i]) * (q.dc_steps[c] / precision);
+    if (c == 2) r = __builtin_fmaf(ry, 1.0f, r);
+    reconstructed[c * n + i] = r;
+  }
+}
+
+// ComputeEpfInverseSigma (src/codec/epf.cpp:207-290), neutral sharpness.
+__global__ __launch_bounds__(256) void EpfSigmaKernel(
+  const uint8_t* __restrict__ cells, int bw, int bh,
+  const int32_t* __restrict__ raw, const QuantizerState* __restrict__ q,
+  float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= bw) return;
+  const uint8_t c = cells[static_cast<size_t>(y) * bw + x];
+  if (!(c & 1)) return;
+  const float sigma_quant =
+    0.46f / (q->scale * static_cast<float>(raw[static_cast<size_t>(y) * bw + x]) *
+             -1.1715728752538099024f);
+  float sigma = sigma_quant * (4.0f / 7.0f);
+  sigma = DStdMin(-1.0e-4f, sigma);
+  const float value = 1.0f / sigma;
+  const int cw = CoveredW(c >> 1), ch = CoveredH(c >> 1);
+  for (int dy = 0; dy < ch; ++dy)
+    for (int dx = 0; dx < cw; ++dx) out[static_cast<size_t>(y + dy) * bw + x + dx] = value;
+}
+
+// SmoothDcCoefficients (src/codec/dc_smoothing.cpp).
+__global__ __launch_bounds__(256) void SmoothDcKernel(
+  const float* __restrict__ dc, int bw, int bh,
+  const QuantizerState* __restrict__ qstate, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= bw) return;
+  const size_t n = static_cast<size_t>(bw) * bh;
+  const size_t i = static_cast<size_t>(y) * bw + x;
+  if (y == 0 || x == 0 || y + 1 >= bh || x + 1 >= bw) {
+    for (int c = 0; c < 3; ++c) out[c * n + i] = dc[c * n + i];
+    return;
+  }
+  constexpr float kSide = 0.20345139757231578f, kCorner = 0.0334829185968739f;
+  constexpr float kCenter = 1.0f - 4.0f * (kSide + kCorner);
+  float center[3], smoothed[3];
+  float gap = 0.5f;
+  for (int c = 0; c < 3; ++c) {
+    const float* top = dc + c * n + static_cast<size_t>(y - 1) * bw;
+    const float* row = dc + c * n + static_cast<size_t>(y) * bw;
+    const float* bottom = dc + c * n + static_cast<size_t>(y + 1) * bw;
+    const float corners = (top[x - 1] + top[x + 1]) + (bottom[x - 1] + bottom[x + 1]);
+    const float sides = (row[x - 1] + row[x + 1]) + (top[x] + bottom[x]);
+    center[c] = row[x];
+    smoothed[c] = __builtin_fmaf(corners, kCorner,
+                                 __builtin_fmaf(sides, kSide, row[x] * kCenter));
+    gap = DStdMax(gap, fabsf((row[x] - smoothed[c]) / qstate->dc_steps[c]));
+  }
+  const float factor = DStdMax(0.0f, __builtin_fmaf(-4.0f, gap, 3.0f));
+  for (int c = 0; c < 3; ++c) {
+    out[c * n + i] = __builtin_fmaf(smoothed[c] - center[c], factor, center[c]);
+  }
+}
+
+// ReconstructQuantizedCoefficients (src/codec/reconstruction.cpp:790-936),
+// one wave per transform: dequantise, add CfL, restore the LLF from the
+// smoothed DC, inverse DCT into the padded XYB image.
+__global__ __launch_bounds__(kAnchorThreads) void ReconstructKernel(
+  const int32_t* __restrict__ quantized, size_t store,
+  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
+  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+  int width, int height, const int32_t* __restrict__ raw,
+  const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
+  const QuantizerState* __restrict__ qstate, const float* __restrict__ tables,
+  const double* __restrict__ bases, CodingParams params,
+  const float* __restrict__ smoothed_dc, float* __restrict__ out) {
+  __shared__ float coef[3 * 1024];
+  __shared__ double tmp[3 * 1024];
+  __shared__ float pix[3 * 1024];
+  const int bw = width / 8, bh = height / 8;
+  AnchorRef a;
+  if (!LoadAnchor(list, count, cells, offsets, bw, &a)) return;
+  const QuantizerState q = *qstate;
+  const int N = a.N;
+  const int32_t r = raw[static_cast<size_t>(a.by) * bw + a.bx];
+  const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
+  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
+    const int k = e / N, i = e % N;
+    const float* dq = tables + DequantOffset(a.strategy, k);
+    const float scale = q.inverse_global_scale / static_cast<float>(r) / mults[k];
+    const int32_t qv = quantized[k * store + a.offset + i];
+    coef[e] = QuantBias(qv, k) * (dq[i] * scale);
+  }
+  __syncthreads();
+  const int tile = (a.by / 8) * ((bw + 7) / 8) + a.bx / 8;
+  const float fx = static_cast<float>(y_to_x[tile]) / 84.0f;
+  const float fb = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+  for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
+    coef[i] += fx * coef[N + i];
+    coef[2 * N + i] += fb * coef[N + i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    const size_t n = static_cast<size_t>(bw) * bh;
+    float dcv[16];
+    for (int dy = 0; dy < a.ch; ++dy)
+      for (int dx = 0; dx < a.cw; ++dx)
+        dcv[dy * a.cw + dx] =
+          smoothed_dc[k * n + static_cast<size_t>(a.by + dy) * bw + a.bx + dx];
+    DcToLlf(params.dc_basis, a.cw, a.ch, dcv, coef + k * N);
+  }
+  __syncthreads();
+  InverseDctLds(bases, a.W, a.H, 3, coef, tmp, pix);
+  const size_t plane = static_cast<size_t>(width) * height;
+  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
+    const int k = e / N, rr = e % N, y = rr / a.W, x = rr % a.W;
+    out[k * plane + static_cast<size_t>(a.by * 8 + y) * width + a.bx * 8 + x] = pix[e];
+  }
+}
+
+DcBasisTable HostDcBasis() {
+  DcBasisTable t{};
+  const double sqrt2 = 1.4142135623730951, pi = 3.141592653589793;
+  const int lengths[3] = {1, 2, 4};
+  for (int fwd = 0; fwd < 2; ++fwd) {
+    for (int li = 0; li < 3; ++li) {
+      const int n = lengths[li];
+      for (int f = 0; f < n; ++f) {
+        for (int s = 0; s < n; ++s) {
+          const double alpha = f == 0 ? sqrt2 / 2.0 : 1.0;
+          volatile double angle = (static_cast<double>(s) + 0.5) * static_cast<double>(f) *
+                               pi / static_cast<double>(n);
+          // index 0: InverseBasis (used by LLF -> DC), 1: ForwardBasis
+          t.v[fwd][li][f][s] = fwd == 0 ? sqrt2 * alpha * std::cos(angle)
+                                        : sqrt2 * alpha * std::cos(angle) /
+                                            static_cast<double>(n);
+        }
+      }
+    }
+  }
+  return t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host entry points (used by the C ABI in aq_api.hip).
+
+AqLayout MakeAqLayout(int width, int height) {
+  AqLayout l;
+  l.bw = width / 8;
+  l.bh = height / 8;
+  l.groups_w = (l.bw + kGroupBlocks - 1) / kGroupBlocks;
+  l.groups_h = (l.bh + kGroupBlocks - 1) / kGroupBlocks;
+  l.groups = l.groups_w * l.groups_h;
+  l.store = static_cast<size_t>(l.groups) * kGroupCoefficients;
+  return l;
+}
+
+GJXLResult LaunchAnchorLayout(hipStream_t s, const uint8_t* cells, int width,
+                              int height, uint32_t* offsets, uint32_t* list,
+                              uint32_t* count, uint32_t* error) {
+  const AqLayout l = MakeAqLayout(width, height);
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, error, 1, 0u);
+  hipLaunchKernelGGL(AnchorLayoutKernel, dim3(l.groups), dim3(64), 0, s, cells,
+                     l.bw, l.bh, offsets, list, count, error);
+  return CheckLaunch("AnchorLayoutKernel");
+}
+
+GJXLResult LaunchAnchorDct(GJXLAmdDevice* d, const float* opsin, int width,
+                           int height, const uint8_t* cells,
+                           const uint32_t* offsets, const uint32_t* list,
+                           const uint32_t* count, float* coefs) {
+  const AqLayout l = MakeAqLayout(width, height);
+  hipLaunchKernelGGL(AnchorDctKernel, dim3(kGroupAnchors, l.groups),
+                     dim3(kAnchorThreads), 0, d->stream, opsin, width, height,
+                     cells, offsets, list, count, d->dct_bases, l.store, coefs);
+  return CheckLaunch("AnchorDctKernel");
+}
+
+GJXLResult LaunchAdjustField(hipStream_t s, const uint8_t* cells, int width,
+                             int height, float target, const float* in,
+                             float* out) {
+  const AqLayout l = MakeAqLayout(width, height);
+  float mixer = 1.0f;
+  if (target > 1.54138f) {
+    mixer = mixer - (target - 1.54138f) * 0.56391f;
+    mixer = mixer < 0.0f ? 0.0f : mixer;
+  }
+  hipLaunchKernelGGL(AdjustFieldKernel, dim3(CeilDiv(l.bw, 256), l.bh), dim3(256),
+                     0, s, cells, l.bw, l.bh, mixer, in, out);
+  return CheckLaunch("AdjustFieldKernel");
+}
+
+GJXLResult LaunchMatrixStats(hipStream_t s, const float* opsin, int width,
+                             int height, int padded_w, int padded_h,
+                             uint32_t* stats) {
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, stats, 3, 0u);
+  hipLaunchKernelGGL(MatrixStatsKernel, dim3(CeilDiv(width, 256), height), dim3(256), 0,
+                     s, opsin, width, height, padded_w,
+                     static_cast<size_t>(padded_w) * padded_h, stats);
+  return CheckLaunch("MatrixStatsKernel");
+}
+
+GJXLResult LaunchRange(hipStream_t s, const float* v, size_t n, uint32_t* range) {
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, range, 1, 0xffffffffu);
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, range + 1, 1, 0u);
+  hipLaunchKernelGGL(RangeKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)), dim3(256),
+                     0, s, v, n, range);
+  return CheckLaunch("RangeKernel");
+}
+
+// select: state (4 words), hist (256 words) of device scratch.
+GJXLResult LaunchQuantizer(hipStream_t s, const float* field, size_t n,
+                           float quant_dc, uint32_t* select_state,
+                           uint32_t* hist, QuantizerState* q, int32_t* raw) {
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(256), 0, s, hist, 256, 0u);
+  const uint32_t k = static_cast<uint32_t>(n / 2);
+  const int blocks = static_cast<int>(std::min<size_t>(CeilDiv(static_cast<uint32_t>(n), 256), 1024));
+  for (int kind = 0; kind < 2; ++kind) {
+    hipLaunchKernelGGL(RadixResetKernel, dim3(1), dim3(1), 0, s, select_state, k, kind);
+    for (int shift = 24; shift >= 0; shift -= 8) {
+      hipLaunchKernelGGL(RadixHistogramKernel, dim3(blocks), dim3(256), 0, s, field, n,
+                         shift, kind, select_state, hist);
+      hipLaunchKernelGGL(RadixPickKernel, dim3(1), dim3(64), 0, s, hist, shift, kind,
+                         select_state);
+    }
+  }
+  hipLaunchKernelGGL(QuantizerParamsKernel, dim3(1), dim3(64), 0, s, select_state,
+                     quant_dc, q);
+  hipLaunchKernelGGL(RawQuantKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)), dim3(256),
+                     0, s, field, n, q, raw);
+  return CheckLaunch("RawQuantKernel");
+}
+
+GJXLResult LaunchFinalCfl(GJXLAmdDevice* d, const float* coefs, int width,
+                          int height, const uint8_t* cells,
+                          const uint32_t* offsets, const int32_t* raw,
+                          const QuantizerState* q, int8_t* y_to_x,
+                          int8_t* y_to_b) {
+  const AqLayout l = MakeAqLayout(width, height);
+  hipLaunchKernelGGL(FinalCflKernel, dim3(CeilDiv(l.bw, 8), CeilDiv(l.bh, 8)), dim3(64),
+                     0, d->stream, coefs, l.store, cells, offsets, l.bw, l.bh, raw, q,
+                     d->quant_tables, y_to_x, y_to_b);
+  return CheckLaunch("FinalCflKernel");
+}
+
+GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
+                                  int width, int height, const uint8_t* cells,
+                                  const uint32_t* offsets, const uint32_t* list,
+                                  const uint32_t* count, int32_t* raw,
+                                  const int8_t* y_to_x, const int8_t* y_to_b,
+                                  const QuantizerState* q, float x_mult,
+                                  float b_mult, int32_t* quantized, float* dc,
+                                  int32_t* dc_quantized, float* dc_rec,
+                                  uint32_t* dc_scratch) {
+  const AqLayout l = MakeAqLayout(width, height);
+  CodingParams p;
+  p.x_mult = x_mult;
+  p.b_mult = b_mult;
+  static const DcBasisTable table = HostDcBasis();
+  p.dc_basis = table;
+  const size_t n = static_cast<size_t>(l.bw) * l.bh;
+  hipLaunchKernelGGL(CodeTransformKernel, dim3(kGroupAnchors, l.groups),
+                     dim3(kAnchorThreads), 0, d->stream, coefs, l.store, cells,
+                     offsets, list, count, l.bw, raw, y_to_x, y_to_b, q,
+                     d->quant_tables, p, quantized, dc, n);
+  GJXL_AMD_TRY(CheckLaunch("CodeTransformKernel"));
+  const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup));
+  for (int channel : {1, 0, 2}) {
+    hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw,
+                       l.bh, q, channel, 1, dc_quantized, dc_scratch);
+    GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
+  }
+  hipLaunchKernelGGL(DcReconstructKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)),
+                     dim3(256), 0, d->stream, dc_quantized, n, q, 1, dc_rec);
+  return CheckLaunch("DcReconstructKernel");
+}
+
+size_t DcScratchWords(int width, int height) {
+  const int bw = width / 8, bh = height / 8;
+  return static_cast<size_t>(CeilDiv(bw, kDcGroup)) * CeilDiv(bh, kDcGroup) * 5 *
+         kDcGroup * kDcGroup;
+}
+
+GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
+                             int width, int height, const uint8_t* cells,
+                             const uint32_t* offsets, const uint32_t* list,
+                             const uint32_t* count, const int32_t* raw,
+                             const int8_t* y_to_x, const int8_t* y_to_b,
+                             const QuantizerState* q, float x_mult, float b_mult,
+                             const float* dc_rec, float* smoothed,
+                             float* inv_sigma, float* out) {
+  const AqLayout l = MakeAqLayout(width, height);
+  hipLaunchKernelGGL(EpfSigmaKernel, dim3(CeilDiv(l.bw, 256), l.bh), dim3(256), 0,
+                     d->stream, cells, l.bw, l.bh, raw, q, inv_sigma);
+  GJXL_AMD_TRY(CheckLaunch("EpfSigmaKernel"));
+  hipLaunchKernelGGL(SmoothDcKernel, dim3(CeilDiv(l.bw, 256), l.bh), dim3(256), 0,
+                     d->stream, dc_rec, l.bw, l.bh, q, smoothed);
+  GJXL_AMD_TRY(CheckLaunch("SmoothDcKernel"));
+  CodingParams p;
+  p.x_mult = x_mult;
+  p.b_mult = b_mult;
+  static const DcBasisTable table = HostDcBasis();
+  p.dc_basis = table;
+  hipLaunchKernelGGL(ReconstructKernel, dim3(kGroupAnchors, l.groups),
+                     dim3(kAnchorThreads), 0, d->stream, quantized, l.store, cells,
+                     offsets, list, count, width, height, raw, y_to_x, y_to_b, q,
+                     d->quant_tables, d->dct_bases, p, smoothed, out);
+  return CheckLaunch("ReconstructKernel");
+}
+
+}  // namespace gjxl_amd

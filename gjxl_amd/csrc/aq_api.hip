@@ -1,0 +1,285 @@
+// C ABI of the adaptive-quantization evaluation: a session owns the device
+// storage of one image's evaluation (the counterpart of the reference's
+// PreparedAqEvaluation, src/gpu/ops/aq_evaluation.h:219-420).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "aq_common.h"
+#include "common.h"
+
+
+struct GJXLAmdAqSession {
+  GJXLAmdDevice* device = nullptr;
+  uint32_t width = 0, height = 0, pw = 0, ph = 0;
+  gjxl_amd::AqLayout layout{};
+  // borrowed inputs
+  const float* opsin = nullptr;
+  const float* pre = nullptr;
+  const uint8_t* cells = nullptr;
+  float target = 1.0f;
+  // policy (host)
+  float lower = 0.0f, upper = 0.0f, quant_dc = 0.0f;
+  int x_scale = 2, b_scale = 2;
+  float x_mult = 1.0f, b_mult = 1.0f;
+  bool prepared = false;
+  // owned device storage, indexed by GJXL_AMD_AQ_* ids
+  void* buffers[GJXL_AMD_AQ_BUFFER_COUNT] = {};
+  size_t sizes[GJXL_AMD_AQ_BUFFER_COUNT] = {};
+};
+
+using gjxl_amd::Check;
+using gjxl_amd::Fail;
+
+namespace {
+
+template <typename T>
+T* Buf(GJXLAmdAqSession* s, int id) {
+  return static_cast<T*>(s->buffers[id]);
+}
+
+GJXLResult ValidateSession(GJXLAmdAqSession* s) {
+  if (s == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session is null");
+  return gjxl_amd::ValidateDevice(s->device);
+}
+
+// src/codestream/workflow.cpp:1435-1484 SelectQuantizationMatrixScales
+void SelectMatrixScales(const float stats[3], float target, int* xs, int* bs) {
+  int x = 3, b = 2;
+  for (float threshold : {2.5f, 5.5f, 9.5f}) {
+    if (target > threshold) ++x;
+  }
+  int xa = 0;
+  if (stats[0] >= 0.026f) xa = 3;
+  else if (stats[0] >= 0.022f) xa = 2;
+  else if (stats[0] >= 0.015f) xa = 1;
+  if (x < 2 + xa) x = 2 + xa;
+  const int eb = stats[2] >= 0.13f ? 1 : 0;
+  int ba = 0;
+  if (stats[1] > 0.38f) ba = 2 + eb;
+  else if (stats[1] > 0.33f) ba = 1 + eb;
+  else if (stats[1] > 0.28f) ba = eb;
+  *xs = x;
+  *bs = b + ba;
+}
+
+}  // namespace
+
+extern "C" {
+
+GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
+                                      uint32_t height,
+                                      GJXLAmdAqSession** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || *out != nullptr || width == 0 || height == 0 ||
+      width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session geometry is invalid");
+  }
+  auto* s = new (std::nothrow) GJXLAmdAqSession;
+  if (s == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "AQ session allocation failed");
+  s->device = device;
+  s->width = width;
+  s->height = height;
+  s->pw = gjxl_amd::PadTo8(width);
+  s->ph = gjxl_amd::PadTo8(height);
+  s->layout = gjxl_amd::MakeAqLayout(s->pw, s->ph);
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  const size_t tiles = static_cast<size_t>(gjxl_amd::CeilDiv(s->layout.bw, 8)) *
+                       gjxl_amd::CeilDiv(s->layout.bh, 8);
+  const size_t store = s->layout.store;
+  const size_t groups = s->layout.groups;
+  const size_t pixels = static_cast<size_t>(width) * height;
+  const size_t padded = static_cast<size_t>(s->pw) * s->ph;
+  size_t* z = s->sizes;
+  z[GJXL_AMD_AQ_OFFSETS] = 4 * n;
+  z[GJXL_AMD_AQ_ANCHOR_LIST] = 4 * groups * gjxl_amd::kGroupAnchors;
+  z[GJXL_AMD_AQ_ANCHOR_COUNT] = 4 * groups;
+  z[GJXL_AMD_AQ_COEFFICIENTS] = 4 * 3 * store;
+  z[GJXL_AMD_AQ_ADJUSTED_FIELD] = 4 * n;
+  z[GJXL_AMD_AQ_STATS] = 4 * 8;
+  z[GJXL_AMD_AQ_SELECT] = 4 * (4 + 256);
+  z[GJXL_AMD_AQ_QUANTIZER] = sizeof(gjxl_amd::QuantizerState);
+  z[GJXL_AMD_AQ_RAW_QUANT] = 4 * n;
+  z[GJXL_AMD_AQ_Y_TO_X] = tiles;
+  z[GJXL_AMD_AQ_Y_TO_B] = tiles;
+  z[GJXL_AMD_AQ_QUANTIZED] = 4 * 3 * store;
+  z[GJXL_AMD_AQ_DC] = 4 * 3 * n;
+  z[GJXL_AMD_AQ_DC_QUANTIZED] = 4 * 3 * n;
+  z[GJXL_AMD_AQ_DC_RECONSTRUCTED] = 4 * 3 * n;
+  z[GJXL_AMD_AQ_DC_SCRATCH] = 4 * gjxl_amd::DcScratchWords(s->pw, s->ph);
+  z[GJXL_AMD_AQ_DC_SMOOTHED] = 4 * 3 * n;
+  z[GJXL_AMD_AQ_INVERSE_SIGMA] = 4 * n;
+  z[GJXL_AMD_AQ_RECONSTRUCTED] = 4 * 3 * padded;
+  z[GJXL_AMD_AQ_SCRATCH_A] = 4 * 3 * pixels;
+  z[GJXL_AMD_AQ_SCRATCH_B] = 4 * 3 * pixels;
+  z[GJXL_AMD_AQ_FILTERED] = 4 * 3 * pixels;
+  z[GJXL_AMD_AQ_LINEAR] = 4 * 3 * pixels;
+  for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
+    const GJXLResult r = Check(hipMalloc(&s->buffers[i], z[i]), "hipMalloc(AQ session)");
+    if (r != GJXL_OK) {
+      gjxl_amd_aq_session_destroy(s);
+      return r;
+    }
+  }
+  // the coefficient stores hold zeros in unused group tails
+  for (int id : {GJXL_AMD_AQ_COEFFICIENTS, GJXL_AMD_AQ_QUANTIZED}) {
+    const GJXLResult r = Check(hipMemset(s->buffers[id], 0, z[id]), "hipMemset");
+    if (r != GJXL_OK) {
+      gjxl_amd_aq_session_destroy(s);
+      return r;
+    }
+  }
+  *out = s;
+  return GJXL_OK;
+}
+
+void gjxl_amd_aq_session_destroy(GJXLAmdAqSession* s) {
+  if (s == nullptr) return;
+  if (s->device != nullptr) {
+    (void)hipSetDevice(s->device->ordinal);
+    (void)hipStreamSynchronize(s->device->stream);
+  }
+  for (void*& b : s->buffers) {
+    if (b != nullptr) (void)hipFree(b);
+  }
+  delete s;
+}
+
+GJXLResult gjxl_amd_aq_prepare(GJXLAmdAqSession* s, const float* opsin,
+                               const float* preprocessed, const uint8_t* cells,
+                               const float* initial_quant_field,
+                               float butteraugli_target) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (opsin == nullptr || preprocessed == nullptr || cells == nullptr ||
+      initial_quant_field == nullptr || !std::isfinite(butteraugli_target) ||
+      butteraugli_target <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ preparation inputs are invalid");
+  }
+  GJXLAmdDevice* d = s->device;
+  hipStream_t st = d->stream;
+  s->prepared = false;
+  s->opsin = opsin;
+  s->pre = preprocessed;
+  s->cells = cells;
+  s->target = butteraugli_target;
+  using namespace gjxl_amd;
+  GJXL_AMD_TRY(LaunchMatrixStats(st, opsin, s->width, s->height, s->pw, s->ph,
+                                 Buf<uint32_t>(s, GJXL_AMD_AQ_STATS)));
+  GJXL_AMD_TRY(LaunchAnchorLayout(st, cells, s->pw, s->ph,
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT),
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + 6));
+  GJXL_AMD_TRY(LaunchAnchorDct(d, preprocessed, s->pw, s->ph, cells,
+                               Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
+                               Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
+                               Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT),
+                               Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS)));
+  GJXL_AMD_TRY(LaunchAdjustField(st, cells, s->pw, s->ph, butteraugli_target,
+                                 initial_quant_field,
+                                 Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD)));
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  uint32_t* stats = Buf<uint32_t>(s, GJXL_AMD_AQ_STATS);
+  GJXL_AMD_TRY(LaunchRange(st, Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD), n, stats + 4));
+  uint32_t host[8];
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, st),
+                     "hipMemcpyAsync(stats)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
+  if (host[6] != 0) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Strategy cells do not form a valid grid of searched transforms");
+  }
+  float fstats[3];
+  for (int i = 0; i < 3; ++i) std::memcpy(&fstats[i], &host[i], 4);
+  SelectMatrixScales(fstats, butteraugli_target, &s->x_scale, &s->b_scale);
+  // src/codec/codestream.cpp:92-96
+  s->x_mult = static_cast<float>(std::pow(1.25, static_cast<double>(s->x_scale - 2)));
+  s->b_mult = static_cast<float>(std::pow(1.25, static_cast<double>(s->b_scale - 2)));
+  // PrepareButteraugliPolicyFromRange (adaptive_quantization.cpp:1197-1242)
+  float mn, mx;
+  std::memcpy(&mn, &host[4], 4);
+  std::memcpy(&mx, &host[5], 4);
+  const float ratio = mx / mn;
+  const float max_dev = std::sqrt(250.0f / ratio);
+  const float asym = max_dev < 2.0f ? max_dev : 2.0f;
+  s->lower = mn / (asym * max_dev);
+  s->upper = mx * (max_dev / asym);
+  // ComputeInitialQuantDc (adaptive_quantization.cpp:556-580)
+  const float nonlinear = 0.3f * std::pow(butteraugli_target / 0.3f, 0.83f);
+  const float tdc = std::max(0.5f * butteraugli_target,
+                             std::min(butteraugli_target, nonlinear));
+  s->quant_dc = std::min(1.095924047623553f / tdc, 50.0f);
+  if (!(s->upper / s->lower < 253.0f)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Initial quant field cannot form libjxl AQ bounds");
+  }
+  s->prepared = true;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_aq_policy(GJXLAmdAqSession* s, float* lower, float* upper,
+                              float* quant_dc, int32_t* x_scale,
+                              int32_t* b_scale) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (!s->prepared) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session is not prepared");
+  if (lower) *lower = s->lower;
+  if (upper) *upper = s->upper;
+  if (quant_dc) *quant_dc = s->quant_dc;
+  if (x_scale) *x_scale = s->x_scale;
+  if (b_scale) *b_scale = s->b_scale;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (!s->prepared || quant_field == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ evaluation needs a prepared session and a field");
+  }
+  using namespace gjxl_amd;
+  GJXLAmdDevice* d = s->device;
+  hipStream_t st = d->stream;
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  uint32_t* select = Buf<uint32_t>(s, GJXL_AMD_AQ_SELECT);
+  auto* q = Buf<QuantizerState>(s, GJXL_AMD_AQ_QUANTIZER);
+  GJXL_AMD_TRY(LaunchQuantizer(st, quant_field, n, s->quant_dc, select, select + 4, q,
+                               Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT)));
+  GJXL_AMD_TRY(LaunchFinalCfl(d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph,
+                              s->cells, Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
+                              Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT), q,
+                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X),
+                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B)));
+  GJXL_AMD_TRY(LaunchCodeCoefficients(
+    d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph, s->cells,
+    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
+    Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
+    Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q,
+    s->x_mult, s->b_mult, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED),
+    Buf<float>(s, GJXL_AMD_AQ_DC), Buf<int32_t>(s, GJXL_AMD_AQ_DC_QUANTIZED),
+    Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED), Buf<uint32_t>(s, GJXL_AMD_AQ_DC_SCRATCH)));
+  GJXL_AMD_TRY(LaunchReconstruct(
+    d, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED), s->pw, s->ph, s->cells,
+    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
+    Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
+    Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q, s->x_mult,
+    s->b_mult, Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED),
+    Buf<float>(s, GJXL_AMD_AQ_DC_SMOOTHED), Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA),
+    Buf<float>(s, GJXL_AMD_AQ_RECONSTRUCTED)));
+  return LaunchLoopFiltersAndColor(
+    st, Buf<float>(s, GJXL_AMD_AQ_RECONSTRUCTED), s->pw, s->ph, s->width, s->height,
+    Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA), Buf<float>(s, GJXL_AMD_AQ_SCRATCH_A),
+    Buf<float>(s, GJXL_AMD_AQ_SCRATCH_B), Buf<float>(s, GJXL_AMD_AQ_FILTERED),
+    Buf<float>(s, GJXL_AMD_AQ_LINEAR), 255.0f);
+}
+
+GJXLResult gjxl_amd_aq_buffer(GJXLAmdAqSession* s, int32_t id, void** pointer,
+                              size_t* bytes) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (id < 0 || id >= GJXL_AMD_AQ_BUFFER_COUNT || pointer == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ buffer id is invalid");
+  }
+  *pointer = s->buffers[id];
+  if (bytes) *bytes = s->sizes[id];
+  return GJXL_OK;
+}
+
+}  // extern "C"

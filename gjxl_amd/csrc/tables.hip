@@ -133,7 +133,7 @@ void BuildDctBases(double* out) {
       for (int f = 0; f < n; ++f) {
         const double alpha = f == 0 ? 1.0 / sqrt_two : 1.0;
         for (int s = 0; s < n; ++s) {
-          const double angle = (static_cast<double>(s) + 0.5) *
+          volatile double angle = (static_cast<double>(s) + 0.5) *
                                static_cast<double>(f) * pi / static_cast<double>(n);
           b[f * n + s] = scale * alpha * std::cos(angle);
         }

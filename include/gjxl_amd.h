@@ -159,6 +159,93 @@ GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_search(
   GJXLAmdDevice* device, const float* costs, uint32_t width, uint32_t height,
   float butteraugli_target, uint8_t* cells);
 
+/* ---------------------------------------------------------------------- */
+/* a3, a11-a15: adaptive-quantization evaluation session.  Replaces         */
+/* PreparedAqEvaluation (src/gpu/ops/aq_evaluation.h:219-420) and the        */
+/* GpuQuantizationEvaluationSession behind it (src/gpu/ops/aq_evaluation.h:  */
+/* 112-160): the session owns the device storage of one image's evaluation.  */
+/* width, height: the source extent; the coding extent is ceil8 of each.     */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdAqSession GJXLAmdAqSession;
+
+/* Buffer ids for gjxl_amd_aq_buffer.  Block fields are (pw/8)*(ph/8);
+ * coefficient stores hold 3 planes of groups*65536 values, one 65536-slot
+ * region per 256x256 AC group with its transforms in raster order of their
+ * anchors (the reference's per-group coefficient layout, src/codec/
+ * reconstruction.cpp:386-760). */
+enum {
+  GJXL_AMD_AQ_OFFSETS = 0,          /* u32 per block: store offset or ~0 */
+  GJXL_AMD_AQ_ANCHOR_LIST,          /* u32 groups*1024: by << 16 | bx */
+  GJXL_AMD_AQ_ANCHOR_COUNT,         /* u32 per group */
+  GJXL_AMD_AQ_COEFFICIENTS,         /* f32 forward DCT of every transform */
+  GJXL_AMD_AQ_ADJUSTED_FIELD,       /* f32 AdjustQuantField output */
+  GJXL_AMD_AQ_STATS,                /* u32[8]: matrix stats 0-2, range 4-5 */
+  GJXL_AMD_AQ_SELECT,               /* u32[4 + 256]: median / MAD selection */
+  GJXL_AMD_AQ_QUANTIZER,            /* quantizer state (10 words) */
+  GJXL_AMD_AQ_RAW_QUANT,            /* i32 per block: adjusted raw quant */
+  GJXL_AMD_AQ_Y_TO_X,               /* i8 per 64x64 tile: final CfL */
+  GJXL_AMD_AQ_Y_TO_B,
+  GJXL_AMD_AQ_QUANTIZED,            /* i32 quantized AC, store layout */
+  GJXL_AMD_AQ_DC,                   /* f32 3 block planes: DC before coding */
+  GJXL_AMD_AQ_DC_QUANTIZED,         /* i32 3 block planes */
+  GJXL_AMD_AQ_DC_RECONSTRUCTED,     /* f32 3 block planes */
+  GJXL_AMD_AQ_DC_SCRATCH,           /* weighted-predictor state */
+  GJXL_AMD_AQ_DC_SMOOTHED,          /* f32 3 block planes */
+  GJXL_AMD_AQ_INVERSE_SIGMA,        /* f32 per block: EPF inverse sigma */
+  GJXL_AMD_AQ_RECONSTRUCTED,        /* f32 3 planes pw*ph: XYB */
+  GJXL_AMD_AQ_SCRATCH_A,            /* f32 3 planes w*h: after Gaborish */
+  GJXL_AMD_AQ_SCRATCH_B,            /* f32 3 planes w*h: after EPF pass 1 */
+  GJXL_AMD_AQ_FILTERED,             /* f32 3 planes w*h: after EPF pass 2 */
+  GJXL_AMD_AQ_LINEAR,               /* f32 3 planes w*h: linear RGB */
+  GJXL_AMD_AQ_BUFFER_COUNT
+};
+
+GJXL_AMD_API GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device,
+                                                   uint32_t width,
+                                                   uint32_t height,
+                                                   GJXLAmdAqSession** out);
+GJXL_AMD_API void gjxl_amd_aq_session_destroy(GJXLAmdAqSession* session);
+
+/* Per-encode preparation (the inputs stay borrowed until the session is
+ * destroyed or prepared again): quantization-matrix scale statistics and
+ * selection (src/codestream/workflow.cpp:1318-1484), AdjustQuantField
+ * (src/codec/adaptive_quantization.cpp:740-835), the Butteraugli policy
+ * bounds (:1178-1242) and initial DC quant (:556-580), the AC-group layout
+ * of the strategy grid and the forward DCT of every transform.
+ * opsin: padded pre-Gaborish XYB; preprocessed: padded Gaborish-inverted
+ * XYB; cells: strategy cells as gjxl_amd_ac_strategy_search writes them;
+ * initial_quant_field: block field before adjustment.  Waits for the stream
+ * (the matrix scales are chosen on the host). */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_prepare(
+  GJXLAmdAqSession* session, const float* opsin, const float* preprocessed,
+  const uint8_t* cells, const float* initial_quant_field,
+  float butteraugli_target);
+
+/* The host-side policy of the last preparation. */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_policy(GJXLAmdAqSession* session,
+                                           float* lower_bound,
+                                           float* upper_bound,
+                                           float* quant_dc, int32_t* x_scale,
+                                           int32_t* b_scale);
+
+/* One evaluation of a quant field (device, (pw/8)*(ph/8) floats), up to the
+ * linear-RGB decode: CreateQuantizerFromField (src/codec/quantization.cpp:
+ * 536-639), ComputeFinalColorCorrelationMap (src/codec/chroma_from_luma.cpp:
+ * 543-670), ComputeQuantizedCoefficients with prediction-aware DC
+ * (src/codec/reconstruction.cpp:386-760, src/codec/vardct/dc_quantization.
+ * cpp:121-240), ReconstructQuantizedCoefficients with DC smoothing
+ * (src/codec/reconstruction.cpp:790-936), ApplyLoopFilters (src/codec/
+ * loop_filter.cpp:14-65) and OpsinToLinearRgb (src/codec/color_transform.
+ * cpp:433-471): EvaluateQuantization (src/codec/adaptive_quantization.cpp:
+ * 940-1010) without the distance.  Enqueued on the device stream. */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* session,
+                                             const float* quant_field);
+
+/* Device pointer and size in bytes of one session buffer. */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_buffer(GJXLAmdAqSession* session,
+                                           int32_t id, void** pointer,
+                                           size_t* bytes);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif
