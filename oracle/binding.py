@@ -384,3 +384,156 @@ def aq_evaluate(fe: dict, setup: dict, field: np.ndarray, width: int,
             "y_to_b": fb, "quantized": quantized, "dc_q": dc_q,
             "dc_rec": dc_rec, "inv_sigma": inv_sigma, "reconstructed": rec,
             "filtered": filtered, "linear": linear}
+
+
+# ---- a16-a18: Butteraugli, 16-norm reduction, AQ update -------------------
+
+_B_DECLARED = False
+
+
+def _declare_butteraugli():
+    global _B_DECLARED
+    if _B_DECLARED:
+        return
+    lib = oracle()
+    P, F_, SZ, I = _P, _F, _SZ, _I
+    sig = {
+        "oracle_butteraugli_kernel": ([F_, P], I),
+        "oracle_butteraugli_blur": ([P, SZ, SZ, F_, P], I),
+        "oracle_butteraugli_opsin": ([P, SZ, SZ, F_, P], I),
+        "oracle_butteraugli_frequencies": ([P, SZ, SZ, P], I),
+        "oracle_butteraugli_erosion": ([P, SZ, SZ, P], None),
+        "oracle_butteraugli_differences": ([P, P, SZ, SZ, F_, F_, P], I),
+        "oracle_butteraugli_subsample": ([P, SZ, SZ, P], None),
+        "oracle_butteraugli_distance": ([P, P, SZ, SZ, F_, F_, F_, P,
+                                         ctypes.POINTER(ctypes.c_double)], I),
+        "oracle_reduce_distance_map": ([P, SZ, SZ, P, SZ, SZ, P], None),
+        "oracle_aq_update": ([P, P, P, SZ, F_, I, F_, F_, F_, F_], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    ref = reference()
+    if ref is not None:
+        D = ctypes.POINTER(ctypes.c_double)
+        rsig = {
+            "ref_butteraugli_distance": [P, P, SZ, SZ, F_, F_, F_, P, D],
+            "ref_butteraugli_prepared": [P, P, SZ, SZ, SZ, F_, F_, F_, P, P],
+            "ref_butteraugli_blur": [P, SZ, SZ, F_, P],
+            "ref_butteraugli_opsin": [P, SZ, SZ, F_, P],
+            "ref_butteraugli_frequencies": [P, SZ, SZ, P],
+            "ref_butteraugli_stages": [P, P, SZ, SZ, F_, F_, F_, P],
+            "ref_reduce_distance_map": [P, SZ, SZ, P, SZ, SZ, P],
+        }
+        for name, args in rsig.items():
+            fn = getattr(ref, name)
+            fn.argtypes = args
+            fn.restype = I
+    _B_DECLARED = True
+
+
+def _f32(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32)
+
+
+def butteraugli_blur(plane: np.ndarray, sigma: float) -> np.ndarray:
+    _declare_butteraugli()
+    plane = _f32(plane)
+    h, w = plane.shape
+    out = np.empty_like(plane)
+    assert oracle().oracle_butteraugli_blur(ptr(plane), w, h, sigma, ptr(out)) == 0
+    return out
+
+
+def butteraugli_opsin(rgb: np.ndarray, intensity: float = 80.0) -> np.ndarray:
+    _declare_butteraugli()
+    rgb = _f32(rgb)
+    _, h, w = rgb.shape
+    out = np.empty_like(rgb)
+    assert oracle().oracle_butteraugli_opsin(ptr(rgb), w, h, intensity, ptr(out)) == 0
+    return out
+
+
+def butteraugli_frequencies(xyb: np.ndarray) -> np.ndarray:
+    _declare_butteraugli()
+    xyb = _f32(xyb)
+    _, h, w = xyb.shape
+    out = np.empty((10, h, w), np.float32)
+    assert oracle().oracle_butteraugli_frequencies(ptr(xyb), w, h, ptr(out)) == 0
+    return out
+
+
+def butteraugli_stages(reference_rgb: np.ndarray, distorted_rgb: np.ndarray,
+                       hf_asymmetry: float = 1.0, x_multiplier: float = 1.0,
+                       intensity: float = 80.0) -> np.ndarray:
+    """The 9 difference-stage planes of one scale (no subscale)."""
+    x0 = butteraugli_opsin(reference_rgb, intensity)
+    x1 = butteraugli_opsin(distorted_rgb, intensity)
+    p0, p1 = butteraugli_frequencies(x0), butteraugli_frequencies(x1)
+    _, h, w = x0.shape
+    out = np.empty((9, h, w), np.float32)
+    assert oracle().oracle_butteraugli_differences(
+        ptr(p0), ptr(p1), w, h, hf_asymmetry, x_multiplier, ptr(out)) == 0
+    return out
+
+
+def butteraugli_distance(reference_rgb: np.ndarray, distorted_rgb: np.ndarray,
+                         hf_asymmetry: float = 1.0, x_multiplier: float = 1.0,
+                         intensity: float = 80.0):
+    _declare_butteraugli()
+    r, d = _f32(reference_rgb), _f32(distorted_rgb)
+    _, h, w = r.shape
+    out = np.empty((h, w), np.float32)
+    score = ctypes.c_double()
+    assert oracle().oracle_butteraugli_distance(
+        ptr(r), ptr(d), w, h, hf_asymmetry, x_multiplier, intensity, ptr(out),
+        ctypes.byref(score)) == 0
+    return out, score.value
+
+
+def reduce_distance_map(distance_map: np.ndarray, cells: np.ndarray) -> np.ndarray:
+    _declare_butteraugli()
+    m = _f32(distance_map)
+    cells = np.ascontiguousarray(cells, np.uint8)
+    h, w = m.shape
+    bh, bw = cells.shape
+    out = np.empty((bh, bw), np.float32)
+    oracle().oracle_reduce_distance_map(ptr(m), w, h, ptr(cells), bw, bh, ptr(out))
+    return out
+
+
+def aq_update(field: np.ndarray, adjusted_initial: np.ndarray,
+              block_distance: np.ndarray, iteration: int, lower: float,
+              upper: float, q: np.ndarray, target: float = 1.0) -> np.ndarray:
+    """One field update after evaluation `iteration`; q is the evaluation's
+    quantizer as oracle_quantizer_from_field fills it."""
+    _declare_butteraugli()
+    out = _f32(field).copy()
+    a0, bd = _f32(adjusted_initial), _f32(block_distance)
+    oracle().oracle_aq_update(ptr(out), ptr(a0), ptr(bd), out.size, target,
+                              iteration, lower, upper, float(q[0]), float(q[1]))
+    return out
+
+
+def aq_loop(fe: dict, setup: dict, linear: np.ndarray, width: int, height: int,
+            iterations: int = 2, target: float = 1.0) -> dict:
+    """RunAdaptiveQuantizationPolicyImpl (src/codec/adaptive_quantization.cpp:
+    1600-1738): iterations + 1 evaluations with Butteraugli against the source
+    linear RGB, field updates between them."""
+    field = setup["adjusted"].copy()
+    scores, fields, distances = [], [], []
+    for it in range(iterations + 1):
+        ev = aq_evaluate(fe, setup, field, width, height)
+        dmap, score = butteraugli_distance(linear, ev["linear"])
+        bd = reduce_distance_map(dmap, fe["cells"])
+        scores.append(score)
+        fields.append(field.copy())
+        distances.append(bd)
+        if it == iterations:
+            break
+        field = aq_update(field, setup["adjusted"], bd, it, setup["lower"],
+                          setup["upper"], ev["q"], target)
+    return {"field": field, "evaluation": ev, "distance_map": dmap,
+            "block_distance": bd, "scores": scores, "fields": fields,
+            "block_distances": distances}

@@ -180,6 +180,46 @@ void oracle_gaborish(const float* in, size_t w, size_t h, float* out);
 void oracle_loop_filters(const float* in, size_t w, size_t h,
                          const float* inverse_sigma, float* out);
 
+/* ---- a16-a18: Butteraugli, 16-norm reduction, AQ update (butteraugli.c) */
+/* src/codec/butteraugli_internal.cpp:565-620: kernel taps (returns size). */
+int oracle_butteraugli_kernel(float sigma, float* kernel /* <= 33 */);
+/* GaussianBlur (:576-620).  Pinned: bit-exact against the compiled
+ * reference; libjxl blur goldens (tests/butteraugli_goldens_generated.h). */
+int oracle_butteraugli_blur(const float* in, size_t w, size_t h, float sigma,
+                            float* out);
+/* OpsinDynamicsImage (:622-700).  Pinned as above. */
+int oracle_butteraugli_opsin(const float* rgb, size_t w, size_t h,
+                             float intensity, float* xyb);
+/* SeparateFrequencies (:702-846): 10 planes LF(3) MF(3) HF(2) UHF(2). */
+int oracle_butteraugli_frequencies(const float* xyb, size_t w, size_t h,
+                                   float* psycho);
+/* FuzzyErosion (src/codec/butteraugli_distance_internal.cpp:579-630). */
+void oracle_butteraugli_erosion(const float* in, size_t w, size_t h, float* out);
+/* ComputeDifferenceStages (:657-870): 9 stage planes. */
+int oracle_butteraugli_differences(const float* ps0, const float* ps1, size_t w,
+                                   size_t h, float hf_asymmetry,
+                                   float x_multiplier, float* stages);
+/* Subsample2x (:380-407): 3 planes ((w+1)/2) x ((h+1)/2). */
+void oracle_butteraugli_subsample(const float* in, size_t w, size_t h,
+                                  float* out);
+/* ComputeButteraugliDistanceNative (:872-1003).  Pinned: compiled reference
+ * and libjxl full-map goldens. */
+int oracle_butteraugli_distance(const float* ref, const float* dist, size_t w,
+                                size_t h, float hf_asymmetry,
+                                float x_multiplier, float intensity, float* map,
+                                double* score);
+/* src/codec/butteraugli.cpp:116-222.  Pinned: compiled reference. */
+void oracle_reduce_distance_map(const float* map, size_t w, size_t h,
+                                const uint8_t* cells, size_t bw, size_t bh,
+                                float* out);
+/* src/codec/adaptive_quantization.cpp:1686-1733, one update after
+ * evaluation `iteration`.  Pinned: closed-form cases (the unit is behind the
+ * reference's toolchain gate). */
+void oracle_aq_update(float* field, const float* adjusted_initial,
+                      const float* block_distance, size_t n, float target,
+                      int iteration, float lower, float upper, float scale,
+                      float inverse_global_scale);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);

@@ -19,6 +19,8 @@
 #include "c_api/image_conversion.h"
 #include "codec/ac_strategy.h"
 #include "codec/butteraugli.h"
+#include "codec/butteraugli_distance_internal.h"
+#include "codec/butteraugli_internal.h"
 #include "codec/chroma_from_luma.h"
 #include "codec/dc_conversion.h"
 #include "codec/dc_quantization.h"
@@ -352,6 +354,136 @@ int ref_loop_filters(const float* in, size_t width, size_t height,
   return Code(gjxl::ApplyLoopFilters(
     ConstImage(in, width, height), {inverse_sigma, {bw, bh}, bw}, {},
     MutableImage(out, width, height)));
+}
+
+namespace {
+gjxl::ButteraugliOptions BOptions(float hf_asymmetry, float x_multiplier,
+                                  float intensity) {
+  return {.hf_asymmetry = hf_asymmetry, .x_multiplier = x_multiplier,
+          .intensity_target = intensity};
+}
+namespace bi = gjxl::butteraugli_internal;
+}  // namespace
+
+// src/codec/butteraugli.cpp:98-115
+int ref_butteraugli_distance(const float* reference, const float* distorted,
+                             size_t width, size_t height, float hf_asymmetry,
+                             float x_multiplier, float intensity, float* map,
+                             double* score) {
+  return Code(gjxl::ComputeButteraugliDistance(
+    ConstImage(reference, width, height), ConstImage(distorted, width, height),
+    BOptions(hf_asymmetry, x_multiplier, intensity),
+    {map, {width, height}, width}, score));
+}
+
+// src/codec/butteraugli.cpp:31-71: one Prepare, then `count` Compares of the
+// distorted images (count x 3 planes), maps written one after another.
+int ref_butteraugli_prepared(const float* reference, const float* distorted,
+                             size_t count, size_t width, size_t height,
+                             float hf_asymmetry, float x_multiplier,
+                             float intensity, float* maps, double* scores) {
+  gjxl::PreparedButteraugliReference prepared;
+  gjxl::Status status = prepared.Prepare(
+    ConstImage(reference, width, height),
+    BOptions(hf_asymmetry, x_multiplier, intensity));
+  if (!status.ok()) return Code(status);
+  const size_t n = width * height;
+  for (size_t i = 0; i < count; ++i) {
+    status = prepared.Compare(ConstImage(distorted + i * 3 * n, width, height),
+                              {maps + i * n, {width, height}, width},
+                              scores + i);
+    if (!status.ok()) return Code(status);
+  }
+  return 0;
+}
+
+// src/codec/butteraugli_internal.cpp:576-620
+int ref_butteraugli_blur(const float* in, size_t width, size_t height,
+                         float sigma, float* out) {
+  bi::BlurScratch scratch;
+  return Code(bi::GaussianBlur({in, {width, height}, width}, sigma, &scratch,
+                               {out, {width, height}, width}));
+}
+
+// :622-700
+int ref_butteraugli_opsin(const float* rgb, size_t width, size_t height,
+                          float intensity, float* xyb) {
+  bi::OpsinScratch scratch;
+  return Code(bi::OpsinDynamicsImage(ConstImage(rgb, width, height), intensity,
+                                     &scratch, MutableImage(xyb, width, height)));
+}
+
+namespace {
+void CopyPsycho(const bi::OwnedPsychoImage& p, size_t n, float* out) {
+  const gjxl::ConstPlaneF32View planes[10] = {
+    p.LowFrequencyView().plane[0], p.LowFrequencyView().plane[1],
+    p.LowFrequencyView().plane[2], p.MediumFrequencyView().plane[0],
+    p.MediumFrequencyView().plane[1], p.MediumFrequencyView().plane[2],
+    p.HighFrequencyView(0), p.HighFrequencyView(1),
+    p.UltraHighFrequencyView(0), p.UltraHighFrequencyView(1)};
+  for (size_t k = 0; k < 10; ++k) std::memcpy(out + k * n, planes[k].data, n * 4);
+}
+}  // namespace
+
+// :702-846; psycho: LF(3), MF(3), HF(2), UHF(2)
+int ref_butteraugli_frequencies(const float* xyb, size_t width, size_t height,
+                                float* psycho) {
+  bi::FrequencyScratch scratch;
+  bi::OwnedPsychoImage out;
+  gjxl::Status status =
+    bi::SeparateFrequencies(ConstImage(xyb, width, height), &scratch, &out);
+  if (!status.ok()) return Code(status);
+  CopyPsycho(out, width * height, psycho);
+  return 0;
+}
+
+// src/codec/butteraugli_distance_internal.cpp:657-870 for two linear images:
+// both pass through opsin dynamics and frequency separation first.
+// stages: 9 planes (DifferenceStage order).
+int ref_butteraugli_stages(const float* reference, const float* distorted,
+                           size_t width, size_t height, float hf_asymmetry,
+                           float x_multiplier, float intensity, float* stages) {
+  const size_t n = width * height;
+  std::vector<float> x0(3 * n), x1(3 * n);
+  bi::OpsinScratch opsin;
+  gjxl::Status status = bi::OpsinDynamicsImage(
+    ConstImage(reference, width, height), intensity, &opsin,
+    MutableImage(x0.data(), width, height));
+  if (!status.ok()) return Code(status);
+  status = bi::OpsinDynamicsImage(ConstImage(distorted, width, height),
+                                  intensity, &opsin,
+                                  MutableImage(x1.data(), width, height));
+  if (!status.ok()) return Code(status);
+  bi::FrequencyScratch frequency;
+  bi::OwnedPsychoImage p0, p1;
+  status = bi::SeparateFrequencies(ConstImage(x0.data(), width, height),
+                                   &frequency, &p0);
+  if (!status.ok()) return Code(status);
+  status = bi::SeparateFrequencies(ConstImage(x1.data(), width, height),
+                                   &frequency, &p1);
+  if (!status.ok()) return Code(status);
+  bi::DifferenceScratch scratch;
+  bi::OwnedDifferenceStages out;
+  status = bi::ComputeDifferenceStages(
+    p0, p1, {.hf_asymmetry = hf_asymmetry, .x_multiplier = x_multiplier,
+             .intensity_target = intensity}, &scratch, &out);
+  if (!status.ok()) return Code(status);
+  for (size_t k = 0; k < bi::kDifferenceStageCount; ++k) {
+    std::memcpy(stages + k * n,
+                out.StageView(static_cast<bi::DifferenceStage>(k)).data, n * 4);
+  }
+  return 0;
+}
+
+// src/codec/butteraugli.cpp:116-222
+int ref_reduce_distance_map(const float* map, size_t width, size_t height,
+                            const uint8_t* cells, size_t bw, size_t bh,
+                            float* out) {
+  gjxl::AcStrategyGrid grid;
+  gjxl::Status status = GridFromCells(cells, bw, bh, &grid);
+  if (!status.ok()) return Code(status);
+  return Code(gjxl::ReduceButteraugliDistanceMap(
+    {map, {width, height}, width}, grid, {out, {bw, bh}, bw}));
 }
 
 }  // extern "C"
