@@ -86,8 +86,19 @@ _SIGNATURES = {
     "gjxl_amd_aq_evaluate": [_P, _P],
     "gjxl_amd_aq_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
                            ctypes.POINTER(_SZ)],
+    "gjxl_amd_butteraugli_create": [_P, _U32, _U32, _F, _F, _F, ctypes.POINTER(_P)],
+    "gjxl_amd_butteraugli_destroy": [_P],
+    "gjxl_amd_butteraugli_prepare": [_P, _P],
+    "gjxl_amd_butteraugli_compare": [_P, _P, _P],
+    "gjxl_amd_butteraugli_score": [_P, ctypes.POINTER(ctypes.c_double)],
+    "gjxl_amd_butteraugli_buffer": [_P, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.POINTER(_P), ctypes.POINTER(_SZ)],
+    "gjxl_amd_reduce_distance_map": [_P, _P, _U32, _U32, _P, _P],
+    "gjxl_amd_aq_update": [_P, _P, _P, ctypes.c_int32, _F],
+    "gjxl_amd_aq_run": [_P, _P, ctypes.c_int32, _F, _P],
 }
-_VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy"}
+_VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
+         "gjxl_amd_butteraugli_destroy"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
@@ -298,7 +309,7 @@ AQ_BUFFERS = (
     "stats", "select", "quantizer", "raw_quant", "y_to_x", "y_to_b",
     "quantized", "dc", "dc_quantized", "dc_reconstructed", "dc_scratch",
     "dc_smoothed", "inverse_sigma", "reconstructed", "scratch_a", "scratch_b",
-    "filtered", "linear",
+    "filtered", "linear", "field", "block_distance", "scores",
 )
 
 
@@ -358,6 +369,88 @@ class AqSession:
         if view.nbytes > size.value:
             raise ValueError(f"{name}: view of {view.nbytes} bytes exceeds {size.value}")
         return view
+
+
+    def update(self, quant_field: DeviceArray, block_distance: DeviceArray,
+               iteration: int, target: float = 1.0) -> None:
+        _check(library().gjxl_amd_aq_update(
+            self.handle, quant_field.ptr, block_distance.ptr, iteration, target))
+
+    def run(self, butteraugli: "Butteraugli", iterations: int = 2,
+            target: float = 1.0) -> list[float]:
+        """The Butteraugli AQ loop; returns the score of every evaluation."""
+        scores = (ctypes.c_double * (iterations + 1))()
+        _check(library().gjxl_amd_aq_run(self.handle, butteraugli.handle,
+                                         iterations, target, scores))
+        return list(scores)
+
+
+BUTTERAUGLI_BUFFERS = ("map", "reference_psycho", "distorted_psycho", "stages", "xyb")
+
+
+class Butteraugli:
+    """Device Butteraugli with a prepared reference (gjxl_amd_butteraugli_*)."""
+
+    def __init__(self, device: Device, width: int, height: int,
+                 hf_asymmetry: float = 1.0, x_multiplier: float = 1.0,
+                 intensity_target: float = 80.0):
+        self.device = device
+        self.width, self.height = width, height
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_butteraugli_create(
+            device.handle, width, height, hf_asymmetry, x_multiplier,
+            intensity_target, ctypes.byref(handle)))
+        self.handle = handle.value
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_butteraugli_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, reference_linear: DeviceArray) -> None:
+        _check(library().gjxl_amd_butteraugli_prepare(self.handle, reference_linear.ptr))
+
+    def compare(self, distorted_linear: DeviceArray,
+                distance_map: DeviceArray | None = None) -> None:
+        _check(library().gjxl_amd_butteraugli_compare(
+            self.handle, distorted_linear.ptr,
+            distance_map.ptr if distance_map is not None else None))
+
+    def score(self) -> float:
+        v = ctypes.c_double()
+        _check(library().gjxl_amd_butteraugli_score(self.handle, ctypes.byref(v)))
+        return v.value
+
+    def buffer(self, name: str, scale: int = 0) -> DeviceArray:
+        ptr, size = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(library().gjxl_amd_butteraugli_buffer(
+            self.handle, scale, BUTTERAUGLI_BUFFERS.index(name), ctypes.byref(ptr),
+            ctypes.byref(size)))
+        if name == "map":
+            shape = (self.height, self.width)
+        else:
+            w = self.width if scale == 0 else (self.width + 1) // 2
+            h = self.height if scale == 0 else (self.height + 1) // 2
+            if scale == 0 and (self.width < 8 or self.height < 8):
+                w, h = max(w, 8), max(h, 8)
+            shape = (size.value // (4 * w * h), h, w)
+        return DeviceArray(self.device, shape, np.float32, pointer=ptr.value)
+
+
+def reduce_distance_map(device: Device, distance_map: DeviceArray,
+                        cells: DeviceArray,
+                        out: DeviceArray | None = None) -> DeviceArray:
+    h, w = distance_map.shape
+    out = out or device.empty(cells.shape)
+    _check(library().gjxl_amd_reduce_distance_map(
+        device.handle, distance_map.ptr, w, h, cells.ptr, out.ptr))
+    return out
 
 
 def gpu_available() -> bool:

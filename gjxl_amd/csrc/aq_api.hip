@@ -7,6 +7,7 @@
 #include <new>
 
 #include "aq_common.h"
+#include "butteraugli.h"
 #include "common.h"
 
 
@@ -42,6 +43,55 @@ T* Buf(GJXLAmdAqSession* s, int id) {
 GJXLResult ValidateSession(GJXLAmdAqSession* s) {
   if (s == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session is null");
   return gjxl_amd::ValidateDevice(s->device);
+}
+
+constexpr int kMaxIterations = 62;  // scores hold iterations + 1 words
+constexpr int kErrorWord = 7;       // in the stats buffer
+
+__global__ void CopyWordKernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+__global__ void ZeroWordKernel(uint32_t* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = 0;
+}
+
+__global__ void CopyFloatsKernel(const float* __restrict__ src, size_t n,
+                                 float* __restrict__ dst) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// One update of RunAdaptiveQuantizationPolicyImpl
+// (src/codec/adaptive_quantization.cpp:1686-1733) per block.  x^0.2 is the
+// binary64 power rounded to binary32, as std::pow(float, double) is.
+__global__ __launch_bounds__(256) void AqUpdateKernel(
+  float* __restrict__ field, const float* __restrict__ initial,
+  const float* __restrict__ distance, size_t n, float target, int iteration,
+  float lower, float upper, const gjxl_amd::QuantizerState* __restrict__ q,
+  uint32_t* __restrict__ error) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float f = field[i];
+  if (iteration == 1) {
+    const float clamp = 0.4f * f + 0.6f * initial[i];
+    if (f < clamp) f = clamp < lower ? lower : (upper < clamp ? upper : clamp);
+  }
+  const float difference = distance[i] / target;
+  if (!isfinite(difference) || difference < 0.0f) {
+    atomicOr(error, 1u);
+    return;
+  }
+  if (difference <= 1.0f) {
+    if (iteration < 2) f *= static_cast<float>(pow(static_cast<double>(difference), 0.2));
+  } else {
+    const float old = f;
+    f *= difference;
+    const float igs = q->inverse_global_scale;
+    // lround: nearest, halves away from zero
+    if (roundf(old * igs) == roundf(f * igs)) f = old + q->scale;
+  }
+  field[i] = f < lower ? lower : (upper < f ? upper : f);
 }
 
 // src/codestream/workflow.cpp:1435-1484 SelectQuantizationMatrixScales
@@ -115,6 +165,9 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_SCRATCH_B] = 4 * 3 * pixels;
   z[GJXL_AMD_AQ_FILTERED] = 4 * 3 * pixels;
   z[GJXL_AMD_AQ_LINEAR] = 4 * 3 * pixels;
+  z[GJXL_AMD_AQ_FIELD] = 4 * n;
+  z[GJXL_AMD_AQ_BLOCK_DISTANCE] = 4 * n;
+  z[GJXL_AMD_AQ_SCORES] = 4 * 64;
   for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
     const GJXLResult r = Check(hipMalloc(&s->buffers[i], z[i]), "hipMalloc(AQ session)");
     if (r != GJXL_OK) {
@@ -269,6 +322,73 @@ GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
     Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA), Buf<float>(s, GJXL_AMD_AQ_SCRATCH_A),
     Buf<float>(s, GJXL_AMD_AQ_SCRATCH_B), Buf<float>(s, GJXL_AMD_AQ_FILTERED),
     Buf<float>(s, GJXL_AMD_AQ_LINEAR), 255.0f);
+}
+
+GJXLResult gjxl_amd_aq_update(GJXLAmdAqSession* s, float* quant_field,
+                              const float* block_distance, int32_t iteration,
+                              float butteraugli_target) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (!s->prepared || quant_field == nullptr || block_distance == nullptr ||
+      iteration < 0 || !std::isfinite(butteraugli_target) || butteraugli_target <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ update arguments are invalid");
+  }
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  hipLaunchKernelGGL(AqUpdateKernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256),
+                     0, s->device->stream, quant_field,
+                     Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD), block_distance, n,
+                     butteraugli_target, static_cast<int>(iteration), s->lower, s->upper,
+                     Buf<gjxl_amd::QuantizerState>(s, GJXL_AMD_AQ_QUANTIZER),
+                     Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + kErrorWord);
+  return gjxl_amd::CheckLaunch("AqUpdateKernel");
+}
+
+GJXLResult gjxl_amd_aq_run(GJXLAmdAqSession* s, GJXLAmdButteraugli* b,
+                           int32_t iterations, float butteraugli_target,
+                           double* scores) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (b == nullptr || !b->prepared || b->device != s->device ||
+      b->width != static_cast<int>(s->width) || b->height != static_cast<int>(s->height) ||
+      !s->prepared || iterations < 0 || iterations > kMaxIterations) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "AQ run needs a prepared session and a prepared Butteraugli of its extent");
+  }
+  hipStream_t st = s->device->stream;
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  float* field = Buf<float>(s, GJXL_AMD_AQ_FIELD);
+  float* distance = Buf<float>(s, GJXL_AMD_AQ_BLOCK_DISTANCE);
+  uint32_t* score_bits = Buf<uint32_t>(s, GJXL_AMD_AQ_SCORES);
+  uint32_t* error = Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + kErrorWord;
+  hipLaunchKernelGGL(ZeroWordKernel, dim3(1), dim3(64), 0, st, error);
+  hipLaunchKernelGGL(CopyFloatsKernel, dim3(static_cast<uint32_t>((n + 255) / 256)), dim3(256),
+                     0, st, Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD), n, field);
+  GJXL_AMD_TRY(gjxl_amd::CheckLaunch("CopyFloatsKernel"));
+  for (int it = 0; it <= iterations; ++it) {
+    GJXL_AMD_TRY(gjxl_amd_aq_evaluate(s, field));
+    GJXL_AMD_TRY(gjxl_amd::ButteraugliCompare(b, Buf<float>(s, GJXL_AMD_AQ_LINEAR), nullptr));
+    hipLaunchKernelGGL(CopyWordKernel, dim3(1), dim3(64), 0, st, b->score_bits,
+                       score_bits + it);
+    GJXL_AMD_TRY(gjxl_amd::LaunchReduceDistanceMap(st, b->map, s->width, s->height, s->cells,
+                                                   s->layout.bw, s->layout.bh, distance));
+    if (it == iterations) break;
+    GJXL_AMD_TRY(gjxl_amd_aq_update(s, field, distance, it, butteraugli_target));
+  }
+  if (scores == nullptr) return GJXL_OK;
+  uint32_t host[kMaxIterations + 2] = {};
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, score_bits, 4 * (iterations + 1),
+                                    hipMemcpyDeviceToHost, st), "hipMemcpyAsync(scores)"));
+  uint32_t err = 0;
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(&err, error, 4, hipMemcpyDeviceToHost, st),
+                     "hipMemcpyAsync(error)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
+  if (err != 0) {
+    return Fail(GJXL_ERROR_INTERNAL, "Adaptive quantization produced an invalid block distance");
+  }
+  for (int it = 0; it <= iterations; ++it) {
+    float v;
+    std::memcpy(&v, &host[it], 4);
+    scores[it] = v;
+  }
+  return GJXL_OK;
 }
 
 GJXLResult gjxl_amd_aq_buffer(GJXLAmdAqSession* s, int32_t id, void** pointer,

@@ -1,0 +1,1008 @@
+// a16-a17: Butteraugli on the device and the per-transform 16-norm
+// reduction of its map.
+//
+//   OpsinDynamicsKernel   5x5 mirrored blur + opsin dynamics (Gamma via the
+//                         unfused FastLog2), butteraugli_internal.cpp:622-700
+//   BlurRows/ColsKernel   separable Gaussian of radius 3, 6, 7 or 16 with the
+//                         reference's interior summation order and its
+//                         renormalised border columns (:245-620)
+//   MediumLow/HighSplit/UltraSplitKernel
+//                         SeparateFrequencies (:702-846)
+//   MaltaDiff/MaltaKernel MaltaDiffMap (butteraugli_distance_internal.cpp:
+//                         482-577), zero outside the image
+//   AcDcKernel            Malta accumulation (UHF, HF, MF) and the L2 terms
+//   MaskSource/MaskKernel masking activity, blur, fuzzy erosion, MaskY/MaskDcY
+//   FinalKernel           masked AC Y and the final composition (:657-870)
+//   Subsample/Expand/ComposeKernel
+//                         the two-scale map and its maximum (:872-1003)
+//   ReduceKernel          ReduceButteraugliDistanceMap (butteraugli.cpp:
+//                         116-222)
+//
+// Every output is computed by one thread in the reference's own operation
+// order (binary32, no contraction; binary64 where the reference promotes), so
+// the maps equal the CPU's bit for bit.  The reference image's planes and
+// masking terms are computed once per prepare and stay resident.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "butteraugli.h"
+#include "tables.h"
+
+namespace gjxl_amd {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTapStride = 64;  // 33 unscaled taps, then 17 scaled taps
+constexpr float kSigmas[kSigmaCount] = {1.2f, 7.15593339443f, 3.22489901262f,
+                                        1.56416327805f, 2.7f};
+
+__device__ __forceinline__ int BMirror(int c, int size) {
+  while (c < 0 || c >= size) c = c < 0 ? -c - 1 : 2 * size - 1 - c;
+  return c;
+}
+__device__ __forceinline__ float BMax(float a, float b) { return a < b ? b : a; }
+
+// ---- Gaussian blurs --------------------------------------------------------
+
+// One output of ConvolutionWithTranspose: position `pos` of a line of `len`
+// samples spaced `stride` floats apart.
+template <int R>
+__device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
+                                              int stride, int pos, int len,
+                                              const float* __restrict__ taps) {
+  const float* k = taps;
+  const float* sk = taps + 33;
+  const int first = len < R ? len : R;
+  const int last = len > R ? len - R : 0;
+  if (pos >= first && pos < last) {
+    const float* s = line + static_cast<ptrdiff_t>(pos - R) * stride;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const float pair = (s[i * stride] + s[(2 * R - i) * stride]) * sk[i];
+      acc[i % 4] = i < 4 ? pair : acc[i % 4] + pair;
+    }
+    float r = s[R * stride] * sk[R];
+#pragma unroll
+    for (int i = 0; i < 4 && i < R; ++i) r = r + acc[i];
+    return r;
+  }
+  const int lo = pos < R ? 0 : pos - R;
+  const int hi = len - 1 < pos + R ? len - 1 : pos + R;
+  float weight = 0.0f;
+  for (int sx = lo; sx <= hi; ++sx) weight += k[sx + R - pos];
+  const float scale = 1.0f / weight;
+  float sum = 0.0f;
+  for (int sx = lo; sx <= hi; ++sx) {
+    sum += line[static_cast<ptrdiff_t>(sx) * stride] * k[sx + R - pos];
+  }
+  return sum * scale;
+}
+
+template <int R>
+__global__ __launch_bounds__(kThreads) void BlurRowsKernel(
+  const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
+  float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  out[static_cast<size_t>(y) * w + x] =
+    ConvolveLine<R>(in + static_cast<size_t>(y) * w, 1, x, w, taps);
+}
+
+template <int R>
+__global__ __launch_bounds__(kThreads) void BlurColsKernel(
+  const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
+  float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  out[static_cast<size_t>(y) * w + x] = ConvolveLine<R>(in + x, w, y, h, taps);
+}
+
+// ---- opsin dynamics ---------------------------------------------------------
+
+__device__ __forceinline__ float ButteraugliFastLog2(float v) {
+  const uint32_t bits = __float_as_uint(v);
+  const int32_t e = static_cast<int32_t>(bits - 0x3f2aaaabu) >> 23;
+  const float x = __uint_as_float(bits - (static_cast<uint32_t>(e) << 23)) - 1.0f;
+  float num = 0.74245873327820566f * x + 1.4287160470083755f;
+  num = num * x + -1.8503833400518310e-06f;
+  float den = 0.17409343003366853f * x + 1.0096718572241148f;
+  den = den * x + 0.99032814277590719f;
+  return num / den + static_cast<float>(e);
+}
+
+__device__ __forceinline__ float ButteraugliGamma(float v) {
+  constexpr float kRetMul = 19.245013259874995f * 0.6931471805599453f;
+  v = BMax(v, 0.0f);
+  return kRetMul * ButteraugliFastLog2(v + 9.9710635769299145f) + -23.16046239805755f;
+}
+
+__device__ __forceinline__ void Absorbance(float r, float g, float b, bool clamp,
+                                           float out[3]) {
+  out[0] = 0.29956550340058319f * r +
+           (0.63373087833825936f * g + (0.077705617820981968f * b + 1.7557483643287353f));
+  out[1] = 0.22158691104574774f * r +
+           (0.69391388044116142f * g + (0.0987313588422f * b + 1.7557483643287353f));
+  out[2] = 0.02f * r + (0.02f * g + (0.20480129041026129f * b + 12.226454707163354f));
+  if (clamp) {
+    out[0] = BMax(out[0], 1.7557483643287353f);
+    out[1] = BMax(out[1], 1.7557483643287353f);
+    out[2] = BMax(out[2], 12.226454707163354f);
+  }
+}
+
+__device__ __forceinline__ float Horizontal5(const float* row, int x, int w,
+                                             float w0, float w1, float w2) {
+  float r = row[x] * w0;
+  r = (row[BMirror(x - 1, w)] + row[BMirror(x + 1, w)]) * w1 + r;
+  r = (row[BMirror(x - 2, w)] + row[BMirror(x + 2, w)]) * w2 + r;
+  return r;
+}
+
+__global__ __launch_bounds__(kThreads) void OpsinDynamicsKernel(
+  const float* __restrict__ rgb, int w, int h, float intensity,
+  const float* __restrict__ taps, float* __restrict__ xyb) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const size_t n = static_cast<size_t>(w) * h;
+  const float w0 = taps[33 + 2], w1 = taps[33 + 1], w2 = taps[33 + 0];
+  const int ym1 = BMirror(y - 1, h), yp1 = BMirror(y + 1, h);
+  const int ym2 = BMirror(y - 2, h), yp2 = BMirror(y + 2, h);
+  float blurred[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float* p = rgb + c * n;
+    const float center = Horizontal5(p + static_cast<size_t>(y) * w, x, w, w0, w1, w2);
+    const float t1 = Horizontal5(p + static_cast<size_t>(ym1) * w, x, w, w0, w1, w2);
+    const float b1 = Horizontal5(p + static_cast<size_t>(yp1) * w, x, w, w0, w1, w2);
+    const float t2 = Horizontal5(p + static_cast<size_t>(ym2) * w, x, w, w0, w1, w2);
+    const float b2 = Horizontal5(p + static_cast<size_t>(yp2) * w, x, w, w0, w1, w2);
+    float r = center * w0;
+    r = (t1 + b1) * w1 + r;
+    r = (t2 + b2) * w2 + r;
+    blurred[c] = r;
+  }
+  const size_t i = static_cast<size_t>(y) * w + x;
+  float pre[3], cur[3], sens[3];
+  Absorbance(blurred[0] * intensity, blurred[1] * intensity, blurred[2] * intensity,
+             true, pre);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    pre[c] = BMax(pre[c], 1.0e-4f);
+    sens[c] = BMax(ButteraugliGamma(pre[c]) / pre[c], 1.0e-4f);
+  }
+  Absorbance(rgb[i] * intensity, rgb[n + i] * intensity, rgb[2 * n + i] * intensity,
+             false, cur);
+  cur[0] = BMax(cur[0] * sens[0], 1.7557483643287353f);
+  cur[1] = BMax(cur[1] * sens[1], 1.7557483643287353f);
+  cur[2] = BMax(cur[2] * sens[2], 12.226454707163354f);
+  xyb[i] = cur[0] - cur[1];
+  xyb[n + i] = cur[0] + cur[1];
+  xyb[2 * n + i] = cur[2];
+}
+
+// ---- frequency separation ---------------------------------------------------
+
+__device__ __forceinline__ float MaximumClamp(float v, float maximum) {
+  constexpr float k = 0.724216145665f;
+  if (v >= maximum) return (v - maximum) * k + maximum;
+  if (v < -maximum) return (v + maximum) * k + -maximum;
+  return v;
+}
+__device__ __forceinline__ float RemoveRange(float v, float width) {
+  if (v > width) return v - width;
+  if (v < -width) return v + width;
+  return 0.0f;
+}
+__device__ __forceinline__ float AmplifyRange(float v, float width) {
+  if (v > width) return v + width;
+  if (v < -width) return v - width;
+  return v + v;
+}
+
+// psycho LF planes hold the raw LF blurs on entry: MF = XYB - LF, then the
+// LF planes are scaled.
+__global__ __launch_bounds__(kThreads) void MediumLowKernel(
+  const float* __restrict__ xyb, size_t n, float* __restrict__ psycho) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* lf = psycho;
+  float* mf = psycho + 3 * n;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) mf[c * n + i] = xyb[c * n + i] - lf[c * n + i];
+  const float ly = lf[n + i];
+  lf[2 * n + i] = (-0.362267051518f * ly + lf[2 * n + i]) * 49.87984651440f;
+  lf[i] *= 33.832837186260f;
+  lf[n + i] = ly * 14.458268100570f;
+}
+
+// After the 3.22 blur of MF channel c (`blurred`): HF and MF of channel c;
+// channel 1 also applies SuppressXByY to HF X.
+__global__ __launch_bounds__(kThreads) void HighSplitKernel(
+  int c, const float* __restrict__ blurred, size_t n, float* __restrict__ psycho) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* mf = psycho + 3 * n;
+  float* hf = psycho + 6 * n;
+  const float b = blurred[i];
+  if (c == 2) {
+    mf[2 * n + i] = b;
+    return;
+  }
+  const float high = mf[c * n + i] - b;
+  hf[c * n + i] = high;
+  mf[c * n + i] = c == 0 ? RemoveRange(b, 0.29f) : AmplifyRange(b, 0.1f);
+  if (c == 1) {
+    constexpr float kS = 46.0f, kMin = 0.653020556257f;
+    const float scaler = (kS / (high * high + kS)) * (1.0f - kMin) + kMin;
+    hf[i] = scaler * hf[i];
+  }
+}
+
+// After the 1.56 blur of HF channel c: UHF and HF of channel c.
+__global__ __launch_bounds__(kThreads) void UltraSplitKernel(
+  int c, const float* __restrict__ blurred, size_t n, float* __restrict__ psycho) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* hf = psycho + (6 + c) * n;
+  float* uhf = psycho + (8 + c) * n;
+  const float original = hf[i];
+  float hv = blurred[i];
+  if (c == 0) {
+    uhf[i] = RemoveRange(original - hv, 0.04f);
+    hf[i] = RemoveRange(hv, 1.5f);
+  } else {
+    hv = MaximumClamp(hv, 28.4691806922f);
+    float uv = original - hv;
+    uv = MaximumClamp(uv, 5.19175294647f);
+    uhf[i] = uv * 2.69313763794f;
+    hf[i] = AmplifyRange(hv * 2.155f, 0.132f);
+  }
+}
+
+// ---- difference stages -------------------------------------------------------
+
+struct MaltaScale {
+  float gt;    // norm2_0_gt_1
+  float lt;    // norm2_0_lt_1
+  float norm;  // (float)norm
+};
+
+__global__ __launch_bounds__(kThreads) void MaltaDiffKernel(
+  const float* __restrict__ p0, const float* __restrict__ p1, size_t n,
+  MaltaScale m, float* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v0 = p0[i], v1 = p1[i];
+  const float absolute = 0.5f * (fabsf(v0) + fabsf(v1));
+  const float diff = v0 - v1;
+  const float scaler = m.gt / (m.norm + absolute);
+  float s = scaler * diff;
+  const float scaler2 = m.lt / (m.norm + absolute);
+  const double magnitude = fabs(static_cast<double>(v0));
+  const double too_small = 0.55 * magnitude, too_big = 1.05 * magnitude;
+  const double dv1 = v1, ds2 = scaler2;
+  if (v0 < 0.0f) {
+    if (dv1 > -too_small) {
+      s = static_cast<float>(static_cast<double>(s) - ds2 * (dv1 + too_small));
+    } else if (dv1 < -too_big) {
+      s = static_cast<float>(static_cast<double>(s) + ds2 * (-dv1 - too_big));
+    }
+  } else if (dv1 < too_small) {
+    s = static_cast<float>(static_cast<double>(s) + ds2 * (too_small - dv1));
+  } else if (dv1 > too_big) {
+    s = static_cast<float>(static_cast<double>(s) - ds2 * (dv1 - too_big));
+  }
+  out[i] = s;
+}
+
+__device__ __forceinline__ float Sum4(float a, float b, float c, float d) {
+  return (a + b) + (c + d);
+}
+
+// The 16 Malta lines (MaltaUnitLf / MaltaUnitFull).  Offsets as (dx, dy).
+struct MaltaTap {
+  signed char dx, dy;
+};
+constexpr MaltaTap kLfLines[16][5] = {
+  {{-4, 0}, {-2, 0}, {0, 0}, {2, 0}, {4, 0}},
+  {{0, -4}, {0, -2}, {0, 0}, {0, 2}, {0, 4}},
+  {{-3, -3}, {-2, -2}, {0, 0}, {2, 2}, {3, 3}},
+  {{3, -3}, {2, -2}, {0, 0}, {-2, 2}, {-3, 3}},
+  {{1, -4}, {1, -2}, {0, 0}, {-1, 2}, {-1, 4}},
+  {{-1, -4}, {-1, -2}, {0, 0}, {1, 2}, {1, 4}},
+  {{-4, -1}, {-2, -1}, {0, 0}, {2, 1}, {4, 1}},
+  {{-4, 1}, {-2, 1}, {0, 0}, {2, -1}, {4, -1}},
+  {{-2, -3}, {-1, -2}, {0, 0}, {1, 2}, {2, 3}},
+  {{2, -3}, {1, -2}, {0, 0}, {-1, 2}, {-2, 3}},
+  {{-3, -2}, {-2, -1}, {0, 0}, {2, 1}, {3, 2}},
+  {{3, -2}, {2, -1}, {0, 0}, {-2, 1}, {-3, 2}},
+  {{-4, 2}, {-2, 1}, {0, 0}, {2, -1}, {4, -2}},
+  {{-4, -2}, {-2, -1}, {0, 0}, {2, 1}, {4, 2}},
+  {{-2, -4}, {-1, -2}, {0, 0}, {1, 2}, {2, 4}},
+  {{2, -4}, {1, -2}, {0, 0}, {-1, 2}, {-2, 4}},
+};
+constexpr int kFullTaps[16] = {9, 9, 7, 7, 9, 9, 9, 9, 7, 7, 7, 7, 9, 9, 9, 9};
+constexpr MaltaTap kFullLines[16][9] = {
+  {{-4, 0}, {-3, 0}, {-2, 0}, {-1, 0}, {0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 0}},
+  {{0, -4}, {0, -3}, {0, -2}, {0, -1}, {0, 0}, {0, 1}, {0, 2}, {0, 3}, {0, 4}},
+  {{-3, -3}, {-2, -2}, {-1, -1}, {0, 0}, {1, 1}, {2, 2}, {3, 3}},
+  {{3, -3}, {2, -2}, {1, -1}, {0, 0}, {-1, 1}, {-2, 2}, {-3, 3}},
+  {{1, -4}, {1, -3}, {1, -2}, {0, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}, {-1, 4}},
+  {{-1, -4}, {-1, -3}, {-1, -2}, {0, -1}, {0, 0}, {0, 1}, {1, 2}, {1, 3}, {1, 4}},
+  {{-4, -1}, {-3, -1}, {-2, -1}, {-1, 0}, {0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 1}},
+  {{-4, 1}, {-3, 1}, {-2, 1}, {-1, 0}, {0, 0}, {1, 0}, {2, -1}, {3, -1}, {4, -1}},
+  {{-2, -3}, {-1, -2}, {-1, -1}, {0, 0}, {1, 1}, {1, 2}, {2, 3}},
+  {{2, -3}, {1, -2}, {1, -1}, {0, 0}, {-1, 1}, {-1, 2}, {-2, 3}},
+  {{-3, -2}, {-2, -1}, {-1, -1}, {0, 0}, {1, 1}, {2, 1}, {3, 2}},
+  {{3, -2}, {2, -1}, {1, -1}, {0, 0}, {-1, 1}, {-2, 1}, {-3, 2}},
+  {{-4, 1}, {-3, 1}, {-2, 1}, {-1, 0}, {0, 0}, {1, 0}, {2, -1}, {3, -1}, {4, -1}},
+  {{-4, -1}, {-3, -1}, {-2, -1}, {-1, 0}, {0, 0}, {1, 0}, {2, 1}, {3, 1}, {4, 1}},
+  {{-1, -4}, {-1, -3}, {-1, -2}, {0, -1}, {0, 0}, {0, 1}, {1, 2}, {1, 3}, {1, 4}},
+  {{1, -4}, {1, -3}, {1, -2}, {0, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}, {-1, 4}},
+};
+
+// Malta response of the scaled differences `d` (zero outside the image)
+// into `out` (0 + response, as the reference accumulates into a zeroed
+// plane).
+template <bool kLowFrequency>
+__global__ __launch_bounds__(kThreads) void MaltaKernel(const float* __restrict__ d,
+                                                        int w, int h,
+                                                        float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  auto P = [&](int dx, int dy) -> float {
+    const int sx = x + dx, sy = y + dy;
+    if (sx < 0 || sy < 0 || sx >= w || sy >= h) return 0.0f;
+    return d[static_cast<size_t>(sy) * w + sx];
+  };
+  float result = 0.0f;
+#pragma unroll
+  for (int l = 0; l < 16; ++l) {
+    float s;
+    if constexpr (kLowFrequency) {
+      const MaltaTap* t = kLfLines[l];
+      s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
+               P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
+    } else {
+      const MaltaTap* t = kFullLines[l];
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
+      s = kFullTaps[l] == 9
+            ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
+            : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
+    }
+    result = l == 0 ? s * s : s * s + result;
+  }
+  out[static_cast<size_t>(y) * w + x] = 0.0f + result;
+}
+
+struct L2Weights {
+  float asym_gt[2];  // weight_0_gt_1 * 0.8 for HF X, Y
+  float asym_lt[2];  // weight_0_lt_1 * 0.8
+  bool asym_on[2];
+  float mf[3];
+  float lf[3];
+};
+
+// Malta accumulation in the reference's order (UHF, HF, MF), then the
+// asymmetric HF and symmetric MF L2 terms; DC differences from LF.
+__global__ __launch_bounds__(kThreads) void AcDcKernel(
+  const float* __restrict__ ps0, const float* __restrict__ ps1,
+  const float* __restrict__ stages, size_t n, L2Weights lw,
+  float* __restrict__ ac, float* __restrict__ dc) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a[3];
+  a[1] = 0.0f;
+  a[1] += stages[4 * n + i];
+  a[1] += stages[2 * n + i];
+  a[1] += stages[0 * n + i];
+  a[0] = 0.0f;
+  a[0] += stages[5 * n + i];
+  a[0] += stages[3 * n + i];
+  a[0] += stages[1 * n + i];
+  a[2] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c < 2 && lw.asym_on[c]) {
+      const float v0 = ps0[(6 + c) * n + i], v1 = ps1[(6 + c) * n + i];
+      const float d = v0 - v1;
+      const float total = (d * d) * lw.asym_gt[c] + a[c];
+      const float mag = fabsf(v0);
+      const float small = 0.4f * mag;
+      float sec = 0.0f;
+      if (v0 < 0.0f) {
+        if (v1 > -small) sec = v1 + small;
+        else if (v1 < -mag) sec = -v1 - mag;
+      } else if (v1 < small) {
+        sec = small - v1;
+      } else if (v1 > mag) {
+        sec = v1 - mag;
+      }
+      a[c] = lw.asym_lt[c] * (sec * sec) + total;
+    }
+    if (lw.mf[c] != 0.0f) {
+      const float d = ps0[(3 + c) * n + i] - ps1[(3 + c) * n + i];
+      a[c] = (d * d) * lw.mf[c] + a[c];
+    }
+    const float e = ps0[c * n + i] - ps1[c * n + i];
+    dc[c * n + i] = lw.lf[c] == 0.0f ? 0.0f : e * e * lw.lf[c];
+    ac[c * n + i] = a[c];
+  }
+}
+
+// CombineChannelsForMasking + DiffPrecompute.
+__global__ __launch_bounds__(kThreads) void MaskSourceKernel(
+  const float* __restrict__ ps, size_t n, float* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* hf = ps + 6 * n;
+  const float* uhf = ps + 8 * n;
+  constexpr float kMul = 6.19424080439f;
+  const float bias = kMul * 12.61050594197f;
+  const float sqrt_bias = sqrtf(bias);
+  const float xd = (uhf[i] + hf[i]) * 2.5f;
+  const float yd = uhf[n + i] * 0.4f + hf[n + i] * 0.4f;
+  const float activity = sqrtf(xd * xd + yd * yd);
+  out[i] = sqrtf(kMul * fabsf(activity) + bias) - sqrt_bias;
+}
+
+__device__ __forceinline__ void StoreMin3(float v, float& m0, float& m1, float& m2) {
+  if (v < m2) {
+    if (v < m0) {
+      m2 = m1;
+      m1 = m0;
+      m0 = v;
+    } else if (v < m1) {
+      m2 = m1;
+      m1 = v;
+    } else {
+      m2 = v;
+    }
+  }
+}
+
+// FuzzyErosion of the reference's blurred activity, and MaskY / MaskDcY of
+// the result.
+__global__ __launch_bounds__(kThreads) void MaskKernel(
+  const float* __restrict__ in, int w, int h, float* __restrict__ mask,
+  float* __restrict__ mask_y, float* __restrict__ mask_dc) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  constexpr int s = 3;
+  auto at = [&](int xx, int yy) { return in[static_cast<size_t>(yy) * w + xx]; };
+  float m0 = at(x, y);
+  float m1 = 2.0f * m0, m2 = m1;
+  if (x >= s) {
+    StoreMin3(at(x - s, y), m0, m1, m2);
+    if (y >= s) StoreMin3(at(x - s, y - s), m0, m1, m2);
+    if (y + s < h) StoreMin3(at(x - s, y + s), m0, m1, m2);
+  }
+  if (x + s < w) {
+    StoreMin3(at(x + s, y), m0, m1, m2);
+    if (y >= s) StoreMin3(at(x + s, y - s), m0, m1, m2);
+    if (y + s < h) StoreMin3(at(x + s, y + s), m0, m1, m2);
+  }
+  if (y >= s) StoreMin3(at(x, y - s), m0, m1, m2);
+  if (y + s < h) StoreMin3(at(x, y + s), m0, m1, m2);
+  const float m = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
+  const size_t i = static_cast<size_t>(y) * w + x;
+  mask[i] = m;
+  constexpr double kGlobal = 1.0 / (17.83 * 0.79079917404);
+  const double vy = kGlobal * (1.0 + 2.5485944793 / (0.451936922203 * m + 0.829591754942));
+  const double vd = kGlobal * (1.0 + 0.505054525019 / (3.87449418804 * m + 0.20025578522));
+  mask_y[i] = static_cast<float>(vy * vy);
+  mask_dc[i] = static_cast<float>(vd * vd);
+}
+
+__global__ __launch_bounds__(kThreads) void FinalKernel(
+  const float* __restrict__ ref_blurred, const float* __restrict__ dist_blurred,
+  const float* __restrict__ mask_y, const float* __restrict__ mask_dc,
+  float* __restrict__ ac, const float* __restrict__ dc, size_t n,
+  float x_multiplier, float* __restrict__ masked_ac_y, float* __restrict__ final_map) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float d = ref_blurred[i] - dist_blurred[i];
+  const float ay = ac[n + i] + 10.0f * d * d;
+  ac[n + i] = ay;
+  masked_ac_y[i] = ay;
+  const float my = mask_y[i], mdc = mask_dc[i];
+  const float dc0 = dc[i] * x_multiplier;
+  const float ac0 = ac[i] * x_multiplier;
+  const float masked_dc = dc0 * mdc + dc[n + i] * mdc + dc[2 * n + i] * mdc;
+  const float masked_ac = ac0 * my + ay * my + ac[2 * n + i] * my;
+  final_map[i] = sqrtf(masked_dc + masked_ac);
+}
+
+// ---- scales ------------------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void SubsampleKernel(
+  const float* __restrict__ in, int w, int h, float* __restrict__ out) {
+  const int sw = (w + 1) / 2, sh = (h + 1) / 2;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int c = blockIdx.z;
+  if (x >= sw) return;
+  const float* p = in + static_cast<size_t>(c) * w * h;
+  const int x0 = 2 * x, y0 = 2 * y;
+  float v = 0.0f;
+  v += 0.25f * p[static_cast<size_t>(y0) * w + x0];
+  if (x0 + 1 < w) v += 0.25f * p[static_cast<size_t>(y0) * w + x0 + 1];
+  if (y0 + 1 < h) {
+    v += 0.25f * p[static_cast<size_t>(y0 + 1) * w + x0];
+    if (x0 + 1 < w) v += 0.25f * p[static_cast<size_t>(y0 + 1) * w + x0 + 1];
+  }
+  if ((w & 1) && x == sw - 1) v *= 2.0f;
+  if ((h & 1) && y == sh - 1) v *= 2.0f;
+  out[static_cast<size_t>(c) * sw * sh + static_cast<size_t>(y) * sw + x] = v;
+}
+
+// ExpandImage for extents below 8: edge-replicated into ew x eh.
+__global__ __launch_bounds__(kThreads) void ExpandKernel(
+  const float* __restrict__ in, int w, int h, int xb, int yb, int ew, int eh,
+  float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int c = blockIdx.z;
+  if (x >= ew) return;
+  int sy = y > yb ? y - yb : 0;
+  sy = sy < h - 1 ? sy : h - 1;
+  int sx = x > xb ? x - xb : 0;
+  sx = sx < w - 1 ? sx : w - 1;
+  out[static_cast<size_t>(c) * ew * eh + static_cast<size_t>(y) * ew + x] =
+    in[static_cast<size_t>(c) * w * h + static_cast<size_t>(sy) * w + sx];
+}
+
+// The requested-extent map: 0.85 main + 0.5 sub (two scales), the main map
+// cropped from the expanded extent, or the main map alone; and its maximum
+// (non-negative floats compare as their bit patterns).
+__global__ __launch_bounds__(kThreads) void ComposeKernel(
+  const float* __restrict__ main_map, int mw, const float* __restrict__ sub_map,
+  int sw, int w, int h, int xb, int yb, float* __restrict__ map,
+  float* __restrict__ map_out, uint32_t* __restrict__ score_bits) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  float v = 0.0f;
+  if (x < w) {
+    const float m = main_map[static_cast<size_t>(y + yb) * mw + x + xb];
+    v = sub_map != nullptr ? m * 0.85f + 0.5f * sub_map[static_cast<size_t>(y / 2) * sw + x / 2]
+                           : m;
+    map[static_cast<size_t>(y) * w + x] = v;
+    if (map_out != nullptr) map_out[static_cast<size_t>(y) * w + x] = v;
+  }
+  uint32_t bits = __float_as_uint(v);
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), off));
+    bits = other > bits ? other : bits;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(score_bits, bits);
+}
+
+__global__ void FillWordKernel(uint32_t* p, uint32_t value) {
+  if (threadIdx.x == 0) *p = value;
+}
+
+// ---- a17 reduction ------------------------------------------------------------
+
+// x^(1/16) by four binary64 square roots, rounded once to binary32.
+__device__ __forceinline__ float Root16(float x) {
+  return static_cast<float>(sqrt(sqrt(sqrt(sqrt(static_cast<double>(x))))));
+}
+
+__global__ __launch_bounds__(kThreads) void ReduceKernel(
+  const float* __restrict__ map, int w, int h, const uint8_t* __restrict__ cells,
+  int bw, int bh, float* __restrict__ out) {
+  const int bx = blockIdx.x * blockDim.x + threadIdx.x, by = blockIdx.y;
+  if (bx >= bw) return;
+  const uint8_t c = cells[static_cast<size_t>(by) * bw + bx];
+  if (!(c & 1)) return;
+  const int cw = CoveredW(c >> 1), ch = CoveredH(c >> 1);
+  // a footprint past the grid is not a valid transform: nothing is written
+  if (bx + cw > bw || by + ch > bh) return;
+  const int x0 = bx * 8, y0 = by * 8;
+  const int x1 = min(w, x0 + cw * 8), y1 = min(h, y0 + ch * 8);
+  float norm = 0.0f;
+  int count = 0;
+  for (int y = y0; y < y1; ++y) {
+    for (int x = x0; x < x1; ++x) {
+      float v = map[static_cast<size_t>(y) * w + x];
+      v *= v;
+      v *= v;
+      v *= v;
+      v *= v;
+      norm += v;
+      ++count;
+    }
+  }
+  if (count == 0) return;
+  const float d = 1.2f * Root16(norm / static_cast<float>(count));
+  for (int dy = 0; dy < ch; ++dy)
+    for (int dx = 0; dx < cw; ++dx) out[static_cast<size_t>(by + dy) * bw + bx + dx] = d;
+}
+
+// ---- host side -------------------------------------------------------------------
+
+void HostTaps(float sigma, float* out) {
+  float k[33] = {};
+  const float scaled = 2.25f * std::fabs(sigma);
+  int radius = static_cast<int>(scaled);
+  radius = radius < 1 ? 1 : radius;
+  const int size = 2 * radius + 1;
+  // the host libm's exp, as the reference evaluates it; volatile keeps the
+  // compiler from folding it
+  volatile double exponent_scale = -1.0 / (2.0 * sigma * sigma);
+  for (int i = -radius; i <= radius; ++i) {
+    k[i + radius] = static_cast<float>(std::exp(exponent_scale * i * i));
+  }
+  float full = 0.0f;
+  for (int i = 0; i < size; ++i) full += k[i];
+  const float scale = 1.0f / full;
+  for (int i = 0; i < 33; ++i) out[i] = k[i];
+  for (int i = 0; i < 17; ++i) out[33 + i] = i <= radius ? k[i] * scale : 0.0f;
+}
+
+inline dim3 Grid2(int w, int h, int z = 1) {
+  return dim3(CeilDiv(static_cast<uint32_t>(w), kThreads), h, z);
+}
+inline dim3 Grid1(size_t n) {
+  return dim3(static_cast<uint32_t>((n + kThreads - 1) / kThreads));
+}
+
+const float* Taps(const GJXLAmdButteraugli* b, ButteraugliSigma s) {
+  return b->taps + s * kTapStride;
+}
+
+template <int R>
+GJXLResult Blur(hipStream_t st, const float* in, int w, int h, const float* taps,
+                float* tmp, float* out) {
+  hipLaunchKernelGGL(BlurRowsKernel<R>, Grid2(w, h), dim3(kThreads), 0, st, in, w, h,
+                     taps, tmp);
+  hipLaunchKernelGGL(BlurColsKernel<R>, Grid2(w, h), dim3(kThreads), 0, st, tmp, w, h,
+                     taps, out);
+  return CheckLaunch("BlurKernel");
+}
+
+// Opsin dynamics and frequency separation of `in` into `psycho`.
+GJXLResult Psycho(GJXLAmdButteraugli* b, ButteraugliScale& s, const float* in,
+                  float* psycho) {
+  hipStream_t st = b->device->stream;
+  const int w = s.width, h = s.height;
+  const size_t n = s.n;
+  hipLaunchKernelGGL(OpsinDynamicsKernel, Grid2(w, h), dim3(kThreads), 0, st, in, w, h,
+                     b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
+  GJXL_AMD_TRY(CheckLaunch("OpsinDynamicsKernel"));
+  for (int c = 0; c < 3; ++c) {
+    GJXL_AMD_TRY(Blur<16>(st, s.xyb + c * n, w, h, Taps(b, kSigmaLow), s.tmp[0],
+                          psycho + c * n));
+  }
+  hipLaunchKernelGGL(MediumLowKernel, Grid1(n), dim3(kThreads), 0, st, s.xyb, n, psycho);
+  GJXL_AMD_TRY(CheckLaunch("MediumLowKernel"));
+  for (int c = 0; c < 3; ++c) {
+    GJXL_AMD_TRY(Blur<7>(st, psycho + (3 + c) * n, w, h, Taps(b, kSigmaHigh), s.tmp[0],
+                         s.tmp[1]));
+    hipLaunchKernelGGL(HighSplitKernel, Grid1(n), dim3(kThreads), 0, st, c, s.tmp[1], n,
+                       psycho);
+    GJXL_AMD_TRY(CheckLaunch("HighSplitKernel"));
+  }
+  for (int c = 0; c < 2; ++c) {
+    GJXL_AMD_TRY(Blur<3>(st, psycho + (6 + c) * n, w, h, Taps(b, kSigmaUltra), s.tmp[0],
+                         s.tmp[1]));
+    hipLaunchKernelGGL(UltraSplitKernel, Grid1(n), dim3(kThreads), 0, st, c, s.tmp[1], n,
+                       psycho);
+    GJXL_AMD_TRY(CheckLaunch("UltraSplitKernel"));
+  }
+  return GJXL_OK;
+}
+
+// Blurred masking activity of a psycho image into `out`.
+GJXLResult MaskActivity(GJXLAmdButteraugli* b, ButteraugliScale& s, const float* psycho,
+                        float* out) {
+  hipStream_t st = b->device->stream;
+  hipLaunchKernelGGL(MaskSourceKernel, Grid1(s.n), dim3(kThreads), 0, st, psycho, s.n,
+                     s.tmp[0]);
+  GJXL_AMD_TRY(CheckLaunch("MaskSourceKernel"));
+  return Blur<6>(st, s.tmp[0], s.width, s.height, Taps(b, kSigmaMask), s.tmp[1], out);
+}
+
+constexpr double kMaltaWeights[6] = {37.0819870399, 8246.75321353, 18.7237414387,
+                                     6923.99476109, 1.10039032555, 173.5};
+constexpr double kMaltaNorms[6] = {130262059.556, 1009002.70582, 4498534.45232,
+                                   8051.15833247, 71.7800275169, 5.0};
+constexpr float kL2[9] = {400.0f, 1.50815703118f, 0.0f, 2150.0f, 10.6195433239f,
+                          16.2176043152f, 29.2353797994f, 0.844626970982f,
+                          0.703646627719f};
+
+GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
+  hipStream_t st = b->device->stream;
+  const int w = s.width, h = s.height;
+  const size_t n = s.n;
+  const float asym = b->params.hf_asymmetry;
+  const float sqrt_asym = std::sqrt(asym);
+  const double up[6] = {kMaltaWeights[0], kMaltaWeights[1], kMaltaWeights[2] * sqrt_asym,
+                        kMaltaWeights[3] * sqrt_asym, kMaltaWeights[4] * asym,
+                        kMaltaWeights[5] * asym};
+  const double down[6] = {kMaltaWeights[0], kMaltaWeights[1], kMaltaWeights[2] / sqrt_asym,
+                          kMaltaWeights[3] / sqrt_asym, kMaltaWeights[4] / asym,
+                          kMaltaWeights[5] / asym};
+  const int plane_of[6] = {4, 3, 7, 6, 9, 8};
+  for (int k = 0; k < 6; ++k) {
+    const bool lf = k < 4;
+    const double multiplier = lf ? 0.611612573796 : 0.39905817637;
+    const double pre_gt = multiplier * std::sqrt(0.5f * up[k]) / (3.75 * 2.0 + 1.0);
+    const double pre_lt = multiplier * std::sqrt(0.33f * down[k]) / (3.75 * 2.0 + 1.0);
+    const MaltaScale m{static_cast<float>(pre_gt * kMaltaNorms[k]),
+                       static_cast<float>(pre_lt * kMaltaNorms[k]),
+                       static_cast<float>(kMaltaNorms[k])};
+    hipLaunchKernelGGL(MaltaDiffKernel, Grid1(n), dim3(kThreads), 0, st,
+                       s.ref_psycho + plane_of[k] * n, s.dist_psycho + plane_of[k] * n, n, m,
+                       s.tmp[0]);
+    if (lf) {
+      hipLaunchKernelGGL(MaltaKernel<true>, Grid2(w, h), dim3(kThreads), 0, st, s.tmp[0], w,
+                         h, s.stages + k * n);
+    } else {
+      hipLaunchKernelGGL(MaltaKernel<false>, Grid2(w, h), dim3(kThreads), 0, st, s.tmp[0],
+                         w, h, s.stages + k * n);
+    }
+    GJXL_AMD_TRY(CheckLaunch("MaltaKernel"));
+  }
+  L2Weights lw{};
+  for (int c = 0; c < 2; ++c) {
+    const float gt = kL2[c] * asym, lt = kL2[c] / asym;
+    lw.asym_on[c] = !(gt == 0.0f && lt == 0.0f);
+    lw.asym_gt[c] = gt * 0.8f;
+    lw.asym_lt[c] = lt * 0.8f;
+  }
+  for (int c = 0; c < 3; ++c) {
+    lw.mf[c] = kL2[3 + c];
+    lw.lf[c] = kL2[6 + c];
+  }
+  hipLaunchKernelGGL(AcDcKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_psycho,
+                     s.dist_psycho, s.stages, n, lw, s.ac, s.dc);
+  GJXL_AMD_TRY(CheckLaunch("AcDcKernel"));
+  GJXL_AMD_TRY(MaskActivity(b, s, s.dist_psycho, s.tmp[2]));
+  hipLaunchKernelGGL(FinalKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_blurred,
+                     s.tmp[2], s.mask_y, s.mask_dc, s.ac, s.dc, n,
+                     b->params.x_multiplier, s.stages + 7 * n, s.stages + 8 * n);
+  return CheckLaunch("FinalKernel");
+}
+
+constexpr int kScalePlanes = 3 + 3 + 10 + 10 + 3 + 9 + 3 + 3 + 3;
+
+}  // namespace
+
+GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
+                              float* map_out) {
+  hipStream_t st = b->device->stream;
+  ButteraugliScale& m = b->scale[0];
+  const float* main_in = distorted;
+  if (b->expanded) {
+    hipLaunchKernelGGL(ExpandKernel, Grid2(m.width, m.height, 3), dim3(kThreads), 0, st,
+                       distorted, b->width, b->height, b->xborder, b->yborder, m.width,
+                       m.height, m.input);
+    GJXL_AMD_TRY(CheckLaunch("ExpandKernel"));
+    main_in = m.input;
+  }
+  GJXL_AMD_TRY(Psycho(b, m, main_in, m.dist_psycho));
+  GJXL_AMD_TRY(Differences(b, m));
+  const float* sub_map = nullptr;
+  int sw = 0;
+  if (b->has_subscale) {
+    ButteraugliScale& s = b->scale[1];
+    hipLaunchKernelGGL(SubsampleKernel, Grid2(s.width, s.height, 3), dim3(kThreads), 0, st,
+                       distorted, b->width, b->height, s.input);
+    GJXL_AMD_TRY(CheckLaunch("SubsampleKernel"));
+    GJXL_AMD_TRY(Psycho(b, s, s.input, s.dist_psycho));
+    GJXL_AMD_TRY(Differences(b, s));
+    sub_map = s.stages + 8 * s.n;
+    sw = s.width;
+  }
+  hipLaunchKernelGGL(FillWordKernel, dim3(1), dim3(64), 0, st, b->score_bits, 0u);
+  hipLaunchKernelGGL(ComposeKernel, Grid2(b->width, b->height), dim3(kThreads), 0, st,
+                     m.stages + 8 * m.n, m.width, sub_map, sw, b->width, b->height,
+                     b->xborder, b->yborder, b->map, map_out, b->score_bits);
+  return CheckLaunch("ComposeKernel");
+}
+
+GJXLResult LaunchReduceDistanceMap(hipStream_t s, const float* map, int width,
+                                   int height, const uint8_t* cells, int bw,
+                                   int bh, float* out) {
+  hipLaunchKernelGGL(ReduceKernel, Grid2(bw, bh), dim3(kThreads), 0, s, map, width,
+                     height, cells, bw, bh, out);
+  return CheckLaunch("ReduceKernel");
+}
+
+}  // namespace gjxl_amd
+
+using gjxl_amd::Check;
+using gjxl_amd::Fail;
+
+extern "C" {
+
+GJXLResult gjxl_amd_butteraugli_create(GJXLAmdDevice* device, uint32_t width,
+                                       uint32_t height, float hf_asymmetry,
+                                       float x_multiplier, float intensity_target,
+                                       GJXLAmdButteraugli** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || *out != nullptr || width == 0 || height == 0 ||
+      width > 65535 || height > 65535 || !std::isfinite(hf_asymmetry) ||
+      hf_asymmetry <= 0.0f || !std::isfinite(x_multiplier) || x_multiplier <= 0.0f ||
+      !std::isfinite(intensity_target) || intensity_target <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli geometry or options are invalid");
+  }
+  auto* b = new (std::nothrow) GJXLAmdButteraugli;
+  if (b == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "Butteraugli allocation failed");
+  b->device = device;
+  b->params = {hf_asymmetry, x_multiplier, intensity_target};
+  b->width = static_cast<int>(width);
+  b->height = static_cast<int>(height);
+  b->expanded = width < 8 || height < 8;
+  b->has_subscale = !b->expanded && width >= 15 && height >= 15;
+  b->xborder = width < 8 ? (8 - static_cast<int>(width)) / 2 : 0;
+  b->yborder = height < 8 ? (8 - static_cast<int>(height)) / 2 : 0;
+  const int mw = width < 8 ? 8 : static_cast<int>(width);
+  const int mh = height < 8 ? 8 : static_cast<int>(height);
+  b->scale[0].width = mw;
+  b->scale[0].height = mh;
+  b->scale[1].width = (static_cast<int>(width) + 1) / 2;
+  b->scale[1].height = (static_cast<int>(height) + 1) / 2;
+  size_t floats = gjxl_amd::kSigmaCount * 64 + static_cast<size_t>(width) * height + 64;
+  const int scales = b->has_subscale ? 2 : 1;
+  for (int k = 0; k < scales; ++k) {
+    auto& s = b->scale[k];
+    s.n = static_cast<size_t>(s.width) * s.height;
+    floats += gjxl_amd::kScalePlanes * s.n;
+  }
+  GJXLResult r = Check(hipSetDevice(device->ordinal), "hipSetDevice");
+  if (r == GJXL_OK) r = Check(hipMalloc(&b->allocation, floats * sizeof(float)),
+                             "hipMalloc(Butteraugli)");
+  if (r != GJXL_OK) {
+    delete b;
+    return r;
+  }
+  float* p = static_cast<float*>(b->allocation);
+  b->taps = p;
+  p += gjxl_amd::kSigmaCount * 64;
+  b->score_bits = reinterpret_cast<uint32_t*>(p);
+  p += 64;
+  b->map = p;
+  p += static_cast<size_t>(width) * height;
+  for (int k = 0; k < scales; ++k) {
+    auto& s = b->scale[k];
+    const size_t n = s.n;
+    s.input = p; p += 3 * n;
+    s.xyb = p; p += 3 * n;
+    s.ref_psycho = p; p += 10 * n;
+    s.dist_psycho = p; p += 10 * n;
+    s.ref_blurred = p; p += n;
+    s.mask_y = p; p += n;
+    s.mask_dc = p; p += n;
+    s.stages = p; p += 9 * n;
+    s.ac = p; p += 3 * n;
+    s.dc = p; p += 3 * n;
+    for (auto& t : s.tmp) { t = p; p += n; }
+  }
+  float host[gjxl_amd::kSigmaCount * 64] = {};
+  for (int k = 0; k < gjxl_amd::kSigmaCount; ++k) {
+    gjxl_amd::HostTaps(gjxl_amd::kSigmas[k], host + k * 64);
+  }
+  r = Check(hipMemcpy(b->taps, host, sizeof(host), hipMemcpyHostToDevice),
+            "hipMemcpy(Butteraugli taps)");
+  if (r != GJXL_OK) {
+    gjxl_amd_butteraugli_destroy(b);
+    return r;
+  }
+  *out = b;
+  return GJXL_OK;
+}
+
+void gjxl_amd_butteraugli_destroy(GJXLAmdButteraugli* b) {
+  if (b == nullptr) return;
+  if (b->device != nullptr) {
+    (void)hipSetDevice(b->device->ordinal);
+    (void)hipStreamSynchronize(b->device->stream);
+  }
+  if (b->allocation != nullptr) (void)hipFree(b->allocation);
+  delete b;
+}
+
+GJXLResult gjxl_amd_butteraugli_prepare(GJXLAmdButteraugli* b,
+                                        const float* reference_linear) {
+  if (b == nullptr || reference_linear == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli prepare arguments are null");
+  }
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(b->device));
+  using namespace gjxl_amd;
+  b->prepared = false;
+  hipStream_t st = b->device->stream;
+  for (int k = 0; k < (b->has_subscale ? 2 : 1); ++k) {
+    ButteraugliScale& s = b->scale[k];
+    const float* in = reference_linear;
+    if (k == 0 && b->expanded) {
+      hipLaunchKernelGGL(ExpandKernel, Grid2(s.width, s.height, 3), dim3(kThreads), 0, st,
+                         reference_linear, b->width, b->height, b->xborder, b->yborder,
+                         s.width, s.height, s.input);
+      GJXL_AMD_TRY(CheckLaunch("ExpandKernel"));
+      in = s.input;
+    } else if (k == 1) {
+      hipLaunchKernelGGL(SubsampleKernel, Grid2(s.width, s.height, 3), dim3(kThreads), 0,
+                         st, reference_linear, b->width, b->height, s.input);
+      GJXL_AMD_TRY(CheckLaunch("SubsampleKernel"));
+      in = s.input;
+    }
+    GJXL_AMD_TRY(Psycho(b, s, in, s.ref_psycho));
+    GJXL_AMD_TRY(MaskActivity(b, s, s.ref_psycho, s.ref_blurred));
+    hipLaunchKernelGGL(MaskKernel, Grid2(s.width, s.height), dim3(kThreads), 0, st,
+                       s.ref_blurred, s.width, s.height, s.stages + 6 * s.n, s.mask_y,
+                       s.mask_dc);
+    GJXL_AMD_TRY(CheckLaunch("MaskKernel"));
+  }
+  b->prepared = true;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_butteraugli_compare(GJXLAmdButteraugli* b,
+                                        const float* distorted_linear,
+                                        float* distance_map) {
+  if (b == nullptr || distorted_linear == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli compare arguments are null");
+  }
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(b->device));
+  if (!b->prepared) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli reference has not been prepared");
+  }
+  return gjxl_amd::ButteraugliCompare(b, distorted_linear, distance_map);
+}
+
+GJXLResult gjxl_amd_butteraugli_score(GJXLAmdButteraugli* b, double* score) {
+  if (b == nullptr || score == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli score arguments are null");
+  }
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(b->device));
+  uint32_t bits = 0;
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(&bits, b->score_bits, 4, hipMemcpyDeviceToHost,
+                                    b->device->stream), "hipMemcpyAsync(score)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(b->device->stream), "hipStreamSynchronize"));
+  float v;
+  std::memcpy(&v, &bits, 4);
+  *score = v;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_butteraugli_buffer(GJXLAmdButteraugli* b, int32_t scale,
+                                       int32_t id, void** pointer, size_t* bytes) {
+  if (b == nullptr || pointer == nullptr || scale < 0 || scale > 1 ||
+      (scale == 1 && !b->has_subscale)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli buffer request is invalid");
+  }
+  const gjxl_amd::ButteraugliScale& s = b->scale[scale];
+  const size_t plane = s.n * sizeof(float);
+  void* p = nullptr;
+  size_t z = 0;
+  switch (id) {
+    case GJXL_AMD_BUTTERAUGLI_MAP: p = b->map; z = sizeof(float) * b->width * b->height; break;
+    case GJXL_AMD_BUTTERAUGLI_REFERENCE_PSYCHO: p = s.ref_psycho; z = 10 * plane; break;
+    case GJXL_AMD_BUTTERAUGLI_DISTORTED_PSYCHO: p = s.dist_psycho; z = 10 * plane; break;
+    case GJXL_AMD_BUTTERAUGLI_STAGES: p = s.stages; z = 9 * plane; break;
+    case GJXL_AMD_BUTTERAUGLI_XYB: p = s.xyb; z = 3 * plane; break;
+    default: return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli buffer id is invalid");
+  }
+  *pointer = p;
+  if (bytes) *bytes = z;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_reduce_distance_map(GJXLAmdDevice* device, const float* map,
+                                        uint32_t width, uint32_t height,
+                                        const uint8_t* cells, float* block_distance) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (map == nullptr || cells == nullptr || block_distance == nullptr || width == 0 ||
+      height == 0) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Distance-map reduction arguments are invalid");
+  }
+  const int bw = static_cast<int>(gjxl_amd::CeilDiv(width, 8));
+  const int bh = static_cast<int>(gjxl_amd::CeilDiv(height, 8));
+  return gjxl_amd::LaunchReduceDistanceMap(device->stream, map, width, height, cells, bw,
+                                           bh, block_distance);
+}
+
+}  // extern "C"

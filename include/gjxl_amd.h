@@ -197,6 +197,9 @@ enum {
   GJXL_AMD_AQ_SCRATCH_B,            /* f32 3 planes w*h: after EPF pass 1 */
   GJXL_AMD_AQ_FILTERED,             /* f32 3 planes w*h: after EPF pass 2 */
   GJXL_AMD_AQ_LINEAR,               /* f32 3 planes w*h: linear RGB */
+  GJXL_AMD_AQ_FIELD,                /* f32 per block: gjxl_amd_aq_run field */
+  GJXL_AMD_AQ_BLOCK_DISTANCE,       /* f32 per block: last 16-norms */
+  GJXL_AMD_AQ_SCORES,               /* u32[64]: score bits per evaluation */
   GJXL_AMD_AQ_BUFFER_COUNT
 };
 
@@ -245,6 +248,80 @@ GJXL_AMD_API GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* session,
 GJXL_AMD_API GJXLResult gjxl_amd_aq_buffer(GJXLAmdAqSession* session,
                                            int32_t id, void** pointer,
                                            size_t* bytes);
+
+/* ---------------------------------------------------------------------- */
+/* a16: Butteraugli with a prepared reference.  Replaces                    */
+/* PreparedButteraugliReference::{Prepare, Compare} (src/codec/butteraugli.h:*/
+/* 27-54, src/codec/butteraugli.cpp:31-71; the native scalar pipeline of     */
+/* src/codec/butteraugli_distance_internal.cpp:872-1202) and                 */
+/* DeviceButteraugliOperation (src/gpu/ops/butteraugli.h:110).  Images are   */
+/* 3 planes of width*height linear RGB floats on the device.                 */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdButteraugli GJXLAmdButteraugli;
+
+enum {
+  GJXL_AMD_BUTTERAUGLI_MAP = 0,          /* f32 width*height: last map */
+  GJXL_AMD_BUTTERAUGLI_REFERENCE_PSYCHO, /* 10 planes: LF(3) MF(3) HF(2) UHF(2) */
+  GJXL_AMD_BUTTERAUGLI_DISTORTED_PSYCHO,
+  GJXL_AMD_BUTTERAUGLI_STAGES,           /* 9 planes, DifferenceStage order */
+  GJXL_AMD_BUTTERAUGLI_XYB               /* 3 planes: last opsin image */
+};
+
+GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_create(
+  GJXLAmdDevice* device, uint32_t width, uint32_t height, float hf_asymmetry,
+  float x_multiplier, float intensity_target, GJXLAmdButteraugli** out);
+GJXL_AMD_API void gjxl_amd_butteraugli_destroy(GJXLAmdButteraugli* b);
+GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_prepare(
+  GJXLAmdButteraugli* b, const float* reference_linear);
+/* Enqueues one comparison; distance_map (device, width*height) may be null
+ * (the map stays in the object's own buffer). */
+GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_compare(
+  GJXLAmdButteraugli* b, const float* distorted_linear, float* distance_map);
+/* Waits for the stream and returns the last comparison's score (the map's
+ * maximum). */
+GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_score(GJXLAmdButteraugli* b,
+                                                   double* score);
+/* A device buffer of scale 0 (full) or 1 (half resolution, present from 15
+ * pixels in both axes): see GJXL_AMD_BUTTERAUGLI_*. */
+GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_buffer(GJXLAmdButteraugli* b,
+                                                    int32_t scale, int32_t id,
+                                                    void** pointer,
+                                                    size_t* bytes);
+
+/* ---------------------------------------------------------------------- */
+/* a17: per-transform 16-norm of a distance map.  Replaces                   */
+/* ReduceButteraugliDistanceMap (src/codec/butteraugli.cpp:116-222).  map:   */
+/* width*height floats; cells and block_distance: ceil(w/8)*ceil(h/8).       */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_reduce_distance_map(
+  GJXLAmdDevice* device, const float* map, uint32_t width, uint32_t height,
+  const uint8_t* cells, float* block_distance);
+
+/* ---------------------------------------------------------------------- */
+/* a18: the Butteraugli AQ loop.  Replaces RunAdaptiveQuantizationPolicyImpl */
+/* (src/codec/adaptive_quantization.cpp:1600-1738) behind                    */
+/* PreparedAqEvaluation::EvaluateResidentButteraugliPolicy (src/gpu/ops/     */
+/* aq_evaluation.h:219-420).                                                 */
+/* gjxl_amd_aq_update: one field update after evaluation `iteration` of a    */
+/* prepared session, with the session's last quantizer, adjusted initial     */
+/* field and bounds (device field and block_distance, (pw/8)*(ph/8)).        */
+/* gjxl_amd_aq_run: iterations + 1 evaluations from the adjusted initial     */
+/* field, each compared with Butteraugli (b prepared with the source linear  */
+/* RGB) and reduced, with updates between them.  The final field, block      */
+/* distances and the last evaluation's frame stay in the session             */
+/* (GJXL_AMD_AQ_FIELD, GJXL_AMD_AQ_BLOCK_DISTANCE, ...); scores (host,       */
+/* iterations + 1 doubles) may be null, otherwise the call waits for them.   */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_update(GJXLAmdAqSession* session,
+                                           float* quant_field,
+                                           const float* block_distance,
+                                           int32_t iteration,
+                                           float butteraugli_target);
+GJXL_AMD_API GJXLResult gjxl_amd_aq_run(GJXLAmdAqSession* session,
+                                        GJXLAmdButteraugli* butteraugli,
+                                        int32_t iterations,
+                                        float butteraugli_target,
+                                        double* scores);
 
 #ifdef __cplusplus
 }  /* extern "C" */
