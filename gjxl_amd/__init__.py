@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -179,9 +180,13 @@ class Device:
         handle = ctypes.c_void_p()
         _check(library().gjxl_amd_device_create(ordinal, ctypes.byref(handle)))
         self.handle = handle.value
+        # sessions and comparators bound to this device: closed first
+        self._children = weakref.WeakSet()
 
     def close(self) -> None:
         if self.handle:
+            for child in list(self._children):
+                child.close()
             library().gjxl_amd_device_destroy(self.handle)
             self.handle = None
 
@@ -307,7 +312,7 @@ class Device:
 AQ_BUFFERS = (
     "offsets", "anchor_list", "anchor_count", "coefficients", "adjusted_field",
     "stats", "select", "quantizer", "raw_quant", "y_to_x", "y_to_b",
-    "quantized", "dc", "dc_quantized", "dc_reconstructed", "dc_scratch",
+    "quantized", "dc", "dc_quantized", "dc_reconstructed",
     "dc_smoothed", "inverse_sigma", "reconstructed", "scratch_a", "scratch_b",
     "filtered", "linear", "field", "block_distance", "scores",
 )
@@ -325,6 +330,7 @@ class AqSession:
         _check(library().gjxl_amd_aq_session_create(
             device.handle, width, height, ctypes.byref(handle)))
         self.handle = handle.value
+        device._children.add(self)
 
     def close(self) -> None:
         if self.handle:
@@ -377,12 +383,13 @@ class AqSession:
             self.handle, quant_field.ptr, block_distance.ptr, iteration, target))
 
     def run(self, butteraugli: "Butteraugli", iterations: int = 2,
-            target: float = 1.0) -> list[float]:
-        """The Butteraugli AQ loop; returns the score of every evaluation."""
-        scores = (ctypes.c_double * (iterations + 1))()
+            target: float = 1.0, scores: bool = True) -> list[float] | None:
+        """The Butteraugli AQ loop.  With scores, waits and returns the score
+        of every evaluation; otherwise only enqueues."""
+        out = (ctypes.c_double * (iterations + 1))() if scores else None
         _check(library().gjxl_amd_aq_run(self.handle, butteraugli.handle,
-                                         iterations, target, scores))
-        return list(scores)
+                                         iterations, target, out))
+        return list(out) if scores else None
 
 
 BUTTERAUGLI_BUFFERS = ("map", "reference_psycho", "distorted_psycho", "stages", "xyb")
@@ -401,6 +408,7 @@ class Butteraugli:
             device.handle, width, height, hf_asymmetry, x_multiplier,
             intensity_target, ctypes.byref(handle)))
         self.handle = handle.value
+        device._children.add(self)
 
     def close(self) -> None:
         if self.handle:

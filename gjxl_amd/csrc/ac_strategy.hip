@@ -54,18 +54,19 @@ constexpr int kCflThreads = 256;
 
 // One work group per 64x64 colour tile (src/codec/chroma_from_luma.cpp:
 // 434-529): DCT8 of each block and channel, LLF cleared, scaled by the DCT8
-// X/B inverse-dequant tables into the four value arrays in block-raster then
-// coefficient order; then 20 damped Newton steps per map
-// (:40-94, :112-165), each lane of the reference's 4-lane sums on one thread.
+// X/B inverse-dequant tables into the reference's luma/chroma value arrays
+// (block-raster then coefficient order), kept as the per-element a = luma /
+// 84 and b = base * luma - chroma of both maps; then up to 20 damped Newton
+// steps per map (:40-94, :112-165).  Each step's derivative terms are formed
+// by all threads, 256 elements at a time, and summed by one thread per lane
+// of the reference's 4-lane sums, in element order.
 __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
   const float* __restrict__ opsin, int width, int height,
   const float* __restrict__ tables, const double* __restrict__ bases,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
-  // DCT8 coefficients of the tile per channel (LLF cleared); the four value
-  // arrays of the reference are products with the DCT8 tables, formed on
-  // the fly (multiplying by the quant scale 1.0 is exact).
-  __shared__ float coefs[3][4096];
-  __shared__ float inv[2][64];
+  // DCT phase: X, Y, B coefficients in values[0..2]; then a_x, b_x, a_b, b_b
+  __shared__ float values[4][4096];
+  __shared__ float terms[2][3][kCflThreads];
   __shared__ float lane_sum[2][3][4];
   __shared__ float state_x[2];
   __shared__ int state_done[2];
@@ -112,47 +113,75 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
     }
     coef[0] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 64; ++i) coefs[c][k * 64 + i] = coef[i];
-  }
-  if (tid < 64) {
-    inv[0][tid] = inv_x[tid];
-    inv[1][tid] = inv_b[tid];
+    for (int i = 0; i < 64; ++i) values[c][k * 64 + i] = coef[i];
   }
   if (tid < 2) {
     state_x[tid] = 0.0f;
     state_done[tid] = 0;
   }
   __syncthreads();
-
   const int n = nblocks * 64;
+  constexpr int kPer = 4096 / kCflThreads;
+  float yv[kPer], xv[kPer], bv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = tid + j * kCflThreads;
+    yv[j] = i < n ? values[1][i] : 0.0f;
+    xv[j] = i < n ? values[0][i] : 0.0f;
+    bv[j] = i < n ? values[2][i] : 0.0f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = tid + j * kCflThreads;
+    if (i >= n) continue;
+    const float ix = inv_x[i & 63], ib = inv_b[i & 63];
+    const float luma_x = yv[j] * ix, chroma_x = xv[j] * ix;
+    const float luma_b = yv[j] * ib, chroma_b = bv[j] * ib;
+    values[0][i] = luma_x / 84.0f;
+    values[1][i] = 0.0f * luma_x - chroma_x;
+    values[2][i] = luma_b / 84.0f;
+    values[3][i] = 1.0f * luma_b - chroma_b;
+  }
+  __syncthreads();
+
   const float count = static_cast<float>(n);
   constexpr float kEps = 100.0f;
+  // summing threads: problem p, derivative kind, lane
+  const int sp = tid / 12, skind = (tid / 4) % 3, slane = tid % 4;
   for (int it = 0; it < 20; ++it) {
-    if (tid < 8) {
-      const int problem = tid / 4, lane = tid % 4;
-      if (!state_done[problem]) {
-        const float* luma_coef = coefs[1];
-        const float* chroma_coef = coefs[problem == 0 ? 0 : 2];
-        const float* table = inv[problem];
-        const float base = problem == 0 ? 0.0f : 1.0f;
-        const float x = state_x[problem];
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-        for (int i = lane; i < n; i += 4) {
-          const float luma = luma_coef[i] * table[i & 63];
-          const float chroma = chroma_coef[i] * table[i & 63];
-          const float a = luma / 84.0f;
-          const float b = base * luma - chroma;
+    const int done0 = state_done[0], done1 = state_done[1];
+    if (done0 && done1) break;
+    const float x0 = state_x[0], x1 = state_x[1];
+    float acc = 0.0f;
+    for (int chunk = 0; chunk < n; chunk += kCflThreads) {
+      const int i = chunk + tid;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (p == 0 ? done0 : done1) continue;
+        float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+        if (i < n) {
+          const float a = values[2 * p][i], b = values[2 * p + 1][i];
+          const float x = p == 0 ? x0 : x1;
           const float cr = __builtin_fmaf(a, x, b);
-          if (fabsf(cr) >= 100.0f) continue;
-          s0 += CflDerivative(a, cr);
-          s1 += CflDerivative(a, __builtin_fmaf(a, x + kEps, b));
-          s2 += CflDerivative(a, __builtin_fmaf(a, x - kEps, b));
+          if (!(fabsf(cr) >= 100.0f)) {
+            t0 = CflDerivative(a, cr);
+            t1 = CflDerivative(a, __builtin_fmaf(a, x + kEps, b));
+            t2 = CflDerivative(a, __builtin_fmaf(a, x - kEps, b));
+          }
         }
-        lane_sum[problem][0][lane] = s0;
-        lane_sum[problem][1][lane] = s1;
-        lane_sum[problem][2][lane] = s2;
+        terms[p][0][tid] = t0;
+        terms[p][1][tid] = t1;
+        terms[p][2][tid] = t2;
       }
+      __syncthreads();
+      if (tid < 24 && !(sp == 0 ? done0 : done1)) {
+        const int limit = min(kCflThreads, n - chunk);
+        for (int j = slane; j < limit; j += 4) acc += terms[sp][skind][j];
+      }
+      __syncthreads();
     }
+    if (tid < 24) lane_sum[sp][skind][slane] = acc;
     __syncthreads();
     if (tid < 2 && !state_done[tid]) {
       const float x = state_x[tid];
@@ -191,8 +220,9 @@ __device__ __forceinline__ int CeilLog2Nonzero(int v) {
   return v <= 1 ? 0 : 32 - __clz(v - 1);
 }
 
-constexpr int kCostThreads = 256;
+constexpr int kCostThreads = 512;
 constexpr int kCostSlots = 3072;
+constexpr int kUnroll = 4;  // outputs per thread in the DCT passes
 
 template <int kStrategy>
 struct CostShape {
@@ -204,8 +234,19 @@ struct CostShape {
   static constexpr int kCands = kCostSlots / (3 * kN);
   static constexpr int kStep =
     (kStrategy == kDct32x16 || kStrategy == kDct16x32 || kStrategy == kDct32x32) ? 2 : 1;
+  // anchor positions inside a 64x64 tile, in raster order
+  static constexpr int kNx = (8 - kCw) / kStep + 1;
+  static constexpr int kNy = (8 - kCh) / kStep + 1;
+  static constexpr int kPositions = kNx * kNy;
+  // LDS copies of the bases: forward W, forward H, inverse W, inverse H
+  // (square transforms share the W and H copies)
+  static constexpr int kBasis = kW == kH ? 2 * kW * kW : 2 * (kW * kW + kH * kH);
 };
 
+// One work group: kCands anchor positions of one 64x64 tile (blockIdx.y),
+// positions blockIdx.x * kCands + k of the tile's raster list.  Only
+// positions whose footprint lies inside the tile are launched; the rest of
+// the cost plane is NaN (filled beforehand).
 template <int kStrategy>
 __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   const float* __restrict__ opsin, int width, int height,
@@ -215,31 +256,46 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   CostParams params, float* __restrict__ costs) {
   using S = CostShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
-  __shared__ float pix[K][3][N];     // pixels, then residual coefficients
+  __shared__ float pix[K][3][N];     // pixels; residual; weighted loss terms
   __shared__ double tmp[K][3][N];    // horizontal pass (forward, inverse)
-  __shared__ float coef[K][3][N];    // coefficients, then weighted loss terms
-  __shared__ float sqa[K][3][N];     // sqrt(|rounded|)
+  __shared__ float coef[K][3][N];    // coefficients; then sqrt(|rounded|)
+  __shared__ double basis[S::kBasis];
   __shared__ int nnz[K][3];
   __shared__ float sums[K][3][2];
   __shared__ float qn[K];
   __shared__ float cfl[K][3];
   __shared__ int valid[K];
+  __shared__ int bxs[K], bys[K];
 
   const int tid = threadIdx.x;
   const int bw = width / 8, bh = height / 8;
-  const int by = blockIdx.y;
+  const int tiles_w = (bw + 7) / 8;
+  const int tx = blockIdx.y % tiles_w, ty = blockIdx.y / tiles_w;
   const size_t plane = static_cast<size_t>(width) * height;
+  double* fh = basis;
+  double* ih = basis + W * W;
+  double* fv = W == H ? fh : ih + W * W;
+  double* iv = W == H ? ih : fv + H * H;
+  for (int i = tid; i < W * W; i += kCostThreads) {
+    fh[i] = bases[BasisOffset(W, true) + i];
+    ih[i] = bases[BasisOffset(W, false) + i];
+  }
+  if (W != H) {
+    for (int i = tid; i < H * H; i += kCostThreads) {
+      fv[i] = bases[BasisOffset(H, true) + i];
+      iv[i] = bases[BasisOffset(H, false) + i];
+    }
+  }
 
   if (tid < K) {
-    const int bx = blockIdx.x * K + tid;
-    int ok = bx < bw && by < bh;
-    if (ok) {
-      const int lx = bx % 8, ly = by % 8;
-      const int tbw = min(8, bw - (bx - lx)), tbh = min(8, bh - (by - ly));
-      ok = lx + S::kCw <= tbw && ly + S::kCh <= tbh &&
-           (S::kStep == 1 || ((lx | ly) & 1) == 0);
-    }
+    const int p = blockIdx.x * K + tid;
+    const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
+    const int bx = tx * 8 + lx, by = ty * 8 + ly;
+    const int tbw = min(8, bw - tx * 8), tbh = min(8, bh - ty * 8);
+    const int ok = p < S::kPositions && lx + S::kCw <= tbw && ly + S::kCh <= tbh;
     valid[tid] = ok;
+    bxs[tid] = bx;
+    bys[tid] = by;
     if (ok) {
       // src/codec/ac_strategy.cpp:103-137 ComputeQuantNormUnchecked
       const float* q = quant_field + static_cast<size_t>(by) * bw + bx;
@@ -263,7 +319,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
         norm = FastPow(sum, 1.0f / 16.0f);
       }
       qn[tid] = norm;
-      const int tile = (by / 8) * ((bw + 7) / 8) + bx / 8;
+      const int tile = ty * tiles_w + tx;
       cfl[tid][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
       cfl[tid][1] = 0.0f;
       cfl[tid][2] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
@@ -278,127 +334,166 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     const int y = r / W, x = r % W;
     float v = 0.0f;
     if (valid[k]) {
-      const int bx = blockIdx.x * K + k;
-      v = opsin[c * plane + static_cast<size_t>(by * 8 + y) * width + bx * 8 + x];
+      v = opsin[c * plane + static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
     }
     pix[k][c][r] = v;
   }
   __syncthreads();
 
-  const double* fh = bases + BasisOffset(W, true);
-  const double* fv = bases + BasisOffset(H, true);
-  const double* ih = bases + BasisOffset(W, false);
-  const double* iv = bases + BasisOffset(H, false);
-
+  // Every pass computes kUnroll outputs per thread with independent binary64
+  // chains, each summed in the reference's order (src/codec/dct.cpp:135-233).
+  constexpr int kRowTasks = K * 3 * H * (W / kUnroll);
+  constexpr int kColTasks = K * 3 * W * (H / kUnroll);
   // Forward horizontal: tmp[y][u] = sum_x pix[y][x] * fh[u][x].
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
-    const int y = r / W, u = r % W;
-    const float* row = pix[k][c] + y * W;
-    double sum = 0.0;
+  for (int task = tid; task < kRowTasks; task += kCostThreads) {
+    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
+    const int kc = row / H, y = row % H;
+    const float* src = &pix[0][0][0] + kc * N + y * W;
+    double acc[kUnroll] = {};
 #pragma unroll 8
-    for (int x = 0; x < W; ++x) sum += static_cast<double>(row[x]) * fh[u * W + x];
-    tmp[k][c][r] = sum;
+    for (int x = 0; x < W; ++x) {
+      const double v = static_cast<double>(src[x]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += v * fh[(g * kUnroll + j) * W + x];
+    }
+    double* dst = &tmp[0][0][0] + kc * N + y * W + g * kUnroll;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) dst[j] = acc[j];
   }
   __syncthreads();
   // Forward vertical: coef[idx(v,u)] = sum_y fv[v][y] * tmp[y][u].
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
-    const int v = r / W, u = r % W;
-    double sum = 0.0;
+  for (int task = tid; task < kColTasks; task += kCostThreads) {
+    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+    const int kc = col / W, u = col % W;
+    const double* src = &tmp[0][0][0] + kc * N + u;
+    double acc[kUnroll] = {};
 #pragma unroll 8
-    for (int y = 0; y < H; ++y) sum += fv[v * H + y] * tmp[k][c][y * W + u];
-    coef[k][c][CoefficientIndex<W, H>(v, u)] = static_cast<float>(sum);
+    for (int y = 0; y < H; ++y) {
+      const double v = src[y * W];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += fv[(g * kUnroll + j) * H + y] * v;
+    }
+    float* dst = &coef[0][0][0] + kc * N;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      dst[CoefficientIndex<W, H>(g * kUnroll + j, u)] = static_cast<float>(acc[j]);
+    }
   }
   __syncthreads();
-  // Quantise (src/codec/ac_strategy.cpp:914-926).
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
-    const float* dq = tables + DequantOffset(kStrategy, c);
-    const float* inv = tables + InverseOffset(kStrategy, c);
-    const float dec = coef[k][c][i] - coef[k][1][i] * cfl[k][c];
-    const float scaled = dec * inv[i] * qn[k];
-    const float rounded = roundf(scaled);
-    const float diff = scaled - rounded;
-    pix[k][c][i] = dq[i] * diff;
-    sqa[k][c][i] = sqrtf(fabsf(rounded));
-    if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
+  // Quantise (src/codec/ac_strategy.cpp:914-926): residual into pix, and
+  // sqrt(|rounded|) into coef once every thread has read its coefficients.
+  constexpr int kPer = (K * 3 * N + kCostThreads - 1) / kCostThreads;
+  float root[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    root[j] = 0.0f;
+    if (e < K * 3 * N) {
+      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+      const float* dq = tables + DequantOffset(kStrategy, c);
+      const float* inv = tables + InverseOffset(kStrategy, c);
+      const float dec = coef[k][c][i] - coef[k][1][i] * cfl[k][c];
+      const float scaled = dec * inv[i] * qn[k];
+      const float rounded = roundf(scaled);
+      const float diff = scaled - rounded;
+      pix[k][c][i] = dq[i] * diff;
+      root[j] = sqrtf(fabsf(rounded));
+      if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
+    }
   }
   __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    if (e < K * 3 * N) (&coef[0][0][0])[e] = root[j];
+  }
   // Inverse horizontal: tmp[v][x] = sum_u res[idx(v,u)] * ih[u][x].
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
-    const int v = r / W, x = r % W;
-    double sum = 0.0;
+  for (int task = tid; task < kRowTasks; task += kCostThreads) {
+    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
+    const int kc = row / H, v = row % H;
+    const float* src = &pix[0][0][0] + kc * N;
+    double acc[kUnroll] = {};
 #pragma unroll 8
     for (int u = 0; u < W; ++u) {
-      sum += static_cast<double>(pix[k][c][CoefficientIndex<W, H>(v, u)]) * ih[u * W + x];
+      const double r = static_cast<double>(src[CoefficientIndex<W, H>(v, u)]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += r * ih[u * W + g * kUnroll + j];
     }
-    tmp[k][c][r] = sum;
+    double* dst = &tmp[0][0][0] + kc * N + v * W + g * kUnroll;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) dst[j] = acc[j];
   }
   __syncthreads();
-  // Inverse vertical and the masked 8th power (:935-952).
+  // Inverse vertical and the masked 8th power (:935-952) into pix.
   constexpr float kMaskOffset[3] = {12.0f, 0.0f, 4.0f};
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
-    const int y = r / W, x = r % W;
-    double sum = 0.0;
+  for (int task = tid; task < kColTasks; task += kCostThreads) {
+    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+    const int kc = col / W, x = col % W;
+    const int k = kc / 3, c = kc % 3;
+    const double* src = &tmp[0][0][0] + kc * N + x;
+    double acc[kUnroll] = {};
 #pragma unroll 8
-    for (int v = 0; v < H; ++v) sum += iv[v * H + y] * tmp[k][c][v * W + x];
-    float weighted = 0.0f;
-    if (valid[k]) {
-      const int bx = blockIdx.x * K + k;
-      const float m = pixel_mask[static_cast<size_t>(by * 8 + y) * width + bx * 8 + x];
-      weighted = (m + kMaskOffset[c]) * static_cast<float>(sum);
-      weighted *= weighted;
-      weighted *= weighted;
-      weighted *= weighted;
+    for (int v = 0; v < H; ++v) {
+      const double t = src[v * W];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += iv[v * H + g * kUnroll + j] * t;
     }
-    coef[k][c][r] = weighted;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const int y = g * kUnroll + j;
+      float weighted = 0.0f;
+      if (valid[k]) {
+        const float m =
+          pixel_mask[static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
+        weighted = (m + kMaskOffset[c]) * static_cast<float>(acc[j]);
+        weighted *= weighted;
+        weighted *= weighted;
+        weighted *= weighted;
+      }
+      pix[k][c][y * W + x] = weighted;
+    }
   }
   __syncthreads();
   // Ordered sums, one thread each.
   if (tid < K * 6) {
     const int k = tid / 6, c = (tid / 2) % 3, kind = tid % 2;
-    const float* src = kind == 0 ? sqa[k][c] : coef[k][c];
+    const float* src = kind == 0 ? coef[k][c] : pix[k][c];
     float s = 0.0f;
     for (int i = 0; i < N; ++i) s += src[i];
     sums[k][c][kind] = s;
   }
   __syncthreads();
-  if (tid < K) {
+  if (tid < K && valid[tid]) {
     const int k = tid;
-    const int bx = blockIdx.x * K + k;
-    if (bx < bw) {
-      float result = __builtin_nanf("");
-      if (valid[k]) {
-        constexpr float kChannelMultiplier[3] = {2.0441408586549744e7f, 1.0f,
-                                                 1.266770081387616f};
-        float entropy = 0.0f, loss = 0.0f;
-        for (int c = 0; c < 3; ++c) {
-          loss += sums[k][c][1] * kChannelMultiplier[c];
-          entropy += params.cost_delta * sums[k][c][0];
-          const int nz_bits = CeilLog2Nonzero(nnz[k][c] + 1) + 1;
-          entropy += params.zeros_multiplier *
-                     static_cast<float>(CeilLog2Nonzero(nz_bits + 17) + nz_bits);
-          if (c == 0 && S::kCw * S::kCh >= 2) {
-            const float weight =
-              1.0f + StdMin(3.0f, static_cast<float>(S::kCw * S::kCh) / 8.0f);
-            entropy *= weight;
-            loss *= weight;
-          }
-        }
-        const float normalized = loss / static_cast<float>(N);
-        // x^(1/8) as three binary64 square roots, rounded once to binary32.
-        const float root8 = static_cast<float>(
-          sqrt(sqrt(sqrt(static_cast<double>(normalized)))));
-        const float loss_cost = root8 * static_cast<float>(N) / qn[k];
-        result = entropy * params.entropy_multiplier +
-                 params.info_loss_multiplier * loss_cost;
+    constexpr float kChannelMultiplier[3] = {2.0441408586549744e7f, 1.0f,
+                                             1.266770081387616f};
+    float entropy = 0.0f, loss = 0.0f;
+    for (int c = 0; c < 3; ++c) {
+      loss += sums[k][c][1] * kChannelMultiplier[c];
+      entropy += params.cost_delta * sums[k][c][0];
+      const int nz_bits = CeilLog2Nonzero(nnz[k][c] + 1) + 1;
+      entropy += params.zeros_multiplier *
+                 static_cast<float>(CeilLog2Nonzero(nz_bits + 17) + nz_bits);
+      if (c == 0 && S::kCw * S::kCh >= 2) {
+        const float weight =
+          1.0f + StdMin(3.0f, static_cast<float>(S::kCw * S::kCh) / 8.0f);
+        entropy *= weight;
+        loss *= weight;
       }
-      costs[static_cast<size_t>(by) * bw + bx] = result;
     }
+    const float normalized = loss / static_cast<float>(N);
+    // x^(1/8) as three binary64 square roots, rounded once to binary32.
+    const float root8 = static_cast<float>(
+      sqrt(sqrt(sqrt(static_cast<double>(normalized)))));
+    const float loss_cost = root8 * static_cast<float>(N) / qn[k];
+    costs[static_cast<size_t>(bys[k]) * bw + bxs[k]] =
+      entropy * params.entropy_multiplier + params.info_loss_multiplier * loss_cost;
   }
+}
+
+__global__ void FillNanKernel(float* __restrict__ p, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __builtin_nanf("");
 }
 
 // ---- a10: greedy merge, one thread per 64x64 tile --------------------------
@@ -627,7 +722,7 @@ GJXLResult LaunchCost(GJXLAmdDevice* device, const float* opsin, int width,
                       const CostParams& params, float* stage_costs) {
   using S = CostShape<kStrategy>;
   const int bw = width / 8, bh = height / 8;
-  const dim3 grid(CeilDiv(bw, S::kCands), bh);
+  const dim3 grid(CeilDiv(S::kPositions, S::kCands), CeilDiv(bw, 8) * CeilDiv(bh, 8));
   hipLaunchKernelGGL(CandidateCostKernel<kStrategy>, grid, dim3(kCostThreads),
                      0, device->stream, opsin, width, height, qf, mask, yx, yb,
                      device->quant_tables, device->dct_bases, params,
@@ -659,6 +754,9 @@ GJXLResult LaunchCandidateCosts(GJXLAmdDevice* device, const float* opsin,
   p.zeros_multiplier = 9.3089059022677905f * std::pow(ratio, 0.50990926717963703f);
   p.cost_delta = 10.833273317067883f * std::pow(ratio, 0.36702940662370243f);
   const size_t stage = static_cast<size_t>(width / 8) * (height / 8);
+  hipLaunchKernelGGL(FillNanKernel, dim3(static_cast<uint32_t>((7 * stage + 255) / 256)),
+                     dim3(256), 0, device->stream, costs, 7 * stage);
+  GJXL_AMD_TRY(CheckLaunch("FillNanKernel"));
   constexpr float kEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
   p.entropy_multiplier = kEntropy[0];
   GJXL_AMD_TRY(LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 0 * stage));

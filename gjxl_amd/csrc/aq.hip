@@ -172,56 +172,56 @@ __global__ __launch_bounds__(256) void AdjustFieldKernel(
     for (int dx = 0; dx < cw; ++dx) out[static_cast<size_t>(y + dy) * bw + x + dx] = result;
 }
 
-// Non-negative float maxima through their bit patterns (order-free).
+// Non-negative float maxima through their bit patterns (order-free), one
+// atomic per work group.
 __global__ __launch_bounds__(256) void MatrixStatsKernel(
   const float* __restrict__ opsin, int width, int height, int stride,
   size_t plane, uint32_t* __restrict__ stats) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const float* X = opsin;
+  const float* Y = opsin + plane;
+  const float* B = opsin + 2 * plane;
   float xe = 0.0f, be = 0.0f, ex = 0.0f;
-  if (x >= 1 && x < width && y >= 1) {
-    const float* X = opsin;
-    const float* Y = opsin + plane;
-    const float* B = opsin + 2 * plane;
+  const size_t n = static_cast<size_t>(width) * height;
+  for (size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+       k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(k % width), y = static_cast<int>(k / width);
+    if (x < 1 || y < 1) continue;
     const size_t i = static_cast<size_t>(y) * stride + x, l = i - 1, u = i - stride;
-    xe = DStdMax(fabsf(X[i] - X[l]), fabsf(X[i] - X[u]));
+    xe = DStdMax(xe, DStdMax(fabsf(X[i] - X[l]), fabsf(X[i] - X[u])));
     const float d = B[i] - Y[i];
-    be = DStdMax(fabsf(d - (B[l] - Y[l])), fabsf(d - (B[u] - Y[u])));
+    be = DStdMax(be, DStdMax(fabsf(d - (B[l] - Y[l])), fabsf(d - (B[u] - Y[u]))));
     float e = B[i] - 1.2f * Y[i];
     if (e >= 0.0f) {
       e *= fabsf(B[i] - B[l]) + fabsf(B[i] - B[u]);
-      ex = e;
+      ex = DStdMax(ex, e);
     }
   }
-  // wave-level max, then one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) {
-    xe = DStdMax(xe, __shfl_xor(xe, off));
-    be = DStdMax(be, __shfl_xor(be, off));
-    ex = DStdMax(ex, __shfl_xor(ex, off));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&stats[0], __float_as_uint(xe));
-    atomicMax(&stats[1], __float_as_uint(be));
-    atomicMax(&stats[2], __float_as_uint(ex));
+  uint32_t v[3] = {__float_as_uint(xe), __float_as_uint(be), __float_as_uint(ex)};
+  BlockReduceU32(v, U32Max());
+  if (threadIdx.x == 0) {
+    atomicMax(&stats[0], v[0]);
+    atomicMax(&stats[1], v[1]);
+    atomicMax(&stats[2], v[2]);
   }
 }
 
-// min / max of a positive field via bit patterns.
+// min / max of a positive field via bit patterns, one atomic pair per group.
 __global__ __launch_bounds__(256) void RangeKernel(const float* __restrict__ v,
                                                    size_t n,
                                                    uint32_t* __restrict__ range) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  uint32_t lo = 0xffffffffu, hi = 0;
-  if (i < n) {
-    lo = __float_as_uint(v[i]);
-    hi = lo;
+  uint32_t lo[1] = {0xffffffffu}, hi[1] = {0u};
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const uint32_t b = __float_as_uint(v[i]);
+    lo[0] = b < lo[0] ? b : lo[0];
+    hi[0] = b > hi[0] ? b : hi[0];
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), off)));
-    hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), off)));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&range[0], lo);
-    atomicMax(&range[1], hi);
+  BlockReduceU32(lo, U32Min());
+  __syncthreads();
+  BlockReduceU32(hi, U32Max());
+  if (threadIdx.x == 0) {
+    atomicMin(&range[0], lo[0]);
+    atomicMax(&range[1], hi[0]);
   }
 }
 
@@ -611,15 +611,20 @@ __global__ __launch_bounds__(kAnchorThreads) void CodeTransformKernel(
 // (x, y) is e(x, y-1) + e(x-1, y) because the update of (x-1, y) adds its
 // errors into the previous row's slot x; "north-west" is
 // e(x-1, y-1) + e(x-2, y); "north-east" e(x+1, y-1).  Every input of (x, y)
-// is therefore final at wavefront step t = x + 2y, and one work group walks
-// the wavefront of one DC group (<= 256 x 256 blocks) per channel.
+// is therefore final at wavefront step t = x + 2y.  One work group walks the
+// wavefront of one DC group (<= 256 x 256 blocks): thread y owns row y and
+// keeps its own last two samples in registers; rows y-1 and y-2 read the
+// samples they need (x-1, x, x+1 of row y-1, x of row y-2, produced 1 to 4
+// steps earlier) from an 8-slot ring in LDS that its owner writes each step.
 
 constexpr int kDcGroup = 256;
 constexpr int kDcThreads = 256;
+constexpr int kRing = 8;
 
-struct WpPlanes {
-  uint32_t* e[4];  // per-sample sub-predictor errors
-  int32_t* err;    // signed prediction error
+struct DcRing {
+  int32_t q[kRing][kDcGroup];       // quantized value
+  int32_t err[kRing][kDcGroup];     // signed prediction error
+  uint32_t e[4][kRing][kDcGroup];   // sub-predictor errors
 };
 
 __device__ __forceinline__ uint32_t Reciprocal(uint32_t i) {
@@ -630,19 +635,17 @@ __device__ __forceinline__ int BitWidth64(uint64_t v) {
   return v == 0 ? 0 : 64 - __clzll(static_cast<long long>(v));
 }
 
+// grid: (DC groups x, DC groups y, channels); channels are `first_channel`
+// for z = 0 and 2 for z = 1 (X and B both follow Y).
 __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
   const float* __restrict__ dc, int bw, int bh,
-  const QuantizerState* __restrict__ qstate, int channel, int extra_precision,
-  int32_t* __restrict__ quantized, uint32_t* __restrict__ scratch) {
+  const QuantizerState* __restrict__ qstate, int first_channel,
+  int extra_precision, int32_t* __restrict__ quantized) {
+  __shared__ DcRing ring;
+  const int channel = blockIdx.z == 0 ? first_channel : 2;
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
   const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
   const size_t n = static_cast<size_t>(bw) * bh;
-  const size_t group = static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x;
-  // scratch per group: 5 planes of 256*256
-  uint32_t* base = scratch + group * 5 * kDcGroup * kDcGroup;
-  WpPlanes p;
-  for (int i = 0; i < 4; ++i) p.e[i] = base + i * kDcGroup * kDcGroup;
-  p.err = reinterpret_cast<int32_t*>(base + 4 * kDcGroup * kDcGroup);
   const QuantizerState q = *qstate;
   const float precision = static_cast<float>(1u << extra_precision);
   const float inv = q.inverse_dc_steps[channel] * precision;
@@ -650,33 +653,34 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
   int32_t* plane = quantized + channel * n;
   constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
   constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
+  const int y = threadIdx.x;
+  // this row's samples x-1 and x-2
+  int32_t q1 = 0, err1 = 0;
+  uint32_t e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
   const int steps = w + 2 * h;
   for (int t = 0; t < steps; ++t) {
-    // samples on this wavefront: x = t - 2y
-    for (int y = threadIdx.x; y < h; y += kDcThreads) {
-      const int x = t - 2 * y;
-      if (x < 0 || x >= w) continue;
-      const size_t index = static_cast<size_t>(gy0 + y) * bw + gx0 + x;
-      const int64_t W_ = x ? plane[index - 1] : y ? plane[index - bw] : 0;
-      const int64_t N_ = y ? plane[index - bw] : W_;
-      const int64_t NW_ = x && y ? plane[index - bw - 1] : W_;
-      const int64_t NE_ = y && x + 1 < w ? plane[index - bw + 1] : N_;
-      const int64_t NN_ = y > 1 ? plane[index - 2 * bw] : N_;
-      auto E = [&](int i, int xx, int yy) -> uint32_t {
-        if (xx < 0 || yy < 0) return 0u;
-        return p.e[i][yy * kDcGroup + xx];
-      };
-      auto R = [&](int xx, int yy) -> int64_t {
-        if (xx < 0 || yy < 0) return 0;
-        return p.err[yy * kDcGroup + xx];
-      };
+    const int x = t - 2 * y;
+    if (y < h && x >= 0 && x < w) {
+      const int s0 = x & (kRing - 1);
+      const int sm = (x - 1) & (kRing - 1), sp = (x + 1) & (kRing - 1);
+      const bool has_ne = x + 1 < w;
+      // quantized neighbours
+      const int64_t up = y ? ring.q[s0][y - 1] : 0;
+      const int64_t W_ = x ? q1 : up;
+      const int64_t N_ = y ? up : W_;
+      const int64_t NW_ = x && y ? ring.q[sm][y - 1] : W_;
+      const int64_t NE_ = y && has_ne ? ring.q[sp][y - 1] : N_;
+      const int64_t NN_ = y > 1 ? ring.q[s0][y - 2] : N_;
       // Predict (prediction.h:75-123)
       uint32_t weights[4];
       uint32_t weight_sum = 0;
+#pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const uint32_t north = E(i, x, y - 1) + E(i, x - 1, y);
-        const uint32_t ne = x + 1 < w ? E(i, x + 1, y - 1) : north;
-        const uint32_t nw = x ? E(i, x - 1, y - 1) + E(i, x - 2, y) : north;
+        const uint32_t en_i = y ? ring.e[i][s0][y - 1] : 0u;
+        const uint32_t north = en_i + (x ? e1[i] : 0u);
+        const uint32_t ne = has_ne ? (y ? ring.e[i][sp][y - 1] : 0u) : north;
+        const uint32_t nw =
+          x ? (y ? ring.e[i][sm][y - 1] : 0u) + (x > 1 ? e2[i] : 0u) : north;
         const uint32_t e = north + ne + nw;
         int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
         if (shift < 0) shift = 0;
@@ -685,14 +689,15 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
       }
       const unsigned log_weight = BitWidth64(weight_sum) - 1;
       weight_sum = 0;
+#pragma unroll
       for (int i = 0; i < 4; ++i) {
         weights[i] >>= log_weight - 4;
         weight_sum += weights[i];
       }
-      const int64_t ew = x ? R(x - 1, y) : 0;
-      const int64_t en = R(x, y - 1);
-      const int64_t enw = x ? R(x - 1, y - 1) : en;
-      const int64_t ene = x + 1 < w ? R(x + 1, y - 1) : en;
+      const int64_t ew = x ? err1 : 0;
+      const int64_t en = y ? ring.err[s0][y - 1] : 0;
+      const int64_t enw = x ? (y ? ring.err[sm][y - 1] : 0) : en;
+      const int64_t ene = has_ne ? (y ? ring.err[sp][y - 1] : 0) : en;
       const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
       int64_t pred[4];
       pred[0] = w8 + ne8 - n8;
@@ -701,15 +706,19 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
       pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
                        (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
       int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
+#pragma unroll
       for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
       int64_t prediction = (sum * static_cast<int64_t>(Reciprocal(weight_sum - 1))) >> 24;
       if (((en ^ ew) | (en ^ enw)) <= 0) {
-        int64_t lo = w8 < ne8 ? w8 : ne8; lo = lo < n8 ? lo : n8;
-        int64_t hi = w8 > ne8 ? w8 : ne8; hi = hi > n8 ? hi : n8;
+        int64_t lo = w8 < ne8 ? w8 : ne8;
+        lo = lo < n8 ? lo : n8;
+        int64_t hi = w8 > ne8 ? w8 : ne8;
+        hi = hi > n8 ? hi : n8;
         prediction = prediction < lo ? lo : (prediction > hi ? hi : prediction);
       }
       const int64_t predicted = (prediction + 3) >> 3;
       // quantise the residual (dc_quantization.cpp:176-203)
+      const size_t index = static_cast<size_t>(gy0 + y) * bw + gx0 + x;
       float value = dc[channel * n + index];
       if (channel == 2) {
         value = __builtin_fmaf(-static_cast<float>(quantized[n + index]), step_y, value);
@@ -724,14 +733,21 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
       plane[index] = value_q;
       // Update (prediction.h:124-144)
       const int64_t scaled = static_cast<int64_t>(value_q) * 8;
-      p.err[y * kDcGroup + x] = static_cast<int32_t>(prediction - scaled);
+      const int32_t err = static_cast<int32_t>(prediction - scaled);
+      ring.q[s0][y] = value_q;
+      ring.err[s0][y] = err;
+#pragma unroll
       for (int i = 0; i < 4; ++i) {
         int64_t dd = pred[i] - scaled;
         if (dd < 0) dd = -dd;
-        p.e[i][y * kDcGroup + x] = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
+        const uint32_t ei = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
+        ring.e[i][s0][y] = ei;
+        e2[i] = e1[i];
+        e1[i] = ei;
       }
+      q1 = value_q;
+      err1 = err;
     }
-    __threadfence_block();
     __syncthreads();
   }
 }
@@ -940,7 +956,8 @@ GJXLResult LaunchMatrixStats(hipStream_t s, const float* opsin, int width,
                              int height, int padded_w, int padded_h,
                              uint32_t* stats) {
   hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, stats, 3, 0u);
-  hipLaunchKernelGGL(MatrixStatsKernel, dim3(CeilDiv(width, 256), height), dim3(256), 0,
+  hipLaunchKernelGGL(MatrixStatsKernel,
+                     dim3(ReductionGroups(static_cast<size_t>(width) * height)), dim3(256), 0,
                      s, opsin, width, height, padded_w,
                      static_cast<size_t>(padded_w) * padded_h, stats);
   return CheckLaunch("MatrixStatsKernel");
@@ -949,7 +966,7 @@ GJXLResult LaunchMatrixStats(hipStream_t s, const float* opsin, int width,
 GJXLResult LaunchRange(hipStream_t s, const float* v, size_t n, uint32_t* range) {
   hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, range, 1, 0xffffffffu);
   hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, range + 1, 1, 0u);
-  hipLaunchKernelGGL(RangeKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)), dim3(256),
+  hipLaunchKernelGGL(RangeKernel, dim3(ReductionGroups(n)), dim3(256),
                      0, s, v, n, range);
   return CheckLaunch("RangeKernel");
 }
@@ -996,8 +1013,7 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                                   const int8_t* y_to_x, const int8_t* y_to_b,
                                   const QuantizerState* q, float x_mult,
                                   float b_mult, int32_t* quantized, float* dc,
-                                  int32_t* dc_quantized, float* dc_rec,
-                                  uint32_t* dc_scratch) {
+                                  int32_t* dc_quantized, float* dc_rec) {
   const AqLayout l = MakeAqLayout(width, height);
   CodingParams p;
   p.x_mult = x_mult;
@@ -1010,21 +1026,17 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                      offsets, list, count, l.bw, raw, y_to_x, y_to_b, q,
                      d->quant_tables, p, quantized, dc, n);
   GJXL_AMD_TRY(CheckLaunch("CodeTransformKernel"));
-  const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup));
-  for (int channel : {1, 0, 2}) {
-    hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw,
-                       l.bh, q, channel, 1, dc_quantized, dc_scratch);
-    GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
-  }
+  const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 1);
+  hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
+                     q, 1, 1, dc_quantized);
+  GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
+  const dim3 both(groups.x, groups.y, 2);
+  hipLaunchKernelGGL(DcQuantKernel, both, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
+                     q, 0, 1, dc_quantized);
+  GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
   hipLaunchKernelGGL(DcReconstructKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)),
                      dim3(256), 0, d->stream, dc_quantized, n, q, 1, dc_rec);
   return CheckLaunch("DcReconstructKernel");
-}
-
-size_t DcScratchWords(int width, int height) {
-  const int bw = width / 8, bh = height / 8;
-  return static_cast<size_t>(CeilDiv(bw, kDcGroup)) * CeilDiv(bh, kDcGroup) * 5 *
-         kDcGroup * kDcGroup;
 }
 
 GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,

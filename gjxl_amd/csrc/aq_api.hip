@@ -157,7 +157,6 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_DC] = 4 * 3 * n;
   z[GJXL_AMD_AQ_DC_QUANTIZED] = 4 * 3 * n;
   z[GJXL_AMD_AQ_DC_RECONSTRUCTED] = 4 * 3 * n;
-  z[GJXL_AMD_AQ_DC_SCRATCH] = 4 * gjxl_amd::DcScratchWords(s->pw, s->ph);
   z[GJXL_AMD_AQ_DC_SMOOTHED] = 4 * 3 * n;
   z[GJXL_AMD_AQ_INVERSE_SIGMA] = 4 * n;
   z[GJXL_AMD_AQ_RECONSTRUCTED] = 4 * 3 * padded;
@@ -308,7 +307,7 @@ GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
     Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q,
     s->x_mult, s->b_mult, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED),
     Buf<float>(s, GJXL_AMD_AQ_DC), Buf<int32_t>(s, GJXL_AMD_AQ_DC_QUANTIZED),
-    Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED), Buf<uint32_t>(s, GJXL_AMD_AQ_DC_SCRATCH)));
+    Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED)));
   GJXL_AMD_TRY(LaunchReconstruct(
     d, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED), s->pw, s->ph, s->cells,
     Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),

@@ -228,9 +228,7 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                                   const int8_t* y_to_x, const int8_t* y_to_b,
                                   const QuantizerState* q, float x_mult,
                                   float b_mult, int32_t* quantized, float* dc,
-                                  int32_t* dc_quantized, float* dc_rec,
-                                  uint32_t* dc_scratch);
-size_t DcScratchWords(int width, int height);
+                                  int32_t* dc_quantized, float* dc_rec);
 GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
                              int width, int height, const uint8_t* cells,
                              const uint32_t* offsets, const uint32_t* list,

@@ -556,26 +556,27 @@ __global__ __launch_bounds__(kThreads) void ExpandKernel(
 
 // The requested-extent map: 0.85 main + 0.5 sub (two scales), the main map
 // cropped from the expanded extent, or the main map alone; and its maximum
-// (non-negative floats compare as their bit patterns).
+// (non-negative floats compare as their bit patterns), one atomic per group.
 __global__ __launch_bounds__(kThreads) void ComposeKernel(
   const float* __restrict__ main_map, int mw, const float* __restrict__ sub_map,
   int sw, int w, int h, int xb, int yb, float* __restrict__ map,
   float* __restrict__ map_out, uint32_t* __restrict__ score_bits) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  float v = 0.0f;
-  if (x < w) {
+  uint32_t best[1] = {0u};
+  const size_t n = static_cast<size_t>(w) * h;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(i % w), y = static_cast<int>(i / w);
     const float m = main_map[static_cast<size_t>(y + yb) * mw + x + xb];
-    v = sub_map != nullptr ? m * 0.85f + 0.5f * sub_map[static_cast<size_t>(y / 2) * sw + x / 2]
-                           : m;
-    map[static_cast<size_t>(y) * w + x] = v;
-    if (map_out != nullptr) map_out[static_cast<size_t>(y) * w + x] = v;
+    const float v = sub_map != nullptr
+                      ? m * 0.85f + 0.5f * sub_map[static_cast<size_t>(y / 2) * sw + x / 2]
+                      : m;
+    map[i] = v;
+    if (map_out != nullptr) map_out[i] = v;
+    const uint32_t bits = __float_as_uint(v);
+    best[0] = bits > best[0] ? bits : best[0];
   }
-  uint32_t bits = __float_as_uint(v);
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(bits), off));
-    bits = other > bits ? other : bits;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(score_bits, bits);
+  BlockReduceU32(best, U32Max());
+  if (threadIdx.x == 0) atomicMax(score_bits, best[0]);
 }
 
 __global__ void FillWordKernel(uint32_t* p, uint32_t value) {
@@ -797,7 +798,9 @@ GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
     sw = s.width;
   }
   hipLaunchKernelGGL(FillWordKernel, dim3(1), dim3(64), 0, st, b->score_bits, 0u);
-  hipLaunchKernelGGL(ComposeKernel, Grid2(b->width, b->height), dim3(kThreads), 0, st,
+  hipLaunchKernelGGL(ComposeKernel,
+                     dim3(ReductionGroups(static_cast<size_t>(b->width) * b->height)),
+                     dim3(kThreads), 0, st,
                      m.stages + 8 * m.n, m.width, sub_map, sw, b->width, b->height,
                      b->xborder, b->yborder, b->map, map_out, b->score_bits);
   return CheckLaunch("ComposeKernel");

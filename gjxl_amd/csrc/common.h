@@ -114,4 +114,41 @@ __device__ __forceinline__ float FastPow(float base, float exponent) {
   return FastPow2(FastLog2(base) * exponent);
 }
 
+// Work-group reduction of K words with `op` (max or min of unsigned bit
+// patterns); the result is valid in thread 0.  blockDim.x must be a multiple
+// of 64 and at most 1024; every thread of the group must call it.
+template <int K, typename Op>
+__device__ inline void BlockReduceU32(uint32_t (&v)[K], Op op) {
+  __shared__ uint32_t partial[16][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    for (int off = 32; off > 0; off >>= 1) v[k] = op(v[k], __shfl_xor(v[k], off));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[wave][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = op(v[k], partial[w][k]);
+    }
+  }
+}
+
+struct U32Max {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a < b ? b : a; }
+};
+struct U32Min {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return b < a ? b : a; }
+};
+
+// Work groups for a grid-stride reduction over n items.
+inline uint32_t ReductionGroups(size_t n, uint32_t threads = 256) {
+  const size_t groups = (n + threads - 1) / threads;
+  return static_cast<uint32_t>(groups < 1024 ? (groups < 1 ? 1 : groups) : 1024);
+}
+
 }  // namespace gjxl_amd

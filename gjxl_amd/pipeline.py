@@ -2,14 +2,18 @@
 
 Mirrors the order of RunPreparedQuantizationPipelineWithProviders
 (src/codec/quantization_pipeline.cpp:428-586) and PrepareWorkflow
-(src/codestream/workflow.cpp:636-768) for the stages built so far.  Every
-stage is a HIP kernel behind the C ABI; this module only allocates and calls.
+(src/codestream/workflow.cpp:636-768): XYB, Gaborish inverse, initial quant
+field, initial CfL, AC strategy search, then the Butteraugli AQ loop
+(src/codec/adaptive_quantization.cpp:1600-1738).  Every stage is a HIP
+kernel behind the C ABI; this module only allocates and calls.
 """
 from __future__ import annotations
 
+from typing import Callable
+
 import numpy as np
 
-from . import Device, DeviceArray, pad8  # noqa: F401
+from . import AqSession, Butteraugli, Device, DeviceArray, library, pad8
 
 
 def run_frontend(dev: Device, srgb: np.ndarray, intensity_target: float = 255.0,
@@ -27,3 +31,65 @@ def run_frontend(dev: Device, srgb: np.ndarray, intensity_target: float = 255.0,
                                   intensity_target=intensity_target,
                                   linear=out["linear"], opsin=out["opsin"])
     return out
+
+
+class PlainPath:
+    """Device buffers for one image extent and the plain path over them.
+
+    run() takes the packed sRGB8 pixels already on the device and leaves the
+    strategy cells, the final quant field, the block distances and the last
+    evaluation's frame in the session (d1.0 e7: two AQ updates, three
+    evaluations)."""
+
+    def __init__(self, dev: Device, width: int, height: int,
+                 target: float = 1.0, iterations: int = 2):
+        self.dev = dev
+        self.width, self.height = width, height
+        self.target, self.iterations = target, iterations
+        ph, pw = pad8(height), pad8(width)
+        self.linear = dev.empty((3, height, width))
+        self.opsin = dev.empty((3, ph, pw))
+        self.preprocessed = dev.empty((3, ph, pw))
+        self.aq: dict = {}
+        self.cfl: dict = {}
+        self.costs = dev.empty((7, ph // 8, pw // 8))
+        self.cells = dev.empty((ph // 8, pw // 8), np.uint8)
+        nbytes = library().gjxl_amd_initial_quant_field_scratch_bytes(pw, ph)
+        self.aq_scratch = dev.empty((nbytes,), np.uint8)
+        self.session = AqSession(dev, width, height)
+        self.butteraugli = Butteraugli(dev, width, height)
+
+    def close(self) -> None:
+        self.session.close()
+        self.butteraugli.close()
+
+    def run(self, srgb: DeviceArray, bytes_per_pixel: int = 3,
+            mark: Callable[[str], None] | None = None) -> None:
+        dev, w, h, t = self.dev, self.width, self.height, self.target
+        mark = mark or (lambda name: None)
+        dev.srgb8_to_linear_and_opsin(srgb, w, h, bytes_per_pixel=bytes_per_pixel,
+                                      linear=self.linear, opsin=self.opsin)
+        mark("srgb_to_linear_and_opsin")
+        dev.gaborish_inverse(self.opsin, out=self.preprocessed)
+        mark("gaborish_inverse")
+        dev.initial_quant_field(self.opsin, t, 1.0, out=self.aq, scratch=self.aq_scratch)
+        mark("initial_quant_field")
+        dev.initial_cfl(self.preprocessed, out=self.cfl)
+        mark("initial_cfl")
+        dev.ac_strategy_costs(self.preprocessed, self.aq["quant_field"],
+                              self.aq["pixel_mask"], self.cfl["y_to_x"],
+                              self.cfl["y_to_b"], t, costs=self.costs)
+        mark("ac_strategy_costs")
+        dev.ac_strategy_search(self.costs, t, cells=self.cells)
+        mark("ac_strategy_search")
+        self.session.prepare(self.opsin, self.preprocessed, self.cells,
+                             self.aq["quant_field"], t)
+        mark("aq_prepare")
+        self.butteraugli.prepare(self.linear)
+        mark("butteraugli_prepare")
+        self.session.run(self.butteraugli, self.iterations, t, scores=False)
+        mark("aq_loop")
+
+    def scores(self) -> list[float]:
+        words = self.session.buffer("scores", (self.iterations + 1,), np.uint32).numpy()
+        return [float(v) for v in words.view(np.float32)]

@@ -189,7 +189,6 @@ enum {
   GJXL_AMD_AQ_DC,                   /* f32 3 block planes: DC before coding */
   GJXL_AMD_AQ_DC_QUANTIZED,         /* i32 3 block planes */
   GJXL_AMD_AQ_DC_RECONSTRUCTED,     /* f32 3 block planes */
-  GJXL_AMD_AQ_DC_SCRATCH,           /* weighted-predictor state */
   GJXL_AMD_AQ_DC_SMOOTHED,          /* f32 3 block planes */
   GJXL_AMD_AQ_INVERSE_SIGMA,        /* f32 per block: EPF inverse sigma */
   GJXL_AMD_AQ_RECONSTRUCTED,        /* f32 3 planes pw*ph: XYB */
