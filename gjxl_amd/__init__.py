@@ -314,7 +314,7 @@ AQ_BUFFERS = (
     "stats", "select", "quantizer", "raw_quant", "y_to_x", "y_to_b",
     "quantized", "dc", "dc_quantized", "dc_reconstructed",
     "dc_smoothed", "inverse_sigma", "reconstructed", "scratch_a", "scratch_b",
-    "filtered", "linear", "field", "block_distance", "scores",
+    "filtered", "linear", "field", "block_distance", "scores", "strategy_lists",
 )
 
 

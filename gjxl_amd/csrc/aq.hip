@@ -48,7 +48,9 @@ __device__ __forceinline__ bool SearchedStrategy(int st) {
 __global__ __launch_bounds__(64) void AnchorLayoutKernel(
   const uint8_t* __restrict__ cells, int bw, int bh,
   uint32_t* __restrict__ offsets, uint32_t* __restrict__ anchor_list,
-  uint32_t* __restrict__ anchor_count, uint32_t* __restrict__ error) {
+  uint32_t* __restrict__ anchor_count, uint32_t* __restrict__ error,
+  uint32_t* __restrict__ strategy_lists, size_t capacity,
+  uint32_t* __restrict__ strategy_counts) {
   __shared__ uint32_t covered[kGroupBlocks];  // one bit per cell of the group
   const int gw = (bw + kGroupBlocks - 1) / kGroupBlocks;
   const int g = blockIdx.x;
@@ -91,58 +93,209 @@ __global__ __launch_bounds__(64) void AnchorLayoutKernel(
     atomicOr(error, 1u);
   }
   anchor_count[g] = count;
+  // the group's anchors appended to the per-strategy lists (one reservation
+  // per strategy and group)
+  uint32_t per[kStrategyStages] = {};
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
+    const uint8_t c = cells[static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff)];
+    ++per[StrategyStage(c >> 1)];
+  }
+  uint32_t base[kStrategyStages];
+  for (int st = 0; st < kStrategyStages; ++st) {
+    base[st] = per[st] ? atomicAdd(&strategy_counts[st], per[st]) : 0u;
+  }
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
+    const uint8_t c = cells[static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff)];
+    const int st = StrategyStage(c >> 1);
+    strategy_lists[static_cast<size_t>(st) * capacity + base[st]++] = packed;
+  }
 }
 
-struct AnchorRef {
-  int bx, by, strategy, cw, ch, W, H, N;
-  uint32_t offset;
+// ---- per-strategy transform kernels -----------------------------------------
+//
+// Each launch handles the transforms of one strategy (from its list), kK of
+// them per work group, with compile-time extents.  Their separable binary64
+// DCT passes give each thread kUnroll outputs with independent chains, each
+// summed in the reference's order (src/codec/dct.cpp:135-233), with the
+// bases in LDS.
+
+constexpr int kAqThreads = 256;
+constexpr int kSlots = 3072;
+constexpr int kUnroll = 4;
+
+template <int kStrategy>
+struct AnchorShape {
+  static constexpr int kCw = CoveredW(kStrategy), kCh = CoveredH(kStrategy);
+  static constexpr int kW = kCw * 8, kH = kCh * 8, kN = kW * kH;
+  static constexpr int kK = kSlots / (3 * kN);
+  static constexpr int kBasis = kW == kH ? kW * kW : kW * kW + kH * kH;
 };
 
-__device__ __forceinline__ bool LoadAnchor(const uint32_t* list,
-                                           const uint32_t* count,
-                                           const uint8_t* cells,
-                                           const uint32_t* offsets, int bw,
-                                           AnchorRef* a) {
-  const int g = blockIdx.y, k = blockIdx.x;
-  if (static_cast<uint32_t>(k) >= count[g]) return false;
-  const uint32_t packed = list[static_cast<size_t>(g) * kGroupAnchors + k];
-  a->bx = packed & 0xffff;
-  a->by = packed >> 16;
-  a->strategy = cells[static_cast<size_t>(a->by) * bw + a->bx] >> 1;
-  a->cw = CoveredW(a->strategy);
-  a->ch = CoveredH(a->strategy);
-  a->W = a->cw * 8;
-  a->H = a->ch * 8;
-  a->N = a->W * a->H;
-  a->offset = offsets[static_cast<size_t>(a->by) * bw + a->bx];
-  return true;
+struct TransformSlot {
+  int bx, by;
+  uint32_t offset;
+  int ok;
+};
+
+// Thread k < kK decodes this group's k-th list entry.
+template <int kK>
+__device__ __forceinline__ void LoadSlots(const uint32_t* __restrict__ entries,
+                                          uint32_t count,
+                                          const uint32_t* __restrict__ offsets, int bw,
+                                          TransformSlot* slots) {
+  if (threadIdx.x < kK) {
+    const uint32_t idx = blockIdx.x * kK + threadIdx.x;
+    TransformSlot t{0, 0, 0u, 0};
+    if (idx < count) {
+      const uint32_t packed = entries[idx];
+      t.bx = packed & 0xffff;
+      t.by = packed >> 16;
+      t.offset = offsets[static_cast<size_t>(t.by) * bw + t.bx];
+      t.ok = 1;
+    }
+    slots[threadIdx.x] = t;
+  }
 }
 
-constexpr int kAnchorThreads = 64;
+// Loads the W and H bases of one direction into LDS (shared when square).
+template <int W, int H>
+__device__ __forceinline__ void LoadBases(const double* __restrict__ bases, bool forward,
+                                          double* lds, const double** bw_out,
+                                          const double** bh_out) {
+  for (int i = threadIdx.x; i < W * W; i += blockDim.x) lds[i] = bases[BasisOffset(W, forward) + i];
+  if (W != H) {
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
+      lds[W * W + i] = bases[BasisOffset(H, forward) + i];
+    }
+  }
+  *bw_out = lds;
+  *bh_out = W == H ? lds : lds + W * W;
+}
 
-// Forward DCT of the three channels of every anchor into the coefficient
-// store (3 planes of groups * 65536 floats).
-__global__ __launch_bounds__(kAnchorThreads) void AnchorDctKernel(
+// Forward rows: dst[p][y][u] = sum_x src[p][y][x] * fh[u][x].
+template <int W, int H, int kPlanes>
+__device__ __forceinline__ void ForwardRows(const double* fh, const float* src, double* dst) {
+  constexpr int N = W * H;
+  constexpr int kTasks = kPlanes * H * (W / kUnroll);
+  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
+    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
+    const float* s = src + (row / H) * N + (row % H) * W;
+    double acc[kUnroll] = {};
+#pragma unroll 8
+    for (int x = 0; x < W; ++x) {
+      const double v = static_cast<double>(s[x]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += v * fh[(g * kUnroll + j) * W + x];
+    }
+    double* d = dst + (row / H) * N + (row % H) * W + g * kUnroll;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) d[j] = acc[j];
+  }
+}
+
+// Forward columns: coefficient (v, u) of plane p = sum_y fv[v][y] *
+// src[p][y][u], handed to store(p, libjxl index, value).
+template <int W, int H, int kPlanes, typename Store>
+__device__ __forceinline__ void ForwardCols(const double* fv, const double* src, Store store) {
+  constexpr int N = W * H;
+  constexpr int kTasks = kPlanes * W * (H / kUnroll);
+  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
+    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+    const int p = col / W, u = col % W;
+    const double* s = src + p * N + u;
+    double acc[kUnroll] = {};
+#pragma unroll 8
+    for (int y = 0; y < H; ++y) {
+      const double v = s[y * W];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += fv[(g * kUnroll + j) * H + y] * v;
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      store(p, CoefIndex(W, H, g * kUnroll + j, u), static_cast<float>(acc[j]));
+    }
+  }
+}
+
+// Inverse rows: dst[p][v][x] = sum_u src[p][idx(v, u)] * ih[u][x].
+template <int W, int H, int kPlanes>
+__device__ __forceinline__ void InverseRows(const double* ih, const float* src, double* dst) {
+  constexpr int N = W * H;
+  constexpr int kTasks = kPlanes * H * (W / kUnroll);
+  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
+    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
+    const int p = row / H, v = row % H;
+    const float* s = src + p * N;
+    double acc[kUnroll] = {};
+#pragma unroll 8
+    for (int u = 0; u < W; ++u) {
+      const double c = static_cast<double>(s[CoefIndex(W, H, v, u)]);
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += c * ih[u * W + g * kUnroll + j];
+    }
+    double* d = dst + p * N + v * W + g * kUnroll;
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) d[j] = acc[j];
+  }
+}
+
+// Inverse columns: pixel (y, x) of plane p = sum_v iv[v][y] * src[p][v][x],
+// handed to store(p, y, x, value).
+template <int W, int H, int kPlanes, typename Store>
+__device__ __forceinline__ void InverseCols(const double* iv, const double* src, Store store) {
+  constexpr int N = W * H;
+  constexpr int kTasks = kPlanes * W * (H / kUnroll);
+  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
+    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+    const int p = col / W, x = col % W;
+    const double* s = src + p * N + x;
+    double acc[kUnroll] = {};
+#pragma unroll 8
+    for (int v = 0; v < H; ++v) {
+      const double t = s[v * W];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) acc[j] += iv[v * H + g * kUnroll + j] * t;
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) store(p, g * kUnroll + j, x, static_cast<float>(acc[j]));
+  }
+}
+
+// Forward DCT of the three channels of every transform of one strategy into
+// the coefficient store (3 planes of groups * 65536 floats).
+template <int kStrategy>
+__global__ __launch_bounds__(kAqThreads) void AnchorDctKernel(
   const float* __restrict__ opsin, int width, int height,
-  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
-  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-  const double* __restrict__ bases, size_t store, float* __restrict__ coefs) {
-  __shared__ float pix[3 * 1024];
-  __shared__ double tmp[3 * 1024];
-  __shared__ float out[3 * 1024];
-  AnchorRef a;
-  if (!LoadAnchor(list, count, cells, offsets, width / 8, &a)) return;
+  const uint32_t* __restrict__ entries, uint32_t count,
+  const uint32_t* __restrict__ offsets, const double* __restrict__ bases,
+  size_t store, float* __restrict__ coefs) {
+  using S = AnchorShape<kStrategy>;
+  constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
+  __shared__ float pix[K * 3 * N];
+  __shared__ double tmp[K * 3 * N];
+  __shared__ double basis[S::kBasis];
+  __shared__ TransformSlot slots[K];
+  const double *fh, *fv;
+  LoadBases<W, H>(bases, true, basis, &fh, &fv);
+  LoadSlots<K>(entries, count, offsets, width / 8, slots);
+  __syncthreads();
   const size_t plane = static_cast<size_t>(width) * height;
-  for (int e = threadIdx.x; e < 3 * a.N; e += kAnchorThreads) {
-    const int c = e / a.N, r = e % a.N, y = r / a.W, x = r % a.W;
-    pix[e] = opsin[c * plane + static_cast<size_t>(a.by * 8 + y) * width + a.bx * 8 + x];
+  for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const TransformSlot& t = slots[k];
+    pix[e] = t.ok ? opsin[c * plane + static_cast<size_t>(t.by * 8 + r / W) * width +
+                          t.bx * 8 + r % W]
+                  : 0.0f;
   }
   __syncthreads();
-  ForwardDctLds(bases, a.W, a.H, 3, pix, tmp, out);
-  for (int e = threadIdx.x; e < 3 * a.N; e += kAnchorThreads) {
-    const int c = e / a.N, i = e % a.N;
-    coefs[c * store + a.offset + i] = out[e];
-  }
+  ForwardRows<W, H, 3 * K>(fh, pix, tmp);
+  __syncthreads();
+  ForwardCols<W, H, 3 * K>(fv, tmp, [&](int p, int i, float v) {
+    const TransformSlot& t = slots[p / 3];
+    if (t.ok) coefs[(p % 3) * store + t.offset + i] = v;
+  });
 }
 
 // AdjustQuantField (src/codec/adaptive_quantization.cpp:740-835), one
@@ -500,106 +653,129 @@ __device__ void AdjustQuantForChannel(float qscale, int channel, float mult,
   *raw_quant = adjusted;
 }
 
-__global__ __launch_bounds__(kAnchorThreads) void CodeTransformKernel(
-  const float* __restrict__ coefs, size_t store,
-  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
-  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-  int bw, int32_t* __restrict__ raw, const int8_t* __restrict__ y_to_x,
+// One launch per strategy: kK transforms per work group.  Three threads per
+// transform run AdjustQuantForChannel, one thread per transform decides the
+// shared raw quant and converts the DC values, and all threads quantise.
+template <int kStrategy>
+__global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
+  const float* __restrict__ coefs, size_t store, const uint32_t* __restrict__ entries,
+  uint32_t count, const uint32_t* __restrict__ offsets, int bw,
+  int32_t* __restrict__ raw, const int8_t* __restrict__ y_to_x,
   const int8_t* __restrict__ y_to_b, const QuantizerState* __restrict__ qstate,
   const float* __restrict__ tables, CodingParams params,
   int32_t* __restrict__ quantized, float* __restrict__ dc, size_t dc_plane) {
-  __shared__ float coef[3 * 1024];
-  __shared__ float thr_y[4];
-  __shared__ int32_t cand[3];
-  __shared__ int32_t decided;
-  AnchorRef a;
-  if (!LoadAnchor(list, count, cells, offsets, bw, &a)) return;
+  using S = AnchorShape<kStrategy>;
+  constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
+  constexpr int cols = W > H ? W : H, rows = W > H ? H : W;
+  __shared__ float coef[K][3][N];
+  __shared__ float thr_y[K][4];
+  __shared__ int32_t cand[K][3];
+  __shared__ int32_t decided[K];
+  __shared__ float cfl[K][2];
+  __shared__ TransformSlot slots[K];
+  LoadSlots<K>(entries, count, offsets, bw, slots);
+  __syncthreads();
   const QuantizerState q = *qstate;
-  const int N = a.N;
-  const int cols = a.W > a.H ? a.W : a.H, rows = a.W > a.H ? a.H : a.W;
-  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
-    coef[e] = coefs[(e / N) * store + a.offset + e % N];
-  }
-  __syncthreads();
   const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
-  const int32_t initial = raw[static_cast<size_t>(a.by) * bw + a.bx];
-  if (threadIdx.x < 3) {
+  for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+    const TransformSlot& t = slots[k];
+    coef[k][c][i] = t.ok ? coefs[c * store + t.offset + i] : 0.0f;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * K) {
+    const int k = threadIdx.x / 3, c = threadIdx.x % 3;
+    const TransformSlot& t = slots[k];
+    if (t.ok) {
+      float thr[4] = {0.58f, 0.64f, 0.64f, 0.64f};
+      int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
+      AdjustQuantForChannel(q.scale, c, mults[c], kStrategy, cols, rows,
+                            tables + InverseOffset(kStrategy, c), coef[k][c], thr, &r);
+      cand[k][c] = r;
+      if (c == 1) {
+        for (int i = 0; i < 4; ++i) thr_y[k][i] = thr[i];
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
     const int k = threadIdx.x;
-    float thr[4] = {0.58f, 0.64f, 0.64f, 0.64f};
-    int32_t c = initial;
-    AdjustQuantForChannel(q.scale, k, mults[k], a.strategy, cols, rows,
-                          tables + InverseOffset(a.strategy, k), coef + k * N,
-                          thr, &c);
-    cand[k] = c;
-    if (k == 1) for (int i = 0; i < 4; ++i) thr_y[i] = thr[i];
+    const TransformSlot& t = slots[k];
+    if (t.ok) {
+      // max over Y, X, B (starting from zero) as SelectAdjustedAcQuantization
+      int32_t d = 0;
+      const int order[3] = {1, 0, 2};
+      for (int i = 0; i < 3; ++i) d = cand[k][order[i]] > d ? cand[k][order[i]] : d;
+      decided[k] = d;
+      raw[static_cast<size_t>(t.by) * bw + t.bx] = d;
+      const int tile = (t.by / 8) * ((bw + 7) / 8) + t.bx / 8;
+      cfl[k][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
+      cfl[k][1] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+      // Y DC from the unmodified Y coefficients
+      float dcv[16];
+      LlfToDc(params.dc_basis, S::kCw, S::kCh, coef[k][1], dcv);
+      for (int dy = 0; dy < S::kCh; ++dy)
+        for (int dx = 0; dx < S::kCw; ++dx)
+          dc[dc_plane + static_cast<size_t>(t.by + dy) * bw + t.bx + dx] = dcv[dy * S::kCw + dx];
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // max over Y, X, B (starting from zero) as SelectAdjustedAcQuantization
-    int32_t d = 0;
-    const int order[3] = {1, 0, 2};
-    for (int i = 0; i < 3; ++i) d = cand[order[i]] > d ? cand[order[i]] : d;
-    decided = d;
-    raw[static_cast<size_t>(a.by) * bw + a.bx] = d;
-    // Y DC from the unmodified Y coefficients
-    float dcv[16];
-    LlfToDc(params.dc_basis, a.cw, a.ch, coef + N, dcv);
-    for (int dy = 0; dy < a.ch; ++dy)
-      for (int dx = 0; dx < a.cw; ++dx)
-        dc[dc_plane + static_cast<size_t>(a.by + dy) * bw + a.bx + dx] = dcv[dy * a.cw + dx];
-  }
-  __syncthreads();
-  const int32_t d = decided;
   // Y: quantise with the adjusted thresholds, store, dequantise in place
   {
-    const float* inv = tables + InverseOffset(a.strategy, 1);
-    const float* dq = tables + DequantOffset(a.strategy, 1);
-    const float qscale = q.scale * static_cast<float>(d) * 1.0f;
-    const float dscale = q.inverse_global_scale / static_cast<float>(d) / 1.0f;
-    for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
+    const float* inv = tables + InverseOffset(kStrategy, 1);
+    const float* dq = tables + DequantOffset(kStrategy, 1);
+    for (int e = threadIdx.x; e < K * N; e += kAqThreads) {
+      const int k = e / N, i = e % N;
+      const TransformSlot& t = slots[k];
+      if (!t.ok) continue;
+      const int32_t d = decided[k];
+      const float qscale = q.scale * static_cast<float>(d) * 1.0f;
+      const float dscale = q.inverse_global_scale / static_cast<float>(d) / 1.0f;
       const int x = i % cols, y = i / cols;
-      const int t = (y >= rows / 2) * 2 + (x >= cols / 2);
-      const float value = inv[i] * qscale * coef[N + i];
+      const int tq = (y >= rows / 2) * 2 + (x >= cols / 2);
+      const float value = inv[i] * qscale * coef[k][1][i];
       int32_t qv = 0;
-      if (!(fabsf(value) < thr_y[t])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
-      quantized[store + a.offset + i] = qv;
-      coef[N + i] = QuantBias(qv, 1) * (dq[i] * dscale);
+      if (!(fabsf(value) < thr_y[k][tq])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
+      quantized[store + t.offset + i] = qv;
+      coef[k][1][i] = QuantBias(qv, 1) * (dq[i] * dscale);
     }
   }
   __syncthreads();
-  const int tile = (a.by / 8) * ((bw + 7) / 8) + a.bx / 8;
-  const float fx = static_cast<float>(y_to_x[tile]) / 84.0f;
-  const float fb = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
-  for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
-    coef[i] -= fx * coef[N + i];
-    coef[2 * N + i] -= fb * coef[N + i];
+  for (int e = threadIdx.x; e < K * N; e += kAqThreads) {
+    const int k = e / N, i = e % N;
+    coef[k][0][i] -= cfl[k][0] * coef[k][1][i];
+    coef[k][2][i] -= cfl[k][1] * coef[k][1][i];
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
-    const int k = threadIdx.x * 2;
-    float dcv[16];
-    LlfToDc(params.dc_basis, a.cw, a.ch, coef + k * N, dcv);
-    for (int dy = 0; dy < a.ch; ++dy)
-      for (int dx = 0; dx < a.cw; ++dx)
-        dc[k * dc_plane + static_cast<size_t>(a.by + dy) * bw + a.bx + dx] =
-          dcv[dy * a.cw + dx];
+  if (threadIdx.x < 2 * K) {
+    const int k = threadIdx.x / 2, c = (threadIdx.x % 2) * 2;
+    const TransformSlot& t = slots[k];
+    if (t.ok) {
+      float dcv[16];
+      LlfToDc(params.dc_basis, S::kCw, S::kCh, coef[k][c], dcv);
+      for (int dy = 0; dy < S::kCh; ++dy)
+        for (int dx = 0; dx < S::kCw; ++dx)
+          dc[c * dc_plane + static_cast<size_t>(t.by + dy) * bw + t.bx + dx] =
+            dcv[dy * S::kCw + dx];
+    }
   }
-  const int covered = a.cw * a.ch;
-  for (int k = 0; k < 3; k += 2) {
+  constexpr int covered = S::kCw * S::kCh;
+  for (int e = threadIdx.x; e < K * 2 * N; e += kAqThreads) {
+    const int k = e / (2 * N), c = ((e / N) % 2) * 2, i = e % N;
+    const TransformSlot& t = slots[k];
+    if (!t.ok) continue;
     float thr[4] = {0.58f, 0.62f, 0.62f, 0.62f};
     if (covered >= 4)
-      for (int t = 0; t < 4; ++t)
-        thr[t] = DStdMax(0.5f, thr[t] - 0.00744f * static_cast<float>(covered));
-    const float* inv = tables + InverseOffset(a.strategy, k);
-    const float qscale = q.scale * static_cast<float>(d) * mults[k];
-    for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
-      const int x = i % cols, y = i / cols;
-      const int t = (y >= rows / 2) * 2 + (x >= cols / 2);
-      const float value = inv[i] * qscale * coef[k * N + i];
-      int32_t qv = 0;
-      if (!(fabsf(value) < thr[t])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
-      quantized[k * store + a.offset + i] = qv;
-    }
+      for (int j = 0; j < 4; ++j)
+        thr[j] = DStdMax(0.5f, thr[j] - 0.00744f * static_cast<float>(covered));
+    const float* inv = tables + InverseOffset(kStrategy, c);
+    const float qscale = q.scale * static_cast<float>(decided[k]) * mults[c];
+    const int x = i % cols, y = i / cols;
+    const int tq = (y >= rows / 2) * 2 + (x >= cols / 2);
+    const float value = inv[i] * qscale * coef[k][c][i];
+    int32_t qv = 0;
+    if (!(fabsf(value) < thr[tq])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
+    quantized[c * store + t.offset + i] = qv;
   }
 }
 
@@ -822,60 +998,83 @@ __global__ __launch_bounds__(256) void SmoothDcKernel(
 }
 
 // ReconstructQuantizedCoefficients (src/codec/reconstruction.cpp:790-936),
-// one wave per transform: dequantise, add CfL, restore the LLF from the
-// smoothed DC, inverse DCT into the padded XYB image.
-__global__ __launch_bounds__(kAnchorThreads) void ReconstructKernel(
+// one launch per strategy: dequantise, add CfL, restore the LLF from the
+// smoothed DC, inverse DCT straight into the padded XYB image.
+template <int kStrategy>
+__global__ __launch_bounds__(kAqThreads) void ReconstructKernel(
   const int32_t* __restrict__ quantized, size_t store,
-  const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
-  const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-  int width, int height, const int32_t* __restrict__ raw,
-  const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
-  const QuantizerState* __restrict__ qstate, const float* __restrict__ tables,
-  const double* __restrict__ bases, CodingParams params,
-  const float* __restrict__ smoothed_dc, float* __restrict__ out) {
-  __shared__ float coef[3 * 1024];
-  __shared__ double tmp[3 * 1024];
-  __shared__ float pix[3 * 1024];
+  const uint32_t* __restrict__ entries, uint32_t count,
+  const uint32_t* __restrict__ offsets, int width, int height,
+  const int32_t* __restrict__ raw, const int8_t* __restrict__ y_to_x,
+  const int8_t* __restrict__ y_to_b, const QuantizerState* __restrict__ qstate,
+  const float* __restrict__ tables, const double* __restrict__ bases,
+  CodingParams params, const float* __restrict__ smoothed_dc, float* __restrict__ out) {
+  using S = AnchorShape<kStrategy>;
+  constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
+  __shared__ float coef[K * 3 * N];
+  __shared__ double tmp[K * 3 * N];
+  __shared__ double basis[S::kBasis];
+  __shared__ TransformSlot slots[K];
+  __shared__ float cfl[K][2];
+  __shared__ float scale[K][3];
   const int bw = width / 8, bh = height / 8;
-  AnchorRef a;
-  if (!LoadAnchor(list, count, cells, offsets, bw, &a)) return;
+  const double *ih, *iv;
+  LoadBases<W, H>(bases, false, basis, &ih, &iv);
+  LoadSlots<K>(entries, count, offsets, bw, slots);
+  __syncthreads();
   const QuantizerState q = *qstate;
-  const int N = a.N;
-  const int32_t r = raw[static_cast<size_t>(a.by) * bw + a.bx];
-  const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
-  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
-    const int k = e / N, i = e % N;
-    const float* dq = tables + DequantOffset(a.strategy, k);
-    const float scale = q.inverse_global_scale / static_cast<float>(r) / mults[k];
-    const int32_t qv = quantized[k * store + a.offset + i];
-    coef[e] = QuantBias(qv, k) * (dq[i] * scale);
-  }
-  __syncthreads();
-  const int tile = (a.by / 8) * ((bw + 7) / 8) + a.bx / 8;
-  const float fx = static_cast<float>(y_to_x[tile]) / 84.0f;
-  const float fb = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
-  for (int i = threadIdx.x; i < N; i += kAnchorThreads) {
-    coef[i] += fx * coef[N + i];
-    coef[2 * N + i] += fb * coef[N + i];
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < K) {
     const int k = threadIdx.x;
-    const size_t n = static_cast<size_t>(bw) * bh;
-    float dcv[16];
-    for (int dy = 0; dy < a.ch; ++dy)
-      for (int dx = 0; dx < a.cw; ++dx)
-        dcv[dy * a.cw + dx] =
-          smoothed_dc[k * n + static_cast<size_t>(a.by + dy) * bw + a.bx + dx];
-    DcToLlf(params.dc_basis, a.cw, a.ch, dcv, coef + k * N);
+    const TransformSlot& t = slots[k];
+    if (t.ok) {
+      const int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
+      const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
+      for (int c = 0; c < 3; ++c) scale[k][c] = q.inverse_global_scale / static_cast<float>(r) / mults[c];
+      const int tile = (t.by / 8) * ((bw + 7) / 8) + t.bx / 8;
+      cfl[k][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
+      cfl[k][1] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+    }
   }
   __syncthreads();
-  InverseDctLds(bases, a.W, a.H, 3, coef, tmp, pix);
-  const size_t plane = static_cast<size_t>(width) * height;
-  for (int e = threadIdx.x; e < 3 * N; e += kAnchorThreads) {
-    const int k = e / N, rr = e % N, y = rr / a.W, x = rr % a.W;
-    out[k * plane + static_cast<size_t>(a.by * 8 + y) * width + a.bx * 8 + x] = pix[e];
+  for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
+    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+    const TransformSlot& t = slots[k];
+    float v = 0.0f;
+    if (t.ok) {
+      const float* dq = tables + DequantOffset(kStrategy, c);
+      v = QuantBias(quantized[c * store + t.offset + i], c) * (dq[i] * scale[k][c]);
+    }
+    coef[e] = v;
   }
+  __syncthreads();
+  for (int e = threadIdx.x; e < K * N; e += kAqThreads) {
+    const int k = e / N, i = e % N;
+    float* ck = coef + k * 3 * N;
+    ck[i] += cfl[k][0] * ck[N + i];
+    ck[2 * N + i] += cfl[k][1] * ck[N + i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * K) {
+    const int k = threadIdx.x / 3, c = threadIdx.x % 3;
+    const TransformSlot& t = slots[k];
+    if (t.ok) {
+      const size_t n = static_cast<size_t>(bw) * bh;
+      float dcv[16];
+      for (int dy = 0; dy < S::kCh; ++dy)
+        for (int dx = 0; dx < S::kCw; ++dx)
+          dcv[dy * S::kCw + dx] =
+            smoothed_dc[c * n + static_cast<size_t>(t.by + dy) * bw + t.bx + dx];
+      DcToLlf(params.dc_basis, S::kCw, S::kCh, dcv, coef + (k * 3 + c) * N);
+    }
+  }
+  __syncthreads();
+  InverseRows<W, H, 3 * K>(ih, coef, tmp);
+  __syncthreads();
+  const size_t plane = static_cast<size_t>(width) * height;
+  InverseCols<W, H, 3 * K>(iv, tmp, [&](int p, int y, int x, float v) {
+    const TransformSlot& t = slots[p / 3];
+    if (t.ok) out[(p % 3) * plane + static_cast<size_t>(t.by * 8 + y) * width + t.bx * 8 + x] = v;
+  });
 }
 
 DcBasisTable HostDcBasis() {
@@ -919,23 +1118,54 @@ AqLayout MakeAqLayout(int width, int height) {
 
 GJXLResult LaunchAnchorLayout(hipStream_t s, const uint8_t* cells, int width,
                               int height, uint32_t* offsets, uint32_t* list,
-                              uint32_t* count, uint32_t* error) {
+                              uint32_t* count, uint32_t* error,
+                              uint32_t* strategy_lists, size_t capacity,
+                              uint32_t* strategy_counts) {
   const AqLayout l = MakeAqLayout(width, height);
   hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, error, 1, 0u);
+  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, strategy_counts,
+                     kStrategyStages, 0u);
   hipLaunchKernelGGL(AnchorLayoutKernel, dim3(l.groups), dim3(64), 0, s, cells,
-                     l.bw, l.bh, offsets, list, count, error);
+                     l.bw, l.bh, offsets, list, count, error, strategy_lists, capacity,
+                     strategy_counts);
   return CheckLaunch("AnchorLayoutKernel");
 }
 
+namespace {
+
+// Calls launch.template operator()<strategy>(entries, count) for every
+// strategy with transforms in the lists.
+template <typename Launch>
+GJXLResult ForEachStrategy(const StrategyLists& lists, Launch&& launch) {
+  const uint32_t* e = lists.entries;
+  const size_t c = lists.capacity;
+  if (lists.counts[0]) GJXL_AMD_TRY(launch.template operator()<kDct8>(e, lists.counts[0]));
+  if (lists.counts[1]) GJXL_AMD_TRY(launch.template operator()<kDct16x8>(e + c, lists.counts[1]));
+  if (lists.counts[2]) GJXL_AMD_TRY(launch.template operator()<kDct8x16>(e + 2 * c, lists.counts[2]));
+  if (lists.counts[3]) GJXL_AMD_TRY(launch.template operator()<kDct16x16>(e + 3 * c, lists.counts[3]));
+  if (lists.counts[4]) GJXL_AMD_TRY(launch.template operator()<kDct32x16>(e + 4 * c, lists.counts[4]));
+  if (lists.counts[5]) GJXL_AMD_TRY(launch.template operator()<kDct16x32>(e + 5 * c, lists.counts[5]));
+  if (lists.counts[6]) GJXL_AMD_TRY(launch.template operator()<kDct32x32>(e + 6 * c, lists.counts[6]));
+  return GJXL_OK;
+}
+
+template <int kStrategy>
+dim3 StrategyGrid(uint32_t count) {
+  return dim3((count + AnchorShape<kStrategy>::kK - 1) / AnchorShape<kStrategy>::kK);
+}
+
+}  // namespace
+
 GJXLResult LaunchAnchorDct(GJXLAmdDevice* d, const float* opsin, int width,
-                           int height, const uint8_t* cells,
-                           const uint32_t* offsets, const uint32_t* list,
-                           const uint32_t* count, float* coefs) {
+                           int height, const uint32_t* offsets,
+                           const StrategyLists& lists, float* coefs) {
   const AqLayout l = MakeAqLayout(width, height);
-  hipLaunchKernelGGL(AnchorDctKernel, dim3(kGroupAnchors, l.groups),
-                     dim3(kAnchorThreads), 0, d->stream, opsin, width, height,
-                     cells, offsets, list, count, d->dct_bases, l.store, coefs);
-  return CheckLaunch("AnchorDctKernel");
+  return ForEachStrategy(lists, [&]<int kStrategy>(const uint32_t* entries, uint32_t count) {
+    hipLaunchKernelGGL(AnchorDctKernel<kStrategy>, StrategyGrid<kStrategy>(count),
+                       dim3(kAqThreads), 0, d->stream, opsin, width, height, entries, count,
+                       offsets, d->dct_bases, l.store, coefs);
+    return CheckLaunch("AnchorDctKernel");
+  });
 }
 
 GJXLResult LaunchAdjustField(hipStream_t s, const uint8_t* cells, int width,
@@ -1008,12 +1238,13 @@ GJXLResult LaunchFinalCfl(GJXLAmdDevice* d, const float* coefs, int width,
 
 GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                                   int width, int height, const uint8_t* cells,
-                                  const uint32_t* offsets, const uint32_t* list,
-                                  const uint32_t* count, int32_t* raw,
+                                  const uint32_t* offsets,
+                                  const StrategyLists& lists, int32_t* raw,
                                   const int8_t* y_to_x, const int8_t* y_to_b,
                                   const QuantizerState* q, float x_mult,
                                   float b_mult, int32_t* quantized, float* dc,
                                   int32_t* dc_quantized, float* dc_rec) {
+  (void)cells;
   const AqLayout l = MakeAqLayout(width, height);
   CodingParams p;
   p.x_mult = x_mult;
@@ -1021,11 +1252,13 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
   static const DcBasisTable table = HostDcBasis();
   p.dc_basis = table;
   const size_t n = static_cast<size_t>(l.bw) * l.bh;
-  hipLaunchKernelGGL(CodeTransformKernel, dim3(kGroupAnchors, l.groups),
-                     dim3(kAnchorThreads), 0, d->stream, coefs, l.store, cells,
-                     offsets, list, count, l.bw, raw, y_to_x, y_to_b, q,
-                     d->quant_tables, p, quantized, dc, n);
-  GJXL_AMD_TRY(CheckLaunch("CodeTransformKernel"));
+  GJXL_AMD_TRY(ForEachStrategy(lists, [&]<int kStrategy>(const uint32_t* entries,
+                                                         uint32_t count) {
+    hipLaunchKernelGGL(CodeTransformKernel<kStrategy>, StrategyGrid<kStrategy>(count),
+                       dim3(kAqThreads), 0, d->stream, coefs, l.store, entries, count, offsets,
+                       l.bw, raw, y_to_x, y_to_b, q, d->quant_tables, p, quantized, dc, n);
+    return CheckLaunch("CodeTransformKernel");
+  }));
   const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 1);
   hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
                      q, 1, 1, dc_quantized);
@@ -1041,8 +1274,8 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
 
 GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
                              int width, int height, const uint8_t* cells,
-                             const uint32_t* offsets, const uint32_t* list,
-                             const uint32_t* count, const int32_t* raw,
+                             const uint32_t* offsets, const StrategyLists& lists,
+                             const int32_t* raw,
                              const int8_t* y_to_x, const int8_t* y_to_b,
                              const QuantizerState* q, float x_mult, float b_mult,
                              const float* dc_rec, float* smoothed,
@@ -1059,11 +1292,13 @@ GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
   p.b_mult = b_mult;
   static const DcBasisTable table = HostDcBasis();
   p.dc_basis = table;
-  hipLaunchKernelGGL(ReconstructKernel, dim3(kGroupAnchors, l.groups),
-                     dim3(kAnchorThreads), 0, d->stream, quantized, l.store, cells,
-                     offsets, list, count, width, height, raw, y_to_x, y_to_b, q,
-                     d->quant_tables, d->dct_bases, p, smoothed, out);
-  return CheckLaunch("ReconstructKernel");
+  return ForEachStrategy(lists, [&]<int kStrategy>(const uint32_t* entries, uint32_t count) {
+    hipLaunchKernelGGL(ReconstructKernel<kStrategy>, StrategyGrid<kStrategy>(count),
+                       dim3(kAqThreads), 0, d->stream, quantized, l.store, entries, count,
+                       offsets, width, height, raw, y_to_x, y_to_b, q, d->quant_tables,
+                       d->dct_bases, p, smoothed, out);
+    return CheckLaunch("ReconstructKernel");
+  });
 }
 
 }  // namespace gjxl_amd

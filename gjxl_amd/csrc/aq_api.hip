@@ -25,6 +25,7 @@ struct GJXLAmdAqSession {
   int x_scale = 2, b_scale = 2;
   float x_mult = 1.0f, b_mult = 1.0f;
   bool prepared = false;
+  uint32_t strategy_counts[gjxl_amd::kStrategyStages] = {};
   // owned device storage, indexed by GJXL_AMD_AQ_* ids
   void* buffers[GJXL_AMD_AQ_BUFFER_COUNT] = {};
   size_t sizes[GJXL_AMD_AQ_BUFFER_COUNT] = {};
@@ -38,6 +39,14 @@ namespace {
 template <typename T>
 T* Buf(GJXLAmdAqSession* s, int id) {
   return static_cast<T*>(s->buffers[id]);
+}
+
+gjxl_amd::StrategyLists Lists(GJXLAmdAqSession* s) {
+  gjxl_amd::StrategyLists l{};
+  l.entries = Buf<uint32_t>(s, GJXL_AMD_AQ_STRATEGY_LISTS);
+  l.capacity = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  for (int i = 0; i < gjxl_amd::kStrategyStages; ++i) l.counts[i] = s->strategy_counts[i];
+  return l;
 }
 
 GJXLResult ValidateSession(GJXLAmdAqSession* s) {
@@ -147,7 +156,7 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_ANCHOR_COUNT] = 4 * groups;
   z[GJXL_AMD_AQ_COEFFICIENTS] = 4 * 3 * store;
   z[GJXL_AMD_AQ_ADJUSTED_FIELD] = 4 * n;
-  z[GJXL_AMD_AQ_STATS] = 4 * 8;
+  z[GJXL_AMD_AQ_STATS] = 4 * 16;
   z[GJXL_AMD_AQ_SELECT] = 4 * (4 + 256);
   z[GJXL_AMD_AQ_QUANTIZER] = sizeof(gjxl_amd::QuantizerState);
   z[GJXL_AMD_AQ_RAW_QUANT] = 4 * n;
@@ -167,6 +176,7 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_FIELD] = 4 * n;
   z[GJXL_AMD_AQ_BLOCK_DISTANCE] = 4 * n;
   z[GJXL_AMD_AQ_SCORES] = 4 * 64;
+  z[GJXL_AMD_AQ_STRATEGY_LISTS] = 4 * gjxl_amd::kStrategyStages * n;
   for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
     const GJXLResult r = Check(hipMalloc(&s->buffers[i], z[i]), "hipMalloc(AQ session)");
     if (r != GJXL_OK) {
@@ -218,23 +228,21 @@ GJXLResult gjxl_amd_aq_prepare(GJXLAmdAqSession* s, const float* opsin,
   using namespace gjxl_amd;
   GJXL_AMD_TRY(LaunchMatrixStats(st, opsin, s->width, s->height, s->pw, s->ph,
                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS)));
+  const size_t blocks = static_cast<size_t>(s->layout.bw) * s->layout.bh;
   GJXL_AMD_TRY(LaunchAnchorLayout(st, cells, s->pw, s->ph,
                                   Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
                                   Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
                                   Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT),
-                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + 6));
-  GJXL_AMD_TRY(LaunchAnchorDct(d, preprocessed, s->pw, s->ph, cells,
-                               Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
-                               Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
-                               Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT),
-                               Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS)));
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + 6,
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STRATEGY_LISTS), blocks,
+                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + 8));
   GJXL_AMD_TRY(LaunchAdjustField(st, cells, s->pw, s->ph, butteraugli_target,
                                  initial_quant_field,
                                  Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD)));
   const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
   uint32_t* stats = Buf<uint32_t>(s, GJXL_AMD_AQ_STATS);
   GJXL_AMD_TRY(LaunchRange(st, Buf<float>(s, GJXL_AMD_AQ_ADJUSTED_FIELD), n, stats + 4));
-  uint32_t host[8];
+  uint32_t host[16];
   GJXL_AMD_TRY(Check(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, st),
                      "hipMemcpyAsync(stats)"));
   GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
@@ -242,6 +250,10 @@ GJXLResult gjxl_amd_aq_prepare(GJXLAmdAqSession* s, const float* opsin,
     return Fail(GJXL_ERROR_INVALID_ARGUMENT,
                 "Strategy cells do not form a valid grid of searched transforms");
   }
+  for (int i = 0; i < kStrategyStages; ++i) s->strategy_counts[i] = host[8 + i];
+  GJXL_AMD_TRY(LaunchAnchorDct(d, preprocessed, s->pw, s->ph,
+                               Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s),
+                               Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS)));
   float fstats[3];
   for (int i = 0; i < 3; ++i) std::memcpy(&fstats[i], &host[i], 4);
   SelectMatrixScales(fstats, butteraugli_target, &s->x_scale, &s->b_scale);
@@ -302,16 +314,14 @@ GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
                               Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B)));
   GJXL_AMD_TRY(LaunchCodeCoefficients(
     d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph, s->cells,
-    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
-    Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
+    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
     Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q,
     s->x_mult, s->b_mult, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED),
     Buf<float>(s, GJXL_AMD_AQ_DC), Buf<int32_t>(s, GJXL_AMD_AQ_DC_QUANTIZED),
     Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED)));
   GJXL_AMD_TRY(LaunchReconstruct(
     d, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED), s->pw, s->ph, s->cells,
-    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_LIST),
-    Buf<uint32_t>(s, GJXL_AMD_AQ_ANCHOR_COUNT), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
+    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
     Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q, s->x_mult,
     s->b_mult, Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED),
     Buf<float>(s, GJXL_AMD_AQ_DC_SMOOTHED), Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA),

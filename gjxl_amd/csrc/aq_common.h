@@ -16,6 +16,23 @@ constexpr int kGroupAnchors = kGroupBlocks * kGroupBlocks;
 constexpr int kGroupCoefficients = 65536;
 constexpr uint32_t kNoAnchor = 0xffffffffu;
 
+// Per-strategy anchor lists, in the stage order of the candidate search:
+// DCT8, 16x8, 8x16, 16x16, 32x16, 16x32, 32x32.
+constexpr int kStrategyStages = 7;
+__host__ __device__ constexpr int StrategyStage(int strategy) {
+  return strategy == kDct8 ? 0 : strategy == kDct16x8 ? 1 : strategy == kDct8x16 ? 2
+       : strategy == kDct16x16 ? 3 : strategy == kDct32x16 ? 4
+       : strategy == kDct16x32 ? 5 : 6;
+}
+
+// The strategy lists of a layout: entries[stage * capacity + k] = by << 16 |
+// bx, counts[stage] on the host.
+struct StrategyLists {
+  const uint32_t* entries;
+  size_t capacity;
+  uint32_t counts[kStrategyStages];
+};
+
 __device__ __forceinline__ float DStdMax(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float DStdMin(float a, float b) { return b < a ? b : a; }
 
@@ -201,11 +218,12 @@ struct AqLayout {
 AqLayout MakeAqLayout(int width, int height);
 GJXLResult LaunchAnchorLayout(hipStream_t s, const uint8_t* cells, int width,
                               int height, uint32_t* offsets, uint32_t* list,
-                              uint32_t* count, uint32_t* error);
+                              uint32_t* count, uint32_t* error,
+                              uint32_t* strategy_lists, size_t capacity,
+                              uint32_t* strategy_counts);
 GJXLResult LaunchAnchorDct(GJXLAmdDevice* d, const float* opsin, int width,
-                           int height, const uint8_t* cells,
-                           const uint32_t* offsets, const uint32_t* list,
-                           const uint32_t* count, float* coefs);
+                           int height, const uint32_t* offsets,
+                           const StrategyLists& lists, float* coefs);
 GJXLResult LaunchAdjustField(hipStream_t s, const uint8_t* cells, int width,
                              int height, float target, const float* in,
                              float* out);
@@ -223,16 +241,16 @@ GJXLResult LaunchFinalCfl(GJXLAmdDevice* d, const float* coefs, int width,
                           int8_t* y_to_b);
 GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                                   int width, int height, const uint8_t* cells,
-                                  const uint32_t* offsets, const uint32_t* list,
-                                  const uint32_t* count, int32_t* raw,
+                                  const uint32_t* offsets,
+                                  const StrategyLists& lists, int32_t* raw,
                                   const int8_t* y_to_x, const int8_t* y_to_b,
                                   const QuantizerState* q, float x_mult,
                                   float b_mult, int32_t* quantized, float* dc,
                                   int32_t* dc_quantized, float* dc_rec);
 GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
                              int width, int height, const uint8_t* cells,
-                             const uint32_t* offsets, const uint32_t* list,
-                             const uint32_t* count, const int32_t* raw,
+                             const uint32_t* offsets, const StrategyLists& lists,
+                             const int32_t* raw,
                              const int8_t* y_to_x, const int8_t* y_to_b,
                              const QuantizerState* q, float x_mult, float b_mult,
                              const float* dc_rec, float* smoothed,

@@ -179,7 +179,8 @@ enum {
   GJXL_AMD_AQ_ANCHOR_COUNT,         /* u32 per group */
   GJXL_AMD_AQ_COEFFICIENTS,         /* f32 forward DCT of every transform */
   GJXL_AMD_AQ_ADJUSTED_FIELD,       /* f32 AdjustQuantField output */
-  GJXL_AMD_AQ_STATS,                /* u32[8]: matrix stats 0-2, range 4-5 */
+  GJXL_AMD_AQ_STATS,                /* u32[16]: matrix stats 0-2, range 4-5,
+                                       grid error 6, strategy counts 8-14 */
   GJXL_AMD_AQ_SELECT,               /* u32[4 + 256]: median / MAD selection */
   GJXL_AMD_AQ_QUANTIZER,            /* quantizer state (10 words) */
   GJXL_AMD_AQ_RAW_QUANT,            /* i32 per block: adjusted raw quant */
@@ -199,6 +200,7 @@ enum {
   GJXL_AMD_AQ_FIELD,                /* f32 per block: gjxl_amd_aq_run field */
   GJXL_AMD_AQ_BLOCK_DISTANCE,       /* f32 per block: last 16-norms */
   GJXL_AMD_AQ_SCORES,               /* u32[64]: score bits per evaluation */
+  GJXL_AMD_AQ_STRATEGY_LISTS,       /* u32 7 x blocks: anchors per strategy */
   GJXL_AMD_AQ_BUFFER_COUNT
 };
 
