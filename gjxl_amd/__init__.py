@@ -63,6 +63,7 @@ _SIGNATURES = {
     "gjxl_amd_free": [_P, _P],
     "gjxl_amd_upload": [_P, _P, _P, _SZ],
     "gjxl_amd_download": [_P, _P, _P, _SZ],
+    "gjxl_amd_copy": [_P, _P, _P, _SZ],
     "gjxl_amd_synchronize": [_P],
     "gjxl_amd_last_error": [],
     "gjxl_amd_event_record": [_P, _I],
@@ -211,6 +212,13 @@ class Device:
 
     def synchronize(self) -> None:
         _check(library().gjxl_amd_synchronize(self.handle))
+
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int | None = None,
+             dst_offset: int = 0, src_offset: int = 0) -> None:
+        """Device-to-device copy on the stream (byte offsets)."""
+        nbytes = min(dst.nbytes, src.nbytes) if nbytes is None else nbytes
+        _check(library().gjxl_amd_copy(self.handle, dst.ptr + dst_offset,
+                                       src.ptr + src_offset, nbytes))
 
     def record(self, slot: int) -> None:
         _check(library().gjxl_amd_event_record(self.handle, slot))
@@ -392,7 +400,8 @@ class AqSession:
         return list(out) if scores else None
 
 
-BUTTERAUGLI_BUFFERS = ("map", "reference_psycho", "distorted_psycho", "stages", "xyb")
+BUTTERAUGLI_BUFFERS = ("map", "reference_psycho", "distorted_psycho", "stages", "xyb",
+                       "score")
 
 
 class Butteraugli:
@@ -442,6 +451,8 @@ class Butteraugli:
             ctypes.byref(size)))
         if name == "map":
             shape = (self.height, self.width)
+        elif name == "score":
+            return DeviceArray(self.device, (1,), np.uint32, pointer=ptr.value)
         else:
             w = self.width if scale == 0 else (self.width + 1) // 2
             h = self.height if scale == 0 else (self.height + 1) // 2

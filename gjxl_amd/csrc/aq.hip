@@ -803,9 +803,10 @@ struct DcRing {
   uint32_t e[4][kRing][kDcGroup];   // sub-predictor errors
 };
 
-__device__ __forceinline__ uint32_t Reciprocal(uint32_t i) {
-  return (1u << 24) / (i + 1);
-}
+// (1 << 24) / (i + 1) for the arguments the predictor forms: e >> shift
+// is below 64 and the final weight sum minus one below 127, so one table of
+// 128 entries covers both (filled per work group).
+constexpr int kReciprocals = 128;
 
 __device__ __forceinline__ int BitWidth64(uint64_t v) {
   return v == 0 ? 0 : 64 - __clzll(static_cast<long long>(v));
@@ -818,6 +819,9 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
   const QuantizerState* __restrict__ qstate, int first_channel,
   int extra_precision, int32_t* __restrict__ quantized) {
   __shared__ DcRing ring;
+  __shared__ uint32_t reciprocal[kReciprocals];
+  if (threadIdx.x < kReciprocals) reciprocal[threadIdx.x] = (1u << 24) / (threadIdx.x + 1);
+  __syncthreads();
   const int channel = blockIdx.z == 0 ? first_channel : 2;
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
   const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
@@ -860,7 +864,7 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
         const uint32_t e = north + ne + nw;
         int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
         if (shift < 0) shift = 0;
-        weights[i] = 4 + ((kMaxWeights[i] * Reciprocal(e >> shift)) >> shift);
+        weights[i] = 4 + ((kMaxWeights[i] * reciprocal[e >> shift]) >> shift);
         weight_sum += weights[i];
       }
       const unsigned log_weight = BitWidth64(weight_sum) - 1;
@@ -884,7 +888,7 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
       int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
 #pragma unroll
       for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
-      int64_t prediction = (sum * static_cast<int64_t>(Reciprocal(weight_sum - 1))) >> 24;
+      int64_t prediction = (sum * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24;
       if (((en ^ ew) | (en ^ enw)) <= 0) {
         int64_t lo = w8 < ne8 ? w8 : ne8;
         lo = lo < n8 ? lo : n8;

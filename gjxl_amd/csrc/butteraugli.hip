@@ -269,12 +269,9 @@ struct MaltaScale {
   float norm;  // (float)norm
 };
 
-__global__ __launch_bounds__(kThreads) void MaltaDiffKernel(
-  const float* __restrict__ p0, const float* __restrict__ p1, size_t n,
-  MaltaScale m, float* __restrict__ out) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float v0 = p0[i], v1 = p1[i];
+// The scaled difference MaltaDiffMap feeds its lines (butteraugli_distance_
+// internal.cpp:512-555).
+__device__ __forceinline__ float MaltaScaledDiff(float v0, float v1, const MaltaScale& m) {
   const float absolute = 0.5f * (fabsf(v0) + fabsf(v1));
   const float diff = v0 - v1;
   const float scaler = m.gt / (m.norm + absolute);
@@ -294,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void MaltaDiffKernel(
   } else if (dv1 > too_big) {
     s = static_cast<float>(static_cast<double>(s) - ds2 * (dv1 - too_big));
   }
-  out[i] = s;
+  return s;
 }
 
 __device__ __forceinline__ float Sum4(float a, float b, float c, float d) {
@@ -343,40 +340,58 @@ constexpr MaltaTap kFullLines[16][9] = {
   {{1, -4}, {1, -3}, {1, -2}, {0, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}, {-1, 4}},
 };
 
-// Malta response of the scaled differences `d` (zero outside the image)
-// into `out` (0 + response, as the reference accumulates into a zeroed
-// plane).
+// MaltaDiffMap fused: a work group scales the differences of a 64x16 tile
+// plus a 4-pixel halo into LDS (zero outside the image, as the reference's
+// ReadWithZeroBorder), then evaluates the 16 Malta lines of every pixel and
+// writes 0 + response (the reference accumulates into a zeroed plane).
+constexpr int kMaltaTileW = 64, kMaltaTileH = 16, kMaltaHalo = 4;
+constexpr int kMaltaLw = kMaltaTileW + 2 * kMaltaHalo, kMaltaLh = kMaltaTileH + 2 * kMaltaHalo;
+
 template <bool kLowFrequency>
-__global__ __launch_bounds__(kThreads) void MaltaKernel(const float* __restrict__ d,
-                                                        int w, int h,
-                                                        float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
-  auto P = [&](int dx, int dy) -> float {
-    const int sx = x + dx, sy = y + dy;
-    if (sx < 0 || sy < 0 || sx >= w || sy >= h) return 0.0f;
-    return d[static_cast<size_t>(sy) * w + sx];
-  };
-  float result = 0.0f;
-#pragma unroll
-  for (int l = 0; l < 16; ++l) {
-    float s;
-    if constexpr (kLowFrequency) {
-      const MaltaTap* t = kLfLines[l];
-      s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
-               P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
-    } else {
-      const MaltaTap* t = kFullLines[l];
-      float v[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
-      s = kFullTaps[l] == 9
-            ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
-            : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
+__global__ __launch_bounds__(kThreads) void MaltaKernel(
+  const float* __restrict__ p0, const float* __restrict__ p1, int w, int h,
+  MaltaScale m, float* __restrict__ out) {
+  __shared__ float tile[kMaltaLh][kMaltaLw];
+  const int x0 = blockIdx.x * kMaltaTileW, y0 = blockIdx.y * kMaltaTileH;
+  for (int e = threadIdx.x; e < kMaltaLh * kMaltaLw; e += blockDim.x) {
+    const int r = e / kMaltaLw, col = e % kMaltaLw;
+    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+    float v = 0.0f;
+    if (sx >= 0 && sy >= 0 && sx < w && sy < h) {
+      const size_t i = static_cast<size_t>(sy) * w + sx;
+      v = MaltaScaledDiff(p0[i], p1[i], m);
     }
-    result = l == 0 ? s * s : s * s + result;
+    tile[r][col] = v;
   }
-  out[static_cast<size_t>(y) * w + x] = 0.0f + result;
+  __syncthreads();
+  const int tx = threadIdx.x % kMaltaTileW;
+  for (int ty = threadIdx.x / kMaltaTileW; ty < kMaltaTileH; ty += blockDim.x / kMaltaTileW) {
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) continue;
+    auto P = [&](int dx, int dy) -> float {
+      return tile[ty + kMaltaHalo + dy][tx + kMaltaHalo + dx];
+    };
+    float result = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      float s;
+      if constexpr (kLowFrequency) {
+        const MaltaTap* t = kLfLines[l];
+        s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
+                 P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
+      } else {
+        const MaltaTap* t = kFullLines[l];
+        float v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
+        s = kFullTaps[l] == 9
+              ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
+              : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
+      }
+      result = l == 0 ? s * s : s * s + result;
+    }
+    out[static_cast<size_t>(y) * w + x] = 0.0f + result;
+  }
 }
 
 struct L2Weights {
@@ -734,15 +749,16 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
     const MaltaScale m{static_cast<float>(pre_gt * kMaltaNorms[k]),
                        static_cast<float>(pre_lt * kMaltaNorms[k]),
                        static_cast<float>(kMaltaNorms[k])};
-    hipLaunchKernelGGL(MaltaDiffKernel, Grid1(n), dim3(kThreads), 0, st,
-                       s.ref_psycho + plane_of[k] * n, s.dist_psycho + plane_of[k] * n, n, m,
-                       s.tmp[0]);
+    const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
+                     CeilDiv(static_cast<uint32_t>(h), kMaltaTileH));
+    const float* r0 = s.ref_psycho + plane_of[k] * n;
+    const float* r1 = s.dist_psycho + plane_of[k] * n;
     if (lf) {
-      hipLaunchKernelGGL(MaltaKernel<true>, Grid2(w, h), dim3(kThreads), 0, st, s.tmp[0], w,
-                         h, s.stages + k * n);
+      hipLaunchKernelGGL(MaltaKernel<true>, tiles, dim3(kThreads), 0, st, r0, r1, w, h, m,
+                         s.stages + k * n);
     } else {
-      hipLaunchKernelGGL(MaltaKernel<false>, Grid2(w, h), dim3(kThreads), 0, st, s.tmp[0],
-                         w, h, s.stages + k * n);
+      hipLaunchKernelGGL(MaltaKernel<false>, tiles, dim3(kThreads), 0, st, r0, r1, w, h, m,
+                         s.stages + k * n);
     }
     GJXL_AMD_TRY(CheckLaunch("MaltaKernel"));
   }
@@ -987,6 +1003,7 @@ GJXLResult gjxl_amd_butteraugli_buffer(GJXLAmdButteraugli* b, int32_t scale,
     case GJXL_AMD_BUTTERAUGLI_DISTORTED_PSYCHO: p = s.dist_psycho; z = 10 * plane; break;
     case GJXL_AMD_BUTTERAUGLI_STAGES: p = s.stages; z = 9 * plane; break;
     case GJXL_AMD_BUTTERAUGLI_XYB: p = s.xyb; z = 3 * plane; break;
+    case GJXL_AMD_BUTTERAUGLI_SCORE: p = b->score_bits; z = 4; break;
     default: return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli buffer id is invalid");
   }
   *pointer = p;

@@ -152,6 +152,17 @@ GJXLResult gjxl_amd_download(GJXLAmdDevice* device, void* dst, const void* src,
   return Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
 }
 
+GJXLResult gjxl_amd_copy(GJXLAmdDevice* device, void* dst, const void* src,
+                         size_t bytes) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (bytes == 0) return GJXL_OK;
+  if (dst == nullptr || src == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Copy pointer is null");
+  }
+  return Check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, device->stream),
+               "hipMemcpyAsync(D2D)");
+}
+
 GJXLResult gjxl_amd_synchronize(GJXLAmdDevice* device) {
   GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
   return Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");

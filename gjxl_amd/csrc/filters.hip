@@ -46,58 +46,75 @@ __global__ __launch_bounds__(256) void GaborishKernel(
 }
 
 // ApplyEpfPass (src/codec/epf.cpp:61-205), pass 1 (plus-patch SAD) or pass 2
-// (pixel SAD), four cardinal neighbours, default options.
+// (pixel SAD), four cardinal neighbours, default options.  A work group
+// filters a 64x16 tile from an LDS copy of its three channels with a
+// mirrored 2-pixel halo (every sample either pass reads is within 2 pixels).
+constexpr int kEpfTileW = 64, kEpfTileH = 16, kEpfHalo = 2;
+constexpr int kEpfLw = kEpfTileW + 2 * kEpfHalo, kEpfLh = kEpfTileH + 2 * kEpfHalo;
+
 template <int kPass>
 __global__ __launch_bounds__(256) void EpfKernel(
   const float* __restrict__ in, int w, int h,
   const float* __restrict__ inverse_sigma, float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
+  __shared__ float tile[3][kEpfLh][kEpfLw];
+  const int x0 = blockIdx.x * kEpfTileW, y0 = blockIdx.y * kEpfTileH;
   const size_t plane = static_cast<size_t>(w) * h;
-  const size_t i = static_cast<size_t>(y) * w + x;
-  const int bw = (w + 7) / 8;
-  const float bis = inverse_sigma[static_cast<size_t>(y / 8) * bw + x / 8];
-  constexpr float kMinInverseSigma = -1.1715728752538099024f / 0.3f;
-  if (bis < kMinInverseSigma) {
-    for (int c = 0; c < 3; ++c) out[c * plane + i] = in[c * plane + i];
-    return;
+  for (int e = threadIdx.x; e < 3 * kEpfLh * kEpfLw; e += blockDim.x) {
+    const int c = e / (kEpfLh * kEpfLw), r = (e / kEpfLw) % kEpfLh, col = e % kEpfLw;
+    const int sy = MirrorInt(y0 + r - kEpfHalo, h), sx = MirrorInt(x0 + col - kEpfHalo, w);
+    tile[c][r][col] = in[c * plane + static_cast<size_t>(sy) * w + sx];
   }
+  __syncthreads();
+  const int bw = (w + 7) / 8;
+  constexpr float kMinInverseSigma = -1.1715728752538099024f / 0.3f;
   constexpr float kSigmaScale = 1.65f * (kPass == 2 ? 6.5f : 1.0f);
-  const bool border = x % 8 == 0 || x % 8 == 7 || y % 8 == 0 || y % 8 == 7;
-  const float sis = bis * kSigmaScale * (border ? 2.0f / 3.0f : 1.0f);
-  auto S = [&](int c, int sx, int sy) {
-    return in[c * plane + static_cast<size_t>(MirrorInt(sy, h)) * w + MirrorInt(sx, w)];
-  };
   constexpr int kCard[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
   constexpr int kPlus[5][2] = {{0, 0}, {0, -1}, {-1, 0}, {0, 1}, {1, 0}};
   constexpr float kScale[3] = {40.0f, 5.0f, 3.5f};
-  float sum[3] = {in[i], in[plane + i], in[2 * plane + i]};
-  float weight_sum = 1.0f;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int dx = kCard[k][0], dy = kCard[k][1];
-    float sad = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if constexpr (kPass == 1) {
-        float cs = 0.0f;
-#pragma unroll
-        for (int o = 0; o < 5; ++o) {
-          cs += fabsf(S(c, x + kPlus[o][0], y + kPlus[o][1]) -
-                      S(c, x + dx + kPlus[o][0], y + dy + kPlus[o][1]));
-        }
-        sad = __builtin_fmaf(cs, kScale[c], sad);
-      } else {
-        sad = __builtin_fmaf(fabsf(S(c, x, y) - S(c, x + dx, y + dy)), kScale[c], sad);
-      }
+  const int tx = threadIdx.x % kEpfTileW;
+  for (int ty = threadIdx.x / kEpfTileW; ty < kEpfTileH; ty += blockDim.x / kEpfTileW) {
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) continue;
+    const size_t i = static_cast<size_t>(y) * w + x;
+    // tile sample at offset (dx, dy) from (x, y)
+    auto S = [&](int c, int dx, int dy) {
+      return tile[c][ty + kEpfHalo + dy][tx + kEpfHalo + dx];
+    };
+    const float bis = inverse_sigma[static_cast<size_t>(y / 8) * bw + x / 8];
+    if (bis < kMinInverseSigma) {
+      for (int c = 0; c < 3; ++c) out[c * plane + i] = S(c, 0, 0);
+      continue;
     }
-    float weight = __builtin_fmaf(sad, sis, 1.0f);
-    weight = 0.0f < weight ? weight : 0.0f;
-    weight_sum += weight;
+    const bool border = x % 8 == 0 || x % 8 == 7 || y % 8 == 0 || y % 8 == 7;
+    const float sis = bis * kSigmaScale * (border ? 2.0f / 3.0f : 1.0f);
+    float sum[3] = {S(0, 0, 0), S(1, 0, 0), S(2, 0, 0)};
+    float weight_sum = 1.0f;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sum[c] = __builtin_fmaf(weight, S(c, x + dx, y + dy), sum[c]);
+    for (int k = 0; k < 4; ++k) {
+      const int dx = kCard[k][0], dy = kCard[k][1];
+      float sad = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (kPass == 1) {
+          float cs = 0.0f;
+#pragma unroll
+          for (int o = 0; o < 5; ++o) {
+            cs += fabsf(S(c, kPlus[o][0], kPlus[o][1]) -
+                        S(c, dx + kPlus[o][0], dy + kPlus[o][1]));
+          }
+          sad = __builtin_fmaf(cs, kScale[c], sad);
+        } else {
+          sad = __builtin_fmaf(fabsf(S(c, 0, 0) - S(c, dx, dy)), kScale[c], sad);
+        }
+      }
+      float weight = __builtin_fmaf(sad, sis, 1.0f);
+      weight = 0.0f < weight ? weight : 0.0f;
+      weight_sum += weight;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sum[c] = __builtin_fmaf(weight, S(c, dx, dy), sum[c]);
+    }
+    for (int c = 0; c < 3; ++c) out[c * plane + i] = sum[c] / weight_sum;
   }
-  for (int c = 0; c < 3; ++c) out[c * plane + i] = sum[c] / weight_sum;
 }
 
 struct InverseOpsin {
@@ -143,7 +160,7 @@ GJXLResult LaunchLoopFiltersAndColor(hipStream_t s, const float* reconstructed,
   hipLaunchKernelGGL(GaborishKernel, grid, dim3(256), 0, s, reconstructed, pw,
                      static_cast<size_t>(pw) * ph, w, h, g, scratch_a);
   GJXL_AMD_TRY(CheckLaunch("GaborishKernel"));
-  const dim3 grid2(CeilDiv(w, 256), h);
+  const dim3 grid2(CeilDiv(w, kEpfTileW), CeilDiv(h, kEpfTileH));
   hipLaunchKernelGGL(EpfKernel<1>, grid2, dim3(256), 0, s, scratch_a, w, h,
                      inverse_sigma, scratch_b);
   GJXL_AMD_TRY(CheckLaunch("EpfKernel<1>"));

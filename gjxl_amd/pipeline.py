@@ -13,7 +13,7 @@ from typing import Callable
 
 import numpy as np
 
-from . import AqSession, Butteraugli, Device, DeviceArray, library, pad8
+from . import AqSession, Butteraugli, Device, DeviceArray, library, pad8, reduce_distance_map
 
 
 def run_frontend(dev: Device, srgb: np.ndarray, intensity_target: float = 255.0,
@@ -58,6 +58,14 @@ class PlainPath:
         self.aq_scratch = dev.empty((nbytes,), np.uint8)
         self.session = AqSession(dev, width, height)
         self.butteraugli = Butteraugli(dev, width, height)
+        s = self.session
+        self.field = s.buffer("field", (s.bh, s.bw))
+        self.adjusted = s.buffer("adjusted_field", (s.bh, s.bw))
+        self.block_distance = s.buffer("block_distance", (s.bh, s.bw))
+        self.reconstructed_linear = s.buffer("linear", (3, height, width))
+        self.score_words = s.buffer("scores", (64,), np.uint32)
+        self.score_bits = self.butteraugli.buffer("score")
+        self.distance_map = self.butteraugli.buffer("map")
 
     def close(self) -> None:
         self.session.close()
@@ -87,8 +95,19 @@ class PlainPath:
         mark("aq_prepare")
         self.butteraugli.prepare(self.linear)
         mark("butteraugli_prepare")
-        self.session.run(self.butteraugli, self.iterations, t, scores=False)
-        mark("aq_loop")
+        # the AQ loop, call by call (the same device work as
+        # AqSession.run, with a mark after each part)
+        dev.copy(self.field, self.adjusted)
+        for it in range(self.iterations + 1):
+            self.session.evaluate(self.field)
+            mark("aq_evaluate")
+            self.butteraugli.compare(self.reconstructed_linear)
+            mark("butteraugli_compare")
+            dev.copy(self.score_words, self.score_bits, 4, dst_offset=4 * it)
+            reduce_distance_map(dev, self.distance_map, self.cells, self.block_distance)
+            if it < self.iterations:
+                self.session.update(self.field, self.block_distance, it, t)
+            mark("aq_reduce_update")
 
     def scores(self) -> list[float]:
         words = self.session.buffer("scores", (self.iterations + 1,), np.uint32).numpy()

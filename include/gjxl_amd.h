@@ -64,6 +64,9 @@ GJXL_AMD_API GJXLResult gjxl_amd_upload(GJXLAmdDevice* device, void* dst,
                                         const void* src, size_t bytes);
 GJXL_AMD_API GJXLResult gjxl_amd_download(GJXLAmdDevice* device, void* dst,
                                           const void* src, size_t bytes);
+/* Device-to-device copy, enqueued on the device stream. */
+GJXL_AMD_API GJXLResult gjxl_amd_copy(GJXLAmdDevice* device, void* dst,
+                                      const void* src, size_t bytes);
 GJXL_AMD_API GJXLResult gjxl_amd_synchronize(GJXLAmdDevice* device);
 GJXL_AMD_API const char* gjxl_amd_last_error(void);
 
@@ -265,7 +268,8 @@ enum {
   GJXL_AMD_BUTTERAUGLI_REFERENCE_PSYCHO, /* 10 planes: LF(3) MF(3) HF(2) UHF(2) */
   GJXL_AMD_BUTTERAUGLI_DISTORTED_PSYCHO,
   GJXL_AMD_BUTTERAUGLI_STAGES,           /* 9 planes, DifferenceStage order */
-  GJXL_AMD_BUTTERAUGLI_XYB               /* 3 planes: last opsin image */
+  GJXL_AMD_BUTTERAUGLI_XYB,              /* 3 planes: last opsin image */
+  GJXL_AMD_BUTTERAUGLI_SCORE             /* u32: last score's float bits */
 };
 
 GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_create(
