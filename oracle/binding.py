@@ -537,3 +537,109 @@ def aq_loop(fe: dict, setup: dict, linear: np.ndarray, width: int, height: int,
     return {"field": field, "evaluation": ev, "distance_map": dmap,
             "block_distance": bd, "scores": scores, "fields": fields,
             "block_distances": distances}
+
+
+# ---- a19: AC tokenization -------------------------------------------------
+
+_T_DECLARED = False
+# strategy code of each stage of the order table, and the stage sizes
+STAGE_STRATEGIES = (0, 6, 7, 4, 10, 11, 5)
+ORDER_TABLE_SIZE = 3 * 2624
+# src/codestream/simple_ac_context.h:29-33 (default map, no thresholds)
+DEFAULT_CONTEXT_MAP = np.array([0] * 4 + [1] * 9 + ([2] * 4 + [3] * 9) * 2, np.uint8)
+
+
+def _declare_tokenize():
+    global _T_DECLARED
+    if _T_DECLARED:
+        return
+    lib = oracle()
+    P, SZ, I, U32 = _P, _SZ, _I, _U32
+    sig = {
+        "oracle_natural_order": ([I, P], SZ),
+        "oracle_stage_strategy": ([I], I),
+        "oracle_strategy_stage": ([I], I),
+        "oracle_order_table_offset": ([I], SZ),
+        "oracle_natural_order_table": ([P], None),
+        "oracle_pack_signed": ([ctypes.c_int32], U32),
+        "oracle_token_symbol": ([U32], U32),
+        "oracle_validate_context_map": ([P, SZ, P, U32], I),
+        "oracle_tokenize_group": ([P, SZ, SZ, P, SZ, SZ, SZ, SZ, P, P, P, SZ, P,
+                                   U32, P, P, SZ, ctypes.POINTER(ctypes.c_size_t),
+                                   P], I),
+        "oracle_tokenize_frame": ([P, P, SZ, SZ, P, P, P, SZ, P, U32, P, P, SZ, P,
+                                   P], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _T_DECLARED = True
+
+
+def natural_order(strategy: int) -> np.ndarray:
+    _declare_tokenize()
+    out = np.zeros(1024, np.uint32)
+    n = oracle().oracle_natural_order(strategy, ptr(out))
+    if n == 0:
+        raise ValueError(f"no natural order for strategy {strategy}")
+    return out[:n].copy()
+
+
+def natural_order_table() -> np.ndarray:
+    _declare_tokenize()
+    table = np.zeros(ORDER_TABLE_SIZE, np.uint32)
+    oracle().oracle_natural_order_table(ptr(table))
+    return table
+
+
+def order_slice(table: np.ndarray, strategy: int, channel: int) -> np.ndarray:
+    """The entries of `table` holding the order of (strategy, channel)."""
+    _declare_tokenize()
+    lib = oracle()
+    stage = lib.oracle_strategy_stage(strategy)
+    n = natural_order(strategy).size
+    base = 3 * lib.oracle_order_table_offset(stage) + channel * n
+    return table[base:base + n]
+
+
+def token_capacity(cells: np.ndarray) -> int:
+    anchors = int(np.count_nonzero(cells & 1))
+    return 3 * (64 * cells.size + anchors)
+
+
+def tokenize_frame(quantized: np.ndarray, cells: np.ndarray, raw: np.ndarray,
+                   orders: np.ndarray | None = None, thresholds=(),
+                   context_map: np.ndarray | None = None,
+                   num_contexts: int | None = None,
+                   histogram: bool = False) -> dict:
+    """Tokens of every AC group of a frame in the per-group coefficient
+    layout (3 planes of groups * 65536 int32)."""
+    _declare_tokenize()
+    bh, bw = cells.shape
+    groups = ((bw + 31) // 32) * ((bh + 31) // 32)
+    quantized = np.ascontiguousarray(quantized, np.int32)
+    assert quantized.size == 3 * groups * 65536
+    cells = np.ascontiguousarray(cells, np.uint8)
+    raw = np.ascontiguousarray(raw, np.int32)
+    orders = natural_order_table() if orders is None else np.ascontiguousarray(orders, np.uint32)
+    thr = np.ascontiguousarray(np.array(thresholds, np.uint32).reshape(-1))
+    cmap = DEFAULT_CONTEXT_MAP if context_map is None else np.ascontiguousarray(context_map, np.uint8)
+    nctx = int(cmap.max()) + 1 if num_contexts is None else num_contexts
+    cap = token_capacity(cells)
+    values = np.zeros(max(cap, 1), np.uint32)
+    contexts = np.zeros(max(cap, 1), np.uint16)
+    offsets = np.zeros(groups + 1, np.uint32)
+    hist = np.zeros(nctx * 495 * 128, np.uint32) if histogram else None
+    rc = oracle().oracle_tokenize_frame(
+        ptr(quantized), ptr(cells), bw, bh, ptr(raw), ptr(orders),
+        ptr(thr) if thr.size else None, thr.size, ptr(cmap), nctx, ptr(values),
+        ptr(contexts), cap, ptr(offsets), ptr(hist) if histogram else None)
+    if rc:
+        raise ValueError(f"oracle_tokenize_frame failed with {rc}")
+    n = int(offsets[-1])
+    out = {"values": values[:n].copy(), "contexts": contexts[:n].copy(),
+           "group_offsets": offsets}
+    if histogram:
+        out["histogram"] = hist.reshape(nctx * 495, 128)
+    return out

@@ -220,6 +220,42 @@ void oracle_aq_update(float* field, const float* adjusted_initial,
                       int iteration, float lower, float upper, float scale,
                       float inverse_global_scale);
 
+/* ---- a19: AC tokenization (tokenize.c) ------------------------------ */
+/* src/codestream/ac_group.cpp:381-487.  Returns the size, 0 if unsupported.
+ * Pinned: FNV-1a hashes and samples of tests/ac_group_test.cpp:56-73. */
+size_t oracle_natural_order(int strategy, uint32_t* order /* <= 1024 */);
+/* Strategy stages DCT8, 16x8, 8x16, 16x16, 32x16, 16x32, 32x32. */
+int oracle_stage_strategy(int stage);
+int oracle_strategy_stage(int strategy);
+size_t oracle_order_table_offset(int stage);
+/* 3 * 2624 entries: natural orders for every stage and channel. */
+void oracle_natural_order_table(uint32_t* table);
+uint32_t oracle_pack_signed(int32_t v);
+uint32_t oracle_token_symbol(uint32_t value);
+/* src/codestream/block_context_map.cpp:281-320 */
+int oracle_validate_context_map(const uint32_t* thresholds, size_t nthr,
+                                const uint8_t* map, uint32_t num_contexts);
+/* TokenizeSimpleAcGroupDirectValidated (ac_group.cpp:740-932) on one group.
+ * Pinned: token-stream hashes and block contexts of
+ * tests/ac_group_test.cpp:130-260; decode round trip. */
+int oracle_tokenize_group(const int32_t* quantized, size_t store,
+                          size_t group_base, const uint8_t* cells, size_t bw,
+                          size_t bh, size_t gx0, size_t gy0,
+                          const int32_t* raw, const uint32_t* orders,
+                          const uint32_t* thresholds, size_t nthr,
+                          const uint8_t* context_map, uint32_t num_contexts,
+                          uint32_t* values, uint16_t* contexts,
+                          size_t capacity, size_t* count,
+                          uint32_t* histogram /* nullable, contexts*128 */);
+/* Every group of the frame in group order; group_offsets: groups + 1. */
+int oracle_tokenize_frame(const int32_t* quantized, const uint8_t* cells,
+                          size_t bw, size_t bh, const int32_t* raw,
+                          const uint32_t* orders, const uint32_t* thresholds,
+                          size_t nthr, const uint8_t* context_map,
+                          uint32_t num_contexts, uint32_t* values,
+                          uint16_t* contexts, size_t capacity,
+                          uint32_t* group_offsets, uint32_t* histogram);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);
