@@ -27,7 +27,7 @@ HBM_PEAK_GBPS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def stage_bytes(name: str, w: int, h: int) -> int:
+def stage_bytes(name: str, w: int, h: int, tokens: int = 0) -> int:
     """Algorithmic HBM bytes of one launch: declared inputs read once and
     declared outputs written once."""
     n = w * h
@@ -64,6 +64,10 @@ def stage_bytes(name: str, w: int, h: int) -> int:
         return int((12 + 52 + 4) * n * 1.25)
     if name == "aq_reduce_update":
         return 4 * n + 4 * 4 * (pn // 64)
+    if name == "ac_tokenize":
+        # quantized AC (3 x 64 int32 per block), cells and raw quant in;
+        # tokens (u32 value + u16 context) out
+        return 3 * 4 * pn + (1 + 4) * (pn // 64) + 6 * tokens
     raise KeyError(name)
 
 
@@ -120,7 +124,9 @@ def cpu_baseline(srgb: np.ndarray, budget_s: float = 15.0) -> dict:
     while True:
         fe = binding.frontend(crop)
         setup = binding.aq_setup(fe, cw, ch)
-        binding.aq_loop(fe, setup, fe["linear"], cw, ch)
+        loop = binding.aq_loop(fe, setup, fe["linear"], cw, ch)
+        ev = loop["evaluation"]
+        binding.tokenize_frame(ev["quantized"], fe["cells"], ev["raw"])
         done += 1
         if time.perf_counter() - t0 > budget_s or done >= 50:
             break
@@ -132,7 +138,8 @@ def cpu_baseline(srgb: np.ndarray, budget_s: float = 15.0) -> dict:
         "kind": "port",
         "sample": f"{done} passes over a {cw}x{ch} crop of the same image, "
                   "same stages as the GPU step (frontend, AC search, AQ "
-                  "setup, 3 evaluations with Butteraugli), single thread",
+                  "setup, 3 evaluations with Butteraugli, AC tokens), single "
+                  "thread",
     }
 
 
@@ -189,7 +196,7 @@ def main() -> int:
                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                     "traffic": None}
     else:
-        nbytes = stage_bytes(dominant, w, h)
+        nbytes = stage_bytes(dominant, w, h, path.token_count)
         achieved = nbytes / (per_call[dominant] * 1e-3) / 1e9
         roofline = {"kernel": dominant, "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
@@ -198,7 +205,7 @@ def main() -> int:
                     "algorithmic_bytes": nbytes, "traffic": None}
     result = {
         "metric": "MP/s at d1.0 e7, plain GPU path (XYB, AC strategy search, "
-                  "Butteraugli AQ loop), 1 GPU",
+                  "Butteraugli AQ loop, AC tokenization), 1 GPU",
         "value": round(mp / (ms_per_step * 1e-3), 3),
         "unit": "MP/s",
         "n_gpus": 1,
@@ -219,6 +226,7 @@ def main() -> int:
         "roofline": roofline,
     }
     result["config"]["scores"] = [round(v, 6) for v in path.scores()]
+    result["config"]["ac_tokens"] = path.token_count
     path.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(srgb)

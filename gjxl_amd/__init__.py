@@ -98,9 +98,18 @@ _SIGNATURES = {
     "gjxl_amd_reduce_distance_map": [_P, _P, _U32, _U32, _P, _P],
     "gjxl_amd_aq_update": [_P, _P, _P, ctypes.c_int32, _F],
     "gjxl_amd_aq_run": [_P, _P, ctypes.c_int32, _F, _P],
+    "gjxl_amd_tokenizer_create": [_P, _U32, _U32, ctypes.POINTER(_P)],
+    "gjxl_amd_tokenizer_destroy": [_P],
+    "gjxl_amd_natural_order": [ctypes.c_int32, _P, _SZ, ctypes.POINTER(_SZ)],
+    "gjxl_amd_tokenizer_set_orders": [_P, ctypes.c_uint16, _P],
+    "gjxl_amd_tokenizer_set_context_map": [_P, _P, _U32, _P, _U32],
+    "gjxl_amd_tokenize": [_P, _P, _P, _P, ctypes.c_int32,
+                          ctypes.POINTER(ctypes.c_uint64)],
+    "gjxl_amd_tokenizer_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
+                                  ctypes.POINTER(_SZ)],
 }
 _VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
-         "gjxl_amd_butteraugli_destroy"}
+         "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
@@ -470,6 +479,107 @@ def reduce_distance_map(device: Device, distance_map: DeviceArray,
     _check(library().gjxl_amd_reduce_distance_map(
         device.handle, distance_map.ptr, w, h, cells.ptr, out.ptr))
     return out
+
+
+TOKEN_BUFFERS = (
+    "values", "contexts", "group_offsets", "histogram", "nonzero_map", "tasks",
+    "task_offsets", "group_tokens", "offsets", "anchor_list", "anchor_count",
+    "orders", "inverse_orders", "status",
+)
+ORDER_FAMILIES = 13
+
+
+def natural_order(strategy: int) -> np.ndarray:
+    """ComputeSimpleNaturalCoefficientOrder of the library (host code)."""
+    out = np.zeros(1024, np.uint32)
+    n = ctypes.c_size_t()
+    _check(library().gjxl_amd_natural_order(
+        strategy, out.ctypes.data, out.size, ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
+class Tokenizer:
+    """AC tokenization of a quantized frame (gjxl_amd_tokenizer_*)."""
+
+    def __init__(self, device: Device, width: int, height: int):
+        self.device = device
+        self.width, self.height = width, height
+        self.bw, self.bh = pad8(width) // 8, pad8(height) // 8
+        self.groups = ((self.bw + 31) // 32) * ((self.bh + 31) // 32)
+        self.num_contexts = 4
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_tokenizer_create(
+            device.handle, width, height, ctypes.byref(handle)))
+        self.handle = handle.value
+        device._children.add(self)
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_tokenizer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_orders(self, used_order_mask: int = 0, orders: dict | None = None) -> None:
+        """orders[(family, channel)] -> uint32 array for every family in the
+        mask (channel 0 = X, 1 = Y, 2 = B)."""
+        table = (_P * (3 * ORDER_FAMILIES))()
+        keep = []
+        for (family, channel), order in (orders or {}).items():
+            a = np.ascontiguousarray(order, np.uint32)
+            keep.append(a)
+            table[3 * family + channel] = a.ctypes.data
+        _check(library().gjxl_amd_tokenizer_set_orders(
+            self.handle, used_order_mask, table if used_order_mask else None))
+
+    def set_context_map(self, thresholds, context_map, num_contexts: int) -> None:
+        thr = np.ascontiguousarray(np.array(thresholds, np.uint32).reshape(-1))
+        cmap = np.ascontiguousarray(context_map, np.uint8)
+        _check(library().gjxl_amd_tokenizer_set_context_map(
+            self.handle, thr.ctypes.data if thr.size else None, thr.size,
+            cmap.ctypes.data, num_contexts))
+        self.num_contexts = num_contexts
+
+    def tokenize(self, quantized: DeviceArray, cells: DeviceArray,
+                 raw_quant: DeviceArray, populations: bool = False) -> int:
+        n = ctypes.c_uint64()
+        _check(library().gjxl_amd_tokenize(
+            self.handle, quantized.ptr, cells.ptr, raw_quant.ptr,
+            1 if populations else 0, ctypes.byref(n)))
+        return n.value
+
+    def buffer(self, name: str, shape=None, dtype=None) -> DeviceArray:
+        ptr, size = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(library().gjxl_amd_tokenizer_buffer(
+            self.handle, TOKEN_BUFFERS.index(name), ctypes.byref(ptr),
+            ctypes.byref(size)))
+        if dtype is None:
+            dtype = {"contexts": np.uint16, "inverse_orders": np.uint16,
+                     "nonzero_map": np.uint8}.get(name, np.uint32)
+        dtype = np.dtype(dtype)
+        if shape is None:
+            shape = (size.value // dtype.itemsize,)
+        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value)
+        if view.nbytes > size.value:
+            raise ValueError(f"{name}: view of {view.nbytes} bytes exceeds {size.value}")
+        return view
+
+    def tokens(self, count: int) -> dict:
+        """The last stream on the host: values, contexts, group offsets."""
+        out = {"values": self.buffer("values", (count,)).numpy() if count else
+               np.zeros(0, np.uint32),
+               "contexts": self.buffer("contexts", (count,)).numpy() if count else
+               np.zeros(0, np.uint16),
+               "group_offsets": self.buffer("group_offsets", (self.groups + 1,)).numpy()}
+        return out
+
+    def histogram(self) -> np.ndarray:
+        rows = self.num_contexts * 495
+        return self.buffer("histogram", (rows, 128)).numpy()
 
 
 def gpu_available() -> bool:

@@ -40,77 +40,84 @@ __device__ __forceinline__ bool SearchedStrategy(int st) {
          st == kDct8x16 || st == kDct32x16 || st == kDct16x32;
 }
 
-// One thread per group walks its cells in raster order.  The grid is
-// validated on the way (every cell covered by exactly one transform of a
-// searched strategy, footprints inside the group): a group that fails gets
-// no anchors and raises *error, so that no later kernel indexes past the
-// group's 65536 coefficient slots.
-__global__ __launch_bounds__(64) void AnchorLayoutKernel(
+// One work group per AC group, one thread per cell of its 32x32 blocks.
+// The grid is validated on the way (every cell covered by exactly one
+// transform of a searched strategy, footprints inside the group): a group
+// that fails gets no anchors and raises *error, so that no later kernel
+// indexes past the group's 65536 coefficient slots.  Anchors are numbered in
+// raster order and their coefficient offsets are the prefix sums of the
+// transform sizes in that order (an exclusive scan over the group).
+// strategy_lists may be null; otherwise each group appends its anchors to
+// the per-strategy lists (one reservation per strategy and group; the order
+// within a list is not specified).
+__global__ __launch_bounds__(1024) void AnchorLayoutKernel(
   const uint8_t* __restrict__ cells, int bw, int bh,
   uint32_t* __restrict__ offsets, uint32_t* __restrict__ anchor_list,
   uint32_t* __restrict__ anchor_count, uint32_t* __restrict__ error,
   uint32_t* __restrict__ strategy_lists, size_t capacity,
   uint32_t* __restrict__ strategy_counts) {
-  __shared__ uint32_t covered[kGroupBlocks];  // one bit per cell of the group
+  __shared__ uint32_t cover[kGroupAnchors];
+  __shared__ uint32_t per[kStrategyStages], base[kStrategyStages];
+  __shared__ int bad;
   const int gw = (bw + kGroupBlocks - 1) / kGroupBlocks;
   const int g = blockIdx.x;
   const int gx0 = (g % gw) * kGroupBlocks, gy0 = (g / gw) * kGroupBlocks;
-  const int gx1 = min(bw, gx0 + kGroupBlocks), gy1 = min(bh, gy0 + kGroupBlocks);
-  if (threadIdx.x < kGroupBlocks) covered[threadIdx.x] = 0;
+  const int ew = min(kGroupBlocks, bw - gx0), eh = min(kGroupBlocks, bh - gy0);
+  const int t = threadIdx.x, lx = t & (kGroupBlocks - 1), ly = t / kGroupBlocks;
+  const bool inside = lx < ew && ly < eh;
+  const size_t cell_index = static_cast<size_t>(gy0 + ly) * bw + gx0 + lx;
+  cover[t] = 0;
+  if (t < kStrategyStages) per[t] = 0;
+  if (t == 0) bad = 0;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  uint32_t used = 0, count = 0;
-  bool valid = true;
-  for (int y = gy0; y < gy1 && valid; ++y) {
-    for (int x = gx0; x < gx1; ++x) {
-      const uint8_t c = cells[static_cast<size_t>(y) * bw + x];
-      const uint32_t bit = 1u << (x - gx0);
-      if (!(c & 1)) {
-        offsets[static_cast<size_t>(y) * bw + x] = kNoAnchor;
-        if (!(covered[y - gy0] & bit)) { valid = false; break; }
-        continue;
+  const uint8_t c = inside ? cells[cell_index] : 0;
+  const bool anchor = inside && (c & 1);
+  const int st = c >> 1;
+  int cw = 1, ch = 1;
+  if (anchor) {
+    if (!SearchedStrategy(st)) {
+      bad = 1;
+    } else {
+      cw = CoveredW(st);
+      ch = CoveredH(st);
+      if (lx + cw > ew || ly + ch > eh) {
+        bad = 1;
+      } else {
+        for (int dy = 0; dy < ch; ++dy)
+          for (int dx = 0; dx < cw; ++dx) atomicAdd(&cover[(ly + dy) * kGroupBlocks + lx + dx], 1u);
       }
-      const int st = c >> 1;
-      if (!SearchedStrategy(st)) { valid = false; break; }
-      const int cw = CoveredW(st), ch = CoveredH(st);
-      if (x + cw > gx1 || y + ch > gy1) { valid = false; break; }
-      const uint32_t span = ((1u << cw) - 1u) << (x - gx0);
-      for (int dy = 0; dy < ch; ++dy) {
-        if (covered[y - gy0 + dy] & span) valid = false;
-        covered[y - gy0 + dy] |= span;
-      }
-      if (!valid) break;
-      offsets[static_cast<size_t>(y) * bw + x] =
-        static_cast<uint32_t>(g) * kGroupCoefficients + used;
-      used += cw * ch * 64;
-      anchor_list[static_cast<size_t>(g) * kGroupAnchors + count] =
-        (static_cast<uint32_t>(y) << 16) | static_cast<uint32_t>(x);
-      ++count;
     }
   }
-  if (!valid) {
-    count = 0;
-    atomicOr(error, 1u);
+  __syncthreads();
+  if (inside && cover[t] != 1) bad = 1;
+  __syncthreads();
+  if (bad) {
+    if (inside) offsets[cell_index] = kNoAnchor;
+    if (t == 0) {
+      anchor_count[g] = 0;
+      atomicOr(error, 1u);
+    }
+    return;
   }
-  anchor_count[g] = count;
-  // the group's anchors appended to the per-strategy lists (one reservation
-  // per strategy and group)
-  uint32_t per[kStrategyStages] = {};
-  for (uint32_t k = 0; k < count; ++k) {
-    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
-    const uint8_t c = cells[static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff)];
-    ++per[StrategyStage(c >> 1)];
+  // anchors in the high bits, coefficient slots (at most 65536) in the low 20
+  uint32_t total;
+  const uint32_t mine = anchor ? (1u << 20) | static_cast<uint32_t>(cw * ch * 64) : 0u;
+  const uint32_t before = BlockExclusiveScanU32(mine, &total);
+  const uint32_t packed = (static_cast<uint32_t>(gy0 + ly) << 16) | static_cast<uint32_t>(gx0 + lx);
+  if (inside) {
+    offsets[cell_index] = anchor ? static_cast<uint32_t>(g) * kGroupCoefficients + (before & 0xfffffu)
+                                 : kNoAnchor;
   }
-  uint32_t base[kStrategyStages];
-  for (int st = 0; st < kStrategyStages; ++st) {
-    base[st] = per[st] ? atomicAdd(&strategy_counts[st], per[st]) : 0u;
-  }
-  for (uint32_t k = 0; k < count; ++k) {
-    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
-    const uint8_t c = cells[static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff)];
-    const int st = StrategyStage(c >> 1);
-    strategy_lists[static_cast<size_t>(st) * capacity + base[st]++] = packed;
-  }
+  if (anchor) anchor_list[static_cast<size_t>(g) * kGroupAnchors + (before >> 20)] = packed;
+  if (t == 0) anchor_count[g] = total >> 20;
+  if (strategy_lists == nullptr) return;
+  uint32_t rank = 0;
+  const int stage = StrategyStage(st);
+  if (anchor) rank = atomicAdd(&per[stage], 1u);
+  __syncthreads();
+  if (t < kStrategyStages) base[t] = per[t] ? atomicAdd(&strategy_counts[t], per[t]) : 0u;
+  __syncthreads();
+  if (anchor) strategy_lists[static_cast<size_t>(stage) * capacity + base[stage] + rank] = packed;
 }
 
 // ---- per-strategy transform kernels -----------------------------------------
@@ -1127,9 +1134,11 @@ GJXLResult LaunchAnchorLayout(hipStream_t s, const uint8_t* cells, int width,
                               uint32_t* strategy_counts) {
   const AqLayout l = MakeAqLayout(width, height);
   hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, error, 1, 0u);
-  hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, strategy_counts,
-                     kStrategyStages, 0u);
-  hipLaunchKernelGGL(AnchorLayoutKernel, dim3(l.groups), dim3(64), 0, s, cells,
+  if (strategy_lists != nullptr) {
+    hipLaunchKernelGGL(FillWordsKernel, dim3(1), dim3(64), 0, s, strategy_counts,
+                       kStrategyStages, 0u);
+  }
+  hipLaunchKernelGGL(AnchorLayoutKernel, dim3(l.groups), dim3(kGroupAnchors), 0, s, cells,
                      l.bw, l.bh, offsets, list, count, error, strategy_lists, capacity,
                      strategy_counts);
   return CheckLaunch("AnchorLayoutKernel");

@@ -138,6 +138,31 @@ __device__ inline void BlockReduceU32(uint32_t (&v)[K], Op op) {
   }
 }
 
+// Exclusive prefix sum of one word per thread over the work group (thread
+// order), with the group total in *total.  blockDim.x must be a multiple of
+// 64 and at most 1024; every thread of the group must call it.
+__device__ inline uint32_t BlockExclusiveScanU32(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wave_sums[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inclusive = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t other = __shfl_up(inclusive, off);
+    if (lane >= off) inclusive += other;
+  }
+  if (lane == 63) wave_sums[wave] = inclusive;
+  __syncthreads();
+  uint32_t before = 0, sum = 0;
+  const int waves = static_cast<int>(blockDim.x >> 6);
+  for (int w = 0; w < waves; ++w) {
+    const uint32_t s = wave_sums[w];
+    if (w < wave) before += s;
+    sum += s;
+  }
+  __syncthreads();  // wave_sums may be reused by the next call
+  *total = sum;
+  return before + inclusive - v;
+}
+
 struct U32Max {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a < b ? b : a; }
 };

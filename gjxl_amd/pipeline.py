@@ -13,7 +13,8 @@ from typing import Callable
 
 import numpy as np
 
-from . import AqSession, Butteraugli, Device, DeviceArray, library, pad8, reduce_distance_map
+from . import (AqSession, Butteraugli, Device, DeviceArray, Tokenizer, library, pad8,
+               reduce_distance_map)
 
 
 def run_frontend(dev: Device, srgb: np.ndarray, intensity_target: float = 255.0,
@@ -39,7 +40,8 @@ class PlainPath:
     run() takes the packed sRGB8 pixels already on the device and leaves the
     strategy cells, the final quant field, the block distances and the last
     evaluation's frame in the session (d1.0 e7: two AQ updates, three
-    evaluations)."""
+    evaluations), and that frame's AC tokens in the tokenizer (natural
+    orders, default block-context map)."""
 
     def __init__(self, dev: Device, width: int, height: int,
                  target: float = 1.0, iterations: int = 2):
@@ -58,7 +60,12 @@ class PlainPath:
         self.aq_scratch = dev.empty((nbytes,), np.uint8)
         self.session = AqSession(dev, width, height)
         self.butteraugli = Butteraugli(dev, width, height)
+        self.tokenizer = Tokenizer(dev, width, height)
+        self.token_count = 0
         s = self.session
+        store = ((s.bw + 31) // 32) * ((s.bh + 31) // 32) * 65536
+        self.quantized = s.buffer("quantized", (3, store), np.int32)
+        self.raw_quant = s.buffer("raw_quant", (s.bh, s.bw), np.int32)
         self.field = s.buffer("field", (s.bh, s.bw))
         self.adjusted = s.buffer("adjusted_field", (s.bh, s.bw))
         self.block_distance = s.buffer("block_distance", (s.bh, s.bw))
@@ -70,6 +77,7 @@ class PlainPath:
     def close(self) -> None:
         self.session.close()
         self.butteraugli.close()
+        self.tokenizer.close()
 
     def run(self, srgb: DeviceArray, bytes_per_pixel: int = 3,
             mark: Callable[[str], None] | None = None) -> None:
@@ -108,6 +116,10 @@ class PlainPath:
             if it < self.iterations:
                 self.session.update(self.field, self.block_distance, it, t)
             mark("aq_reduce_update")
+        # AC tokens of the last evaluation's frame (waits for the count)
+        self.token_count = self.tokenizer.tokenize(self.quantized, self.cells,
+                                                   self.raw_quant)
+        mark("ac_tokenize")
 
     def scores(self) -> list[float]:
         words = self.session.buffer("scores", (self.iterations + 1,), np.uint32).numpy()

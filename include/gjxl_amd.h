@@ -328,6 +328,94 @@ GJXL_AMD_API GJXLResult gjxl_amd_aq_run(GJXLAmdAqSession* session,
                                         float butteraugli_target,
                                         double* scores);
 
+/* ---------------------------------------------------------------------- */
+/* a19: AC tokenization of a quantized frame.  Replaces                      */
+/* AcTokenizationProvider::{Begin, Finish} (src/codestream/                  */
+/* ac_tokenization_provider_internal.h:18-31) over the per-group tokenizer   */
+/* TokenizeSimpleAcGroupDirectValidated (src/codestream/ac_group.cpp:        */
+/* 740-932): per transform and channel (Y, X, B) the non-LLF nonzero count   */
+/* with its predicted-nonzero context, then the zig-zag signed coefficients  */
+/* along the coefficient order up to the last nonzero with zero-density      */
+/* contexts.  The stream of every AC group is the reference's; the groups    */
+/* are concatenated in group order (GJXL_AMD_TOKENS_GROUP_OFFSETS).  Tokens  */
+/* are SoA: u32 values and u16 contexts (src/codestream/entropy.h:33-39).    */
+/* A tokenizer starts with the natural orders and the default block-context  */
+/* map (src/codestream/block_context_map.cpp:245-252).                       */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdTokenizer GJXLAmdTokenizer;
+
+enum {
+  GJXL_AMD_TOKENS_VALUES = 0,       /* u32 per token */
+  GJXL_AMD_TOKENS_CONTEXTS,         /* u16 per token */
+  GJXL_AMD_TOKENS_GROUP_OFFSETS,    /* u32 groups + 1: first token per group,
+                                       then the total */
+  GJXL_AMD_TOKENS_HISTOGRAM,        /* u32 (num_contexts * 495) x 128:
+                                       tokens per context and symbol */
+  GJXL_AMD_TOKENS_NONZERO_MAP,      /* u8 3 block planes: scaled nonzeros */
+  GJXL_AMD_TOKENS_TASKS,            /* u32 groups x 3072: nonzeros | tokens << 16
+                                       per (anchor, channel) in stream order */
+  GJXL_AMD_TOKENS_TASK_OFFSETS,     /* u32 groups x 3072: first token in group */
+  GJXL_AMD_TOKENS_GROUP_TOKENS,     /* u32 per group */
+  GJXL_AMD_TOKENS_OFFSETS,          /* u32 per block: anchor layout */
+  GJXL_AMD_TOKENS_ANCHOR_LIST,      /* u32 groups*1024: by << 16 | bx */
+  GJXL_AMD_TOKENS_ANCHOR_COUNT,     /* u32 per group */
+  GJXL_AMD_TOKENS_ORDERS,           /* u32 3 x 2624: scan -> coefficient */
+  GJXL_AMD_TOKENS_INVERSE_ORDERS,   /* u16 3 x 2624: coefficient -> scan */
+  GJXL_AMD_TOKENS_STATUS,           /* u32[4]: total tokens, error bits */
+  GJXL_AMD_TOKENS_BUFFER_COUNT
+};
+
+/* width, height: the source extent (blocks of its 8-padded extent). */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device,
+                                                  uint32_t width,
+                                                  uint32_t height,
+                                                  GJXLAmdTokenizer** out);
+GJXL_AMD_API void gjxl_amd_tokenizer_destroy(GJXLAmdTokenizer* tokenizer);
+
+/* ComputeSimpleNaturalCoefficientOrder (src/codestream/ac_group.cpp:381-487)
+ * on the host: scan position -> coefficient index of one of the seven
+ * searched strategies. */
+GJXL_AMD_API GJXLResult gjxl_amd_natural_order(int32_t strategy,
+                                               uint32_t* order,
+                                               size_t capacity, size_t* count);
+
+/* The coefficient orders of SimpleCoefficientOrders (src/codestream/
+ * coefficient_order.h:32-41): for each order family f set in
+ * used_order_mask, orders[3 * f + c] (host, channel c = X, Y, B) replaces the
+ * natural order; it must be a permutation with the LLF prefix, as
+ * ValidateSimpleCoefficientOrders requires.  orders may be null when the
+ * mask is 0. */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_set_orders(
+  GJXLAmdTokenizer* tokenizer, uint16_t used_order_mask,
+  const uint32_t* const* orders);
+
+/* A SimpleBlockContextMap (src/codestream/block_context_map.h): at most 15
+ * increasing quant-field thresholds in [1, 255], 3 * 13 * (thresholds + 1)
+ * labels, num_contexts in [1, 16] with every label used
+ * (ValidateSimpleBlockContextMap, src/codestream/block_context_map.cpp:
+ * 281-320). */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_set_context_map(
+  GJXLAmdTokenizer* tokenizer, const uint32_t* qf_thresholds,
+  uint32_t threshold_count, const uint8_t* context_map,
+  uint32_t num_contexts);
+
+/* Tokenizes a frame on the device: quantized AC in the per-group store
+ * layout (3 planes of groups * 65536 i32, as GJXL_AMD_AQ_QUANTIZED), the
+ * strategy cells and the raw quant field (i32 per block).  populations != 0
+ * also fills GJXL_AMD_TOKENS_HISTOGRAM (the sum of the groups' fixed-ANS
+ * populations, AppendDirectAcToken, ac_group.cpp:688-720).  Waits for the
+ * stream; *token_count receives the total. */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenize(GJXLAmdTokenizer* tokenizer,
+                                          const int32_t* quantized,
+                                          const uint8_t* cells,
+                                          const int32_t* raw_quant,
+                                          int32_t populations,
+                                          uint64_t* token_count);
+
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_buffer(GJXLAmdTokenizer* tokenizer,
+                                                  int32_t id, void** pointer,
+                                                  size_t* bytes);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -11,24 +11,9 @@ import pytest
 
 from oracle import binding as b
 
-COVERED = {0: (1, 1), 4: (2, 2), 5: (4, 4), 6: (1, 2), 7: (2, 1), 10: (2, 4), 11: (4, 2)}
+from tests.token_frames import COVERED, fnv_tokens, fnv_values, poison_llf
+
 FAMILY = {0: 0, 4: 2, 5: 3, 6: 4, 7: 4, 10: 6, 11: 6}
-MASK64 = (1 << 64) - 1
-
-
-def fnv_values(values):
-    h = 1469598103934665603
-    for v in values:
-        h = ((h ^ int(v)) * 1099511628211) & MASK64
-    return h
-
-
-def fnv_tokens(contexts, values):
-    h = 1469598103934665603
-    for c, v in zip(contexts, values):
-        h = ((h ^ int(c)) * 1099511628211) & MASK64
-        h = ((h ^ int(v)) * 1099511628211) & MASK64
-    return h
 
 
 def single(strategy, storage, orders=None, raw=1, **kw):
@@ -40,16 +25,6 @@ def single(strategy, storage, orders=None, raw=1, **kw):
     cells = np.full((ch, cw), strategy << 1, np.uint8)
     cells[0, 0] |= 1
     return b.tokenize_frame(q, cells, np.full((ch, cw), raw, np.int32), orders, **kw)
-
-
-def poison_llf(strategy, base):
-    cw, ch = COVERED[strategy]
-    lw, lh, W = max(cw, ch), min(cw, ch), max(cw, ch) * 8
-    v = np.zeros(cw * ch * 64, np.int32)
-    for y in range(lh):
-        for x in range(lw):
-            v[y * W + x] = base + y * lw + x
-    return v
 
 
 @pytest.mark.parametrize("strategy,size,digest,mid,mid_value", [
