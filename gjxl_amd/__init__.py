@@ -330,7 +330,7 @@ AQ_BUFFERS = (
     "offsets", "anchor_list", "anchor_count", "coefficients", "adjusted_field",
     "stats", "select", "quantizer", "raw_quant", "y_to_x", "y_to_b",
     "quantized", "dc", "dc_quantized", "dc_reconstructed",
-    "dc_smoothed", "inverse_sigma", "reconstructed", "scratch_a", "scratch_b",
+    "dc_smoothed", "inverse_sigma", "reconstructed",
     "filtered", "linear", "field", "block_distance", "scores", "strategy_lists",
 )
 

@@ -176,8 +176,17 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
       }
       __syncthreads();
       if (tid < 24 && !(sp == 0 ? done0 : done1)) {
+        // n is a multiple of 64: whole batches of 16 terms per lane, each
+        // batch's loads issued before its ordered adds
         const int limit = min(kCflThreads, n - chunk);
-        for (int j = slane; j < limit; j += 4) acc += terms[sp][skind][j];
+        const float* src = &terms[sp][skind][slane];
+        for (int base = 0; base < limit; base += 64) {
+          float v[16];
+#pragma unroll
+          for (int m = 0; m < 16; ++m) v[m] = src[base + 4 * m];
+#pragma unroll
+          for (int m = 0; m < 16; ++m) acc += v[m];
+        }
       }
       __syncthreads();
     }
@@ -222,7 +231,6 @@ __device__ __forceinline__ int CeilLog2Nonzero(int v) {
 
 constexpr int kCostThreads = 512;
 constexpr int kCostSlots = 3072;
-constexpr int kUnroll = 4;  // outputs per thread in the DCT passes
 
 template <int kStrategy>
 struct CostShape {
@@ -238,9 +246,6 @@ struct CostShape {
   static constexpr int kNx = (8 - kCw) / kStep + 1;
   static constexpr int kNy = (8 - kCh) / kStep + 1;
   static constexpr int kPositions = kNx * kNy;
-  // LDS copies of the bases: forward W, forward H, inverse W, inverse H
-  // (square transforms share the W and H copies)
-  static constexpr int kBasis = kW == kH ? 2 * kW * kW : 2 * (kW * kW + kH * kH);
 };
 
 // One work group: kCands anchor positions of one 64x64 tile (blockIdx.y),
@@ -256,10 +261,14 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   CostParams params, float* __restrict__ costs) {
   using S = CostShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
-  __shared__ float pix[K][3][N];     // pixels; residual; weighted loss terms
-  __shared__ double tmp[K][3][N];    // horizontal pass (forward, inverse)
-  __shared__ float coef[K][3][N];    // coefficients; then sqrt(|rounded|)
-  __shared__ double basis[S::kBasis];
+  constexpr int KC = 3 * K;     // candidate-channel planes
+  constexpr int TS = W + 1;     // padded row stride of the binary64 passes
+  // A: pixels (transposed, [x][y]); then sqrt(|rounded|) in storage order
+  // B: coefficients ([v][u]); residuals (transposed, [u][v]); loss terms
+  // T: binary64 rows of the separable passes, stride TS
+  __shared__ float A[KC * N];
+  __shared__ float B[KC * N];
+  __shared__ double T[KC * H * TS];
   __shared__ int nnz[K][3];
   __shared__ float sums[K][3][2];
   __shared__ float qn[K];
@@ -268,24 +277,18 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   __shared__ int bxs[K], bys[K];
 
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr int kWaves = kCostThreads / 64;
   const int bw = width / 8, bh = height / 8;
   const int tiles_w = (bw + 7) / 8;
   const int tx = blockIdx.y % tiles_w, ty = blockIdx.y / tiles_w;
   const size_t plane = static_cast<size_t>(width) * height;
-  double* fh = basis;
-  double* ih = basis + W * W;
-  double* fv = W == H ? fh : ih + W * W;
-  double* iv = W == H ? ih : fv + H * H;
-  for (int i = tid; i < W * W; i += kCostThreads) {
-    fh[i] = bases[BasisOffset(W, true) + i];
-    ih[i] = bases[BasisOffset(W, false) + i];
-  }
-  if (W != H) {
-    for (int i = tid; i < H * H; i += kCostThreads) {
-      fv[i] = bases[BasisOffset(H, true) + i];
-      iv[i] = bases[BasisOffset(H, false) + i];
-    }
-  }
+  // binary64 bases, read with uniform (scalar) loads
+  const double* fhT = bases + BasisOffsetT(W);        // [x][u]
+  const double* fvT = bases + BasisOffsetT(H);        // [y][v]
+  const double* ih = bases + BasisOffset(W, false);   // [u][x]
+  const double* iv = bases + BasisOffset(H, false);   // [v][y]
 
   if (tid < K) {
     const int p = blockIdx.x * K + tid;
@@ -328,76 +331,84 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   }
   __syncthreads();
 
-  // Load pixels (invalid candidates load zeros and are discarded).
-  for (int e = tid; e < K * 3 * N; e += kCostThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+  // Load pixels transposed (invalid candidates load zeros and are discarded).
+  for (int e = tid; e < KC * N; e += kCostThreads) {
+    const int kc = e / N, r = e % N;
+    const int k = kc / 3, c = kc % 3;
     const int y = r / W, x = r % W;
     float v = 0.0f;
     if (valid[k]) {
       v = opsin[c * plane + static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
     }
-    pix[k][c][r] = v;
+    A[kc * N + x * H + y] = v;
   }
   __syncthreads();
 
-  // Every pass computes kUnroll outputs per thread with independent binary64
-  // chains, each summed in the reference's order (src/codec/dct.cpp:135-233).
-  constexpr int kRowTasks = K * 3 * H * (W / kUnroll);
-  constexpr int kColTasks = K * 3 * W * (H / kUnroll);
-  // Forward horizontal: tmp[y][u] = sum_x pix[y][x] * fh[u][x].
-  for (int task = tid; task < kRowTasks; task += kCostThreads) {
-    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
-    const int kc = row / H, y = row % H;
-    const float* src = &pix[0][0][0] + kc * N + y * W;
-    double acc[kUnroll] = {};
-#pragma unroll 8
-    for (int x = 0; x < W; ++x) {
-      const double v = static_cast<double>(src[x]);
+  // The separable passes: a wave takes 64 rows (or columns) and one group of
+  // kOut outputs, so that the basis entries are the same for all its lanes;
+  // each output is one binary64 chain in the reference's order
+  // (src/codec/dct.cpp:135-233).
+  constexpr int kOut = 8;
+  constexpr int kRows = KC * H, kCols = KC * W;
+  constexpr int kRowBlocks = (kRows + 63) / 64, kColBlocks = (kCols + 63) / 64;
+  // Forward horizontal: T[y][u] = sum_x pix[y][x] * fh[u][x].
+  for (int task = wave; task < kRowBlocks * (W / kOut); task += kWaves) {
+    const int g = task % (W / kOut), r = (task / (W / kOut)) * 64 + lane;
+    if (r < kRows) {
+      const int kc = r / H, y = r % H;
+      const float* src = A + kc * N + y;
+      double acc[kOut] = {};
+#pragma unroll 4
+      for (int x = 0; x < W; ++x) {
+        const double v = static_cast<double>(src[x * H]);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += v * fh[(g * kUnroll + j) * W + x];
-    }
-    double* dst = &tmp[0][0][0] + kc * N + y * W + g * kUnroll;
+        for (int j = 0; j < kOut; ++j) acc[j] += v * fhT[x * W + g * kOut + j];
+      }
+      double* dst = T + (kc * H + y) * TS + g * kOut;
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) dst[j] = acc[j];
-  }
-  __syncthreads();
-  // Forward vertical: coef[idx(v,u)] = sum_y fv[v][y] * tmp[y][u].
-  for (int task = tid; task < kColTasks; task += kCostThreads) {
-    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
-    const int kc = col / W, u = col % W;
-    const double* src = &tmp[0][0][0] + kc * N + u;
-    double acc[kUnroll] = {};
-#pragma unroll 8
-    for (int y = 0; y < H; ++y) {
-      const double v = src[y * W];
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += fv[(g * kUnroll + j) * H + y] * v;
-    }
-    float* dst = &coef[0][0][0] + kc * N;
-#pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-      dst[CoefficientIndex<W, H>(g * kUnroll + j, u)] = static_cast<float>(acc[j]);
+      for (int j = 0; j < kOut; ++j) dst[j] = acc[j];
     }
   }
   __syncthreads();
-  // Quantise (src/codec/ac_strategy.cpp:914-926): residual into pix, and
-  // sqrt(|rounded|) into coef once every thread has read its coefficients.
-  constexpr int kPer = (K * 3 * N + kCostThreads - 1) / kCostThreads;
-  float root[kPer];
+  // Forward vertical: coef[v][u] = sum_y fv[v][y] * T[y][u].
+  for (int task = wave; task < kColBlocks * (H / kOut); task += kWaves) {
+    const int g = task % (H / kOut), col = (task / (H / kOut)) * 64 + lane;
+    if (col < kCols) {
+      const int kc = col / W, u = col % W;
+      const double* src = T + kc * H * TS + u;
+      double acc[kOut] = {};
+#pragma unroll 4
+      for (int y = 0; y < H; ++y) {
+        const double v = src[y * TS];
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) acc[j] += fvT[y * H + g * kOut + j] * v;
+      }
+#pragma unroll
+      for (int j = 0; j < kOut; ++j) B[kc * N + (g * kOut + j) * W + u] = static_cast<float>(acc[j]);
+    }
+  }
+  __syncthreads();
+  // Quantise (src/codec/ac_strategy.cpp:914-926): sqrt(|rounded|) into A in
+  // storage order; residuals staged in registers, then into B transposed.
+  constexpr int kPer = (KC * N + kCostThreads - 1) / kCostThreads;
+  float residual[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
-    root[j] = 0.0f;
-    if (e < K * 3 * N) {
-      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+    residual[j] = 0.0f;
+    if (e < KC * N) {
+      const int kc = e / N, rem = e % N;
+      const int k = kc / 3, c = kc % 3;
+      const int v = rem / W, u = rem % W;
+      const int i = CoefficientIndex<W, H>(v, u);
       const float* dq = tables + DequantOffset(kStrategy, c);
       const float* inv = tables + InverseOffset(kStrategy, c);
-      const float dec = coef[k][c][i] - coef[k][1][i] * cfl[k][c];
+      const float dec = B[e] - B[(k * 3 + 1) * N + rem] * cfl[k][c];
       const float scaled = dec * inv[i] * qn[k];
       const float rounded = roundf(scaled);
       const float diff = scaled - rounded;
-      pix[k][c][i] = dq[i] * diff;
-      root[j] = sqrtf(fabsf(rounded));
+      residual[j] = dq[i] * diff;
+      A[kc * N + i] = sqrtf(fabsf(rounded));
       if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
     }
   }
@@ -405,61 +416,77 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
-    if (e < K * 3 * N) (&coef[0][0][0])[e] = root[j];
-  }
-  // Inverse horizontal: tmp[v][x] = sum_u res[idx(v,u)] * ih[u][x].
-  for (int task = tid; task < kRowTasks; task += kCostThreads) {
-    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
-    const int kc = row / H, v = row % H;
-    const float* src = &pix[0][0][0] + kc * N;
-    double acc[kUnroll] = {};
-#pragma unroll 8
-    for (int u = 0; u < W; ++u) {
-      const double r = static_cast<double>(src[CoefficientIndex<W, H>(v, u)]);
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += r * ih[u * W + g * kUnroll + j];
+    if (e < KC * N) {
+      const int kc = e / N, rem = e % N;
+      const int v = rem / W, u = rem % W;
+      B[kc * N + u * H + v] = residual[j];
     }
-    double* dst = &tmp[0][0][0] + kc * N + v * W + g * kUnroll;
-#pragma unroll
-    for (int j = 0; j < kUnroll; ++j) dst[j] = acc[j];
   }
   __syncthreads();
-  // Inverse vertical and the masked 8th power (:935-952) into pix.
-  constexpr float kMaskOffset[3] = {12.0f, 0.0f, 4.0f};
-  for (int task = tid; task < kColTasks; task += kCostThreads) {
-    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
-    const int kc = col / W, x = col % W;
-    const int k = kc / 3, c = kc % 3;
-    const double* src = &tmp[0][0][0] + kc * N + x;
-    double acc[kUnroll] = {};
-#pragma unroll 8
-    for (int v = 0; v < H; ++v) {
-      const double t = src[v * W];
+  // Inverse horizontal: T[v][x] = sum_u res(v, u) * ih[u][x].
+  for (int task = wave; task < kRowBlocks * (W / kOut); task += kWaves) {
+    const int g = task % (W / kOut), r = (task / (W / kOut)) * 64 + lane;
+    if (r < kRows) {
+      const int kc = r / H, v = r % H;
+      const float* src = B + kc * N + v;
+      double acc[kOut] = {};
+#pragma unroll 4
+      for (int u = 0; u < W; ++u) {
+        const double t = static_cast<double>(src[u * H]);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += iv[v * H + g * kUnroll + j] * t;
-    }
-#pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-      const int y = g * kUnroll + j;
-      float weighted = 0.0f;
-      if (valid[k]) {
-        const float m =
-          pixel_mask[static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
-        weighted = (m + kMaskOffset[c]) * static_cast<float>(acc[j]);
-        weighted *= weighted;
-        weighted *= weighted;
-        weighted *= weighted;
+        for (int j = 0; j < kOut; ++j) acc[j] += t * ih[u * W + g * kOut + j];
       }
-      pix[k][c][y * W + x] = weighted;
+      double* dst = T + (kc * H + v) * TS + g * kOut;
+#pragma unroll
+      for (int j = 0; j < kOut; ++j) dst[j] = acc[j];
+    }
+  }
+  __syncthreads();
+  // Inverse vertical and the masked 8th power (:935-952) into B, raster.
+  constexpr float kMaskOffset[3] = {12.0f, 0.0f, 4.0f};
+  for (int task = wave; task < kColBlocks * (H / kOut); task += kWaves) {
+    const int g = task % (H / kOut), col = (task / (H / kOut)) * 64 + lane;
+    if (col < kCols) {
+      const int kc = col / W, x = col % W;
+      const int k = kc / 3, c = kc % 3;
+      const double* src = T + kc * H * TS + x;
+      double acc[kOut] = {};
+#pragma unroll 4
+      for (int v = 0; v < H; ++v) {
+        const double t = src[v * TS];
+#pragma unroll
+        for (int j = 0; j < kOut; ++j) acc[j] += iv[v * H + g * kOut + j] * t;
+      }
+#pragma unroll
+      for (int j = 0; j < kOut; ++j) {
+        const int y = g * kOut + j;
+        float weighted = 0.0f;
+        if (valid[k]) {
+          const float m =
+            pixel_mask[static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
+          weighted = (m + kMaskOffset[c]) * static_cast<float>(acc[j]);
+          weighted *= weighted;
+          weighted *= weighted;
+          weighted *= weighted;
+        }
+        B[kc * N + y * W + x] = weighted;
+      }
     }
   }
   __syncthreads();
   // Ordered sums, one thread each.
   if (tid < K * 6) {
     const int k = tid / 6, c = (tid / 2) % 3, kind = tid % 2;
-    const float* src = kind == 0 ? coef[k][c] : pix[k][c];
+    const float* src = (kind == 0 ? A : B) + (k * 3 + c) * N;
     float s = 0.0f;
-    for (int i = 0; i < N; ++i) s += src[i];
+    // batches of 16: the loads of a batch issue before its ordered adds
+    for (int base = 0; base < N; base += 16) {
+      float v[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = src[base + m];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) s += v[m];
+    }
     sums[k][c][kind] = s;
   }
   __syncthreads();

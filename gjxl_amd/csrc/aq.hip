@@ -123,21 +123,16 @@ __global__ __launch_bounds__(1024) void AnchorLayoutKernel(
 // ---- per-strategy transform kernels -----------------------------------------
 //
 // Each launch handles the transforms of one strategy (from its list), kK of
-// them per work group, with compile-time extents.  Their separable binary64
-// DCT passes give each thread kUnroll outputs with independent chains, each
-// summed in the reference's order (src/codec/dct.cpp:135-233), with the
-// bases in LDS.
+// them per work group, with compile-time extents.
 
 constexpr int kAqThreads = 256;
 constexpr int kSlots = 3072;
-constexpr int kUnroll = 4;
 
 template <int kStrategy>
 struct AnchorShape {
   static constexpr int kCw = CoveredW(kStrategy), kCh = CoveredH(kStrategy);
   static constexpr int kW = kCw * 8, kH = kCh * 8, kN = kW * kH;
   static constexpr int kK = kSlots / (3 * kN);
-  static constexpr int kBasis = kW == kH ? kW * kW : kW * kW + kH * kH;
 };
 
 struct TransformSlot {
@@ -166,107 +161,115 @@ __device__ __forceinline__ void LoadSlots(const uint32_t* __restrict__ entries,
   }
 }
 
-// Loads the W and H bases of one direction into LDS (shared when square).
-template <int W, int H>
-__device__ __forceinline__ void LoadBases(const double* __restrict__ bases, bool forward,
-                                          double* lds, const double** bw_out,
-                                          const double** bh_out) {
-  for (int i = threadIdx.x; i < W * W; i += blockDim.x) lds[i] = bases[BasisOffset(W, forward) + i];
-  if (W != H) {
-    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
-      lds[W * W + i] = bases[BasisOffset(H, forward) + i];
-    }
-  }
-  *bw_out = lds;
-  *bh_out = W == H ? lds : lds + W * W;
+// The separable binary64 passes.  A wave takes 64 rows (or columns) of the
+// kPlanes planes and one group of kOut outputs, so that the basis entries,
+// read from global memory, are the same for all its lanes (scalar loads);
+// each output is one chain in the reference's order (src/codec/dct.cpp:
+// 135-233).  Row passes write binary64 rows of stride W + 1 (padded against
+// LDS bank conflicts).
+constexpr int kOut = 8;
+
+__device__ __forceinline__ int WaveIndex() {
+  return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
 }
 
-// Forward rows: dst[p][y][u] = sum_x src[p][y][x] * fh[u][x].
+// Forward rows: dst[p][y][u] = sum_x src[p][x][y] * fh[u][x]; src holds the
+// pixels transposed ([p][x][y]); fhT is the transposed forward basis.
 template <int W, int H, int kPlanes>
-__device__ __forceinline__ void ForwardRows(const double* fh, const float* src, double* dst) {
-  constexpr int N = W * H;
-  constexpr int kTasks = kPlanes * H * (W / kUnroll);
-  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
-    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
-    const float* s = src + (row / H) * N + (row % H) * W;
-    double acc[kUnroll] = {};
-#pragma unroll 8
+__device__ __forceinline__ void ForwardRows(const double* __restrict__ fhT, const float* src,
+                                            double* dst) {
+  constexpr int N = W * H, TS = W + 1, kRows = kPlanes * H, kBlocks = (kRows + 63) / 64;
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (int task = WaveIndex(); task < kBlocks * (W / kOut); task += waves) {
+    const int g = task % (W / kOut), row = (task / (W / kOut)) * 64 + lane;
+    if (row >= kRows) continue;
+    const int p = row / H, y = row % H;
+    const float* s = src + p * N + y;
+    double acc[kOut] = {};
+#pragma unroll 4
     for (int x = 0; x < W; ++x) {
-      const double v = static_cast<double>(s[x]);
+      const double v = static_cast<double>(s[x * H]);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += v * fh[(g * kUnroll + j) * W + x];
+      for (int j = 0; j < kOut; ++j) acc[j] += v * fhT[x * W + g * kOut + j];
     }
-    double* d = dst + (row / H) * N + (row % H) * W + g * kUnroll;
+    double* d = dst + row * TS + g * kOut;
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) d[j] = acc[j];
+    for (int j = 0; j < kOut; ++j) d[j] = acc[j];
   }
 }
 
 // Forward columns: coefficient (v, u) of plane p = sum_y fv[v][y] *
-// src[p][y][u], handed to store(p, libjxl index, value).
+// src[p][y][u], handed to store(p, libjxl index, value); fvT is the
+// transposed forward basis.
 template <int W, int H, int kPlanes, typename Store>
-__device__ __forceinline__ void ForwardCols(const double* fv, const double* src, Store store) {
-  constexpr int N = W * H;
-  constexpr int kTasks = kPlanes * W * (H / kUnroll);
-  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
-    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+__device__ __forceinline__ void ForwardCols(const double* __restrict__ fvT, const double* src,
+                                            Store store) {
+  constexpr int TS = W + 1, kCols = kPlanes * W, kBlocks = (kCols + 63) / 64;
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (int task = WaveIndex(); task < kBlocks * (H / kOut); task += waves) {
+    const int g = task % (H / kOut), col = (task / (H / kOut)) * 64 + lane;
+    if (col >= kCols) continue;
     const int p = col / W, u = col % W;
-    const double* s = src + p * N + u;
-    double acc[kUnroll] = {};
-#pragma unroll 8
+    const double* s = src + p * H * TS + u;
+    double acc[kOut] = {};
+#pragma unroll 4
     for (int y = 0; y < H; ++y) {
-      const double v = s[y * W];
+      const double v = s[y * TS];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += fv[(g * kUnroll + j) * H + y] * v;
+      for (int j = 0; j < kOut; ++j) acc[j] += fvT[y * H + g * kOut + j] * v;
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) {
-      store(p, CoefIndex(W, H, g * kUnroll + j, u), static_cast<float>(acc[j]));
+    for (int j = 0; j < kOut; ++j) {
+      store(p, CoefIndex(W, H, g * kOut + j, u), static_cast<float>(acc[j]));
     }
   }
 }
 
 // Inverse rows: dst[p][v][x] = sum_u src[p][idx(v, u)] * ih[u][x].
 template <int W, int H, int kPlanes>
-__device__ __forceinline__ void InverseRows(const double* ih, const float* src, double* dst) {
-  constexpr int N = W * H;
-  constexpr int kTasks = kPlanes * H * (W / kUnroll);
-  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
-    const int g = task % (W / kUnroll), row = task / (W / kUnroll);
+__device__ __forceinline__ void InverseRows(const double* __restrict__ ih, const float* src,
+                                            double* dst) {
+  constexpr int N = W * H, TS = W + 1, kRows = kPlanes * H, kBlocks = (kRows + 63) / 64;
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (int task = WaveIndex(); task < kBlocks * (W / kOut); task += waves) {
+    const int g = task % (W / kOut), row = (task / (W / kOut)) * 64 + lane;
+    if (row >= kRows) continue;
     const int p = row / H, v = row % H;
     const float* s = src + p * N;
-    double acc[kUnroll] = {};
-#pragma unroll 8
+    double acc[kOut] = {};
+#pragma unroll 4
     for (int u = 0; u < W; ++u) {
       const double c = static_cast<double>(s[CoefIndex(W, H, v, u)]);
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += c * ih[u * W + g * kUnroll + j];
+      for (int j = 0; j < kOut; ++j) acc[j] += c * ih[u * W + g * kOut + j];
     }
-    double* d = dst + p * N + v * W + g * kUnroll;
+    double* d = dst + row * TS + g * kOut;
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) d[j] = acc[j];
+    for (int j = 0; j < kOut; ++j) d[j] = acc[j];
   }
 }
 
 // Inverse columns: pixel (y, x) of plane p = sum_v iv[v][y] * src[p][v][x],
 // handed to store(p, y, x, value).
 template <int W, int H, int kPlanes, typename Store>
-__device__ __forceinline__ void InverseCols(const double* iv, const double* src, Store store) {
-  constexpr int N = W * H;
-  constexpr int kTasks = kPlanes * W * (H / kUnroll);
-  for (int task = threadIdx.x; task < kTasks; task += blockDim.x) {
-    const int g = task % (H / kUnroll), col = task / (H / kUnroll);
+__device__ __forceinline__ void InverseCols(const double* __restrict__ iv, const double* src,
+                                            Store store) {
+  constexpr int TS = W + 1, kCols = kPlanes * W, kBlocks = (kCols + 63) / 64;
+  const int lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+  for (int task = WaveIndex(); task < kBlocks * (H / kOut); task += waves) {
+    const int g = task % (H / kOut), col = (task / (H / kOut)) * 64 + lane;
+    if (col >= kCols) continue;
     const int p = col / W, x = col % W;
-    const double* s = src + p * N + x;
-    double acc[kUnroll] = {};
-#pragma unroll 8
+    const double* s = src + p * H * TS + x;
+    double acc[kOut] = {};
+#pragma unroll 4
     for (int v = 0; v < H; ++v) {
-      const double t = s[v * W];
+      const double t = s[v * TS];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) acc[j] += iv[v * H + g * kUnroll + j] * t;
+      for (int j = 0; j < kOut; ++j) acc[j] += iv[v * H + g * kOut + j] * t;
     }
 #pragma unroll
-    for (int j = 0; j < kUnroll; ++j) store(p, g * kUnroll + j, x, static_cast<float>(acc[j]));
+    for (int j = 0; j < kOut; ++j) store(p, g * kOut + j, x, static_cast<float>(acc[j]));
   }
 }
 
@@ -280,26 +283,23 @@ __global__ __launch_bounds__(kAqThreads) void AnchorDctKernel(
   size_t store, float* __restrict__ coefs) {
   using S = AnchorShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
-  __shared__ float pix[K * 3 * N];
-  __shared__ double tmp[K * 3 * N];
-  __shared__ double basis[S::kBasis];
+  __shared__ float pix[K * 3 * N];            // transposed: [p][x][y]
+  __shared__ double tmp[K * 3 * H * (W + 1)];
   __shared__ TransformSlot slots[K];
-  const double *fh, *fv;
-  LoadBases<W, H>(bases, true, basis, &fh, &fv);
   LoadSlots<K>(entries, count, offsets, width / 8, slots);
   __syncthreads();
   const size_t plane = static_cast<size_t>(width) * height;
   for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, r = e % N;
+    const int p = e / N, k = p / 3, c = p % 3, r = e % N;
+    const int y = r / W, x = r % W;
     const TransformSlot& t = slots[k];
-    pix[e] = t.ok ? opsin[c * plane + static_cast<size_t>(t.by * 8 + r / W) * width +
-                          t.bx * 8 + r % W]
-                  : 0.0f;
+    pix[p * N + x * H + y] =
+      t.ok ? opsin[c * plane + static_cast<size_t>(t.by * 8 + y) * width + t.bx * 8 + x] : 0.0f;
   }
   __syncthreads();
-  ForwardRows<W, H, 3 * K>(fh, pix, tmp);
+  ForwardRows<W, H, 3 * K>(bases + BasisOffsetT(W), pix, tmp);
   __syncthreads();
-  ForwardCols<W, H, 3 * K>(fv, tmp, [&](int p, int i, float v) {
+  ForwardCols<W, H, 3 * K>(bases + BasisOffsetT(H), tmp, [&](int p, int i, float v) {
     const TransformSlot& t = slots[p / 3];
     if (t.ok) coefs[(p % 3) * store + t.offset + i] = v;
   });
@@ -476,55 +476,93 @@ __global__ __launch_bounds__(256) void RawQuantKernel(
 }
 
 // ---- final CfL, fast least squares (chroma_from_luma.cpp:543-670, :112-
-// 134), one work group per 64x64 colour tile.  Sixteen threads run the
+// 134), one work group per 64x64 colour tile.  The work group first forms
+// every term of the tile's anchors (raster order, each transform's
+// coefficients in storage order) in LDS; then sixteen threads run the
 // reference's four-lane fma chains (2 maps x {quadratic, linear} x 4 lanes)
-// over the tile's anchors in raster order.
+// over them.
 
-__global__ __launch_bounds__(64) void FinalCflKernel(
+constexpr int kCflTerms = 64 * 64;  // coefficients of one tile
+
+__global__ __launch_bounds__(256) void FinalCflKernel(
   const float* __restrict__ coefs, size_t store,
   const uint8_t* __restrict__ cells, const uint32_t* __restrict__ offsets,
   int bw, int bh, const int32_t* __restrict__ raw,
   const QuantizerState* __restrict__ q, const float* __restrict__ tables,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
+  __shared__ float terms[2][2][kCflTerms];  // [map][a, b][position]
   __shared__ float lanes[2][2][4];
-  __shared__ int total;
+  __shared__ uint32_t anchor_block[64], anchor_base[64];
+  __shared__ int anchors, total;
   const int tx = blockIdx.x, ty = blockIdx.y;
   const int t = threadIdx.x;
-  if (t == 0) total = 0;
+  // the tile's anchors in raster order with their term offsets
+  if (t < 64) {
+    const int bx = tx * 8 + (t & 7), by = ty * 8 + (t >> 3);
+    uint32_t n = 0;
+    if (bx < bw && by < bh) {
+      const uint8_t c = cells[static_cast<size_t>(by) * bw + bx];
+      if (c & 1) n = static_cast<uint32_t>(CoveredW(c >> 1) * CoveredH(c >> 1) * 64);
+    }
+    uint32_t base = n;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t other = __shfl_up(base, off);
+      if (t >= off) base += other;
+    }
+    base -= n;  // exclusive
+    // anchors whose terms fit (all of them in a valid grid) form a prefix
+    const bool fits = n != 0 && base + n <= kCflTerms;
+    const uint64_t kept = __ballot(fits);
+    const int rank = __popcll(kept & ((uint64_t{1} << t) - 1));
+    if (fits) {
+      anchor_block[rank] = (static_cast<uint32_t>(by) << 16) | static_cast<uint32_t>(bx);
+      anchor_base[rank] = base;
+    }
+    if (t == 63) {
+      anchors = __popcll(kept);
+      total = static_cast<int>(base + n);
+    }
+  }
+  __syncthreads();
+  const float scale128 = q->scale * 128.0f;
+  const int count = anchors, terms_used = min(total, kCflTerms);
+  for (int p = t; p < terms_used && count > 0; p += blockDim.x) {
+    int lo = 0, hi = count - 1;  // the last anchor starting at or before p
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (anchor_base[mid] <= static_cast<uint32_t>(p)) lo = mid; else hi = mid - 1;
+    }
+    const int bx = anchor_block[lo] & 0xffff, by = anchor_block[lo] >> 16;
+    const int st = cells[static_cast<size_t>(by) * bw + bx] >> 1;
+    const int cw = CoveredW(st), ch = CoveredH(st);
+    const int W = cw * 8, H = ch * 8, N = W * H;
+    const int i = p - static_cast<int>(anchor_base[lo]);
+    if (i >= N) continue;
+    const int cols = W > H ? W : H;
+    const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
+    const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
+    const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
+    const int xx = i % cols, yy = i / cols;
+    const bool llf = xx < lw && yy < lh;
+    const float cy = llf ? 0.0f : coefs[store + off + i];
+#pragma unroll
+    for (int problem = 0; problem < 2; ++problem) {
+      const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
+      const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
+      const float luma = cy * inv[i] * qs;
+      const float chroma = cc * inv[i] * qs;
+      terms[problem][0][p] = luma / 84.0f;
+      terms[problem][1][p] = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
+    }
+  }
   __syncthreads();
   if (t < 16) {
     const int problem = t / 8, kind = (t / 4) % 2, lane = t % 4;
+    const float* a = terms[problem][0];
+    const float* b = terms[problem][kind == 0 ? 0 : 1];
     float acc = 0.0f;
-    int n_total = 0;
-    const float scale128 = q->scale * 128.0f;
-    for (int by = ty * 8; by < min(bh, ty * 8 + 8); ++by) {
-      for (int bx = tx * 8; bx < min(bw, tx * 8 + 8); ++bx) {
-        const uint8_t c = cells[static_cast<size_t>(by) * bw + bx];
-        if (!(c & 1)) continue;
-        const int st = c >> 1;
-        const int cw = CoveredW(st), ch = CoveredH(st);
-        const int W = cw * 8, H = ch * 8, N = W * H;
-        const int cols = W > H ? W : H;
-        const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
-        const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
-        const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
-        const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
-        for (int i = lane; i < N; i += 4) {
-          const int xx = i % cols, yy = i / cols;
-          const bool llf = xx < lw && yy < lh;
-          const float cy = llf ? 0.0f : coefs[store + off + i];
-          const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
-          const float luma = cy * inv[i] * qs;
-          const float chroma = cc * inv[i] * qs;
-          const float a = luma / 84.0f;
-          const float b = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
-          acc = kind == 0 ? __builtin_fmaf(a, a, acc) : __builtin_fmaf(a, b, acc);
-        }
-        n_total += N;
-      }
-    }
+    for (int p = lane; p < terms_used; p += 4) acc = __builtin_fmaf(a[p], b[p], acc);
     lanes[problem][kind][lane] = acc;
-    if (t == 0) total = n_total;
   }
   __syncthreads();
   if (t < 2) {
@@ -555,40 +593,71 @@ struct CodingParams {
   DcBasisTable dc_basis;
 };
 
-// AdjustQuantForChannel (src/codec/quantization.cpp:292-533), one thread.
-__device__ void AdjustQuantForChannel(float qscale, int channel, float mult,
-                                      int strategy, int cols, int rows,
-                                      const float* inv, const float* coef,
-                                      float* thr, int32_t* raw_quant) {
+// AdjustQuantForChannel (src/codec/quantization.cpp:292-533) in three
+// parts: the quadrant thresholds, the per-coefficient terms of its scan
+// (formed in parallel: the quantisation error and |quantized|, zero in the
+// LLF corner the scan skips), and the ordered sums of the scan with the
+// decision that follows (one thread per channel, in the scan's raster order;
+// adding a zero term to these non-negative float sums changes nothing).
+
+// Quadrant thresholds before the scan.
+__device__ __forceinline__ void AdjustThresholds(int cols, int rows, float thr[4]) {
+  thr[0] = 0.58f;
+  thr[1] = thr[2] = thr[3] = 0.64f;
   const int xsize = cols / 8, ysize = rows / 8;
-  const float qac = qscale * static_cast<float>(*raw_quant);
   if (xsize > 1 || ysize > 1) {
     float reduction = 0.003f * static_cast<float>(xsize * ysize);
     reduction = reduction < 0.0f ? 0.0f : (0.08f < reduction ? 0.08f : reduction);
     for (int i = 0; i < 4; ++i) thr[i] = DStdMax(0.54f, thr[i] - reduction);
   }
-  float border = 0.0f, err_sum = 0.0f, val_sum = 0.0f;
+}
+
+__device__ __forceinline__ float AdjustQuantized(float value, float threshold) {
+  return fabsf(value) < threshold ? 0.0f : rintf(value);
+}
+
+// The scan of one channel over its terms (error, |quantized|) and the
+// decision.  coef and inv give the border terms again.
+template <int cols, int rows>
+__device__ void AdjustQuantFromTerms(const float2* __restrict__ terms,
+                                     const float* __restrict__ coef,
+                                     const float* __restrict__ inv, float qac,
+                                     float mult, int channel, int strategy, float thr[4],
+                                     int32_t* raw_quant) {
+  constexpr int xsize = cols / 8, ysize = rows / 8;
+  float err_sum = 0.0f, val_sum = 0.0f;
   float nz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int y = 0; y < rows; ++y) {
-    for (int x = 0; x < cols; ++x) {
-      if (x < xsize && y < ysize) continue;
-      const int index = y * cols + x;
-      const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
-      const float value = coef[index] * (inv[index] * qac * mult);
-      const float quantized = fabsf(value) < thr[quadrant] ? 0.0f : rintf(value);
-      const float error = fabsf(value - quantized);
-      err_sum += error;
-      val_sum += fabsf(quantized);
-      if (channel == 1 && quantized == 0.0f) mx[quadrant] = DStdMax(mx[quadrant], error);
-      if (quantized != 0.0f) {
-        nz[quadrant] += fabsf(quantized);
-        const bool in_corner = y >= 7 * ysize && x >= 7 * xsize;
-        const bool on_border = y + 1 == rows || x + 1 == cols;
-        const bool in_larger = x >= 4 * xsize && y >= 4 * ysize;
-        if (in_corner || (on_border && in_larger)) border += fabsf(value);
+#pragma unroll
+  for (int yh = 0; yh < 2; ++yh) {
+    for (int y = yh * rows / 2; y < (yh + 1) * rows / 2; ++y) {
+#pragma unroll
+      for (int xh = 0; xh < 2; ++xh) {
+        const int qd = yh * 2 + xh;
+#pragma unroll 8
+        for (int x = xh * cols / 2; x < (xh + 1) * cols / 2; ++x) {
+          const float2 t = terms[y * cols + x];
+          err_sum += t.x;
+          val_sum += t.y;
+          nz[qd] += t.y;
+          if (channel == 1) mx[qd] = DStdMax(mx[qd], t.y == 0.0f ? t.x : 0.0f);
+        }
       }
     }
   }
+  // border terms in raster order: x = cols - 1 on rows [4 ysize, 7 ysize),
+  // x >= 7 xsize on rows [7 ysize, rows - 1), x >= 4 xsize on the last row
+  float border = 0.0f;
+  auto add_border = [&](int y, int x) {
+    const int index = y * cols + x;
+    const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
+    const float value = coef[index] * (inv[index] * qac * mult);
+    if (AdjustQuantized(value, thr[quadrant]) != 0.0f) border += fabsf(value);
+  };
+  for (int y = 4 * ysize; y < 7 * ysize; ++y) add_border(y, cols - 1);
+  for (int y = 7 * ysize; y < rows - 1; ++y)
+    for (int x = 7 * xsize; x < cols; ++x) add_border(y, x);
+  for (int x = 4 * xsize; x < cols; ++x) add_border(rows - 1, x);
+
   if (channel == 1 && val_sum * 8.0f < static_cast<float>(xsize * ysize)) {
     const double limit = 0.46, multiplier = 0.9999;
     const int32_t original = *raw_quant;
@@ -660,8 +729,9 @@ __device__ void AdjustQuantForChannel(float qscale, int channel, float mult,
   *raw_quant = adjusted;
 }
 
-// One launch per strategy: kK transforms per work group.  Three threads per
-// transform run AdjustQuantForChannel, one thread per transform decides the
+// One launch per strategy: kK transforms per work group.  All threads form
+// the terms of the AdjustQuantForChannel scans, three threads per transform
+// run their ordered sums and decisions, one thread per transform decides the
 // shared raw quant and converts the DC values, and all threads quantise.
 template <int kStrategy>
 __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
@@ -675,6 +745,7 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
   constexpr int cols = W > H ? W : H, rows = W > H ? H : W;
   __shared__ float coef[K][3][N];
+  __shared__ float2 terms[K][3][N];
   __shared__ float thr_y[K][4];
   __shared__ int32_t cand[K][3];
   __shared__ int32_t decided[K];
@@ -690,14 +761,39 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
     coef[k][c][i] = t.ok ? coefs[c * store + t.offset + i] : 0.0f;
   }
   __syncthreads();
+  // terms of the AdjustQuantForChannel scans
+  {
+    float thr0[4];
+    AdjustThresholds(cols, rows, thr0);
+    for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
+      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+      const TransformSlot& t = slots[k];
+      const int x = i % cols, y = i / cols;
+      float2 term = make_float2(0.0f, 0.0f);
+      if (t.ok && !(x < cols / 8 && y < rows / 8)) {
+        const float qac =
+          q.scale * static_cast<float>(raw[static_cast<size_t>(t.by) * bw + t.bx]);
+        const float* inv = tables + InverseOffset(kStrategy, c);
+        const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
+        const float value = coef[k][c][i] * (inv[i] * qac * mults[c]);
+        const float quantized = AdjustQuantized(value, thr0[quadrant]);
+        term = make_float2(fabsf(value - quantized), fabsf(quantized));
+      }
+      terms[k][c][i] = term;
+    }
+  }
+  __syncthreads();
   if (threadIdx.x < 3 * K) {
     const int k = threadIdx.x / 3, c = threadIdx.x % 3;
     const TransformSlot& t = slots[k];
     if (t.ok) {
-      float thr[4] = {0.58f, 0.64f, 0.64f, 0.64f};
+      float thr[4];
+      AdjustThresholds(cols, rows, thr);
       int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
-      AdjustQuantForChannel(q.scale, c, mults[c], kStrategy, cols, rows,
-                            tables + InverseOffset(kStrategy, c), coef[k][c], thr, &r);
+      const float qac = q.scale * static_cast<float>(r);
+      AdjustQuantFromTerms<cols, rows>(terms[k][c], coef[k][c],
+                                       tables + InverseOffset(kStrategy, c), qac, mults[c],
+                                       c, kStrategy, thr, &r);
       cand[k][c] = r;
       if (c == 1) {
         for (int i = 0; i < 4; ++i) thr_y[k][i] = thr[i];
@@ -819,121 +915,158 @@ __device__ __forceinline__ int BitWidth64(uint64_t v) {
   return v == 0 ? 0 : 64 - __clzll(static_cast<long long>(v));
 }
 
-// grid: (DC groups x, DC groups y, channels); channels are `first_channel`
-// for z = 0 and 2 for z = 1 (X and B both follow Y).
+// The predictor state a row keeps in registers: its own last two samples.
+struct DcRowState {
+  int32_t q1 = 0, err1 = 0;
+  uint32_t e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+};
+
+// One sample (x, y) of one channel: weighted prediction from the ring and
+// the row state, quantisation of the residual, update of the ring slot and
+// the row state.  Returns the quantized DC value.
+__device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
+                                            const uint32_t* reciprocal, int x, int y,
+                                            bool has_ne, float value, float inv) {
+  constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
+  constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
+  const int s0 = x & (kRing - 1);
+  const int sm = (x - 1) & (kRing - 1), sp = (x + 1) & (kRing - 1);
+  // quantized neighbours
+  const int64_t up = y ? ring.q[s0][y - 1] : 0;
+  const int64_t W_ = x ? st.q1 : up;
+  const int64_t N_ = y ? up : W_;
+  const int64_t NW_ = x && y ? ring.q[sm][y - 1] : W_;
+  const int64_t NE_ = y && has_ne ? ring.q[sp][y - 1] : N_;
+  const int64_t NN_ = y > 1 ? ring.q[s0][y - 2] : N_;
+  // Predict (prediction.h:75-123)
+  uint32_t weights[4];
+  uint32_t weight_sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t en_i = y ? ring.e[i][s0][y - 1] : 0u;
+    const uint32_t north = en_i + (x ? st.e1[i] : 0u);
+    const uint32_t ne = has_ne ? (y ? ring.e[i][sp][y - 1] : 0u) : north;
+    const uint32_t nw =
+      x ? (y ? ring.e[i][sm][y - 1] : 0u) + (x > 1 ? st.e2[i] : 0u) : north;
+    const uint32_t e = north + ne + nw;
+    int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
+    if (shift < 0) shift = 0;
+    weights[i] = 4 + ((kMaxWeights[i] * reciprocal[e >> shift]) >> shift);
+    weight_sum += weights[i];
+  }
+  const unsigned log_weight = BitWidth64(weight_sum) - 1;
+  weight_sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    weights[i] >>= log_weight - 4;
+    weight_sum += weights[i];
+  }
+  const int64_t ew = x ? st.err1 : 0;
+  const int64_t en = y ? ring.err[s0][y - 1] : 0;
+  const int64_t enw = x ? (y ? ring.err[sm][y - 1] : 0) : en;
+  const int64_t ene = has_ne ? (y ? ring.err[sp][y - 1] : 0) : en;
+  const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
+  int64_t pred[4];
+  pred[0] = w8 + ne8 - n8;
+  pred[1] = n8 - (((en + ew + ene) * kCoeff[0]) >> 5);
+  pred[2] = w8 - (((en + ew + enw) * kCoeff[1]) >> 5);
+  pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
+                   (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
+  int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
+  int64_t prediction = (sum * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24;
+  if (((en ^ ew) | (en ^ enw)) <= 0) {
+    int64_t lo = w8 < ne8 ? w8 : ne8;
+    lo = lo < n8 ? lo : n8;
+    int64_t hi = w8 > ne8 ? w8 : ne8;
+    hi = hi > n8 ? hi : n8;
+    prediction = prediction < lo ? lo : (prediction > hi ? hi : prediction);
+  }
+  const int64_t predicted = (prediction + 3) >> 3;
+  // quantise the residual (dc_quantization.cpp:176-203)
+  float residual = value * inv;
+  residual -= static_cast<float>(predicted);
+  if (residual > -0.62f && residual < 0.62f) residual = 0;
+  float rounded = roundf(residual);
+  if (rounded > 2 || rounded < -2) rounded = roundf(residual * 0.5f) * 2;
+  const int32_t ir = static_cast<int32_t>(roundf(rounded));
+  const int32_t value_q = static_cast<int32_t>(predicted + ir);
+  // Update (prediction.h:124-144)
+  const int64_t scaled = static_cast<int64_t>(value_q) * 8;
+  const int32_t err = static_cast<int32_t>(prediction - scaled);
+  ring.q[s0][y] = value_q;
+  ring.err[s0][y] = err;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int64_t dd = pred[i] - scaled;
+    if (dd < 0) dd = -dd;
+    const uint32_t ei = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
+    ring.e[i][s0][y] = ei;
+    st.e2[i] = st.e1[i];
+    st.e1[i] = ei;
+  }
+  st.q1 = value_q;
+  st.err1 = err;
+  return value_q;
+}
+
+// grid: DC groups.  Each work group runs the three channels of its DC group
+// in one wavefront, each with its own ring (3 x 48 KB of LDS): Y, then X,
+// then B, which subtracts this sample's quantized Y (times the Y step) from
+// its DC first, as the reference does after Y's plane is complete.  Each row
+// reads its DC samples kDcPrefetch steps before it needs them.
+constexpr int kDcPrefetch = 4;
+
 __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
   const float* __restrict__ dc, int bw, int bh,
-  const QuantizerState* __restrict__ qstate, int first_channel,
-  int extra_precision, int32_t* __restrict__ quantized) {
-  __shared__ DcRing ring;
+  const QuantizerState* __restrict__ qstate, int extra_precision,
+  int32_t* __restrict__ quantized) {
+  __shared__ DcRing ring[3];
   __shared__ uint32_t reciprocal[kReciprocals];
   if (threadIdx.x < kReciprocals) reciprocal[threadIdx.x] = (1u << 24) / (threadIdx.x + 1);
   __syncthreads();
-  const int channel = blockIdx.z == 0 ? first_channel : 2;
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
   const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
   const size_t n = static_cast<size_t>(bw) * bh;
   const QuantizerState q = *qstate;
   const float precision = static_cast<float>(1u << extra_precision);
-  const float inv = q.inverse_dc_steps[channel] * precision;
+  float inv[3];
+  for (int c = 0; c < 3; ++c) inv[c] = q.inverse_dc_steps[c] * precision;
   const float step_y = q.dc_steps[1] / precision;
-  int32_t* plane = quantized + channel * n;
-  constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
-  constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
   const int y = threadIdx.x;
-  // this row's samples x-1 and x-2
-  int32_t q1 = 0, err1 = 0;
-  uint32_t e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+  const size_t row = static_cast<size_t>(gy0 + min(y, h - 1)) * bw + gx0;
+  // next kDcPrefetch samples of this row, per channel
+  float ahead[3][kDcPrefetch];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int k = 0; k < kDcPrefetch; ++k) {
+      ahead[c][k] = y < h && k < w ? dc[c * n + row + k] : 0.0f;
+    }
+  }
+  DcRowState st[3];
   const int steps = w + 2 * h;
   for (int t = 0; t < steps; ++t) {
     const int x = t - 2 * y;
     if (y < h && x >= 0 && x < w) {
-      const int s0 = x & (kRing - 1);
-      const int sm = (x - 1) & (kRing - 1), sp = (x + 1) & (kRing - 1);
+      float value[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        value[c] = ahead[c][0];
+#pragma unroll
+        for (int k = 0; k + 1 < kDcPrefetch; ++k) ahead[c][k] = ahead[c][k + 1];
+        ahead[c][kDcPrefetch - 1] =
+          x + kDcPrefetch < w ? dc[c * n + row + x + kDcPrefetch] : 0.0f;
+      }
       const bool has_ne = x + 1 < w;
-      // quantized neighbours
-      const int64_t up = y ? ring.q[s0][y - 1] : 0;
-      const int64_t W_ = x ? q1 : up;
-      const int64_t N_ = y ? up : W_;
-      const int64_t NW_ = x && y ? ring.q[sm][y - 1] : W_;
-      const int64_t NE_ = y && has_ne ? ring.q[sp][y - 1] : N_;
-      const int64_t NN_ = y > 1 ? ring.q[s0][y - 2] : N_;
-      // Predict (prediction.h:75-123)
-      uint32_t weights[4];
-      uint32_t weight_sum = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t en_i = y ? ring.e[i][s0][y - 1] : 0u;
-        const uint32_t north = en_i + (x ? e1[i] : 0u);
-        const uint32_t ne = has_ne ? (y ? ring.e[i][sp][y - 1] : 0u) : north;
-        const uint32_t nw =
-          x ? (y ? ring.e[i][sm][y - 1] : 0u) + (x > 1 ? e2[i] : 0u) : north;
-        const uint32_t e = north + ne + nw;
-        int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
-        if (shift < 0) shift = 0;
-        weights[i] = 4 + ((kMaxWeights[i] * reciprocal[e >> shift]) >> shift);
-        weight_sum += weights[i];
-      }
-      const unsigned log_weight = BitWidth64(weight_sum) - 1;
-      weight_sum = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        weights[i] >>= log_weight - 4;
-        weight_sum += weights[i];
-      }
-      const int64_t ew = x ? err1 : 0;
-      const int64_t en = y ? ring.err[s0][y - 1] : 0;
-      const int64_t enw = x ? (y ? ring.err[sm][y - 1] : 0) : en;
-      const int64_t ene = has_ne ? (y ? ring.err[sp][y - 1] : 0) : en;
-      const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
-      int64_t pred[4];
-      pred[0] = w8 + ne8 - n8;
-      pred[1] = n8 - (((en + ew + ene) * kCoeff[0]) >> 5);
-      pred[2] = w8 - (((en + ew + enw) * kCoeff[1]) >> 5);
-      pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
-                       (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
-      int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
-#pragma unroll
-      for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
-      int64_t prediction = (sum * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24;
-      if (((en ^ ew) | (en ^ enw)) <= 0) {
-        int64_t lo = w8 < ne8 ? w8 : ne8;
-        lo = lo < n8 ? lo : n8;
-        int64_t hi = w8 > ne8 ? w8 : ne8;
-        hi = hi > n8 ? hi : n8;
-        prediction = prediction < lo ? lo : (prediction > hi ? hi : prediction);
-      }
-      const int64_t predicted = (prediction + 3) >> 3;
-      // quantise the residual (dc_quantization.cpp:176-203)
-      const size_t index = static_cast<size_t>(gy0 + y) * bw + gx0 + x;
-      float value = dc[channel * n + index];
-      if (channel == 2) {
-        value = __builtin_fmaf(-static_cast<float>(quantized[n + index]), step_y, value);
-      }
-      float residual = value * inv;
-      residual -= static_cast<float>(predicted);
-      if (residual > -0.62f && residual < 0.62f) residual = 0;
-      float rounded = roundf(residual);
-      if (rounded > 2 || rounded < -2) rounded = roundf(residual * 0.5f) * 2;
-      const int32_t ir = static_cast<int32_t>(roundf(rounded));
-      const int32_t value_q = static_cast<int32_t>(predicted + ir);
-      plane[index] = value_q;
-      // Update (prediction.h:124-144)
-      const int64_t scaled = static_cast<int64_t>(value_q) * 8;
-      const int32_t err = static_cast<int32_t>(prediction - scaled);
-      ring.q[s0][y] = value_q;
-      ring.err[s0][y] = err;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int64_t dd = pred[i] - scaled;
-        if (dd < 0) dd = -dd;
-        const uint32_t ei = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
-        ring.e[i][s0][y] = ei;
-        e2[i] = e1[i];
-        e1[i] = ei;
-      }
-      q1 = value_q;
-      err1 = err;
+      const size_t index = row + x;
+      const int32_t qy = DcSample(ring[1], st[1], reciprocal, x, y, has_ne, value[1], inv[1]);
+      quantized[n + index] = qy;
+      const int32_t qx = DcSample(ring[0], st[0], reciprocal, x, y, has_ne, value[0], inv[0]);
+      quantized[index] = qx;
+      const float b = __builtin_fmaf(-static_cast<float>(qy), step_y, value[2]);
+      quantized[2 * n + index] = DcSample(ring[2], st[2], reciprocal, x, y, has_ne, b, inv[2]);
     }
     __syncthreads();
   }
@@ -1023,14 +1156,13 @@ __global__ __launch_bounds__(kAqThreads) void ReconstructKernel(
   using S = AnchorShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kK;
   __shared__ float coef[K * 3 * N];
-  __shared__ double tmp[K * 3 * N];
-  __shared__ double basis[S::kBasis];
+  __shared__ double tmp[K * 3 * H * (W + 1)];
   __shared__ TransformSlot slots[K];
   __shared__ float cfl[K][2];
   __shared__ float scale[K][3];
   const int bw = width / 8, bh = height / 8;
-  const double *ih, *iv;
-  LoadBases<W, H>(bases, false, basis, &ih, &iv);
+  const double* ih = bases + BasisOffset(W, false);
+  const double* iv = bases + BasisOffset(H, false);
   LoadSlots<K>(entries, count, offsets, bw, slots);
   __syncthreads();
   const QuantizerState q = *qstate;
@@ -1243,7 +1375,7 @@ GJXLResult LaunchFinalCfl(GJXLAmdDevice* d, const float* coefs, int width,
                           const QuantizerState* q, int8_t* y_to_x,
                           int8_t* y_to_b) {
   const AqLayout l = MakeAqLayout(width, height);
-  hipLaunchKernelGGL(FinalCflKernel, dim3(CeilDiv(l.bw, 8), CeilDiv(l.bh, 8)), dim3(64),
+  hipLaunchKernelGGL(FinalCflKernel, dim3(CeilDiv(l.bw, 8), CeilDiv(l.bh, 8)), dim3(256),
                      0, d->stream, coefs, l.store, cells, offsets, l.bw, l.bh, raw, q,
                      d->quant_tables, y_to_x, y_to_b);
   return CheckLaunch("FinalCflKernel");
@@ -1274,11 +1406,7 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
   }));
   const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 1);
   hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
-                     q, 1, 1, dc_quantized);
-  GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
-  const dim3 both(groups.x, groups.y, 2);
-  hipLaunchKernelGGL(DcQuantKernel, both, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
-                     q, 0, 1, dc_quantized);
+                     q, 1, dc_quantized);
   GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
   hipLaunchKernelGGL(DcReconstructKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)),
                      dim3(256), 0, d->stream, dc_quantized, n, q, 1, dc_rec);

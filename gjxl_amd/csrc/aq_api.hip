@@ -169,8 +169,6 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_DC_SMOOTHED] = 4 * 3 * n;
   z[GJXL_AMD_AQ_INVERSE_SIGMA] = 4 * n;
   z[GJXL_AMD_AQ_RECONSTRUCTED] = 4 * 3 * padded;
-  z[GJXL_AMD_AQ_SCRATCH_A] = 4 * 3 * pixels;
-  z[GJXL_AMD_AQ_SCRATCH_B] = 4 * 3 * pixels;
   z[GJXL_AMD_AQ_FILTERED] = 4 * 3 * pixels;
   z[GJXL_AMD_AQ_LINEAR] = 4 * 3 * pixels;
   z[GJXL_AMD_AQ_FIELD] = 4 * n;
@@ -328,8 +326,7 @@ GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
     Buf<float>(s, GJXL_AMD_AQ_RECONSTRUCTED)));
   return LaunchLoopFiltersAndColor(
     st, Buf<float>(s, GJXL_AMD_AQ_RECONSTRUCTED), s->pw, s->ph, s->width, s->height,
-    Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA), Buf<float>(s, GJXL_AMD_AQ_SCRATCH_A),
-    Buf<float>(s, GJXL_AMD_AQ_SCRATCH_B), Buf<float>(s, GJXL_AMD_AQ_FILTERED),
+    Buf<float>(s, GJXL_AMD_AQ_INVERSE_SIGMA), Buf<float>(s, GJXL_AMD_AQ_FILTERED),
     Buf<float>(s, GJXL_AMD_AQ_LINEAR), 255.0f);
 }
 

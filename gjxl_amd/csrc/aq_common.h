@@ -258,7 +258,6 @@ GJXLResult LaunchReconstruct(GJXLAmdDevice* d, const int32_t* quantized,
 GJXLResult LaunchLoopFiltersAndColor(hipStream_t s, const float* reconstructed,
                                      int pw, int ph, int w, int h,
                                      const float* inverse_sigma,
-                                     float* scratch_a, float* scratch_b,
                                      float* filtered, float* linear,
                                      float intensity_target);
 

@@ -45,6 +45,24 @@ __device__ __forceinline__ float BMax(float a, float b) { return a < b ? b : a; 
 
 // ---- Gaussian blurs --------------------------------------------------------
 
+// An interior output of ConvolutionWithTranspose from the 2R + 1 samples
+// s[0 .. 2R] around it (s[R] is the centre): mirrored pairs times the scaled
+// taps, accumulated into four interleaved partial sums, in the reference's
+// order.
+template <int R>
+__device__ __forceinline__ float ConvolveInterior(const float* s, const float* sk) {
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const float pair = (s[i] + s[2 * R - i]) * sk[i];
+    acc[i % 4] = i < 4 ? pair : acc[i % 4] + pair;
+  }
+  float r = s[R] * sk[R];
+#pragma unroll
+  for (int i = 0; i < 4 && i < R; ++i) r = r + acc[i];
+  return r;
+}
+
 // One output of ConvolutionWithTranspose: position `pos` of a line of `len`
 // samples spaced `stride` floats apart.
 template <int R>
@@ -57,16 +75,10 @@ __device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
   const int last = len > R ? len - R : 0;
   if (pos >= first && pos < last) {
     const float* s = line + static_cast<ptrdiff_t>(pos - R) * stride;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float window[2 * R + 1];
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-      const float pair = (s[i * stride] + s[(2 * R - i) * stride]) * sk[i];
-      acc[i % 4] = i < 4 ? pair : acc[i % 4] + pair;
-    }
-    float r = s[R * stride] * sk[R];
-#pragma unroll
-    for (int i = 0; i < 4 && i < R; ++i) r = r + acc[i];
-    return r;
+    for (int i = 0; i <= 2 * R; ++i) window[i] = s[i * stride];
+    return ConvolveInterior<R>(window, sk);
   }
   const int lo = pos < R ? 0 : pos - R;
   const int hi = len - 1 < pos + R ? len - 1 : pos + R;
@@ -80,23 +92,82 @@ __device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
   return sum * scale;
 }
 
+// Rows: a work group blurs kRowSegment pixels of one row from an LDS copy
+// of the segment and its halo; each thread produces kRowOutputs neighbouring
+// outputs from one register window (read as float4).
+constexpr int kRowOutputs = 4;
+constexpr int kRowSegment = kThreads * kRowOutputs;
+
 template <int R>
 __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
-  out[static_cast<size_t>(y) * w + x] =
-    ConvolveLine<R>(in + static_cast<size_t>(y) * w, 1, x, w, taps);
+  constexpr int kWindow = kRowOutputs + 2 * R;
+  constexpr int kWindow4 = (kWindow + 3) / 4;
+  constexpr int kLds = kRowSegment + 2 * R + 4;
+  __shared__ __align__(16) float seg[kLds];
+  const int y = blockIdx.y, x0 = blockIdx.x * kRowSegment;
+  const float* row = in + static_cast<size_t>(y) * w;
+  for (int i = threadIdx.x; i < kLds; i += kThreads) {
+    const int gx = x0 - R + i;
+    seg[i] = gx >= 0 && gx < w ? row[gx] : 0.0f;
+  }
+  __syncthreads();
+  const int xs = x0 + threadIdx.x * kRowOutputs;
+  if (xs >= w) return;
+  float window[kWindow4 * 4];
+  const float4* src = reinterpret_cast<const float4*>(seg + threadIdx.x * kRowOutputs);
+#pragma unroll
+  for (int j = 0; j < kWindow4; ++j) {
+    const float4 v = src[j];
+    window[4 * j] = v.x;
+    window[4 * j + 1] = v.y;
+    window[4 * j + 2] = v.z;
+    window[4 * j + 3] = v.w;
+  }
+  const float* sk = taps + 33;
+  const int first = w < R ? w : R;
+  const int last = w > R ? w - R : 0;
+  float* dst = out + static_cast<size_t>(y) * w;
+#pragma unroll
+  for (int p = 0; p < kRowOutputs; ++p) {
+    const int x = xs + p;
+    if (x >= w) break;
+    dst[x] = x >= first && x < last ? ConvolveInterior<R>(window + p, sk)
+                                    : ConvolveLine<R>(row, 1, x, w, taps);
+  }
 }
+
+// Columns: each thread produces kColOutputs outputs down one column from one
+// register window (coalesced across the wave).
+constexpr int kColOutputs = 8;
 
 template <int R>
 __global__ __launch_bounds__(kThreads) void BlurColsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kColOutputs;
   if (x >= w) return;
-  out[static_cast<size_t>(y) * w + x] = ConvolveLine<R>(in + x, w, y, h, taps);
+  const int first = h < R ? h : R;
+  const int last = h > R ? h - R : 0;
+  if (y0 >= first && y0 + kColOutputs - 1 < last) {
+    float window[kColOutputs + 2 * R];
+#pragma unroll
+    for (int j = 0; j < kColOutputs + 2 * R; ++j) {
+      window[j] = in[static_cast<size_t>(y0 - R + j) * w + x];
+    }
+    const float* sk = taps + 33;
+#pragma unroll
+    for (int p = 0; p < kColOutputs; ++p) {
+      out[static_cast<size_t>(y0 + p) * w + x] = ConvolveInterior<R>(window + p, sk);
+    }
+    return;
+  }
+  for (int p = 0; p < kColOutputs; ++p) {
+    const int y = y0 + p;
+    if (y >= h) break;
+    out[static_cast<size_t>(y) * w + x] = ConvolveLine<R>(in + x, w, y, h, taps);
+  }
 }
 
 // ---- opsin dynamics ---------------------------------------------------------
@@ -132,54 +203,74 @@ __device__ __forceinline__ void Absorbance(float r, float g, float b, bool clamp
   }
 }
 
-__device__ __forceinline__ float Horizontal5(const float* row, int x, int w,
-                                             float w0, float w1, float w2) {
-  float r = row[x] * w0;
-  r = (row[BMirror(x - 1, w)] + row[BMirror(x + 1, w)]) * w1 + r;
-  r = (row[BMirror(x - 2, w)] + row[BMirror(x + 2, w)]) * w2 + r;
-  return r;
-}
+// One work group per 64x16 tile: the three planes' tile with a mirrored
+// 2-pixel halo in LDS, then per pixel the 5-tap horizontal sums of the five
+// rows it needs and their vertical combination, in the reference's order.
+constexpr int kOpsinTileW = 64, kOpsinTileH = 16;
+constexpr int kOpsinRows = kOpsinTileH / (kThreads / kOpsinTileW);  // per thread
 
 __global__ __launch_bounds__(kThreads) void OpsinDynamicsKernel(
   const float* __restrict__ rgb, int w, int h, float intensity,
   const float* __restrict__ taps, float* __restrict__ xyb) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
+  constexpr int TW = kOpsinTileW + 4, TH = kOpsinTileH + 4;
+  __shared__ float tile[3][TH][TW];
+  const int x0 = blockIdx.x * kOpsinTileW, y0 = blockIdx.y * kOpsinTileH;
   const size_t n = static_cast<size_t>(w) * h;
+  for (int e = threadIdx.x; e < 3 * TH * TW; e += kThreads) {
+    const int c = e / (TH * TW), r = (e / TW) % TH, col = e % TW;
+    const int sy = BMirror(y0 - 2 + r, h), sx = BMirror(x0 - 2 + col, w);
+    tile[c][r][col] = rgb[c * n + static_cast<size_t>(sy) * w + sx];
+  }
+  __syncthreads();
   const float w0 = taps[33 + 2], w1 = taps[33 + 1], w2 = taps[33 + 0];
-  const int ym1 = BMirror(y - 1, h), yp1 = BMirror(y + 1, h);
-  const int ym2 = BMirror(y - 2, h), yp2 = BMirror(y + 2, h);
-  float blurred[3];
+  const int lx = threadIdx.x % kOpsinTileW;
+  const int ly0 = (threadIdx.x / kOpsinTileW) * kOpsinRows;
+  const int x = x0 + lx;
+  if (x >= w) return;
+  // horizontal sums of tile rows ly0 .. ly0 + kOpsinRows + 3, per channel
+  float hz[3][kOpsinRows + 4];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const float* p = rgb + c * n;
-    const float center = Horizontal5(p + static_cast<size_t>(y) * w, x, w, w0, w1, w2);
-    const float t1 = Horizontal5(p + static_cast<size_t>(ym1) * w, x, w, w0, w1, w2);
-    const float b1 = Horizontal5(p + static_cast<size_t>(yp1) * w, x, w, w0, w1, w2);
-    const float t2 = Horizontal5(p + static_cast<size_t>(ym2) * w, x, w, w0, w1, w2);
-    const float b2 = Horizontal5(p + static_cast<size_t>(yp2) * w, x, w, w0, w1, w2);
-    float r = center * w0;
-    r = (t1 + b1) * w1 + r;
-    r = (t2 + b2) * w2 + r;
-    blurred[c] = r;
-  }
-  const size_t i = static_cast<size_t>(y) * w + x;
-  float pre[3], cur[3], sens[3];
-  Absorbance(blurred[0] * intensity, blurred[1] * intensity, blurred[2] * intensity,
-             true, pre);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    pre[c] = BMax(pre[c], 1.0e-4f);
-    sens[c] = BMax(ButteraugliGamma(pre[c]) / pre[c], 1.0e-4f);
+    for (int r = 0; r < kOpsinRows + 4; ++r) {
+      const float* row = &tile[c][ly0 + r][lx + 2];
+      float v = row[0] * w0;
+      v = (row[-1] + row[1]) * w1 + v;
+      v = (row[-2] + row[2]) * w2 + v;
+      hz[c][r] = v;
+    }
   }
-  Absorbance(rgb[i] * intensity, rgb[n + i] * intensity, rgb[2 * n + i] * intensity,
-             false, cur);
-  cur[0] = BMax(cur[0] * sens[0], 1.7557483643287353f);
-  cur[1] = BMax(cur[1] * sens[1], 1.7557483643287353f);
-  cur[2] = BMax(cur[2] * sens[2], 12.226454707163354f);
-  xyb[i] = cur[0] - cur[1];
-  xyb[n + i] = cur[0] + cur[1];
-  xyb[2 * n + i] = cur[2];
+#pragma unroll
+  for (int j = 0; j < kOpsinRows; ++j) {
+    const int y = y0 + ly0 + j;
+    if (y >= h) break;
+    float blurred[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float r = hz[c][j + 2] * w0;
+      r = (hz[c][j + 1] + hz[c][j + 3]) * w1 + r;
+      r = (hz[c][j] + hz[c][j + 4]) * w2 + r;
+      blurred[c] = r;
+    }
+    const size_t i = static_cast<size_t>(y) * w + x;
+    float pre[3], cur[3], sens[3];
+    Absorbance(blurred[0] * intensity, blurred[1] * intensity, blurred[2] * intensity,
+               true, pre);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pre[c] = BMax(pre[c], 1.0e-4f);
+      sens[c] = BMax(ButteraugliGamma(pre[c]) / pre[c], 1.0e-4f);
+    }
+    const int ly = ly0 + j + 2, tx = lx + 2;
+    Absorbance(tile[0][ly][tx] * intensity, tile[1][ly][tx] * intensity,
+               tile[2][ly][tx] * intensity, false, cur);
+    cur[0] = BMax(cur[0] * sens[0], 1.7557483643287353f);
+    cur[1] = BMax(cur[1] * sens[1], 1.7557483643287353f);
+    cur[2] = BMax(cur[2] * sens[2], 12.226454707163354f);
+    xyb[i] = cur[0] - cur[1];
+    xyb[n + i] = cur[0] + cur[1];
+    xyb[2 * n + i] = cur[2];
+  }
 }
 
 // ---- frequency separation ---------------------------------------------------
@@ -364,13 +455,24 @@ __global__ __launch_bounds__(kThreads) void MaltaKernel(
     tile[r][col] = v;
   }
   __syncthreads();
+  // each thread: kMaltaRows pixels down one column, from the 9-wide
+  // neighbourhood of those rows held in registers
+  constexpr int kMaltaRows = kMaltaTileH / (kThreads / kMaltaTileW);
+  constexpr int kNbRows = kMaltaRows + 2 * kMaltaHalo, kNbCols = 2 * kMaltaHalo + 1;
   const int tx = threadIdx.x % kMaltaTileW;
-  for (int ty = threadIdx.x / kMaltaTileW; ty < kMaltaTileH; ty += blockDim.x / kMaltaTileW) {
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= w || y >= h) continue;
-    auto P = [&](int dx, int dy) -> float {
-      return tile[ty + kMaltaHalo + dy][tx + kMaltaHalo + dx];
-    };
+  const int ty0 = (threadIdx.x / kMaltaTileW) * kMaltaRows;
+  const int x = x0 + tx;
+  if (x >= w) return;
+  float nb[kNbRows][kNbCols];
+#pragma unroll
+  for (int r = 0; r < kNbRows; ++r)
+#pragma unroll
+    for (int c = 0; c < kNbCols; ++c) nb[r][c] = tile[ty0 + r][tx + c];
+#pragma unroll
+  for (int j = 0; j < kMaltaRows; ++j) {
+    const int y = y0 + ty0 + j;
+    if (y >= h) break;
+    auto P = [&](int dx, int dy) -> float { return nb[j + kMaltaHalo + dy][kMaltaHalo + dx]; };
     float result = 0.0f;
 #pragma unroll
     for (int l = 0; l < 16; ++l) {
@@ -671,10 +773,13 @@ const float* Taps(const GJXLAmdButteraugli* b, ButteraugliSigma s) {
 template <int R>
 GJXLResult Blur(hipStream_t st, const float* in, int w, int h, const float* taps,
                 float* tmp, float* out) {
-  hipLaunchKernelGGL(BlurRowsKernel<R>, Grid2(w, h), dim3(kThreads), 0, st, in, w, h,
-                     taps, tmp);
-  hipLaunchKernelGGL(BlurColsKernel<R>, Grid2(w, h), dim3(kThreads), 0, st, tmp, w, h,
-                     taps, out);
+  hipLaunchKernelGGL(BlurRowsKernel<R>,
+                     dim3(CeilDiv(static_cast<uint32_t>(w), kRowSegment), h), dim3(kThreads),
+                     0, st, in, w, h, taps, tmp);
+  hipLaunchKernelGGL(BlurColsKernel<R>,
+                     dim3(CeilDiv(static_cast<uint32_t>(w), kThreads),
+                          CeilDiv(static_cast<uint32_t>(h), kColOutputs)),
+                     dim3(kThreads), 0, st, tmp, w, h, taps, out);
   return CheckLaunch("BlurKernel");
 }
 
@@ -684,7 +789,10 @@ GJXLResult Psycho(GJXLAmdButteraugli* b, ButteraugliScale& s, const float* in,
   hipStream_t st = b->device->stream;
   const int w = s.width, h = s.height;
   const size_t n = s.n;
-  hipLaunchKernelGGL(OpsinDynamicsKernel, Grid2(w, h), dim3(kThreads), 0, st, in, w, h,
+  hipLaunchKernelGGL(OpsinDynamicsKernel,
+                     dim3(CeilDiv(static_cast<uint32_t>(w), kOpsinTileW),
+                          CeilDiv(static_cast<uint32_t>(h), kOpsinTileH)),
+                     dim3(kThreads), 0, st, in, w, h,
                      b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
   GJXL_AMD_TRY(CheckLaunch("OpsinDynamicsKernel"));
   for (int c = 0; c < 3; ++c) {

@@ -27,110 +27,19 @@ struct GaborishWeights {
   float center, axis, diagonal;
 };
 
-// ApplyGaborish (src/codec/gaborish.cpp:101-190).  in: stride in_stride,
-// plane in_plane; out: unpadded w x h planes.
-__global__ __launch_bounds__(256) void GaborishKernel(
-  const float* __restrict__ in, int in_stride, size_t in_plane, int w, int h,
-  GaborishWeights g, float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  const int c = blockIdx.z;
-  if (x >= w) return;
-  const float* p = in + c * in_plane;
-  auto S = [&](int dx, int dy) {
-    return p[static_cast<size_t>(MirrorInt(y + dy, h)) * in_stride + MirrorInt(x + dx, w)];
-  };
-  const float axes = (S(-1, 0) + S(1, 0)) + (S(0, -1) + S(0, 1));
-  const float diagonals = (S(-1, -1) + S(1, -1)) + (S(-1, 1) + S(1, 1));
-  out[static_cast<size_t>(c) * w * h + static_cast<size_t>(y) * w + x] =
-    g.center * S(0, 0) + g.axis * axes + g.diagonal * diagonals;
-}
-
-// ApplyEpfPass (src/codec/epf.cpp:61-205), pass 1 (plus-patch SAD) or pass 2
-// (pixel SAD), four cardinal neighbours, default options.  A work group
-// filters a 64x16 tile from an LDS copy of its three channels with a
-// mirrored 2-pixel halo (every sample either pass reads is within 2 pixels).
-constexpr int kEpfTileW = 64, kEpfTileH = 16, kEpfHalo = 2;
-constexpr int kEpfLw = kEpfTileW + 2 * kEpfHalo, kEpfLh = kEpfTileH + 2 * kEpfHalo;
-
-template <int kPass>
-__global__ __launch_bounds__(256) void EpfKernel(
-  const float* __restrict__ in, int w, int h,
-  const float* __restrict__ inverse_sigma, float* __restrict__ out) {
-  __shared__ float tile[3][kEpfLh][kEpfLw];
-  const int x0 = blockIdx.x * kEpfTileW, y0 = blockIdx.y * kEpfTileH;
-  const size_t plane = static_cast<size_t>(w) * h;
-  for (int e = threadIdx.x; e < 3 * kEpfLh * kEpfLw; e += blockDim.x) {
-    const int c = e / (kEpfLh * kEpfLw), r = (e / kEpfLw) % kEpfLh, col = e % kEpfLw;
-    const int sy = MirrorInt(y0 + r - kEpfHalo, h), sx = MirrorInt(x0 + col - kEpfHalo, w);
-    tile[c][r][col] = in[c * plane + static_cast<size_t>(sy) * w + sx];
-  }
-  __syncthreads();
-  const int bw = (w + 7) / 8;
-  constexpr float kMinInverseSigma = -1.1715728752538099024f / 0.3f;
-  constexpr float kSigmaScale = 1.65f * (kPass == 2 ? 6.5f : 1.0f);
-  constexpr int kCard[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-  constexpr int kPlus[5][2] = {{0, 0}, {0, -1}, {-1, 0}, {0, 1}, {1, 0}};
-  constexpr float kScale[3] = {40.0f, 5.0f, 3.5f};
-  const int tx = threadIdx.x % kEpfTileW;
-  for (int ty = threadIdx.x / kEpfTileW; ty < kEpfTileH; ty += blockDim.x / kEpfTileW) {
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= w || y >= h) continue;
-    const size_t i = static_cast<size_t>(y) * w + x;
-    // tile sample at offset (dx, dy) from (x, y)
-    auto S = [&](int c, int dx, int dy) {
-      return tile[c][ty + kEpfHalo + dy][tx + kEpfHalo + dx];
-    };
-    const float bis = inverse_sigma[static_cast<size_t>(y / 8) * bw + x / 8];
-    if (bis < kMinInverseSigma) {
-      for (int c = 0; c < 3; ++c) out[c * plane + i] = S(c, 0, 0);
-      continue;
-    }
-    const bool border = x % 8 == 0 || x % 8 == 7 || y % 8 == 0 || y % 8 == 7;
-    const float sis = bis * kSigmaScale * (border ? 2.0f / 3.0f : 1.0f);
-    float sum[3] = {S(0, 0, 0), S(1, 0, 0), S(2, 0, 0)};
-    float weight_sum = 1.0f;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int dx = kCard[k][0], dy = kCard[k][1];
-      float sad = 0.0f;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if constexpr (kPass == 1) {
-          float cs = 0.0f;
-#pragma unroll
-          for (int o = 0; o < 5; ++o) {
-            cs += fabsf(S(c, kPlus[o][0], kPlus[o][1]) -
-                        S(c, dx + kPlus[o][0], dy + kPlus[o][1]));
-          }
-          sad = __builtin_fmaf(cs, kScale[c], sad);
-        } else {
-          sad = __builtin_fmaf(fabsf(S(c, 0, 0) - S(c, dx, dy)), kScale[c], sad);
-        }
-      }
-      float weight = __builtin_fmaf(sad, sis, 1.0f);
-      weight = 0.0f < weight ? weight : 0.0f;
-      weight_sum += weight;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sum[c] = __builtin_fmaf(weight, S(c, dx, dy), sum[c]);
-    }
-    for (int c = 0; c < 3; ++c) out[c * plane + i] = sum[c] / weight_sum;
-  }
-}
-
+// ApplyGaborish (src/codec/gaborish.cpp:101-190) and ApplyEpfPass
+// (src/codec/epf.cpp:61-205, pass 1 plus-patch SAD, pass 2 pixel SAD, four
+// cardinal neighbours, default options) are evaluated in LoopFilterKernel.
 struct InverseOpsin {
   float m[3][3];
   float scale;       // 255 / intensity
   float bias_cbrt;   // host cbrtf(bias)
 };
 
-// OpsinToLinearRgb (src/codec/color_transform.cpp:433-471).
-__global__ __launch_bounds__(256) void OpsinToLinearKernel(
-  const float* __restrict__ in, size_t count, InverseOpsin p,
-  float* __restrict__ out) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+// OpsinToLinearRgb (src/codec/color_transform.cpp:433-471) of one pixel.
+__device__ __forceinline__ void OpsinToLinearPixel(float x, float y, float b,
+                                                   const InverseOpsin& p, float out[3]) {
   constexpr float kOpsinBias = 0.0037930732552754493f;
-  const float x = in[i], y = in[count + i], b = in[2 * count + i];
   const float gamma[3] = {y + x + p.bias_cbrt, y - x + p.bias_cbrt, b + p.bias_cbrt};
   float mixed[3];
   for (int c = 0; c < 3; ++c) mixed[c] = gamma[c] * gamma[c] * gamma[c] - kOpsinBias;
@@ -138,35 +47,155 @@ __global__ __launch_bounds__(256) void OpsinToLinearKernel(
     float v = p.scale * p.m[row][0] * mixed[0];
     v = __builtin_fmaf(p.scale * p.m[row][1], mixed[1], v);
     v = __builtin_fmaf(p.scale * p.m[row][2], mixed[2], v);
-    out[row * count + i] = v;
+    out[row] = v;
+  }
+}
+
+
+// The three loop filters and XYB -> linear fused: one work group per 64x16
+// output tile.  LDS holds the mirrored input with a 4-pixel halo, Gaborish on
+// the tile plus 3, EPF pass 1 on the tile plus 1, and pass 2 on the tile.
+// Each filter's samples outside the image are its own values at the
+// mirrored coordinate (every filter mirrors its input at the image extent),
+// copied from the slot of that coordinate, which lies within the same region.
+constexpr int kLfTileW = 64, kLfTileH = 16;
+constexpr int kRawW = kLfTileW + 8, kRawH = kLfTileH + 8;  // halo 4
+constexpr int kGabW = kLfTileW + 6, kGabH = kLfTileH + 6;  // halo 3
+constexpr int kE1W = kLfTileW + 2, kE1H = kLfTileH + 2;    // halo 1
+
+template <int kPass, typename Sample>
+__device__ __forceinline__ void EpfPixel(Sample S, float bis, int x, int y,
+                                         float out[3]) {
+  constexpr float kMinInverseSigma = -1.1715728752538099024f / 0.3f;
+  constexpr float kSigmaScale = 1.65f * (kPass == 2 ? 6.5f : 1.0f);
+  constexpr int kCard[4][2] = {{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
+  constexpr int kPlus[5][2] = {{0, 0}, {0, -1}, {-1, 0}, {0, 1}, {1, 0}};
+  constexpr float kScale[3] = {40.0f, 5.0f, 3.5f};
+  if (bis < kMinInverseSigma) {
+    for (int c = 0; c < 3; ++c) out[c] = S(c, 0, 0);
+    return;
+  }
+  const bool border = x % 8 == 0 || x % 8 == 7 || y % 8 == 0 || y % 8 == 7;
+  const float sis = bis * kSigmaScale * (border ? 2.0f / 3.0f : 1.0f);
+  float sum[3] = {S(0, 0, 0), S(1, 0, 0), S(2, 0, 0)};
+  float weight_sum = 1.0f;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int dx = kCard[k][0], dy = kCard[k][1];
+    float sad = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (kPass == 1) {
+        float cs = 0.0f;
+#pragma unroll
+        for (int o = 0; o < 5; ++o) {
+          cs += fabsf(S(c, kPlus[o][0], kPlus[o][1]) -
+                      S(c, dx + kPlus[o][0], dy + kPlus[o][1]));
+        }
+        sad = __builtin_fmaf(cs, kScale[c], sad);
+      } else {
+        sad = __builtin_fmaf(fabsf(S(c, 0, 0) - S(c, dx, dy)), kScale[c], sad);
+      }
+    }
+    float weight = __builtin_fmaf(sad, sis, 1.0f);
+    weight = 0.0f < weight ? weight : 0.0f;
+    weight_sum += weight;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sum[c] = __builtin_fmaf(weight, S(c, dx, dy), sum[c]);
+  }
+  for (int c = 0; c < 3; ++c) out[c] = sum[c] / weight_sum;
+}
+
+__global__ __launch_bounds__(256) void LoopFilterKernel(
+  const float* __restrict__ in, int in_stride, size_t in_plane, int w, int h,
+  GaborishWeights g, const float* __restrict__ inverse_sigma, InverseOpsin p,
+  float* __restrict__ filtered, float* __restrict__ linear) {
+  __shared__ float raw[3][kRawH][kRawW];
+  __shared__ float gab[3][kGabH][kGabW];
+  __shared__ float e1[3][kE1H][kE1W];
+  const int x0 = blockIdx.x * kLfTileW, y0 = blockIdx.y * kLfTileH;
+  const int bw = (w + 7) / 8;
+  const size_t plane = static_cast<size_t>(w) * h;
+  for (int e = threadIdx.x; e < 3 * kRawH * kRawW; e += blockDim.x) {
+    const int c = e / (kRawH * kRawW), r = (e / kRawW) % kRawH, col = e % kRawW;
+    const int sy = MirrorInt(y0 - 4 + r, h), sx = MirrorInt(x0 - 4 + col, w);
+    raw[c][r][col] = in[c * in_plane + static_cast<size_t>(sy) * in_stride + sx];
+  }
+  __syncthreads();
+  // Gaborish (gaborish.cpp:101-190) at the in-image slots of its region
+  for (int e = threadIdx.x; e < kGabH * kGabW; e += blockDim.x) {
+    const int r = e / kGabW, col = e % kGabW;
+    const int gx = x0 - 3 + col, gy = y0 - 3 + r;
+    if (gx < 0 || gy < 0 || gx >= w || gy >= h) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      auto S = [&](int dx, int dy) { return raw[c][r + 1 + dy][col + 1 + dx]; };
+      const float axes = (S(-1, 0) + S(1, 0)) + (S(0, -1) + S(0, 1));
+      const float diagonals = (S(-1, -1) + S(1, -1)) + (S(-1, 1) + S(1, 1));
+      gab[c][r][col] = g.center * S(0, 0) + g.axis * axes + g.diagonal * diagonals;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kGabH * kGabW; e += blockDim.x) {
+    const int r = e / kGabW, col = e % kGabW;
+    const int gx = x0 - 3 + col, gy = y0 - 3 + r;
+    if (gx >= 0 && gy >= 0 && gx < w && gy < h) continue;
+    const int mr = MirrorInt(gy, h) - (y0 - 3), mc = MirrorInt(gx, w) - (x0 - 3);
+    if (mr < 0 || mc < 0 || mr >= kGabH || mc >= kGabW) continue;  // never read
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gab[c][r][col] = gab[c][mr][mc];
+  }
+  __syncthreads();
+  // EPF pass 1 at the in-image slots of its region
+  for (int e = threadIdx.x; e < kE1H * kE1W; e += blockDim.x) {
+    const int r = e / kE1W, col = e % kE1W;
+    const int x = x0 - 1 + col, y = y0 - 1 + r;
+    if (x < 0 || y < 0 || x >= w || y >= h) continue;
+    auto S = [&](int c, int dx, int dy) { return gab[c][r + 2 + dy][col + 2 + dx]; };
+    float v[3];
+    EpfPixel<1>(S, inverse_sigma[static_cast<size_t>(y / 8) * bw + x / 8], x, y, v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e1[c][r][col] = v[c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kE1H * kE1W; e += blockDim.x) {
+    const int r = e / kE1W, col = e % kE1W;
+    const int x = x0 - 1 + col, y = y0 - 1 + r;
+    if (x >= 0 && y >= 0 && x < w && y < h) continue;
+    const int mr = MirrorInt(y, h) - (y0 - 1), mc = MirrorInt(x, w) - (x0 - 1);
+    if (mr < 0 || mc < 0 || mr >= kE1H || mc >= kE1W) continue;  // never read
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e1[c][r][col] = e1[c][mr][mc];
+  }
+  __syncthreads();
+  // EPF pass 2 on the tile, then XYB -> linear
+  const int tx = threadIdx.x % kLfTileW;
+  for (int ty = threadIdx.x / kLfTileW; ty < kLfTileH; ty += blockDim.x / kLfTileW) {
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) continue;
+    auto S = [&](int c, int dx, int dy) { return e1[c][ty + 1 + dy][tx + 1 + dx]; };
+    float v[3];
+    EpfPixel<2>(S, inverse_sigma[static_cast<size_t>(y / 8) * bw + x / 8], x, y, v);
+    const size_t i = static_cast<size_t>(y) * w + x;
+    for (int c = 0; c < 3; ++c) filtered[c * plane + i] = v[c];
+    float rgb[3];
+    OpsinToLinearPixel(v[0], v[1], v[2], p, rgb);
+    for (int c = 0; c < 3; ++c) linear[c * plane + i] = rgb[c];
   }
 }
 
 }  // namespace
 
-// Gaborish from the padded reconstruction (stride pw, plane pw*ph) into
-// scratch_a, EPF pass 1 into scratch_b, pass 2 into filtered, then XYB ->
-// linear into linear.  All outputs unpadded w x h planes.
+// Gaborish, EPF passes 1 and 2 of the padded reconstruction (stride pw,
+// plane pw*ph) into filtered, and XYB -> linear into linear.  Both outputs
+// are unpadded w x h planes.
 GJXLResult LaunchLoopFiltersAndColor(hipStream_t s, const float* reconstructed,
                                      int pw, int ph, int w, int h,
-                                     const float* inverse_sigma,
-                                     float* scratch_a, float* scratch_b,
-                                     float* filtered, float* linear,
-                                     float intensity_target) {
+                                     const float* inverse_sigma, float* filtered,
+                                     float* linear, float intensity_target) {
   const float w1 = 1.1f * 0.104699568f, w2 = 1.1f * 0.055680538f;
   const float divisor = 1.0f + 4.0f * (w1 + w2);
   const GaborishWeights g{1.0f / divisor, w1 / divisor, w2 / divisor};
-  const dim3 grid(CeilDiv(w, 256), h, 3);
-  hipLaunchKernelGGL(GaborishKernel, grid, dim3(256), 0, s, reconstructed, pw,
-                     static_cast<size_t>(pw) * ph, w, h, g, scratch_a);
-  GJXL_AMD_TRY(CheckLaunch("GaborishKernel"));
-  const dim3 grid2(CeilDiv(w, kEpfTileW), CeilDiv(h, kEpfTileH));
-  hipLaunchKernelGGL(EpfKernel<1>, grid2, dim3(256), 0, s, scratch_a, w, h,
-                     inverse_sigma, scratch_b);
-  GJXL_AMD_TRY(CheckLaunch("EpfKernel<1>"));
-  hipLaunchKernelGGL(EpfKernel<2>, grid2, dim3(256), 0, s, scratch_b, w, h,
-                     inverse_sigma, filtered);
-  GJXL_AMD_TRY(CheckLaunch("EpfKernel<2>"));
   // The host libm's cbrtf, as the reference evaluates it; the volatile keeps
   // the compiler from folding it with a different method.
   volatile float bias = 0.0037930732552754493f;
@@ -175,10 +204,11 @@ GJXLResult LaunchLoopFiltersAndColor(hipStream_t s, const float* reconstructed,
                   {-3.6588512862745097f, 2.7129230470588235f, 1.9459282392156863f}},
                  255.0f / intensity_target,
                  cbrtf(bias)};
-  const size_t count = static_cast<size_t>(w) * h;
-  hipLaunchKernelGGL(OpsinToLinearKernel, dim3(CeilDiv(static_cast<uint32_t>(count), 256)),
-                     dim3(256), 0, s, filtered, count, p, linear);
-  return CheckLaunch("OpsinToLinearKernel");
+  const dim3 grid(CeilDiv(w, kLfTileW), CeilDiv(h, kLfTileH));
+  hipLaunchKernelGGL(LoopFilterKernel, grid, dim3(256), 0, s, reconstructed, pw,
+                     static_cast<size_t>(pw) * ph, w, h, g, inverse_sigma, p, filtered,
+                     linear);
+  return CheckLaunch("LoopFilterKernel");
 }
 
 }  // namespace gjxl_amd

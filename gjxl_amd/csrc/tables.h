@@ -55,7 +55,11 @@ __host__ __device__ constexpr int BasisOffset(int n, bool forward) {
   const int base = forward ? 0 : 1344;
   return base + (n == 8 ? 0 : n == 16 ? 64 : 320);
 }
-constexpr int kBasisDoubles = 2688;
+// Transposed forward bases (basis[s * n + f]) follow: 8, 16, 32.
+__host__ __device__ constexpr int BasisOffsetT(int n) {
+  return 2688 + (n == 8 ? 0 : n == 16 ? 64 : 320);
+}
+constexpr int kBasisDoubles = 4032;
 
 // Covered 8x8 blocks of the searched strategies.
 __host__ __device__ constexpr int CoveredW(int strategy) {

@@ -140,6 +140,12 @@ void BuildDctBases(double* out) {
       }
     }
   }
+  for (int n : {8, 16, 32}) {
+    const double* b = out + BasisOffset(n, true);
+    double* t = out + BasisOffsetT(n);
+    for (int f = 0; f < n; ++f)
+      for (int s = 0; s < n; ++s) t[s * n + f] = b[f * n + s];
+  }
 }
 
 }  // namespace gjxl_amd

@@ -196,8 +196,6 @@ enum {
   GJXL_AMD_AQ_DC_SMOOTHED,          /* f32 3 block planes */
   GJXL_AMD_AQ_INVERSE_SIGMA,        /* f32 per block: EPF inverse sigma */
   GJXL_AMD_AQ_RECONSTRUCTED,        /* f32 3 planes pw*ph: XYB */
-  GJXL_AMD_AQ_SCRATCH_A,            /* f32 3 planes w*h: after Gaborish */
-  GJXL_AMD_AQ_SCRATCH_B,            /* f32 3 planes w*h: after EPF pass 1 */
   GJXL_AMD_AQ_FILTERED,             /* f32 3 planes w*h: after EPF pass 2 */
   GJXL_AMD_AQ_LINEAR,               /* f32 3 planes w*h: linear RGB */
   GJXL_AMD_AQ_FIELD,                /* f32 per block: gjxl_amd_aq_run field */
