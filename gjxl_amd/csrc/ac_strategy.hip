@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
   // DCT phase: X, Y, B coefficients in values[0..2]; then a_x, b_x, a_b, b_b
   __shared__ float values[4][4096];
-  __shared__ float terms[2][3][kCflThreads];
+  // padded so that the 24 summing lanes read 24 different banks
+  __shared__ float terms[2][3][kCflThreads + 4];
   __shared__ float lane_sum[2][3][4];
   __shared__ float state_x[2];
   __shared__ int state_done[2];

@@ -490,7 +490,8 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
   int bw, int bh, const int32_t* __restrict__ raw,
   const QuantizerState* __restrict__ q, const float* __restrict__ tables,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
-  __shared__ float terms[2][2][kCflTerms];  // [map][a, b][position]
+  // [map][a, b][position], padded so that the chains read distinct banks
+  __shared__ float terms[2][2][kCflTerms + 4];
   __shared__ float lanes[2][2][4];
   __shared__ uint32_t anchor_block[64], anchor_base[64];
   __shared__ int anchors, total;
@@ -561,7 +562,18 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
     const float* a = terms[problem][0];
     const float* b = terms[problem][kind == 0 ? 0 : 1];
     float acc = 0.0f;
-    for (int p = lane; p < terms_used; p += 4) acc = __builtin_fmaf(a[p], b[p], acc);
+    // terms_used is a multiple of 64: batches of 16 terms per lane, each
+    // batch's loads issued before its ordered fmas
+    for (int base = 0; base < terms_used; base += 64) {
+      float va[16], vb[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        va[m] = a[base + lane + 4 * m];
+        vb[m] = b[base + lane + 4 * m];
+      }
+#pragma unroll
+      for (int m = 0; m < 16; ++m) acc = __builtin_fmaf(va[m], vb[m], acc);
+    }
     lanes[problem][kind][lane] = acc;
   }
   __syncthreads();
@@ -931,23 +943,37 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
   constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
   const int s0 = x & (kRing - 1);
   const int sm = (x - 1) & (kRing - 1), sp = (x + 1) & (kRing - 1);
+  // Every ring word the sample may need is read unconditionally (rows y - 1
+  // and y - 2 clamped to valid rows; slots of absent neighbours hold values
+  // that the selects below discard), so that the reads issue together.
+  const int ym1 = y > 0 ? y - 1 : 0, ym2 = y > 1 ? y - 2 : 0;
+  const int32_t q_n = ring.q[s0][ym1], q_nw = ring.q[sm][ym1];
+  const int32_t q_ne = ring.q[sp][ym1], q_nn = ring.q[s0][ym2];
+  const int32_t err_n = ring.err[s0][ym1], err_nw = ring.err[sm][ym1];
+  const int32_t err_ne = ring.err[sp][ym1];
+  uint32_t e_n[4], e_nw[4], e_ne[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    e_n[i] = ring.e[i][s0][ym1];
+    e_nw[i] = ring.e[i][sm][ym1];
+    e_ne[i] = ring.e[i][sp][ym1];
+  }
   // quantized neighbours
-  const int64_t up = y ? ring.q[s0][y - 1] : 0;
+  const int64_t up = y ? q_n : 0;
   const int64_t W_ = x ? st.q1 : up;
   const int64_t N_ = y ? up : W_;
-  const int64_t NW_ = x && y ? ring.q[sm][y - 1] : W_;
-  const int64_t NE_ = y && has_ne ? ring.q[sp][y - 1] : N_;
-  const int64_t NN_ = y > 1 ? ring.q[s0][y - 2] : N_;
+  const int64_t NW_ = x && y ? q_nw : W_;
+  const int64_t NE_ = y && has_ne ? q_ne : N_;
+  const int64_t NN_ = y > 1 ? q_nn : N_;
   // Predict (prediction.h:75-123)
   uint32_t weights[4];
   uint32_t weight_sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const uint32_t en_i = y ? ring.e[i][s0][y - 1] : 0u;
+    const uint32_t en_i = y ? e_n[i] : 0u;
     const uint32_t north = en_i + (x ? st.e1[i] : 0u);
-    const uint32_t ne = has_ne ? (y ? ring.e[i][sp][y - 1] : 0u) : north;
-    const uint32_t nw =
-      x ? (y ? ring.e[i][sm][y - 1] : 0u) + (x > 1 ? st.e2[i] : 0u) : north;
+    const uint32_t ne = has_ne ? (y ? e_ne[i] : 0u) : north;
+    const uint32_t nw = x ? (y ? e_nw[i] : 0u) + (x > 1 ? st.e2[i] : 0u) : north;
     const uint32_t e = north + ne + nw;
     int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
     if (shift < 0) shift = 0;
@@ -962,9 +988,9 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
     weight_sum += weights[i];
   }
   const int64_t ew = x ? st.err1 : 0;
-  const int64_t en = y ? ring.err[s0][y - 1] : 0;
-  const int64_t enw = x ? (y ? ring.err[sm][y - 1] : 0) : en;
-  const int64_t ene = has_ne ? (y ? ring.err[sp][y - 1] : 0) : en;
+  const int64_t en = y ? err_n : 0;
+  const int64_t enw = x ? (y ? err_nw : 0) : en;
+  const int64_t ene = has_ne ? (y ? err_ne : 0) : en;
   const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
   int64_t pred[4];
   pred[0] = w8 + ne8 - n8;
@@ -1012,13 +1038,18 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
 }
 
 // grid: DC groups.  Each work group runs the three channels of its DC group
-// in one wavefront, each with its own ring (3 x 48 KB of LDS): Y, then X,
-// then B, which subtracts this sample's quantized Y (times the Y step) from
-// its DC first, as the reference does after Y's plane is complete.  Each row
-// reads its DC samples kDcPrefetch steps before it needs them.
-constexpr int kDcPrefetch = 4;
+// side by side, each with its own ring (3 x 48 KB of LDS) and its own four
+// waves (thread y of a channel owns row y): Y and X at wavefront step
+// t = x + 2y, B one step behind, since it subtracts the sample's quantized Y
+// (times the Y step) from its DC first, which Y produced in the step before
+// (the reference quantizes B after Y's whole plane).  The steps run in
+// batches of kDcBatch; each row reads its DC samples a batch ahead, and the
+// barrier between steps orders LDS only, so those reads stay in flight.
+constexpr int kDcBatch = 8;
+constexpr int kDcChannelThreads = kDcThreads;
+constexpr int kDcWorkGroup = 3 * kDcChannelThreads;
 
-__global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
+__global__ __launch_bounds__(kDcWorkGroup) void DcQuantKernel(
   const float* __restrict__ dc, int bw, int bh,
   const QuantizerState* __restrict__ qstate, int extra_precision,
   int32_t* __restrict__ quantized) {
@@ -1029,46 +1060,54 @@ __global__ __launch_bounds__(kDcThreads) void DcQuantKernel(
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
   const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
   const size_t n = static_cast<size_t>(bw) * bh;
+  // waves 0-3: Y, 4-7: X, 8-11: B
+  const int slot = threadIdx.x / kDcChannelThreads;
+  const int channel = slot == 0 ? 1 : slot == 1 ? 0 : 2;
+  const int lag = channel == 2 ? 1 : 0;
   const QuantizerState q = *qstate;
   const float precision = static_cast<float>(1u << extra_precision);
-  float inv[3];
-  for (int c = 0; c < 3; ++c) inv[c] = q.inverse_dc_steps[c] * precision;
+  const float inverse_step = channel == 0 ? q.inverse_dc_steps[0]
+                           : channel == 1 ? q.inverse_dc_steps[1] : q.inverse_dc_steps[2];
+  const float inv = inverse_step * precision;
   const float step_y = q.dc_steps[1] / precision;
-  const int y = threadIdx.x;
+  const int y = threadIdx.x % kDcChannelThreads;
   const size_t row = static_cast<size_t>(gy0 + min(y, h - 1)) * bw + gx0;
-  // next kDcPrefetch samples of this row, per channel
-  float ahead[3][kDcPrefetch];
+  const float* src = dc + channel * n + row;
+  // this row's DC samples for the next two batches of steps
+  auto fetch = [&](float (&dst)[kDcBatch], int t0) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int k = 0; k < kDcPrefetch; ++k) {
-      ahead[c][k] = y < h && k < w ? dc[c * n + row + k] : 0.0f;
+    for (int j = 0; j < kDcBatch; ++j) {
+      const int x = t0 + j - lag - 2 * y;
+      dst[j] = y < h && x >= 0 && x < w ? src[x] : 0.0f;
     }
-  }
-  DcRowState st[3];
-  const int steps = w + 2 * h;
-  for (int t = 0; t < steps; ++t) {
-    const int x = t - 2 * y;
-    if (y < h && x >= 0 && x < w) {
-      float value[3];
+  };
+  float cur[kDcBatch], next[kDcBatch];
+  fetch(cur, 0);
+  fetch(next, kDcBatch);
+  DcRowState st;
+  DcRing& own = ring[channel];
+  const int steps = w + 2 * h + 1;
+  for (int t0 = 0; t0 < steps; t0 += kDcBatch) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        value[c] = ahead[c][0];
-#pragma unroll
-        for (int k = 0; k + 1 < kDcPrefetch; ++k) ahead[c][k] = ahead[c][k + 1];
-        ahead[c][kDcPrefetch - 1] =
-          x + kDcPrefetch < w ? dc[c * n + row + x + kDcPrefetch] : 0.0f;
+    for (int j = 0; j < kDcBatch; ++j) {
+      const int x = t0 + j - lag - 2 * y;
+      if (y < h && x >= 0 && x < w) {
+        float value = cur[j];
+        if (channel == 2) {
+          // Y's sample (x, y), written one step ago; its ring slot is
+          // reused only eight samples later
+          const int32_t qy = ring[1].q[x & (kRing - 1)][y];
+          value = __builtin_fmaf(-static_cast<float>(qy), step_y, value);
+        }
+        quantized[channel * n + row + x] =
+          DcSample(own, st, reciprocal, x, y, x + 1 < w, value, inv);
       }
-      const bool has_ne = x + 1 < w;
-      const size_t index = row + x;
-      const int32_t qy = DcSample(ring[1], st[1], reciprocal, x, y, has_ne, value[1], inv[1]);
-      quantized[n + index] = qy;
-      const int32_t qx = DcSample(ring[0], st[0], reciprocal, x, y, has_ne, value[0], inv[0]);
-      quantized[index] = qx;
-      const float b = __builtin_fmaf(-static_cast<float>(qy), step_y, value[2]);
-      quantized[2 * n + index] = DcSample(ring[2], st[2], reciprocal, x, y, has_ne, b, inv[2]);
+      // the ring is the only state shared between rows
+      LdsBarrier();
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kDcBatch; ++j) cur[j] = next[j];
+    fetch(next, t0 + 2 * kDcBatch);
   }
 }
 
@@ -1405,8 +1444,8 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
     return CheckLaunch("CodeTransformKernel");
   }));
   const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 1);
-  hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcThreads), 0, d->stream, dc, l.bw, l.bh,
-                     q, 1, dc_quantized);
+  hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcWorkGroup), 0, d->stream, dc, l.bw,
+                     l.bh, q, 1, dc_quantized);
   GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
   hipLaunchKernelGGL(DcReconstructKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)),
                      dim3(256), 0, d->stream, dc_quantized, n, q, 1, dc_rec);

@@ -63,6 +63,22 @@ __device__ __forceinline__ float ConvolveInterior(const float* s, const float* s
   return r;
 }
 
+// A border output of ConvolutionWithTranspose (fewer than R samples on one
+// side): the taps that fall inside the line, renormalised; sample(sx) gives
+// the line's sample sx.
+template <int R, typename Sample>
+__device__ __forceinline__ float ConvolveBorder(Sample sample, int pos, int len,
+                                                const float* __restrict__ k) {
+  const int lo = pos < R ? 0 : pos - R;
+  const int hi = len - 1 < pos + R ? len - 1 : pos + R;
+  float weight = 0.0f;
+  for (int sx = lo; sx <= hi; ++sx) weight += k[sx + R - pos];
+  const float scale = 1.0f / weight;
+  float sum = 0.0f;
+  for (int sx = lo; sx <= hi; ++sx) sum += sample(sx) * k[sx + R - pos];
+  return sum * scale;
+}
+
 // One output of ConvolutionWithTranspose: position `pos` of a line of `len`
 // samples spaced `stride` floats apart.
 template <int R>
@@ -80,16 +96,8 @@ __device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
     for (int i = 0; i <= 2 * R; ++i) window[i] = s[i * stride];
     return ConvolveInterior<R>(window, sk);
   }
-  const int lo = pos < R ? 0 : pos - R;
-  const int hi = len - 1 < pos + R ? len - 1 : pos + R;
-  float weight = 0.0f;
-  for (int sx = lo; sx <= hi; ++sx) weight += k[sx + R - pos];
-  const float scale = 1.0f / weight;
-  float sum = 0.0f;
-  for (int sx = lo; sx <= hi; ++sx) {
-    sum += line[static_cast<ptrdiff_t>(sx) * stride] * k[sx + R - pos];
-  }
-  return sum * scale;
+  return ConvolveBorder<R>(
+    [&](int sx) { return line[static_cast<ptrdiff_t>(sx) * stride]; }, pos, len, k);
 }
 
 // Rows: a work group blurs kRowSegment pixels of one row from an LDS copy
@@ -133,8 +141,9 @@ __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
   for (int p = 0; p < kRowOutputs; ++p) {
     const int x = xs + p;
     if (x >= w) break;
-    dst[x] = x >= first && x < last ? ConvolveInterior<R>(window + p, sk)
-                                    : ConvolveLine<R>(row, 1, x, w, taps);
+    dst[x] = x >= first && x < last
+               ? ConvolveInterior<R>(window + p, sk)
+               : ConvolveBorder<R>([&](int sx) { return seg[sx - x0 + R]; }, x, w, taps);
   }
 }
 
@@ -455,24 +464,13 @@ __global__ __launch_bounds__(kThreads) void MaltaKernel(
     tile[r][col] = v;
   }
   __syncthreads();
-  // each thread: kMaltaRows pixels down one column, from the 9-wide
-  // neighbourhood of those rows held in registers
-  constexpr int kMaltaRows = kMaltaTileH / (kThreads / kMaltaTileW);
-  constexpr int kNbRows = kMaltaRows + 2 * kMaltaHalo, kNbCols = 2 * kMaltaHalo + 1;
   const int tx = threadIdx.x % kMaltaTileW;
-  const int ty0 = (threadIdx.x / kMaltaTileW) * kMaltaRows;
-  const int x = x0 + tx;
-  if (x >= w) return;
-  float nb[kNbRows][kNbCols];
-#pragma unroll
-  for (int r = 0; r < kNbRows; ++r)
-#pragma unroll
-    for (int c = 0; c < kNbCols; ++c) nb[r][c] = tile[ty0 + r][tx + c];
-#pragma unroll
-  for (int j = 0; j < kMaltaRows; ++j) {
-    const int y = y0 + ty0 + j;
-    if (y >= h) break;
-    auto P = [&](int dx, int dy) -> float { return nb[j + kMaltaHalo + dy][kMaltaHalo + dx]; };
+  for (int ty = threadIdx.x / kMaltaTileW; ty < kMaltaTileH; ty += blockDim.x / kMaltaTileW) {
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= w || y >= h) continue;
+    auto P = [&](int dx, int dy) -> float {
+      return tile[ty + kMaltaHalo + dy][tx + kMaltaHalo + dx];
+    };
     float result = 0.0f;
 #pragma unroll
     for (int l = 0; l < 16; ++l) {

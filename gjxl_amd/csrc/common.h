@@ -138,6 +138,16 @@ __device__ inline void BlockReduceU32(uint32_t (&v)[K], Op op) {
   }
 }
 
+// A work-group barrier that orders LDS accesses only: global loads and stores
+// issued before it stay in flight (a thread's own global traffic needs no
+// barrier, and waiting for it at every step of a wavefront would expose its
+// full latency).
+__device__ __forceinline__ void LdsBarrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Exclusive prefix sum of one word per thread over the work group (thread
 // order), with the group total in *total.  blockDim.x must be a multiple of
 // 64 and at most 1024; every thread of the group must call it.

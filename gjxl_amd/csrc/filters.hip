@@ -62,6 +62,7 @@ constexpr int kLfTileW = 64, kLfTileH = 16;
 constexpr int kRawW = kLfTileW + 8, kRawH = kLfTileH + 8;  // halo 4
 constexpr int kGabW = kLfTileW + 6, kGabH = kLfTileH + 6;  // halo 3
 constexpr int kE1W = kLfTileW + 2, kE1H = kLfTileH + 2;    // halo 1
+static_assert(kE1W * kE1H <= kRawW * kRawH, "pass 1 reuses the input tile's space");
 
 template <int kPass, typename Sample>
 __device__ __forceinline__ void EpfPixel(Sample S, float bis, int x, int y,
@@ -110,9 +111,11 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
   const float* __restrict__ in, int in_stride, size_t in_plane, int w, int h,
   GaborishWeights g, const float* __restrict__ inverse_sigma, InverseOpsin p,
   float* __restrict__ filtered, float* __restrict__ linear) {
-  __shared__ float raw[3][kRawH][kRawW];
+  // the input tile is dead once Gaborish is done: pass 1 reuses its space
+  __shared__ float raw_e1[3 * kRawH * kRawW];
   __shared__ float gab[3][kGabH][kGabW];
-  __shared__ float e1[3][kE1H][kE1W];
+  auto raw = reinterpret_cast<float (*)[kRawH][kRawW]>(raw_e1);
+  auto e1 = reinterpret_cast<float (*)[kE1H][kE1W]>(raw_e1);
   const int x0 = blockIdx.x * kLfTileW, y0 = blockIdx.y * kLfTileH;
   const int bw = (w + 7) / 8;
   const size_t plane = static_cast<size_t>(w) * h;
