@@ -476,13 +476,14 @@ __global__ __launch_bounds__(256) void RawQuantKernel(
 }
 
 // ---- final CfL, fast least squares (chroma_from_luma.cpp:543-670, :112-
-// 134), one work group per 64x64 colour tile.  The work group first forms
-// every term of the tile's anchors (raster order, each transform's
-// coefficients in storage order) in LDS; then sixteen threads run the
-// reference's four-lane fma chains (2 maps x {quadratic, linear} x 4 lanes)
-// over them.
+// 134), one work group per 64x64 colour tile.  The tile's terms (its anchors
+// in raster order, each transform's coefficients in storage order) are
+// formed by the whole work group kCflChunk at a time in LDS; sixteen threads
+// carry the reference's four-lane fma chains (2 maps x {quadratic, linear}
+// x 4 lanes) through the chunks in order.
 
 constexpr int kCflTerms = 64 * 64;  // coefficients of one tile
+constexpr int kCflChunk = 1024;
 
 __global__ __launch_bounds__(256) void FinalCflKernel(
   const float* __restrict__ coefs, size_t store,
@@ -490,8 +491,9 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
   int bw, int bh, const int32_t* __restrict__ raw,
   const QuantizerState* __restrict__ q, const float* __restrict__ tables,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
-  // [map][a, b][position], padded so that the chains read distinct banks
-  __shared__ float terms[2][2][kCflTerms + 4];
+  // [map][a, b][position in chunk], padded so that the chains read
+  // distinct banks
+  __shared__ float terms[2][2][kCflChunk + 4];
   __shared__ float lanes[2][2][4];
   __shared__ uint32_t anchor_block[64], anchor_base[64];
   __shared__ int anchors, total;
@@ -527,55 +529,60 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
   __syncthreads();
   const float scale128 = q->scale * 128.0f;
   const int count = anchors, terms_used = min(total, kCflTerms);
-  for (int p = t; p < terms_used && count > 0; p += blockDim.x) {
-    int lo = 0, hi = count - 1;  // the last anchor starting at or before p
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (anchor_base[mid] <= static_cast<uint32_t>(p)) lo = mid; else hi = mid - 1;
-    }
-    const int bx = anchor_block[lo] & 0xffff, by = anchor_block[lo] >> 16;
-    const int st = cells[static_cast<size_t>(by) * bw + bx] >> 1;
-    const int cw = CoveredW(st), ch = CoveredH(st);
-    const int W = cw * 8, H = ch * 8, N = W * H;
-    const int i = p - static_cast<int>(anchor_base[lo]);
-    if (i >= N) continue;
-    const int cols = W > H ? W : H;
-    const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
-    const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
-    const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
-    const int xx = i % cols, yy = i / cols;
-    const bool llf = xx < lw && yy < lh;
-    const float cy = llf ? 0.0f : coefs[store + off + i];
-#pragma unroll
-    for (int problem = 0; problem < 2; ++problem) {
-      const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
-      const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
-      const float luma = cy * inv[i] * qs;
-      const float chroma = cc * inv[i] * qs;
-      terms[problem][0][p] = luma / 84.0f;
-      terms[problem][1][p] = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
-    }
-  }
-  __syncthreads();
-  if (t < 16) {
-    const int problem = t / 8, kind = (t / 4) % 2, lane = t % 4;
-    const float* a = terms[problem][0];
-    const float* b = terms[problem][kind == 0 ? 0 : 1];
-    float acc = 0.0f;
-    // terms_used is a multiple of 64: batches of 16 terms per lane, each
-    // batch's loads issued before its ordered fmas
-    for (int base = 0; base < terms_used; base += 64) {
-      float va[16], vb[16];
-#pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        va[m] = a[base + lane + 4 * m];
-        vb[m] = b[base + lane + 4 * m];
+  const int problem_t = t / 8, kind_t = (t / 4) % 2, lane_t = t % 4;
+  float acc = 0.0f;
+  for (int first = 0; first < terms_used; first += kCflChunk) {
+    const int chunk = min(kCflChunk, terms_used - first);
+    for (int j = t; j < chunk && count > 0; j += blockDim.x) {
+      const int p = first + j;
+      int lo = 0, hi = count - 1;  // the last anchor starting at or before p
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (anchor_base[mid] <= static_cast<uint32_t>(p)) lo = mid; else hi = mid - 1;
       }
+      const int bx = anchor_block[lo] & 0xffff, by = anchor_block[lo] >> 16;
+      const int st = cells[static_cast<size_t>(by) * bw + bx] >> 1;
+      const int cw = CoveredW(st), ch = CoveredH(st);
+      const int W = cw * 8, H = ch * 8, N = W * H;
+      const int i = p - static_cast<int>(anchor_base[lo]);
+      if (i >= N) continue;
+      const int cols = W > H ? W : H;
+      const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
+      const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
+      const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
+      const int xx = i % cols, yy = i / cols;
+      const bool llf = xx < lw && yy < lh;
+      const float cy = llf ? 0.0f : coefs[store + off + i];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) acc = __builtin_fmaf(va[m], vb[m], acc);
+      for (int problem = 0; problem < 2; ++problem) {
+        const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
+        const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
+        const float luma = cy * inv[i] * qs;
+        const float chroma = cc * inv[i] * qs;
+        terms[problem][0][j] = luma / 84.0f;
+        terms[problem][1][j] = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
+      }
     }
-    lanes[problem][kind][lane] = acc;
+    __syncthreads();
+    if (t < 16) {
+      const float* a = terms[problem_t][0];
+      const float* b = terms[problem_t][kind_t == 0 ? 0 : 1];
+      // chunk is a multiple of 64: batches of 16 terms per lane, each
+      // batch's loads issued before its ordered fmas
+      for (int base = 0; base < chunk; base += 64) {
+        float va[16], vb[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+          va[m] = a[base + lane_t + 4 * m];
+          vb[m] = b[base + lane_t + 4 * m];
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc = __builtin_fmaf(va[m], vb[m], acc);
+      }
+    }
+    __syncthreads();
   }
+  if (t < 16) lanes[problem_t][kind_t][lane_t] = acc;
   __syncthreads();
   if (t < 2) {
     const float (*l)[4] = lanes[t];
