@@ -100,50 +100,48 @@ __device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
     [&](int sx) { return line[static_cast<ptrdiff_t>(sx) * stride]; }, pos, len, k);
 }
 
-// Rows: a work group blurs kRowSegment pixels of one row from an LDS copy
-// of the segment and its halo; each thread produces kRowOutputs neighbouring
-// outputs from one register window (read as float4).
-constexpr int kRowOutputs = 4;
-constexpr int kRowSegment = kThreads * kRowOutputs;
+// Rows: a work group blurs a tile of kRowTileH rows x kRowTileW pixels from
+// an LDS copy of the tile and its halo; each thread produces kRowOutputs
+// outputs of one column, each from a window read across consecutive lanes
+// (conflict-free).
+constexpr int kRowTileW = 64, kRowTileH = 16;
+constexpr int kRowOutputs = kRowTileH / (kThreads / kRowTileW);
 
 template <int R>
 __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   float* __restrict__ out) {
-  constexpr int kWindow = kRowOutputs + 2 * R;
-  constexpr int kWindow4 = (kWindow + 3) / 4;
-  constexpr int kLds = kRowSegment + 2 * R + 4;
-  __shared__ __align__(16) float seg[kLds];
-  const int y = blockIdx.y, x0 = blockIdx.x * kRowSegment;
-  const float* row = in + static_cast<size_t>(y) * w;
-  for (int i = threadIdx.x; i < kLds; i += kThreads) {
-    const int gx = x0 - R + i;
-    seg[i] = gx >= 0 && gx < w ? row[gx] : 0.0f;
+  constexpr int kStride = kRowTileW + 2 * R + 1;  // odd: rows start in distinct banks
+  __shared__ float tile[kRowTileH][kStride];
+  const int x0 = blockIdx.x * kRowTileW, y0 = blockIdx.y * kRowTileH;
+  for (int e = threadIdx.x; e < kRowTileH * (kRowTileW + 2 * R); e += kThreads) {
+    const int r = e / (kRowTileW + 2 * R), col = e % (kRowTileW + 2 * R);
+    const int gx = x0 - R + col, gy = y0 + r;
+    tile[r][col] = gx >= 0 && gx < w && gy < h ? in[static_cast<size_t>(gy) * w + gx] : 0.0f;
   }
   __syncthreads();
-  const int xs = x0 + threadIdx.x * kRowOutputs;
-  if (xs >= w) return;
-  float window[kWindow4 * 4];
-  const float4* src = reinterpret_cast<const float4*>(seg + threadIdx.x * kRowOutputs);
-#pragma unroll
-  for (int j = 0; j < kWindow4; ++j) {
-    const float4 v = src[j];
-    window[4 * j] = v.x;
-    window[4 * j + 1] = v.y;
-    window[4 * j + 2] = v.z;
-    window[4 * j + 3] = v.w;
-  }
+  const int lx = threadIdx.x % kRowTileW, x = x0 + lx;
+  const int ly0 = (threadIdx.x / kRowTileW) * kRowOutputs;
+  if (x >= w) return;
   const float* sk = taps + 33;
   const int first = w < R ? w : R;
   const int last = w > R ? w - R : 0;
-  float* dst = out + static_cast<size_t>(y) * w;
+  const bool interior = x >= first && x < last;
 #pragma unroll
-  for (int p = 0; p < kRowOutputs; ++p) {
-    const int x = xs + p;
-    if (x >= w) break;
-    dst[x] = x >= first && x < last
-               ? ConvolveInterior<R>(window + p, sk)
-               : ConvolveBorder<R>([&](int sx) { return seg[sx - x0 + R]; }, x, w, taps);
+  for (int j = 0; j < kRowOutputs; ++j) {
+    const int y = y0 + ly0 + j;
+    if (y >= h) break;
+    const float* row = tile[ly0 + j];
+    float r;
+    if (interior) {
+      float window[2 * R + 1];
+#pragma unroll
+      for (int i = 0; i <= 2 * R; ++i) window[i] = row[lx + i];
+      r = ConvolveInterior<R>(window, sk);
+    } else {
+      r = ConvolveBorder<R>([&](int sx) { return row[sx - x0 + R]; }, x, w, taps);
+    }
+    out[static_cast<size_t>(y) * w + x] = r;
   }
 }
 
@@ -772,8 +770,9 @@ template <int R>
 GJXLResult Blur(hipStream_t st, const float* in, int w, int h, const float* taps,
                 float* tmp, float* out) {
   hipLaunchKernelGGL(BlurRowsKernel<R>,
-                     dim3(CeilDiv(static_cast<uint32_t>(w), kRowSegment), h), dim3(kThreads),
-                     0, st, in, w, h, taps, tmp);
+                     dim3(CeilDiv(static_cast<uint32_t>(w), kRowTileW),
+                          CeilDiv(static_cast<uint32_t>(h), kRowTileH)),
+                     dim3(kThreads), 0, st, in, w, h, taps, tmp);
   hipLaunchKernelGGL(BlurColsKernel<R>,
                      dim3(CeilDiv(static_cast<uint32_t>(w), kThreads),
                           CeilDiv(static_cast<uint32_t>(h), kColOutputs)),
