@@ -347,12 +347,14 @@ class AqSession:
         _check(library().gjxl_amd_aq_session_create(
             device.handle, width, height, ctypes.byref(handle)))
         self.handle = handle.value
+        self._borrowed = None
         device._children.add(self)
 
     def close(self) -> None:
         if self.handle:
             library().gjxl_amd_aq_session_destroy(self.handle)
             self.handle = None
+        self._borrowed = None
 
     def __del__(self):
         try:
@@ -363,9 +365,13 @@ class AqSession:
     def prepare(self, opsin: DeviceArray, preprocessed: DeviceArray,
                 cells: DeviceArray, initial_quant_field: DeviceArray,
                 target: float = 1.0) -> None:
+        """The inputs stay borrowed by the session until it is closed or
+        prepared again (the evaluations read the strategy cells), so the
+        session keeps them alive."""
         _check(library().gjxl_amd_aq_prepare(
             self.handle, opsin.ptr, preprocessed.ptr, cells.ptr,
             initial_quant_field.ptr, target))
+        self._borrowed = (opsin, preprocessed, cells, initial_quant_field)
 
     def policy(self) -> dict:
         lo, hi, qdc = _F(), _F(), _F()

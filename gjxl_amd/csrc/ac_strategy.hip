@@ -270,6 +270,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   __shared__ float A[KC * N];
   __shared__ float B[KC * N];
   __shared__ double T[KC * H * TS];
+  __shared__ float mask[K * N];  // pixel mask of each candidate, raster order
   __shared__ int nnz[K][3];
   __shared__ float sums[K][3][2];
   __shared__ float qn[K];
@@ -284,19 +285,62 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   const int bw = width / 8, bh = height / 8;
   const int tiles_w = (bw + 7) / 8;
   const int tx = blockIdx.y % tiles_w, ty = blockIdx.y / tiles_w;
+  const int tbw = min(8, bw - tx * 8), tbh = min(8, bh - ty * 8);
   const size_t plane = static_cast<size_t>(width) * height;
   // binary64 bases, read with uniform (scalar) loads
   const double* fhT = bases + BasisOffsetT(W);        // [x][u]
   const double* fvT = bases + BasisOffsetT(H);        // [y][v]
   const double* ih = bases + BasisOffset(W, false);   // [u][x]
   const double* iv = bases + BasisOffset(H, false);   // [v][y]
+  // candidate k of this work group: anchor block and whether it is launched
+  auto candidate = [&](int k, int* bx, int* by) {
+    const int p = blockIdx.x * K + k;
+    const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
+    *bx = tx * 8 + lx;
+    *by = ty * 8 + ly;
+    return p < S::kPositions && lx + S::kCw <= tbw && ly + S::kCh <= tbh;
+  };
+
+  // Every global read of the work group is issued here, before any result
+  // is needed: pixels and mask for LDS, and each thread's quantisation table
+  // entries for the elements it quantises later.
+  constexpr int kPer = (KC * N + kCostThreads - 1) / kCostThreads;
+  float pixel[kPer], dq_e[kPer], inv_e[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    pixel[j] = dq_e[j] = inv_e[j] = 0.0f;
+    if (e < KC * N) {
+      const int kc = e / N, r = e % N;
+      const int k = kc / 3, c = kc % 3;
+      int bx, by;
+      if (candidate(k, &bx, &by)) {
+        pixel[j] = opsin[c * plane + static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W];
+      }
+      // element e is coefficient (v, u) = (r / W, r % W) in the quantise step
+      const int i = CoefficientIndex<W, H>(r / W, r % W);
+      dq_e[j] = tables[DequantOffset(kStrategy, c) + i];
+      inv_e[j] = tables[InverseOffset(kStrategy, c) + i];
+    }
+  }
+  constexpr int kMaskPer = (K * N + kCostThreads - 1) / kCostThreads;
+  float mask_e[kMaskPer];
+#pragma unroll
+  for (int j = 0; j < kMaskPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    mask_e[j] = 0.0f;
+    if (e < K * N) {
+      const int k = e / N, r = e % N;
+      int bx, by;
+      if (candidate(k, &bx, &by)) {
+        mask_e[j] = pixel_mask[static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W];
+      }
+    }
+  }
 
   if (tid < K) {
-    const int p = blockIdx.x * K + tid;
-    const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
-    const int bx = tx * 8 + lx, by = ty * 8 + ly;
-    const int tbw = min(8, bw - tx * 8), tbh = min(8, bh - ty * 8);
-    const int ok = p < S::kPositions && lx + S::kCw <= tbw && ly + S::kCh <= tbh;
+    int bx, by;
+    const int ok = candidate(tid, &bx, &by);
     valid[tid] = ok;
     bxs[tid] = bx;
     bys[tid] = by;
@@ -330,18 +374,19 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     }
     for (int c = 0; c < 3; ++c) nnz[tid][c] = 0;
   }
-  __syncthreads();
-
-  // Load pixels transposed (invalid candidates load zeros and are discarded).
-  for (int e = tid; e < KC * N; e += kCostThreads) {
-    const int kc = e / N, r = e % N;
-    const int k = kc / 3, c = kc % 3;
-    const int y = r / W, x = r % W;
-    float v = 0.0f;
-    if (valid[k]) {
-      v = opsin[c * plane + static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
+  // pixels transposed ([x][y]) and the mask into LDS
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    if (e < KC * N) {
+      const int kc = e / N, r = e % N;
+      A[kc * N + (r % W) * H + r / W] = pixel[j];
     }
-    A[kc * N + x * H + y] = v;
+  }
+#pragma unroll
+  for (int j = 0; j < kMaskPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    if (e < K * N) mask[e] = mask_e[j];
   }
   __syncthreads();
 
@@ -391,7 +436,6 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   __syncthreads();
   // Quantise (src/codec/ac_strategy.cpp:914-926): sqrt(|rounded|) into A in
   // storage order; residuals staged in registers, then into B transposed.
-  constexpr int kPer = (KC * N + kCostThreads - 1) / kCostThreads;
   float residual[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
@@ -402,13 +446,11 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       const int k = kc / 3, c = kc % 3;
       const int v = rem / W, u = rem % W;
       const int i = CoefficientIndex<W, H>(v, u);
-      const float* dq = tables + DequantOffset(kStrategy, c);
-      const float* inv = tables + InverseOffset(kStrategy, c);
       const float dec = B[e] - B[(k * 3 + 1) * N + rem] * cfl[k][c];
-      const float scaled = dec * inv[i] * qn[k];
+      const float scaled = dec * inv_e[j] * qn[k];
       const float rounded = roundf(scaled);
       const float diff = scaled - rounded;
-      residual[j] = dq[i] * diff;
+      residual[j] = dq_e[j] * diff;
       A[kc * N + i] = sqrtf(fabsf(rounded));
       if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
     }
@@ -463,8 +505,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
         const int y = g * kOut + j;
         float weighted = 0.0f;
         if (valid[k]) {
-          const float m =
-            pixel_mask[static_cast<size_t>(bys[k] * 8 + y) * width + bxs[k] * 8 + x];
+          const float m = mask[k * N + y * W + x];
           weighted = (m + kMaskOffset[c]) * static_cast<float>(acc[j]);
           weighted *= weighted;
           weighted *= weighted;
