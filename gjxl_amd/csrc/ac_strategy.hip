@@ -486,12 +486,13 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   }
   __syncthreads();
   // Inverse vertical and the masked 8th power (:935-952) into B, raster.
-  constexpr float kMaskOffset[3] = {12.0f, 0.0f, 4.0f};
+  // 12, 0, 4 for X, Y, B (selects, not a table: no memory access)
   for (int task = wave; task < kColBlocks * (H / kOut); task += kWaves) {
     const int g = task % (H / kOut), col = (task / (H / kOut)) * 64 + lane;
     if (col < kCols) {
       const int kc = col / W, x = col % W;
       const int k = kc / 3, c = kc % 3;
+      const float mask_offset = c == 0 ? 12.0f : c == 2 ? 4.0f : 0.0f;
       const double* src = T + kc * H * TS + x;
       double acc[kOut] = {};
 #pragma unroll 4
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
         float weighted = 0.0f;
         if (valid[k]) {
           const float m = mask[k * N + y * W + x];
-          weighted = (m + kMaskOffset[c]) * static_cast<float>(acc[j]);
+          weighted = (m + mask_offset) * static_cast<float>(acc[j]);
           weighted *= weighted;
           weighted *= weighted;
           weighted *= weighted;

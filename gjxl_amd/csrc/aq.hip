@@ -648,6 +648,7 @@ __device__ void AdjustQuantFromTerms(const float2* __restrict__ terms,
   float nz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int yh = 0; yh < 2; ++yh) {
+#pragma unroll 1
     for (int y = yh * rows / 2; y < (yh + 1) * rows / 2; ++y) {
 #pragma unroll
       for (int xh = 0; xh < 2; ++xh) {
@@ -666,11 +667,13 @@ __device__ void AdjustQuantFromTerms(const float2* __restrict__ terms,
   // border terms in raster order: x = cols - 1 on rows [4 ysize, 7 ysize),
   // x >= 7 xsize on rows [7 ysize, rows - 1), x >= 4 xsize on the last row
   float border = 0.0f;
+  const float thr_q[4] = {thr[0], thr[1], thr[2], thr[3]};
   auto add_border = [&](int y, int x) {
     const int index = y * cols + x;
-    const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
+    const bool lower = y >= rows / 2, right = x >= cols / 2;
+    const float threshold = lower ? (right ? thr_q[3] : thr_q[2]) : (right ? thr_q[1] : thr_q[0]);
     const float value = coef[index] * (inv[index] * qac * mult);
-    if (AdjustQuantized(value, thr[quadrant]) != 0.0f) border += fabsf(value);
+    if (AdjustQuantized(value, threshold) != 0.0f) border += fabsf(value);
   };
   for (int y = 4 * ysize; y < 7 * ysize; ++y) add_border(y, cols - 1);
   for (int y = 7 * ysize; y < rows - 1; ++y)
@@ -698,10 +701,10 @@ __device__ void AdjustQuantFromTerms(const float2* __restrict__ terms,
     }
   }
   const float all_nz = nz[0] + nz[1] + nz[2] + nz[3] + 1.0f;
-  constexpr float kBorder[3] = {70.0f, 30.0f, 60.0f};
-  if (kBorder[channel] * border >= all_nz) {
+  const float border_weight = channel == 0 ? 70.0f : channel == 1 ? 30.0f : 60.0f;
+  if (border_weight * border >= all_nz) {
     *raw_quant = static_cast<int32_t>(static_cast<float>(*raw_quant) +
-                                      kBorder[channel] * border / all_nz);
+                                      border_weight * border / all_nz);
     if (*raw_quant >= 256) *raw_quant = 255;
   }
   if (strategy == kDct8 && nz[0] + nz[1] + nz[2] + nz[3] < 11.0f) {
@@ -773,7 +776,6 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
   LoadSlots<K>(entries, count, offsets, bw, slots);
   __syncthreads();
   const QuantizerState q = *qstate;
-  const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
   for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
     const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
     const TransformSlot& t = slots[k];
@@ -793,9 +795,11 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
         const float qac =
           q.scale * static_cast<float>(raw[static_cast<size_t>(t.by) * bw + t.bx]);
         const float* inv = tables + InverseOffset(kStrategy, c);
-        const int quadrant = (y >= rows / 2) * 2 + (x >= cols / 2);
-        const float value = coef[k][c][i] * (inv[i] * qac * mults[c]);
-        const float quantized = AdjustQuantized(value, thr0[quadrant]);
+        // thresholds 1-3 start equal; quadrant 0 has its own
+        const bool q0 = y < rows / 2 && x < cols / 2;
+        const float mult = c == 0 ? params.x_mult : c == 2 ? params.b_mult : 1.0f;
+        const float value = coef[k][c][i] * (inv[i] * qac * mult);
+        const float quantized = AdjustQuantized(value, q0 ? thr0[0] : thr0[1]);
         term = make_float2(fabsf(value - quantized), fabsf(quantized));
       }
       terms[k][c][i] = term;
@@ -811,7 +815,8 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
       int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
       const float qac = q.scale * static_cast<float>(r);
       AdjustQuantFromTerms<cols, rows>(terms[k][c], coef[k][c],
-                                       tables + InverseOffset(kStrategy, c), qac, mults[c],
+                                       tables + InverseOffset(kStrategy, c), qac,
+                                       c == 0 ? params.x_mult : c == 2 ? params.b_mult : 1.0f,
                                        c, kStrategy, thr, &r);
       cand[k][c] = r;
       if (c == 1) {
@@ -886,17 +891,19 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
     const int k = e / (2 * N), c = ((e / N) % 2) * 2, i = e % N;
     const TransformSlot& t = slots[k];
     if (!t.ok) continue;
-    float thr[4] = {0.58f, 0.62f, 0.62f, 0.62f};
-    if (covered >= 4)
-      for (int j = 0; j < 4; ++j)
-        thr[j] = DStdMax(0.5f, thr[j] - 0.00744f * static_cast<float>(covered));
+    // quadrant 0 starts at 0.58, quadrants 1-3 at 0.62
+    const float thr0 = covered >= 4 ? DStdMax(0.5f, 0.58f - 0.00744f * static_cast<float>(covered))
+                                    : 0.58f;
+    const float thr1 = covered >= 4 ? DStdMax(0.5f, 0.62f - 0.00744f * static_cast<float>(covered))
+                                    : 0.62f;
     const float* inv = tables + InverseOffset(kStrategy, c);
-    const float qscale = q.scale * static_cast<float>(decided[k]) * mults[c];
+    const float qscale = q.scale * static_cast<float>(decided[k]) *
+                         (c == 0 ? params.x_mult : params.b_mult);
     const int x = i % cols, y = i / cols;
     const int tq = (y >= rows / 2) * 2 + (x >= cols / 2);
     const float value = inv[i] * qscale * coef[k][c][i];
     int32_t qv = 0;
-    if (!(fabsf(value) < thr[tq])) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
+    if (!(fabsf(value) < (tq == 0 ? thr0 : thr1))) qv = static_cast<int32_t>(rint(static_cast<double>(value)));
     quantized[c * store + t.offset + i] = qv;
   }
 }
