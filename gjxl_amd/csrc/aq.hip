@@ -19,6 +19,7 @@
 //   ReconstructKernel       ReconstructQuantizedCoefficients
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "aq_common.h"
 #include "common.h"
@@ -916,14 +917,14 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
 // (x, y) is e(x, y-1) + e(x-1, y) because the update of (x-1, y) adds its
 // errors into the previous row's slot x; "north-west" is
 // e(x-1, y-1) + e(x-2, y); "north-east" e(x+1, y-1).  Every input of (x, y)
-// is therefore final at wavefront step t = x + 2y.  One work group walks the
-// wavefront of one DC group (<= 256 x 256 blocks): thread y owns row y and
-// keeps its own last two samples in registers; rows y-1 and y-2 read the
-// samples they need (x-1, x, x+1 of row y-1, x of row y-2, produced 1 to 4
-// steps earlier) from an 8-slot ring in LDS that its owner writes each step.
+// is therefore final at wavefront step t = x + 2y.  A work group walks the
+// wavefront of one DC group (<= 256 x 256 blocks): the thread that owns row y
+// keeps the row's last two samples in registers; row y reads the samples it
+// needs from rows y-1 and y-2 (x-1, x, x+1 of row y-1, x of row y-2,
+// produced 1 to 4 steps earlier) from an 8-slot ring in LDS that their
+// owners write each step.
 
 constexpr int kDcGroup = 256;
-constexpr int kDcThreads = 256;
 constexpr int kRing = 8;
 
 struct DcRing {
@@ -937,8 +938,12 @@ struct DcRing {
 // 128 entries covers both (filled per work group).
 constexpr int kReciprocals = 128;
 
-__device__ __forceinline__ int BitWidth64(uint64_t v) {
-  return v == 0 ? 0 : 64 - __clzll(static_cast<long long>(v));
+// Bit width of e + 1 for a 32-bit e (33 for 2^32 - 1).
+__device__ __forceinline__ int BitWidthPlusOne(uint32_t e) {
+  return e == 0xffffffffu ? 33 : 32 - __clz(static_cast<int>(e + 1));
+}
+__device__ __forceinline__ int BitWidth32(uint32_t v) {
+  return 32 - __clz(static_cast<int>(v));
 }
 
 // The predictor state a row keeps in registers: its own last two samples.
@@ -947,12 +952,27 @@ struct DcRowState {
   uint32_t e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
 };
 
+// Bounds of the 32-bit evaluation: while every quantized value stays below
+// 2^18 and every stored prediction error below 2^22 in magnitude, each
+// sub-prediction is below 2^23, the weighted sum below 2^28 and the
+// prediction below 2^24, so 32-bit arithmetic gives the 64-bit results.
+constexpr int32_t kDcNarrowQ = 1 << 18;
+constexpr int32_t kDcNarrowErr = 1 << 22;
+
 // One sample (x, y) of one channel: weighted prediction from the ring and
 // the row state, quantisation of the residual, update of the ring slot and
-// the row state.  Returns the quantized DC value.
+// the row state.  Returns the quantized DC value.  kWide evaluates the
+// predictor in 64-bit integers as the reference does; otherwise in 32-bit
+// integers, exact while the inputs keep the bounds above, and `overflow` is
+// set when this sample's outputs leave them (later samples would then read
+// out-of-bound inputs).
+template <bool kWide>
 __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
                                             const uint32_t* reciprocal, int x, int y,
-                                            bool has_ne, float value, float inv) {
+                                            bool has_ne, float value, float inv,
+                                            bool& overflow) {
+  using I = std::conditional_t<kWide, int64_t, int32_t>;
+  using U = std::conditional_t<kWide, uint64_t, uint32_t>;
   constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
   constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
   const int s0 = x & (kRing - 1);
@@ -973,12 +993,12 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
     e_ne[i] = ring.e[i][sp][ym1];
   }
   // quantized neighbours
-  const int64_t up = y ? q_n : 0;
-  const int64_t W_ = x ? st.q1 : up;
-  const int64_t N_ = y ? up : W_;
-  const int64_t NW_ = x && y ? q_nw : W_;
-  const int64_t NE_ = y && has_ne ? q_ne : N_;
-  const int64_t NN_ = y > 1 ? q_nn : N_;
+  const I up = y ? q_n : 0;
+  const I W_ = x ? st.q1 : up;
+  const I N_ = y ? up : W_;
+  const I NW_ = x && y ? q_nw : W_;
+  const I NE_ = y && has_ne ? q_ne : N_;
+  const I NN_ = y > 1 ? q_nn : N_;
   // Predict (prediction.h:75-123)
   uint32_t weights[4];
   uint32_t weight_sum = 0;
@@ -989,41 +1009,51 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
     const uint32_t ne = has_ne ? (y ? e_ne[i] : 0u) : north;
     const uint32_t nw = x ? (y ? e_nw[i] : 0u) + (x > 1 ? st.e2[i] : 0u) : north;
     const uint32_t e = north + ne + nw;
-    int shift = BitWidth64(static_cast<uint64_t>(e) + 1) - 6;
+    int shift = BitWidthPlusOne(e) - 6;
     if (shift < 0) shift = 0;
-    weights[i] = 4 + ((kMaxWeights[i] * reciprocal[e >> shift]) >> shift);
+    // k * r as k * (r - 1) + k: r - 1 < 2^24 fits the 24-bit multiplier
+    const uint32_t k = kMaxWeights[i];
+    weights[i] = 4 + ((__umul24(reciprocal[e >> shift] - 1, k) + k) >> shift);
     weight_sum += weights[i];
   }
-  const unsigned log_weight = BitWidth64(weight_sum) - 1;
+  const unsigned log_weight = BitWidth32(weight_sum) - 1;
   weight_sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     weights[i] >>= log_weight - 4;
     weight_sum += weights[i];
   }
-  const int64_t ew = x ? st.err1 : 0;
-  const int64_t en = y ? err_n : 0;
-  const int64_t enw = x ? (y ? err_nw : 0) : en;
-  const int64_t ene = has_ne ? (y ? err_ne : 0) : en;
-  const int64_t n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
-  int64_t pred[4];
+  const I ew = x ? st.err1 : 0;
+  const I en = y ? err_n : 0;
+  const I enw = x ? (y ? err_nw : 0) : en;
+  const I ene = has_ne ? (y ? err_ne : 0) : en;
+  const I n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
+  I pred[4];
   pred[0] = w8 + ne8 - n8;
   pred[1] = n8 - (((en + ew + ene) * kCoeff[0]) >> 5);
   pred[2] = w8 - (((en + ew + enw) * kCoeff[1]) >> 5);
   pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
                    (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
-  int64_t sum = static_cast<int64_t>(static_cast<uint32_t>((weight_sum >> 1) - 1));
+  I sum = static_cast<I>((weight_sum >> 1) - 1);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) sum += pred[i] * static_cast<int64_t>(weights[i]);
-  int64_t prediction = (sum * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24;
+  for (int i = 0; i < 4; ++i) {
+    // within the 32-bit bounds |pred| < 2^23 fits the signed 24-bit multiplier
+    if constexpr (kWide) {
+      sum += pred[i] * static_cast<I>(weights[i]);
+    } else {
+      sum += __mul24(pred[i], static_cast<int32_t>(weights[i]));
+    }
+  }
+  I prediction = static_cast<I>(
+    (static_cast<int64_t>(sum) * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24);
   if (((en ^ ew) | (en ^ enw)) <= 0) {
-    int64_t lo = w8 < ne8 ? w8 : ne8;
+    I lo = w8 < ne8 ? w8 : ne8;
     lo = lo < n8 ? lo : n8;
-    int64_t hi = w8 > ne8 ? w8 : ne8;
+    I hi = w8 > ne8 ? w8 : ne8;
     hi = hi > n8 ? hi : n8;
     prediction = prediction < lo ? lo : (prediction > hi ? hi : prediction);
   }
-  const int64_t predicted = (prediction + 3) >> 3;
+  const I predicted = (prediction + 3) >> 3;
   // quantise the residual (dc_quantization.cpp:176-203)
   float residual = value * inv;
   residual -= static_cast<float>(predicted);
@@ -1031,98 +1061,159 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
   float rounded = roundf(residual);
   if (rounded > 2 || rounded < -2) rounded = roundf(residual * 0.5f) * 2;
   const int32_t ir = static_cast<int32_t>(roundf(rounded));
-  const int32_t value_q = static_cast<int32_t>(predicted + ir);
-  // Update (prediction.h:124-144)
-  const int64_t scaled = static_cast<int64_t>(value_q) * 8;
-  const int32_t err = static_cast<int32_t>(prediction - scaled);
+  // the low 32 bits of predicted + ir
+  const int32_t value_q =
+    static_cast<int32_t>(static_cast<uint32_t>(predicted) + static_cast<uint32_t>(ir));
+  // Update (prediction.h:124-144); unsigned arithmetic wraps where a 32-bit
+  // evaluation leaves its bounds (the sample is then flagged)
+  const I scaled = static_cast<I>(static_cast<U>(static_cast<I>(value_q)) << 3);
+  const int32_t err = static_cast<int32_t>(static_cast<U>(prediction) - static_cast<U>(scaled));
   ring.q[s0][y] = value_q;
   ring.err[s0][y] = err;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    int64_t dd = pred[i] - scaled;
-    if (dd < 0) dd = -dd;
-    const uint32_t ei = static_cast<uint32_t>(static_cast<uint64_t>(dd + 3) >> 3);
+    I dd = static_cast<I>(static_cast<U>(pred[i]) - static_cast<U>(scaled));
+    const U magnitude = dd < 0 ? U(0) - static_cast<U>(dd) : static_cast<U>(dd);
+    const uint32_t ei = static_cast<uint32_t>((magnitude + 3) >> 3);
     ring.e[i][s0][y] = ei;
     st.e2[i] = st.e1[i];
     st.e1[i] = ei;
   }
   st.q1 = value_q;
   st.err1 = err;
+  if (!kWide) {
+    overflow |= static_cast<uint32_t>(value_q) + kDcNarrowQ >= 2u * kDcNarrowQ ||
+                static_cast<uint32_t>(err) + kDcNarrowErr >= 2u * kDcNarrowErr;
+  }
   return value_q;
 }
 
-// grid: DC groups.  Each work group runs the three channels of its DC group
-// side by side, each with its own ring (3 x 48 KB of LDS) and its own four
-// waves (thread y of a channel owns row y): Y and X at wavefront step
-// t = x + 2y, B one step behind, since it subtracts the sample's quantized Y
-// (times the Y step) from its DC first, which Y produced in the step before
-// (the reference quantizes B after Y's whole plane).  The steps run in
-// batches of kDcBatch; each row reads its DC samples a batch ahead, and the
-// barrier between steps orders LDS only, so those reads stay in flight.
+// grid: DC groups x 2.  Work group z = 0 quantises Y and X, z = 1 Y again
+// (without storing it) and B, which subtracts the sample's quantized Y
+// (times the Y step) from its DC first; Y produces that sample one step
+// earlier, so B runs one step behind (the reference quantizes B after Y's
+// whole plane).  Each channel has its own ring (2 x 48 KB of LDS, so two
+// such work groups never share a CU) and two waves: thread r owns rows r
+// and r + 128, whose samples fall in disjoint step ranges (row y is active
+// at steps 2y .. 2y + w - 1 and w <= 256), so each of the four waves of a
+// work group is busy on most steps and each SIMD runs one of them.  The steps
+// run in batches of kDcBatch; each thread reads its DC samples a batch ahead,
+// and the barrier between steps orders LDS only, so those reads stay in
+// flight.  A group is evaluated in 32-bit arithmetic first and again in
+// 64-bit arithmetic if any of its samples left the 32-bit bounds.
 constexpr int kDcBatch = 8;
-constexpr int kDcChannelThreads = kDcThreads;
-constexpr int kDcWorkGroup = 3 * kDcChannelThreads;
+constexpr int kDcRowThreads = kDcGroup / 2;
+constexpr int kDcWorkGroup = 2 * kDcRowThreads;
+
+struct DcWavefront {
+  const float* src;      // this channel's DC plane
+  int32_t* dst;          // this channel's quantized plane
+  size_t row_a, row_b;   // offsets of rows r and r + kDcRowThreads
+  int r, w, h, lag, slot, channel;
+  bool has_a, has_b, store;
+  float inv, step_y;
+
+  // the row and column of this thread's sample at step t; row < 0 if none
+  __device__ __forceinline__ void Locate(int t, int& row, int& x) const {
+    const int xa = t - lag - 2 * r;
+    const int xb = xa - 2 * kDcRowThreads;
+    if (has_a && xa >= 0 && xa < w) {
+      row = r;
+      x = xa;
+    } else if (has_b && xb >= 0 && xb < w) {
+      row = r + kDcRowThreads;
+      x = xb;
+    } else {
+      row = -1;
+      x = 0;
+    }
+  }
+
+  __device__ __forceinline__ void Fetch(float (&out)[kDcBatch], int t0) const {
+#pragma unroll
+    for (int j = 0; j < kDcBatch; ++j) {
+      int row, x;
+      Locate(t0 + j, row, x);
+      out[j] = row < 0 ? 0.0f : src[(row == r ? row_a : row_b) + x];
+    }
+  }
+
+  // Walks the group's wavefront; returns whether a 32-bit evaluation left
+  // its bounds on this thread.
+  template <bool kWide>
+  __device__ __forceinline__ bool Run(DcRing (&ring)[2], const uint32_t* reciprocal) const {
+    float cur[kDcBatch], next[kDcBatch];
+    Fetch(cur, 0);
+    Fetch(next, kDcBatch);
+    DcRowState st;
+    DcRing& own = ring[slot];
+    bool overflow = false;
+    const int steps = w + 2 * h + 1;
+    for (int t0 = 0; t0 < steps; t0 += kDcBatch) {
+#pragma unroll
+      for (int j = 0; j < kDcBatch; ++j) {
+        int y, x;
+        Locate(t0 + j, y, x);
+        if (y >= 0) {
+          float value = cur[j];
+          if (channel == 2) {
+            // Y's sample (x, y), written one step ago; its ring slot is
+            // reused only eight samples later
+            const int32_t qy = ring[0].q[x & (kRing - 1)][y];
+            value = __builtin_fmaf(-static_cast<float>(qy), step_y, value);
+          }
+          // the second row starts at x = 0, which reads no row state
+          const int32_t v =
+            DcSample<kWide>(own, st, reciprocal, x, y, x + 1 < w, value, inv, overflow);
+          if (store) dst[(y == r ? row_a : row_b) + x] = v;
+        }
+        // the rings are the only state shared between rows
+        LdsBarrier();
+      }
+#pragma unroll
+      for (int j = 0; j < kDcBatch; ++j) cur[j] = next[j];
+      Fetch(next, t0 + 2 * kDcBatch);
+    }
+    return overflow;
+  }
+};
 
 __global__ __launch_bounds__(kDcWorkGroup) void DcQuantKernel(
   const float* __restrict__ dc, int bw, int bh,
   const QuantizerState* __restrict__ qstate, int extra_precision,
   int32_t* __restrict__ quantized) {
-  __shared__ DcRing ring[3];
+  __shared__ DcRing ring[2];  // [0]: Y, [1]: X or B
   __shared__ uint32_t reciprocal[kReciprocals];
   if (threadIdx.x < kReciprocals) reciprocal[threadIdx.x] = (1u << 24) / (threadIdx.x + 1);
   __syncthreads();
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
-  const int w = min(kDcGroup, bw - gx0), h = min(kDcGroup, bh - gy0);
   const size_t n = static_cast<size_t>(bw) * bh;
-  // waves 0-3: Y, 4-7: X, 8-11: B
-  const int slot = threadIdx.x / kDcChannelThreads;
-  const int channel = slot == 0 ? 1 : slot == 1 ? 0 : 2;
-  const int lag = channel == 2 ? 1 : 0;
+  const bool with_b = blockIdx.z == 1;
+  DcWavefront f;
+  f.w = min(kDcGroup, bw - gx0);
+  f.h = min(kDcGroup, bh - gy0);
+  // waves 0-1: Y, waves 2-3: X (z = 0) or B (z = 1)
+  f.slot = threadIdx.x / kDcRowThreads;
+  f.channel = f.slot == 0 ? 1 : with_b ? 2 : 0;
+  f.lag = f.channel == 2 ? 1 : 0;
+  f.store = f.slot == 1 || !with_b;
   const QuantizerState q = *qstate;
   const float precision = static_cast<float>(1u << extra_precision);
-  const float inverse_step = channel == 0 ? q.inverse_dc_steps[0]
-                           : channel == 1 ? q.inverse_dc_steps[1] : q.inverse_dc_steps[2];
-  const float inv = inverse_step * precision;
-  const float step_y = q.dc_steps[1] / precision;
-  const int y = threadIdx.x % kDcChannelThreads;
-  const size_t row = static_cast<size_t>(gy0 + min(y, h - 1)) * bw + gx0;
-  const float* src = dc + channel * n + row;
-  // this row's DC samples for the next two batches of steps
-  auto fetch = [&](float (&dst)[kDcBatch], int t0) {
-#pragma unroll
-    for (int j = 0; j < kDcBatch; ++j) {
-      const int x = t0 + j - lag - 2 * y;
-      dst[j] = y < h && x >= 0 && x < w ? src[x] : 0.0f;
-    }
-  };
-  float cur[kDcBatch], next[kDcBatch];
-  fetch(cur, 0);
-  fetch(next, kDcBatch);
-  DcRowState st;
-  DcRing& own = ring[channel];
-  const int steps = w + 2 * h + 1;
-  for (int t0 = 0; t0 < steps; t0 += kDcBatch) {
-#pragma unroll
-    for (int j = 0; j < kDcBatch; ++j) {
-      const int x = t0 + j - lag - 2 * y;
-      if (y < h && x >= 0 && x < w) {
-        float value = cur[j];
-        if (channel == 2) {
-          // Y's sample (x, y), written one step ago; its ring slot is
-          // reused only eight samples later
-          const int32_t qy = ring[1].q[x & (kRing - 1)][y];
-          value = __builtin_fmaf(-static_cast<float>(qy), step_y, value);
-        }
-        quantized[channel * n + row + x] =
-          DcSample(own, st, reciprocal, x, y, x + 1 < w, value, inv);
-      }
-      // the ring is the only state shared between rows
-      LdsBarrier();
-    }
-#pragma unroll
-    for (int j = 0; j < kDcBatch; ++j) cur[j] = next[j];
-    fetch(next, t0 + 2 * kDcBatch);
-  }
+  const float inverse_step = f.channel == 0 ? q.inverse_dc_steps[0]
+                           : f.channel == 1 ? q.inverse_dc_steps[1] : q.inverse_dc_steps[2];
+  f.inv = inverse_step * precision;
+  f.step_y = q.dc_steps[1] / precision;
+  f.r = threadIdx.x % kDcRowThreads;
+  // rows r (first) and r + kDcRowThreads (second), clamped into the group
+  f.row_a = static_cast<size_t>(gy0 + min(f.r, f.h - 1)) * bw + gx0;
+  f.row_b = static_cast<size_t>(gy0 + min(f.r + kDcRowThreads, f.h - 1)) * bw + gx0;
+  f.has_a = f.r < f.h;
+  f.has_b = f.r + kDcRowThreads < f.h;
+  f.src = dc + f.channel * n;
+  f.dst = quantized + f.channel * n;
+  const bool overflow = f.Run<false>(ring, reciprocal);
+  // the rerun rewrites every sample the first walk stored
+  if (__syncthreads_or(overflow)) f.Run<true>(ring, reciprocal);
 }
 
 __global__ __launch_bounds__(256) void DcReconstructKernel(
@@ -1457,7 +1548,7 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
                        l.bw, raw, y_to_x, y_to_b, q, d->quant_tables, p, quantized, dc, n);
     return CheckLaunch("CodeTransformKernel");
   }));
-  const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 1);
+  const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 2);
   hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcWorkGroup), 0, d->stream, dc, l.bw,
                      l.bh, q, 1, dc_quantized);
   GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));

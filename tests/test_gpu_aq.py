@@ -133,3 +133,30 @@ def test_aq_evaluation_full_size_4032x3024(device):
     device.synchronize()
     _check_evaluation(s, fe, setup, setup["adjusted"], w, h)
     s.close()
+
+
+def test_aq_dc_wide_groups(device):
+    """The DC wavefront evaluates each DC group in 32-bit arithmetic and
+    reruns a group in 64-bit arithmetic once a quantized value reaches 2^18
+    (or a prediction error 2^22): here the left DC group's XYB is scaled by
+    400 (Y DC values near 4e5), the right group keeps ordinary values."""
+    w, h = 2104, 48
+    print(f"aq dc wide groups {w}x{h}", flush=True)
+    srgb = np.ascontiguousarray(synthetic.generate(w, h, 5, "textured")[:h, :w])
+    fe = binding.frontend(srgb)
+    pre = fe["pre"].copy()
+    pre[:, :, :1024] *= np.float32(400)
+    fe = dict(fe, pre=pre)
+    setup = binding.aq_setup(fe, w, h)
+    d_ops, d_pre, d_cells, d_field = _device_inputs(device, fe)
+    s = gjxl_amd.AqSession(device, w, h)
+    s.prepare(d_ops, d_pre, d_cells, d_field, 1.0)
+    device.synchronize()
+    _check_prepare(s, setup)
+    s.evaluate(s.buffer("adjusted_field", (s.bh, s.bw)))
+    device.synchronize()
+    want = binding.aq_evaluate(fe, setup, setup["adjusted"], w, h)
+    assert np.abs(want["dc_q"][:, :, :256]).max() >= 2**18
+    assert np.abs(want["dc_q"][:, :, 256:]).max() < 2**18
+    _check_evaluation(s, fe, setup, setup["adjusted"], w, h)
+    s.close()
