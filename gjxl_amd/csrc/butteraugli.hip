@@ -65,17 +65,34 @@ __device__ __forceinline__ float ConvolveInterior(const float* s, const float* s
 
 // A border output of ConvolutionWithTranspose (fewer than R samples on one
 // side): the taps that fall inside the line, renormalised; sample(sx) gives
-// the line's sample sx.
+// the line's sample sx.  Every tap is visited in ascending order and the
+// ones outside the line are skipped by selects, so the sums are the
+// reference's; samples are read at clamped positions, which stay within
+// pos - R .. pos + R.
 template <int R, typename Sample>
 __device__ __forceinline__ float ConvolveBorder(Sample sample, int pos, int len,
                                                 const float* __restrict__ k) {
-  const int lo = pos < R ? 0 : pos - R;
-  const int hi = len - 1 < pos + R ? len - 1 : pos + R;
   float weight = 0.0f;
-  for (int sx = lo; sx <= hi; ++sx) weight += k[sx + R - pos];
+#pragma unroll
+  for (int i = 0; i <= 2 * R; ++i) {
+    const int sx = pos - R + i;
+    const float next = weight + k[i];
+    weight = sx >= 0 && sx < len ? next : weight;
+  }
   const float scale = 1.0f / weight;
+  float v[2 * R + 1];
+#pragma unroll
+  for (int i = 0; i <= 2 * R; ++i) {
+    const int sx = pos - R + i;
+    v[i] = sample(sx < 0 ? 0 : sx < len ? sx : len - 1);
+  }
   float sum = 0.0f;
-  for (int sx = lo; sx <= hi; ++sx) sum += sample(sx) * k[sx + R - pos];
+#pragma unroll
+  for (int i = 0; i <= 2 * R; ++i) {
+    const int sx = pos - R + i;
+    const float next = sum + v[i] * k[i];
+    sum = sx >= 0 && sx < len ? next : sum;
+  }
   return sum * scale;
 }
 
@@ -100,77 +117,116 @@ __device__ __forceinline__ float ConvolveLine(const float* __restrict__ line,
     [&](int sx) { return line[static_cast<ptrdiff_t>(sx) * stride]; }, pos, len, k);
 }
 
-// Rows: a work group blurs a tile of kRowTileH rows x kRowTileW pixels from
-// an LDS copy of the tile and its halo; each thread produces kRowOutputs
-// outputs of one column, each from a window read across consecutive lanes
-// (conflict-free).
-constexpr int kRowTileW = 64, kRowTileH = 16;
-constexpr int kRowOutputs = kRowTileH / (kThreads / kRowTileW);
+// Rows: a work group blurs kRowRows rows of kRowSpan pixels, one wave per
+// row, from an LDS copy of the rows and their halo.  Each lane produces four
+// adjacent outputs from one register window that it reads with 16-byte LDS
+// reads (lanes read consecutive 16-byte words: conflict-free).
+constexpr int kRowSpan = 256;
+constexpr int kRowLanes = kRowSpan / 4;
+constexpr int kRowRows = kThreads / kRowLanes;
 
 template <int R>
 __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   float* __restrict__ out) {
-  constexpr int kStride = kRowTileW + 2 * R + 1;  // odd: rows start in distinct banks
-  __shared__ float tile[kRowTileH][kStride];
-  const int x0 = blockIdx.x * kRowTileW, y0 = blockIdx.y * kRowTileH;
-  for (int e = threadIdx.x; e < kRowTileH * (kRowTileW + 2 * R); e += kThreads) {
-    const int r = e / (kRowTileW + 2 * R), col = e % (kRowTileW + 2 * R);
-    const int gx = x0 - R + col, gy = y0 + r;
-    tile[r][col] = gx >= 0 && gx < w && gy < h ? in[static_cast<size_t>(gy) * w + gx] : 0.0f;
+  // samples x - R .. x + 3 + R of a lane's outputs x .. x + 3, whole float4s
+  constexpr int kWindow = (2 * R + 4 + 3) / 4 * 4;
+  constexpr int kUsed = kRowSpan - 4 + kWindow;
+  // whole loads per lane, so that every load issues before the first store
+  constexpr int kLoads = (kUsed + kRowLanes - 1) / kRowLanes;
+  constexpr int kStride = kLoads * kRowLanes;
+  __shared__ __align__(16) float tile[kRowRows][kStride];
+  const int x0 = blockIdx.x * kRowSpan;
+  const int row = threadIdx.x / kRowLanes, lane = threadIdx.x % kRowLanes;
+  const int y = blockIdx.y * kRowRows + row;
+  // tile column c holds sample x0 - R + c of the row
+  const float* src = in + static_cast<size_t>(y < h ? y : h - 1) * w;
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) {
+    const int c = lane + k * kRowLanes;
+    const int gx = x0 - R + c;
+    // clamped reads and a select instead of a branch per load
+    const float v = src[gx < 0 ? 0 : gx < w ? gx : w - 1];
+    tile[row][c] = gx >= 0 && gx < w && y < h ? v : 0.0f;
   }
   __syncthreads();
-  const int lx = threadIdx.x % kRowTileW, x = x0 + lx;
-  const int ly0 = (threadIdx.x / kRowTileW) * kRowOutputs;
-  if (x >= w) return;
+  if (y >= h) return;
+  const float* line = tile[row];
   const float* sk = taps + 33;
   const int first = w < R ? w : R;
   const int last = w > R ? w - R : 0;
-  const bool interior = x >= first && x < last;
+  // interior outputs: four adjacent ones per lane
+  const int x = x0 + 4 * lane;
+  float* dst = out + static_cast<size_t>(y) * w;
+  if (x >= first && x + 3 < last) {
+    float window[kWindow];
+    const float4* s4 = reinterpret_cast<const float4*>(&line[4 * lane]);
 #pragma unroll
-  for (int j = 0; j < kRowOutputs; ++j) {
-    const int y = y0 + ly0 + j;
-    if (y >= h) break;
-    const float* row = tile[ly0 + j];
-    float r;
-    if (interior) {
-      float window[2 * R + 1];
-#pragma unroll
-      for (int i = 0; i <= 2 * R; ++i) window[i] = row[lx + i];
-      r = ConvolveInterior<R>(window, sk);
-    } else {
-      r = ConvolveBorder<R>([&](int sx) { return row[sx - x0 + R]; }, x, w, taps);
+    for (int m = 0; m < kWindow / 4; ++m) {
+      const float4 v = s4[m];
+      window[4 * m] = v.x;
+      window[4 * m + 1] = v.y;
+      window[4 * m + 2] = v.z;
+      window[4 * m + 3] = v.w;
     }
-    out[static_cast<size_t>(y) * w + x] = r;
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = ConvolveInterior<R>(window + j, sk);
+    if ((w & 3) == 0) {
+      // rows start on 16-byte boundaries
+      *reinterpret_cast<float4*>(dst + x) = make_float4(r[0], r[1], r[2], r[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) dst[x + j] = r[j];
+    }
+  } else {
+    // a lane that straddles an end of the interior
+#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+      const int xj = x + j;
+      if (xj >= first && xj < last && xj < w) {
+        dst[xj] = ConvolveInterior<R>(line + 4 * lane + j, sk);
+      }
+    }
+  }
+  // border outputs (x < first or x >= max(first, last)), one per lane: at
+  // most R at each end of the row
+  const int left = min(max(first - x0, 0), kRowSpan);
+  const int right_begin = max(max(last, first), x0);
+  const int right = max(min(w, x0 + kRowSpan) - right_begin, 0);
+  if (lane < left + right) {
+    const int xb = lane < left ? x0 + lane : right_begin + (lane - left);
+    dst[xb] = ConvolveBorder<R>([&](int sx) { return line[sx - x0 + R]; }, xb, w, taps);
   }
 }
 
-// Columns: each thread produces kColOutputs outputs down one column from one
-// register window (coalesced across the wave).
+// Columns: each thread produces kColOutputs outputs down one column from
+// one register window (coalesced across the wave).
 constexpr int kColOutputs = 8;
 
 template <int R>
 __global__ __launch_bounds__(kThreads) void BlurColsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   float* __restrict__ out) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kColOutputs;
+  constexpr int kOutputs = kColOutputs;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kOutputs;
   if (x >= w) return;
   const int first = h < R ? h : R;
   const int last = h > R ? h - R : 0;
-  if (y0 >= first && y0 + kColOutputs - 1 < last) {
-    float window[kColOutputs + 2 * R];
+  if (y0 >= first && y0 + kOutputs - 1 < last) {
+    float window[kOutputs + 2 * R];
 #pragma unroll
-    for (int j = 0; j < kColOutputs + 2 * R; ++j) {
+    for (int j = 0; j < kOutputs + 2 * R; ++j) {
       window[j] = in[static_cast<size_t>(y0 - R + j) * w + x];
     }
     const float* sk = taps + 33;
 #pragma unroll
-    for (int p = 0; p < kColOutputs; ++p) {
+    for (int p = 0; p < kOutputs; ++p) {
       out[static_cast<size_t>(y0 + p) * w + x] = ConvolveInterior<R>(window + p, sk);
     }
     return;
   }
-  for (int p = 0; p < kColOutputs; ++p) {
+  for (int p = 0; p < kOutputs; ++p) {
     const int y = y0 + p;
     if (y >= h) break;
     out[static_cast<size_t>(y) * w + x] = ConvolveLine<R>(in + x, w, y, h, taps);
@@ -770,8 +826,8 @@ template <int R>
 GJXLResult Blur(hipStream_t st, const float* in, int w, int h, const float* taps,
                 float* tmp, float* out) {
   hipLaunchKernelGGL(BlurRowsKernel<R>,
-                     dim3(CeilDiv(static_cast<uint32_t>(w), kRowTileW),
-                          CeilDiv(static_cast<uint32_t>(h), kRowTileH)),
+                     dim3(CeilDiv(static_cast<uint32_t>(w), kRowSpan),
+                          CeilDiv(static_cast<uint32_t>(h), kRowRows)),
                      dim3(kThreads), 0, st, in, w, h, taps, tmp);
   hipLaunchKernelGGL(BlurColsKernel<R>,
                      dim3(CeilDiv(static_cast<uint32_t>(w), kThreads),
