@@ -424,28 +424,30 @@ struct MaltaScale {
 };
 
 // The scaled difference MaltaDiffMap feeds its lines (butteraugli_distance_
-// internal.cpp:512-555).
+// internal.cpp:512-555).  Its four corrections s -/+ ds2 * (...) are one
+// s + ds2 * -(dv1 + k) with k = +-too_small or +-too_big picked by selects:
+// x - y is x + (-y) and -(a - b) is b - a exactly, so the result is the
+// reference's.
 __device__ __forceinline__ float MaltaScaledDiff(float v0, float v1, const MaltaScale& m) {
   const float absolute = 0.5f * (fabsf(v0) + fabsf(v1));
   const float diff = v0 - v1;
   const float scaler = m.gt / (m.norm + absolute);
-  float s = scaler * diff;
+  const float s = scaler * diff;
   const float scaler2 = m.lt / (m.norm + absolute);
   const double magnitude = fabs(static_cast<double>(v0));
   const double too_small = 0.55 * magnitude, too_big = 1.05 * magnitude;
   const double dv1 = v1, ds2 = scaler2;
-  if (v0 < 0.0f) {
-    if (dv1 > -too_small) {
-      s = static_cast<float>(static_cast<double>(s) - ds2 * (dv1 + too_small));
-    } else if (dv1 < -too_big) {
-      s = static_cast<float>(static_cast<double>(s) + ds2 * (-dv1 - too_big));
-    }
-  } else if (dv1 < too_small) {
-    s = static_cast<float>(static_cast<double>(s) + ds2 * (too_small - dv1));
-  } else if (dv1 > too_big) {
-    s = static_cast<float>(static_cast<double>(s) - ds2 * (dv1 - too_big));
-  }
-  return s;
+  // v0 < 0: dv1 > -too_small: s - ds2 * (dv1 + too_small)
+  //         dv1 < -too_big:   s + ds2 * (-dv1 - too_big)
+  // v0 >= 0: dv1 < too_small: s + ds2 * (too_small - dv1)
+  //          dv1 > too_big:   s - ds2 * (dv1 - too_big)
+  const bool negative = v0 < 0.0f;
+  const bool near = negative ? dv1 > -too_small : dv1 < too_small;
+  const bool far = negative ? dv1 < -too_big : dv1 > too_big;
+  const double k = near ? too_small : too_big;
+  const double t = -(dv1 + (negative ? k : -k));
+  const float corrected = static_cast<float>(static_cast<double>(s) + ds2 * t);
+  return near || far ? corrected : s;
 }
 
 __device__ __forceinline__ float Sum4(float a, float b, float c, float d) {
@@ -497,54 +499,107 @@ constexpr MaltaTap kFullLines[16][9] = {
 // MaltaDiffMap fused: a work group scales the differences of a 64x16 tile
 // plus a 4-pixel halo into LDS (zero outside the image, as the reference's
 // ReadWithZeroBorder), then evaluates the 16 Malta lines of every pixel and
-// writes 0 + response (the reference accumulates into a zeroed plane).
-constexpr int kMaltaTileW = 64, kMaltaTileH = 16, kMaltaHalo = 4;
+// writes 0 + response (the reference accumulates into a zeroed plane).  Each
+// thread produces four adjacent outputs of one row, for two rows, each from
+// a 9 x 12 register window that it reads with 16-byte LDS reads (the sixteen
+// threads of a row read consecutive words: conflict-free).
+constexpr int kMaltaTileW = 64, kMaltaTileH = 32, kMaltaHalo = 4;
 constexpr int kMaltaLw = kMaltaTileW + 2 * kMaltaHalo, kMaltaLh = kMaltaTileH + 2 * kMaltaHalo;
+constexpr int kMaltaQuads = kMaltaTileW / 4;
+constexpr int kMaltaWinW = 4 + 2 * kMaltaHalo, kMaltaWinH = 1 + 2 * kMaltaHalo;
+constexpr int kMaltaLoads = (kMaltaLh * kMaltaLw + kThreads - 1) / kThreads;
+constexpr int kMaltaRowsPerPass = kThreads / kMaltaQuads;
+static_assert(kMaltaTileH == 2 * kMaltaRowsPerPass, "two rows of four outputs per thread");
+static_assert(kMaltaLw % 4 == 0, "16-byte aligned LDS rows");
 
 template <bool kLowFrequency>
 __global__ __launch_bounds__(kThreads) void MaltaKernel(
   const float* __restrict__ p0, const float* __restrict__ p1, int w, int h,
   MaltaScale m, float* __restrict__ out) {
-  __shared__ float tile[kMaltaLh][kMaltaLw];
+  __shared__ __align__(16) float tile[kMaltaLh][kMaltaLw];
   const int x0 = blockIdx.x * kMaltaTileW, y0 = blockIdx.y * kMaltaTileH;
-  for (int e = threadIdx.x; e < kMaltaLh * kMaltaLw; e += blockDim.x) {
+  // every load of the tile issues before the first scaled difference
+  float v0[kMaltaLoads], v1[kMaltaLoads];
+#pragma unroll
+  for (int k = 0; k < kMaltaLoads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
     const int r = e / kMaltaLw, col = e % kMaltaLw;
     const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-    float v = 0.0f;
-    if (sx >= 0 && sy >= 0 && sx < w && sy < h) {
-      const size_t i = static_cast<size_t>(sy) * w + sx;
-      v = MaltaScaledDiff(p0[i], p1[i], m);
+    const size_t i = static_cast<size_t>(min(max(sy, 0), h - 1)) * w + min(max(sx, 0), w - 1);
+    v0[k] = p0[i];
+    v1[k] = p1[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kMaltaLoads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    const int r = e / kMaltaLw, col = e % kMaltaLw;
+    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+    const bool inside = sx >= 0 && sy >= 0 && sx < w && sy < h;
+    if (e < kMaltaLh * kMaltaLw) {
+      tile[r][col] = inside ? MaltaScaledDiff(v0[k], v1[k], m) : 0.0f;
     }
-    tile[r][col] = v;
   }
   __syncthreads();
-  const int tx = threadIdx.x % kMaltaTileW;
-  for (int ty = threadIdx.x / kMaltaTileW; ty < kMaltaTileH; ty += blockDim.x / kMaltaTileW) {
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= w || y >= h) continue;
-    auto P = [&](int dx, int dy) -> float {
-      return tile[ty + kMaltaHalo + dy][tx + kMaltaHalo + dx];
-    };
-    float result = 0.0f;
+  const int q = threadIdx.x % kMaltaQuads;
+  const int x = x0 + 4 * q;
+  if (x >= w) return;
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ty = threadIdx.x / kMaltaQuads + pass * kMaltaRowsPerPass;
+    const int y = y0 + ty;
+    if (y >= h) return;
+    // win[r][c] = tile[ty + r][4q + c]: rows y - 4 .. y + 4, columns x - 4 ..
+    // x + 7
+    float win[kMaltaWinH][kMaltaWinW];
 #pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      float s;
-      if constexpr (kLowFrequency) {
-        const MaltaTap* t = kLfLines[l];
-        s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
-                 P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
-      } else {
-        const MaltaTap* t = kFullLines[l];
-        float v[9];
+    for (int r = 0; r < kMaltaWinH; ++r) {
+      const float4* s4 = reinterpret_cast<const float4*>(&tile[ty + r][4 * q]);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
-        s = kFullTaps[l] == 9
-              ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
-              : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
+      for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
+        const float4 v = s4[c4];
+        win[r][4 * c4] = v.x;
+        win[r][4 * c4 + 1] = v.y;
+        win[r][4 * c4 + 2] = v.z;
+        win[r][4 * c4 + 3] = v.w;
       }
-      result = l == 0 ? s * s : s * s + result;
     }
-    out[static_cast<size_t>(y) * w + x] = 0.0f + result;
+    float res[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      auto P = [&](int dx, int dy) -> float {
+        return win[kMaltaHalo + dy][j + kMaltaHalo + dx];
+      };
+      float result = 0.0f;
+#pragma unroll
+      for (int l = 0; l < 16; ++l) {
+        float s;
+        if constexpr (kLowFrequency) {
+          const MaltaTap* t = kLfLines[l];
+          s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
+                   P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
+        } else {
+          const MaltaTap* t = kFullLines[l];
+          float v[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
+          s = kFullTaps[l] == 9
+                ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
+                : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
+        }
+        result = l == 0 ? s * s : s * s + result;
+      }
+      res[j] = 0.0f + result;
+    }
+    float* dst = out + static_cast<size_t>(y) * w + x;
+    if ((w & 3) == 0) {
+      // rows start on 16-byte boundaries and x + 3 < w
+      *reinterpret_cast<float4*>(dst) = make_float4(res[0], res[1], res[2], res[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (x + j < w) dst[j] = res[j];
+      }
+    }
   }
 }
 
