@@ -51,6 +51,21 @@ __device__ __forceinline__ float CflDerivative(float a, float residual) {
 }
 
 constexpr int kCflThreads = 256;
+// Newton phase: waves 1-3 produce the derivative terms of kCflChunk elements
+// at a time (kCflPer each) into one of two LDS buffers while wave 0's 24
+// summing lanes add up the previous chunk.
+constexpr int kCflProducers = kCflThreads - 64;
+constexpr int kCflPer = 4;
+constexpr int kCflChunk = kCflProducers * kCflPer;
+constexpr int kCflChunks = (4096 + kCflChunk - 1) / kCflChunk;
+// terms[buffer][problem][kind][lane][m] holds the term of chunk element
+// 4m + lane, so that each summing lane reads its elements with 16-byte
+// reads; the stride (196 = 4 mod 32 words) puts the eight lanes of each
+// 16-byte read group on distinct banks
+constexpr int kCflLaneCount = kCflChunk / 4;
+constexpr int kCflTermStride = kCflLaneCount + 4;
+constexpr int kCflSumBatch = 16;  // divides every chunk's per-lane count (n is a multiple of 64)
+constexpr int kCflSumBatches = kCflLaneCount / kCflSumBatch;
 
 // One work group per 64x64 colour tile (src/codec/chroma_from_luma.cpp:
 // 434-529): DCT8 of each block and channel, LLF cleared, scaled by the DCT8
@@ -58,16 +73,22 @@ constexpr int kCflThreads = 256;
 // (block-raster then coefficient order), kept as the per-element a = luma /
 // 84 and b = base * luma - chroma of both maps; then up to 20 damped Newton
 // steps per map (:40-94, :112-165).  Each step's derivative terms are formed
-// by all threads, 256 elements at a time, and summed by one thread per lane
-// of the reference's 4-lane sums, in element order.
+// by the producer threads from values held in registers, a chunk at a time,
+// and summed by one thread per lane of the reference's 4-lane sums, in
+// element order, one chunk behind.
 __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
   const float* __restrict__ opsin, int width, int height,
   const float* __restrict__ tables, const double* __restrict__ bases,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
-  // DCT phase: X, Y, B coefficients in values[0..2]; then a_x, b_x, a_b, b_b
-  __shared__ float values[4][4096];
-  // padded so that the 24 summing lanes read 24 different banks
-  __shared__ float terms[2][3][kCflThreads + 4];
+  // DCT phase: X, Y, B coefficients, then a_x, b_x, a_b, b_b, in values;
+  // Newton phase: two buffers of terms[problem][kind][element] in the same
+  // memory
+  __shared__ __align__(16) float lds[4 * 4096];
+  float (*values)[4096] = reinterpret_cast<float (*)[4096]>(lds);
+  float (*terms)[2][3][4][kCflTermStride] =
+    reinterpret_cast<float (*)[2][3][4][kCflTermStride]>(lds);
+  static_assert(2 * 2 * 3 * 4 * kCflTermStride <= 4 * 4096, "term buffers fit");
+  static_assert(kCflProducers % 4 == 0, "a producer's elements share one lane");
   __shared__ float lane_sum[2][3][4];
   __shared__ float state_x[2];
   __shared__ int state_done[2];
@@ -122,31 +143,37 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
   }
   __syncthreads();
   const int n = nblocks * 64;
-  constexpr int kPer = 4096 / kCflThreads;
-  float yv[kPer], xv[kPer], bv[kPer];
+  // the producers' elements: chunk c, slot m -> c * kCflChunk + q + m *
+  // kCflProducers; a and b of both problems
+  const bool producer = tid >= 64;
+  const int q = tid - 64;
+  float pa[kCflChunks][kCflPer][2], pb[kCflChunks][kCflPer][2];
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int i = tid + j * kCflThreads;
-    yv[j] = i < n ? values[1][i] : 0.0f;
-    xv[j] = i < n ? values[0][i] : 0.0f;
-    bv[j] = i < n ? values[2][i] : 0.0f;
-  }
-  __syncthreads();
+  for (int c = 0; c < kCflChunks; ++c) {
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int i = tid + j * kCflThreads;
-    if (i >= n) continue;
-    const float ix = inv_x[i & 63], ib = inv_b[i & 63];
-    const float luma_x = yv[j] * ix, chroma_x = xv[j] * ix;
-    const float luma_b = yv[j] * ib, chroma_b = bv[j] * ib;
-    values[0][i] = luma_x / 84.0f;
-    values[1][i] = 0.0f * luma_x - chroma_x;
-    values[2][i] = luma_b / 84.0f;
-    values[3][i] = 1.0f * luma_b - chroma_b;
+    for (int m = 0; m < kCflPer; ++m) {
+      const int i = c * kCflChunk + q + m * kCflProducers;
+      float ax = 0.0f, bxv = 0.0f, ab = 0.0f, bbv = 0.0f;
+      if (producer && i < n) {
+        const float ix = inv_x[i & 63], ib = inv_b[i & 63];
+        const float yv = values[1][i], xv = values[0][i], bv = values[2][i];
+        const float luma_x = yv * ix, chroma_x = xv * ix;
+        const float luma_b = yv * ib, chroma_b = bv * ib;
+        ax = luma_x / 84.0f;
+        bxv = 0.0f * luma_x - chroma_x;
+        ab = luma_b / 84.0f;
+        bbv = 1.0f * luma_b - chroma_b;
+      }
+      pa[c][m][0] = ax;
+      pb[c][m][0] = bxv;
+      pa[c][m][1] = ab;
+      pb[c][m][1] = bbv;
+    }
   }
-  __syncthreads();
+  __syncthreads();  // the values memory becomes the term buffers
 
   const float count = static_cast<float>(n);
+  const int chunks = (n + kCflChunk - 1) / kCflChunk;
   constexpr float kEps = 100.0f;
   // summing threads: problem p, derivative kind, lane
   const int sp = tid / 12, skind = (tid / 4) % 3, slane = tid % 4;
@@ -154,42 +181,69 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
     const int done0 = state_done[0], done1 = state_done[1];
     if (done0 && done1) break;
     const float x0 = state_x[0], x1 = state_x[1];
-    float acc = 0.0f;
-    for (int chunk = 0; chunk < n; chunk += kCflThreads) {
-      const int i = chunk + tid;
+    // terms of chunk c into buffer c & 1 (c is a constant after unrolling)
+    auto produce = [&](int c) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         if (p == 0 ? done0 : done1) continue;
-        float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
-        if (i < n) {
-          const float a = values[2 * p][i], b = values[2 * p + 1][i];
-          const float x = p == 0 ? x0 : x1;
+        const float x = p == 0 ? x0 : x1;
+#pragma unroll
+        for (int m = 0; m < kCflPer; ++m) {
+          const float a = pa[c][m][p], b = pb[c][m][p];
+          float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
           const float cr = __builtin_fmaf(a, x, b);
           if (!(fabsf(cr) >= 100.0f)) {
             t0 = CflDerivative(a, cr);
             t1 = CflDerivative(a, __builtin_fmaf(a, x + kEps, b));
             t2 = CflDerivative(a, __builtin_fmaf(a, x - kEps, b));
           }
-        }
-        terms[p][0][tid] = t0;
-        terms[p][1][tid] = t1;
-        terms[p][2][tid] = t2;
-      }
-      __syncthreads();
-      if (tid < 24 && !(sp == 0 ? done0 : done1)) {
-        // n is a multiple of 64: whole batches of 16 terms per lane, each
-        // batch's loads issued before its ordered adds
-        const int limit = min(kCflThreads, n - chunk);
-        const float* src = &terms[sp][skind][slane];
-        for (int base = 0; base < limit; base += 64) {
-          float v[16];
-#pragma unroll
-          for (int m = 0; m < 16; ++m) v[m] = src[base + 4 * m];
-#pragma unroll
-          for (int m = 0; m < 16; ++m) acc += v[m];
+          // element q + m * kCflProducers: lane q & 3, slot q / 4 + m *
+          // kCflProducers / 4
+          const int slot = (q >> 2) + m * (kCflProducers / 4);
+          terms[c & 1][p][0][q & 3][slot] = t0;
+          terms[c & 1][p][1][q & 3][slot] = t1;
+          terms[c & 1][p][2][q & 3][slot] = t2;
         }
       }
-      __syncthreads();
+    };
+    if (producer) produce(0);
+    __syncthreads();
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kCflChunks; ++c) {
+      if (c < chunks) {
+        if (tid < 24 && !(sp == 0 ? done0 : done1)) {
+          // n is a multiple of 64, so the lane's count is a whole number of
+          // batches; every batch is read (in bounds) two batches ahead of
+          // its ordered adds, which only the batches in the chunk make
+          const int batches = min(kCflChunk, n - c * kCflChunk) / 4 / kCflSumBatch;
+          const float4* src =
+            reinterpret_cast<const float4*>(&terms[c & 1][sp][skind][slane][0]);
+          float4 r[3][kCflSumBatch / 4];
+          auto load = [&](int b, float4 (&dst)[kCflSumBatch / 4]) {
+#pragma unroll
+            for (int m = 0; m < kCflSumBatch / 4; ++m) dst[m] = src[b * (kCflSumBatch / 4) + m];
+          };
+          load(0, r[0]);
+          load(1, r[1]);
+#pragma unroll
+          for (int b = 0; b < kCflSumBatches; ++b) {
+            if (b + 2 < kCflSumBatches) load(b + 2, r[(b + 2) % 3]);
+            if (b < batches) {
+#pragma unroll
+              for (int m = 0; m < kCflSumBatch / 4; ++m) {
+                const float4 v = r[b % 3][m];
+                acc += v.x;
+                acc += v.y;
+                acc += v.z;
+                acc += v.w;
+              }
+            }
+          }
+        }
+        if (c + 1 < kCflChunks && producer && c + 1 < chunks) produce(c + 1);
+        __syncthreads();
+      }
     }
     if (tid < 24) lane_sum[sp][skind][slane] = acc;
     __syncthreads();
