@@ -13,6 +13,7 @@
 // strategy (3072 coefficient slots: 1x32x32, 2x32x16, 4x16x16, 8x16x8 or
 // 16x8x8), three channels each, entirely in LDS.
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 #include "tables.h"
@@ -66,6 +67,10 @@ constexpr int kCflLaneCount = kCflChunk / 4;
 constexpr int kCflTermStride = kCflLaneCount + 4;
 constexpr int kCflSumBatch = 16;  // divides every chunk's per-lane count (n is a multiple of 64)
 constexpr int kCflSumBatches = kCflLaneCount / kCflSumBatch;
+// the read-ahead past a lane's last batch (at most four batches) stays
+// inside the LDS array
+static_assert((2 * 2 * 3 * 4 - 1) * (kCflLaneCount + 4) +
+              (kCflSumBatches + 4) * kCflSumBatch <= 4 * 4096, "read-ahead in bounds");
 
 // One work group per 64x64 colour tile (src/codec/chroma_from_luma.cpp:
 // 434-529): DCT8 of each block and channel, LLF cleared, scaled by the DCT8
@@ -214,31 +219,41 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
       if (c < chunks) {
         if (tid < 24 && !(sp == 0 ? done0 : done1)) {
           // n is a multiple of 64, so the lane's count is a whole number of
-          // batches; every batch is read (in bounds) two batches ahead of
-          // its ordered adds, which only the batches in the chunk make
+          // batches.  Batches go in groups of three through a ring of three
+          // register sets, each read two batches ahead of its ordered adds
+          // (reads past the lane's count stay inside the term buffers); a
+          // batch past the chunk's end leaves acc unchanged by a select, as
+          // a branch would let the reads sink behind it.
           const int batches = min(kCflChunk, n - c * kCflChunk) / 4 / kCflSumBatch;
           const float4* src =
             reinterpret_cast<const float4*>(&terms[c & 1][sp][skind][slane][0]);
-          float4 r[3][kCflSumBatch / 4];
-          auto load = [&](int b, float4 (&dst)[kCflSumBatch / 4]) {
+          constexpr int kQuads = kCflSumBatch / 4;
+          float4 r0[kQuads], r1[kQuads], r2[kQuads];
+          auto load = [&](int b, float4 (&dst)[kQuads]) {
 #pragma unroll
-            for (int m = 0; m < kCflSumBatch / 4; ++m) dst[m] = src[b * (kCflSumBatch / 4) + m];
+            for (int m = 0; m < kQuads; ++m) dst[m] = src[b * kQuads + m];
           };
-          load(0, r[0]);
-          load(1, r[1]);
+          auto add = [&](int b, const float4 (&v)[kQuads]) {
+            const bool live = b < batches;
 #pragma unroll
-          for (int b = 0; b < kCflSumBatches; ++b) {
-            if (b + 2 < kCflSumBatches) load(b + 2, r[(b + 2) % 3]);
-            if (b < batches) {
-#pragma unroll
-              for (int m = 0; m < kCflSumBatch / 4; ++m) {
-                const float4 v = r[b % 3][m];
-                acc += v.x;
-                acc += v.y;
-                acc += v.z;
-                acc += v.w;
-              }
+            for (int m = 0; m < kQuads; ++m) {
+              float t = acc + v[m].x;
+              t = t + v[m].y;
+              t = t + v[m].z;
+              t = t + v[m].w;
+              acc = live ? t : acc;
             }
+          };
+          load(0, r0);
+          load(1, r1);
+#pragma unroll 1
+          for (int b = 0; b < batches; b += 3) {
+            load(b + 2, r2);
+            add(b, r0);
+            load(b + 3, r0);
+            add(b + 1, r1);
+            load(b + 4, r1);
+            add(b + 2, r2);
           }
         }
         if (c + 1 < kCflChunks && producer && c + 1 < chunks) produce(c + 1);
