@@ -25,6 +25,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+CALIBRATION_BYTES = 256 * 1024 * 1024
+# HBM bytes per Butteraugli compare, from the FETCH_SIZE / WRITE_SIZE counter
+# runs summarised in this file (see --pmc-summary)
+TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_butteraugli_compare_4032x3024.md"
 
 
 def stage_bytes(name: str, w: int, h: int, tokens: int = 0) -> int:
@@ -99,6 +103,77 @@ def candidate_flops(w: int, h: int) -> int:
     return total
 
 
+def calibration_copy() -> None:
+    """Three device copies of CALIBRATION_BYTES, so that counter runs can
+    scale FETCH_SIZE / WRITE_SIZE by what they read for a copy of known
+    size."""
+    import torch
+    a = torch.ones(CALIBRATION_BYTES // 4, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {CALIBRATION_BYTES} bytes read and written, 3 times",
+          flush=True)
+
+
+def _pmc_dispatches(path: str, counter: str) -> list[tuple[int, str, float]]:
+    import csv
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return rows
+
+
+def pmc_summary(fetch_csv: str, write_csv: str) -> dict:
+    """HBM bytes per Butteraugli compare from two rocprofv3 --pmc runs of
+    `bench.py --calibration-copy --steps 1 --warmup 1 --no-cpu-baseline`
+    (FETCH_SIZE, then WRITE_SIZE; both in KiB per dispatch).  Each counter is
+    scaled by what it reads for the calibration copy (CALIBRATION_BYTES read
+    and written).  A compare is the dispatches after a LoopFilterKernel up to
+    and including the next ComposeKernel."""
+    out = {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = _pmc_dispatches(path, counter)
+        copies = [v for _, name, v in rows if "copy" in name.lower() and
+                  "elementwise" in name.lower()]
+        if not copies:
+            copies = [v for _, name, v in rows if "copy" in name.lower() and v > 1e5]
+        measured = float(np.median(copies)) * 1024.0
+        factor = CALIBRATION_BYTES / measured
+        compares, current = [], None
+        for _, name, v in rows:
+            if "LoopFilterKernel" in name:
+                current = 0.0
+            elif current is not None:
+                current += v
+                if "ComposeKernel" in name:
+                    compares.append(current)
+                    current = None
+        per_compare = float(np.mean(compares)) * 1024.0 * factor
+        out[counter] = {"calibration_factor": round(factor, 4),
+                        "compares": len(compares),
+                        "bytes_per_compare": round(per_compare)}
+    out["traffic_bytes_per_launch"] = (out["FETCH_SIZE"]["bytes_per_compare"] +
+                                       out["WRITE_SIZE"]["bytes_per_compare"])
+    return out
+
+
+def committed_traffic() -> int | None:
+    """The per-compare HBM traffic recorded in TRAFFIC_PROFILE, if any."""
+    import re
+    try:
+        text = TRAFFIC_PROFILE.read_text()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def run_step(dev, path):
     """Enqueues one pass; records event slot i+1 after stage i."""
     names = []
@@ -150,7 +225,16 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--workload", default="4032x3024")
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--calibration-copy", action="store_true",
+                        help="run a device copy of known size first (counter runs)")
+    parser.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                        help="print the HBM traffic per Butteraugli compare from two "
+                             "counter runs and exit (no GPU needed)")
     args = parser.parse_args()
+
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary)))
+        return 0
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU "
@@ -161,6 +245,8 @@ def main() -> int:
     import gjxl_amd
     from gjxl_amd import synthetic
 
+    if args.calibration_copy:
+        calibration_copy()
     spec = synthetic.WORKLOADS[args.workload]
     w, h = spec["width"], spec["height"]
     srgb = synthetic.generate(**spec)
@@ -198,11 +284,14 @@ def main() -> int:
     else:
         nbytes = stage_bytes(dominant, w, h, path.token_count)
         achieved = nbytes / (per_call[dominant] * 1e-3) / 1e9
+        traffic = None
+        if dominant == "butteraugli_compare" and args.workload == "4032x3024":
+            traffic = committed_traffic()
         roofline = {"kernel": dominant, "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                     "ms_per_launch": round(per_call[dominant], 4),
-                    "algorithmic_bytes": nbytes, "traffic": None}
+                    "algorithmic_bytes": nbytes, "traffic": traffic}
     result = {
         "metric": "MP/s at d1.0 e7, plain GPU path (XYB, AC strategy search, "
                   "Butteraugli AQ loop, AC tokenization), 1 GPU",

@@ -25,8 +25,6 @@ struct ButteraugliScale {
   float* mask_y = nullptr;       // MaskY(mask) per pixel
   float* mask_dc = nullptr;      // MaskDcY(mask) per pixel
   float* stages = nullptr;       // 9 planes
-  float* ac = nullptr;           // 3 planes: AC differences (Y ends masked)
-  float* dc = nullptr;           // 3 planes: DC differences
   float* tmp[3] = {nullptr, nullptr, nullptr};
 };
 

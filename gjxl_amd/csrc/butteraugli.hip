@@ -10,9 +10,9 @@
 //                         SeparateFrequencies (:702-846)
 //   MaltaDiff/MaltaKernel MaltaDiffMap (butteraugli_distance_internal.cpp:
 //                         482-577), zero outside the image
-//   AcDcKernel            Malta accumulation (UHF, HF, MF) and the L2 terms
 //   MaskSource/MaskKernel masking activity, blur, fuzzy erosion, MaskY/MaskDcY
-//   FinalKernel           masked AC Y and the final composition (:657-870)
+//   AcDcFinalKernel       Malta accumulation (UHF, HF, MF), the L2 terms,
+//                         masked AC Y and the final composition (:657-870)
 //   Subsample/Expand/ComposeKernel
 //                         the two-scale map and its maximum (:872-1003)
 //   ReduceKernel          ReduceButteraugliDistanceMap (butteraugli.cpp:
@@ -612,14 +612,19 @@ struct L2Weights {
 };
 
 // Malta accumulation in the reference's order (UHF, HF, MF), then the
-// asymmetric HF and symmetric MF L2 terms; DC differences from LF.
-__global__ __launch_bounds__(kThreads) void AcDcKernel(
+// asymmetric HF and symmetric MF L2 terms and the DC differences from LF;
+// then, per pixel, the Y blurred-activity term, masking and the final map
+// (MaskPsychoImage / CombineChannelsToDiffmap).  The AC and DC values stay
+// in registers; the masked Y AC and the map are stored (stages 7 and 8).
+__global__ __launch_bounds__(kThreads) void AcDcFinalKernel(
   const float* __restrict__ ps0, const float* __restrict__ ps1,
   const float* __restrict__ stages, size_t n, L2Weights lw,
-  float* __restrict__ ac, float* __restrict__ dc) {
+  const float* __restrict__ ref_blurred, const float* __restrict__ dist_blurred,
+  const float* __restrict__ mask_y, const float* __restrict__ mask_dc,
+  float x_multiplier, float* __restrict__ masked_ac_y, float* __restrict__ final_map) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float a[3];
+  float a[3], dcv[3];
   a[1] = 0.0f;
   a[1] += stages[4 * n + i];
   a[1] += stages[2 * n + i];
@@ -653,9 +658,17 @@ __global__ __launch_bounds__(kThreads) void AcDcKernel(
       a[c] = (d * d) * lw.mf[c] + a[c];
     }
     const float e = ps0[c * n + i] - ps1[c * n + i];
-    dc[c * n + i] = lw.lf[c] == 0.0f ? 0.0f : e * e * lw.lf[c];
-    ac[c * n + i] = a[c];
+    dcv[c] = lw.lf[c] == 0.0f ? 0.0f : e * e * lw.lf[c];
   }
+  const float d = ref_blurred[i] - dist_blurred[i];
+  const float ay = a[1] + 10.0f * d * d;
+  masked_ac_y[i] = ay;
+  const float my = mask_y[i], mdc = mask_dc[i];
+  const float dc0 = dcv[0] * x_multiplier;
+  const float ac0 = a[0] * x_multiplier;
+  const float masked_dc = dc0 * mdc + dcv[1] * mdc + dcv[2] * mdc;
+  const float masked_ac = ac0 * my + ay * my + a[2] * my;
+  final_map[i] = sqrtf(masked_dc + masked_ac);
 }
 
 // CombineChannelsForMasking + DiffPrecompute.
@@ -720,25 +733,6 @@ __global__ __launch_bounds__(kThreads) void MaskKernel(
   const double vd = kGlobal * (1.0 + 0.505054525019 / (3.87449418804 * m + 0.20025578522));
   mask_y[i] = static_cast<float>(vy * vy);
   mask_dc[i] = static_cast<float>(vd * vd);
-}
-
-__global__ __launch_bounds__(kThreads) void FinalKernel(
-  const float* __restrict__ ref_blurred, const float* __restrict__ dist_blurred,
-  const float* __restrict__ mask_y, const float* __restrict__ mask_dc,
-  float* __restrict__ ac, const float* __restrict__ dc, size_t n,
-  float x_multiplier, float* __restrict__ masked_ac_y, float* __restrict__ final_map) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float d = ref_blurred[i] - dist_blurred[i];
-  const float ay = ac[n + i] + 10.0f * d * d;
-  ac[n + i] = ay;
-  masked_ac_y[i] = ay;
-  const float my = mask_y[i], mdc = mask_dc[i];
-  const float dc0 = dc[i] * x_multiplier;
-  const float ac0 = ac[i] * x_multiplier;
-  const float masked_dc = dc0 * mdc + dc[n + i] * mdc + dc[2 * n + i] * mdc;
-  const float masked_ac = ac0 * my + ay * my + ac[2 * n + i] * my;
-  final_map[i] = sqrtf(masked_dc + masked_ac);
 }
 
 // ---- scales ------------------------------------------------------------------
@@ -989,17 +983,14 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
     lw.mf[c] = kL2[3 + c];
     lw.lf[c] = kL2[6 + c];
   }
-  hipLaunchKernelGGL(AcDcKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_psycho,
-                     s.dist_psycho, s.stages, n, lw, s.ac, s.dc);
-  GJXL_AMD_TRY(CheckLaunch("AcDcKernel"));
   GJXL_AMD_TRY(MaskActivity(b, s, s.dist_psycho, s.tmp[2]));
-  hipLaunchKernelGGL(FinalKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_blurred,
-                     s.tmp[2], s.mask_y, s.mask_dc, s.ac, s.dc, n,
-                     b->params.x_multiplier, s.stages + 7 * n, s.stages + 8 * n);
-  return CheckLaunch("FinalKernel");
+  hipLaunchKernelGGL(AcDcFinalKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_psycho,
+                     s.dist_psycho, s.stages, n, lw, s.ref_blurred, s.tmp[2], s.mask_y,
+                     s.mask_dc, b->params.x_multiplier, s.stages + 7 * n, s.stages + 8 * n);
+  return CheckLaunch("AcDcFinalKernel");
 }
 
-constexpr int kScalePlanes = 3 + 3 + 10 + 10 + 3 + 9 + 3 + 3 + 3;
+constexpr int kScalePlanes = 3 + 3 + 10 + 10 + 3 + 9 + 3;
 
 }  // namespace
 
@@ -1112,8 +1103,6 @@ GJXLResult gjxl_amd_butteraugli_create(GJXLAmdDevice* device, uint32_t width,
     s.mask_y = p; p += n;
     s.mask_dc = p; p += n;
     s.stages = p; p += 9 * n;
-    s.ac = p; p += 3 * n;
-    s.dc = p; p += 3 * n;
     for (auto& t : s.tmp) { t = p; p += n; }
   }
   float host[gjxl_amd::kSigmaCount * 64] = {};
