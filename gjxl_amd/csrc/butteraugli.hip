@@ -6,8 +6,9 @@
 //   BlurRows/ColsKernel   separable Gaussian of radius 3, 6, 7 or 16 with the
 //                         reference's interior summation order and its
 //                         renormalised border columns (:245-620)
-//   MediumLow/HighSplit/UltraSplitKernel
-//                         SeparateFrequencies (:702-846)
+//   LowSplit/HighSplit/UltraSplit
+//                         SeparateFrequencies (:702-846), as epilogues of the
+//                         blurs' column passes
 //   MaltaDiff/MaltaKernel MaltaDiffMap (butteraugli_distance_internal.cpp:
 //                         482-577), zero outside the image
 //   MaskSource/MaskKernel masking activity, blur, fuzzy erosion, MaskY/MaskDcY
@@ -201,13 +202,15 @@ __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
 }
 
 // Columns: each thread produces kColOutputs outputs down one column from
-// one register window (coalesced across the wave).
+// one register window (coalesced across the wave).  `epilogue(i, v)` takes
+// the blurred value v of pixel i: a store, or the frequency split that
+// consumes the blur, so that the blurred plane is never written.
 constexpr int kColOutputs = 8;
 
-template <int R>
+template <int R, typename Epilogue>
 __global__ __launch_bounds__(kThreads) void BlurColsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
-  float* __restrict__ out) {
+  Epilogue epilogue) {
   constexpr int kOutputs = kColOutputs;
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kOutputs;
   if (x >= w) return;
@@ -222,14 +225,14 @@ __global__ __launch_bounds__(kThreads) void BlurColsKernel(
     const float* sk = taps + 33;
 #pragma unroll
     for (int p = 0; p < kOutputs; ++p) {
-      out[static_cast<size_t>(y0 + p) * w + x] = ConvolveInterior<R>(window + p, sk);
+      epilogue(static_cast<size_t>(y0 + p) * w + x, ConvolveInterior<R>(window + p, sk));
     }
     return;
   }
   for (int p = 0; p < kOutputs; ++p) {
     const int y = y0 + p;
     if (y >= h) break;
-    out[static_cast<size_t>(y) * w + x] = ConvolveLine<R>(in + x, w, y, h, taps);
+    epilogue(static_cast<size_t>(y) * w + x, ConvolveLine<R>(in + x, w, y, h, taps));
   }
 }
 
@@ -355,65 +358,83 @@ __device__ __forceinline__ float AmplifyRange(float v, float width) {
   return v + v;
 }
 
-// psycho LF planes hold the raw LF blurs on entry: MF = XYB - LF, then the
-// LF planes are scaled.
-__global__ __launch_bounds__(kThreads) void MediumLowKernel(
-  const float* __restrict__ xyb, size_t n, float* __restrict__ psycho) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float* lf = psycho;
-  float* mf = psycho + 3 * n;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) mf[c * n + i] = xyb[c * n + i] - lf[c * n + i];
-  const float ly = lf[n + i];
-  lf[2 * n + i] = (-0.362267051518f * ly + lf[2 * n + i]) * 49.87984651440f;
-  lf[i] *= 33.832837186260f;
-  lf[n + i] = ly * 14.458268100570f;
-}
+// Column-pass epilogues of the separable blurs (SeparateFrequencies).
 
-// After the 3.22 blur of MF channel c (`blurred`): HF and MF of channel c;
-// channel 1 also applies SuppressXByY to HF X.
-__global__ __launch_bounds__(kThreads) void HighSplitKernel(
-  int c, const float* __restrict__ blurred, size_t n, float* __restrict__ psycho) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float* mf = psycho + 3 * n;
-  float* hf = psycho + 6 * n;
-  const float b = blurred[i];
-  if (c == 2) {
-    mf[2 * n + i] = b;
-    return;
-  }
-  const float high = mf[c * n + i] - b;
-  hf[c * n + i] = high;
-  mf[c * n + i] = c == 0 ? RemoveRange(b, 0.29f) : AmplifyRange(b, 0.1f);
-  if (c == 1) {
-    constexpr float kS = 46.0f, kMin = 0.653020556257f;
-    const float scaler = (kS / (high * high + kS)) * (1.0f - kMin) + kMin;
-    hf[i] = scaler * hf[i];
-  }
-}
+struct StoreBlur {
+  float* out;
+  __device__ __forceinline__ void operator()(size_t i, float v) const { out[i] = v; }
+};
 
-// After the 1.56 blur of HF channel c: UHF and HF of channel c.
-__global__ __launch_bounds__(kThreads) void UltraSplitKernel(
-  int c, const float* __restrict__ blurred, size_t n, float* __restrict__ psycho) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float* hf = psycho + (6 + c) * n;
-  float* uhf = psycho + (8 + c) * n;
-  const float original = hf[i];
-  float hv = blurred[i];
-  if (c == 0) {
-    uhf[i] = RemoveRange(original - hv, 0.04f);
-    hf[i] = RemoveRange(hv, 1.5f);
-  } else {
-    hv = MaximumClamp(hv, 28.4691806922f);
-    float uv = original - hv;
-    uv = MaximumClamp(uv, 5.19175294647f);
-    uhf[i] = uv * 2.69313763794f;
-    hf[i] = AmplifyRange(hv * 2.155f, 0.132f);
+// The LF blur v of channel c: MF = XYB - LF, then the LF plane scaled.  The
+// channels go Y, X, B: Y's LF stays raw until B's pass, which needs it and
+// then scales it.
+struct LowSplit {
+  int c;
+  const float* xyb;
+  float* psycho;
+  size_t n;
+  __device__ __forceinline__ void operator()(size_t i, float v) const {
+    float* lf = psycho;
+    float* mf = psycho + 3 * n;
+    mf[c * n + i] = xyb[c * n + i] - v;
+    if (c == 0) {
+      lf[i] = v * 33.832837186260f;
+    } else if (c == 1) {
+      lf[n + i] = v;
+    } else {
+      const float ly = lf[n + i];
+      lf[2 * n + i] = (-0.362267051518f * ly + v) * 49.87984651440f;
+      lf[n + i] = ly * 14.458268100570f;
+    }
   }
-}
+};
+
+// The 3.22 blur v of MF channel c: HF and MF of channel c; channel 1 also
+// applies SuppressXByY to HF X.
+struct HighSplit {
+  int c;
+  float* psycho;
+  size_t n;
+  __device__ __forceinline__ void operator()(size_t i, float v) const {
+    float* mf = psycho + 3 * n;
+    float* hf = psycho + 6 * n;
+    if (c == 2) {
+      mf[2 * n + i] = v;
+      return;
+    }
+    const float high = mf[c * n + i] - v;
+    hf[c * n + i] = high;
+    mf[c * n + i] = c == 0 ? RemoveRange(v, 0.29f) : AmplifyRange(v, 0.1f);
+    if (c == 1) {
+      constexpr float kS = 46.0f, kMin = 0.653020556257f;
+      const float scaler = (kS / (high * high + kS)) * (1.0f - kMin) + kMin;
+      hf[i] = scaler * hf[i];
+    }
+  }
+};
+
+// The 1.56 blur v of HF channel c: UHF and HF of channel c.
+struct UltraSplit {
+  int c;
+  float* psycho;
+  size_t n;
+  __device__ __forceinline__ void operator()(size_t i, float v) const {
+    float* hf = psycho + (6 + c) * n;
+    float* uhf = psycho + (8 + c) * n;
+    const float original = hf[i];
+    float hv = v;
+    if (c == 0) {
+      uhf[i] = RemoveRange(original - hv, 0.04f);
+      hf[i] = RemoveRange(hv, 1.5f);
+    } else {
+      hv = MaximumClamp(hv, 28.4691806922f);
+      float uv = original - hv;
+      uv = MaximumClamp(uv, 5.19175294647f);
+      uhf[i] = uv * 2.69313763794f;
+      hf[i] = AmplifyRange(hv * 2.155f, 0.132f);
+    }
+  }
+};
 
 // ---- difference stages -------------------------------------------------------
 
@@ -871,17 +892,17 @@ const float* Taps(const GJXLAmdButteraugli* b, ButteraugliSigma s) {
   return b->taps + s * kTapStride;
 }
 
-template <int R>
+template <int R, typename Epilogue>
 GJXLResult Blur(hipStream_t st, const float* in, int w, int h, const float* taps,
-                float* tmp, float* out) {
+                float* tmp, Epilogue epilogue) {
   hipLaunchKernelGGL(BlurRowsKernel<R>,
                      dim3(CeilDiv(static_cast<uint32_t>(w), kRowSpan),
                           CeilDiv(static_cast<uint32_t>(h), kRowRows)),
                      dim3(kThreads), 0, st, in, w, h, taps, tmp);
-  hipLaunchKernelGGL(BlurColsKernel<R>,
+  hipLaunchKernelGGL((BlurColsKernel<R, Epilogue>),
                      dim3(CeilDiv(static_cast<uint32_t>(w), kThreads),
                           CeilDiv(static_cast<uint32_t>(h), kColOutputs)),
-                     dim3(kThreads), 0, st, tmp, w, h, taps, out);
+                     dim3(kThreads), 0, st, tmp, w, h, taps, epilogue);
   return CheckLaunch("BlurKernel");
 }
 
@@ -897,25 +918,19 @@ GJXLResult Psycho(GJXLAmdButteraugli* b, ButteraugliScale& s, const float* in,
                      dim3(kThreads), 0, st, in, w, h,
                      b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
   GJXL_AMD_TRY(CheckLaunch("OpsinDynamicsKernel"));
-  for (int c = 0; c < 3; ++c) {
+  // LF of Y, X, B with the MF = XYB - LF split (B's pass scales Y's LF)
+  const int lf_order[3] = {1, 0, 2};
+  for (int c : lf_order) {
     GJXL_AMD_TRY(Blur<16>(st, s.xyb + c * n, w, h, Taps(b, kSigmaLow), s.tmp[0],
-                          psycho + c * n));
+                          LowSplit{c, s.xyb, psycho, n}));
   }
-  hipLaunchKernelGGL(MediumLowKernel, Grid1(n), dim3(kThreads), 0, st, s.xyb, n, psycho);
-  GJXL_AMD_TRY(CheckLaunch("MediumLowKernel"));
   for (int c = 0; c < 3; ++c) {
     GJXL_AMD_TRY(Blur<7>(st, psycho + (3 + c) * n, w, h, Taps(b, kSigmaHigh), s.tmp[0],
-                         s.tmp[1]));
-    hipLaunchKernelGGL(HighSplitKernel, Grid1(n), dim3(kThreads), 0, st, c, s.tmp[1], n,
-                       psycho);
-    GJXL_AMD_TRY(CheckLaunch("HighSplitKernel"));
+                         HighSplit{c, psycho, n}));
   }
   for (int c = 0; c < 2; ++c) {
     GJXL_AMD_TRY(Blur<3>(st, psycho + (6 + c) * n, w, h, Taps(b, kSigmaUltra), s.tmp[0],
-                         s.tmp[1]));
-    hipLaunchKernelGGL(UltraSplitKernel, Grid1(n), dim3(kThreads), 0, st, c, s.tmp[1], n,
-                       psycho);
-    GJXL_AMD_TRY(CheckLaunch("UltraSplitKernel"));
+                         UltraSplit{c, psycho, n}));
   }
   return GJXL_OK;
 }
@@ -927,7 +942,8 @@ GJXLResult MaskActivity(GJXLAmdButteraugli* b, ButteraugliScale& s, const float*
   hipLaunchKernelGGL(MaskSourceKernel, Grid1(s.n), dim3(kThreads), 0, st, psycho, s.n,
                      s.tmp[0]);
   GJXL_AMD_TRY(CheckLaunch("MaskSourceKernel"));
-  return Blur<6>(st, s.tmp[0], s.width, s.height, Taps(b, kSigmaMask), s.tmp[1], out);
+  return Blur<6>(st, s.tmp[0], s.width, s.height, Taps(b, kSigmaMask), s.tmp[1],
+                 StoreBlur{out});
 }
 
 constexpr double kMaltaWeights[6] = {37.0819870399, 8246.75321353, 18.7237414387,
