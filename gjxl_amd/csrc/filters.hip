@@ -23,6 +23,12 @@ __device__ __forceinline__ int MirrorInt(int c, int size) {
   return static_cast<int>(phase < size ? phase : period - 1 - phase);
 }
 
+// MirrorInt for -size <= c < 2 * size (one reflection), by selects.
+__device__ __forceinline__ int MirrorOnce(int c, int size) {
+  c = c < 0 ? -c - 1 : c;
+  return c >= size ? 2 * size - 1 - c : c;
+}
+
 struct GaborishWeights {
   float center, axis, diagonal;
 };
@@ -107,6 +113,9 @@ __device__ __forceinline__ void EpfPixel(Sample S, float bis, int x, int y,
   for (int c = 0; c < 3; ++c) out[c] = sum[c] / weight_sum;
 }
 
+// kOnce: w and h are at least 4, so every coordinate the tile mirrors (at
+// most 4 outside the image) needs one reflection.
+template <bool kOnce>
 __global__ __launch_bounds__(256) void LoopFilterKernel(
   const float* __restrict__ in, int in_stride, size_t in_plane, int w, int h,
   GaborishWeights g, const float* __restrict__ inverse_sigma, InverseOpsin p,
@@ -119,10 +128,22 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
   const int x0 = blockIdx.x * kLfTileW, y0 = blockIdx.y * kLfTileH;
   const int bw = (w + 7) / 8;
   const size_t plane = static_cast<size_t>(w) * h;
-  for (int e = threadIdx.x; e < 3 * kRawH * kRawW; e += blockDim.x) {
+  auto mirror = [](int c, int size) { return kOnce ? MirrorOnce(c, size) : MirrorInt(c, size); };
+  // a fixed trip count: every load of the tile issues before the first store
+  constexpr int kRawCount = 3 * kRawH * kRawW;
+  constexpr int kRawLoads = (kRawCount + 255) / 256;
+  float loaded[kRawLoads];
+#pragma unroll
+  for (int k = 0; k < kRawLoads; ++k) {
+    const int e = min(static_cast<int>(threadIdx.x) + k * 256, kRawCount - 1);
     const int c = e / (kRawH * kRawW), r = (e / kRawW) % kRawH, col = e % kRawW;
-    const int sy = MirrorInt(y0 - 4 + r, h), sx = MirrorInt(x0 - 4 + col, w);
-    raw[c][r][col] = in[c * in_plane + static_cast<size_t>(sy) * in_stride + sx];
+    const int sy = mirror(y0 - 4 + r, h), sx = mirror(x0 - 4 + col, w);
+    loaded[k] = in[c * in_plane + static_cast<size_t>(sy) * in_stride + sx];
+  }
+#pragma unroll
+  for (int k = 0; k < kRawLoads; ++k) {
+    const int e = threadIdx.x + k * 256;
+    if (e < kRawCount) raw_e1[e] = loaded[k];
   }
   __syncthreads();
   // Gaborish (gaborish.cpp:101-190) at the in-image slots of its region
@@ -143,7 +164,7 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
     const int r = e / kGabW, col = e % kGabW;
     const int gx = x0 - 3 + col, gy = y0 - 3 + r;
     if (gx >= 0 && gy >= 0 && gx < w && gy < h) continue;
-    const int mr = MirrorInt(gy, h) - (y0 - 3), mc = MirrorInt(gx, w) - (x0 - 3);
+    const int mr = mirror(gy, h) - (y0 - 3), mc = mirror(gx, w) - (x0 - 3);
     if (mr < 0 || mc < 0 || mr >= kGabH || mc >= kGabW) continue;  // never read
 #pragma unroll
     for (int c = 0; c < 3; ++c) gab[c][r][col] = gab[c][mr][mc];
@@ -165,7 +186,7 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
     const int r = e / kE1W, col = e % kE1W;
     const int x = x0 - 1 + col, y = y0 - 1 + r;
     if (x >= 0 && y >= 0 && x < w && y < h) continue;
-    const int mr = MirrorInt(y, h) - (y0 - 1), mc = MirrorInt(x, w) - (x0 - 1);
+    const int mr = mirror(y, h) - (y0 - 1), mc = mirror(x, w) - (x0 - 1);
     if (mr < 0 || mc < 0 || mr >= kE1H || mc >= kE1W) continue;  // never read
 #pragma unroll
     for (int c = 0; c < 3; ++c) e1[c][r][col] = e1[c][mr][mc];
@@ -208,9 +229,15 @@ GJXLResult LaunchLoopFiltersAndColor(hipStream_t s, const float* reconstructed,
                  255.0f / intensity_target,
                  cbrtf(bias)};
   const dim3 grid(CeilDiv(w, kLfTileW), CeilDiv(h, kLfTileH));
-  hipLaunchKernelGGL(LoopFilterKernel, grid, dim3(256), 0, s, reconstructed, pw,
-                     static_cast<size_t>(pw) * ph, w, h, g, inverse_sigma, p, filtered,
-                     linear);
+  if (w >= 4 && h >= 4) {
+    hipLaunchKernelGGL(LoopFilterKernel<true>, grid, dim3(256), 0, s, reconstructed, pw,
+                       static_cast<size_t>(pw) * ph, w, h, g, inverse_sigma, p, filtered,
+                       linear);
+  } else {
+    hipLaunchKernelGGL(LoopFilterKernel<false>, grid, dim3(256), 0, s, reconstructed, pw,
+                       static_cast<size_t>(pw) * ph, w, h, g, inverse_sigma, p, filtered,
+                       linear);
+  }
   return CheckLaunch("LoopFilterKernel");
 }
 

@@ -83,9 +83,10 @@ def _check_evaluation(s, fe, setup, field, w, h):
         _same(key, got[key], want[key])
 
 
+# 3x5, 2x9 and 11x3 mirror the loop filters' tiles more than once per side
 @pytest.mark.parametrize("w,h,seed", [(512, 512, 1), (200, 136, 4), (72, 40, 7),
                                       (8, 8, 3), (136, 264, 9), (2104, 48, 5),
-                                      (40, 2072, 6)])
+                                      (40, 2072, 6), (3, 5, 2), (2, 9, 8), (11, 3, 10)])
 def test_aq_evaluation_parity(device, w, h, seed):
     print(f"aq parity {w}x{h} seed {seed}", flush=True)
     srgb = np.ascontiguousarray(
