@@ -42,6 +42,11 @@ __device__ __forceinline__ int BMirror(int c, int size) {
   while (c < 0 || c >= size) c = c < 0 ? -c - 1 : 2 * size - 1 - c;
   return c;
 }
+// BMirror for -size <= c < 2 * size (one reflection), by selects.
+__device__ __forceinline__ int BMirrorOnce(int c, int size) {
+  c = c < 0 ? -c - 1 : c;
+  return c >= size ? 2 * size - 1 - c : c;
+}
 __device__ __forceinline__ float BMax(float a, float b) { return a < b ? b : a; }
 
 // ---- Gaussian blurs --------------------------------------------------------
@@ -275,17 +280,32 @@ __device__ __forceinline__ void Absorbance(float r, float g, float b, bool clamp
 constexpr int kOpsinTileW = 64, kOpsinTileH = 16;
 constexpr int kOpsinRows = kOpsinTileH / (kThreads / kOpsinTileW);  // per thread
 
+// kOnce: w and h are at least 2, so every coordinate of the 2-pixel halo
+// needs one reflection.
+template <bool kOnce>
 __global__ __launch_bounds__(kThreads) void OpsinDynamicsKernel(
   const float* __restrict__ rgb, int w, int h, float intensity,
   const float* __restrict__ taps, float* __restrict__ xyb) {
   constexpr int TW = kOpsinTileW + 4, TH = kOpsinTileH + 4;
+  constexpr int kCount = 3 * TH * TW;
+  constexpr int kLoads = (kCount + kThreads - 1) / kThreads;
   __shared__ float tile[3][TH][TW];
   const int x0 = blockIdx.x * kOpsinTileW, y0 = blockIdx.y * kOpsinTileH;
   const size_t n = static_cast<size_t>(w) * h;
-  for (int e = threadIdx.x; e < 3 * TH * TW; e += kThreads) {
+  auto mirror = [](int c, int size) { return kOnce ? BMirrorOnce(c, size) : BMirror(c, size); };
+  // a fixed trip count: every load of the tile issues before the first store
+  float loaded[kLoads];
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) {
+    const int e = min(static_cast<int>(threadIdx.x) + k * kThreads, kCount - 1);
     const int c = e / (TH * TW), r = (e / TW) % TH, col = e % TW;
-    const int sy = BMirror(y0 - 2 + r, h), sx = BMirror(x0 - 2 + col, w);
-    tile[c][r][col] = rgb[c * n + static_cast<size_t>(sy) * w + sx];
+    const int sy = mirror(y0 - 2 + r, h), sx = mirror(x0 - 2 + col, w);
+    loaded[k] = rgb[c * n + static_cast<size_t>(sy) * w + sx];
+  }
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) {
+    const int e = threadIdx.x + k * kThreads;
+    if (e < kCount) (&tile[0][0][0])[e] = loaded[k];
   }
   __syncthreads();
   const float w0 = taps[33 + 2], w1 = taps[33 + 1], w2 = taps[33 + 0];
@@ -731,21 +751,30 @@ __global__ __launch_bounds__(kThreads) void MaskKernel(
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= w) return;
   constexpr int s = 3;
+  // every neighbour is read (at clamped positions) before the selection,
+  // which then takes the ones inside the image in the reference's order
+  const int xl = max(x - s, 0), xr = min(x + s, w - 1);
+  const int yu = max(y - s, 0), yd = min(y + s, h - 1);
   auto at = [&](int xx, int yy) { return in[static_cast<size_t>(yy) * w + xx]; };
-  float m0 = at(x, y);
+  const float c0 = at(x, y);
+  const float l0 = at(xl, y), lu = at(xl, yu), ld = at(xl, yd);
+  const float r0 = at(xr, y), ru = at(xr, yu), rd = at(xr, yd);
+  const float u0 = at(x, yu), d0 = at(x, yd);
+  const bool left = x >= s, right = x + s < w, up = y >= s, down = y + s < h;
+  float m0 = c0;
   float m1 = 2.0f * m0, m2 = m1;
-  if (x >= s) {
-    StoreMin3(at(x - s, y), m0, m1, m2);
-    if (y >= s) StoreMin3(at(x - s, y - s), m0, m1, m2);
-    if (y + s < h) StoreMin3(at(x - s, y + s), m0, m1, m2);
+  if (left) {
+    StoreMin3(l0, m0, m1, m2);
+    if (up) StoreMin3(lu, m0, m1, m2);
+    if (down) StoreMin3(ld, m0, m1, m2);
   }
-  if (x + s < w) {
-    StoreMin3(at(x + s, y), m0, m1, m2);
-    if (y >= s) StoreMin3(at(x + s, y - s), m0, m1, m2);
-    if (y + s < h) StoreMin3(at(x + s, y + s), m0, m1, m2);
+  if (right) {
+    StoreMin3(r0, m0, m1, m2);
+    if (up) StoreMin3(ru, m0, m1, m2);
+    if (down) StoreMin3(rd, m0, m1, m2);
   }
-  if (y >= s) StoreMin3(at(x, y - s), m0, m1, m2);
-  if (y + s < h) StoreMin3(at(x, y + s), m0, m1, m2);
+  if (up) StoreMin3(u0, m0, m1, m2);
+  if (down) StoreMin3(d0, m0, m1, m2);
   const float m = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
   const size_t i = static_cast<size_t>(y) * w + x;
   mask[i] = m;
@@ -912,11 +941,15 @@ GJXLResult Psycho(GJXLAmdButteraugli* b, ButteraugliScale& s, const float* in,
   hipStream_t st = b->device->stream;
   const int w = s.width, h = s.height;
   const size_t n = s.n;
-  hipLaunchKernelGGL(OpsinDynamicsKernel,
-                     dim3(CeilDiv(static_cast<uint32_t>(w), kOpsinTileW),
-                          CeilDiv(static_cast<uint32_t>(h), kOpsinTileH)),
-                     dim3(kThreads), 0, st, in, w, h,
-                     b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
+  const dim3 opsin_grid(CeilDiv(static_cast<uint32_t>(w), kOpsinTileW),
+                        CeilDiv(static_cast<uint32_t>(h), kOpsinTileH));
+  if (w >= 2 && h >= 2) {
+    hipLaunchKernelGGL(OpsinDynamicsKernel<true>, opsin_grid, dim3(kThreads), 0, st, in, w, h,
+                       b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
+  } else {
+    hipLaunchKernelGGL(OpsinDynamicsKernel<false>, opsin_grid, dim3(kThreads), 0, st, in, w,
+                       h, b->params.intensity_target, Taps(b, kSigmaOpsin), s.xyb);
+  }
   GJXL_AMD_TRY(CheckLaunch("OpsinDynamicsKernel"));
   // LF of Y, X, B with the MF = XYB - LF split (B's pass scales Y's LF)
   const int lf_order[3] = {1, 0, 2};
