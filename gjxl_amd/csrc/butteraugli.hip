@@ -142,9 +142,10 @@ __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
   constexpr int kLoads = (kUsed + kRowLanes - 1) / kRowLanes;
   constexpr int kStride = kLoads * kRowLanes;
   __shared__ __align__(16) float tile[kRowRows][kStride];
-  const int x0 = blockIdx.x * kRowSpan;
+  const TileIndex tile_index = XcdTile();
+  const int x0 = tile_index.x * kRowSpan;
   const int row = threadIdx.x / kRowLanes, lane = threadIdx.x % kRowLanes;
-  const int y = blockIdx.y * kRowRows + row;
+  const int y = tile_index.y * kRowRows + row;
   // tile column c holds sample x0 - R + c of the row
   const float* src = in + static_cast<size_t>(y < h ? y : h - 1) * w;
 #pragma unroll
@@ -217,6 +218,8 @@ __global__ __launch_bounds__(kThreads) void BlurColsKernel(
   const float* __restrict__ in, int w, int h, const float* __restrict__ taps,
   Epilogue epilogue) {
   constexpr int kOutputs = kColOutputs;
+  // dispatch order (no XCD remap): horizontally adjacent groups read the
+  // same rows, and a band's halo rows are re-read from L2 either way
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kOutputs;
   if (x >= w) return;
   const int first = h < R ? h : R;
@@ -290,7 +293,8 @@ __global__ __launch_bounds__(kThreads) void OpsinDynamicsKernel(
   constexpr int kCount = 3 * TH * TW;
   constexpr int kLoads = (kCount + kThreads - 1) / kThreads;
   __shared__ float tile[3][TH][TW];
-  const int x0 = blockIdx.x * kOpsinTileW, y0 = blockIdx.y * kOpsinTileH;
+  const TileIndex tile_index = XcdTile();
+  const int x0 = tile_index.x * kOpsinTileW, y0 = tile_index.y * kOpsinTileH;
   const size_t n = static_cast<size_t>(w) * h;
   auto mirror = [](int c, int size) { return kOnce ? BMirrorOnce(c, size) : BMirror(c, size); };
   // a fixed trip count: every load of the tile issues before the first store
@@ -558,7 +562,8 @@ __global__ __launch_bounds__(kThreads) void MaltaKernel(
   const float* __restrict__ p0, const float* __restrict__ p1, int w, int h,
   MaltaScale m, float* __restrict__ out) {
   __shared__ __align__(16) float tile[kMaltaLh][kMaltaLw];
-  const int x0 = blockIdx.x * kMaltaTileW, y0 = blockIdx.y * kMaltaTileH;
+  const TileIndex tile_index = XcdTile();
+  const int x0 = tile_index.x * kMaltaTileW, y0 = tile_index.y * kMaltaTileH;
   // every load of the tile issues before the first scaled difference
   float v0[kMaltaLoads], v1[kMaltaLoads];
 #pragma unroll

@@ -173,6 +173,23 @@ __device__ inline uint32_t BlockExclusiveScanU32(uint32_t v, uint32_t* total) {
   return before + inclusive - v;
 }
 
+// XCD-aware tile order for a 2-D grid.  Work groups are dispatched to the 8
+// XCDs round-robin (group i to XCD i % 8), and each XCD has its own L2; this
+// maps the groups of one XCD to a contiguous run of tiles (row-major), so
+// that neighbouring tiles, which share halo cache lines, are processed by
+// the same L2.  The last (count % 8) groups keep their own index.
+struct TileIndex {
+  int x, y;
+};
+__device__ __forceinline__ TileIndex XcdTile() {
+  constexpr unsigned kXcds = 8;
+  const unsigned count = gridDim.x * gridDim.y;
+  const unsigned i = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned per = count / kXcds;
+  const unsigned t = i < per * kXcds ? (i % kXcds) * per + i / kXcds : i;
+  return {static_cast<int>(t % gridDim.x), static_cast<int>(t / gridDim.x)};
+}
+
 struct U32Max {
   __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a < b ? b : a; }
 };

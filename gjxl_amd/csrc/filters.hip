@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
   __shared__ float gab[3][kGabH][kGabW];
   auto raw = reinterpret_cast<float (*)[kRawH][kRawW]>(raw_e1);
   auto e1 = reinterpret_cast<float (*)[kE1H][kE1W]>(raw_e1);
-  const int x0 = blockIdx.x * kLfTileW, y0 = blockIdx.y * kLfTileH;
+  const TileIndex tile_index = XcdTile();
+  const int x0 = tile_index.x * kLfTileW, y0 = tile_index.y * kLfTileH;
   const int bw = (w + 7) / 8;
   const size_t plane = static_cast<size_t>(w) * h;
   auto mirror = [](int c, int size) { return kOnce ? MirrorOnce(c, size) : MirrorInt(c, size); };
