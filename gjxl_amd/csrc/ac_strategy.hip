@@ -318,8 +318,9 @@ struct CostShape {
   static constexpr int kPositions = kNx * kNy;
 };
 
-// One work group: kCands anchor positions of one 64x64 tile (blockIdx.y),
-// positions blockIdx.x * kCands + k of the tile's raster list.  Only
+// One work group: kCands anchor positions of one 64x64 tile (grid y, after
+// the XCD-aware order), positions x * kCands + k of the tile's raster list
+// (grid x).  Only
 // positions whose footprint lies inside the tile are launched; the rest of
 // the cost plane is NaN (filled beforehand).
 template <int kStrategy>
@@ -353,7 +354,10 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   constexpr int kWaves = kCostThreads / 64;
   const int bw = width / 8, bh = height / 8;
   const int tiles_w = (bw + 7) / 8;
-  const int tx = blockIdx.y % tiles_w, ty = blockIdx.y / tiles_w;
+  // the groups of one 64x64 tile (consecutive in dispatch order) share its
+  // pixels: the XCD-aware order keeps them on one L2
+  const TileIndex group = XcdTile();
+  const int tx = group.y % tiles_w, ty = group.y / tiles_w;
   const int tbw = min(8, bw - tx * 8), tbh = min(8, bh - ty * 8);
   const size_t plane = static_cast<size_t>(width) * height;
   // binary64 bases, read with uniform (scalar) loads
@@ -363,22 +367,24 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   const double* iv = bases + BasisOffset(H, false);   // [v][y]
   // candidate k of this work group: anchor block and whether it is launched
   auto candidate = [&](int k, int* bx, int* by) {
-    const int p = blockIdx.x * K + k;
+    const int p = group.x * K + k;
     const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
     *bx = tx * 8 + lx;
     *by = ty * 8 + ly;
     return p < S::kPositions && lx + S::kCw <= tbw && ly + S::kCh <= tbh;
   };
 
-  // Every global read of the work group is issued here, before any result
-  // is needed: pixels and mask for LDS, and each thread's quantisation table
-  // entries for the elements it quantises later.
+  // Every global read of the work group is issued here, in the order the
+  // results are needed (reads complete in issue order): the pixels, which
+  // the first barrier waits for, then the mask, the quantisation table
+  // entries of the elements this thread quantises and the quant-field and
+  // CfL inputs, which reach LDS behind the forward passes.
   constexpr int kPer = (KC * N + kCostThreads - 1) / kCostThreads;
   float pixel[kPer], dq_e[kPer], inv_e[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
-    pixel[j] = dq_e[j] = inv_e[j] = 0.0f;
+    pixel[j] = 0.0f;
     if (e < KC * N) {
       const int kc = e / N, r = e % N;
       const int k = kc / 3, c = kc % 3;
@@ -386,10 +392,6 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       if (candidate(k, &bx, &by)) {
         pixel[j] = opsin[c * plane + static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W];
       }
-      // element e is coefficient (v, u) = (r / W, r % W) in the quantise step
-      const int i = CoefficientIndex<W, H>(r / W, r % W);
-      dq_e[j] = tables[DequantOffset(kStrategy, c) + i];
-      inv_e[j] = tables[InverseOffset(kStrategy, c) + i];
     }
   }
   constexpr int kMaskPer = (K * N + kCostThreads - 1) / kCostThreads;
@@ -406,44 +408,42 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       }
     }
   }
-
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    dq_e[j] = inv_e[j] = 0.0f;
+    if (e < KC * N) {
+      const int r = e % N, c = (e / N) % 3;
+      // element e is coefficient (v, u) = (r / W, r % W) in the quantise step
+      const int i = CoefficientIndex<W, H>(r / W, r % W);
+      dq_e[j] = tables[DequantOffset(kStrategy, c) + i];
+      inv_e[j] = tables[InverseOffset(kStrategy, c) + i];
+    }
+  }
+  // thread k < K: candidate k's quant-field blocks and the tile's CfL
+  // multipliers (ComputeQuantNormUnchecked, src/codec/ac_strategy.cpp:103-137)
+  constexpr int kBlocks = S::kCw * S::kCh;
+  float qv[kBlocks];
+  int8_t yx = 0, yb = 0;
+  bool ok = false;
   if (tid < K) {
     int bx, by;
-    const int ok = candidate(tid, &bx, &by);
+    ok = candidate(tid, &bx, &by);
     valid[tid] = ok;
     bxs[tid] = bx;
     bys[tid] = by;
-    if (ok) {
-      // src/codec/ac_strategy.cpp:103-137 ComputeQuantNormUnchecked
-      const float* q = quant_field + static_cast<size_t>(by) * bw + bx;
-      float norm;
-      if constexpr (S::kCw * S::kCh == 1) {
-        norm = q[0];
-      } else if constexpr (S::kCw * S::kCh == 2) {
-        norm = S::kCh == 2 ? StdMax(q[0], q[bw]) : StdMax(q[0], q[1]);
-      } else {
-        float sum = 0.0f;
-        for (int dy = 0; dy < S::kCh; ++dy) {
-          for (int dx = 0; dx < S::kCw; ++dx) {
-            float v = q[static_cast<size_t>(dy) * bw + dx];
-            v *= v;
-            v *= v;
-            v *= v;
-            sum += v * v;
-          }
-        }
-        sum /= static_cast<float>(S::kCw * S::kCh);
-        norm = FastPow(sum, 1.0f / 16.0f);
-      }
-      qn[tid] = norm;
-      const int tile = ty * tiles_w + tx;
-      cfl[tid][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
-      cfl[tid][1] = 0.0f;
-      cfl[tid][2] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+    const float* q = quant_field + static_cast<size_t>(ok ? by : 0) * bw + (ok ? bx : 0);
+#pragma unroll
+    for (int dy = 0; dy < S::kCh; ++dy) {
+#pragma unroll
+      for (int dx = 0; dx < S::kCw; ++dx) qv[dy * S::kCw + dx] = q[static_cast<size_t>(dy) * bw + dx];
     }
+    const int tile = ty * tiles_w + tx;
+    yx = y_to_x[tile];
+    yb = y_to_b[tile];
     for (int c = 0; c < 3; ++c) nnz[tid][c] = 0;
   }
-  // pixels transposed ([x][y]) and the mask into LDS
+  // pixels transposed ([x][y]) into LDS
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
@@ -451,11 +451,6 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       const int kc = e / N, r = e % N;
       A[kc * N + (r % W) * H + r / W] = pixel[j];
     }
-  }
-#pragma unroll
-  for (int j = 0; j < kMaskPer; ++j) {
-    const int e = tid + j * kCostThreads;
-    if (e < K * N) mask[e] = mask_e[j];
   }
   __syncthreads();
 
@@ -501,6 +496,36 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
 #pragma unroll
       for (int j = 0; j < kOut; ++j) B[kc * N + (g * kOut + j) * W + u] = static_cast<float>(acc[j]);
     }
+  }
+  // the reads issued at the start: mask into LDS, quant norms and CfL
+#pragma unroll
+  for (int j = 0; j < kMaskPer; ++j) {
+    const int e = tid + j * kCostThreads;
+    if (e < K * N) mask[e] = mask_e[j];
+  }
+  if (tid < K && ok) {
+    float norm;
+    if constexpr (kBlocks == 1) {
+      norm = qv[0];
+    } else if constexpr (kBlocks == 2) {
+      norm = StdMax(qv[0], qv[1]);
+    } else {
+      float sum = 0.0f;
+#pragma unroll
+      for (int b = 0; b < kBlocks; ++b) {
+        float v = qv[b];
+        v *= v;
+        v *= v;
+        v *= v;
+        sum += v * v;
+      }
+      sum /= static_cast<float>(kBlocks);
+      norm = FastPow(sum, 1.0f / 16.0f);
+    }
+    qn[tid] = norm;
+    cfl[tid][0] = static_cast<float>(yx) / 84.0f;
+    cfl[tid][1] = 0.0f;
+    cfl[tid][2] = 1.0f + static_cast<float>(yb) / 84.0f;
   }
   __syncthreads();
   // Quantise (src/codec/ac_strategy.cpp:914-926): sqrt(|rounded|) into A in
