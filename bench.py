@@ -26,9 +26,10 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBPS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 CALIBRATION_BYTES = 256 * 1024 * 1024
-# HBM bytes per Butteraugli compare, from the FETCH_SIZE / WRITE_SIZE counter
-# runs summarised in this file (see --pmc-summary)
-TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_butteraugli_compare_4032x3024.md"
+# HBM bytes per AQ evaluation and per Butteraugli compare, from the
+# FETCH_SIZE / WRITE_SIZE counter runs summarised in this file (see
+# --pmc-summary)
+TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_4032x3024.md"
 
 
 def stage_bytes(name: str, w: int, h: int, tokens: int = 0) -> int:
@@ -129,48 +130,62 @@ def _pmc_dispatches(path: str, counter: str) -> list[tuple[int, str, float]]:
     return rows
 
 
+def _stage_spans(names: list[str]) -> dict[str, list[tuple[int, int]]]:
+    """Dispatch index ranges (inclusive) of the AQ evaluations and the
+    Butteraugli compares in one counter run: an evaluation runs from a
+    FillWordsKernel (its first dispatch) to the next LoopFilterKernel; a
+    compare from the dispatch after that to the next ComposeKernel."""
+    spans = {"aq_evaluate": [], "butteraugli_compare": []}
+    last_fill = None
+    i = 0
+    while i < len(names):
+        if "FillWordsKernel" in names[i]:
+            last_fill = i
+        if "LoopFilterKernel" in names[i]:
+            if last_fill is not None:
+                spans["aq_evaluate"].append((last_fill, i))
+            j = i + 1
+            while j < len(names) and "ComposeKernel" not in names[j]:
+                j += 1
+            if j < len(names):
+                spans["butteraugli_compare"].append((i + 1, j))
+            last_fill = None
+        i += 1
+    return spans
+
+
 def pmc_summary(fetch_csv: str, write_csv: str) -> dict:
-    """HBM bytes per Butteraugli compare from two rocprofv3 --pmc runs of
-    `bench.py --calibration-copy --steps 1 --warmup 1 --no-cpu-baseline`
-    (FETCH_SIZE, then WRITE_SIZE; both in KiB per dispatch).  Each counter is
-    scaled by what it reads for the calibration copy (CALIBRATION_BYTES read
-    and written).  A compare is the dispatches after a LoopFilterKernel up to
-    and including the next ComposeKernel."""
+    """HBM bytes per AQ evaluation and per Butteraugli compare from two
+    rocprofv3 --pmc runs of `bench.py --calibration-copy --steps 1 --warmup 1
+    --no-cpu-baseline` (FETCH_SIZE, then WRITE_SIZE; both in KiB per
+    dispatch).  Each counter is scaled by what it reads for the calibration
+    copy (CALIBRATION_BYTES read and written)."""
     out = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_dispatches(path, counter)
-        copies = [v for _, name, v in rows if "copy" in name.lower() and
-                  "elementwise" in name.lower()]
-        if not copies:
-            copies = [v for _, name, v in rows if "copy" in name.lower() and v > 1e5]
-        measured = float(np.median(copies)) * 1024.0
-        factor = CALIBRATION_BYTES / measured
-        compares, current = [], None
-        for _, name, v in rows:
-            if "LoopFilterKernel" in name:
-                current = 0.0
-            elif current is not None:
-                current += v
-                if "ComposeKernel" in name:
-                    compares.append(current)
-                    current = None
-        per_compare = float(np.mean(compares)) * 1024.0 * factor
-        out[counter] = {"calibration_factor": round(factor, 4),
-                        "compares": len(compares),
-                        "bytes_per_compare": round(per_compare)}
-    out["traffic_bytes_per_launch"] = (out["FETCH_SIZE"]["bytes_per_compare"] +
-                                       out["WRITE_SIZE"]["bytes_per_compare"])
+        copies = [v for _, name, v in rows if "copy" in name.lower() and v > 1e5]
+        factor = CALIBRATION_BYTES / (float(np.median(copies)) * 1024.0)
+        names = [name for _, name, _ in rows]
+        values = [v for _, _, v in rows]
+        for stage, spans in _stage_spans(names).items():
+            per = [sum(values[a:b + 1]) for a, b in spans]
+            entry = out.setdefault(stage, {"launches": len(per)})
+            entry[counter] = round(float(np.mean(per)) * 1024.0 * factor)
+        out.setdefault("calibration_factor", {})[counter] = round(factor, 4)
+    for stage in ("aq_evaluate", "butteraugli_compare"):
+        out[stage]["traffic_bytes_per_launch"] = (out[stage]["FETCH_SIZE"] +
+                                                  out[stage]["WRITE_SIZE"])
     return out
 
 
-def committed_traffic() -> int | None:
-    """The per-compare HBM traffic recorded in TRAFFIC_PROFILE, if any."""
+def committed_traffic(stage: str) -> int | None:
+    """The per-launch HBM traffic of `stage` recorded in TRAFFIC_PROFILE."""
     import re
     try:
         text = TRAFFIC_PROFILE.read_text()
     except OSError:
         return None
-    m = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
+    m = re.search(r"^traffic_bytes_per_launch %s: (\d+)$" % stage, text, re.M)
     return int(m.group(1)) if m else None
 
 
@@ -228,8 +243,9 @@ def main() -> int:
     parser.add_argument("--calibration-copy", action="store_true",
                         help="run a device copy of known size first (counter runs)")
     parser.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                        help="print the HBM traffic per Butteraugli compare from two "
-                             "counter runs and exit (no GPU needed)")
+                        help="print the HBM traffic per AQ evaluation and per "
+                             "Butteraugli compare from two counter runs and exit "
+                             "(no GPU needed)")
     args = parser.parse_args()
 
     if args.pmc_summary:
@@ -285,8 +301,8 @@ def main() -> int:
         nbytes = stage_bytes(dominant, w, h, path.token_count)
         achieved = nbytes / (per_call[dominant] * 1e-3) / 1e9
         traffic = None
-        if dominant == "butteraugli_compare" and args.workload == "4032x3024":
-            traffic = committed_traffic()
+        if args.workload == "4032x3024":
+            traffic = committed_traffic(dominant)
         roofline = {"kernel": dominant, "bound": "hbm",
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
