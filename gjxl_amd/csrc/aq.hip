@@ -636,35 +636,56 @@ __device__ __forceinline__ float AdjustQuantized(float value, float threshold) {
   return fabsf(value) < threshold ? 0.0f : rintf(value);
 }
 
-// The scan of one channel over its terms (error, |quantized|) and the
-// decision.  coef and inv give the border terms again.
+// The scan's running sums are independent chains: the error and |quantized|
+// sums over all terms and, per quadrant, the |quantized| sum and (Y) the
+// largest error of a zero coefficient, each over its terms in the scan's
+// raster order.  Chain `role` of one channel: 0 error sum, 1 |quantized|
+// sum, 2 + q quadrant q's sum and maximum.
+constexpr int kScanChains = 6;
+
 template <int cols, int rows>
-__device__ void AdjustQuantFromTerms(const float2* __restrict__ terms,
-                                     const float* __restrict__ coef,
-                                     const float* __restrict__ inv, float qac,
-                                     float mult, int channel, int strategy, float thr[4],
-                                     int32_t* raw_quant) {
-  constexpr int xsize = cols / 8, ysize = rows / 8;
-  float err_sum = 0.0f, val_sum = 0.0f;
-  float nz[4] = {0.0f, 0.0f, 0.0f, 0.0f}, mx[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+__device__ void ScanChain(const float2* __restrict__ terms, int role, int channel,
+                          float* sum_out, float* max_out) {
+  if (role < 2) {
+    // .x (error) or .y (|quantized|) of every term, in raster order; the
+    // loads of a batch issue before its ordered adds
+    const float* src = reinterpret_cast<const float*>(terms) + role;
+    float sum = 0.0f;
+    for (int base = 0; base < cols * rows; base += 16) {
+      float v[16];
 #pragma unroll
-  for (int yh = 0; yh < 2; ++yh) {
+      for (int m = 0; m < 16; ++m) v[m] = src[2 * (base + m)];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) sum += v[m];
+    }
+    *sum_out = sum;
+    return;
+  }
+  const int qd = role - 2, yh = qd / 2, xh = qd % 2;
+  float sum = 0.0f, mx = 0.0f;
 #pragma unroll 1
-    for (int y = yh * rows / 2; y < (yh + 1) * rows / 2; ++y) {
+  for (int y = yh * rows / 2; y < (yh + 1) * rows / 2; ++y) {
+    const float2* row = terms + y * cols + xh * (cols / 2);
 #pragma unroll
-      for (int xh = 0; xh < 2; ++xh) {
-        const int qd = yh * 2 + xh;
-#pragma unroll 8
-        for (int x = xh * cols / 2; x < (xh + 1) * cols / 2; ++x) {
-          const float2 t = terms[y * cols + x];
-          err_sum += t.x;
-          val_sum += t.y;
-          nz[qd] += t.y;
-          if (channel == 1) mx[qd] = DStdMax(mx[qd], t.y == 0.0f ? t.x : 0.0f);
-        }
-      }
+    for (int x = 0; x < cols / 2; ++x) {
+      const float2 t = row[x];
+      sum += t.y;
+      if (channel == 1) mx = DStdMax(mx, t.y == 0.0f ? t.x : 0.0f);
     }
   }
+  *sum_out = sum;
+  *max_out = mx;
+}
+
+// The decision of one channel from its scan sums.  coef and inv give the
+// border terms again.
+template <int cols, int rows>
+__device__ void AdjustQuantDecide(float err_sum, float val_sum, const float (&nz)[4],
+                                  const float (&mx)[4], const float* __restrict__ coef,
+                                  const float* __restrict__ inv, float qac,
+                                  float mult, int channel, int strategy, float thr[4],
+                                  int32_t* raw_quant) {
+  constexpr int xsize = cols / 8, ysize = rows / 8;
   // border terms in raster order: x = cols - 1 on rows [4 ysize, 7 ysize),
   // x >= 7 xsize on rows [7 ysize, rows - 1), x >= 4 xsize on the last row
   float border = 0.0f;
@@ -769,6 +790,8 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
   constexpr int cols = W > H ? W : H, rows = W > H ? H : W;
   __shared__ float coef[K][3][N];
   __shared__ float2 terms[K][3][N];
+  __shared__ float chain_sum[K][3][kScanChains];
+  __shared__ float chain_max[K][3][4];
   __shared__ float thr_y[K][4];
   __shared__ int32_t cand[K][3];
   __shared__ int32_t decided[K];
@@ -807,6 +830,16 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
     }
   }
   __syncthreads();
+  // the scan's chains, one thread each
+  for (int item = threadIdx.x; item < K * 3 * kScanChains; item += kAqThreads) {
+    const int k = item / (3 * kScanChains), c = (item / kScanChains) % 3;
+    const int role = item % kScanChains;
+    if (slots[k].ok) {
+      ScanChain<cols, rows>(terms[k][c], role, c, &chain_sum[k][c][role],
+                            &chain_max[k][c][role < 2 ? 0 : role - 2]);
+    }
+  }
+  __syncthreads();
   if (threadIdx.x < 3 * K) {
     const int k = threadIdx.x / 3, c = threadIdx.x % 3;
     const TransformSlot& t = slots[k];
@@ -815,10 +848,14 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
       AdjustThresholds(cols, rows, thr);
       int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
       const float qac = q.scale * static_cast<float>(r);
-      AdjustQuantFromTerms<cols, rows>(terms[k][c], coef[k][c],
-                                       tables + InverseOffset(kStrategy, c), qac,
-                                       c == 0 ? params.x_mult : c == 2 ? params.b_mult : 1.0f,
-                                       c, kStrategy, thr, &r);
+      const float nz[4] = {chain_sum[k][c][2], chain_sum[k][c][3], chain_sum[k][c][4],
+                           chain_sum[k][c][5]};
+      const float mx[4] = {c == 1 ? chain_max[k][c][0] : 0.0f, c == 1 ? chain_max[k][c][1] : 0.0f,
+                           c == 1 ? chain_max[k][c][2] : 0.0f, c == 1 ? chain_max[k][c][3] : 0.0f};
+      AdjustQuantDecide<cols, rows>(chain_sum[k][c][0], chain_sum[k][c][1], nz, mx, coef[k][c],
+                                    tables + InverseOffset(kStrategy, c), qac,
+                                    c == 0 ? params.x_mult : c == 2 ? params.b_mult : 1.0f,
+                                    c, kStrategy, thr, &r);
       cand[k][c] = r;
       if (c == 1) {
         for (int i = 0; i < 4; ++i) thr_y[k][i] = thr[i];
