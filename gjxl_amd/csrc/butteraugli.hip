@@ -9,11 +9,12 @@
 //   LowSplit/HighSplit/UltraSplit
 //                         SeparateFrequencies (:702-846), as epilogues of the
 //                         blurs' column passes
-//   MaltaDiff/MaltaKernel MaltaDiffMap (butteraugli_distance_internal.cpp:
-//                         482-577), zero outside the image
 //   MaskSource/MaskKernel masking activity, blur, fuzzy erosion, MaskY/MaskDcY
-//   AcDcFinalKernel       Malta accumulation (UHF, HF, MF), the L2 terms,
-//                         masked AC Y and the final composition (:657-870)
+//   DifferencesKernel     MaltaDiffMap of the six stages (butteraugli_
+//                         distance_internal.cpp:482-577, zero outside the
+//                         image), their accumulation (UHF, HF, MF), the L2
+//                         terms, masked AC Y and the final composition
+//                         (:657-870), per tile
 //   Subsample/Expand/ComposeKernel
 //                         the two-scale map and its maximum (:872-1003)
 //   ReduceKernel          ReduceButteraugliDistanceMap (butteraugli.cpp:
@@ -541,13 +542,7 @@ constexpr MaltaTap kFullLines[16][9] = {
   {{1, -4}, {1, -3}, {1, -2}, {0, -1}, {0, 0}, {0, 1}, {-1, 2}, {-1, 3}, {-1, 4}},
 };
 
-// MaltaDiffMap fused: a work group scales the differences of a 64x16 tile
-// plus a 4-pixel halo into LDS (zero outside the image, as the reference's
-// ReadWithZeroBorder), then evaluates the 16 Malta lines of every pixel and
-// writes 0 + response (the reference accumulates into a zeroed plane).  Each
-// thread produces four adjacent outputs of one row, for two rows, each from
-// a 9 x 12 register window that it reads with 16-byte LDS reads (the sixteen
-// threads of a row read consecutive words: conflict-free).
+// Tiles of the difference stages: 64x32 outputs, a 4-pixel halo.
 constexpr int kMaltaTileW = 64, kMaltaTileH = 32, kMaltaHalo = 4;
 constexpr int kMaltaLw = kMaltaTileW + 2 * kMaltaHalo, kMaltaLh = kMaltaTileH + 2 * kMaltaHalo;
 constexpr int kMaltaQuads = kMaltaTileW / 4;
@@ -557,98 +552,6 @@ constexpr int kMaltaRowsPerPass = kThreads / kMaltaQuads;
 static_assert(kMaltaTileH == 2 * kMaltaRowsPerPass, "two rows of four outputs per thread");
 static_assert(kMaltaLw % 4 == 0, "16-byte aligned LDS rows");
 
-template <bool kLowFrequency>
-__global__ __launch_bounds__(kThreads) void MaltaKernel(
-  const float* __restrict__ p0, const float* __restrict__ p1, int w, int h,
-  MaltaScale m, float* __restrict__ out) {
-  __shared__ __align__(16) float tile[kMaltaLh][kMaltaLw];
-  const TileIndex tile_index = XcdTile();
-  const int x0 = tile_index.x * kMaltaTileW, y0 = tile_index.y * kMaltaTileH;
-  // every load of the tile issues before the first scaled difference
-  float v0[kMaltaLoads], v1[kMaltaLoads];
-#pragma unroll
-  for (int k = 0; k < kMaltaLoads; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    const int r = e / kMaltaLw, col = e % kMaltaLw;
-    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-    const size_t i = static_cast<size_t>(min(max(sy, 0), h - 1)) * w + min(max(sx, 0), w - 1);
-    v0[k] = p0[i];
-    v1[k] = p1[i];
-  }
-#pragma unroll
-  for (int k = 0; k < kMaltaLoads; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    const int r = e / kMaltaLw, col = e % kMaltaLw;
-    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-    const bool inside = sx >= 0 && sy >= 0 && sx < w && sy < h;
-    if (e < kMaltaLh * kMaltaLw) {
-      tile[r][col] = inside ? MaltaScaledDiff(v0[k], v1[k], m) : 0.0f;
-    }
-  }
-  __syncthreads();
-  const int q = threadIdx.x % kMaltaQuads;
-  const int x = x0 + 4 * q;
-  if (x >= w) return;
-#pragma unroll 1
-  for (int pass = 0; pass < 2; ++pass) {
-    const int ty = threadIdx.x / kMaltaQuads + pass * kMaltaRowsPerPass;
-    const int y = y0 + ty;
-    if (y >= h) return;
-    // win[r][c] = tile[ty + r][4q + c]: rows y - 4 .. y + 4, columns x - 4 ..
-    // x + 7
-    float win[kMaltaWinH][kMaltaWinW];
-#pragma unroll
-    for (int r = 0; r < kMaltaWinH; ++r) {
-      const float4* s4 = reinterpret_cast<const float4*>(&tile[ty + r][4 * q]);
-#pragma unroll
-      for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
-        const float4 v = s4[c4];
-        win[r][4 * c4] = v.x;
-        win[r][4 * c4 + 1] = v.y;
-        win[r][4 * c4 + 2] = v.z;
-        win[r][4 * c4 + 3] = v.w;
-      }
-    }
-    float res[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      auto P = [&](int dx, int dy) -> float {
-        return win[kMaltaHalo + dy][j + kMaltaHalo + dx];
-      };
-      float result = 0.0f;
-#pragma unroll
-      for (int l = 0; l < 16; ++l) {
-        float s;
-        if constexpr (kLowFrequency) {
-          const MaltaTap* t = kLfLines[l];
-          s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
-                   P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
-        } else {
-          const MaltaTap* t = kFullLines[l];
-          float v[9];
-#pragma unroll
-          for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
-          s = kFullTaps[l] == 9
-                ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
-                : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
-        }
-        result = l == 0 ? s * s : s * s + result;
-      }
-      res[j] = 0.0f + result;
-    }
-    float* dst = out + static_cast<size_t>(y) * w + x;
-    if ((w & 3) == 0) {
-      // rows start on 16-byte boundaries and x + 3 < w
-      *reinterpret_cast<float4*>(dst) = make_float4(res[0], res[1], res[2], res[3]);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (x + j < w) dst[j] = res[j];
-      }
-    }
-  }
-}
-
 struct L2Weights {
   float asym_gt[2];  // weight_0_gt_1 * 0.8 for HF X, Y
   float asym_lt[2];  // weight_0_lt_1 * 0.8
@@ -657,64 +560,226 @@ struct L2Weights {
   float lf[3];
 };
 
-// Malta accumulation in the reference's order (UHF, HF, MF), then the
-// asymmetric HF and symmetric MF L2 terms and the DC differences from LF;
-// then, per pixel, the Y blurred-activity term, masking and the final map
-// (MaskPsychoImage / CombineChannelsToDiffmap).  The AC and DC values stay
-// in registers; the masked Y AC and the map are stored (stages 7 and 8).
-__global__ __launch_bounds__(kThreads) void AcDcFinalKernel(
-  const float* __restrict__ ps0, const float* __restrict__ ps1,
-  const float* __restrict__ stages, size_t n, L2Weights lw,
-  const float* __restrict__ ref_blurred, const float* __restrict__ dist_blurred,
-  const float* __restrict__ mask_y, const float* __restrict__ mask_dc,
-  float x_multiplier, float* __restrict__ masked_ac_y, float* __restrict__ final_map) {
-  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float a[3], dcv[3];
-  a[1] = 0.0f;
-  a[1] += stages[4 * n + i];
-  a[1] += stages[2 * n + i];
-  a[1] += stages[0 * n + i];
-  a[0] = 0.0f;
-  a[0] += stages[5 * n + i];
-  a[0] += stages[3 * n + i];
-  a[0] += stages[1 * n + i];
-  a[2] = 0.0f;
+// The 16 Malta lines of one pixel (MaltaUnitLf / MaltaUnitFull) from a
+// register window P(dx, dy), summed in the reference's order, 0 + response.
+template <bool kLowFrequency, typename Window>
+__device__ __forceinline__ float MaltaResponse(Window P) {
+  float result = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    if (c < 2 && lw.asym_on[c]) {
-      const float v0 = ps0[(6 + c) * n + i], v1 = ps1[(6 + c) * n + i];
-      const float d = v0 - v1;
-      const float total = (d * d) * lw.asym_gt[c] + a[c];
-      const float mag = fabsf(v0);
-      const float small = 0.4f * mag;
-      float sec = 0.0f;
-      if (v0 < 0.0f) {
-        if (v1 > -small) sec = v1 + small;
-        else if (v1 < -mag) sec = -v1 - mag;
-      } else if (v1 < small) {
-        sec = small - v1;
-      } else if (v1 > mag) {
-        sec = v1 - mag;
-      }
-      a[c] = lw.asym_lt[c] * (sec * sec) + total;
+  for (int l = 0; l < 16; ++l) {
+    float s;
+    if constexpr (kLowFrequency) {
+      const MaltaTap* t = kLfLines[l];
+      s = Sum4(P(t[0].dx, t[0].dy), P(t[1].dx, t[1].dy), P(t[2].dx, t[2].dy),
+               P(t[3].dx, t[3].dy) + P(t[4].dx, t[4].dy));
+    } else {
+      const MaltaTap* t = kFullLines[l];
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) v[k] = k < kFullTaps[l] ? P(t[k].dx, t[k].dy) : 0.0f;
+      s = kFullTaps[l] == 9
+            ? (Sum4(v[0], v[1], v[2], v[3]) + Sum4(v[4], v[5], v[6], v[7])) + v[8]
+            : Sum4(v[0], v[1], v[2], Sum4(v[3], v[4], v[5], v[6]));
     }
-    if (lw.mf[c] != 0.0f) {
-      const float d = ps0[(3 + c) * n + i] - ps1[(3 + c) * n + i];
-      a[c] = (d * d) * lw.mf[c] + a[c];
-    }
-    const float e = ps0[c * n + i] - ps1[c * n + i];
-    dcv[c] = lw.lf[c] == 0.0f ? 0.0f : e * e * lw.lf[c];
+    result = l == 0 ? s * s : s * s + result;
   }
-  const float d = ref_blurred[i] - dist_blurred[i];
-  const float ay = a[1] + 10.0f * d * d;
-  masked_ac_y[i] = ay;
-  const float my = mask_y[i], mdc = mask_dc[i];
-  const float dc0 = dcv[0] * x_multiplier;
-  const float ac0 = a[0] * x_multiplier;
-  const float masked_dc = dc0 * mdc + dcv[1] * mdc + dcv[2] * mdc;
-  const float masked_ac = ac0 * my + ay * my + a[2] * my;
-  final_map[i] = sqrtf(masked_dc + masked_ac);
+  return 0.0f + result;
+}
+
+// The psycho plane of Malta stage k (MF Y, MF X, HF Y, HF X, UHF Y, UHF X)
+// and its line set (low-frequency lines for the first four).
+constexpr int kMaltaPlanes[6] = {4, 3, 7, 6, 9, 8};
+struct MaltaScales {
+  MaltaScale m[6];
+};
+
+// The difference stages of one scale, fused per 64x32 tile:
+// * MaltaDiffMap of all six stages: the scaled differences of the tile and
+//   its 4-pixel halo for each stage go to LDS (zero outside the image, as the
+//   reference's ReadWithZeroBorder), the reads of one stage issued before
+//   the previous stage's differences; each thread then evaluates four
+//   adjacent outputs of a row, for two rows, from 9 x 12 register windows of
+//   16-byte LDS reads, and stores stages 0-5;
+// * from those values in registers, the Malta accumulation in the
+//   reference's order (UHF, HF, MF), the asymmetric HF and symmetric MF L2
+//   terms, the DC differences from LF, the Y blurred-activity term, masking
+//   and the final map (MaskPsychoImage / CombineChannelsToDiffmap): stages 7
+//   and 8.
+__global__ __launch_bounds__(kThreads) void DifferencesKernel(
+  const float* __restrict__ ps0, const float* __restrict__ ps1, int w, int h,
+  MaltaScales ms, L2Weights lw, const float* __restrict__ ref_blurred,
+  const float* __restrict__ dist_blurred, const float* __restrict__ mask_y,
+  const float* __restrict__ mask_dc, float x_multiplier, float* __restrict__ stages) {
+  __shared__ __align__(16) float tile[6][kMaltaLh][kMaltaLw];
+  const TileIndex tile_index = XcdTile();
+  const int x0 = tile_index.x * kMaltaTileW, y0 = tile_index.y * kMaltaTileH;
+  const size_t n = static_cast<size_t>(w) * h;
+  const int tid = threadIdx.x;
+  constexpr int kCount = kMaltaLh * kMaltaLw;
+  auto load = [&](int k, float (&v0)[kMaltaLoads], float (&v1)[kMaltaLoads]) {
+    const float* p0 = ps0 + kMaltaPlanes[k] * n;
+    const float* p1 = ps1 + kMaltaPlanes[k] * n;
+#pragma unroll
+    for (int j = 0; j < kMaltaLoads; ++j) {
+      const int e = min(tid + j * kThreads, kCount - 1);
+      const int r = e / kMaltaLw, col = e % kMaltaLw;
+      const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+      const size_t i = static_cast<size_t>(min(max(sy, 0), h - 1)) * w + min(max(sx, 0), w - 1);
+      v0[j] = p0[i];
+      v1[j] = p1[i];
+    }
+  };
+  auto store = [&](int k, const float (&v0)[kMaltaLoads], const float (&v1)[kMaltaLoads]) {
+#pragma unroll
+    for (int j = 0; j < kMaltaLoads; ++j) {
+      const int e = tid + j * kThreads;
+      if (e < kCount) {
+        const int r = e / kMaltaLw, col = e % kMaltaLw;
+        const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+        const bool inside = sx >= 0 && sy >= 0 && sx < w && sy < h;
+        tile[k][r][col] = inside ? MaltaScaledDiff(v0[j], v1[j], ms.m[k]) : 0.0f;
+      }
+    }
+  };
+  {
+    float a0[kMaltaLoads], a1[kMaltaLoads], b0[kMaltaLoads], b1[kMaltaLoads];
+    load(0, a0, a1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (k % 2 == 0) {
+        if (k + 1 < 6) load(k + 1, b0, b1);
+        store(k, a0, a1);
+      } else {
+        if (k + 1 < 6) load(k + 1, a0, a1);
+        store(k, b0, b1);
+      }
+    }
+  }
+  __syncthreads();
+  const int q = tid % kMaltaQuads;
+  const int x = x0 + 4 * q;
+  if (x >= w) return;
+  // four consecutive pixels of a plane: one 16-byte access when rows start
+  // on 16-byte boundaries (then x + 3 < w), else one access per pixel
+  const bool vector = (w & 3) == 0;
+  auto load4 = [&](const float* plane, size_t i, float (&out)[4]) {
+    if (vector) {
+      const float4 v = *reinterpret_cast<const float4*>(plane + i);
+      out[0] = v.x;
+      out[1] = v.y;
+      out[2] = v.z;
+      out[3] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[j] = x + j < w ? plane[i + j] : 0.0f;
+    }
+  };
+  auto store4 = [&](float* plane, size_t i, const float (&v)[4]) {
+    if (vector) {
+      *reinterpret_cast<float4*>(plane + i) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (x + j < w) plane[i + j] = v[j];
+      }
+    }
+  };
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ty = tid / kMaltaQuads + pass * kMaltaRowsPerPass;
+    const int y = y0 + ty;
+    if (y >= h) return;
+    const size_t base = static_cast<size_t>(y) * w + x;
+    float sv[6][4];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      // win[r][c] = tile[k][ty + r][4q + c]: rows y - 4 .. y + 4, columns
+      // x - 4 .. x + 7
+      float win[kMaltaWinH][kMaltaWinW];
+#pragma unroll
+      for (int r = 0; r < kMaltaWinH; ++r) {
+        const float4* s4 = reinterpret_cast<const float4*>(&tile[k][ty + r][4 * q]);
+#pragma unroll
+        for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
+          const float4 v = s4[c4];
+          win[r][4 * c4] = v.x;
+          win[r][4 * c4 + 1] = v.y;
+          win[r][4 * c4 + 2] = v.z;
+          win[r][4 * c4 + 3] = v.w;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        auto P = [&](int dx, int dy) -> float {
+          return win[kMaltaHalo + dy][j + kMaltaHalo + dx];
+        };
+        sv[k][j] = k < 4 ? MaltaResponse<true>(P) : MaltaResponse<false>(P);
+      }
+      store4(stages + k * n, base, sv[k]);
+    }
+    // the per-pixel inputs of the L2 terms and the final map
+    float p0[8][4], p1[8][4];  // planes LF 0-2, MF 3-5, HF 6-7
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      load4(ps0 + c * n, base, p0[c]);
+      load4(ps1 + c * n, base, p1[c]);
+    }
+    float rb[4], db[4], my[4], mdc[4];
+    load4(ref_blurred, base, rb);
+    load4(dist_blurred, base, db);
+    load4(mask_y, base, my);
+    load4(mask_dc, base, mdc);
+    float masked_y[4], final_map[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float a[3], dcv[3];
+      a[1] = 0.0f;
+      a[1] += sv[4][j];
+      a[1] += sv[2][j];
+      a[1] += sv[0][j];
+      a[0] = 0.0f;
+      a[0] += sv[5][j];
+      a[0] += sv[3][j];
+      a[0] += sv[1][j];
+      a[2] = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (c < 2 && lw.asym_on[c]) {
+          const float v0 = p0[6 + c][j], v1 = p1[6 + c][j];
+          const float d = v0 - v1;
+          const float total = (d * d) * lw.asym_gt[c] + a[c];
+          const float mag = fabsf(v0);
+          const float small = 0.4f * mag;
+          float sec = 0.0f;
+          if (v0 < 0.0f) {
+            if (v1 > -small) sec = v1 + small;
+            else if (v1 < -mag) sec = -v1 - mag;
+          } else if (v1 < small) {
+            sec = small - v1;
+          } else if (v1 > mag) {
+            sec = v1 - mag;
+          }
+          a[c] = lw.asym_lt[c] * (sec * sec) + total;
+        }
+        if (lw.mf[c] != 0.0f) {
+          const float d = p0[3 + c][j] - p1[3 + c][j];
+          a[c] = (d * d) * lw.mf[c] + a[c];
+        }
+        const float e = p0[c][j] - p1[c][j];
+        dcv[c] = lw.lf[c] == 0.0f ? 0.0f : e * e * lw.lf[c];
+      }
+      const float d = rb[j] - db[j];
+      const float ay = a[1] + 10.0f * d * d;
+      masked_y[j] = ay;
+      const float dc0 = dcv[0] * x_multiplier;
+      const float ac0 = a[0] * x_multiplier;
+      const float masked_dc = dc0 * mdc[j] + dcv[1] * mdc[j] + dcv[2] * mdc[j];
+      const float masked_ac = ac0 * my[j] + ay * my[j] + a[2] * my[j];
+      final_map[j] = sqrtf(masked_dc + masked_ac);
+    }
+    store4(stages + 7 * n, base, masked_y);
+    store4(stages + 8 * n, base, final_map);
+  }
 }
 
 // CombineChannelsForMasking + DiffPrecompute.
@@ -1004,27 +1069,15 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
   const double down[6] = {kMaltaWeights[0], kMaltaWeights[1], kMaltaWeights[2] / sqrt_asym,
                           kMaltaWeights[3] / sqrt_asym, kMaltaWeights[4] / asym,
                           kMaltaWeights[5] / asym};
-  const int plane_of[6] = {4, 3, 7, 6, 9, 8};
+  MaltaScales ms;
   for (int k = 0; k < 6; ++k) {
     const bool lf = k < 4;
     const double multiplier = lf ? 0.611612573796 : 0.39905817637;
     const double pre_gt = multiplier * std::sqrt(0.5f * up[k]) / (3.75 * 2.0 + 1.0);
     const double pre_lt = multiplier * std::sqrt(0.33f * down[k]) / (3.75 * 2.0 + 1.0);
-    const MaltaScale m{static_cast<float>(pre_gt * kMaltaNorms[k]),
-                       static_cast<float>(pre_lt * kMaltaNorms[k]),
-                       static_cast<float>(kMaltaNorms[k])};
-    const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
-                     CeilDiv(static_cast<uint32_t>(h), kMaltaTileH));
-    const float* r0 = s.ref_psycho + plane_of[k] * n;
-    const float* r1 = s.dist_psycho + plane_of[k] * n;
-    if (lf) {
-      hipLaunchKernelGGL(MaltaKernel<true>, tiles, dim3(kThreads), 0, st, r0, r1, w, h, m,
-                         s.stages + k * n);
-    } else {
-      hipLaunchKernelGGL(MaltaKernel<false>, tiles, dim3(kThreads), 0, st, r0, r1, w, h, m,
-                         s.stages + k * n);
-    }
-    GJXL_AMD_TRY(CheckLaunch("MaltaKernel"));
+    ms.m[k] = MaltaScale{static_cast<float>(pre_gt * kMaltaNorms[k]),
+                         static_cast<float>(pre_lt * kMaltaNorms[k]),
+                         static_cast<float>(kMaltaNorms[k])};
   }
   L2Weights lw{};
   for (int c = 0; c < 2; ++c) {
@@ -1038,10 +1091,12 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
     lw.lf[c] = kL2[6 + c];
   }
   GJXL_AMD_TRY(MaskActivity(b, s, s.dist_psycho, s.tmp[2]));
-  hipLaunchKernelGGL(AcDcFinalKernel, Grid1(n), dim3(kThreads), 0, st, s.ref_psycho,
-                     s.dist_psycho, s.stages, n, lw, s.ref_blurred, s.tmp[2], s.mask_y,
-                     s.mask_dc, b->params.x_multiplier, s.stages + 7 * n, s.stages + 8 * n);
-  return CheckLaunch("AcDcFinalKernel");
+  const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
+                   CeilDiv(static_cast<uint32_t>(h), kMaltaTileH));
+  hipLaunchKernelGGL(DifferencesKernel, tiles, dim3(kThreads), 0, st, s.ref_psycho,
+                     s.dist_psycho, w, h, ms, lw, s.ref_blurred, s.tmp[2], s.mask_y,
+                     s.mask_dc, b->params.x_multiplier, s.stages);
+  return CheckLaunch("DifferencesKernel");
 }
 
 constexpr int kScalePlanes = 3 + 3 + 10 + 10 + 3 + 9 + 3;
