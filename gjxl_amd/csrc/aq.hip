@@ -496,16 +496,28 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
   // distinct banks
   __shared__ float terms[2][2][kCflChunk + 4];
   __shared__ float lanes[2][2][4];
-  __shared__ uint32_t anchor_block[64], anchor_base[64];
+  __shared__ uint32_t anchor_base[64], anchor_offset[64];
+  __shared__ float anchor_qs[64];
+  __shared__ uint8_t anchor_strategy[64];
   __shared__ int anchors, total;
   const int tx = blockIdx.x, ty = blockIdx.y;
   const int t = threadIdx.x;
-  // the tile's anchors in raster order with their term offsets
+  const float scale128 = q->scale * 128.0f;
+  // the tile's anchors in raster order with their term offsets, strategy,
+  // coefficient offset and quantisation scale
   if (t < 64) {
     const int bx = tx * 8 + (t & 7), by = ty * 8 + (t >> 3);
     uint32_t n = 0;
+    int strategy = 0;
+    uint32_t offset = 0;
+    float qs = 0.0f;
     if (bx < bw && by < bh) {
-      const uint8_t c = cells[static_cast<size_t>(by) * bw + bx];
+      const size_t b = static_cast<size_t>(by) * bw + bx;
+      const uint8_t c = cells[b];
+      const int32_t r = raw[b];
+      offset = offsets[b];
+      strategy = c >> 1;
+      qs = scale128 * static_cast<float>(r);
       if (c & 1) n = static_cast<uint32_t>(CoveredW(c >> 1) * CoveredH(c >> 1) * 64);
     }
     uint32_t base = n;
@@ -519,8 +531,10 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
     const uint64_t kept = __ballot(fits);
     const int rank = __popcll(kept & ((uint64_t{1} << t) - 1));
     if (fits) {
-      anchor_block[rank] = (static_cast<uint32_t>(by) << 16) | static_cast<uint32_t>(bx);
       anchor_base[rank] = base;
+      anchor_offset[rank] = offset;
+      anchor_qs[rank] = qs;
+      anchor_strategy[rank] = static_cast<uint8_t>(strategy);
     }
     if (t == 63) {
       anchors = __popcll(kept);
@@ -528,38 +542,64 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
     }
   }
   __syncthreads();
-  const float scale128 = q->scale * 128.0f;
   const int count = anchors, terms_used = min(total, kCflTerms);
   const int problem_t = t / 8, kind_t = (t / 4) % 2, lane_t = t % 4;
   float acc = 0.0f;
+  constexpr int kPerThread = kCflChunk / 256;
   for (int first = 0; first < terms_used; first += kCflChunk) {
     const int chunk = min(kCflChunk, terms_used - first);
-    for (int j = t; j < chunk && count > 0; j += blockDim.x) {
+    // this thread's elements: their anchor (binary search over the LDS
+    // list), then every global read, then the terms
+    int idx[kPerThread], strategy[kPerThread];
+    float qs_e[kPerThread], cy[kPerThread], cx[kPerThread], cb[kPerThread];
+    float inv_x[kPerThread], inv_b[kPerThread];
+    bool live[kPerThread];
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+      const int j = t + m * 256;
       const int p = first + j;
+      live[m] = j < chunk && count > 0;
       int lo = 0, hi = count - 1;  // the last anchor starting at or before p
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (anchor_base[mid] <= static_cast<uint32_t>(p)) lo = mid; else hi = mid - 1;
       }
-      const int bx = anchor_block[lo] & 0xffff, by = anchor_block[lo] >> 16;
-      const int st = cells[static_cast<size_t>(by) * bw + bx] >> 1;
+      const int st = anchor_strategy[lo];
       const int cw = CoveredW(st), ch = CoveredH(st);
       const int W = cw * 8, H = ch * 8, N = W * H;
       const int i = p - static_cast<int>(anchor_base[lo]);
-      if (i >= N) continue;
+      live[m] = live[m] && i < N;
       const int cols = W > H ? W : H;
       const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
-      const float qs = scale128 * static_cast<float>(raw[static_cast<size_t>(by) * bw + bx]);
-      const size_t off = offsets[static_cast<size_t>(by) * bw + bx];
       const int xx = i % cols, yy = i / cols;
       const bool llf = xx < lw && yy < lh;
-      const float cy = llf ? 0.0f : coefs[store + off + i];
+      const size_t off = anchor_offset[lo];
+      const int ii = live[m] ? i : 0;
+      idx[m] = ii;
+      strategy[m] = st;
+      qs_e[m] = anchor_qs[lo];
+      // LLF coefficients count as zero (their reads are kept in bounds)
+      cy[m] = coefs[store + off + ii];
+      cx[m] = coefs[off + ii];
+      cb[m] = coefs[2 * store + off + ii];
+      inv_x[m] = tables[InverseOffset(st, 0) + ii];
+      inv_b[m] = tables[InverseOffset(st, 2) + ii];
+      if (llf) {
+        cy[m] = 0.0f;
+        cx[m] = 0.0f;
+        cb[m] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+      if (!live[m]) continue;
+      const int j = t + m * 256;
 #pragma unroll
       for (int problem = 0; problem < 2; ++problem) {
-        const float* inv = tables + InverseOffset(st, problem == 0 ? 0 : 2);
-        const float cc = llf ? 0.0f : coefs[(problem == 0 ? 0 : 2) * store + off + i];
-        const float luma = cy * inv[i] * qs;
-        const float chroma = cc * inv[i] * qs;
+        const float inv = problem == 0 ? inv_x[m] : inv_b[m];
+        const float cc = problem == 0 ? cx[m] : cb[m];
+        const float luma = cy[m] * inv * qs_e[m];
+        const float chroma = cc * inv * qs_e[m];
         terms[problem][0][j] = luma / 84.0f;
         terms[problem][1][j] = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
       }
