@@ -837,32 +837,54 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
   __shared__ int32_t decided[K];
   __shared__ float cfl[K][2];
   __shared__ TransformSlot slots[K];
+  __shared__ int32_t raw_k[K];
   LoadSlots<K>(entries, count, offsets, bw, slots);
   __syncthreads();
   const QuantizerState q = *qstate;
-  for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
-    const TransformSlot& t = slots[k];
-    coef[k][c][i] = t.ok ? coefs[c * store + t.offset + i] : 0.0f;
+  // every global read that depends only on the slots, issued together: the
+  // coefficients, the inverse-quantisation entries of the same elements and
+  // the raw quant of each transform
+  constexpr int kPer = (K * 3 * N + kAqThreads - 1) / kAqThreads;
+  float cv[kPer], invv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + j * kAqThreads;
+    cv[j] = invv[j] = 0.0f;
+    if (e < K * 3 * N) {
+      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+      const TransformSlot& t = slots[k];
+      if (t.ok) cv[j] = coefs[c * store + t.offset + i];
+      invv[j] = tables[InverseOffset(kStrategy, c) + i];
+    }
+  }
+  if (threadIdx.x < K) {
+    const TransformSlot& t = slots[threadIdx.x];
+    raw_k[threadIdx.x] = t.ok ? raw[static_cast<size_t>(t.by) * bw + t.bx] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + j * kAqThreads;
+    if (e < K * 3 * N) (&coef[0][0][0])[e] = cv[j];
   }
   __syncthreads();
   // terms of the AdjustQuantForChannel scans
   {
     float thr0[4];
     AdjustThresholds(cols, rows, thr0);
-    for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int e = threadIdx.x + j * kAqThreads;
+      if (e >= K * 3 * N) continue;
       const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
       const TransformSlot& t = slots[k];
       const int x = i % cols, y = i / cols;
       float2 term = make_float2(0.0f, 0.0f);
       if (t.ok && !(x < cols / 8 && y < rows / 8)) {
-        const float qac =
-          q.scale * static_cast<float>(raw[static_cast<size_t>(t.by) * bw + t.bx]);
-        const float* inv = tables + InverseOffset(kStrategy, c);
+        const float qac = q.scale * static_cast<float>(raw_k[k]);
         // thresholds 1-3 start equal; quadrant 0 has its own
         const bool q0 = y < rows / 2 && x < cols / 2;
         const float mult = c == 0 ? params.x_mult : c == 2 ? params.b_mult : 1.0f;
-        const float value = coef[k][c][i] * (inv[i] * qac * mult);
+        const float value = cv[j] * (invv[j] * qac * mult);
         const float quantized = AdjustQuantized(value, q0 ? thr0[0] : thr0[1]);
         term = make_float2(fabsf(value - quantized), fabsf(quantized));
       }
@@ -886,7 +908,7 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
     if (t.ok) {
       float thr[4];
       AdjustThresholds(cols, rows, thr);
-      int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
+      int32_t r = raw_k[k];
       const float qac = q.scale * static_cast<float>(r);
       const float nz[4] = {chain_sum[k][c][2], chain_sum[k][c][3], chain_sum[k][c][4],
                            chain_sum[k][c][5]};
@@ -1387,28 +1409,68 @@ __global__ __launch_bounds__(kAqThreads) void ReconstructKernel(
   LoadSlots<K>(entries, count, offsets, bw, slots);
   __syncthreads();
   const QuantizerState q = *qstate;
+  // every global read that depends only on the slots, issued together:
+  // quantized coefficients and dequantisation entries, the raw quant and
+  // CfL multipliers, the smoothed DC samples
+  constexpr int kPer = (K * 3 * N + kAqThreads - 1) / kAqThreads;
+  int32_t qv[kPer];
+  float dqv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + j * kAqThreads;
+    qv[j] = 0;
+    dqv[j] = 0.0f;
+    if (e < K * 3 * N) {
+      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
+      const TransformSlot& t = slots[k];
+      if (t.ok) {
+        qv[j] = quantized[c * store + t.offset + i];
+        dqv[j] = tables[DequantOffset(kStrategy, c) + i];
+      }
+    }
+  }
+  int32_t r = 1;
+  int8_t yx = 0, yb = 0;
   if (threadIdx.x < K) {
-    const int k = threadIdx.x;
+    const TransformSlot& t = slots[threadIdx.x];
+    if (t.ok) {
+      r = raw[static_cast<size_t>(t.by) * bw + t.bx];
+      const int tile = (t.by / 8) * ((bw + 7) / 8) + t.bx / 8;
+      yx = y_to_x[tile];
+      yb = y_to_b[tile];
+    }
+  }
+  float dcv[16];
+  if (threadIdx.x < 3 * K) {
+    const int k = threadIdx.x / 3, c = threadIdx.x % 3;
     const TransformSlot& t = slots[k];
     if (t.ok) {
-      const int32_t r = raw[static_cast<size_t>(t.by) * bw + t.bx];
+      const size_t n = static_cast<size_t>(bw) * bh;
+#pragma unroll
+      for (int dy = 0; dy < S::kCh; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < S::kCw; ++dx)
+          dcv[dy * S::kCw + dx] =
+            smoothed_dc[c * n + static_cast<size_t>(t.by + dy) * bw + t.bx + dx];
+    }
+  }
+  if (threadIdx.x < K) {
+    const int k = threadIdx.x;
+    if (slots[k].ok) {
       const float mults[3] = {params.x_mult, 1.0f, params.b_mult};
       for (int c = 0; c < 3; ++c) scale[k][c] = q.inverse_global_scale / static_cast<float>(r) / mults[c];
-      const int tile = (t.by / 8) * ((bw + 7) / 8) + t.bx / 8;
-      cfl[k][0] = static_cast<float>(y_to_x[tile]) / 84.0f;
-      cfl[k][1] = 1.0f + static_cast<float>(y_to_b[tile]) / 84.0f;
+      cfl[k][0] = static_cast<float>(yx) / 84.0f;
+      cfl[k][1] = 1.0f + static_cast<float>(yb) / 84.0f;
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < K * 3 * N; e += kAqThreads) {
-    const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
-    const TransformSlot& t = slots[k];
-    float v = 0.0f;
-    if (t.ok) {
-      const float* dq = tables + DequantOffset(kStrategy, c);
-      v = QuantBias(quantized[c * store + t.offset + i], c) * (dq[i] * scale[k][c]);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + j * kAqThreads;
+    if (e < K * 3 * N) {
+      const int k = e / (3 * N), c = (e / N) % 3;
+      coef[e] = slots[k].ok ? QuantBias(qv[j], c) * (dqv[j] * scale[k][c]) : 0.0f;
     }
-    coef[e] = v;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < K * N; e += kAqThreads) {
@@ -1420,16 +1482,7 @@ __global__ __launch_bounds__(kAqThreads) void ReconstructKernel(
   __syncthreads();
   if (threadIdx.x < 3 * K) {
     const int k = threadIdx.x / 3, c = threadIdx.x % 3;
-    const TransformSlot& t = slots[k];
-    if (t.ok) {
-      const size_t n = static_cast<size_t>(bw) * bh;
-      float dcv[16];
-      for (int dy = 0; dy < S::kCh; ++dy)
-        for (int dx = 0; dx < S::kCw; ++dx)
-          dcv[dy * S::kCw + dx] =
-            smoothed_dc[c * n + static_cast<size_t>(t.by + dy) * bw + t.bx + dx];
-      DcToLlf(params.dc_basis, S::kCw, S::kCh, dcv, coef + (k * 3 + c) * N);
-    }
+    if (slots[k].ok) DcToLlf(params.dc_basis, S::kCw, S::kCh, dcv, coef + (k * 3 + c) * N);
   }
   __syncthreads();
   InverseRows<W, H, 3 * K>(ih, coef, tmp);
