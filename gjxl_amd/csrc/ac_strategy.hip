@@ -660,7 +660,7 @@ __global__ void FillNanKernel(float* __restrict__ p, size_t n) {
   if (i < n) p[i] = __builtin_nanf("");
 }
 
-// ---- a10: greedy merge, one thread per 64x64 tile --------------------------
+// ---- a10: greedy merge, one work group per 64x64 tile ----------------------
 
 constexpr int kStageStrategy[7] = {kDct8, kDct16x8, kDct8x16, kDct16x16,
                                    kDct32x16, kDct16x32, kDct32x32};
@@ -679,23 +679,23 @@ __device__ __forceinline__ int StageOf(int strategy) {
 
 constexpr int kSearchThreads = 64;
 
+// The tile's state and its candidate costs live in LDS; one thread of the
+// work group runs the search.
 struct TileSearch {
-  const float* costs;
+  const float (*table)[64];  // [stage][block], the tile's candidate costs
   int bw, bh;
   int tbx, tby;  // tile origin in blocks
   int tw, th;
-  uint8_t* cells;     // [64][kSearchThreads] per-thread column
+  uint8_t* cells;  // [block]
   float* cost;
   uint8_t* prio;
-  int lane;
 
-  __device__ uint8_t& Cell(int lx, int ly) { return cells[(ly * 8 + lx) * kSearchThreads + lane]; }
-  __device__ float& Cost(int lx, int ly) { return cost[(ly * 8 + lx) * kSearchThreads + lane]; }
-  __device__ uint8_t& Prio(int lx, int ly) { return prio[(ly * 8 + lx) * kSearchThreads + lane]; }
+  __device__ uint8_t& Cell(int lx, int ly) { return cells[ly * 8 + lx]; }
+  __device__ float& Cost(int lx, int ly) { return cost[ly * 8 + lx]; }
+  __device__ uint8_t& Prio(int lx, int ly) { return prio[ly * 8 + lx]; }
 
   __device__ float Table(int strategy, int lx, int ly) const {
-    return costs[static_cast<size_t>(StageOf(strategy)) * bw * bh +
-                 static_cast<size_t>(tby + ly) * bw + tbx + lx];
+    return table[StageOf(strategy)][ly * 8 + lx];
   }
 
   __device__ void Set(int lx, int ly, int strategy) {
@@ -811,17 +811,20 @@ struct TileSearch {
   }
 };
 
+// grid: one work group of 64 threads per tile.  Thread i loads block i's
+// seven candidate costs and initialises its state; thread 0 searches; thread
+// i stores block i's cell.
 __global__ __launch_bounds__(kSearchThreads) void SearchKernel(
   const float* __restrict__ costs, int bw, int bh, float target,
   uint8_t* __restrict__ out_cells) {
-  __shared__ uint8_t cells[64 * kSearchThreads];
-  __shared__ float cost[64 * kSearchThreads];
-  __shared__ uint8_t prio[64 * kSearchThreads];
-  const int tiles_w = (bw + 7) / 8, tiles_h = (bh + 7) / 8;
-  const int tile = blockIdx.x * kSearchThreads + threadIdx.x;
-  if (tile >= tiles_w * tiles_h) return;
+  __shared__ float table[7][64];
+  __shared__ uint8_t cells[64];
+  __shared__ float cost[64];
+  __shared__ uint8_t prio[64];
+  const int tiles_w = (bw + 7) / 8;
+  const int tile = blockIdx.x;
   TileSearch s;
-  s.costs = costs;
+  s.table = table;
   s.bw = bw;
   s.bh = bh;
   s.tbx = (tile % tiles_w) * 8;
@@ -831,52 +834,60 @@ __global__ __launch_bounds__(kSearchThreads) void SearchKernel(
   s.cells = cells;
   s.cost = cost;
   s.prio = prio;
-  s.lane = threadIdx.x;
-  for (int i = 0; i < 64; ++i) {
-    s.Cell(i % 8, i / 8) = static_cast<uint8_t>((kDct8 << 1) | 1);
-    s.Cost(i % 8, i / 8) = 0.0f;
-    s.Prio(i % 8, i / 8) = 0;
-  }
-  // src/codec/ac_strategy.cpp:591-773 SearchTile
-  const float baseline = 1.0f + -0.4f / (target + 1.4f);
-  for (int y = 0; y < s.th; ++y)
-    for (int x = 0; x < s.tw; ++x) s.Cost(x, y) = s.Table(kDct8, x, y) * baseline;
-  constexpr int kMergeStrategy[4] = {kDct16x8, kDct8x16, kDct16x32, kDct32x16};
-  constexpr uint8_t kMergePriority[4] = {2, 2, 4, 4};
-  for (int m = 0; m < 4; ++m) {
-    const int st = kMergeStrategy[m];
-    const int cw = CoveredW(st), ch = CoveredH(st);
-    for (int y = 0; y + ch <= s.th; y += ch) {
-      for (int x = 0; x + cw <= s.tw; x += cw) {
-        if (y + 3 < s.th && x + 3 < s.tw) {
-          if (st == kDct16x32) {
-            if ((y | x) % 4 == 0) s.FirstLevelDivision(4, x, y);
-            continue;
+  const int i = threadIdx.x, lx = i % 8, ly = i / 8;
+  const bool inside = lx < s.tw && ly < s.th;
+  const size_t plane = static_cast<size_t>(bw) * bh;
+  const size_t block = static_cast<size_t>(s.tby + (inside ? ly : 0)) * bw + s.tbx + (inside ? lx : 0);
+  float v[7];
+#pragma unroll
+  for (int st = 0; st < 7; ++st) v[st] = costs[st * plane + block];
+#pragma unroll
+  for (int st = 0; st < 7; ++st) table[st][i] = inside ? v[st] : 0.0f;
+  cells[i] = static_cast<uint8_t>((kDct8 << 1) | 1);
+  cost[i] = 0.0f;
+  prio[i] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // src/codec/ac_strategy.cpp:591-773 SearchTile
+    const float baseline = 1.0f + -0.4f / (target + 1.4f);
+    for (int y = 0; y < s.th; ++y)
+      for (int x = 0; x < s.tw; ++x) s.Cost(x, y) = s.Table(kDct8, x, y) * baseline;
+    constexpr int kMergeStrategy[4] = {kDct16x8, kDct8x16, kDct16x32, kDct32x16};
+    constexpr uint8_t kMergePriority[4] = {2, 2, 4, 4};
+    for (int m = 0; m < 4; ++m) {
+      const int st = kMergeStrategy[m];
+      const int cw = CoveredW(st), ch = CoveredH(st);
+      for (int y = 0; y + ch <= s.th; y += ch) {
+        for (int x = 0; x + cw <= s.tw; x += cw) {
+          if (y + 3 < s.th && x + 3 < s.tw) {
+            if (st == kDct16x32) {
+              if ((y | x) % 4 == 0) s.FirstLevelDivision(4, x, y);
+              continue;
+            }
+            if (st == kDct32x16) continue;
           }
-          if (st == kDct32x16) continue;
-        }
-        if ((st == kDct16x32 && y % 4 != 0) || (st == kDct32x16 && x % 4 != 0)) continue;
-        if (y + 1 < s.th && x + 1 < s.tw) {
-          if (st == kDct8x16) {
-            if ((y | x) % 2 == 0) s.FirstLevelDivision(2, x, y);
-            continue;
+          if ((st == kDct16x32 && y % 4 != 0) || (st == kDct32x16 && x % 4 != 0)) continue;
+          if (y + 1 < s.th && x + 1 < s.tw) {
+            if (st == kDct8x16) {
+              if ((y | x) % 2 == 0) s.FirstLevelDivision(2, x, y);
+              continue;
+            }
+            if (st == kDct16x8) continue;
           }
-          if (st == kDct16x8) continue;
+          if ((st == kDct8x16 && y % 2 == 1) || (st == kDct16x8 && x % 2 == 1)) continue;
+          s.TryMerge(st, x, y, kMergePriority[m]);
         }
-        if ((st == kDct8x16 && y % 2 == 1) || (st == kDct16x8 && x % 2 == 1)) continue;
-        s.TryMerge(st, x, y, kMergePriority[m]);
       }
     }
+    for (int y = 0; y + 1 < s.th; ++y)
+      for (int x = 0; x + 1 < s.tw; ++x)
+        if ((y | x) % 2 != 0) s.FirstLevelDivision(2, x, y);
+    for (int y = 0; y + 3 < s.th; y += 2)
+      for (int x = 0; x + 3 < s.tw; x += 2)
+        if ((y | x) % 4 != 0) s.FirstLevelDivision(4, x, y);
   }
-  for (int y = 0; y + 1 < s.th; ++y)
-    for (int x = 0; x + 1 < s.tw; ++x)
-      if ((y | x) % 2 != 0) s.FirstLevelDivision(2, x, y);
-  for (int y = 0; y + 3 < s.th; y += 2)
-    for (int x = 0; x + 3 < s.tw; x += 2)
-      if ((y | x) % 4 != 0) s.FirstLevelDivision(4, x, y);
-  for (int y = 0; y < s.th; ++y)
-    for (int x = 0; x < s.tw; ++x)
-      out_cells[static_cast<size_t>(s.tby + y) * bw + s.tbx + x] = s.Cell(x, y);
+  __syncthreads();
+  if (inside) out_cells[block] = cells[i];
 }
 
 template <int kStrategy>
@@ -942,7 +953,7 @@ GJXLResult LaunchSearch(GJXLAmdDevice* device, const float* costs, int width,
                         int height, float target, uint8_t* cells) {
   const int bw = width / 8, bh = height / 8;
   const int tiles = CeilDiv(bw, 8) * CeilDiv(bh, 8);
-  hipLaunchKernelGGL(SearchKernel, dim3(CeilDiv(tiles, kSearchThreads)),
+  hipLaunchKernelGGL(SearchKernel, dim3(tiles),
                      dim3(kSearchThreads), 0, device->stream, costs, bw, bh,
                      target, cells);
   return CheckLaunch("SearchKernel");
