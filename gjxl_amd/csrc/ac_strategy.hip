@@ -379,52 +379,52 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   // the first barrier waits for, then the mask, the quantisation table
   // entries of the elements this thread quantises and the quant-field and
   // CfL inputs, which reach LDS behind the forward passes.
-  constexpr int kPer = (KC * N + kCostThreads - 1) / kCostThreads;
+  // Each strategy's slots hold a whole number of elements per thread, so
+  // every read below is unconditional: a candidate that is not launched
+  // reads element 0 of the plane and discards it.  (A read under a branch
+  // makes the compiler wait for the earlier reads at the branch's join.)
+  constexpr int kPer = KC * N / kCostThreads;
+  constexpr int kMaskPer = K * N / kCostThreads;
+  static_assert(KC * N == kPer * kCostThreads && K * N == kMaskPer * kCostThreads,
+                "whole elements per thread");
   float pixel[kPer], dq_e[kPer], inv_e[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
-    pixel[j] = 0.0f;
-    if (e < KC * N) {
-      const int kc = e / N, r = e % N;
-      const int k = kc / 3, c = kc % 3;
-      int bx, by;
-      if (candidate(k, &bx, &by)) {
-        pixel[j] = opsin[c * plane + static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W];
-      }
-    }
+    const int kc = e / N, r = e % N;
+    const int k = kc / 3, c = kc % 3;
+    int bx, by;
+    const bool live = candidate(k, &bx, &by);
+    const size_t at = c * plane + static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W;
+    const float v = opsin[live ? at : 0];
+    pixel[j] = live ? v : 0.0f;
   }
-  constexpr int kMaskPer = (K * N + kCostThreads - 1) / kCostThreads;
   float mask_e[kMaskPer];
 #pragma unroll
   for (int j = 0; j < kMaskPer; ++j) {
     const int e = tid + j * kCostThreads;
-    mask_e[j] = 0.0f;
-    if (e < K * N) {
-      const int k = e / N, r = e % N;
-      int bx, by;
-      if (candidate(k, &bx, &by)) {
-        mask_e[j] = pixel_mask[static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W];
-      }
-    }
+    const int k = e / N, r = e % N;
+    int bx, by;
+    const bool live = candidate(k, &bx, &by);
+    const size_t at = static_cast<size_t>(by * 8 + r / W) * width + bx * 8 + r % W;
+    const float v = pixel_mask[live ? at : 0];
+    mask_e[j] = live ? v : 0.0f;
   }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = tid + j * kCostThreads;
-    dq_e[j] = inv_e[j] = 0.0f;
-    if (e < KC * N) {
-      const int r = e % N, c = (e / N) % 3;
-      // element e is coefficient (v, u) = (r / W, r % W) in the quantise step
-      const int i = CoefficientIndex<W, H>(r / W, r % W);
-      dq_e[j] = tables[DequantOffset(kStrategy, c) + i];
-      inv_e[j] = tables[InverseOffset(kStrategy, c) + i];
-    }
+    const int r = e % N, c = (e / N) % 3;
+    // element e is coefficient (v, u) = (r / W, r % W) in the quantise step
+    const int i = CoefficientIndex<W, H>(r / W, r % W);
+    dq_e[j] = tables[DequantOffset(kStrategy, c) + i];
+    inv_e[j] = tables[InverseOffset(kStrategy, c) + i];
   }
   // thread k < K: candidate k's quant-field blocks and the tile's CfL
   // multipliers (ComputeQuantNormUnchecked, src/codec/ac_strategy.cpp:103-137)
   constexpr int kBlocks = S::kCw * S::kCh;
   float qv[kBlocks];
-  int8_t yx = 0, yb = 0;
+  const int tile = ty * tiles_w + tx;
+  const int8_t yx = y_to_x[tile], yb = y_to_b[tile];
   bool ok = false;
   if (tid < K) {
     int bx, by;
@@ -438,9 +438,6 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
 #pragma unroll
       for (int dx = 0; dx < S::kCw; ++dx) qv[dy * S::kCw + dx] = q[static_cast<size_t>(dy) * bw + dx];
     }
-    const int tile = ty * tiles_w + tx;
-    yx = y_to_x[tile];
-    yb = y_to_b[tile];
     for (int c = 0; c < 3; ++c) nnz[tid][c] = 0;
   }
   // pixels transposed ([x][y]) into LDS

@@ -499,6 +499,7 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
   __shared__ uint32_t anchor_base[64], anchor_offset[64];
   __shared__ float anchor_qs[64];
   __shared__ uint8_t anchor_strategy[64];
+  __shared__ int anchor_inverse[64][2];  // X and B inverse-table offsets
   __shared__ int anchors, total;
   const int tx = blockIdx.x, ty = blockIdx.y;
   const int t = threadIdx.x;
@@ -535,6 +536,8 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
       anchor_offset[rank] = offset;
       anchor_qs[rank] = qs;
       anchor_strategy[rank] = static_cast<uint8_t>(strategy);
+      anchor_inverse[rank][0] = InverseOffset(strategy, 0);
+      anchor_inverse[rank][1] = InverseOffset(strategy, 2);
     }
     if (t == 63) {
       anchors = __popcll(kept);
@@ -553,7 +556,7 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
     int idx[kPerThread], strategy[kPerThread];
     float qs_e[kPerThread], cy[kPerThread], cx[kPerThread], cb[kPerThread];
     float inv_x[kPerThread], inv_b[kPerThread];
-    bool live[kPerThread];
+    bool live[kPerThread], llf[kPerThread];
 #pragma unroll
     for (int m = 0; m < kPerThread; ++m) {
       const int j = t + m * 256;
@@ -572,33 +575,30 @@ __global__ __launch_bounds__(256) void FinalCflKernel(
       const int cols = W > H ? W : H;
       const int lw = W > H ? cw : ch, lh = W > H ? ch : cw;
       const int xx = i % cols, yy = i / cols;
-      const bool llf = xx < lw && yy < lh;
+      llf[m] = xx < lw && yy < lh;
       const size_t off = anchor_offset[lo];
       const int ii = live[m] ? i : 0;
       idx[m] = ii;
       strategy[m] = st;
       qs_e[m] = anchor_qs[lo];
-      // LLF coefficients count as zero (their reads are kept in bounds)
+      // LLF coefficients count as zero (their reads are kept in bounds and
+      // zeroed only when the terms are formed, after every read is issued)
       cy[m] = coefs[store + off + ii];
       cx[m] = coefs[off + ii];
       cb[m] = coefs[2 * store + off + ii];
-      inv_x[m] = tables[InverseOffset(st, 0) + ii];
-      inv_b[m] = tables[InverseOffset(st, 2) + ii];
-      if (llf) {
-        cy[m] = 0.0f;
-        cx[m] = 0.0f;
-        cb[m] = 0.0f;
-      }
+      inv_x[m] = tables[anchor_inverse[lo][0] + ii];
+      inv_b[m] = tables[anchor_inverse[lo][1] + ii];
     }
 #pragma unroll
     for (int m = 0; m < kPerThread; ++m) {
       if (!live[m]) continue;
       const int j = t + m * 256;
+      const float y = llf[m] ? 0.0f : cy[m];
 #pragma unroll
       for (int problem = 0; problem < 2; ++problem) {
         const float inv = problem == 0 ? inv_x[m] : inv_b[m];
-        const float cc = problem == 0 ? cx[m] : cb[m];
-        const float luma = cy[m] * inv * qs_e[m];
+        const float cc = llf[m] ? 0.0f : problem == 0 ? cx[m] : cb[m];
+        const float luma = y * inv * qs_e[m];
         const float chroma = cc * inv * qs_e[m];
         terms[problem][0][j] = luma / 84.0f;
         terms[problem][1][j] = (problem == 0 ? 0.0f : 1.0f) * luma - chroma;
@@ -843,28 +843,30 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
   const QuantizerState q = *qstate;
   // every global read that depends only on the slots, issued together: the
   // coefficients, the inverse-quantisation entries of the same elements and
-  // the raw quant of each transform
+  // the raw quant of each transform.  The reads are unconditional (an unused
+  // slot reads element 0 and discards it): a read under a branch makes the
+  // compiler wait for it, and for every earlier read, at the branch.
   constexpr int kPer = (K * 3 * N + kAqThreads - 1) / kAqThreads;
   float cv[kPer], invv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = threadIdx.x + j * kAqThreads;
-    cv[j] = invv[j] = 0.0f;
-    if (e < K * 3 * N) {
-      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
-      const TransformSlot& t = slots[k];
-      if (t.ok) cv[j] = coefs[c * store + t.offset + i];
-      invv[j] = tables[InverseOffset(kStrategy, c) + i];
-    }
+    const bool in_range = e < K * 3 * N;
+    const int ec = in_range ? e : 0;
+    const int k = ec / (3 * N), c = (ec / N) % 3, i = ec % N;
+    const TransformSlot& t = slots[k];
+    cv[j] = coefs[in_range && t.ok ? c * store + t.offset + i : 0];
+    invv[j] = tables[InverseOffset(kStrategy, c) + i];
   }
-  if (threadIdx.x < K) {
-    const TransformSlot& t = slots[threadIdx.x];
-    raw_k[threadIdx.x] = t.ok ? raw[static_cast<size_t>(t.by) * bw + t.bx] : 0;
+  {
+    const TransformSlot& t = slots[threadIdx.x < K ? threadIdx.x : 0];
+    const int32_t r = raw[t.ok ? static_cast<size_t>(t.by) * bw + t.bx : 0];
+    if (threadIdx.x < K) raw_k[threadIdx.x] = t.ok ? r : 0;  // after the other reads
   }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = threadIdx.x + j * kAqThreads;
-    if (e < K * 3 * N) (&coef[0][0][0])[e] = cv[j];
+    if (e < K * 3 * N) (&coef[0][0][0])[e] = slots[e / (3 * N)].ok ? cv[j] : 0.0f;
   }
   __syncthreads();
   // terms of the AdjustQuantForChannel scans
@@ -1411,48 +1413,41 @@ __global__ __launch_bounds__(kAqThreads) void ReconstructKernel(
   const QuantizerState q = *qstate;
   // every global read that depends only on the slots, issued together:
   // quantized coefficients and dequantisation entries, the raw quant and
-  // CfL multipliers, the smoothed DC samples
+  // CfL multipliers, the smoothed DC samples.  The reads are unconditional
+  // (an unused slot reads element 0 and discards it): a read under a branch
+  // makes the compiler wait for it, and for every earlier read, at the
+  // branch.
   constexpr int kPer = (K * 3 * N + kAqThreads - 1) / kAqThreads;
   int32_t qv[kPer];
   float dqv[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e = threadIdx.x + j * kAqThreads;
-    qv[j] = 0;
-    dqv[j] = 0.0f;
-    if (e < K * 3 * N) {
-      const int k = e / (3 * N), c = (e / N) % 3, i = e % N;
-      const TransformSlot& t = slots[k];
-      if (t.ok) {
-        qv[j] = quantized[c * store + t.offset + i];
-        dqv[j] = tables[DequantOffset(kStrategy, c) + i];
-      }
-    }
+    const bool in_range = e < K * 3 * N;
+    const int ec = in_range ? e : 0;
+    const int k = ec / (3 * N), c = (ec / N) % 3, i = ec % N;
+    const TransformSlot& t = slots[k];
+    qv[j] = quantized[in_range && t.ok ? c * store + t.offset + i : 0];
+    dqv[j] = tables[DequantOffset(kStrategy, c) + i];
   }
-  int32_t r = 1;
-  int8_t yx = 0, yb = 0;
-  if (threadIdx.x < K) {
-    const TransformSlot& t = slots[threadIdx.x];
-    if (t.ok) {
-      r = raw[static_cast<size_t>(t.by) * bw + t.bx];
-      const int tile = (t.by / 8) * ((bw + 7) / 8) + t.bx / 8;
-      yx = y_to_x[tile];
-      yb = y_to_b[tile];
-    }
-  }
+  // thread k < K: slot k's raw quant and CfL multipliers
+  const TransformSlot& tk = slots[threadIdx.x < K ? threadIdx.x : 0];
+  const size_t raw_at = tk.ok ? static_cast<size_t>(tk.by) * bw + tk.bx : 0;
+  const int tile = tk.ok ? (tk.by / 8) * ((bw + 7) / 8) + tk.bx / 8 : 0;
+  const int32_t r = raw[raw_at];
+  const int8_t yx = y_to_x[tile], yb = y_to_b[tile];
+  // thread 3k + c < 3K: the DC samples under slot k, channel c
   float dcv[16];
   if (threadIdx.x < 3 * K) {
     const int k = threadIdx.x / 3, c = threadIdx.x % 3;
     const TransformSlot& t = slots[k];
-    if (t.ok) {
-      const size_t n = static_cast<size_t>(bw) * bh;
+    const size_t n = static_cast<size_t>(bw) * bh;
+    const size_t base = t.ok ? c * n + static_cast<size_t>(t.by) * bw + t.bx : 0;
 #pragma unroll
-      for (int dy = 0; dy < S::kCh; ++dy)
+    for (int dy = 0; dy < S::kCh; ++dy)
 #pragma unroll
-        for (int dx = 0; dx < S::kCw; ++dx)
-          dcv[dy * S::kCw + dx] =
-            smoothed_dc[c * n + static_cast<size_t>(t.by + dy) * bw + t.bx + dx];
-    }
+      for (int dx = 0; dx < S::kCw; ++dx)
+        dcv[dy * S::kCw + dx] = smoothed_dc[t.ok ? base + static_cast<size_t>(dy) * bw + dx : 0];
   }
   if (threadIdx.x < K) {
     const int k = threadIdx.x;

@@ -209,9 +209,15 @@ __global__ __launch_bounds__(kThreads) void BlurRowsKernel(
 }
 
 // Columns: each thread produces kColOutputs outputs down one column from
-// one register window (coalesced across the wave).  `epilogue(i, v)` takes
-// the blurred value v of pixel i: a store, or the frequency split that
-// consumes the blur, so that the blurred plane is never written.
+// one register window (coalesced across the wave).  The epilogue takes the
+// blurred value v of pixel i: a store, or the frequency split that consumes
+// the blur, so that the blurred plane is never written.  `Load(i, src, in)`
+// reads the kLoads values the split of pixel i needs (src: the blur's
+// input plane) and `Store(i, v, in)`
+// writes its results; a thread reads its window and every output's split
+// inputs before it stores anything (its stores may alias its later reads
+// as far as the compiler knows, which would otherwise make each output
+// wait for its own reads).
 constexpr int kColOutputs = 8;
 
 template <int R, typename Epilogue>
@@ -231,17 +237,24 @@ __global__ __launch_bounds__(kThreads) void BlurColsKernel(
     for (int j = 0; j < kOutputs + 2 * R; ++j) {
       window[j] = in[static_cast<size_t>(y0 - R + j) * w + x];
     }
+    float split[kOutputs][Epilogue::kLoads];
+#pragma unroll
+    for (int p = 0; p < kOutputs; ++p) epilogue.Load(static_cast<size_t>(y0 + p) * w + x, in, split[p]);
     const float* sk = taps + 33;
 #pragma unroll
     for (int p = 0; p < kOutputs; ++p) {
-      epilogue(static_cast<size_t>(y0 + p) * w + x, ConvolveInterior<R>(window + p, sk));
+      epilogue.Store(static_cast<size_t>(y0 + p) * w + x, ConvolveInterior<R>(window + p, sk),
+                     split[p]);
     }
     return;
   }
   for (int p = 0; p < kOutputs; ++p) {
     const int y = y0 + p;
     if (y >= h) break;
-    epilogue(static_cast<size_t>(y) * w + x, ConvolveLine<R>(in + x, w, y, h, taps));
+    const size_t i = static_cast<size_t>(y) * w + x;
+    float split[Epilogue::kLoads];
+    epilogue.Load(i, in, split);
+    epilogue.Store(i, ConvolveLine<R>(in + x, w, y, h, taps), split);
   }
 }
 
@@ -385,29 +398,44 @@ __device__ __forceinline__ float AmplifyRange(float v, float width) {
 
 // Column-pass epilogues of the separable blurs (SeparateFrequencies).
 
+// Each split reads kLoads values per pixel with unconditional reads: a
+// value a channel does not use is read from the blur's own input at the
+// same pixel, which the thread's window has just read (no branch around a
+// read, no extra memory traffic).
 struct StoreBlur {
+  static constexpr int kLoads = 1;
   float* out;
-  __device__ __forceinline__ void operator()(size_t i, float v) const { out[i] = v; }
+  __device__ __forceinline__ void Load(size_t, const float*, float (&in)[kLoads]) const {
+    in[0] = 0.0f;
+  }
+  __device__ __forceinline__ void Store(size_t i, float v, const float (&)[kLoads]) const {
+    out[i] = v;
+  }
 };
 
 // The LF blur v of channel c: MF = XYB - LF, then the LF plane scaled.  The
 // channels go Y, X, B: Y's LF stays raw until B's pass, which needs it and
 // then scales it.
 struct LowSplit {
+  static constexpr int kLoads = 2;
   int c;
   const float* xyb;
   float* psycho;
   size_t n;
-  __device__ __forceinline__ void operator()(size_t i, float v) const {
+  __device__ __forceinline__ void Load(size_t i, const float* blur_in, float (&in)[kLoads]) const {
+    in[0] = xyb[c * n + i];
+    in[1] = (c == 2 ? psycho + n : blur_in)[i];  // raw LF Y for channel 2
+  }
+  __device__ __forceinline__ void Store(size_t i, float v, const float (&in)[kLoads]) const {
     float* lf = psycho;
     float* mf = psycho + 3 * n;
-    mf[c * n + i] = xyb[c * n + i] - v;
+    mf[c * n + i] = in[0] - v;
     if (c == 0) {
       lf[i] = v * 33.832837186260f;
     } else if (c == 1) {
       lf[n + i] = v;
     } else {
-      const float ly = lf[n + i];
+      const float ly = in[1];
       lf[2 * n + i] = (-0.362267051518f * ly + v) * 49.87984651440f;
       lf[n + i] = ly * 14.458268100570f;
     }
@@ -417,36 +445,45 @@ struct LowSplit {
 // The 3.22 blur v of MF channel c: HF and MF of channel c; channel 1 also
 // applies SuppressXByY to HF X.
 struct HighSplit {
+  static constexpr int kLoads = 2;
   int c;
   float* psycho;
   size_t n;
-  __device__ __forceinline__ void operator()(size_t i, float v) const {
+  __device__ __forceinline__ void Load(size_t i, const float* blur_in, float (&in)[kLoads]) const {
+    in[0] = (c == 2 ? blur_in : psycho + (3 + c) * n)[i];  // MF of channels 0, 1
+    in[1] = (c == 1 ? psycho + 6 * n : blur_in)[i];        // HF X for channel 1
+  }
+  __device__ __forceinline__ void Store(size_t i, float v, const float (&in)[kLoads]) const {
     float* mf = psycho + 3 * n;
     float* hf = psycho + 6 * n;
     if (c == 2) {
       mf[2 * n + i] = v;
       return;
     }
-    const float high = mf[c * n + i] - v;
+    const float high = in[0] - v;
     hf[c * n + i] = high;
     mf[c * n + i] = c == 0 ? RemoveRange(v, 0.29f) : AmplifyRange(v, 0.1f);
     if (c == 1) {
       constexpr float kS = 46.0f, kMin = 0.653020556257f;
       const float scaler = (kS / (high * high + kS)) * (1.0f - kMin) + kMin;
-      hf[i] = scaler * hf[i];
+      hf[i] = scaler * in[1];
     }
   }
 };
 
 // The 1.56 blur v of HF channel c: UHF and HF of channel c.
 struct UltraSplit {
+  static constexpr int kLoads = 1;
   int c;
   float* psycho;
   size_t n;
-  __device__ __forceinline__ void operator()(size_t i, float v) const {
+  __device__ __forceinline__ void Load(size_t i, const float*, float (&in)[kLoads]) const {
+    in[0] = psycho[(6 + c) * n + i];
+  }
+  __device__ __forceinline__ void Store(size_t i, float v, const float (&in)[kLoads]) const {
     float* hf = psycho + (6 + c) * n;
     float* uhf = psycho + (8 + c) * n;
-    const float original = hf[i];
+    const float original = in[0];
     float hv = v;
     if (c == 0) {
       uhf[i] = RemoveRange(original - hv, 0.04f);
