@@ -501,35 +501,50 @@ struct UltraSplit {
 // ---- difference stages -------------------------------------------------------
 
 struct MaltaScale {
-  float gt;    // norm2_0_gt_1
-  float lt;    // norm2_0_lt_1
+  double gt;   // norm2_0_gt_1, a binary32 value
+  double lt;   // norm2_0_lt_1, a binary32 value
   float norm;  // (float)norm
 };
 
 // The scaled difference MaltaDiffMap feeds its lines (butteraugli_distance_
-// internal.cpp:512-555).  Its four corrections s -/+ ds2 * (...) are one
-// s + ds2 * -(dv1 + k) with k = +-too_small or +-too_big picked by selects:
-// x - y is x + (-y) and -(a - b) is b - a exactly, so the result is the
-// reference's.
+// internal.cpp:512-555).
+//
+// The two binary32 quotients gt / den and lt / den come from one binary64
+// reciprocal: r = 1 / den to within 2^-52 relative (hardware estimate and two
+// Newton steps), then the binary64 product rounded to binary32.  A quotient
+// of two binary32 values is never a binary32 rounding midpoint and lies at
+// least 2^-49 (relative) from every midpoint, so a binary64 value within
+// 2^-51 of it rounds to the same binary32 value: the correctly rounded
+// quotient, as binary32 division gives.
+//
+// The four corrections s -/+ ds2 * (...) are one s +/- ds2 * (u - k) with
+// u = |v1| mirrored by the sign of v0 and k = too_small or too_big picked by
+// selects: negation is exact and rounding is symmetric, so -(x + y) is
+// (-x) + (-y) and x - y is x + (-y), and the result is the reference's.
 __device__ __forceinline__ float MaltaScaledDiff(float v0, float v1, const MaltaScale& m) {
   const float absolute = 0.5f * (fabsf(v0) + fabsf(v1));
   const float diff = v0 - v1;
-  const float scaler = m.gt / (m.norm + absolute);
+  const double den = m.norm + absolute;
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(fma(-den, r, 1.0), r, r);
+  r = fma(fma(-den, r, 1.0), r, r);
+  const float scaler = static_cast<float>(m.gt * r);
   const float s = scaler * diff;
-  const float scaler2 = m.lt / (m.norm + absolute);
+  const double ds2 = static_cast<float>(m.lt * r);
   const double magnitude = fabs(static_cast<double>(v0));
   const double too_small = 0.55 * magnitude, too_big = 1.05 * magnitude;
-  const double dv1 = v1, ds2 = scaler2;
   // v0 < 0: dv1 > -too_small: s - ds2 * (dv1 + too_small)
   //         dv1 < -too_big:   s + ds2 * (-dv1 - too_big)
   // v0 >= 0: dv1 < too_small: s + ds2 * (too_small - dv1)
   //          dv1 > too_big:   s - ds2 * (dv1 - too_big)
   const bool negative = v0 < 0.0f;
-  const bool near = negative ? dv1 > -too_small : dv1 < too_small;
-  const bool far = negative ? dv1 < -too_big : dv1 > too_big;
+  const double u = negative ? -v1 : v1;
+  const bool near = u < too_small;
+  const bool far = u > too_big;
   const double k = near ? too_small : too_big;
-  const double t = -(dv1 + (negative ? k : -k));
-  const float corrected = static_cast<float>(static_cast<double>(s) + ds2 * t);
+  const double w = ds2 * (u - k);
+  const double ds = s;
+  const float corrected = static_cast<float>(negative ? ds + w : ds - w);
   return near || far ? corrected : s;
 }
 
@@ -653,17 +668,23 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
   const size_t n = static_cast<size_t>(w) * h;
   const int tid = threadIdx.x;
   constexpr int kCount = kMaltaLh * kMaltaLw;
+  // byte offsets of this thread's tile elements within a plane (clamped to
+  // the image; a plane is under 4 GiB), the same for every stage
+  uint32_t offset[kMaltaLoads];
+#pragma unroll
+  for (int j = 0; j < kMaltaLoads; ++j) {
+    const int e = min(tid + j * kThreads, kCount - 1);
+    const int r = e / kMaltaLw, col = e % kMaltaLw;
+    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+    offset[j] = 4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w + min(max(sx, 0), w - 1));
+  }
   auto load = [&](int k, float (&v0)[kMaltaLoads], float (&v1)[kMaltaLoads]) {
-    const float* p0 = ps0 + kMaltaPlanes[k] * n;
-    const float* p1 = ps1 + kMaltaPlanes[k] * n;
+    const char* p0 = reinterpret_cast<const char*>(ps0 + kMaltaPlanes[k] * n);
+    const char* p1 = reinterpret_cast<const char*>(ps1 + kMaltaPlanes[k] * n);
 #pragma unroll
     for (int j = 0; j < kMaltaLoads; ++j) {
-      const int e = min(tid + j * kThreads, kCount - 1);
-      const int r = e / kMaltaLw, col = e % kMaltaLw;
-      const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-      const size_t i = static_cast<size_t>(min(max(sy, 0), h - 1)) * w + min(max(sx, 0), w - 1);
-      v0[j] = p0[i];
-      v1[j] = p1[i];
+      v0[j] = *reinterpret_cast<const float*>(p0 + offset[j]);
+      v1[j] = *reinterpret_cast<const float*>(p1 + offset[j]);
     }
   };
   auto store = [&](int k, const float (&v0)[kMaltaLoads], const float (&v1)[kMaltaLoads]) {
@@ -1148,7 +1169,7 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
     const double pre_lt = multiplier * std::sqrt(0.33f * down[k]) / (3.75 * 2.0 + 1.0);
     ms.m[k] = MaltaScale{static_cast<float>(pre_gt * kMaltaNorms[k]),
                          static_cast<float>(pre_lt * kMaltaNorms[k]),
-                         static_cast<float>(kMaltaNorms[k])};
+                         static_cast<float>(kMaltaNorms[k])};  // gt, lt: binary32 values
   }
   L2Weights lw{};
   for (int c = 0; c < 2; ++c) {
@@ -1160,6 +1181,10 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
   for (int c = 0; c < 3; ++c) {
     lw.mf[c] = kL2[3 + c];
     lw.lf[c] = kL2[6 + c];
+  }
+  // the difference kernel addresses a plane with 32-bit byte offsets
+  if (static_cast<uint64_t>(w) * h >= (uint64_t{1} << 30)) {
+    return Fail(GJXL_ERROR_UNSUPPORTED, "Butteraugli differences: plane of 2^30 or more pixels");
   }
   GJXL_AMD_TRY(MaskActivity(b, s, s.dist_psycho, s.tmp[2]));
   const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
