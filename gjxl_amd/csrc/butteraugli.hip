@@ -646,23 +646,25 @@ struct MaltaScales {
 };
 
 // The difference stages of one scale, fused per 64x32 tile:
-// * MaltaDiffMap of all six stages: the scaled differences of the tile and
-//   its 4-pixel halo for each stage go to LDS (zero outside the image, as the
-//   reference's ReadWithZeroBorder), the reads of one stage issued before
-//   the previous stage's differences; each thread then evaluates four
-//   adjacent outputs of a row, for two rows, from 9 x 12 register windows of
-//   16-byte LDS reads, and stores stages 0-5;
-// * from those values in registers, the Malta accumulation in the
-//   reference's order (UHF, HF, MF), the asymmetric HF and symmetric MF L2
-//   terms, the DC differences from LF, the Y blurred-activity term, masking
-//   and the final map (MaskPsychoImage / CombineChannelsToDiffmap): stages 7
-//   and 8.
+// * MaltaDiffMap of the six stages, three at a time: the scaled differences
+//   of the tile and its 4-pixel halo for each of three stages go to LDS
+//   (zero outside the image, as the reference's ReadWithZeroBorder), the
+//   reads of one stage issued before the previous stage's differences; each
+//   thread then evaluates four adjacent outputs of a row, for two rows, from
+//   9 x 12 register windows of 16-byte LDS reads, and stores the stage;
+// * after the last three stages, from their values in registers and the
+//   first three read back (the same thread stored them), the Malta
+//   accumulation in the reference's order (UHF, HF, MF), the asymmetric HF
+//   and symmetric MF L2 terms, the DC differences from LF, the Y
+//   blurred-activity term, masking and the final map (MaskPsychoImage /
+//   CombineChannelsToDiffmap): stages 7 and 8.
+// Three stages of LDS (34.6 KB) let four work groups share a CU.
 __global__ __launch_bounds__(kThreads) void DifferencesKernel(
   const float* __restrict__ ps0, const float* __restrict__ ps1, int w, int h,
   MaltaScales ms, L2Weights lw, const float* __restrict__ ref_blurred,
   const float* __restrict__ dist_blurred, const float* __restrict__ mask_y,
   const float* __restrict__ mask_dc, float x_multiplier, float* __restrict__ stages) {
-  __shared__ __align__(16) float tile[6][kMaltaLh][kMaltaLw];
+  __shared__ __align__(16) float tile[3][kMaltaLh][kMaltaLw];
   const TileIndex tile_index = XcdTile();
   const int x0 = tile_index.x * kMaltaTileW, y0 = tile_index.y * kMaltaTileH;
   const size_t n = static_cast<size_t>(w) * h;
@@ -670,15 +672,23 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
   constexpr int kCount = kMaltaLh * kMaltaLw;
   // byte offsets of this thread's tile elements within a plane (clamped to
   // the image; a plane is under 4 GiB), the same for every stage
-  uint32_t offset[kMaltaLoads];
+  // The index arithmetic of a formation is recomputed each time from an
+  // opaque copy of the thread index: shared with the first formation, its
+  // results would stay live through the first Malta phase and cost the
+  // registers of a fourth wave per SIMD.
+  auto offsets = [&](uint32_t (&offset)[kMaltaLoads]) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
 #pragma unroll
-  for (int j = 0; j < kMaltaLoads; ++j) {
-    const int e = min(tid + j * kThreads, kCount - 1);
-    const int r = e / kMaltaLw, col = e % kMaltaLw;
-    const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-    offset[j] = 4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w + min(max(sx, 0), w - 1));
-  }
-  auto load = [&](int k, float (&v0)[kMaltaLoads], float (&v1)[kMaltaLoads]) {
+    for (int j = 0; j < kMaltaLoads; ++j) {
+      const int e = min(t + j * kThreads, kCount - 1);
+      const int r = e / kMaltaLw, col = e % kMaltaLw;
+      const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
+      offset[j] = 4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w + min(max(sx, 0), w - 1));
+    }
+  };
+  auto load = [&](int k, const uint32_t (&offset)[kMaltaLoads], float (&v0)[kMaltaLoads],
+                  float (&v1)[kMaltaLoads]) {
     const char* p0 = reinterpret_cast<const char*>(ps0 + kMaltaPlanes[k] * n);
     const char* p1 = reinterpret_cast<const char*>(ps1 + kMaltaPlanes[k] * n);
 #pragma unroll
@@ -688,60 +698,113 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
     }
   };
   auto store = [&](int k, const float (&v0)[kMaltaLoads], const float (&v1)[kMaltaLoads]) {
+    int t = tid;
+    asm volatile("" : "+v"(t));
 #pragma unroll
     for (int j = 0; j < kMaltaLoads; ++j) {
-      const int e = tid + j * kThreads;
+      const int e = t + j * kThreads;
       if (e < kCount) {
         const int r = e / kMaltaLw, col = e % kMaltaLw;
         const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
         const bool inside = sx >= 0 && sy >= 0 && sx < w && sy < h;
-        tile[k][r][col] = inside ? MaltaScaledDiff(v0[j], v1[j], ms.m[k]) : 0.0f;
+        tile[k % 3][r][col] = inside ? MaltaScaledDiff(v0[j], v1[j], ms.m[k]) : 0.0f;
       }
     }
   };
-  {
+  // stages first .. first + 2 into LDS
+  auto form = [&](int first) {
+    uint32_t offset[kMaltaLoads];
+    offsets(offset);
     float a0[kMaltaLoads], a1[kMaltaLoads], b0[kMaltaLoads], b1[kMaltaLoads];
-    load(0, a0, a1);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (k % 2 == 0) {
-        if (k + 1 < 6) load(k + 1, b0, b1);
-        store(k, a0, a1);
-      } else {
-        if (k + 1 < 6) load(k + 1, a0, a1);
-        store(k, b0, b1);
-      }
-    }
-  }
-  __syncthreads();
+    load(first, offset, a0, a1);
+    load(first + 1, offset, b0, b1);
+    store(first, a0, a1);
+    load(first + 2, offset, a0, a1);
+    store(first + 1, b0, b1);
+    store(first + 2, a0, a1);
+  };
   const int q = tid % kMaltaQuads;
   const int x = x0 + 4 * q;
-  if (x >= w) return;
+  // a thread past the right edge still takes part in forming the stages
+  const bool active = x < w;
   // four consecutive pixels of a plane: one 16-byte access when rows start
   // on 16-byte boundaries (then x + 3 < w), else one access per pixel
+  // (32-bit byte offsets from the plane's base: a plane is under 4 GiB)
   const bool vector = (w & 3) == 0;
   auto load4 = [&](const float* plane, size_t i, float (&out)[4]) {
+    const char* at = reinterpret_cast<const char*>(plane) + 4u * static_cast<uint32_t>(i);
     if (vector) {
-      const float4 v = *reinterpret_cast<const float4*>(plane + i);
+      const float4 v = *reinterpret_cast<const float4*>(at);
       out[0] = v.x;
       out[1] = v.y;
       out[2] = v.z;
       out[3] = v.w;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[j] = x + j < w ? plane[i + j] : 0.0f;
-    }
-  };
-  auto store4 = [&](float* plane, size_t i, const float (&v)[4]) {
-    if (vector) {
-      *reinterpret_cast<float4*>(plane + i) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-#pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (x + j < w) plane[i + j] = v[j];
+        out[j] = x + j < w ? reinterpret_cast<const float*>(at)[j] : 0.0f;
       }
     }
   };
+  auto store4 = [&](float* plane, size_t i, const float (&v)[4]) {
+    char* at = reinterpret_cast<char*>(plane) + 4u * static_cast<uint32_t>(i);
+    if (vector) {
+      *reinterpret_cast<float4*>(at) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (x + j < w) reinterpret_cast<float*>(at)[j] = v[j];
+      }
+    }
+  };
+  // Malta of stage k (in LDS slot k % 3) for the four outputs of row ty
+  auto malta = [&](auto stage, int ty, float (&out)[4]) {
+    constexpr int k = decltype(stage)::value;
+    // win[r][c] = tile[k % 3][ty + r][4q + c]: rows y - 4 .. y + 4, columns
+    // x - 4 .. x + 7
+    float win[kMaltaWinH][kMaltaWinW];
+#pragma unroll
+    for (int r = 0; r < kMaltaWinH; ++r) {
+      const float4* s4 = reinterpret_cast<const float4*>(&tile[k % 3][ty + r][4 * q]);
+#pragma unroll
+      for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
+        const float4 v = s4[c4];
+        win[r][4 * c4] = v.x;
+        win[r][4 * c4 + 1] = v.y;
+        win[r][4 * c4 + 2] = v.z;
+        win[r][4 * c4 + 3] = v.w;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      auto P = [&](int dx, int dy) -> float {
+        return win[kMaltaHalo + dy][j + kMaltaHalo + dx];
+      };
+      out[j] = MaltaResponse<(k < 4)>(P);
+    }
+  };
+  form(0);
+  __syncthreads();
+  if (active) {
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+      const int ty = tid / kMaltaQuads + pass * kMaltaRowsPerPass;
+      const int y = y0 + ty;
+      if (y >= h) break;
+      const size_t base = static_cast<size_t>(y) * w + x;
+      float sv[3][4];
+      malta(std::integral_constant<int, 0>{}, ty, sv[0]);
+      store4(stages, base, sv[0]);
+      malta(std::integral_constant<int, 1>{}, ty, sv[1]);
+      store4(stages + n, base, sv[1]);
+      malta(std::integral_constant<int, 2>{}, ty, sv[2]);
+      store4(stages + 2 * n, base, sv[2]);
+    }
+  }
+  __syncthreads();
+  form(3);
+  __syncthreads();
+  if (!active) return;
 #pragma unroll 1
   for (int pass = 0; pass < 2; ++pass) {
     const int ty = tid / kMaltaQuads + pass * kMaltaRowsPerPass;
@@ -749,32 +812,15 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
     if (y >= h) return;
     const size_t base = static_cast<size_t>(y) * w + x;
     float sv[6][4];
+    malta(std::integral_constant<int, 3>{}, ty, sv[3]);
+    store4(stages + 3 * n, base, sv[3]);
+    malta(std::integral_constant<int, 4>{}, ty, sv[4]);
+    store4(stages + 4 * n, base, sv[4]);
+    malta(std::integral_constant<int, 5>{}, ty, sv[5]);
+    store4(stages + 5 * n, base, sv[5]);
+    // stages 0-2 as this thread stored them
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      // win[r][c] = tile[k][ty + r][4q + c]: rows y - 4 .. y + 4, columns
-      // x - 4 .. x + 7
-      float win[kMaltaWinH][kMaltaWinW];
-#pragma unroll
-      for (int r = 0; r < kMaltaWinH; ++r) {
-        const float4* s4 = reinterpret_cast<const float4*>(&tile[k][ty + r][4 * q]);
-#pragma unroll
-        for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
-          const float4 v = s4[c4];
-          win[r][4 * c4] = v.x;
-          win[r][4 * c4 + 1] = v.y;
-          win[r][4 * c4 + 2] = v.z;
-          win[r][4 * c4 + 3] = v.w;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        auto P = [&](int dx, int dy) -> float {
-          return win[kMaltaHalo + dy][j + kMaltaHalo + dx];
-        };
-        sv[k][j] = k < 4 ? MaltaResponse<true>(P) : MaltaResponse<false>(P);
-      }
-      store4(stages + k * n, base, sv[k]);
-    }
+    for (int k = 0; k < 3; ++k) load4(stages + k * n, base, sv[k]);
     // the per-pixel inputs of the L2 terms and the final map
     float p0[8][4], p1[8][4];  // planes LF 0-2, MF 3-5, HF 6-7
 #pragma unroll
