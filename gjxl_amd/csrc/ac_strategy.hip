@@ -385,6 +385,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   // makes the compiler wait for the earlier reads at the branch's join.)
   constexpr int kPer = KC * N / kCostThreads;
   constexpr int kMaskPer = K * N / kCostThreads;
+  static_assert(N % 64 == 0, "a wave's elements lie in one candidate channel");
   static_assert(KC * N == kPer * kCostThreads && K * N == kMaskPer * kCostThreads,
                 "whole elements per thread");
   float pixel[kPer], dq_e[kPer], inv_e[kPer];
@@ -543,7 +544,10 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       const float diff = scaled - rounded;
       residual[j] = dq_e[j] * diff;
       A[kc * N + i] = sqrtf(fabsf(rounded));
-      if (rounded != 0.0f) atomicAdd(&nnz[k][c], 1);
+      // a wave's 64 elements lie in one candidate channel (N is a multiple
+      // of 64): one LDS add of the wave's nonzero count
+      const int count = __popcll(__ballot(rounded != 0.0f));
+      if (lane == 0 && count != 0) atomicAdd(&nnz[k][c], count);
     }
   }
   __syncthreads();

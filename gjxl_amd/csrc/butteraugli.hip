@@ -902,19 +902,18 @@ __global__ __launch_bounds__(kThreads) void MaskSourceKernel(
   out[i] = sqrtf(kMul * fabsf(activity) + bias) - sqrt_bias;
 }
 
+// StoreMin3 of the reference as selects: the same comparisons decide the
+// same moves, also when m0 <= m1 <= m2 does not hold.
 __device__ __forceinline__ void StoreMin3(float v, float& m0, float& m1, float& m2) {
-  if (v < m2) {
-    if (v < m0) {
-      m2 = m1;
-      m1 = m0;
-      m0 = v;
-    } else if (v < m1) {
-      m2 = m1;
-      m1 = v;
-    } else {
-      m2 = v;
-    }
-  }
+  const bool below2 = v < m2, below0 = v < m0, below1 = v < m1;
+  // if (v < m2) { if (v < m0) {m2 = m1; m1 = m0; m0 = v;}
+  //               else if (v < m1) {m2 = m1; m1 = v;} else {m2 = v;} }
+  const float n2 = below0 || below1 ? m1 : v;
+  const float n1 = below0 ? m0 : (below1 ? v : m1);
+  const float n0 = below0 ? v : m0;
+  m2 = below2 ? n2 : m2;
+  m1 = below2 ? n1 : m1;
+  m0 = below2 ? n0 : m0;
 }
 
 // FuzzyErosion of the reference's blurred activity, and MaskY / MaskDcY of
@@ -935,20 +934,19 @@ __global__ __launch_bounds__(kThreads) void MaskKernel(
   const float r0 = at(xr, y), ru = at(xr, yu), rd = at(xr, yd);
   const float u0 = at(x, yu), d0 = at(x, yd);
   const bool left = x >= s, right = x + s < w, up = y >= s, down = y + s < h;
+  // a neighbour outside the image is +inf, which no StoreMin3 comparison
+  // lets in (v < m2 is false for every m2), as the reference skips it
+  constexpr float kSkip = __builtin_inff();
   float m0 = c0;
   float m1 = 2.0f * m0, m2 = m1;
-  if (left) {
-    StoreMin3(l0, m0, m1, m2);
-    if (up) StoreMin3(lu, m0, m1, m2);
-    if (down) StoreMin3(ld, m0, m1, m2);
-  }
-  if (right) {
-    StoreMin3(r0, m0, m1, m2);
-    if (up) StoreMin3(ru, m0, m1, m2);
-    if (down) StoreMin3(rd, m0, m1, m2);
-  }
-  if (up) StoreMin3(u0, m0, m1, m2);
-  if (down) StoreMin3(d0, m0, m1, m2);
+  StoreMin3(left ? l0 : kSkip, m0, m1, m2);
+  StoreMin3(left && up ? lu : kSkip, m0, m1, m2);
+  StoreMin3(left && down ? ld : kSkip, m0, m1, m2);
+  StoreMin3(right ? r0 : kSkip, m0, m1, m2);
+  StoreMin3(right && up ? ru : kSkip, m0, m1, m2);
+  StoreMin3(right && down ? rd : kSkip, m0, m1, m2);
+  StoreMin3(up ? u0 : kSkip, m0, m1, m2);
+  StoreMin3(down ? d0 : kSkip, m0, m1, m2);
   const float m = 0.45f * m0 + 0.3f * m1 + 0.25f * m2;
   const size_t i = static_cast<size_t>(y) * w + x;
   mask[i] = m;
