@@ -304,26 +304,42 @@ __global__ __launch_bounds__(kThreads) void OpsinDynamicsKernel(
   const float* __restrict__ rgb, int w, int h, float intensity,
   const float* __restrict__ taps, float* __restrict__ xyb) {
   constexpr int TW = kOpsinTileW + 4, TH = kOpsinTileH + 4;
-  constexpr int kCount = 3 * TH * TW;
-  constexpr int kLoads = (kCount + kThreads - 1) / kThreads;
   __shared__ float tile[3][TH][TW];
   const TileIndex tile_index = XcdTile();
   const int x0 = tile_index.x * kOpsinTileW, y0 = tile_index.y * kOpsinTileH;
   const size_t n = static_cast<size_t>(w) * h;
   auto mirror = [](int c, int size) { return kOnce ? BMirrorOnce(c, size) : BMirror(c, size); };
-  // a fixed trip count: every load of the tile issues before the first store
-  float loaded[kLoads];
+  // The tile's 3 x TH rows of TW = 64 + 4 columns: each wave reads kWaveRows
+  // whole rows, one lane per column of the first 64, with the row address
+  // uniform (scalar arithmetic); the last 4 columns of every row are one
+  // read per thread.  Every read issues before the first store.
+  static_assert(TW == 64 + 4 && (3 * TH) % (kThreads / 64) == 0 && 3 * TH * 4 <= kThreads,
+                "tile rows split over the waves");
+  constexpr int kWaveRows = 3 * TH / (kThreads / 64);
+  constexpr int kStripCount = 3 * TH * 4;
+  const int lane = threadIdx.x % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
+  const int sx = mirror(x0 - 2 + lane, w);
+  float rows[kWaveRows];
 #pragma unroll
-  for (int k = 0; k < kLoads; ++k) {
-    const int e = min(static_cast<int>(threadIdx.x) + k * kThreads, kCount - 1);
-    const int c = e / (TH * TW), r = (e / TW) % TH, col = e % TW;
-    const int sy = mirror(y0 - 2 + r, h), sx = mirror(x0 - 2 + col, w);
-    loaded[k] = rgb[c * n + static_cast<size_t>(sy) * w + sx];
+  for (int i = 0; i < kWaveRows; ++i) {
+    const int row = wave * kWaveRows + i;
+    const int c = row / TH, r = row % TH;
+    rows[i] = rgb[c * n + static_cast<size_t>(mirror(y0 - 2 + r, h)) * w + sx];
+  }
+  float strip;
+  {
+    const int e = min(static_cast<int>(threadIdx.x), kStripCount - 1);
+    const int row = e / 4, c = row / TH, r = row % TH;
+    strip = rgb[c * n + static_cast<size_t>(mirror(y0 - 2 + r, h)) * w +
+                mirror(x0 - 2 + 64 + e % 4, w)];
   }
 #pragma unroll
-  for (int k = 0; k < kLoads; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if (e < kCount) (&tile[0][0][0])[e] = loaded[k];
+  for (int i = 0; i < kWaveRows; ++i) {
+    (&tile[0][0][0])[(wave * kWaveRows + i) * TW + lane] = rows[i];
+  }
+  if (threadIdx.x < kStripCount) {
+    (&tile[0][0][0])[(threadIdx.x / 4) * TW + 64 + threadIdx.x % 4] = strip;
   }
   __syncthreads();
   const float w0 = taps[33 + 2], w1 = taps[33 + 1], w2 = taps[33 + 0];
@@ -967,13 +983,18 @@ __global__ __launch_bounds__(kThreads) void SubsampleKernel(
   if (x >= sw) return;
   const float* p = in + static_cast<size_t>(c) * w * h;
   const int x0 = 2 * x, y0 = 2 * y;
-  float v = 0.0f;
-  v += 0.25f * p[static_cast<size_t>(y0) * w + x0];
-  if (x0 + 1 < w) v += 0.25f * p[static_cast<size_t>(y0) * w + x0 + 1];
-  if (y0 + 1 < h) {
-    v += 0.25f * p[static_cast<size_t>(y0 + 1) * w + x0];
-    if (x0 + 1 < w) v += 0.25f * p[static_cast<size_t>(y0 + 1) * w + x0 + 1];
-  }
+  // the four reads are unconditional (clamped into the image); the sum takes
+  // the in-image ones in the reference's order
+  const bool right = x0 + 1 < w, below = y0 + 1 < h;
+  const size_t r0 = static_cast<size_t>(y0) * w, r1 = static_cast<size_t>(below ? y0 + 1 : y0) * w;
+  const int x1 = right ? x0 + 1 : x0;
+  float a = p[r0 + x0], b = p[r0 + x1], d = p[r1 + x0], e = p[r1 + x1];
+  // (the empty asm keeps the compiler from moving a read under its select)
+  asm volatile("" : "+v"(a), "+v"(b), "+v"(d), "+v"(e));
+  float v = 0.0f + 0.25f * a;
+  v = right ? v + 0.25f * b : v;
+  const float lower = v + 0.25f * d;
+  v = below ? (right ? lower + 0.25f * e : lower) : v;
   if ((w & 1) && x == sw - 1) v *= 2.0f;
   if ((h & 1) && y == sh - 1) v *= 2.0f;
   out[static_cast<size_t>(c) * sw * sh + static_cast<size_t>(y) * sw + x] = v;

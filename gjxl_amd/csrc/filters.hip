@@ -130,21 +130,41 @@ __global__ __launch_bounds__(256) void LoopFilterKernel(
   const int bw = (w + 7) / 8;
   const size_t plane = static_cast<size_t>(w) * h;
   auto mirror = [](int c, int size) { return kOnce ? MirrorOnce(c, size) : MirrorInt(c, size); };
-  // a fixed trip count: every load of the tile issues before the first store
-  constexpr int kRawCount = 3 * kRawH * kRawW;
-  constexpr int kRawLoads = (kRawCount + 255) / 256;
-  float loaded[kRawLoads];
+  // The tile's 3 x kRawH rows of kRawW = 64 + 8 columns: each wave reads
+  // kWaveRows whole rows, one lane per column of the first 64, with the row
+  // address uniform (scalar arithmetic); the last 8 columns of every row are
+  // three reads per thread.  Every read issues before the first store.
+  static_assert(kRawW == 64 + 8 && (3 * kRawH) % 4 == 0, "tile rows split over four waves");
+  constexpr int kWaveRows = 3 * kRawH / 4;
+  constexpr int kStripCount = 3 * kRawH * 8;
+  constexpr int kStripLoads = (kStripCount + 255) / 256;
+  const int lane = threadIdx.x % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / 64));
+  const int sx = mirror(x0 - 4 + lane, w);
+  float rows[kWaveRows];
 #pragma unroll
-  for (int k = 0; k < kRawLoads; ++k) {
-    const int e = min(static_cast<int>(threadIdx.x) + k * 256, kRawCount - 1);
-    const int c = e / (kRawH * kRawW), r = (e / kRawW) % kRawH, col = e % kRawW;
-    const int sy = mirror(y0 - 4 + r, h), sx = mirror(x0 - 4 + col, w);
-    loaded[k] = in[c * in_plane + static_cast<size_t>(sy) * in_stride + sx];
+  for (int i = 0; i < kWaveRows; ++i) {
+    const int row = wave * kWaveRows + i;
+    const int c = row / kRawH, r = row % kRawH;
+    const float* src = in + c * in_plane + static_cast<size_t>(mirror(y0 - 4 + r, h)) * in_stride;
+    rows[i] = src[sx];
+  }
+  float strip[kStripLoads];
+#pragma unroll
+  for (int k = 0; k < kStripLoads; ++k) {
+    const int e = min(static_cast<int>(threadIdx.x) + k * 256, kStripCount - 1);
+    const int row = e / 8, c = row / kRawH, r = row % kRawH;
+    const int sy = mirror(y0 - 4 + r, h), sxs = mirror(x0 - 4 + 64 + e % 8, w);
+    strip[k] = in[c * in_plane + static_cast<size_t>(sy) * in_stride + sxs];
   }
 #pragma unroll
-  for (int k = 0; k < kRawLoads; ++k) {
+  for (int i = 0; i < kWaveRows; ++i) {
+    raw_e1[(wave * kWaveRows + i) * kRawW + lane] = rows[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kStripLoads; ++k) {
     const int e = threadIdx.x + k * 256;
-    if (e < kRawCount) raw_e1[e] = loaded[k];
+    if (e < kStripCount) raw_e1[(e / 8) * kRawW + 64 + e % 8] = strip[k];
   }
   __syncthreads();
   // Gaborish (gaborish.cpp:101-190) at the in-image slots of its region
