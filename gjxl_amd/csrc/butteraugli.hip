@@ -615,7 +615,6 @@ constexpr int kMaltaTileW = 64, kMaltaTileH = 32, kMaltaHalo = 4;
 constexpr int kMaltaLw = kMaltaTileW + 2 * kMaltaHalo, kMaltaLh = kMaltaTileH + 2 * kMaltaHalo;
 constexpr int kMaltaQuads = kMaltaTileW / 4;
 constexpr int kMaltaWinW = 4 + 2 * kMaltaHalo, kMaltaWinH = 1 + 2 * kMaltaHalo;
-constexpr int kMaltaLoads = (kMaltaLh * kMaltaLw + kThreads - 1) / kThreads;
 constexpr int kMaltaRowsPerPass = kThreads / kMaltaQuads;
 static_assert(kMaltaTileH == 2 * kMaltaRowsPerPass, "two rows of four outputs per thread");
 static_assert(kMaltaLw % 4 == 0, "16-byte aligned LDS rows");
@@ -685,59 +684,89 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
   const int x0 = tile_index.x * kMaltaTileW, y0 = tile_index.y * kMaltaTileH;
   const size_t n = static_cast<size_t>(w) * h;
   const int tid = threadIdx.x;
-  constexpr int kCount = kMaltaLh * kMaltaLw;
-  // byte offsets of this thread's tile elements within a plane (clamped to
-  // the image; a plane is under 4 GiB), the same for every stage
+  // A stage's tile is kMaltaLh rows of kMaltaLw = 64 + 8 columns.  Each wave
+  // forms kFormRows whole rows, one lane per column of the first 64, with
+  // the row's address and its inside test uniform (scalar arithmetic); the
+  // last 8 columns of every row are kFormStrip elements per thread.  Reads
+  // are clamped to the image (32-bit byte offsets: a plane is under 4 GiB)
+  // and elements outside it are zero.
+  static_assert(kMaltaLw == 64 + 8 && kMaltaLh % (kThreads / 64) == 0, "rows split over the waves");
+  constexpr int kFormRows = kMaltaLh / (kThreads / 64);
+  constexpr int kStripCount = kMaltaLh * 8;
+  constexpr int kFormStrip = (kStripCount + kThreads - 1) / kThreads;
+  constexpr int kFormSlots = kFormRows + kFormStrip;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   // The index arithmetic of a formation is recomputed each time from an
   // opaque copy of the thread index: shared with the first formation, its
   // results would stay live through the first Malta phase and cost the
   // registers of a fourth wave per SIMD.
-  auto offsets = [&](uint32_t (&offset)[kMaltaLoads]) {
+  struct FormIndex {
+    uint32_t offset[kFormSlots];  // byte offset within a plane
+    bool inside[kFormSlots];
+    int lds[kFormStrip];          // tile index of the strip elements
+  };
+  auto index = [&](FormIndex& f) {
     int t = tid;
     asm volatile("" : "+v"(t));
+    const int lane = t % 64;
+    const int sx = x0 + lane - kMaltaHalo;
+    const uint32_t sx4 = 4u * static_cast<uint32_t>(min(max(sx, 0), w - 1));
+    const bool col_in = sx >= 0 && sx < w;
 #pragma unroll
-    for (int j = 0; j < kMaltaLoads; ++j) {
-      const int e = min(t + j * kThreads, kCount - 1);
-      const int r = e / kMaltaLw, col = e % kMaltaLw;
-      const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-      offset[j] = 4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w + min(max(sx, 0), w - 1));
+    for (int j = 0; j < kFormRows; ++j) {
+      const int sy = y0 + wave * kFormRows + j - kMaltaHalo;
+      f.offset[j] = 4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w) + sx4;
+      f.inside[j] = col_in && sy >= 0 && sy < h;
+    }
+#pragma unroll
+    for (int j = 0; j < kFormStrip; ++j) {
+      const int e = min(t + j * kThreads, kStripCount - 1);
+      const int r = e / 8, col = 64 + e % 8;
+      const int sy = y0 + r - kMaltaHalo, sxs = x0 + col - kMaltaHalo;
+      f.offset[kFormRows + j] =
+        4u * static_cast<uint32_t>(min(max(sy, 0), h - 1) * w + min(max(sxs, 0), w - 1));
+      f.inside[kFormRows + j] = sxs >= 0 && sy >= 0 && sxs < w && sy < h;
+      f.lds[j] = t + j * kThreads < kStripCount ? r * kMaltaLw + col : -1;
     }
   };
-  auto load = [&](int k, const uint32_t (&offset)[kMaltaLoads], float (&v0)[kMaltaLoads],
-                  float (&v1)[kMaltaLoads]) {
+  auto load = [&](int k, const FormIndex& f, float (&v0)[kFormSlots], float (&v1)[kFormSlots]) {
     const char* p0 = reinterpret_cast<const char*>(ps0 + kMaltaPlanes[k] * n);
     const char* p1 = reinterpret_cast<const char*>(ps1 + kMaltaPlanes[k] * n);
 #pragma unroll
-    for (int j = 0; j < kMaltaLoads; ++j) {
-      v0[j] = *reinterpret_cast<const float*>(p0 + offset[j]);
-      v1[j] = *reinterpret_cast<const float*>(p1 + offset[j]);
+    for (int j = 0; j < kFormSlots; ++j) {
+      v0[j] = *reinterpret_cast<const float*>(p0 + f.offset[j]);
+      v1[j] = *reinterpret_cast<const float*>(p1 + f.offset[j]);
     }
   };
-  auto store = [&](int k, const float (&v0)[kMaltaLoads], const float (&v1)[kMaltaLoads]) {
-    int t = tid;
-    asm volatile("" : "+v"(t));
+  auto store = [&](int k, const FormIndex& f, const float (&v0)[kFormSlots],
+                   const float (&v1)[kFormSlots]) {
+    float* stage = &tile[k % 3][0][0];
+    const int lane = tid % 64;
 #pragma unroll
-    for (int j = 0; j < kMaltaLoads; ++j) {
-      const int e = t + j * kThreads;
-      if (e < kCount) {
-        const int r = e / kMaltaLw, col = e % kMaltaLw;
-        const int sy = y0 + r - kMaltaHalo, sx = x0 + col - kMaltaHalo;
-        const bool inside = sx >= 0 && sy >= 0 && sx < w && sy < h;
-        tile[k % 3][r][col] = inside ? MaltaScaledDiff(v0[j], v1[j], ms.m[k]) : 0.0f;
+    for (int j = 0; j < kFormRows; ++j) {
+      stage[(wave * kFormRows + j) * kMaltaLw + lane] =
+        f.inside[j] ? MaltaScaledDiff(v0[j], v1[j], ms.m[k]) : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kFormStrip; ++j) {
+      if (f.lds[j] >= 0) {
+        stage[f.lds[j]] = f.inside[kFormRows + j]
+                            ? MaltaScaledDiff(v0[kFormRows + j], v1[kFormRows + j], ms.m[k])
+                            : 0.0f;
       }
     }
   };
   // stages first .. first + 2 into LDS
   auto form = [&](int first) {
-    uint32_t offset[kMaltaLoads];
-    offsets(offset);
-    float a0[kMaltaLoads], a1[kMaltaLoads], b0[kMaltaLoads], b1[kMaltaLoads];
-    load(first, offset, a0, a1);
-    load(first + 1, offset, b0, b1);
-    store(first, a0, a1);
-    load(first + 2, offset, a0, a1);
-    store(first + 1, b0, b1);
-    store(first + 2, a0, a1);
+    FormIndex f;
+    index(f);
+    float a0[kFormSlots], a1[kFormSlots], b0[kFormSlots], b1[kFormSlots];
+    load(first, f, a0, a1);
+    load(first + 1, f, b0, b1);
+    store(first, f, a0, a1);
+    load(first + 2, f, a0, a1);
+    store(first + 1, f, b0, b1);
+    store(first + 2, f, a0, a1);
   };
   const int q = tid % kMaltaQuads;
   const int x = x0 + 4 * q;
