@@ -334,11 +334,24 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
   constexpr int KC = 3 * K;     // candidate-channel planes
   constexpr int TS = W + 1;     // padded row stride of the binary64 passes
-  // A: pixels (transposed, [x][y]); then sqrt(|rounded|) in storage order
-  // B: coefficients ([v][u]); residuals (transposed, [u][v]); loss terms
+  // For 32-tall transforms, transposed planes have columns of H + 1 words
+  // (HP): the lanes that write one transposed row then hit distinct banks
+  // instead of sharing one per 32 words (measured: 32x32 12 % and 16x32 5 %
+  // faster; the shorter transforms ran 5-10 % slower padded and keep
+  // HP = H).  A plane's slot count is NP = W * HP.
+  constexpr bool kPad = H == 32;
+  constexpr int HP = kPad ? H + 1 : H, NP = W * HP;
+  // Storage order (coefficient index i = u * H + v when H >= W) is padded
+  // the same way: slot i + i / H; for H < W (i = v * W + u) it is unpadded.
+  constexpr bool kPadStorage = kPad && H >= W;
+  auto storage_slot = [](int i) { return kPadStorage ? i + i / H : i; };
+  // A: pixels (transposed, [x][y], column stride HP); then sqrt(|rounded|)
+  //    in (padded) storage order
+  // B: coefficients ([v][u]); residuals (transposed, [u][v], column stride
+  //    HP); loss terms ([y][x])
   // T: binary64 rows of the separable passes, stride TS
-  __shared__ float A[KC * N];
-  __shared__ float B[KC * N];
+  __shared__ float A[KC * NP];
+  __shared__ float B[KC * NP];
   __shared__ double T[KC * H * TS];
   __shared__ float mask[K * N];  // pixel mask of each candidate, raster order
   __shared__ int nnz[K][3];
@@ -447,7 +460,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     const int e = tid + j * kCostThreads;
     if (e < KC * N) {
       const int kc = e / N, r = e % N;
-      A[kc * N + (r % W) * H + r / W] = pixel[j];
+      A[kc * NP + (r % W) * HP + r / W] = pixel[j];
     }
   }
   __syncthreads();
@@ -464,11 +477,11 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     const int g = task % (W / kOut), r = (task / (W / kOut)) * 64 + lane;
     if (r < kRows) {
       const int kc = r / H, y = r % H;
-      const float* src = A + kc * N + y;
+      const float* src = A + kc * NP + y;
       double acc[kOut] = {};
 #pragma unroll 4
       for (int x = 0; x < W; ++x) {
-        const double v = static_cast<double>(src[x * H]);
+        const double v = static_cast<double>(src[x * HP]);
 #pragma unroll
         for (int j = 0; j < kOut; ++j) acc[j] += v * fhT[x * W + g * kOut + j];
       }
@@ -543,7 +556,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       const float rounded = roundf(scaled);
       const float diff = scaled - rounded;
       residual[j] = dq_e[j] * diff;
-      A[kc * N + i] = sqrtf(fabsf(rounded));
+      A[kc * NP + storage_slot(i)] = sqrtf(fabsf(rounded));
       // a wave's 64 elements lie in one candidate channel (N is a multiple
       // of 64): one LDS add of the wave's nonzero count
       const int count = __popcll(__ballot(rounded != 0.0f));
@@ -557,7 +570,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     if (e < KC * N) {
       const int kc = e / N, rem = e % N;
       const int v = rem / W, u = rem % W;
-      B[kc * N + u * H + v] = residual[j];
+      B[kc * NP + u * HP + v] = residual[j];
     }
   }
   __syncthreads();
@@ -566,11 +579,11 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     const int g = task % (W / kOut), r = (task / (W / kOut)) * 64 + lane;
     if (r < kRows) {
       const int kc = r / H, v = r % H;
-      const float* src = B + kc * N + v;
+      const float* src = B + kc * NP + v;
       double acc[kOut] = {};
 #pragma unroll 4
       for (int u = 0; u < W; ++u) {
-        const double t = static_cast<double>(src[u * H]);
+        const double t = static_cast<double>(src[u * HP]);
 #pragma unroll
         for (int j = 0; j < kOut; ++j) acc[j] += t * ih[u * W + g * kOut + j];
       }
@@ -615,13 +628,19 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   // Ordered sums, one thread each.
   if (tid < K * 6) {
     const int k = tid / 6, c = (tid / 2) % 3, kind = tid % 2;
-    const float* src = (kind == 0 ? A : B) + (k * 3 + c) * N;
+    // kind 0: A in (padded) storage order; kind 1: B in raster order
+    const float* src = kind == 0 ? A + (k * 3 + c) * NP : B + (k * 3 + c) * N;
+    const bool padded = kind == 0 && kPadStorage;
     float s = 0.0f;
     // batches of 16: the loads of a batch issue before its ordered adds
     for (int base = 0; base < N; base += 16) {
+      // kind 0 reads the padded slots i + i / H: a batch lies in one padded
+      // column for H >= 16 and in two for H = 8 (the second one word on)
+      const float* lo = src + base + (padded ? base / H : 0);
+      const float* hi = lo + (padded && H < 16 ? 1 : 0);
       float v[16];
 #pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = src[base + m];
+      for (int m = 0; m < 16; ++m) v[m] = (m >= 8 && H < 16 ? hi : lo)[m];
 #pragma unroll
       for (int m = 0; m < 16; ++m) s += v[m];
     }
