@@ -899,15 +899,13 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
           const float total = (d * d) * lw.asym_gt[c] + a[c];
           const float mag = fabsf(v0);
           const float small = 0.4f * mag;
-          float sec = 0.0f;
-          if (v0 < 0.0f) {
-            if (v1 > -small) sec = v1 + small;
-            else if (v1 < -mag) sec = -v1 - mag;
-          } else if (v1 < small) {
-            sec = small - v1;
-          } else if (v1 > mag) {
-            sec = v1 - mag;
-          }
+          // v0 < 0:  v1 > -small: v1 + small;  else v1 < -mag: -v1 - mag
+          // v0 >= 0: v1 < small:  small - v1;  else v1 > mag:  v1 - mag
+          // (else 0), as selects on the same comparisons
+          const float below = v0 < 0.0f ? (v1 > -small ? v1 + small : (v1 < -mag ? -v1 - mag : 0.0f))
+                                        : 0.0f;
+          const float above = v1 < small ? small - v1 : (v1 > mag ? v1 - mag : 0.0f);
+          const float sec = v0 < 0.0f ? below : above;
           a[c] = lw.asym_lt[c] * (sec * sec) + total;
         }
         if (lw.mf[c] != 0.0f) {
