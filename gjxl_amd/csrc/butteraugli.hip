@@ -1050,18 +1050,19 @@ __global__ __launch_bounds__(kThreads) void ComposeKernel(
   int sw, int w, int h, int xb, int yb, float* __restrict__ map,
   float* __restrict__ map_out, uint32_t* __restrict__ score_bits) {
   uint32_t best[1] = {0u};
-  const size_t n = static_cast<size_t>(w) * h;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int x = static_cast<int>(i % w), y = static_cast<int>(i / w);
-    const float m = main_map[static_cast<size_t>(y + yb) * mw + x + xb];
-    const float v = sub_map != nullptr
-                      ? m * 0.85f + 0.5f * sub_map[static_cast<size_t>(y / 2) * sw + x / 2]
-                      : m;
-    map[i] = v;
-    if (map_out != nullptr) map_out[i] = v;
-    const uint32_t bits = __float_as_uint(v);
-    best[0] = bits > best[0] ? bits : best[0];
+  // rows strided over the groups, columns over the threads (no division)
+  for (int y = blockIdx.x; y < h; y += gridDim.x) {
+    const float* main_row = main_map + static_cast<size_t>(y + yb) * mw + xb;
+    const float* sub_row = sub_map != nullptr ? sub_map + static_cast<size_t>(y / 2) * sw : nullptr;
+    const size_t row = static_cast<size_t>(y) * w;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+      const float m = main_row[x];
+      const float v = sub_row != nullptr ? m * 0.85f + 0.5f * sub_row[x / 2] : m;
+      map[row + x] = v;
+      if (map_out != nullptr) map_out[row + x] = v;
+      const uint32_t bits = __float_as_uint(v);
+      best[0] = bits > best[0] ? bits : best[0];
+    }
   }
   BlockReduceU32(best, U32Max());
   if (threadIdx.x == 0) atomicMax(score_bits, best[0]);
