@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "common.h"
+#include "libm_port.h"
 #include "tables.h"
 
 namespace gjxl_amd {
@@ -666,9 +667,8 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
       }
     }
     const float normalized = loss / static_cast<float>(N);
-    // x^(1/8) as three binary64 square roots, rounded once to binary32.
-    const float root8 = static_cast<float>(
-      sqrt(sqrt(sqrt(static_cast<double>(normalized)))));
+    // std::pow(float, float) as the reference's libm evaluates it
+    const float root8 = GlibcPowf(normalized, 1.0f / 8.0f);
     const float loss_cost = root8 * static_cast<float>(N) / qn[k];
     costs[static_cast<size_t>(bys[k]) * bw + bxs[k]] =
       entropy * params.entropy_multiplier + params.info_loss_multiplier * loss_cost;

@@ -29,6 +29,7 @@
 #include <new>
 
 #include "butteraugli.h"
+#include "libm_port.h"
 #include "tables.h"
 
 namespace gjxl_amd {
@@ -1074,10 +1075,8 @@ __global__ void FillWordKernel(uint32_t* p, uint32_t value) {
 
 // ---- a17 reduction ------------------------------------------------------------
 
-// x^(1/16) by four binary64 square roots, rounded once to binary32.
-__device__ __forceinline__ float Root16(float x) {
-  return static_cast<float>(sqrt(sqrt(sqrt(sqrt(static_cast<double>(x))))));
-}
+// std::pow(x, 1.0f / 16.0f) as the reference's libm evaluates it.
+__device__ __forceinline__ float Root16(float x) { return GlibcPowf(x, 1.0f / 16.0f); }
 
 __global__ __launch_bounds__(kThreads) void ReduceKernel(
   const float* __restrict__ map, int w, int h, const uint8_t* __restrict__ cells,

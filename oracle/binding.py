@@ -3,8 +3,9 @@
 ``oracle()`` loads liboracle.so (the plain-C restatement in this directory);
 ``reference()`` loads _ref/libgjxlref.so (the reference encoder's own CPU
 sources compiled in place, plus ref_harness.cpp) and returns None when it has
-not been built.  Only tests/, the smoke check and the CPU-baseline leg of
-bench.py may use this module.
+not been built; only the CPU tests call it, through the ``ref`` fixture, so
+no GPU process maps it.  Only tests/, the smoke check and the CPU-baseline
+leg of bench.py may use this module.
 """
 from __future__ import annotations
 
@@ -109,8 +110,39 @@ def reference():
             _I, _SZ, _SZ, _P, _SZ, _SZ, _P, _P, _F, _F, _F, _F,
             ctypes.POINTER(_F)]
         lib.ref_find_ac_strategy_grid.argtypes = [_P, _SZ, _SZ, _P, _P, _F, _P]
+        _declare_reference_frame(lib)
         _ref = lib
     return _ref
+
+
+def _declare_reference_frame(ref) -> None:
+    """Signatures of ref_harness.cpp's frame, loop-filter and Butteraugli
+    entry points."""
+    P, F_, SZ, I, U32 = _P, _F, _SZ, _I, _U32
+    D = ctypes.POINTER(ctypes.c_double)
+    rsig = {
+        "ref_quantizer_from_field": [F_, P, SZ, SZ, P, P],
+        "ref_final_cfl": [P, SZ, SZ, P, P, U32, U32, I, P, P],
+        "ref_code_transform": [I, U32, U32, ctypes.c_int32, F_, F_, F_, F_, P, P, P],
+        "ref_dequantize": [I, U32, U32, ctypes.c_int32, I, F_, P, P],
+        "ref_llf_to_dc": [I, P, P],
+        "ref_dc_to_llf": [I, P, P],
+        "ref_quantize_dc": [P, SZ, SZ, U32, U32, P, P],
+        "ref_smooth_dc": [P, SZ, SZ, U32, U32, P],
+        "ref_epf_inverse_sigma": [P, SZ, SZ, P, U32, U32, P],
+        "ref_loop_filters": [P, SZ, SZ, P, P],
+        "ref_butteraugli_distance": [P, P, SZ, SZ, F_, F_, F_, P, D],
+        "ref_butteraugli_prepared": [P, P, SZ, SZ, SZ, F_, F_, F_, P, P],
+        "ref_butteraugli_blur": [P, SZ, SZ, F_, P],
+        "ref_butteraugli_opsin": [P, SZ, SZ, F_, P],
+        "ref_butteraugli_frequencies": [P, SZ, SZ, P],
+        "ref_butteraugli_stages": [P, P, SZ, SZ, F_, F_, F_, P],
+        "ref_reduce_distance_map": [P, SZ, SZ, P, SZ, SZ, P],
+    }
+    for name, args in rsig.items():
+        fn = getattr(ref, name)
+        fn.argtypes = args
+        fn.restype = I
 
 
 def ptr(array: np.ndarray) -> int:
@@ -256,25 +288,6 @@ def _declare_aq():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    ref = reference()
-    if ref is not None:
-        U32 = _U32
-        rsig = {
-            "ref_quantizer_from_field": [F_, P, SZ, SZ, P, P],
-            "ref_final_cfl": [P, SZ, SZ, P, P, U32, U32, I, P, P],
-            "ref_code_transform": [I, U32, U32, ctypes.c_int32, F_, F_, F_, F_, P, P, P],
-            "ref_dequantize": [I, U32, U32, ctypes.c_int32, I, F_, P, P],
-            "ref_llf_to_dc": [I, P, P],
-            "ref_dc_to_llf": [I, P, P],
-            "ref_quantize_dc": [P, SZ, SZ, U32, U32, P, P],
-            "ref_smooth_dc": [P, SZ, SZ, U32, U32, P],
-            "ref_epf_inverse_sigma": [P, SZ, SZ, P, U32, U32, P],
-            "ref_loop_filters": [P, SZ, SZ, P, P],
-        }
-        for name, args in rsig.items():
-            fn = getattr(ref, name)
-            fn.argtypes = args
-            fn.restype = I
     _AQ_DECLARED = True
 
 
@@ -414,22 +427,6 @@ def _declare_butteraugli():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    ref = reference()
-    if ref is not None:
-        D = ctypes.POINTER(ctypes.c_double)
-        rsig = {
-            "ref_butteraugli_distance": [P, P, SZ, SZ, F_, F_, F_, P, D],
-            "ref_butteraugli_prepared": [P, P, SZ, SZ, SZ, F_, F_, F_, P, P],
-            "ref_butteraugli_blur": [P, SZ, SZ, F_, P],
-            "ref_butteraugli_opsin": [P, SZ, SZ, F_, P],
-            "ref_butteraugli_frequencies": [P, SZ, SZ, P],
-            "ref_butteraugli_stages": [P, P, SZ, SZ, F_, F_, F_, P],
-            "ref_reduce_distance_map": [P, SZ, SZ, P, SZ, SZ, P],
-        }
-        for name, args in rsig.items():
-            fn = getattr(ref, name)
-            fn.argtypes = args
-            fn.restype = I
     _B_DECLARED = True
 
 
