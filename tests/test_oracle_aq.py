@@ -185,3 +185,107 @@ def test_evaluation_round_trip_properties(scene):
     assert coarse_lin < 0.06 and fine_lin < 0.5 * coarse_lin
     # final CfL maps are within the int8 range used by the format
     assert ev["y_to_x"].dtype == np.int8
+
+
+def _composed_frame(ref, fe, setup, ev):
+    """ComputeQuantizedCoefficients and ReconstructQuantizedCoefficients
+    (src/codec/reconstruction.cpp:386-760,790-936) composed here, transform
+    by transform in raster order of the anchors, from the reference's own
+    compiled pieces: ForwardDctCpu, ConvertLowFrequenciesToDc,
+    SelectAdjustedAcQuantization + the Y/X/B quantisation with CfL
+    (ref_code_transform), QuantizeDcCoefficients, SmoothDcCoefficients,
+    DequantizeAcBlock, ConvertDcToLowFrequencies and InverseDctCpu.  The
+    quantizer and the final CfL map are the evaluation's (each pinned by its
+    own test above)."""
+    binding._declare_aq()
+    cells = np.ascontiguousarray(fe["cells"])
+    pre = fe["pre"]
+    bh, bw = cells.shape
+    q = ev["q"]
+    gs, qdc = int(q[2]), int(q[3])
+    mult = (F(setup["x_mult"]), F(1.0), F(setup["b_mult"]))
+    fx, fb = ev["y_to_x"], ev["y_to_b"]
+    raw = ev["field_raw"].copy()
+    dc = np.zeros((3, bh, bw), F)
+    gw = (bw + 31) // 32
+    store = np.zeros((3, binding.store_size(bw, bh)), np.int32)
+    used = {}
+    anchors = []
+    for by in range(bh):
+        for bx in range(bw):
+            c = int(cells[by, bx])
+            if not c & 1:
+                continue
+            s = c >> 1
+            cw, ch = binding.COVERED[s]
+            n = cw * ch * 64
+            coef = np.empty((3, n), F)
+            for k in range(3):
+                pix = np.ascontiguousarray(pre[k, by * 8:by * 8 + ch * 8, bx * 8:bx * 8 + cw * 8])
+                assert ref.ref_forward_dct(s, binding.ptr(pix), binding.ptr(coef[k])) == 0
+            block_dc = np.empty(16, F)
+            assert ref.ref_llf_to_dc(s, binding.ptr(coef[1]), binding.ptr(block_dc)) == 0
+            dc[1, by:by + ch, bx:bx + cw] = block_dc[:cw * ch].reshape(ch, cw)
+            cfl_x = F(fx[by // 8, bx // 8]) / F(84)
+            cfl_b = F(1) + F(fb[by // 8, bx // 8]) / F(84)
+            quant = np.empty((3, n), np.int32)
+            decided = ctypes.c_int32()
+            assert ref.ref_code_transform(s, gs, qdc, int(raw[by, bx]), float(mult[0]),
+                                          float(mult[2]), float(cfl_x), float(cfl_b),
+                                          binding.ptr(coef), binding.ptr(quant),
+                                          ctypes.byref(decided)) == 0
+            raw[by, bx] = decided.value
+            for k in (0, 2):
+                assert ref.ref_llf_to_dc(s, binding.ptr(coef[k]), binding.ptr(block_dc)) == 0
+                dc[k, by:by + ch, bx:bx + cw] = block_dc[:cw * ch].reshape(ch, cw)
+            group = (by // 32) * gw + bx // 32
+            off = used.get(group, 0)
+            store[:, group * 65536 + off:group * 65536 + off + n] = quant
+            used[group] = off + n
+            anchors.append((bx, by, s, group * 65536 + off, cfl_x, cfl_b))
+    dc_q = np.empty((3, bh, bw), np.int32)
+    dc_rec = np.empty((3, bh, bw), F)
+    assert ref.ref_quantize_dc(binding.ptr(dc), bw, bh, gs, qdc, binding.ptr(dc_q),
+                               binding.ptr(dc_rec)) == 0
+    smoothed = np.empty_like(dc_rec)
+    assert ref.ref_smooth_dc(binding.ptr(dc_rec), bw, bh, gs, qdc, binding.ptr(smoothed)) == 0
+    rec = np.zeros((3, bh * 8, bw * 8), F)
+    for bx, by, s, slot, cfl_x, cfl_b in anchors:
+        cw, ch = binding.COVERED[s]
+        n = cw * ch * 64
+        deq = np.empty((3, n), F)
+        for k in range(3):
+            qk = np.ascontiguousarray(store[k, slot:slot + n])
+            assert ref.ref_dequantize(s, gs, qdc, int(raw[by, bx]), k, float(mult[k]),
+                                      binding.ptr(qk), binding.ptr(deq[k])) == 0
+        deq[0] = deq[0] + cfl_x * deq[1]
+        deq[2] = deq[2] + cfl_b * deq[1]
+        for k in range(3):
+            block_dc = np.ascontiguousarray(smoothed[k, by:by + ch, bx:bx + cw]).reshape(-1)
+            coef = np.ascontiguousarray(deq[k])
+            assert ref.ref_dc_to_llf(s, binding.ptr(block_dc), binding.ptr(coef)) == 0
+            pix = np.empty(n, F)
+            assert ref.ref_inverse_dct(s, binding.ptr(coef), binding.ptr(pix)) == 0
+            rec[k, by * 8:by * 8 + ch * 8, bx * 8:bx * 8 + cw * 8] = pix.reshape(ch * 8, cw * 8)
+    return {"raw": raw, "quantized": store, "dc_q": dc_q, "dc_rec": dc_rec,
+            "reconstructed": rec}
+
+
+@pytest.mark.parametrize("w,h,seed,kinds", [(200, 136, 4, 3), (520, 300, 11, 6)])
+def test_frame_coding_composed_from_reference_pieces(ref, w, h, seed, kinds):
+    """The oracle's frame-level coding and reconstruction
+    (oracle_code_coefficients, oracle_reconstruct) equal the reference's
+    per-transform pieces composed in the reference's order, on frames with
+    three strategies and (520x300) six of the seven and four AC groups."""
+    srgb = synthetic.generate(w, h, seed, "textured")
+    fe = binding.frontend(srgb)
+    setup = binding.aq_setup(fe, w, h)
+    ev = binding.aq_evaluate(fe, setup, setup["adjusted"], w, h)
+    used = set(int(v) >> 1 for v in fe["cells"][fe["cells"] & 1 == 1].ravel())
+    assert len(used) >= kinds, used
+    want = _composed_frame(ref, fe, setup, ev)
+    assert np.array_equal(ev["raw"], want["raw"])
+    assert np.array_equal(ev["quantized"], want["quantized"])
+    assert np.array_equal(ev["dc_q"], want["dc_q"])
+    assert np.array_equal(_bits(ev["dc_rec"]), _bits(want["dc_rec"]))
+    assert np.array_equal(_bits(ev["reconstructed"]), _bits(want["reconstructed"]))

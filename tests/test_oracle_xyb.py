@@ -2,9 +2,10 @@
 
 The reference's colour transform (src/codec/color_transform.cpp) sits behind
 the reference's GCC-13.3-only toolchain gate, so the XYB restatement is pinned
-by the reference test's own invariants (tests/color_transform_test.cpp:
-101-160) and by closed forms; the sRGB table is compared bit for bit with the
-reference's compiled image_conversion.cpp.
+by the reference test's own closed-form ScalarOpsin within its 1e-6 bound
+(tests/color_transform_test.cpp:33-46,340-364), by its invariants
+(:101-160) and by closed forms; the sRGB table is compared bit for bit with
+the reference's compiled image_conversion.cpp.
 """
 import math
 
@@ -104,3 +105,63 @@ def test_xyb_detects_swapped_matrix_rows():
     red = np.zeros((3, 8, 8), np.float32)
     red[0] = 1.0
     assert binding.linear_to_padded_opsin(red)[0, 0, 0] > 0.01
+
+
+# tests/color_transform_test.cpp:24-30
+_OPSIN_BIAS = np.float32(0.0037930732552754493)
+_OPSIN_MATRIX = np.array([[0.30, 0.622, 0.078], [0.23, 0.692, 0.078],
+                          [0.24342268924547819, 0.20476744424496821,
+                           0.55180986650955360]], np.float32)
+
+
+def _fma32(a, b, c):
+    # binary32 operands: the binary64 product is exact
+    return (a.astype(np.float64) * np.float64(b) + c.astype(np.float64)).astype(np.float32)
+
+
+def _scalar_opsin(rgb: np.ndarray) -> np.ndarray:
+    """ScalarOpsin (tests/color_transform_test.cpp:33-46) per pixel: fma
+    order blue, green, red onto the bias, cube root, X = (g0 - g1) / 2,
+    Y = (g0 + g1) / 2, B = g2."""
+    bias_root = np.float32(np.cbrt(np.float64(_OPSIN_BIAS)))
+    gamma = []
+    for row in range(3):
+        mixed = _fma32(rgb[2], _OPSIN_MATRIX[row][2], np.full_like(rgb[2], _OPSIN_BIAS))
+        mixed = _fma32(rgb[1], _OPSIN_MATRIX[row][1], mixed)
+        mixed = _fma32(rgb[0], _OPSIN_MATRIX[row][0], mixed)
+        root = np.cbrt(np.maximum(mixed, 0).astype(np.float64)).astype(np.float32)
+        gamma.append((root - bias_root).astype(np.float32))
+    return np.stack([np.float32(0.5) * (gamma[0] - gamma[1]),
+                     np.float32(0.5) * (gamma[0] + gamma[1]), gamma[2]])
+
+
+def _ramp_input(w=256, h=256):
+    """The reference test's 256x256 input (tests/color_transform_test.cpp:
+    310-318)."""
+    y, x = np.mgrid[0:h, 0:w]
+    return np.stack([((3 * x + 5 * y) % 257) / np.float32(256.0),
+                     ((7 * x + 2 * y) % 251) / np.float32(250.0),
+                     ((11 * x + 13 * y) % 263) / np.float32(262.0)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["ramp", "uniform", "synthetic"])
+def test_opsin_matches_scalar_opsin_within_1e6(case):
+    if case == "ramp":
+        rgb = _ramp_input()
+    elif case == "uniform":
+        rgb = np.random.default_rng(5).uniform(0, 1, (3, 61, 77)).astype(np.float32)
+    else:
+        from gjxl_amd import synthetic
+        rgb = binding.srgb8_to_linear(synthetic.generate(96, 64, 3, "textured"))
+    _, h, w = rgb.shape
+    got = binding.linear_to_padded_opsin(rgb)[:, :h, :w]
+    want = _scalar_opsin(rgb)
+    err = float(np.max(np.abs(got - want)))
+    assert err <= 1e-6, err
+    # a transposed matrix (columns for rows) is far outside the bound
+    swapped = _OPSIN_MATRIX.copy()
+    try:
+        _OPSIN_MATRIX[:] = swapped.T
+        assert float(np.max(np.abs(got - _scalar_opsin(rgb)))) > 1e-3
+    finally:
+        _OPSIN_MATRIX[:] = swapped
