@@ -107,9 +107,19 @@ _SIGNATURES = {
                           ctypes.POINTER(ctypes.c_uint64)],
     "gjxl_amd_tokenizer_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
                                   ctypes.POINTER(_SZ)],
+    "gjxl_amd_aq_check": [_P],
+    "gjxl_amd_plain_path_options_init": [_P, _SZ],
+    "gjxl_amd_plain_path_create": [_P, _U32, _U32, _P, ctypes.POINTER(_P)],
+    "gjxl_amd_plain_path_destroy": [_P],
+    "gjxl_amd_plain_path_run": [_P, _P, _P, _P],
+    "gjxl_amd_plain_path_stage_times": [_P, _P, _P, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32)],
+    "gjxl_amd_host_alloc": [_SZ, ctypes.POINTER(_P)],
+    "gjxl_amd_host_free": [_P],
 }
 _VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
-         "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy"}
+         "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy",
+         "gjxl_amd_plain_path_destroy", "gjxl_amd_host_free"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
@@ -138,6 +148,13 @@ def _check(code: int) -> None:
         raise GjxlError(code, msg.decode() if msg else "")
 
 
+def _drop_views(owner) -> None:
+    """Clears the pointers of the views an owner lent out, so that a view used
+    after its owner closed raises instead of reading freed memory."""
+    for view in list(getattr(owner, "_views", ())):
+        view.ptr = None
+
+
 def pad8(v: int) -> int:
     return (v + 7) // 8 * 8
 
@@ -145,16 +162,28 @@ def pad8(v: int) -> int:
 class DeviceArray:
     """A device allocation with a numpy shape and dtype."""
 
-    def __init__(self, device: "Device", shape, dtype, pointer: int | None = None):
+    def __init__(self, device: "Device", shape, dtype, pointer: int | None = None,
+                 owner=None):
         self.device = device
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
-        # a borrowed pointer (a session buffer) is never freed here
+        # a borrowed pointer (a session buffer) is never freed here; its owner
+        # clears it when the owner closes
         self.owned = pointer is None
         self.ptr = device._alloc(max(self.nbytes, 1)) if pointer is None else pointer
+        if self.owned:
+            device._arrays.add(self)
+        elif owner is not None:
+            owner._views.add(self)
+
+    def _live(self) -> int:
+        if not self.ptr:
+            raise RuntimeError("device array is freed or its owner is closed")
+        return self.ptr
 
     def numpy(self) -> np.ndarray:
+        self._live()
         out = np.empty(self.shape, self.dtype)
         if self.nbytes:
             _check(library().gjxl_amd_download(
@@ -162,6 +191,7 @@ class DeviceArray:
         return out
 
     def copy_from(self, array: np.ndarray) -> "DeviceArray":
+        self._live()
         array = np.ascontiguousarray(array, dtype=self.dtype)
         if array.nbytes != self.nbytes:
             raise ValueError("size mismatch")
@@ -192,11 +222,15 @@ class Device:
         self.handle = handle.value
         # sessions and comparators bound to this device: closed first
         self._children = weakref.WeakSet()
+        # arrays this device allocated: freed when it closes
+        self._arrays = weakref.WeakSet()
 
     def close(self) -> None:
         if self.handle:
             for child in list(self._children):
                 child.close()
+            for array in list(self._arrays):
+                array.free()
             library().gjxl_amd_device_destroy(self.handle)
             self.handle = None
 
@@ -348,9 +382,11 @@ class AqSession:
             device.handle, width, height, ctypes.byref(handle)))
         self.handle = handle.value
         self._borrowed = None
+        self._views = weakref.WeakSet()
         device._children.add(self)
 
     def close(self) -> None:
+        _drop_views(self)
         if self.handle:
             library().gjxl_amd_aq_session_destroy(self.handle)
             self.handle = None
@@ -382,6 +418,10 @@ class AqSession:
         return {"lower": lo.value, "upper": hi.value, "quant_dc": qdc.value,
                 "x_scale": xs.value, "b_scale": bs.value}
 
+    def check(self) -> None:
+        """Waits and raises if an update met an invalid block distance."""
+        _check(library().gjxl_amd_aq_check(self.handle))
+
     def evaluate(self, quant_field: DeviceArray) -> None:
         _check(library().gjxl_amd_aq_evaluate(self.handle, quant_field.ptr))
 
@@ -394,7 +434,7 @@ class AqSession:
         dtype = np.dtype(dtype)
         if shape is None:
             shape = (size.value // dtype.itemsize,)
-        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value)
+        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value, owner=self)
         if view.nbytes > size.value:
             raise ValueError(f"{name}: view of {view.nbytes} bytes exceeds {size.value}")
         return view
@@ -432,9 +472,11 @@ class Butteraugli:
             device.handle, width, height, hf_asymmetry, x_multiplier,
             intensity_target, ctypes.byref(handle)))
         self.handle = handle.value
+        self._views = weakref.WeakSet()
         device._children.add(self)
 
     def close(self) -> None:
+        _drop_views(self)
         if self.handle:
             library().gjxl_amd_butteraugli_destroy(self.handle)
             self.handle = None
@@ -467,14 +509,14 @@ class Butteraugli:
         if name == "map":
             shape = (self.height, self.width)
         elif name == "score":
-            return DeviceArray(self.device, (1,), np.uint32, pointer=ptr.value)
+            return DeviceArray(self.device, (1,), np.uint32, pointer=ptr.value, owner=self)
         else:
             w = self.width if scale == 0 else (self.width + 1) // 2
             h = self.height if scale == 0 else (self.height + 1) // 2
             if scale == 0 and (self.width < 8 or self.height < 8):
                 w, h = max(w, 8), max(h, 8)
             shape = (size.value // (4 * w * h), h, w)
-        return DeviceArray(self.device, shape, np.float32, pointer=ptr.value)
+        return DeviceArray(self.device, shape, np.float32, pointer=ptr.value, owner=self)
 
 
 def reduce_distance_map(device: Device, distance_map: DeviceArray,
@@ -517,9 +559,11 @@ class Tokenizer:
         _check(library().gjxl_amd_tokenizer_create(
             device.handle, width, height, ctypes.byref(handle)))
         self.handle = handle.value
+        self._views = weakref.WeakSet()
         device._children.add(self)
 
     def close(self) -> None:
+        _drop_views(self)
         if self.handle:
             library().gjxl_amd_tokenizer_destroy(self.handle)
             self.handle = None
@@ -569,7 +613,7 @@ class Tokenizer:
         dtype = np.dtype(dtype)
         if shape is None:
             shape = (size.value // dtype.itemsize,)
-        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value)
+        view = DeviceArray(self.device, shape, dtype, pointer=ptr.value, owner=self)
         if view.nbytes > size.value:
             raise ValueError(f"{name}: view of {view.nbytes} bytes exceeds {size.value}")
         return view
@@ -602,3 +646,159 @@ def gpu_available() -> bool:
 
 
 from . import pipeline  # noqa: E402,F401  (stage sequencing over this API)
+
+
+# ---- the whole plain path (gjxl_amd_plain_path_*) ------------------------------
+
+class PlainPathOptions(ctypes.Structure):
+    _fields_ = [("struct_size", _SZ), ("distance", _F), ("effort", ctypes.c_int32),
+                ("intensity_target", _F)]
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("bytes_per_pixel", _U32),
+                ("device_resident", ctypes.c_int32), ("pixels", _P),
+                ("pixels_size", _SZ), ("row_stride_bytes", _SZ)]
+
+
+class TokenOutput(ctypes.Structure):
+    _fields_ = [("values", _P), ("contexts", _P), ("group_offsets", _P),
+                ("capacity", _SZ), ("group_capacity", _SZ)]
+
+
+class PlainFrame(ctypes.Structure):
+    _fields_ = [("block_width", _U32), ("block_height", _U32), ("groups", _U32),
+                ("cells", _P), ("quant_field", _P), ("block_distance", _P),
+                ("raw_quant", _P), ("quantized", _P), ("dc_quantized", _P),
+                ("y_to_x", _P), ("y_to_b", _P), ("token_values", _P),
+                ("token_contexts", _P), ("token_group_offsets", _P),
+                ("token_count", ctypes.c_uint64), ("x_qm_scale", ctypes.c_int32),
+                ("b_qm_scale", ctypes.c_int32), ("quant_dc", _F),
+                ("evaluations", ctypes.c_int32), ("scores", _F * 8)]
+
+
+STAGE_NAMES = ("begin", "upload", "srgb_to_linear_and_opsin", "gaborish_inverse",
+               "initial_quant_field", "initial_cfl", "ac_strategy_costs",
+               "ac_strategy_search", "aq_prepare", "butteraugli_prepare", "aq_evaluate",
+               "butteraugli_compare", "aq_reduce_update", "ac_tokenize", "token_readback")
+
+
+class HostBuffer:
+    """Pinned host memory (gjxl_amd_host_alloc) with a numpy view."""
+
+    def __init__(self, shape, dtype):
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(int(v) for v in shape)
+        nbytes = max(int(np.prod(self.shape)) * self.dtype.itemsize, 1)
+        ptr = ctypes.c_void_p()
+        _check(library().gjxl_amd_host_alloc(nbytes, ctypes.byref(ptr)))
+        self.ptr = ptr.value
+        raw = (ctypes.c_uint8 * nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(raw, dtype=np.uint8)[:int(np.prod(self.shape)) *
+                                                         self.dtype.itemsize]
+        self.array = self.array.view(self.dtype).reshape(self.shape)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            library().gjxl_amd_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class NativePlainPath:
+    """The plain path as one C++ call (gjxl_amd_plain_path_*): upload from
+    pinned host memory, every stage on the device stream, token read-back
+    into pinned host buffers."""
+
+    def __init__(self, device: Device, width: int, height: int,
+                 distance: float = 1.0, effort: int = 7,
+                 intensity_target: float = 255.0):
+        self.device = device
+        self.width, self.height = width, height
+        opts = PlainPathOptions()
+        _check(library().gjxl_amd_plain_path_options_init(ctypes.byref(opts),
+                                                          ctypes.sizeof(opts)))
+        opts.distance, opts.effort, opts.intensity_target = distance, effort, intensity_target
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_plain_path_create(device.handle, width, height,
+                                                     ctypes.byref(opts),
+                                                     ctypes.byref(handle)))
+        self.handle = handle.value
+        self._views = weakref.WeakSet()
+        device._children.add(self)
+        bw, bh = pad8(width) // 8, pad8(height) // 8
+        self.bw, self.bh = bw, bh
+        self.groups = ((bw + 31) // 32) * ((bh + 31) // 32)
+        capacity = 3 * 65 * bw * bh
+        self.pixels = HostBuffer((height, width, 3), np.uint8)
+        self.values = HostBuffer((capacity,), np.uint32)
+        self.contexts = HostBuffer((capacity,), np.uint16)
+        self.group_offsets = HostBuffer((self.groups + 1,), np.uint32)
+        self.frame = PlainFrame()
+
+    def close(self) -> None:
+        _drop_views(self)
+        if self.handle:
+            library().gjxl_amd_plain_path_destroy(self.handle)
+            self.handle = None
+        for b in ("pixels", "values", "contexts", "group_offsets"):
+            buf = getattr(self, b, None)
+            if buf is not None:
+                buf.free()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, srgb: np.ndarray | None = None, read_tokens: bool = True) -> int:
+        """One pass over `srgb` (h, w, 3 uint8; copied into the pinned input
+        first) or over the pinned input as it stands; returns the token
+        count.  With read_tokens the stream lands in the pinned buffers."""
+        if srgb is not None:
+            self.pixels.array[...] = srgb
+        img = Image(self.width, self.height, 3, 0, self.pixels.ptr,
+                    self.width * self.height * 3, self.width * 3)
+        out = TokenOutput(self.values.ptr, self.contexts.ptr, self.group_offsets.ptr,
+                          self.values.shape[0], self.groups + 1)
+        _check(library().gjxl_amd_plain_path_run(
+            self.handle, ctypes.byref(img), ctypes.byref(out) if read_tokens else None,
+            ctypes.byref(self.frame)))
+        return int(self.frame.token_count)
+
+    def tokens(self) -> dict:
+        n = int(self.frame.token_count)
+        return {"values": self.values.array[:n].copy(),
+                "contexts": self.contexts.array[:n].copy(),
+                "group_offsets": self.group_offsets.array.copy()}
+
+    def scores(self) -> list[float]:
+        return [float(self.frame.scores[i]) for i in range(self.frame.evaluations)]
+
+    def view(self, name: str) -> DeviceArray:
+        """A device view of one frame buffer of the last run."""
+        bw, bh, store = self.bw, self.bh, self.groups * 65536
+        tiles = ((bh + 7) // 8, (bw + 7) // 8)
+        shapes = {"cells": ((bh, bw), np.uint8), "quant_field": ((bh, bw), np.float32),
+                  "block_distance": ((bh, bw), np.float32),
+                  "raw_quant": ((bh, bw), np.int32), "quantized": ((3, store), np.int32),
+                  "dc_quantized": ((3, bh, bw), np.int32), "y_to_x": (tiles, np.int8),
+                  "y_to_b": (tiles, np.int8)}
+        shape, dtype = shapes[name]
+        return DeviceArray(self.device, shape, dtype, pointer=getattr(self.frame, name),
+                           owner=self)
+
+    def stage_times(self) -> list[tuple[str, float]]:
+        stages = (ctypes.c_int32 * 64)()
+        ms = (ctypes.c_float * 64)()
+        count = ctypes.c_int32()
+        _check(library().gjxl_amd_plain_path_stage_times(self.handle, stages, ms, 64,
+                                                         ctypes.byref(count)))
+        return [(STAGE_NAMES[stages[i]], float(ms[i])) for i in range(min(count.value, 64))]

@@ -2,7 +2,9 @@
 
 Usage: python -m gjxl_amd.build   (or gjxl_amd.build.build_library()).
 Object files go to gjxl_amd/_build/; the shared library lands beside this
-file so that it travels with the source tree.
+file so that it travels with the source tree.  build_caller() compiles the
+C++ test caller (tests/plain_path_caller.cpp, plain g++ against
+include/gjxl_amd.h) into tests/_bin/.
 """
 from __future__ import annotations
 
@@ -65,7 +67,25 @@ def build_library(jobs: int | None = None) -> Path:
     return LIBRARY
 
 
+CALLER_SOURCE = REPO_DIR / "tests" / "plain_path_caller.cpp"
+CALLER = REPO_DIR / "tests" / "_bin" / "plain_path_caller"
+
+
+def build_caller() -> Path:
+    """A host-only C++ program that links libgjxl_amd.so through its C
+    header; the rpath finds the library beside the package."""
+    CALLER.parent.mkdir(exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", str(REPO_DIR / "include"),
+           str(CALLER_SOURCE), "-o", str(CALLER), "-L", str(PACKAGE_DIR), "-lgjxl_amd",
+           "-Wl,-rpath,$ORIGIN/../../gjxl_amd"]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"caller build failed:\n{proc.stdout}\n{proc.stderr}")
+    return CALLER
+
+
 if __name__ == "__main__":
     path = build_library()
+    build_caller()
     print(path)
     sys.exit(0)

@@ -65,6 +65,13 @@ __global__ void ZeroWordKernel(uint32_t* __restrict__ dst) {
   if (threadIdx.x == 0) *dst = 0;
 }
 
+__global__ void ZeroWordsKernel(uint32_t* __restrict__ p, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    p[i] = 0;
+  }
+}
+
 __global__ void CopyFloatsKernel(const float* __restrict__ src, size_t n,
                                  float* __restrict__ dst) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -182,13 +189,19 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
       return r;
     }
   }
-  // the coefficient stores hold zeros in unused group tails
-  for (int id : {GJXL_AMD_AQ_COEFFICIENTS, GJXL_AMD_AQ_QUANTIZED}) {
-    const GJXLResult r = Check(hipMemset(s->buffers[id], 0, z[id]), "hipMemset");
-    if (r != GJXL_OK) {
-      gjxl_amd_aq_session_destroy(s);
-      return r;
-    }
+  // the coefficient stores hold zeros in unused group tails (cleared on the
+  // device stream, which every later stage of the session is ordered after)
+  GJXLResult r = GJXL_OK;
+  for (int id : {GJXL_AMD_AQ_COEFFICIENTS, GJXL_AMD_AQ_QUANTIZED, GJXL_AMD_AQ_STATS}) {
+    if (r != GJXL_OK) break;
+    hipLaunchKernelGGL(ZeroWordsKernel, dim3(1024), dim3(256), 0, device->stream,
+                       static_cast<uint32_t*>(s->buffers[id]), z[id] / 4);
+    r = gjxl_amd::CheckLaunch("ZeroWordsKernel");
+  }
+  if (r == GJXL_OK) r = Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+  if (r != GJXL_OK) {
+    gjxl_amd_aq_session_destroy(s);
+    return r;
   }
   *out = s;
   return GJXL_OK;
@@ -224,6 +237,10 @@ GJXLResult gjxl_amd_aq_prepare(GJXLAmdAqSession* s, const float* opsin,
   s->cells = cells;
   s->target = butteraugli_target;
   using namespace gjxl_amd;
+  // the invalid-distance flag of the updates that follow this preparation
+  hipLaunchKernelGGL(ZeroWordKernel, dim3(1), dim3(64), 0, st,
+                     Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + kErrorWord);
+  GJXL_AMD_TRY(CheckLaunch("ZeroWordKernel"));
   GJXL_AMD_TRY(LaunchMatrixStats(st, opsin, s->width, s->height, s->pw, s->ph,
                                  Buf<uint32_t>(s, GJXL_AMD_AQ_STATS)));
   const size_t blocks = static_cast<size_t>(s->layout.bw) * s->layout.bh;
@@ -393,6 +410,21 @@ GJXLResult gjxl_amd_aq_run(GJXLAmdAqSession* s, GJXLAmdButteraugli* b,
     float v;
     std::memcpy(&v, &host[it], 4);
     scores[it] = v;
+  }
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_aq_check(GJXLAmdAqSession* s) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (!s->prepared) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session is not prepared");
+  hipStream_t st = s->device->stream;
+  uint32_t err = 0;
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(&err, Buf<uint32_t>(s, GJXL_AMD_AQ_STATS) + kErrorWord, 4,
+                                    hipMemcpyDeviceToHost, st),
+                     "hipMemcpyAsync(error)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
+  if (err != 0) {
+    return Fail(GJXL_ERROR_INTERNAL, "Adaptive quantization produced an invalid block distance");
   }
   return GJXL_OK;
 }

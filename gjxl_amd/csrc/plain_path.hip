@@ -1,0 +1,446 @@
+// The plain path of one image extent as one C++ host sequence on the device
+// handle's stream: upload of the packed sRGB8 pixels, XYB, Gaborish inverse,
+// initial quant field, initial CfL, candidate costs, strategy search, AQ
+// preparation, iterations + 1 Butteraugli-driven evaluations with field
+// updates, AC tokenization of the last evaluation and the read-back of its
+// token stream.
+//
+// The order is RunPreparedQuantizationPipelineWithProviders
+// (src/codec/quantization_pipeline.cpp:428-586) with every provider on the
+// GPU, followed by the AcTokenizationProvider hand-off
+// (src/codestream/ac_tokenization_provider_internal.h:18-47), as PrepareWorkflow
+// (src/codestream/workflow.cpp:636-768) runs it for gjxl_encode
+// (src/c_api/gjxl.cpp:470-637) at distance d, effort 7.  The host waits twice:
+// once for the quantization-matrix statistics that choose the matrix scales
+// (src/codestream/workflow.cpp:1318-1484) and once for the token count.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "aq_common.h"
+#include "butteraugli.h"
+#include "common.h"
+#include "tokenize.h"
+
+namespace {
+
+constexpr int kMaxMarks = 48;
+constexpr int kMaxEvaluations = 8;
+
+}  // namespace
+
+struct GJXLAmdPlainPath {
+  GJXLAmdDevice* device = nullptr;
+  uint32_t width = 0, height = 0, pw = 0, ph = 0, bw = 0, bh = 0, groups = 0;
+  GJXLAmdPlainPathOptions options{};
+  int32_t iterations = 2;
+  // device storage
+  uint8_t* srgb = nullptr;
+  size_t srgb_bytes = 0;
+  float* linear = nullptr;
+  float* opsin = nullptr;
+  float* preprocessed = nullptr;
+  float* quant_field = nullptr;
+  float* strategy_mask = nullptr;
+  float* pixel_mask = nullptr;
+  float* iqf_scratch = nullptr;
+  int8_t* y_to_x = nullptr;
+  int8_t* y_to_b = nullptr;
+  float* costs = nullptr;
+  uint8_t* cells = nullptr;
+  uint32_t* score_words = nullptr;
+  GJXLAmdAqSession* session = nullptr;
+  GJXLAmdButteraugli* butteraugli = nullptr;
+  GJXLAmdTokenizer* tokenizer = nullptr;
+  // pinned host words: token status (4), AQ error (1), scores (kMaxEvaluations)
+  uint32_t* host_words = nullptr;
+  // stage marks of the last run
+  hipEvent_t marks[kMaxMarks] = {};
+  int32_t mark_stage[kMaxMarks] = {};
+  int mark_count = 0;
+  uint64_t token_count = 0;
+  float scores[kMaxEvaluations] = {};
+  bool ran = false;
+};
+
+using gjxl_amd::Check;
+using gjxl_amd::Fail;
+
+namespace {
+
+GJXLResult Validate(const GJXLAmdPlainPath* p) {
+  if (p == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path is null");
+  return gjxl_amd::ValidateDevice(p->device);
+}
+
+GJXLResult Mark(GJXLAmdPlainPath* p, int32_t stage) {
+  if (p->mark_count >= kMaxMarks) return GJXL_OK;
+  p->mark_stage[p->mark_count] = stage;
+  return Check(hipEventRecord(p->marks[p->mark_count++], p->device->stream), "hipEventRecord");
+}
+
+template <typename T>
+GJXLResult Alloc(T** out, size_t bytes) {
+  return Check(hipMalloc(reinterpret_cast<void**>(out), bytes), "hipMalloc(plain path)");
+}
+
+template <typename T>
+T* AqBuffer(GJXLAmdAqSession* s, int id) {
+  void* p = nullptr;
+  size_t bytes = 0;
+  (void)gjxl_amd_aq_buffer(s, id, &p, &bytes);
+  return static_cast<T*>(p);
+}
+
+template <typename T>
+T* TokenBuffer(GJXLAmdTokenizer* t, int id) {
+  void* p = nullptr;
+  size_t bytes = 0;
+  (void)gjxl_amd_tokenizer_buffer(t, id, &p, &bytes);
+  return static_cast<T*>(p);
+}
+
+__global__ void CopyScoreKernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+GJXLResult ValidateOptions(const GJXLAmdPlainPathOptions* o) {
+  if (o == nullptr || o->struct_size < sizeof(GJXLAmdPlainPathOptions)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path options are missing or too small");
+  }
+  if (!std::isfinite(o->distance) || o->distance <= 0.0f || o->distance > 25.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "distance must be in (0, 25]");
+  }
+  if (o->effort != 7) {
+    return Fail(GJXL_ERROR_UNSUPPORTED,
+                "the device plain path implements effort 7 (two AQ updates)");
+  }
+  if (!std::isfinite(o->intensity_target) || o->intensity_target <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "intensity target must be positive");
+  }
+  return GJXL_OK;
+}
+
+// The run after the upload (device-resident packed pixels in p->srgb or the
+// caller's device buffer).
+GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
+                       uint32_t bpp) {
+  GJXLAmdDevice* d = p->device;
+  const float t = p->options.distance;
+  GJXL_AMD_TRY(gjxl_amd_srgb8_to_linear_and_opsin(d, srgb, row_stride, bpp, p->width,
+                                                  p->height, p->options.intensity_target,
+                                                  p->linear, p->opsin));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_SRGB_TO_LINEAR_AND_OPSIN));
+  const float ones[3] = {1.0f, 1.0f, 1.0f};
+  GJXL_AMD_TRY(gjxl_amd_gaborish_inverse(d, p->opsin, p->pw, p->ph, ones, p->preprocessed));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_GABORISH_INVERSE));
+  GJXL_AMD_TRY(gjxl_amd_initial_quant_field(d, p->opsin, p->pw, p->ph, t, 1.0f, p->quant_field,
+                                            p->strategy_mask, p->pixel_mask, p->iqf_scratch));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_QUANT_FIELD));
+  GJXL_AMD_TRY(gjxl_amd_initial_cfl(d, p->preprocessed, p->pw, p->ph, p->y_to_x, p->y_to_b));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_CFL));
+  GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs(d, p->preprocessed, p->pw, p->ph, p->quant_field,
+                                          p->pixel_mask, p->y_to_x, p->y_to_b, t, p->costs));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_COSTS));
+  GJXL_AMD_TRY(gjxl_amd_ac_strategy_search(d, p->costs, p->pw, p->ph, t, p->cells));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_SEARCH));
+  GJXL_AMD_TRY(gjxl_amd_aq_prepare(p->session, p->opsin, p->preprocessed, p->cells,
+                                   p->quant_field, t));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_PREPARE));
+  GJXL_AMD_TRY(gjxl_amd_butteraugli_prepare(p->butteraugli, p->linear));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
+  // RunAdaptiveQuantizationPolicyImpl (src/codec/adaptive_quantization.cpp:
+  // 1600-1738): evaluate, compare, reduce, update
+  hipStream_t st = d->stream;
+  float* field = AqBuffer<float>(p->session, GJXL_AMD_AQ_FIELD);
+  float* distance = AqBuffer<float>(p->session, GJXL_AMD_AQ_BLOCK_DISTANCE);
+  const float* decoded = AqBuffer<float>(p->session, GJXL_AMD_AQ_LINEAR);
+  const size_t n = static_cast<size_t>(p->bw) * p->bh;
+  GJXL_AMD_TRY(gjxl_amd_copy(d, field, AqBuffer<float>(p->session, GJXL_AMD_AQ_ADJUSTED_FIELD),
+                             4 * n));
+  for (int it = 0; it <= p->iterations; ++it) {
+    GJXL_AMD_TRY(gjxl_amd_aq_evaluate(p->session, field));
+    GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_EVALUATE));
+    GJXL_AMD_TRY(gjxl_amd::ButteraugliCompare(p->butteraugli, decoded, nullptr));
+    GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_COMPARE));
+    hipLaunchKernelGGL(CopyScoreKernel, dim3(1), dim3(64), 0, st, p->butteraugli->score_bits,
+                       p->score_words + it);
+    GJXL_AMD_TRY(gjxl_amd::CheckLaunch("CopyScoreKernel"));
+    GJXL_AMD_TRY(gjxl_amd::LaunchReduceDistanceMap(st, p->butteraugli->map, p->width, p->height,
+                                                   p->cells, p->bw, p->bh, distance));
+    if (it < p->iterations) {
+      GJXL_AMD_TRY(gjxl_amd_aq_update(p->session, field, distance, it, t));
+    }
+    GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_REDUCE_UPDATE));
+  }
+  // AC tokens of the last evaluation (natural orders, default block-context
+  // map), then the status words, the AQ error flag and the scores
+  GJXL_AMD_TRY(gjxl_amd::TokenizeEnqueue(p->tokenizer,
+                                         AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_QUANTIZED),
+                                         p->cells,
+                                         AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_RAW_QUANT),
+                                         false));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_TOKENIZE));
+  uint32_t* host = p->host_words;
+  uint32_t* stats = AqBuffer<uint32_t>(p->session, GJXL_AMD_AQ_STATS);
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, TokenBuffer<uint32_t>(p->tokenizer,
+                                                               GJXL_AMD_TOKENS_STATUS),
+                                    16, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)"));
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 4, stats + 7, 4, hipMemcpyDeviceToHost, st),
+                     "hipMemcpyAsync(AQ error)"));
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * (p->iterations + 1),
+                                    hipMemcpyDeviceToHost, st), "hipMemcpyAsync(scores)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
+  if (host[4] != 0) {
+    return Fail(GJXL_ERROR_INTERNAL, "Adaptive quantization produced an invalid block distance");
+  }
+  GJXL_AMD_TRY(gjxl_amd::TokenizeResult(host, &p->token_count));
+  for (int it = 0; it <= p->iterations; ++it) std::memcpy(&p->scores[it], &host[8 + it], 4);
+  return GJXL_OK;
+}
+
+GJXLResult ReadTokens(GJXLAmdPlainPath* p, const GJXLAmdTokenOutput* out) {
+  if (out == nullptr) return GJXL_OK;
+  if (out->values == nullptr && out->contexts == nullptr && out->group_offsets == nullptr) {
+    return GJXL_OK;
+  }
+  if ((out->values != nullptr || out->contexts != nullptr) && out->capacity < p->token_count) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "token output capacity is below the token count");
+  }
+  if (out->group_offsets != nullptr && out->group_capacity < p->groups + 1u) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "group offset output holds fewer than groups + 1");
+  }
+  hipStream_t st = p->device->stream;
+  const size_t count = static_cast<size_t>(p->token_count);
+  if (out->values != nullptr && count > 0) {
+    GJXL_AMD_TRY(Check(hipMemcpyAsync(out->values,
+                                      TokenBuffer<uint32_t>(p->tokenizer, GJXL_AMD_TOKENS_VALUES),
+                                      4 * count, hipMemcpyDeviceToHost, st),
+                       "hipMemcpyAsync(token values)"));
+  }
+  if (out->contexts != nullptr && count > 0) {
+    GJXL_AMD_TRY(Check(hipMemcpyAsync(out->contexts,
+                                      TokenBuffer<uint16_t>(p->tokenizer,
+                                                            GJXL_AMD_TOKENS_CONTEXTS),
+                                      2 * count, hipMemcpyDeviceToHost, st),
+                       "hipMemcpyAsync(token contexts)"));
+  }
+  if (out->group_offsets != nullptr) {
+    GJXL_AMD_TRY(Check(hipMemcpyAsync(out->group_offsets,
+                                      TokenBuffer<uint32_t>(p->tokenizer,
+                                                            GJXL_AMD_TOKENS_GROUP_OFFSETS),
+                                      4 * (p->groups + 1), hipMemcpyDeviceToHost, st),
+                       "hipMemcpyAsync(group offsets)"));
+  }
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_TOKEN_READBACK));
+  return Check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+GJXLResult FillFrame(GJXLAmdPlainPath* p, GJXLAmdPlainFrame* f) {
+  if (f == nullptr) return GJXL_OK;
+  GJXLAmdPlainFrame r{};
+  r.block_width = p->bw;
+  r.block_height = p->bh;
+  r.groups = p->groups;
+  r.cells = p->cells;
+  r.quant_field = AqBuffer<float>(p->session, GJXL_AMD_AQ_FIELD);
+  r.block_distance = AqBuffer<float>(p->session, GJXL_AMD_AQ_BLOCK_DISTANCE);
+  r.raw_quant = AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_RAW_QUANT);
+  r.quantized = AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_QUANTIZED);
+  r.dc_quantized = AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_DC_QUANTIZED);
+  r.y_to_x = AqBuffer<int8_t>(p->session, GJXL_AMD_AQ_Y_TO_X);
+  r.y_to_b = AqBuffer<int8_t>(p->session, GJXL_AMD_AQ_Y_TO_B);
+  r.token_values = TokenBuffer<uint32_t>(p->tokenizer, GJXL_AMD_TOKENS_VALUES);
+  r.token_contexts = TokenBuffer<uint16_t>(p->tokenizer, GJXL_AMD_TOKENS_CONTEXTS);
+  r.token_group_offsets = TokenBuffer<uint32_t>(p->tokenizer, GJXL_AMD_TOKENS_GROUP_OFFSETS);
+  r.token_count = p->token_count;
+  float lo = 0, hi = 0, qdc = 0;
+  GJXL_AMD_TRY(gjxl_amd_aq_policy(p->session, &lo, &hi, &qdc, &r.x_qm_scale, &r.b_qm_scale));
+  r.quant_dc = qdc;
+  r.evaluations = p->iterations + 1;
+  for (int i = 0; i < r.evaluations; ++i) r.scores[i] = p->scores[i];
+  *f = r;
+  return GJXL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+GJXLResult gjxl_amd_plain_path_options_init(GJXLAmdPlainPathOptions* options,
+                                            size_t struct_size) {
+  if (options == nullptr || struct_size < sizeof(GJXLAmdPlainPathOptions)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path options are missing or too small");
+  }
+  std::memset(options, 0, sizeof(GJXLAmdPlainPathOptions));
+  options->struct_size = sizeof(GJXLAmdPlainPathOptions);
+  options->distance = 1.0f;  // gjxl_encoder_options_init (include/gjxl/gjxl.h:207-211)
+  options->effort = 7;
+  options->intensity_target = 255.0f;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_plain_path_create(GJXLAmdDevice* device, uint32_t width, uint32_t height,
+                                      const GJXLAmdPlainPathOptions* options,
+                                      GJXLAmdPlainPath** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || *out != nullptr || width == 0 || height == 0 ||
+      width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path geometry is invalid");
+  }
+  GJXLAmdPlainPathOptions defaults{};
+  if (options == nullptr) {
+    GJXL_AMD_TRY(gjxl_amd_plain_path_options_init(&defaults, sizeof(defaults)));
+    options = &defaults;
+  }
+  GJXL_AMD_TRY(ValidateOptions(options));
+  auto* p = new (std::nothrow) GJXLAmdPlainPath;
+  if (p == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "plain path allocation failed");
+  p->device = device;
+  p->options = *options;
+  p->iterations = 2;
+  p->width = width;
+  p->height = height;
+  p->pw = gjxl_amd::PadTo8(width);
+  p->ph = gjxl_amd::PadTo8(height);
+  p->bw = p->pw / 8;
+  p->bh = p->ph / 8;
+  p->groups = gjxl_amd::CeilDiv(p->bw, 32) * gjxl_amd::CeilDiv(p->bh, 32);
+  const size_t pixels = static_cast<size_t>(width) * height;
+  const size_t padded = static_cast<size_t>(p->pw) * p->ph;
+  const size_t blocks = static_cast<size_t>(p->bw) * p->bh;
+  const size_t tiles = static_cast<size_t>(gjxl_amd::CeilDiv(p->bw, 8)) *
+                       gjxl_amd::CeilDiv(p->bh, 8);
+  p->srgb_bytes = 4 * pixels;  // room for RGBA rows without padding
+  GJXLResult r = Alloc(&p->srgb, p->srgb_bytes);
+  if (r == GJXL_OK) r = Alloc(&p->linear, 4 * 3 * pixels);
+  if (r == GJXL_OK) r = Alloc(&p->opsin, 4 * 3 * padded);
+  if (r == GJXL_OK) r = Alloc(&p->preprocessed, 4 * 3 * padded);
+  if (r == GJXL_OK) r = Alloc(&p->quant_field, 4 * blocks);
+  if (r == GJXL_OK) r = Alloc(&p->strategy_mask, 4 * blocks);
+  if (r == GJXL_OK) r = Alloc(&p->pixel_mask, 4 * padded);
+  if (r == GJXL_OK) {
+    r = Alloc(&p->iqf_scratch, gjxl_amd_initial_quant_field_scratch_bytes(p->pw, p->ph));
+  }
+  if (r == GJXL_OK) r = Alloc(&p->y_to_x, tiles);
+  if (r == GJXL_OK) r = Alloc(&p->y_to_b, tiles);
+  if (r == GJXL_OK) r = Alloc(&p->costs, 4 * 7 * blocks);
+  if (r == GJXL_OK) r = Alloc(&p->cells, blocks);
+  if (r == GJXL_OK) r = Alloc(&p->score_words, 4 * kMaxEvaluations);
+  if (r == GJXL_OK) {
+    r = Check(hipHostMalloc(reinterpret_cast<void**>(&p->host_words), 4 * 32,
+                            hipHostMallocDefault), "hipHostMalloc(plain path)");
+  }
+  for (int i = 0; r == GJXL_OK && i < kMaxMarks; ++i) {
+    r = Check(hipEventCreate(&p->marks[i]), "hipEventCreate");
+  }
+  if (r == GJXL_OK) r = gjxl_amd_aq_session_create(device, width, height, &p->session);
+  if (r == GJXL_OK) {
+    r = gjxl_amd_butteraugli_create(device, width, height, 1.0f, 1.0f, 80.0f, &p->butteraugli);
+  }
+  if (r == GJXL_OK) r = gjxl_amd_tokenizer_create(device, width, height, &p->tokenizer);
+  if (r != GJXL_OK) {
+    gjxl_amd_plain_path_destroy(p);
+    return r;
+  }
+  *out = p;
+  return GJXL_OK;
+}
+
+void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* p) {
+  if (p == nullptr) return;
+  if (p->device != nullptr) {
+    (void)hipSetDevice(p->device->ordinal);
+    (void)hipStreamSynchronize(p->device->stream);
+  }
+  gjxl_amd_tokenizer_destroy(p->tokenizer);
+  gjxl_amd_butteraugli_destroy(p->butteraugli);
+  gjxl_amd_aq_session_destroy(p->session);
+  for (hipEvent_t e : p->marks) {
+    if (e != nullptr) (void)hipEventDestroy(e);
+  }
+  if (p->host_words != nullptr) (void)hipHostFree(p->host_words);
+  void* buffers[] = {p->srgb, p->linear, p->opsin, p->preprocessed, p->quant_field,
+                     p->strategy_mask, p->pixel_mask, p->iqf_scratch, p->y_to_x, p->y_to_b,
+                     p->costs, p->cells, p->score_words};
+  for (void* b : buffers) {
+    if (b != nullptr) (void)hipFree(b);
+  }
+  delete p;
+}
+
+GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* image,
+                                   const GJXLAmdTokenOutput* tokens,
+                                   GJXLAmdPlainFrame* frame) {
+  GJXL_AMD_TRY(Validate(p));
+  if (image == nullptr || image->pixels == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path image is null");
+  }
+  if (image->width != p->width || image->height != p->height) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "image extent differs from the plain path's");
+  }
+  const uint32_t bpp = image->bytes_per_pixel;
+  if ((bpp != 3 && bpp != 4) ||
+      image->row_stride_bytes < static_cast<size_t>(image->width) * bpp) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "pixel layout must be packed RGB8 or RGBA8");
+  }
+  const size_t row = static_cast<size_t>(image->width) * bpp;
+  const size_t need = (static_cast<size_t>(image->height) - 1) * image->row_stride_bytes + row;
+  if (image->pixels_size < need) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "pixel buffer is smaller than its rows");
+  }
+  p->ran = false;
+  p->mark_count = 0;
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BEGIN));
+  const uint8_t* srgb = static_cast<const uint8_t*>(image->pixels);
+  size_t stride = image->row_stride_bytes;
+  if (!image->device_resident) {
+    // rows packed into the device copy (pinned host memory gives one DMA)
+    hipStream_t st = p->device->stream;
+    if (stride == row) {
+      GJXL_AMD_TRY(Check(hipMemcpyAsync(p->srgb, srgb, row * image->height,
+                                        hipMemcpyHostToDevice, st), "hipMemcpyAsync(sRGB)"));
+    } else {
+      GJXL_AMD_TRY(Check(hipMemcpy2DAsync(p->srgb, row, srgb, stride, row, image->height,
+                                          hipMemcpyHostToDevice, st), "hipMemcpy2DAsync(sRGB)"));
+    }
+    srgb = p->srgb;
+    stride = row;
+  }
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_UPLOAD));
+  GJXL_AMD_TRY(RunResident(p, srgb, stride, bpp));
+  GJXL_AMD_TRY(ReadTokens(p, tokens));
+  p->ran = true;
+  return FillFrame(p, frame);
+}
+
+GJXLResult gjxl_amd_plain_path_stage_times(GJXLAmdPlainPath* p, int32_t* stages,
+                                           float* milliseconds, int32_t capacity,
+                                           int32_t* count) {
+  GJXL_AMD_TRY(Validate(p));
+  if (count == nullptr || (capacity > 0 && (stages == nullptr || milliseconds == nullptr))) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "stage time outputs are null");
+  }
+  if (!p->ran) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "the plain path has not completed a run");
+  const int n = p->mark_count - 1;
+  *count = n;
+  for (int i = 0; i < n && i < capacity; ++i) {
+    GJXL_AMD_TRY(Check(hipEventElapsedTime(&milliseconds[i], p->marks[i], p->marks[i + 1]),
+                       "hipEventElapsedTime"));
+    stages[i] = p->mark_stage[i + 1];
+  }
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_host_alloc(size_t bytes, void** out) {
+  if (out == nullptr || *out != nullptr || bytes == 0) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "host allocation output must be empty");
+  }
+  return Check(hipHostMalloc(out, bytes, hipHostMallocDefault), "hipHostMalloc");
+}
+
+void gjxl_amd_host_free(void* pointer) {
+  if (pointer != nullptr) (void)hipHostFree(pointer);
+}
+
+}  // extern "C"

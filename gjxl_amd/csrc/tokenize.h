@@ -1,0 +1,20 @@
+// AC tokenization in two halves, for hosts that sequence several stages
+// before their one wait: the launches, then the check of the status words
+// that the launches leave in GJXL_AMD_TOKENS_STATUS.
+#pragma once
+
+#include "common.h"
+
+namespace gjxl_amd {
+
+// Enqueues the four tokenization passes on the tokenizer's device stream
+// (gjxl_amd_tokenize without its wait).
+GJXLResult TokenizeEnqueue(GJXLAmdTokenizer* t, const int32_t* quantized,
+                           const uint8_t* cells, const int32_t* raw_quant,
+                           bool populations);
+
+// Errors and the token count from the four status words of a finished
+// tokenization, copied to the host.
+GJXLResult TokenizeResult(const uint32_t status[4], uint64_t* token_count);
+
+}  // namespace gjxl_amd

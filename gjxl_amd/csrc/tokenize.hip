@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "aq_common.h"
+#include "tokenize.h"
 #include "common.h"
 
 namespace gjxl_amd {
@@ -399,6 +400,73 @@ GJXLResult UploadOrders(GJXLAmdTokenizer* t, const std::vector<uint32_t>& table)
 
 }  // namespace
 
+namespace gjxl_amd {
+
+GJXLResult TokenizeEnqueue(GJXLAmdTokenizer* t, const int32_t* quantized,
+                           const uint8_t* cells, const int32_t* raw_quant,
+                           bool populations) {
+  GJXL_AMD_TRY(ValidateTokenizer(t));
+  if (quantized == nullptr || cells == nullptr || raw_quant == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "tokenization arguments are null");
+  }
+  GJXL_AMD_TRY(Check(hipSetDevice(t->device->ordinal), "hipSetDevice"));
+  hipStream_t s = t->device->stream;
+  const AqLayout& l = t->layout;
+  uint32_t* status = Buf<uint32_t>(t, GJXL_AMD_TOKENS_STATUS);
+  uint32_t* offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_OFFSETS);
+  uint32_t* anchors = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_LIST);
+  uint32_t* counts = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_COUNT);
+  uint32_t* tasks = Buf<uint32_t>(t, GJXL_AMD_TOKENS_TASKS);
+  uint32_t* task_offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_TASK_OFFSETS);
+  uint32_t* group_tokens = Buf<uint32_t>(t, GJXL_AMD_TOKENS_GROUP_TOKENS);
+  uint32_t* group_offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_GROUP_OFFSETS);
+  uint8_t* map = Buf<uint8_t>(t, GJXL_AMD_TOKENS_NONZERO_MAP);
+  uint32_t* histogram = populations ? Buf<uint32_t>(t, GJXL_AMD_TOKENS_HISTOGRAM) : nullptr;
+  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, s, status, size_t{4}, 0u);
+  GJXL_AMD_TRY(CheckLaunch("FillKernel"));
+  // status[1] bit 0: the grid does not tile its groups
+  GJXL_AMD_TRY(LaunchAnchorLayout(s, cells, l.bw * 8, l.bh * 8, offsets, anchors, counts,
+                                  status + 1, nullptr, 0, nullptr));
+  const dim3 task_grid(kGroupTasks / kTaskWaves, l.groups);
+  hipLaunchKernelGGL(TokenCountKernel, task_grid, dim3(64 * kTaskWaves), 0, s, quantized,
+                     l.store, cells, l.bw, l.bh, offsets, anchors, counts,
+                     Buf<uint16_t>(t, GJXL_AMD_TOKENS_INVERSE_ORDERS), raw_quant, status,
+                     tasks, map);
+  GJXL_AMD_TRY(CheckLaunch("TokenCountKernel"));
+  hipLaunchKernelGGL(TokenGroupScanKernel, dim3(l.groups), dim3(kGroupAnchors), 0, s, counts,
+                     tasks, task_offsets, group_tokens);
+  GJXL_AMD_TRY(CheckLaunch("TokenGroupScanKernel"));
+  hipLaunchKernelGGL(TokenFrameScanKernel, dim3(1), dim3(1024), 0, s, group_tokens, l.groups,
+                     group_offsets, status);
+  GJXL_AMD_TRY(CheckLaunch("TokenFrameScanKernel"));
+  if (histogram != nullptr) {
+    const size_t words = static_cast<size_t>(t->model.num_contexts) *
+                         (kNonzeroBuckets + kZeroDensityContexts) * kSymbols;
+    hipLaunchKernelGGL(FillKernel, dim3(256), dim3(256), 0, s, histogram, words, 0u);
+    GJXL_AMD_TRY(CheckLaunch("FillKernel"));
+  }
+  hipLaunchKernelGGL(TokenEmitKernel, task_grid, dim3(64 * kTaskWaves), 0, s, quantized,
+                     l.store, cells, l.bw, l.bh, offsets, anchors, counts,
+                     Buf<uint32_t>(t, GJXL_AMD_TOKENS_ORDERS), raw_quant, tasks, task_offsets,
+                     group_offsets, map, t->model, Buf<uint32_t>(t, GJXL_AMD_TOKENS_VALUES),
+                     Buf<uint16_t>(t, GJXL_AMD_TOKENS_CONTEXTS), histogram);
+  GJXL_AMD_TRY(CheckLaunch("TokenEmitKernel"));
+  return GJXL_OK;
+}
+
+GJXLResult TokenizeResult(const uint32_t status[4], uint64_t* token_count) {
+  if (status[1] & kGridError) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "strategy grid does not tile its AC groups");
+  }
+  if (status[1] & kRawQuantError) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AC-group raw quantization is out of range");
+  }
+  *token_count = status[0];
+  return GJXL_OK;
+}
+
+}  // namespace gjxl_amd
+
 GJXLResult gjxl_amd_natural_order(int32_t strategy, uint32_t* order, size_t capacity,
                                   size_t* count) {
   using namespace gjxl_amd;
@@ -439,6 +507,12 @@ GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device, uint32_t width, uint
   // every block holds 64 coefficients per channel and every transform adds
   // one count token: 3 * (64 * blocks + anchors) with anchors <= blocks
   const size_t capacity = 3 * 65 * blocks;
+  if (capacity > 0xffffffffull) {
+    // token offsets and counts are 32-bit words
+    delete t;
+    return Fail(GJXL_ERROR_UNSUPPORTED,
+                "frame too large for 32-bit token offsets (at most 22,025,473 blocks)");
+  }
   size_t* z = t->sizes;
   z[GJXL_AMD_TOKENS_VALUES] = 4 * capacity;
   z[GJXL_AMD_TOKENS_CONTEXTS] = 2 * capacity;
@@ -571,64 +645,17 @@ GJXLResult gjxl_amd_tokenize(GJXLAmdTokenizer* t, const int32_t* quantized,
                              const uint8_t* cells, const int32_t* raw_quant,
                              int32_t populations, uint64_t* token_count) {
   using namespace gjxl_amd;
-  GJXL_AMD_TRY(ValidateTokenizer(t));
-  if (quantized == nullptr || cells == nullptr || raw_quant == nullptr || token_count == nullptr) {
+  if (token_count == nullptr) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "tokenization arguments are null");
   }
-  GJXL_AMD_TRY(Check(hipSetDevice(t->device->ordinal), "hipSetDevice"));
+  GJXL_AMD_TRY(TokenizeEnqueue(t, quantized, cells, raw_quant, populations != 0));
   hipStream_t s = t->device->stream;
-  const AqLayout& l = t->layout;
-  uint32_t* status = Buf<uint32_t>(t, GJXL_AMD_TOKENS_STATUS);
-  uint32_t* offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_OFFSETS);
-  uint32_t* anchors = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_LIST);
-  uint32_t* counts = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_COUNT);
-  uint32_t* tasks = Buf<uint32_t>(t, GJXL_AMD_TOKENS_TASKS);
-  uint32_t* task_offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_TASK_OFFSETS);
-  uint32_t* group_tokens = Buf<uint32_t>(t, GJXL_AMD_TOKENS_GROUP_TOKENS);
-  uint32_t* group_offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_GROUP_OFFSETS);
-  uint8_t* map = Buf<uint8_t>(t, GJXL_AMD_TOKENS_NONZERO_MAP);
-  uint32_t* histogram = populations ? Buf<uint32_t>(t, GJXL_AMD_TOKENS_HISTOGRAM) : nullptr;
-  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, s, status, size_t{4}, 0u);
-  GJXL_AMD_TRY(CheckLaunch("FillKernel"));
-  // status[1] bit 0: the grid does not tile its groups
-  GJXL_AMD_TRY(LaunchAnchorLayout(s, cells, l.bw * 8, l.bh * 8, offsets, anchors, counts,
-                                  status + 1, nullptr, 0, nullptr));
-  const dim3 task_grid(kGroupTasks / kTaskWaves, l.groups);
-  hipLaunchKernelGGL(TokenCountKernel, task_grid, dim3(64 * kTaskWaves), 0, s, quantized,
-                     l.store, cells, l.bw, l.bh, offsets, anchors, counts,
-                     Buf<uint16_t>(t, GJXL_AMD_TOKENS_INVERSE_ORDERS), raw_quant, status,
-                     tasks, map);
-  GJXL_AMD_TRY(CheckLaunch("TokenCountKernel"));
-  hipLaunchKernelGGL(TokenGroupScanKernel, dim3(l.groups), dim3(kGroupAnchors), 0, s, counts,
-                     tasks, task_offsets, group_tokens);
-  GJXL_AMD_TRY(CheckLaunch("TokenGroupScanKernel"));
-  hipLaunchKernelGGL(TokenFrameScanKernel, dim3(1), dim3(1024), 0, s, group_tokens, l.groups,
-                     group_offsets, status);
-  GJXL_AMD_TRY(CheckLaunch("TokenFrameScanKernel"));
-  if (histogram != nullptr) {
-    const size_t words = static_cast<size_t>(t->model.num_contexts) *
-                         (kNonzeroBuckets + kZeroDensityContexts) * kSymbols;
-    hipLaunchKernelGGL(FillKernel, dim3(256), dim3(256), 0, s, histogram, words, 0u);
-    GJXL_AMD_TRY(CheckLaunch("FillKernel"));
-  }
-  hipLaunchKernelGGL(TokenEmitKernel, task_grid, dim3(64 * kTaskWaves), 0, s, quantized,
-                     l.store, cells, l.bw, l.bh, offsets, anchors, counts,
-                     Buf<uint32_t>(t, GJXL_AMD_TOKENS_ORDERS), raw_quant, tasks, task_offsets,
-                     group_offsets, map, t->model, Buf<uint32_t>(t, GJXL_AMD_TOKENS_VALUES),
-                     Buf<uint16_t>(t, GJXL_AMD_TOKENS_CONTEXTS), histogram);
-  GJXL_AMD_TRY(CheckLaunch("TokenEmitKernel"));
   uint32_t host[4];
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, status, sizeof(host), hipMemcpyDeviceToHost, s),
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, Buf<uint32_t>(t, GJXL_AMD_TOKENS_STATUS), sizeof(host),
+                                    hipMemcpyDeviceToHost, s),
                      "hipMemcpyAsync(token status)"));
   GJXL_AMD_TRY(Check(hipStreamSynchronize(s), "hipStreamSynchronize"));
-  if (host[1] & kGridError) {
-    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "strategy grid does not tile its AC groups");
-  }
-  if (host[1] & kRawQuantError) {
-    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AC-group raw quantization is out of range");
-  }
-  *token_count = host[0];
-  return GJXL_OK;
+  return TokenizeResult(host, token_count);
 }
 
 GJXLResult gjxl_amd_tokenizer_buffer(GJXLAmdTokenizer* t, int32_t id, void** pointer,

@@ -1,4 +1,8 @@
-"""Stage sequencing of the plain path on one device.
+"""Stage sequencing of the plain path on one device, call by call.
+
+The product's sequence is the C++ one behind gjxl_amd_plain_path_run
+(gjxl_amd.NativePlainPath); this module repeats it through the per-stage
+entry points so that tests can read every intermediate buffer.
 
 Mirrors the order of RunPreparedQuantizationPipelineWithProviders
 (src/codec/quantization_pipeline.cpp:428-586) and PrepareWorkflow
@@ -116,6 +120,8 @@ class PlainPath:
             if it < self.iterations:
                 self.session.update(self.field, self.block_distance, it, t)
             mark("aq_reduce_update")
+        # the reference fails the loop on an invalid block distance
+        self.session.check()
         # AC tokens of the last evaluation's frame (waits for the count)
         self.token_count = self.tokenizer.tokenize(self.quantized, self.cells,
                                                    self.raw_quant)

@@ -325,6 +325,11 @@ GJXL_AMD_API GJXLResult gjxl_amd_aq_run(GJXLAmdAqSession* session,
                                         int32_t iterations,
                                         float butteraugli_target,
                                         double* scores);
+/* Waits for the stream and fails with GJXL_ERROR_INTERNAL if an update since
+ * the last preparation met a non-finite or negative block distance (the
+ * reference's Status::Internal, src/codec/adaptive_quantization.cpp:1686-
+ * 1695); such blocks keep their field value. */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_check(GJXLAmdAqSession* session);
 
 /* ---------------------------------------------------------------------- */
 /* a19: AC tokenization of a quantized frame.  Replaces                      */
@@ -413,6 +418,127 @@ GJXL_AMD_API GJXLResult gjxl_amd_tokenize(GJXLAmdTokenizer* tokenizer,
 GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_buffer(GJXLAmdTokenizer* tokenizer,
                                                   int32_t id, void** pointer,
                                                   size_t* bytes);
+
+/* ---------------------------------------------------------------------- */
+/* The whole plain path as one call.  Replaces, for one image, the stage     */
+/* sequence that gjxl_encode (src/c_api/gjxl.cpp:470-637) runs through       */
+/* PrepareWorkflow (src/codestream/workflow.cpp:636-768) and                 */
+/* RunPreparedQuantizationPipelineWithProviders (src/codec/quantization_     */
+/* pipeline.cpp:428-586) with the GPU providers installed, up to the         */
+/* AcTokenizationProvider hand-off (src/codestream/ac_tokenization_provider_ */
+/* internal.h:18-47): upload, XYB, Gaborish inverse, initial quant field,    */
+/* initial CfL, AC strategy search, AQ preparation, three Butteraugli-driven */
+/* evaluations with two field updates (effort 7), AC tokenization of the     */
+/* last evaluation and the read-back of its tokens.  One stream; the host    */
+/* waits for the matrix-scale statistics and for the token count.  The       */
+/* entropy coder and bitstream writer that consume the frame stay on the     */
+/* CPU (src/codestream/vardct/encoder.cpp).                                  */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdPlainPath GJXLAmdPlainPath;
+
+/* Subset of GJXLEncoderOptions (include/gjxl/gjxl.h:129-160) that the path
+ * reads: distance and effort (7 only; other efforts return
+ * GJXL_ERROR_UNSUPPORTED), plus the input's intensity target.  The DC
+ * options are those of effort 7: weighted prediction, prediction-aware
+ * quantization, adaptive smoothing. */
+typedef struct {
+  size_t struct_size;
+  float distance;          /* default 1.0 */
+  int32_t effort;          /* default 7 */
+  float intensity_target;  /* default 255 */
+} GJXLAmdPlainPathOptions;
+
+/* A packed RGB8 or RGBA8 sRGB image (GJXLImageView, include/gjxl/gjxl.h:
+ * 176-184), on the host (pinned memory from gjxl_amd_host_alloc gives one
+ * DMA copy) or, when device_resident is non-zero, in device memory. */
+typedef struct {
+  uint32_t width, height;
+  uint32_t bytes_per_pixel;  /* 3 or 4 (alpha ignored) */
+  int32_t device_resident;
+  const void* pixels;
+  size_t pixels_size;
+  size_t row_stride_bytes;
+} GJXLAmdImage;
+
+/* Host buffers for the token stream.  Null pointers skip that read-back;
+ * capacity (tokens) must cover the frame's count, group_capacity its
+ * groups + 1 offsets. */
+typedef struct {
+  uint32_t* values;
+  uint16_t* contexts;
+  uint32_t* group_offsets;
+  size_t capacity;
+  size_t group_capacity;
+} GJXLAmdTokenOutput;
+
+/* The frame of the last run: device pointers into the path's storage, valid
+ * until the next run or destroy.  Layouts as the session and tokenizer
+ * buffers above (GJXL_AMD_AQ_*, GJXL_AMD_TOKENS_*). */
+typedef struct {
+  uint32_t block_width, block_height, groups;
+  const uint8_t* cells;             /* strategy << 1 | anchor per block */
+  const float* quant_field;         /* the last evaluation's field */
+  const float* block_distance;      /* its 16-norms */
+  const int32_t* raw_quant;         /* adjusted raw quant per block */
+  const int32_t* quantized;         /* 3 planes of groups * 65536 */
+  const int32_t* dc_quantized;      /* 3 block planes */
+  const int8_t* y_to_x;             /* final CfL per 64x64 tile */
+  const int8_t* y_to_b;
+  const uint32_t* token_values;
+  const uint16_t* token_contexts;
+  const uint32_t* token_group_offsets;
+  uint64_t token_count;
+  int32_t x_qm_scale, b_qm_scale;
+  float quant_dc;
+  int32_t evaluations;
+  float scores[8];                  /* Butteraugli score per evaluation */
+} GJXLAmdPlainFrame;
+
+/* Stage ids of gjxl_amd_plain_path_stage_times. */
+enum {
+  GJXL_AMD_STAGE_BEGIN = 0,
+  GJXL_AMD_STAGE_UPLOAD,
+  GJXL_AMD_STAGE_SRGB_TO_LINEAR_AND_OPSIN,
+  GJXL_AMD_STAGE_GABORISH_INVERSE,
+  GJXL_AMD_STAGE_INITIAL_QUANT_FIELD,
+  GJXL_AMD_STAGE_INITIAL_CFL,
+  GJXL_AMD_STAGE_AC_STRATEGY_COSTS,
+  GJXL_AMD_STAGE_AC_STRATEGY_SEARCH,
+  GJXL_AMD_STAGE_AQ_PREPARE,
+  GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE,
+  GJXL_AMD_STAGE_AQ_EVALUATE,
+  GJXL_AMD_STAGE_BUTTERAUGLI_COMPARE,
+  GJXL_AMD_STAGE_AQ_REDUCE_UPDATE,
+  GJXL_AMD_STAGE_AC_TOKENIZE,
+  GJXL_AMD_STAGE_TOKEN_READBACK,
+  GJXL_AMD_STAGE_COUNT
+};
+
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_options_init(
+  GJXLAmdPlainPathOptions* options, size_t struct_size);
+/* options may be null (defaults).  Allocates every device buffer of the
+ * extent once; runs reuse them. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_create(
+  GJXLAmdDevice* device, uint32_t width, uint32_t height,
+  const GJXLAmdPlainPathOptions* options, GJXLAmdPlainPath** out);
+GJXL_AMD_API void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* path);
+/* Runs the path on one image of the path's extent and returns when the
+ * tokens are on the host (tokens may be null: they stay on the device).
+ * frame may be null. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* path,
+                                                const GJXLAmdImage* image,
+                                                const GJXLAmdTokenOutput* tokens,
+                                                GJXLAmdPlainFrame* frame);
+/* Device time of each stage of the last completed run, from HIP events on
+ * the device stream: stages[i] (GJXL_AMD_STAGE_*) took milliseconds[i];
+ * *count receives the number of entries (evaluation stages repeat). */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_stage_times(
+  GJXLAmdPlainPath* path, int32_t* stages, float* milliseconds,
+  int32_t capacity, int32_t* count);
+
+/* Pinned host memory (hipHostMalloc) for inputs and token outputs. */
+GJXL_AMD_API GJXLResult gjxl_amd_host_alloc(size_t bytes, void** out);
+GJXL_AMD_API void gjxl_amd_host_free(void* pointer);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -95,3 +95,88 @@ def test_plain_path_matches_oracle_digests(device, name):
              "token_group_offsets"]
     differ = [k for k in order if got[k] != want[k]]
     assert not differ, f"{name}: differs from the oracle at {differ}"
+
+
+FRAME_KEYS = ("cells", "final_field", "raw_quant", "y_to_x", "y_to_b", "quantized",
+              "dc_quantized", "token_count", "token_values", "token_contexts",
+              "token_group_offsets", "score_bits")
+
+
+@pytest.mark.parametrize("name", ["512x512", "4032x3024"])
+def test_native_plain_path_matches_oracle_digests(device, name):
+    """The C++ sequence (gjxl_amd_plain_path_run: upload from pinned host
+    memory, every stage, token read-back) gives the oracle's frame and
+    tokens, on two runs in a row."""
+    want = GOLDEN[name]
+    spec = synthetic.WORKLOADS[name]
+    w, h = spec["width"], spec["height"]
+    print(f"native plain path {name}", flush=True)
+    srgb = synthetic.generate(**spec)
+    path = gjxl_amd.NativePlainPath(device, w, h)
+    for run in range(2):
+        count = path.run(srgb)
+        tokens = path.tokens()
+        got = {
+            "cells": digest(path.view("cells").numpy()),
+            "final_field": digest(path.view("quant_field").numpy()),
+            "raw_quant": digest(path.view("raw_quant").numpy()),
+            "y_to_x": digest(path.view("y_to_x").numpy()),
+            "y_to_b": digest(path.view("y_to_b").numpy()),
+            "quantized": digest(path.view("quantized").numpy()),
+            "dc_quantized": digest(path.view("dc_quantized").numpy()),
+            "token_count": count,
+            "token_values": digest(tokens["values"]),
+            "token_contexts": digest(tokens["contexts"]),
+            "token_group_offsets": digest(tokens["group_offsets"]),
+            "score_bits": [int(np.float32(v).view(np.uint32)) for v in path.scores()],
+        }
+        differ = [k for k in FRAME_KEYS if got[k] != want[k]]
+        assert not differ, f"{name} run {run}: differs from the oracle at {differ}"
+    stages = [s for s, _ in path.stage_times()]
+    assert stages[0] == "upload" and stages[-1] == "token_readback"
+    assert stages.count("aq_evaluate") == 3
+    path.close()
+
+
+def test_cpp_caller_matches_oracle_digests(tmp_path):
+    """A C++ program built with g++ against include/gjxl_amd.h alone
+    (tests/plain_path_caller.cpp) drives the path and writes its outputs."""
+    import subprocess
+
+    from gjxl_amd import build
+
+    name = "512x512"
+    want = GOLDEN[name]
+    spec = synthetic.WORKLOADS[name]
+    w, h = spec["width"], spec["height"]
+    srgb = synthetic.generate(**spec)
+    raw = tmp_path / "input.rgb"
+    raw.write_bytes(np.ascontiguousarray(srgb).tobytes())
+    prefix = tmp_path / "out"
+    assert build.CALLER.exists(), "run gjxl_amd.build.build_caller() first"
+    print("c++ caller 512x512", flush=True)
+    proc = subprocess.run([str(build.CALLER), str(w), str(h), str(raw), str(prefix), "2"],
+                          capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stderr
+    report = json.loads(proc.stdout.strip().splitlines()[-1])
+    bw, bh = w // 8, h // 8
+
+    def load(ext, dtype):
+        return np.fromfile(f"{prefix}.{ext}", dtype=dtype)
+
+    got = {
+        "cells": digest(load("cells", np.uint8).reshape(bh, bw)),
+        "final_field": digest(load("field", np.float32).reshape(bh, bw)),
+        "raw_quant": digest(load("raw_quant", np.int32).reshape(bh, bw)),
+        "y_to_x": digest(load("y_to_x", np.int8).reshape((bh + 7) // 8, (bw + 7) // 8)),
+        "y_to_b": digest(load("y_to_b", np.int8).reshape((bh + 7) // 8, (bw + 7) // 8)),
+        "quantized": digest(load("quantized", np.int32).reshape(3, -1)),
+        "dc_quantized": digest(load("dc_quantized", np.int32).reshape(3, bh, bw)),
+        "token_count": report["token_count"],
+        "token_values": digest(load("values", np.uint32)),
+        "token_contexts": digest(load("contexts", np.uint16)),
+        "token_group_offsets": digest(load("group_offsets", np.uint32)),
+        "score_bits": [int(np.float32(v).view(np.uint32)) for v in report["scores"]],
+    }
+    differ = [k for k in FRAME_KEYS if got[k] != want[k]]
+    assert not differ, f"C++ caller differs from the oracle at {differ}"

@@ -34,3 +34,18 @@ def test_python_binding_covers_header():
 
 def test_library_loads_through_package():
     assert gjxl_amd.library() is not None
+
+
+def test_cpp_caller_links_only_declared_symbols():
+    """tests/plain_path_caller.cpp builds with g++ against include/gjxl_amd.h
+    alone, and every library symbol it imports is declared there."""
+    import subprocess
+
+    from gjxl_amd import build
+
+    caller = build.build_caller()
+    out = subprocess.run(["nm", "-D", "--undefined-only", str(caller)], check=True,
+                         capture_output=True, text=True).stdout
+    imported = {line.split()[-1] for line in out.splitlines() if "gjxl_amd_" in line}
+    assert {"gjxl_amd_plain_path_run", "gjxl_amd_host_alloc"} <= imported
+    assert imported <= set(declared_symbols())
