@@ -75,6 +75,8 @@ _SIGNATURES = {
     "gjxl_amd_initial_quant_field_scratch_bytes": [_U32, _U32],
     "gjxl_amd_initial_quant_field":
         [_P, _P, _U32, _U32, _F, _F, _P, _P, _P, _P],
+    "gjxl_amd_initial_quant_field_no_alloc":
+        [_P, _P, _U32, _U32, _F, _F, _P, _P, _P],
     "gjxl_amd_initial_cfl": [_P, _P, _U32, _U32, _P, _P],
     "gjxl_amd_ac_strategy_costs":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P],

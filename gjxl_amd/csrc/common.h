@@ -41,11 +41,17 @@ struct GJXLAmdDevice {
   // Read-only tables uploaded at creation (see tables.h).
   float* quant_tables = nullptr;
   double* dct_bases = nullptr;
+  // Device address of the initial-quant-field static workspace on this
+  // device (frontend.hip), looked up at creation.
+  float* initial_quant_static_scratch = nullptr;
 };
 
 namespace gjxl_amd {
 
 GJXLResult ValidateDevice(const GJXLAmdDevice* device);
+// Device address of frontend.hip's static initial-quant workspace on the
+// current device.
+GJXLResult LookUpInitialQuantStaticScratch(float** out);
 
 inline uint32_t CeilDiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 inline uint32_t PadTo8(uint32_t v) { return CeilDiv(v, 8) * 8; }

@@ -66,6 +66,10 @@ GJXLResult gjxl_amd_device_create(int ordinal, GJXLAmdDevice** out) {
     result = Check(hipEventCreate(&device->events[i]), "hipEventCreate");
   }
   if (result == GJXL_OK) {
+    // with this ordinal current: each device has its own copy of the array
+    result = gjxl_amd::LookUpInitialQuantStaticScratch(&device->initial_quant_static_scratch);
+  }
+  if (result == GJXL_OK) {
     result = Check(hipMalloc(reinterpret_cast<void**>(&device->quant_tables),
                              sizeof(float) * gjxl_amd::kQuantTableFloats),
                    "hipMalloc(quant tables)");

@@ -371,6 +371,28 @@ size_t InitialQuantScratchFloats(int width, int height) {
          static_cast<size_t>(width / 4) * (height / 4);
 }
 
+// Workspace of gjxl_amd_initial_quant_field_no_alloc: the raw pixel mask and
+// the pre-erosion map of one padded image of at most
+// GJXL_AMD_NO_ALLOC_MAX_PIXELS pixels (17/16 floats per pixel, 17.8 MB).
+// Each device holds its own copy for the life of the process.  In host code
+// the name is not a device address; kernels receive the address that
+// LookUpInitialQuantStaticScratch obtained.
+constexpr size_t kInitialQuantStaticFloats =
+  static_cast<size_t>(GJXL_AMD_NO_ALLOC_MAX_PIXELS) +
+  static_cast<size_t>(GJXL_AMD_NO_ALLOC_MAX_PIXELS) / 16;
+static __device__ __align__(256) float initial_quant_scratch[kInitialQuantStaticFloats];
+
+GJXLResult LookUpInitialQuantStaticScratch(float** out) {
+  void* address = nullptr;
+  GJXL_AMD_TRY(Check(hipGetSymbolAddress(&address, HIP_SYMBOL(initial_quant_scratch)),
+                     "hipGetSymbolAddress(initial quant scratch)"));
+  if (address == nullptr) {
+    return Fail(GJXL_ERROR_BACKEND, "initial quant scratch has no device address");
+  }
+  *out = static_cast<float*>(address);
+  return GJXL_OK;
+}
+
 GJXLResult LaunchInitialQuantField(hipStream_t stream, const float* opsin,
                                    int width, int height, float target,
                                    float rescale, float* quant_field,
@@ -480,6 +502,40 @@ GJXLResult gjxl_amd_initial_quant_field(GJXLAmdDevice* device,
   return gjxl_amd::LaunchInitialQuantField(device->stream, opsin, width, height,
                                            target, rescale, quant_field,
                                            strategy_mask, pixel_mask, scratch);
+}
+
+GJXLResult gjxl_amd_initial_quant_field_no_alloc(GJXLAmdDevice* device,
+                                                 const float* opsin, uint32_t width,
+                                                 uint32_t height, float target,
+                                                 float rescale, float* quant_field,
+                                                 float* strategy_mask,
+                                                 float* pixel_mask) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  GJXL_AMD_TRY(ValidatePadded(width, height));
+  if (static_cast<uint64_t>(width) * height > GJXL_AMD_NO_ALLOC_MAX_PIXELS) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Initial quant field without allocation takes at most "
+                "GJXL_AMD_NO_ALLOC_MAX_PIXELS (2048 * 2048) padded pixels");
+  }
+  if (device->initial_quant_static_scratch == nullptr) {
+    return Fail(GJXL_ERROR_BACKEND, "Device handle has no static scratch address");
+  }
+  if (opsin == nullptr || quant_field == nullptr || strategy_mask == nullptr ||
+      pixel_mask == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Initial quant buffers must be non-null");
+  }
+  if (!std::isfinite(target) || target <= 0.0f || !std::isfinite(rescale) ||
+      rescale <= 0.0f) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "Initial quantization options must be finite and positive");
+  }
+  // Every workspace word the kernels read is written earlier in the same
+  // call (raw mask and pre-erosion map by PixelMaskKernel), so nothing is
+  // cleared.
+  return gjxl_amd::LaunchInitialQuantField(device->stream, opsin, width, height,
+                                           target, rescale, quant_field,
+                                           strategy_mask, pixel_mask,
+                                           device->initial_quant_static_scratch);
 }
 
 }  // extern "C"

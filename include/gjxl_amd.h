@@ -125,6 +125,22 @@ GJXL_AMD_API GJXLResult gjxl_amd_initial_quant_field(
   float butteraugli_target, float rescale, float* quant_field,
   float* strategy_mask, float* pixel_mask, float* scratch);
 
+/* No counterpart in the reference.  gjxl_amd_initial_quant_field for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernels, with
+ * the workspace in a static device array of the library instead of the
+ * scratch argument.  It calls no hipMalloc, hipFree or allocating library
+ * routine.  Limit: w * h <= GJXL_AMD_NO_ALLOC_MAX_PIXELS padded pixels;
+ * larger extents return GJXL_ERROR_INVALID_ARGUMENT.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two
+ * device handles, two host threads) race and must be serialized by the
+ * caller. */
+#define GJXL_AMD_NO_ALLOC_MAX_PIXELS (2048u * 2048u)
+GJXL_AMD_API GJXLResult gjxl_amd_initial_quant_field_no_alloc(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  float butteraugli_target, float rescale, float* quant_field,
+  float* strategy_mask, float* pixel_mask);
+
 /* ---------------------------------------------------------------------- */
 /* a6: initial chroma-from-luma map from the Gaborish-inverted XYB image.   */
 /* Replaces ComputeInitialColorCorrelationMap (src/codec/chroma_from_luma.  */
