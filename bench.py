@@ -1,20 +1,23 @@
 """Benchmark of the plain perceptual-analysis path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 4032x3024]
-                    [--images 3]
 
-A step is one pass of the path over each of `--images` copies of one
-synthetic 8-bit sRGB image whose packed pixels are already resident in HBM
-when the timed region starts.  Each image has its own device handle (its own
-HIP stream, default priority), buffers and host thread, so the images run
-concurrently: while one runs its DC wavefront (eight work groups), the other
-fills the remaining CUs.  `value` is the images' pixels per second of the
-timed region; the run fails if the images' results differ.
-Prints one JSON line (rank 0).  Per-stage kernel time is measured with HIP
-events on one image's stream in single-image steps after the timed region;
-`roofline` describes the dominant stage of those steps.
-`cpu_baseline` times the oracle (the CPU restatement) on the host on a bounded
-crop of the same image.
+A step is one call of gjxl_amd_plain_path_run (the C++ host sequence) on one
+synthetic 8-bit sRGB image at distance 1.0, effort 7: the packed pixels are
+resident in HBM when the timed region starts, and the step ends when the
+token values, contexts and group offsets are in pinned host memory.  `value`
+is pixels per second of the timed region (one image per step, one stream).
+Secondary figures in `config`: the same step with the upload from pinned
+host memory, and several images at once on their own device handles and
+host threads (`concurrent_images`).  Every run's tokens and scores must
+equal the first run's.
+
+`roofline`: the kernel family with the most device time per image is timed
+per launch with HIP events on the device stream over the whole timed region
+(gjxl_amd_device_probe); `achieved` is its algorithmic bytes (or binary64
+flops) per launch over the mean launch time.  `cpu_baseline` times the
+oracle (the CPU restatement) on the host on a bounded crop of the same
+image.
 """
 from __future__ import annotations
 
@@ -197,19 +200,6 @@ def committed_traffic(stage: str) -> int | None:
     return int(m.group(1)) if m else None
 
 
-def run_step(dev, path):
-    """Enqueues one pass; records event slot i+1 after stage i."""
-    names = []
-
-    def mark(name):
-        names.append(name)
-        dev.record(len(names))
-
-    dev.record(0)
-    path.run(path.srgb_device, mark=mark)
-    return names
-
-
 def cpu_baseline(srgb: np.ndarray, budget_s: float = 15.0) -> dict:
     """Times the oracle's whole path on the host on a crop of the image."""
     from oracle import binding
@@ -241,19 +231,91 @@ def cpu_baseline(srgb: np.ndarray, budget_s: float = 15.0) -> dict:
     }
 
 
+
+
+# Speed of the oracle against the reference's own CPU encoder, both timed in
+# the build container (8 x Xeon @ 2.70 GHz, one busy core each) on 512x512
+# synthetic images at d1.0 e7: the reference encoder (`gjxl_encode --backend
+# cpu`, whole encode including entropy coding) took 2.91 s per image, the
+# oracle's plain path 4.45 s (median of 3 after one warm-up, seed-1 "basic"
+# image).  The ratio carries the GPU host's oracle figure over to an estimate
+# of the reference encoder on the same host.
+REFERENCE_CPU_SECONDS_512 = 2.91
+ORACLE_CPU_SECONDS_512 = 4.45
+
+# Algorithmic bytes (or binary64 flops) of one launch of each probed kernel
+# family at image extent w x h.
+def probe_units(kernel: str, w: int, h: int) -> tuple[str, float]:
+    pn = ((w + 7) // 8 * 8) * ((h + 7) // 8 * 8)
+    blocks = pn // 64
+    if kernel == "DcQuantKernel":
+        # three DC planes in (float), three quantized planes out (int32)
+        return "bytes", 24.0 * blocks
+    if kernel == "DifferencesKernel":
+        # per scale (full and half resolution, one launch each; the mean
+        # launch covers 1.25 / 2 of the full-resolution pixels): 10 + 10
+        # psycho planes, both blurred activities, MaskY and MaskDcY in, the
+        # two final planes out
+        return "bytes", 104.0 * w * h * 1.25 / 2
+    if kernel == "CandidateCostKernel":
+        # binary64 DCT flops of all candidates over the 7 launches
+        return "flops", candidate_flops(w, h) / 7.0
+    raise KeyError(kernel)
+
+
+def _digest(path) -> str:
+    import hashlib
+    n = int(path.frame.token_count)
+    h = hashlib.sha256()
+    h.update(path.values.array[:n].tobytes())
+    h.update(path.contexts.array[:n].tobytes())
+    h.update(path.group_offsets.array.tobytes())
+    h.update(np.asarray(path.scores(), np.float32).tobytes())
+    return h.hexdigest()[:16]
+
+
+def _concurrent(gjxl_amd, srgb, w, h, images: int, steps: int) -> dict:
+    """`images` device handles (one stream each) and host threads running the
+    path on copies of the image at once; returns the aggregate rate and the
+    digests of every handle's last run."""
+    devs = [gjxl_amd.Device(0) for _ in range(images)]
+    paths = [gjxl_amd.NativePlainPath(d, w, h) for d in devs]
+    inputs = [d.array(srgb) for d in devs]
+
+    def one(i):
+        paths[i].run(resident=inputs[i])
+
+    with ThreadPoolExecutor(max_workers=images) as pool:
+        list(pool.map(one, range(images)))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            list(pool.map(one, range(images)))
+        elapsed = time.perf_counter() - t0
+    digests = [_digest(p) for p in paths]
+    for p in paths:
+        p.close()
+    for d in devs:
+        d.close()
+    return {"images_per_step": images, "streams": images, "steps": steps,
+            "ms_per_step": round(elapsed * 1e3 / steps, 4),
+            "value": round(images * steps * w * h / 1e6 / elapsed, 3),
+            "unit": "MP/s", "digests": digests}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=20)
-    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--steps", type=int, default=500)
+    parser.add_argument("--warmup", type=int, default=10)
     parser.add_argument("--workload", default="4032x3024")
-    parser.add_argument("--images", type=int, default=3,
-                        help="images per step, each on its own stream and host "
-                             "thread (the DC wavefront of one leaves most CUs "
-                             "to the others)")
-    parser.add_argument("--breakdown-steps", type=int, default=5,
-                        help="single-image steps after the timed region for "
-                             "the stage times and the roofline")
+    parser.add_argument("--probe", default="auto",
+                        help="kernel family timed per launch in the timed region "
+                             "(auto: the one with the most device time per image)")
+    parser.add_argument("--pcie-steps", type=int, default=50,
+                        help="steps with the upload from pinned host memory "
+                             "(secondary figure)")
+    parser.add_argument("--concurrent-images", type=int, default=3)
+    parser.add_argument("--concurrent-steps", type=int, default=60)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="run a device copy of known size first (counter runs)")
@@ -280,73 +342,96 @@ def main() -> int:
         calibration_copy()
     spec = synthetic.WORKLOADS[args.workload]
     w, h = spec["width"], spec["height"]
-    srgb = synthetic.generate(**spec)
-    images = max(1, args.images)
-    gjxl_amd.library()
-    # one device handle (its own stream), path and input per concurrent image
-    devs = [gjxl_amd.Device(0) for _ in range(images)]
-    paths = [gjxl_amd.pipeline.PlainPath(d, w, h) for d in devs]
-    for d, p in zip(devs, paths):
-        p.srgb_device = d.array(srgb)
-
-    def one_image(i):
-        paths[i].run(paths[i].srgb_device)
-        devs[i].synchronize()
-
-    with ThreadPoolExecutor(max_workers=images) as pool:
-        def step():
-            list(pool.map(one_image, range(images)))
-
-        for _ in range(args.warmup):
-            step()
-        for d in devs:
-            d.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        for d in devs:
-            d.synchronize()
-        elapsed = time.perf_counter() - t0
-
-    # stage times and the roofline from single-image steps on an otherwise
-    # idle device (not part of the timed region)
-    dev, path = devs[0], paths[0]
-    calls: dict[str, list[float]] = {}
-    for _ in range(max(1, args.breakdown_steps)):
-        names = run_step(dev, path)
-        dev.synchronize()
-        for i, name in enumerate(names):
-            calls.setdefault(name, []).append(dev.elapsed_ms(i, i + 1))
-    breakdown_steps = max(1, args.breakdown_steps)
-
-    ms_per_step = elapsed * 1e3 / args.steps
     mp = w * h / 1e6
-    per_step = {k: float(np.sum(v)) / breakdown_steps for k, v in calls.items()}
-    per_call = {k: float(np.mean(v)) for k, v in calls.items()}
-    dominant = max(per_step, key=per_step.get)
-    if dominant == "ac_strategy_costs":
-        flops = candidate_flops(w, h)
-        achieved = flops / (per_call[dominant] * 1e-3) / 1e12
-        roofline = {"kernel": "CandidateCostKernel (7 strategy launches)",
-                    "bound": "fp64", "achieved": round(achieved, 3),
-                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+    srgb = synthetic.generate(**spec)
+    gjxl_amd.library()
+    dev = gjxl_amd.Device(0)
+    path = gjxl_amd.NativePlainPath(dev, w, h)
+    resident = dev.array(srgb)        # packed sRGB8 in HBM
+    path.pixels.array[...] = srgb     # and in pinned host memory
+    print(f"bench: {args.workload}, warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(max(1, args.warmup)):
+        path.run(resident=resident)
+    reference_digest = _digest(path)
+
+    # the probed kernel family: the one with the most device time per image
+    # (one pass per family, before the timed region)
+    probe = args.probe
+    family_ms = {}
+    for name in dev.PROBE_KERNELS:
+        dev.probe(name)
+        path.run(resident=resident)
+        family_ms[name] = float(np.sum(dev.probe_times()))
+    if probe == "auto":
+        probe = max(family_ms, key=family_ms.get)
+    dev.probe(probe)
+
+    # timed region: one image per step, packed pixels resident in HBM, every
+    # stage, then the token values, contexts and group offsets read back
+    # into pinned host memory
+    print(f"bench: timing {args.steps} steps (probe {probe})", file=sys.stderr, flush=True)
+    stage_ms: dict[str, list[float]] = {}
+    launches: list[float] = []
+    dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        path.run(resident=resident)
+        launches.extend(dev.probe_times())
+        for name, ms in path.stage_times():
+            stage_ms.setdefault(name, []).append(ms)
+    dev.synchronize()
+    elapsed = time.perf_counter() - t0
+    dev.probe(None)
+    if _digest(path) != reference_digest:
+        print("timed runs disagree with the warm-up run", file=sys.stderr)
+        return 1
+    ms_per_step = elapsed * 1e3 / args.steps
+
+    # secondary: the same step with the upload from pinned host memory
+    t0 = time.perf_counter()
+    for _ in range(args.pcie_steps):
+        path.run()
+    pcie_s = time.perf_counter() - t0
+    if _digest(path) != reference_digest:
+        print("runs from host memory disagree", file=sys.stderr)
+        return 1
+    tokens = int(path.frame.token_count)
+    scores = path.scores()
+    path.close()
+
+    concurrent = None
+    if args.concurrent_images > 1:
+        print(f"bench: {args.concurrent_images} concurrent images", file=sys.stderr, flush=True)
+        concurrent = _concurrent(gjxl_amd, srgb, w, h, args.concurrent_images,
+                                 args.concurrent_steps)
+        if any(d != reference_digest for d in concurrent.pop("digests")):
+            print("concurrent images disagree", file=sys.stderr)
+            return 1
+
+    per_step = {k: float(np.sum(v)) / args.steps for k, v in stage_ms.items()}
+    kind, units = probe_units(probe, w, h)
+    launch_ms = float(np.mean(launches))
+    if kind == "flops":
+        achieved = units / (launch_ms * 1e-3) / 1e12
+        roofline = {"kernel": probe, "bound": "fp64 (VALU, no MFMA)",
+                    "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                     "traffic": None}
     else:
-        nbytes = stage_bytes(dominant, w, h, path.token_count)
-        achieved = nbytes / (per_call[dominant] * 1e-3) / 1e9
-        traffic = None
-        if args.workload == "4032x3024":
-            traffic = committed_traffic(dominant)
-        roofline = {"kernel": dominant, "bound": "hbm",
-                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                    "ms_per_launch": round(per_call[dominant], 4),
-                    "algorithmic_bytes": nbytes, "traffic": traffic}
+        achieved = units / (launch_ms * 1e-3) / 1e9
+        roofline = {"kernel": probe, "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                    "traffic": committed_traffic(probe) if args.workload == "4032x3024"
+                    else None}
+    roofline.update({"launches": len(launches), "ms_per_launch": round(launch_ms, 4),
+                     "ms_per_image": round(launch_ms * len(launches) / args.steps, 4),
+                     "algorithmic_" + kind + "_per_launch": round(units),
+                     "family_ms_per_image": {k: round(v, 4) for k, v in family_ms.items()}})
     result = {
         "metric": "MP/s at d1.0 e7, plain GPU path (XYB, AC strategy search, "
                   "Butteraugli AQ loop, AC tokenization), 1 GPU",
-        "value": round(images * mp / (ms_per_step * 1e-3), 3),
+        "value": round(mp / (ms_per_step * 1e-3), 3),
         "unit": "MP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -358,29 +443,32 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{w}x{h} 8-bit sRGB, seeded synthetic "
-                        f"({spec['recipe']}), distance 1.0 effort 7",
-            "images_per_step": images,
-            "streams": images,
-            "stages": list(per_step),
-            "stage_ms_one_image": {k: round(v, 5) for k, v in per_step.items()},
-            "ms_one_image": round(sum(per_step.values()), 4),
+            "workload": f"{w}x{h} 8-bit sRGB, seeded synthetic ({spec['recipe']}, "
+                        f"seed {spec['seed']}), distance 1.0 effort 7, one image per step",
+            "span": "packed sRGB8 resident in HBM -> every stage -> token values, "
+                    "contexts and group offsets in pinned host memory",
+            "stage_ms": {k: round(v, 4) for k, v in per_step.items()},
+            "ac_tokens": tokens,
+            "scores": [round(v, 6) for v in scores],
+            "pcie_inclusive": {
+                "span": "upload from pinned host memory included",
+                "steps": args.pcie_steps,
+                "ms_per_step": round(pcie_s * 1e3 / args.pcie_steps, 4),
+                "value": round(mp * args.pcie_steps / pcie_s, 3), "unit": "MP/s"},
+            "concurrent_images": concurrent,
         },
         "roofline": roofline,
     }
-    scores = [p.scores() for p in paths]
-    if any(sc != scores[0] for sc in scores) or any(
-            p.token_count != path.token_count for p in paths):
-        print("concurrent images disagree", file=sys.stderr)
-        return 1
-    result["config"]["scores"] = [round(v, 6) for v in scores[0]]
-    result["config"]["ac_tokens"] = path.token_count
-    for p in paths:
-        p.close()
+    dev.close()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(srgb)
-    for d in devs:
-        d.close()
+        cpu = cpu_baseline(srgb)
+        cpu["reference_cpu_estimate"] = {
+            "value": round(cpu["value"] * ORACLE_CPU_SECONDS_512 / REFERENCE_CPU_SECONDS_512, 4),
+            "unit": "MP/s",
+            "how": "oracle rate on this host x (oracle 4.45 s / reference CPU encoder "
+                   "2.91 s per 512x512 image at d1 e7, both timed on one build-container "
+                   "core)"}
+        result["cpu_baseline"] = cpu
     print(json.dumps(result))
     return 0
 

@@ -68,6 +68,9 @@ _SIGNATURES = {
     "gjxl_amd_last_error": [],
     "gjxl_amd_event_record": [_P, _I],
     "gjxl_amd_event_elapsed": [_P, _I, _I, ctypes.POINTER(_F)],
+    "gjxl_amd_device_probe": [_P, ctypes.c_int32],
+    "gjxl_amd_device_probe_times": [_P, _P, ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_int32)],
     "gjxl_amd_srgb8_to_linear_and_opsin":
         [_P, _P, _SZ, _U32, _U32, _U32, _F, _P, _P],
     "gjxl_amd_linear_rgb_to_padded_opsin": [_P, _P, _U32, _U32, _F, _P],
@@ -273,6 +276,24 @@ class Device:
         _check(library().gjxl_amd_event_elapsed(
             self.handle, begin, end, ctypes.byref(ms)))
         return ms.value
+
+    # kernel ids of probe() (GJXL_AMD_KERNEL_*)
+    PROBE_KERNELS = {"CandidateCostKernel": 1, "DcQuantKernel": 2,
+                     "DifferencesKernel": 3}
+
+    def probe(self, kernel: str | None) -> None:
+        """Times every launch of `kernel` (a PROBE_KERNELS name, or None to
+        stop) on this handle's stream from now on."""
+        kid = 0 if kernel is None else self.PROBE_KERNELS[kernel]
+        _check(library().gjxl_amd_device_probe(self.handle, kid))
+
+    def probe_times(self) -> list[float]:
+        """Milliseconds of each probed launch since the last call."""
+        ms = (ctypes.c_float * 64)()
+        count = ctypes.c_int32()
+        _check(library().gjxl_amd_device_probe_times(self.handle, ms, 64,
+                                                     ctypes.byref(count)))
+        return [ms[i] for i in range(count.value)]
 
     # ---- a1 + a2 --------------------------------------------------------
     def srgb8_to_linear_and_opsin(self, srgb: DeviceArray, width: int,
@@ -760,14 +781,23 @@ class NativePlainPath:
         except Exception:
             pass
 
-    def run(self, srgb: np.ndarray | None = None, read_tokens: bool = True) -> int:
+    def run(self, srgb: np.ndarray | None = None, read_tokens: bool = True,
+            resident: DeviceArray | None = None) -> int:
         """One pass over `srgb` (h, w, 3 uint8; copied into the pinned input
-        first) or over the pinned input as it stands; returns the token
-        count.  With read_tokens the stream lands in the pinned buffers."""
-        if srgb is not None:
-            self.pixels.array[...] = srgb
-        img = Image(self.width, self.height, 3, 0, self.pixels.ptr,
-                    self.width * self.height * 3, self.width * 3)
+        first), over the pinned input as it stands, or over `resident`
+        (packed pixels already in device memory: no upload); returns the
+        token count.  With read_tokens the stream lands in the pinned
+        buffers."""
+        if resident is not None:
+            if resident.nbytes < self.width * self.height * 3:
+                raise ValueError("resident input is smaller than the image")
+            img = Image(self.width, self.height, 3, 1, resident.ptr,
+                        resident.nbytes, self.width * 3)
+        else:
+            if srgb is not None:
+                self.pixels.array[...] = srgb
+            img = Image(self.width, self.height, 3, 0, self.pixels.ptr,
+                        self.width * self.height * 3, self.width * 3)
         out = TokenOutput(self.values.ptr, self.contexts.ptr, self.group_offsets.ptr,
                           self.values.shape[0], self.groups + 1)
         _check(library().gjxl_amd_plain_path_run(

@@ -918,10 +918,12 @@ GJXLResult LaunchCost(GJXLAmdDevice* device, const float* opsin, int width,
   using S = CostShape<kStrategy>;
   const int bw = width / 8, bh = height / 8;
   const dim3 grid(CeilDiv(S::kPositions, S::kCands), CeilDiv(bw, 8) * CeilDiv(bh, 8));
+  const bool probed = ProbeBegin(device, GJXL_AMD_KERNEL_CANDIDATE_COST);
   hipLaunchKernelGGL(CandidateCostKernel<kStrategy>, grid, dim3(kCostThreads),
                      0, device->stream, opsin, width, height, qf, mask, yx, yb,
                      device->quant_tables, device->dct_bases, params,
                      stage_costs);
+  ProbeEnd(device, probed);
   return CheckLaunch("CandidateCostKernel");
 }
 

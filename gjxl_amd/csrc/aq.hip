@@ -1674,8 +1674,10 @@ GJXLResult LaunchCodeCoefficients(GJXLAmdDevice* d, const float* coefs,
     return CheckLaunch("CodeTransformKernel");
   }));
   const dim3 groups(CeilDiv(l.bw, kDcGroup), CeilDiv(l.bh, kDcGroup), 2);
+  const bool probed = ProbeBegin(d, GJXL_AMD_KERNEL_DC_QUANT);
   hipLaunchKernelGGL(DcQuantKernel, groups, dim3(kDcWorkGroup), 0, d->stream, dc, l.bw,
                      l.bh, q, 1, dc_quantized);
+  ProbeEnd(d, probed);
   GJXL_AMD_TRY(CheckLaunch("DcQuantKernel"));
   hipLaunchKernelGGL(DcReconstructKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)),
                      dim3(256), 0, d->stream, dc_quantized, n, q, 1, dc_rec);

@@ -1281,9 +1281,11 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
   GJXL_AMD_TRY(MaskActivity(b, s, s.dist_psycho, s.tmp[2]));
   const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
                    CeilDiv(static_cast<uint32_t>(h), kMaltaTileH));
+  const bool probed = ProbeBegin(b->device, GJXL_AMD_KERNEL_BUTTERAUGLI_DIFFERENCES);
   hipLaunchKernelGGL(DifferencesKernel, tiles, dim3(kThreads), 0, st, s.ref_psycho,
                      s.dist_psycho, w, h, ms, lw, s.ref_blurred, s.tmp[2], s.mask_y,
                      s.mask_dc, b->params.x_multiplier, s.stages);
+  ProbeEnd(b->device, probed);
   return CheckLaunch("DifferencesKernel");
 }
 

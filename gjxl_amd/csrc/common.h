@@ -44,6 +44,11 @@ struct GJXLAmdDevice {
   // Device address of the initial-quant-field static workspace on this
   // device (frontend.hip), looked up at creation.
   float* initial_quant_static_scratch = nullptr;
+  // Launch timing of one kernel family (gjxl_amd_device_probe).
+  int32_t probe_kernel = GJXL_AMD_KERNEL_NONE;
+  int32_t probe_count = 0;
+  hipEvent_t probe_begin[GJXL_AMD_PROBE_SLOTS] = {};
+  hipEvent_t probe_end[GJXL_AMD_PROBE_SLOTS] = {};
 };
 
 namespace gjxl_amd {
@@ -52,6 +57,21 @@ GJXLResult ValidateDevice(const GJXLAmdDevice* device);
 // Device address of frontend.hip's static initial-quant workspace on the
 // current device.
 GJXLResult LookUpInitialQuantStaticScratch(float** out);
+
+// Begin and end of one launch of `kernel` on the device stream, recorded
+// when that family is being probed.  Usage:
+//   const bool probed = ProbeBegin(d, GJXL_AMD_KERNEL_X);
+//   hipLaunchKernelGGL(...);
+//   ProbeEnd(d, probed);
+inline bool ProbeBegin(GJXLAmdDevice* d, int32_t kernel) {
+  if (d->probe_kernel != kernel || d->probe_count >= GJXL_AMD_PROBE_SLOTS) return false;
+  return hipEventRecord(d->probe_begin[d->probe_count], d->stream) == hipSuccess;
+}
+inline void ProbeEnd(GJXLAmdDevice* d, bool probed) {
+  if (probed && hipEventRecord(d->probe_end[d->probe_count], d->stream) == hipSuccess) {
+    ++d->probe_count;
+  }
+}
 
 inline uint32_t CeilDiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 inline uint32_t PadTo8(uint32_t v) { return CeilDiv(v, 8) * 8; }

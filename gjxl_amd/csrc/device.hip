@@ -110,6 +110,10 @@ void gjxl_amd_device_destroy(GJXLAmdDevice* device) {
   for (hipEvent_t& event : device->events) {
     if (event != nullptr) (void)hipEventDestroy(event);
   }
+  for (int i = 0; i < GJXL_AMD_PROBE_SLOTS; ++i) {
+    if (device->probe_begin[i] != nullptr) (void)hipEventDestroy(device->probe_begin[i]);
+    if (device->probe_end[i] != nullptr) (void)hipEventDestroy(device->probe_end[i]);
+  }
   if (device->quant_tables != nullptr) (void)hipFree(device->quant_tables);
   if (device->dct_bases != nullptr) (void)hipFree(device->dct_bases);
   if (device->stream != nullptr) (void)hipStreamDestroy(device->stream);
@@ -197,6 +201,44 @@ GJXLResult gjxl_amd_event_elapsed(GJXLAmdDevice* device, int begin_slot,
   return Check(hipEventElapsedTime(milliseconds, device->events[begin_slot],
                                    device->events[end_slot]),
                "hipEventElapsedTime");
+}
+
+GJXLResult gjxl_amd_device_probe(GJXLAmdDevice* device, int32_t kernel) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (kernel < GJXL_AMD_KERNEL_NONE || kernel >= GJXL_AMD_KERNEL_COUNT) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Probe kernel id is out of range");
+  }
+  // events are created on first use (timing enabled: elapsed times are read)
+  for (int i = 0; kernel != GJXL_AMD_KERNEL_NONE && i < GJXL_AMD_PROBE_SLOTS; ++i) {
+    if (device->probe_begin[i] == nullptr) {
+      GJXL_AMD_TRY(Check(hipEventCreate(&device->probe_begin[i]), "hipEventCreate(probe)"));
+    }
+    if (device->probe_end[i] == nullptr) {
+      GJXL_AMD_TRY(Check(hipEventCreate(&device->probe_end[i]), "hipEventCreate(probe)"));
+    }
+  }
+  device->probe_kernel = kernel;
+  device->probe_count = 0;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_device_probe_times(GJXLAmdDevice* device, float* milliseconds,
+                                       int32_t capacity, int32_t* count) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (count == nullptr || capacity < 0 || (capacity > 0 && milliseconds == nullptr)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Probe time outputs are invalid");
+  }
+  const int n = device->probe_count;
+  if (n > 0) {
+    GJXL_AMD_TRY(Check(hipEventSynchronize(device->probe_end[n - 1]), "hipEventSynchronize"));
+  }
+  for (int i = 0; i < n && i < capacity; ++i) {
+    GJXL_AMD_TRY(Check(hipEventElapsedTime(&milliseconds[i], device->probe_begin[i],
+                                           device->probe_end[i]), "hipEventElapsedTime"));
+  }
+  *count = n < capacity ? n : capacity;
+  device->probe_count = 0;
+  return GJXL_OK;
 }
 
 }  // extern "C"

@@ -79,6 +79,29 @@ GJXL_AMD_API GJXLResult gjxl_amd_event_elapsed(GJXLAmdDevice* device,
                                                int begin_slot, int end_slot,
                                                float* milliseconds);
 
+/* Per-launch timing of one kernel family inside any sequence on this device
+ * handle (the plain path included), for roofline figures: while a kernel id
+ * is selected, each launch of that family records a begin and an end event
+ * on the device stream (up to GJXL_AMD_PROBE_SLOTS launches between reads).
+ * No counterpart in the reference, which times stages on the host.
+ * gjxl_amd_device_probe selects (0 = none) and clears; probe_times waits for
+ * the recorded launches, writes their durations in launch order and
+ * clears. */
+enum {
+  GJXL_AMD_KERNEL_NONE = 0,
+  GJXL_AMD_KERNEL_CANDIDATE_COST = 1,           /* one launch per strategy */
+  GJXL_AMD_KERNEL_DC_QUANT = 2,                 /* the DC wavefront */
+  GJXL_AMD_KERNEL_BUTTERAUGLI_DIFFERENCES = 3,  /* one launch per scale */
+  GJXL_AMD_KERNEL_COUNT = 4
+};
+enum { GJXL_AMD_PROBE_SLOTS = 64 };
+GJXL_AMD_API GJXLResult gjxl_amd_device_probe(GJXLAmdDevice* device,
+                                              int32_t kernel);
+GJXL_AMD_API GJXLResult gjxl_amd_device_probe_times(GJXLAmdDevice* device,
+                                                    float* milliseconds,
+                                                    int32_t capacity,
+                                                    int32_t* count);
+
 /* ---------------------------------------------------------------------- */
 /* a1 + a2: packed sRGB8 -> linear RGB planes and padded XYB planes.        */
 /* Replaces PrepareResidentInput (src/gpu/cuda/cuda_input_preparation.cpp:  */
