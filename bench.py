@@ -302,6 +302,130 @@ def _concurrent(gjxl_amd, srgb, w, h, images: int, steps: int) -> dict:
             "unit": "MP/s", "digests": digests}
 
 
+def batch_workload(count: int, seed: int = 4) -> list[np.ndarray]:
+    """`count` images whose width and height are drawn uniformly from 256 to
+    4096 in steps of 8, cropped at random offsets from one seeded 4096x4096
+    synthetic image ("textured", seed 2)."""
+    from gjxl_amd import synthetic
+    rng = np.random.default_rng(seed)
+    source = synthetic.generate(4096, 4096, seed=2, recipe="textured")
+    out = []
+    for _ in range(count):
+        w, h = (int(v) for v in rng.integers(32, 513, size=2) * 8)
+        x0, y0 = int(rng.integers(0, 4096 - w + 1)), int(rng.integers(0, 4096 - h + 1))
+        out.append(np.ascontiguousarray(source[y0:y0 + h, x0:x0 + w]))
+    return out
+
+
+def _batch(gjxl_amd, images: list[np.ndarray], lanes: int) -> tuple[float, list[int]]:
+    """Seconds for one encode call of the batch driver with `lanes` workers,
+    and every image's token count."""
+    batch = gjxl_amd.Batch(0, lanes)
+    t0 = time.perf_counter()
+    results = batch.encode(images)
+    seconds = time.perf_counter() - t0
+    counts = []
+    for i in range(len(images)):
+        if results[i].status != 0:
+            raise RuntimeError(f"batch image {i}: {results[i].message.decode()}")
+        counts.append(int(results[i].token_count))
+    batch.free_results(results, len(images))
+    batch.close()
+    return seconds, counts
+
+
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def _dct_basis(n: int) -> np.ndarray:
+    f = np.arange(n)[:, None]
+    b = np.sqrt(2.0) / n * np.cos((np.arange(n)[None, :] + 0.5) * f * np.pi / n)
+    b[0] /= np.sqrt(2.0)
+    return b
+
+
+def split_dct_errors(gjxl_amd, dev, count: int = 128) -> dict:
+    """Maximum error of the split-precision DCT against a binary64 DCT, per
+    side pair, on seeded blocks of XYB-like magnitudes (relative to each
+    block's largest pixel)."""
+    lib = gjxl_amd.library()
+    out = {}
+    for w in (8, 16, 32, 64):
+        for h in (8, 16, 32, 64):
+            rng = np.random.default_rng(w * 100 + h)
+            ramp = np.linspace(0, 1, w)[None, None, :] + np.linspace(0, 1, h)[None, :, None]
+            scale = rng.choice([1.0, 0.05, 1e-3], size=(count, 1, 1))
+            blocks = (scale * (0.3 * ramp + rng.normal(0, 0.2, size=(count, h, w)))
+                      ).astype(np.float32)
+            d_in = dev.array(blocks)
+            d_out = dev.empty((count, w * h))
+            gjxl_amd._check(lib.gjxl_amd_split_dct_blocks(dev.handle, w, h, d_in.ptr, count,
+                                                          d_out.ptr))
+            got = d_out.numpy()
+            c = np.einsum("vy,kyx,ux->kvu", _dct_basis(h), blocks.astype(np.float64),
+                          _dct_basis(w))
+            want = c.reshape(count, -1) if h < w else np.transpose(c, (0, 2, 1)).reshape(count, -1)
+            peak = np.abs(blocks).reshape(count, -1).max(axis=1)[:, None]
+            out[f"{w}x{h}"] = float((np.abs(got - want) / peak).max())
+    return out
+
+
+def split_dct_figures(gjxl_amd, dev, srgb: np.ndarray, distances) -> dict:
+    """The split-precision forward DCT of every transform of the frame (the
+    chosen precision form): its time per frame from HIP events on the device
+    stream, HBM bytes and MFMA flops against their peaks, its error against
+    the plain path's binary64-basis coefficients, and the share of quantized
+    coefficients that change when the final evaluation codes them, per
+    distance."""
+    h, w = srgb.shape[:2]
+    lib = gjxl_amd.library()
+    out = {"max_error_per_block_peak": split_dct_errors(gjxl_amd, dev), "per_distance": []}
+    for d in distances:
+        path = gjxl_amd.pipeline.PlainPath(dev, w, h, target=d)
+        path.run(dev.array(srgb))
+        s = path.session
+        store = ((s.bw + 31) // 32) * ((s.bh + 31) // 32) * 65536
+        coefs = s.buffer("coefficients", (3, store))
+        plain = coefs.numpy()
+        q_plain = path.quantized.numpy()
+        reps = 10
+        dev.synchronize()
+        dev.record(0)
+        for _ in range(reps):
+            gjxl_amd._check(lib.gjxl_amd_aq_split_forward_dct(s.handle))
+        dev.record(1)
+        ms = dev.elapsed_ms(0, 1) / reps
+        split = coefs.numpy()
+        s.evaluate(path.field)
+        q_split = path.quantized.numpy()
+        changed = int(np.count_nonzero(q_split != q_plain))
+        row = {"distance": d, "ms": round(ms, 4),
+               "max_abs_error_vs_plain": float(np.abs(split - plain).max()),
+               "quantized_mismatch": changed,
+               "quantized_mismatch_rate": changed / q_plain.size}
+        if d == 1.0:
+            cells = path.cells.numpy()
+            anchors = cells[(cells & 1) == 1] >> 1
+            sides = {0: (8, 8), 4: (16, 16), 5: (32, 32), 6: (8, 16), 7: (16, 8),
+                     10: (16, 32), 11: (32, 16)}
+            flops = 0
+            for st, n in zip(*np.unique(anchors, return_counts=True)):
+                tw, th = sides[int(st)]
+                pw_, ph_ = max(tw, 16), max(th, 16)
+                # 3 channels x 3 split products x 2 flops per multiply-add
+                flops += int(n) * 3 * 3 * 2 * (ph_ * pw_ * pw_ + ph_ * ph_ * pw_)
+            nbytes = 2 * 3 * 4 * s.pw * s.ph  # pixels in, coefficients out
+            out.update({"frame": f"{w}x{h}", "mfma_flops": flops, "algorithmic_bytes": nbytes,
+                        "achieved_gbps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                        "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                        "achieved_mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+                        "mfma_frac": round(flops / (ms * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS,
+                                           4)})
+        out["per_distance"].append(row)
+        path.close()
+    return out
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -316,6 +440,12 @@ def main() -> int:
                              "(secondary figure)")
     parser.add_argument("--concurrent-images", type=int, default=3)
     parser.add_argument("--concurrent-steps", type=int, default=60)
+    parser.add_argument("--batch-images", type=int, default=64,
+                        help="images of uneven extents for the batch-driver figure "
+                             "(0: skip)")
+    parser.add_argument("--batch-lanes", type=int, default=3)
+    parser.add_argument("--split-distances", default="0.5,1,2,3",
+                        help="distances of the split-precision DCT figures ('' to skip)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="run a device copy of known size first (counter runs)")
@@ -408,6 +538,31 @@ def main() -> int:
             print("concurrent images disagree", file=sys.stderr)
             return 1
 
+    batch = None
+    if args.batch_images > 0:
+        print(f"bench: batch of {args.batch_images} uneven images", file=sys.stderr, flush=True)
+        images = batch_workload(args.batch_images)
+        total_mp = sum(a.shape[0] * a.shape[1] for a in images) / 1e6
+        _batch(gjxl_amd, images[:2], args.batch_lanes)  # warm-up
+        alone_s, alone_counts = _batch(gjxl_amd, images, 1)
+        lanes_s, lanes_counts = _batch(gjxl_amd, images, args.batch_lanes)
+        if alone_counts != lanes_counts:
+            print("batch lanes disagree", file=sys.stderr)
+            return 1
+        batch = {"images": args.batch_images, "megapixels": round(total_mp, 2),
+                 "extents": "width, height uniform in [256, 4096] step 8",
+                 "one_at_a_time": {"seconds": round(alone_s, 4),
+                                   "value": round(total_mp / alone_s, 3), "unit": "MP/s"},
+                 f"lanes_{args.batch_lanes}": {"seconds": round(lanes_s, 4),
+                                               "value": round(total_mp / lanes_s, 3),
+                                               "unit": "MP/s"}}
+
+    split = None
+    if args.split_distances:
+        print("bench: split-precision DCT", file=sys.stderr, flush=True)
+        split = split_dct_figures(gjxl_amd, dev, srgb,
+                                  [float(v) for v in args.split_distances.split(",")])
+
     per_step = {k: float(np.sum(v)) / args.steps for k, v in stage_ms.items()}
     kind, units = probe_units(probe, w, h)
     launch_ms = float(np.mean(launches))
@@ -456,6 +611,8 @@ def main() -> int:
                 "ms_per_step": round(pcie_s * 1e3 / args.pcie_steps, 4),
                 "value": round(mp * args.pcie_steps / pcie_s, 3), "unit": "MP/s"},
             "concurrent_images": concurrent,
+            "batch_driver": batch,
+            "split_precision_dct": split,
         },
         "roofline": roofline,
     }

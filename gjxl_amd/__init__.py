@@ -113,6 +113,10 @@ _SIGNATURES = {
     "gjxl_amd_tokenizer_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
                                   ctypes.POINTER(_SZ)],
     "gjxl_amd_aq_check": [_P],
+    "gjxl_amd_tokenizer_frame_statistics": [_P, _P, _P, _P, _P],
+    "gjxl_amd_entropy_inputs_from_statistics": [_P, ctypes.c_int32, _P],
+    "gjxl_amd_tokenizer_set_entropy_inputs": [_P, _P],
+    "gjxl_amd_coefficient_order_tokens": [_P, _P, _P, _SZ, ctypes.POINTER(_SZ)],
     "gjxl_amd_plain_path_options_init": [_P, _SZ],
     "gjxl_amd_plain_path_create": [_P, _U32, _U32, _P, ctypes.POINTER(_P)],
     "gjxl_amd_plain_path_destroy": [_P],
@@ -121,10 +125,20 @@ _SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_int32)],
     "gjxl_amd_host_alloc": [_SZ, ctypes.POINTER(_P)],
     "gjxl_amd_host_free": [_P],
+    "gjxl_amd_split_dct_blocks": [_P, _U32, _U32, _P, _SZ, _P],
+    "gjxl_amd_aq_split_forward_dct": [_P],
+    "gjxl_amd_batch_create": [_I, _U32, ctypes.POINTER(_P)],
+    "gjxl_amd_batch_max_in_flight": [_P],
+    "gjxl_amd_batch_encode": [_P, _P, _SZ, _P],
+    "gjxl_amd_batch_results_free": [_P, _SZ],
+    "gjxl_amd_batch_shutdown": [_P],
+    "gjxl_amd_batch_destroy": [_P],
 }
 _VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
          "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy",
-         "gjxl_amd_plain_path_destroy", "gjxl_amd_host_free"}
+         "gjxl_amd_plain_path_destroy", "gjxl_amd_host_free",
+         "gjxl_amd_batch_results_free", "gjxl_amd_batch_shutdown",
+         "gjxl_amd_batch_destroy"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
 
@@ -143,6 +157,8 @@ def _declare(lib: ctypes.CDLL) -> None:
             fn.restype = ctypes.c_char_p
         elif name in _SIZE:
             fn.restype = ctypes.c_size_t
+        elif name == "gjxl_amd_batch_max_in_flight":
+            fn.restype = ctypes.c_uint32
         else:
             fn.restype = ctypes.c_int32
 
@@ -555,9 +571,77 @@ def reduce_distance_map(device: Device, distance_map: DeviceArray,
 TOKEN_BUFFERS = (
     "values", "contexts", "group_offsets", "histogram", "nonzero_map", "tasks",
     "task_offsets", "group_tokens", "offsets", "anchor_list", "anchor_count",
-    "orders", "inverse_orders", "status",
+    "orders", "inverse_orders", "status", "statistics", "sample_decisions",
 )
 ORDER_FAMILIES = 13
+# searched order families: (family, representative strategy, counter offset,
+# size, LLF size); counters of one channel are GJXLAmdFrameStatistics'
+FAMILIES = ((0, 0, 0, 64, 1), (4, 6, 64, 128, 2), (2, 4, 192, 256, 4),
+            (6, 10, 448, 512, 8), (3, 5, 960, 1024, 16))
+
+
+class FrameStatistics(ctypes.Structure):
+    """GJXLAmdFrameStatistics."""
+    _fields_ = [("block_width", _U32), ("block_height", _U32),
+                ("present_mask", ctypes.c_uint16),
+                ("zero_counts", (_U32 * 1984) * 3),
+                ("sampled_dct8_zero_counts", (_U32 * 64) * 3),
+                ("raw_quant_counts", (_U32 * 256) * ORDER_FAMILIES)]
+
+    def arrays(self) -> dict:
+        return {"present_mask": int(self.present_mask),
+                "zero_counts": np.ctypeslib.as_array(self.zero_counts).copy(),
+                "sampled": np.ctypeslib.as_array(self.sampled_dct8_zero_counts).copy(),
+                "raw_counts": np.ctypeslib.as_array(self.raw_quant_counts).copy()}
+
+
+class EntropyInputs(ctypes.Structure):
+    """GJXLAmdEntropyInputs: coefficient orders and block-context map."""
+    _fields_ = [("used_order_mask", ctypes.c_uint16),
+                ("orders", ((_U32 * 1024) * 3) * ORDER_FAMILIES),
+                ("qf_threshold_count", _U32), ("qf_thresholds", _U32 * 15),
+                ("num_contexts", _U32),
+                ("context_map", ctypes.c_uint8 * (3 * ORDER_FAMILIES * 16))]
+
+    def order_table(self) -> np.ndarray:
+        """The orders as the tokenizer's 7-stage table (3 x 2624, natural
+        where the family is unused)."""
+        stage_strategies = (0, 6, 7, 4, 10, 11, 5)
+        family_of = {0: 0, 6: 4, 7: 4, 4: 2, 10: 6, 11: 6, 5: 3}
+        orders = np.ctypeslib.as_array(self.orders)
+        parts = []
+        for st in stage_strategies:
+            f = family_of[st]
+            nat = natural_order(st)
+            for c in range(3):
+                parts.append(orders[f, c, :nat.size] if self.used_order_mask >> f & 1
+                             else nat)
+        return np.concatenate(parts).astype(np.uint32)
+
+    def context_model(self) -> tuple[np.ndarray, np.ndarray, int]:
+        n = int(self.qf_threshold_count)
+        entries = 3 * ORDER_FAMILIES * (n + 1)
+        return (np.array(self.qf_thresholds[:n], np.uint32),
+                np.array(self.context_map[:entries], np.uint8), int(self.num_contexts))
+
+
+def entropy_inputs(statistics: FrameStatistics, effort: int = 7) -> EntropyInputs:
+    """gjxl_amd_entropy_inputs_from_statistics (host code)."""
+    out = EntropyInputs()
+    _check(library().gjxl_amd_entropy_inputs_from_statistics(
+        ctypes.byref(statistics), effort, ctypes.byref(out)))
+    return out
+
+
+def order_tokens(inputs: EntropyInputs) -> tuple[np.ndarray, np.ndarray]:
+    """gjxl_amd_coefficient_order_tokens: (contexts, values)."""
+    cap = 3 * 1984 + 15
+    ctx = np.zeros(cap, np.uint32)
+    val = np.zeros(cap, np.uint32)
+    n = ctypes.c_size_t()
+    _check(library().gjxl_amd_coefficient_order_tokens(
+        ctypes.byref(inputs), ctx.ctypes.data, val.ctypes.data, cap, ctypes.byref(n)))
+    return ctx[:n.value].copy(), val[:n.value].copy()
 
 
 def natural_order(strategy: int) -> np.ndarray:
@@ -616,6 +700,18 @@ class Tokenizer:
             self.handle, thr.ctypes.data if thr.size else None, thr.size,
             cmap.ctypes.data, num_contexts))
         self.num_contexts = num_contexts
+
+    def frame_statistics(self, quantized: DeviceArray, cells: DeviceArray,
+                         raw_quant: DeviceArray) -> FrameStatistics:
+        out = FrameStatistics()
+        _check(library().gjxl_amd_tokenizer_frame_statistics(
+            self.handle, quantized.ptr, cells.ptr, raw_quant.ptr, ctypes.byref(out)))
+        return out
+
+    def set_entropy_inputs(self, inputs: EntropyInputs) -> None:
+        _check(library().gjxl_amd_tokenizer_set_entropy_inputs(self.handle,
+                                                               ctypes.byref(inputs)))
+        self.num_contexts = int(inputs.num_contexts)
 
     def tokenize(self, quantized: DeviceArray, cells: DeviceArray,
                  raw_quant: DeviceArray, populations: bool = False) -> int:
@@ -697,13 +793,15 @@ class PlainFrame(ctypes.Structure):
                 ("token_contexts", _P), ("token_group_offsets", _P),
                 ("token_count", ctypes.c_uint64), ("x_qm_scale", ctypes.c_int32),
                 ("b_qm_scale", ctypes.c_int32), ("quant_dc", _F),
-                ("evaluations", ctypes.c_int32), ("scores", _F * 8)]
+                ("evaluations", ctypes.c_int32), ("scores", _F * 8),
+                ("entropy_inputs", _P)]
 
 
 STAGE_NAMES = ("begin", "upload", "srgb_to_linear_and_opsin", "gaborish_inverse",
                "initial_quant_field", "initial_cfl", "ac_strategy_costs",
                "ac_strategy_search", "aq_prepare", "butteraugli_prepare", "aq_evaluate",
-               "butteraugli_compare", "aq_reduce_update", "ac_tokenize", "token_readback")
+               "butteraugli_compare", "aq_reduce_update", "ac_tokenize", "entropy_inputs",
+               "token_readback")
 
 
 class HostBuffer:
@@ -814,6 +912,14 @@ class NativePlainPath:
     def scores(self) -> list[float]:
         return [float(self.frame.scores[i]) for i in range(self.frame.evaluations)]
 
+    def entropy_inputs(self) -> EntropyInputs:
+        """A copy of the last run's coefficient orders and block-context map."""
+        if not self.frame.entropy_inputs:
+            raise RuntimeError("the plain path has not completed a run")
+        out = EntropyInputs()
+        ctypes.memmove(ctypes.byref(out), self.frame.entropy_inputs, ctypes.sizeof(out))
+        return out
+
     def view(self, name: str) -> DeviceArray:
         """A device view of one frame buffer of the last run."""
         bw, bh, store = self.bw, self.bh, self.groups * 65536
@@ -834,3 +940,95 @@ class NativePlainPath:
         _check(library().gjxl_amd_plain_path_stage_times(self.handle, stages, ms, 64,
                                                          ctypes.byref(count)))
         return [(STAGE_NAMES[stages[i]], float(ms[i])) for i in range(min(count.value, 64))]
+
+
+# ---- batch driver (gjxl_amd_batch_*) -------------------------------------------
+
+class BatchRequest(ctypes.Structure):
+    _fields_ = [("image", Image), ("options", PlainPathOptions)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("message", ctypes.c_char * 256),
+                ("width", _U32), ("height", _U32), ("groups", _U32),
+                ("token_count", ctypes.c_uint64), ("token_values", _P),
+                ("token_contexts", _P), ("token_group_offsets", _P),
+                ("entropy_inputs", _P), ("evaluations", ctypes.c_int32),
+                ("scores", _F * 8), ("queue_nanoseconds", ctypes.c_uint64),
+                ("service_nanoseconds", ctypes.c_uint64),
+                ("ready_nanoseconds", ctypes.c_uint64)]
+
+    def tokens(self) -> dict:
+        n = int(self.token_count)
+
+        def grab(ptr, count, ctype, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array((ctype * count).from_address(ptr)).copy()
+
+        return {"values": grab(self.token_values, n, ctypes.c_uint32, np.uint32),
+                "contexts": grab(self.token_contexts, n, ctypes.c_uint16, np.uint16),
+                "group_offsets": grab(self.token_group_offsets, int(self.groups) + 1,
+                                      ctypes.c_uint32, np.uint32)}
+
+    def scores_list(self) -> list[float]:
+        return [float(self.scores[i]) for i in range(self.evaluations)]
+
+    def entropy(self) -> EntropyInputs:
+        out = EntropyInputs()
+        ctypes.memmove(ctypes.byref(out), self.entropy_inputs, ctypes.sizeof(out))
+        return out
+
+
+class Batch:
+    """The batch driver: max_in_flight workers, each with its own device
+    handle and stream, running the plain path on independent images."""
+
+    def __init__(self, ordinal: int = 0, max_in_flight: int = 3):
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_batch_create(ordinal, max_in_flight,
+                                                ctypes.byref(handle)))
+        self.handle = handle.value
+
+    @property
+    def max_in_flight(self) -> int:
+        return int(library().gjxl_amd_batch_max_in_flight(self.handle))
+
+    def encode(self, images: list[np.ndarray], options: list[dict] | None = None,
+               results=None):
+        """images: (h, w, 3|4) uint8 arrays; returns the BatchResult array
+        (free it with free_results).  Raises on a driver-level failure."""
+        count = len(images)
+        keep = [np.ascontiguousarray(a, np.uint8) for a in images]
+        reqs = (BatchRequest * max(count, 1))()
+        for i, a in enumerate(keep):
+            h, w, bpp = a.shape
+            reqs[i].image = Image(w, h, bpp, 0, a.ctypes.data, a.nbytes, w * bpp)
+            o = (options or [{}] * count)[i]
+            if o:
+                reqs[i].options = PlainPathOptions(ctypes.sizeof(PlainPathOptions),
+                                                   o.get("distance", 1.0), o.get("effort", 7),
+                                                   o.get("intensity_target", 255.0))
+        out = results if results is not None else (BatchResult * max(count, 1))()
+        _check(library().gjxl_amd_batch_encode(self.handle, reqs, count, out))
+        return out
+
+    @staticmethod
+    def free_results(results, count: int) -> None:
+        library().gjxl_amd_batch_results_free(results, count)
+
+    def shutdown(self) -> None:
+        if self.handle:
+            library().gjxl_amd_batch_shutdown(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

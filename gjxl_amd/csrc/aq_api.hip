@@ -183,7 +183,7 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_SCORES] = 4 * 64;
   z[GJXL_AMD_AQ_STRATEGY_LISTS] = 4 * gjxl_amd::kStrategyStages * n;
   for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
-    const GJXLResult r = Check(hipMalloc(&s->buffers[i], z[i]), "hipMalloc(AQ session)");
+    const GJXLResult r = gjxl_amd::DeviceAlloc(device, z[i], &s->buffers[i]);
     if (r != GJXL_OK) {
       gjxl_amd_aq_session_destroy(s);
       return r;
@@ -214,7 +214,7 @@ void gjxl_amd_aq_session_destroy(GJXLAmdAqSession* s) {
     (void)hipStreamSynchronize(s->device->stream);
   }
   for (void*& b : s->buffers) {
-    if (b != nullptr) (void)hipFree(b);
+    if (s->device != nullptr) gjxl_amd::DeviceFree(s->device, b);
   }
   delete s;
 }
@@ -307,6 +307,16 @@ GJXLResult gjxl_amd_aq_policy(GJXLAmdAqSession* s, float* lower, float* upper,
   if (x_scale) *x_scale = s->x_scale;
   if (b_scale) *b_scale = s->b_scale;
   return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_aq_split_forward_dct(GJXLAmdAqSession* s) {
+  GJXL_AMD_TRY(ValidateSession(s));
+  if (!s->prepared || s->pre == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "split forward DCT needs a prepared session");
+  }
+  return gjxl_amd::LaunchSplitAnchorDct(s->device, s->pre, s->pw, s->ph,
+                                        Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s),
+                                        s->layout.store, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS));
 }
 
 GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {

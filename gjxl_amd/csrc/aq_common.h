@@ -33,6 +33,12 @@ struct StrategyLists {
   uint32_t counts[kStrategyStages];
 };
 
+// The split-precision forward DCT of every transform of a frame into the
+// per-group coefficient store (split_dct.hip).
+GJXLResult LaunchSplitAnchorDct(GJXLAmdDevice* d, const float* opsin, int width, int height,
+                                const uint32_t* offsets, const StrategyLists& lists,
+                                size_t store, float* coefs);
+
 __device__ __forceinline__ float DStdMax(float a, float b) { return a < b ? b : a; }
 __device__ __forceinline__ float DStdMin(float a, float b) { return b < a ? b : a; }
 

@@ -1,0 +1,309 @@
+// Batch driver: a persistent pool of workers, each a host thread with its own
+// device handle (stream) on one GPU, running the plain path on independent
+// images.  The semantics follow VarDctBatchEncoder (src/codestream/
+// batch_workflow.h:47-109): request order is kept, a failed image is reported
+// in its own result, results are published only when the whole call is done,
+// encode calls on one driver are serialized, and shutdown drains the active
+// call before joining the workers.  Images of different workers overlap on
+// the GPU (one image's DC wavefront uses eight work groups; the others fill
+// the remaining CUs); a worker keeps its plain path, and so its device
+// buffers, between consecutive images of the same extent and options.
+// No kernels here.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+
+using gjxl_amd::Fail;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+uint64_t Nanoseconds(Clock::duration d) {
+  return static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(d).count());
+}
+
+struct Call {
+  const GJXLAmdBatchRequest* requests = nullptr;
+  size_t count = 0;
+  std::vector<GJXLAmdBatchResult>* staged = nullptr;
+  Clock::time_point arrival;
+  std::atomic<size_t> next{0};
+  size_t done = 0;  // guarded by the driver's mutex
+};
+
+struct Worker {
+  GJXLAmdDevice* device = nullptr;
+  GJXLAmdPlainPath* path = nullptr;
+  uint32_t width = 0, height = 0;
+  GJXLAmdPlainPathOptions options{};
+  std::thread thread;
+};
+
+void FreeResult(GJXLAmdBatchResult* r) {
+  std::free(r->token_values);
+  std::free(r->token_contexts);
+  std::free(r->token_group_offsets);
+  delete r->entropy_inputs;
+  std::memset(r, 0, sizeof(*r));
+}
+
+void SetFailure(GJXLAmdBatchResult* r, GJXLResult code) {
+  r->status = code;
+  const char* message = gjxl_amd_last_error();
+  std::strncpy(r->message, message != nullptr ? message : "", sizeof(r->message) - 1);
+  r->message[sizeof(r->message) - 1] = '\0';
+}
+
+bool SameOptions(const GJXLAmdPlainPathOptions& a, const GJXLAmdPlainPathOptions& b) {
+  return a.distance == b.distance && a.effort == b.effort &&
+         a.intensity_target == b.intensity_target;
+}
+
+// One image on one worker into *r (which starts zeroed).
+void RunImage(Worker* w, const GJXLAmdBatchRequest& request, GJXLAmdBatchResult* r) {
+  GJXLAmdPlainPathOptions options{};
+  if (request.options.struct_size == 0) {
+    (void)gjxl_amd_plain_path_options_init(&options, sizeof(options));
+  } else if (request.options.struct_size < sizeof(GJXLAmdPlainPathOptions)) {
+    return SetFailure(r, Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch request options are too small"));
+  } else {
+    options = request.options;
+  }
+  const GJXLAmdImage& image = request.image;
+  if (image.device_resident) {
+    return SetFailure(r, Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch requests take host pixels"));
+  }
+  r->width = image.width;
+  r->height = image.height;
+  if (w->path != nullptr &&
+      (w->width != image.width || w->height != image.height || !SameOptions(w->options, options))) {
+    gjxl_amd_plain_path_destroy(w->path);
+    w->path = nullptr;
+  }
+  if (w->path == nullptr) {
+    const GJXLResult created =
+      gjxl_amd_plain_path_create(w->device, image.width, image.height, &options, &w->path);
+    if (created != GJXL_OK) {
+      w->path = nullptr;
+      return SetFailure(r, created);
+    }
+    w->width = image.width;
+    w->height = image.height;
+    w->options = options;
+  }
+  GJXLAmdPlainFrame frame{};
+  GJXLResult status = gjxl_amd_plain_path_run(w->path, &image, nullptr, &frame);
+  if (status != GJXL_OK) {
+    // a failed run may leave the path mid-sequence: start the next image afresh
+    gjxl_amd_plain_path_destroy(w->path);
+    w->path = nullptr;
+    return SetFailure(r, status);
+  }
+  const size_t count = static_cast<size_t>(frame.token_count);
+  r->groups = frame.groups;
+  r->token_count = frame.token_count;
+  r->token_values = static_cast<uint32_t*>(std::malloc(std::max<size_t>(4 * count, 4)));
+  r->token_contexts = static_cast<uint16_t*>(std::malloc(std::max<size_t>(2 * count, 2)));
+  r->token_group_offsets = static_cast<uint32_t*>(std::malloc(4 * (frame.groups + 1u)));
+  r->entropy_inputs = new (std::nothrow) GJXLAmdEntropyInputs;
+  if (r->token_values == nullptr || r->token_contexts == nullptr ||
+      r->token_group_offsets == nullptr || r->entropy_inputs == nullptr) {
+    FreeResult(r);
+    return SetFailure(r, Fail(GJXL_ERROR_OUT_OF_MEMORY, "batch result allocation failed"));
+  }
+  status = gjxl_amd_download(w->device, r->token_values, frame.token_values, 4 * count);
+  if (status == GJXL_OK) {
+    status = gjxl_amd_download(w->device, r->token_contexts, frame.token_contexts, 2 * count);
+  }
+  if (status == GJXL_OK) {
+    status = gjxl_amd_download(w->device, r->token_group_offsets, frame.token_group_offsets,
+                               4 * (frame.groups + 1u));
+  }
+  if (status != GJXL_OK) {
+    FreeResult(r);
+    return SetFailure(r, status);
+  }
+  std::memcpy(r->entropy_inputs, frame.entropy_inputs, sizeof(GJXLAmdEntropyInputs));
+  r->evaluations = frame.evaluations;
+  for (int i = 0; i < 8; ++i) r->scores[i] = frame.scores[i];
+  r->status = GJXL_OK;
+}
+
+}  // namespace
+
+struct GJXLAmdBatch {
+  int ordinal = 0;
+  std::vector<std::unique_ptr<Worker>> workers;
+  std::mutex call_mutex;  // serializes encode calls and shutdown's drain
+  std::mutex mutex;       // guards what follows
+  std::condition_variable work, finished;
+  Call* call = nullptr;
+  uint64_t generation = 0;
+  bool closed = false, stopping = false, joined = false;
+};
+
+namespace {
+
+void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
+  uint64_t seen = 0;
+  for (;;) {
+    Call* call = nullptr;
+    {
+      std::unique_lock<std::mutex> lock(b->mutex);
+      b->work.wait(lock, [&] { return b->stopping || (b->call != nullptr && b->generation != seen); });
+      if (b->call == nullptr || b->generation == seen) return;  // stopping, nothing new
+      call = b->call;
+      seen = b->generation;
+    }
+    for (;;) {
+      const size_t i = call->next.fetch_add(1);
+      if (i >= call->count) break;
+      GJXLAmdBatchResult& r = (*call->staged)[i];
+      const Clock::time_point start = Clock::now();
+      RunImage(w, call->requests[i], &r);
+      const Clock::time_point end = Clock::now();
+      r.queue_nanoseconds = Nanoseconds(start - call->arrival);
+      r.service_nanoseconds = Nanoseconds(end - start);
+      r.ready_nanoseconds = r.queue_nanoseconds + r.service_nanoseconds;
+      std::lock_guard<std::mutex> lock(b->mutex);
+      if (++call->done == call->count) b->finished.notify_all();
+    }
+  }
+}
+
+void JoinAndRelease(GJXLAmdBatch* b) {
+  for (auto& w : b->workers) {
+    if (w->thread.joinable()) w->thread.join();
+    gjxl_amd_plain_path_destroy(w->path);
+    w->path = nullptr;
+    gjxl_amd_device_destroy(w->device);
+    w->device = nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+GJXLResult gjxl_amd_batch_create(int device_ordinal, uint32_t max_in_flight,
+                                 GJXLAmdBatch** out) {
+  if (out == nullptr || *out != nullptr || max_in_flight == 0 || max_in_flight > 16) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "batch output must be empty and max_in_flight in [1, 16]");
+  }
+  auto* b = new (std::nothrow) GJXLAmdBatch;
+  if (b == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "batch allocation failed");
+  b->ordinal = device_ordinal;
+  GJXLResult r = GJXL_OK;
+  try {
+    for (uint32_t i = 0; r == GJXL_OK && i < max_in_flight; ++i) {
+      auto w = std::make_unique<Worker>();
+      r = gjxl_amd_device_create(device_ordinal, &w->device);
+      if (r == GJXL_OK) b->workers.push_back(std::move(w));
+    }
+    if (r == GJXL_OK) {
+      for (auto& w : b->workers) w->thread = std::thread(WorkerLoop, b, w.get());
+    }
+  } catch (...) {
+    r = Fail(GJXL_ERROR_OUT_OF_MEMORY, "batch worker start failed");
+  }
+  if (r != GJXL_OK) {
+    {
+      std::lock_guard<std::mutex> lock(b->mutex);
+      b->closed = b->stopping = true;
+    }
+    b->work.notify_all();
+    JoinAndRelease(b);
+    delete b;
+    return r;
+  }
+  *out = b;
+  return GJXL_OK;
+}
+
+uint32_t gjxl_amd_batch_max_in_flight(const GJXLAmdBatch* batch) {
+  return batch == nullptr ? 0u : static_cast<uint32_t>(batch->workers.size());
+}
+
+GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* b, const GJXLAmdBatchRequest* requests,
+                                 size_t count, GJXLAmdBatchResult* results) {
+  if (b == nullptr || (count != 0 && (requests == nullptr || results == nullptr))) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch arguments are null");
+  }
+  const Clock::time_point arrival = Clock::now();
+  std::lock_guard<std::mutex> serial(b->call_mutex);
+  {
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->closed) return Fail(GJXL_ERROR_UNAVAILABLE, "batch driver is shut down");
+  }
+  if (count == 0) return GJXL_OK;
+  std::vector<GJXLAmdBatchResult> staged;
+  try {
+    staged.resize(count);
+  } catch (...) {
+    return Fail(GJXL_ERROR_OUT_OF_MEMORY, "batch result staging failed");
+  }
+  std::memset(staged.data(), 0, count * sizeof(GJXLAmdBatchResult));
+  Call call;
+  call.requests = requests;
+  call.count = count;
+  call.staged = &staged;
+  call.arrival = arrival;
+  {
+    std::lock_guard<std::mutex> lock(b->mutex);
+    b->call = &call;
+    ++b->generation;
+  }
+  b->work.notify_all();
+  {
+    std::unique_lock<std::mutex> lock(b->mutex);
+    b->finished.wait(lock, [&] { return call.done == call.count; });
+    b->call = nullptr;
+  }
+  std::memcpy(results, staged.data(), count * sizeof(GJXLAmdBatchResult));
+  return GJXL_OK;
+}
+
+void gjxl_amd_batch_results_free(GJXLAmdBatchResult* results, size_t count) {
+  if (results == nullptr) return;
+  for (size_t i = 0; i < count; ++i) FreeResult(&results[i]);
+}
+
+void gjxl_amd_batch_shutdown(GJXLAmdBatch* b) {
+  if (b == nullptr) return;
+  {
+    std::lock_guard<std::mutex> lock(b->mutex);
+    b->closed = true;
+  }
+  // the active call (if any) finishes first; calls queued behind it see
+  // `closed` and return
+  std::lock_guard<std::mutex> serial(b->call_mutex);
+  {
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->joined) return;
+    b->stopping = true;
+  }
+  b->work.notify_all();
+  JoinAndRelease(b);
+  std::lock_guard<std::mutex> lock(b->mutex);
+  b->joined = true;
+}
+
+void gjxl_amd_batch_destroy(GJXLAmdBatch* b) {
+  if (b == nullptr) return;
+  gjxl_amd_batch_shutdown(b);
+  delete b;
+}
+
+}  // extern "C"

@@ -1378,8 +1378,7 @@ GJXLResult gjxl_amd_butteraugli_create(GJXLAmdDevice* device, uint32_t width,
     floats += gjxl_amd::kScalePlanes * s.n;
   }
   GJXLResult r = Check(hipSetDevice(device->ordinal), "hipSetDevice");
-  if (r == GJXL_OK) r = Check(hipMalloc(&b->allocation, floats * sizeof(float)),
-                             "hipMalloc(Butteraugli)");
+  if (r == GJXL_OK) r = gjxl_amd::DeviceAlloc(device, floats * sizeof(float), &b->allocation);
   if (r != GJXL_OK) {
     delete b;
     return r;
@@ -1408,8 +1407,9 @@ GJXLResult gjxl_amd_butteraugli_create(GJXLAmdDevice* device, uint32_t width,
   for (int k = 0; k < gjxl_amd::kSigmaCount; ++k) {
     gjxl_amd::HostTaps(gjxl_amd::kSigmas[k], host + k * 64);
   }
-  r = Check(hipMemcpy(b->taps, host, sizeof(host), hipMemcpyHostToDevice),
-            "hipMemcpy(Butteraugli taps)");
+  r = Check(hipMemcpyAsync(b->taps, host, sizeof(host), hipMemcpyHostToDevice, device->stream),
+            "hipMemcpyAsync(Butteraugli taps)");
+  if (r == GJXL_OK) r = Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
   if (r != GJXL_OK) {
     gjxl_amd_butteraugli_destroy(b);
     return r;
@@ -1424,7 +1424,7 @@ void gjxl_amd_butteraugli_destroy(GJXLAmdButteraugli* b) {
     (void)hipSetDevice(b->device->ordinal);
     (void)hipStreamSynchronize(b->device->stream);
   }
-  if (b->allocation != nullptr) (void)hipFree(b->allocation);
+  if (b->device != nullptr) gjxl_amd::DeviceFree(b->device, b->allocation);
   delete b;
 }
 

@@ -41,6 +41,8 @@ struct GJXLAmdDevice {
   // Read-only tables uploaded at creation (see tables.h).
   float* quant_tables = nullptr;
   double* dct_bases = nullptr;
+  // binary16 hi and lo halves of the split-precision DCT bases (split_dct.hip)
+  _Float16* split_bases = nullptr;
   // Device address of the initial-quant-field static workspace on this
   // device (frontend.hip), looked up at creation.
   float* initial_quant_static_scratch = nullptr;
@@ -54,9 +56,23 @@ struct GJXLAmdDevice {
 namespace gjxl_amd {
 
 GJXLResult ValidateDevice(const GJXLAmdDevice* device);
+// Device memory of the objects of a device handle (paths, sessions,
+// Butteraugli, tokenizers), allocated in the handle's stream order from the
+// device's memory pool.  The pool keeps freed memory for later allocations
+// instead of returning it to the driver, so that creating and destroying
+// objects of changing extents (the batch driver) neither calls the driver's
+// allocator nor waits for the whole device.  Memory is usable by work
+// enqueued on the handle's stream after the call, and by the host or other
+// streams after a wait on that stream.
+GJXLResult DeviceAlloc(GJXLAmdDevice* device, size_t bytes, void** out);
+// Releases in stream order (after the work enqueued before it).
+void DeviceFree(GJXLAmdDevice* device, void* pointer);
 // Device address of frontend.hip's static initial-quant workspace on the
 // current device.
 GJXLResult LookUpInitialQuantStaticScratch(float** out);
+// Host: the split-precision DCT bases (SplitBasisHalves() binary16 values).
+void BuildSplitBases(_Float16* out);
+size_t SplitBasisHalves();
 
 // Begin and end of one launch of `kernel` on the device stream, recorded
 // when that family is being probed.  Usage:

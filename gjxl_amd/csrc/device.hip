@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "common.h"
 #include "tables.h"
@@ -36,6 +37,15 @@ GJXLResult ValidateDevice(const GJXLAmdDevice* device) {
   return Check(hipSetDevice(device->ordinal), "hipSetDevice");
 }
 
+GJXLResult DeviceAlloc(GJXLAmdDevice* device, size_t bytes, void** out) {
+  *out = nullptr;
+  return Check(hipMallocAsync(out, bytes, device->stream), "hipMallocAsync");
+}
+
+void DeviceFree(GJXLAmdDevice* device, void* pointer) {
+  if (pointer != nullptr) (void)hipFreeAsync(pointer, device->stream);
+}
+
 }  // namespace gjxl_amd
 
 using gjxl_amd::Check;
@@ -66,6 +76,16 @@ GJXLResult gjxl_amd_device_create(int ordinal, GJXLAmdDevice** out) {
     result = Check(hipEventCreate(&device->events[i]), "hipEventCreate");
   }
   if (result == GJXL_OK) {
+    // the device's pool keeps what objects free for the next allocation
+    hipMemPool_t pool = nullptr;
+    result = Check(hipDeviceGetDefaultMemPool(&pool, ordinal), "hipDeviceGetDefaultMemPool");
+    uint64_t keep = ~uint64_t{0};
+    if (result == GJXL_OK) {
+      result = Check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep),
+                     "hipMemPoolSetAttribute");
+    }
+  }
+  if (result == GJXL_OK) {
     // with this ordinal current: each device has its own copy of the array
     result = gjxl_amd::LookUpInitialQuantStaticScratch(&device->initial_quant_static_scratch);
   }
@@ -78,6 +98,21 @@ GJXLResult gjxl_amd_device_create(int ordinal, GJXLAmdDevice** out) {
     result = Check(hipMalloc(reinterpret_cast<void**>(&device->dct_bases),
                              sizeof(double) * gjxl_amd::kBasisDoubles),
                    "hipMalloc(DCT bases)");
+  }
+  if (result == GJXL_OK) {
+    result = Check(hipMalloc(reinterpret_cast<void**>(&device->split_bases),
+                             sizeof(_Float16) * gjxl_amd::SplitBasisHalves()),
+                   "hipMalloc(split DCT bases)");
+  }
+  if (result == GJXL_OK) {
+    static std::vector<_Float16> split(gjxl_amd::SplitBasisHalves());
+    static const bool split_built = [] {
+      gjxl_amd::BuildSplitBases(split.data());
+      return true;
+    }();
+    (void)split_built;
+    result = Check(hipMemcpy(device->split_bases, split.data(), sizeof(_Float16) * split.size(),
+                             hipMemcpyHostToDevice), "hipMemcpy(split DCT bases)");
   }
   if (result == GJXL_OK) {
     static float tables[gjxl_amd::kQuantTableFloats];
@@ -116,6 +151,7 @@ void gjxl_amd_device_destroy(GJXLAmdDevice* device) {
   }
   if (device->quant_tables != nullptr) (void)hipFree(device->quant_tables);
   if (device->dct_bases != nullptr) (void)hipFree(device->dct_bases);
+  if (device->split_bases != nullptr) (void)hipFree(device->split_bases);
   if (device->stream != nullptr) (void)hipStreamDestroy(device->stream);
   delete device;
 }

@@ -3,16 +3,20 @@
 // initial quant field, initial CfL, candidate costs, strategy search, AQ
 // preparation, iterations + 1 Butteraugli-driven evaluations with field
 // updates, AC tokenization of the last evaluation and the read-back of its
-// token stream.
+// token stream.  At effort 7 the coefficient orders and the block-context map
+// of the tokens are chosen on the host from the final frame's statistics, as
+// the reference's serializer chooses them (src/codestream/vardct/
+// encoder.cpp:1985-2010).
 //
 // The order is RunPreparedQuantizationPipelineWithProviders
 // (src/codec/quantization_pipeline.cpp:428-586) with every provider on the
 // GPU, followed by the AcTokenizationProvider hand-off
 // (src/codestream/ac_tokenization_provider_internal.h:18-47), as PrepareWorkflow
 // (src/codestream/workflow.cpp:636-768) runs it for gjxl_encode
-// (src/c_api/gjxl.cpp:470-637) at distance d, effort 7.  The host waits twice:
+// (src/c_api/gjxl.cpp:470-637) at distance d, effort 7.  The host waits three times:
 // once for the quantization-matrix statistics that choose the matrix scales
-// (src/codestream/workflow.cpp:1318-1484) and once for the token count.
+// (src/codestream/workflow.cpp:1318-1484), once for the frame statistics and
+// once for the token count.
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -54,6 +58,11 @@ struct GJXLAmdPlainPath {
   GJXLAmdTokenizer* tokenizer = nullptr;
   // pinned host words: token status (4), AQ error (1), scores (kMaxEvaluations)
   uint32_t* host_words = nullptr;
+  // pinned host copy of the frame statistics; the entropy inputs chosen
+  // from them (effort 7) for the last run
+  uint32_t* host_statistics = nullptr;
+  GJXLAmdFrameStatistics* statistics = nullptr;
+  GJXLAmdEntropyInputs* entropy_inputs = nullptr;
   // stage marks of the last run
   hipEvent_t marks[kMaxMarks] = {};
   int32_t mark_stage[kMaxMarks] = {};
@@ -80,8 +89,8 @@ GJXLResult Mark(GJXLAmdPlainPath* p, int32_t stage) {
 }
 
 template <typename T>
-GJXLResult Alloc(T** out, size_t bytes) {
-  return Check(hipMalloc(reinterpret_cast<void**>(out), bytes), "hipMalloc(plain path)");
+GJXLResult Alloc(GJXLAmdDevice* d, T** out, size_t bytes) {
+  return gjxl_amd::DeviceAlloc(d, bytes, reinterpret_cast<void**>(out));
 }
 
 template <typename T>
@@ -173,19 +182,20 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
     }
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_REDUCE_UPDATE));
   }
-  // AC tokens of the last evaluation (natural orders, default block-context
-  // map), then the status words, the AQ error flag and the scores
-  GJXL_AMD_TRY(gjxl_amd::TokenizeEnqueue(p->tokenizer,
-                                         AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_QUANTIZED),
-                                         p->cells,
-                                         AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_RAW_QUANT),
-                                         false));
-  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_TOKENIZE));
+  // The CPU-chosen tokenization inputs of effort 7 (coefficient orders and
+  // the adaptive block-context map) from the final frame's statistics, with
+  // the AQ error flag and the scores in the same wait
+  const int32_t* quantized = AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_QUANTIZED);
+  const int32_t* raw_quant = AqBuffer<int32_t>(p->session, GJXL_AMD_AQ_RAW_QUANT);
+  GJXL_AMD_TRY(gjxl_amd::FrameStatisticsEnqueue(p->tokenizer, quantized, p->cells, raw_quant));
   uint32_t* host = p->host_words;
   uint32_t* stats = AqBuffer<uint32_t>(p->session, GJXL_AMD_AQ_STATS);
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, TokenBuffer<uint32_t>(p->tokenizer,
-                                                               GJXL_AMD_TOKENS_STATUS),
-                                    16, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)"));
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(p->host_statistics,
+                                    TokenBuffer<uint32_t>(p->tokenizer,
+                                                          GJXL_AMD_TOKENS_STATISTICS),
+                                    4 * gjxl_amd::FrameStatisticsWords(),
+                                    hipMemcpyDeviceToHost, st),
+                     "hipMemcpyAsync(frame statistics)"));
   GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 4, stats + 7, 4, hipMemcpyDeviceToHost, st),
                      "hipMemcpyAsync(AQ error)"));
   GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * (p->iterations + 1),
@@ -194,6 +204,19 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   if (host[4] != 0) {
     return Fail(GJXL_ERROR_INTERNAL, "Adaptive quantization produced an invalid block distance");
   }
+  GJXL_AMD_TRY(gjxl_amd::FrameStatisticsResult(p->tokenizer, p->host_statistics,
+                                               p->statistics));
+  GJXL_AMD_TRY(gjxl_amd_entropy_inputs_from_statistics(p->statistics, p->options.effort,
+                                                       p->entropy_inputs));
+  GJXL_AMD_TRY(gjxl_amd_tokenizer_set_entropy_inputs(p->tokenizer, p->entropy_inputs));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_ENTROPY_INPUTS));
+  // AC tokens of the last evaluation, then their status words
+  GJXL_AMD_TRY(gjxl_amd::TokenizeEnqueue(p->tokenizer, quantized, p->cells, raw_quant, false));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_TOKENIZE));
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host, TokenBuffer<uint32_t>(p->tokenizer,
+                                                               GJXL_AMD_TOKENS_STATUS),
+                                    16, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
   GJXL_AMD_TRY(gjxl_amd::TokenizeResult(host, &p->token_count));
   for (int it = 0; it <= p->iterations; ++it) std::memcpy(&p->scores[it], &host[8 + it], 4);
   return GJXL_OK;
@@ -259,6 +282,7 @@ GJXLResult FillFrame(GJXLAmdPlainPath* p, GJXLAmdPlainFrame* f) {
   r.quant_dc = qdc;
   r.evaluations = p->iterations + 1;
   for (int i = 0; i < r.evaluations; ++i) r.scores[i] = p->scores[i];
+  r.entropy_inputs = p->entropy_inputs;
   *f = r;
   return GJXL_OK;
 }
@@ -312,24 +336,36 @@ GJXLResult gjxl_amd_plain_path_create(GJXLAmdDevice* device, uint32_t width, uin
   const size_t tiles = static_cast<size_t>(gjxl_amd::CeilDiv(p->bw, 8)) *
                        gjxl_amd::CeilDiv(p->bh, 8);
   p->srgb_bytes = 4 * pixels;  // room for RGBA rows without padding
-  GJXLResult r = Alloc(&p->srgb, p->srgb_bytes);
-  if (r == GJXL_OK) r = Alloc(&p->linear, 4 * 3 * pixels);
-  if (r == GJXL_OK) r = Alloc(&p->opsin, 4 * 3 * padded);
-  if (r == GJXL_OK) r = Alloc(&p->preprocessed, 4 * 3 * padded);
-  if (r == GJXL_OK) r = Alloc(&p->quant_field, 4 * blocks);
-  if (r == GJXL_OK) r = Alloc(&p->strategy_mask, 4 * blocks);
-  if (r == GJXL_OK) r = Alloc(&p->pixel_mask, 4 * padded);
+  GJXLResult r = Alloc(device, &p->srgb, p->srgb_bytes);
+  if (r == GJXL_OK) r = Alloc(device, &p->linear, 4 * 3 * pixels);
+  if (r == GJXL_OK) r = Alloc(device, &p->opsin, 4 * 3 * padded);
+  if (r == GJXL_OK) r = Alloc(device, &p->preprocessed, 4 * 3 * padded);
+  if (r == GJXL_OK) r = Alloc(device, &p->quant_field, 4 * blocks);
+  if (r == GJXL_OK) r = Alloc(device, &p->strategy_mask, 4 * blocks);
+  if (r == GJXL_OK) r = Alloc(device, &p->pixel_mask, 4 * padded);
   if (r == GJXL_OK) {
-    r = Alloc(&p->iqf_scratch, gjxl_amd_initial_quant_field_scratch_bytes(p->pw, p->ph));
+    r = Alloc(device, &p->iqf_scratch, gjxl_amd_initial_quant_field_scratch_bytes(p->pw, p->ph));
   }
-  if (r == GJXL_OK) r = Alloc(&p->y_to_x, tiles);
-  if (r == GJXL_OK) r = Alloc(&p->y_to_b, tiles);
-  if (r == GJXL_OK) r = Alloc(&p->costs, 4 * 7 * blocks);
-  if (r == GJXL_OK) r = Alloc(&p->cells, blocks);
-  if (r == GJXL_OK) r = Alloc(&p->score_words, 4 * kMaxEvaluations);
+  if (r == GJXL_OK) r = Alloc(device, &p->y_to_x, tiles);
+  if (r == GJXL_OK) r = Alloc(device, &p->y_to_b, tiles);
+  if (r == GJXL_OK) r = Alloc(device, &p->costs, 4 * 7 * blocks);
+  if (r == GJXL_OK) r = Alloc(device, &p->cells, blocks);
+  if (r == GJXL_OK) r = Alloc(device, &p->score_words, 4 * kMaxEvaluations);
   if (r == GJXL_OK) {
     r = Check(hipHostMalloc(reinterpret_cast<void**>(&p->host_words), 4 * 32,
                             hipHostMallocDefault), "hipHostMalloc(plain path)");
+  }
+  if (r == GJXL_OK) {
+    r = Check(hipHostMalloc(reinterpret_cast<void**>(&p->host_statistics),
+                            4 * gjxl_amd::FrameStatisticsWords(), hipHostMallocDefault),
+              "hipHostMalloc(frame statistics)");
+  }
+  if (r == GJXL_OK) {
+    p->statistics = new (std::nothrow) GJXLAmdFrameStatistics;
+    p->entropy_inputs = new (std::nothrow) GJXLAmdEntropyInputs;
+    if (p->statistics == nullptr || p->entropy_inputs == nullptr) {
+      r = Fail(GJXL_ERROR_OUT_OF_MEMORY, "plain path allocation failed");
+    }
   }
   for (int i = 0; r == GJXL_OK && i < kMaxMarks; ++i) {
     r = Check(hipEventCreate(&p->marks[i]), "hipEventCreate");
@@ -360,11 +396,14 @@ void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* p) {
     if (e != nullptr) (void)hipEventDestroy(e);
   }
   if (p->host_words != nullptr) (void)hipHostFree(p->host_words);
+  if (p->host_statistics != nullptr) (void)hipHostFree(p->host_statistics);
+  delete p->statistics;
+  delete p->entropy_inputs;
   void* buffers[] = {p->srgb, p->linear, p->opsin, p->preprocessed, p->quant_field,
                      p->strategy_mask, p->pixel_mask, p->iqf_scratch, p->y_to_x, p->y_to_b,
                      p->costs, p->cells, p->score_words};
   for (void* b : buffers) {
-    if (b != nullptr) (void)hipFree(b);
+    gjxl_amd::DeviceFree(p->device, b);
   }
   delete p;
 }

@@ -17,4 +17,13 @@ GJXLResult TokenizeEnqueue(GJXLAmdTokenizer* t, const int32_t* quantized,
 // tokenization, copied to the host.
 GJXLResult TokenizeResult(const uint32_t status[4], uint64_t* token_count);
 
+// The frame statistics of gjxl_amd_tokenizer_frame_statistics in two halves:
+// the launches (their words land in GJXL_AMD_TOKENS_STATISTICS), then the
+// check and unpacking of FrameStatisticsWords() words copied to the host.
+GJXLResult FrameStatisticsEnqueue(GJXLAmdTokenizer* t, const int32_t* quantized,
+                                  const uint8_t* cells, const int32_t* raw_quant);
+GJXLResult FrameStatisticsResult(GJXLAmdTokenizer* t, const uint32_t* words,
+                                 GJXLAmdFrameStatistics* out);
+size_t FrameStatisticsWords();
+
 }  // namespace gjxl_amd

@@ -286,6 +286,122 @@ __global__ __launch_bounds__(64 * kTaskWaves) void TokenEmitKernel(
   }
 }
 
+// ---- frame statistics (CountGroupZeros, CountRawQuantAndOrders) ------------
+
+// Word layout of GJXL_AMD_TOKENS_STATISTICS, the fields of
+// GJXLAmdFrameStatistics in order, then the present mask and error bits.
+constexpr int kZeroCounters = GJXL_AMD_ZERO_COUNTS;
+constexpr int kStatSampled = 3 * kZeroCounters;
+constexpr int kStatRaw = kStatSampled + 3 * 64;
+constexpr int kStatPresent = kStatRaw + kOrderFamilies * 256;
+constexpr int kStatError = kStatPresent + 1;
+constexpr int kStatWords = kStatError + 1;
+constexpr int kSearchedFamilies = 5;
+
+// The counters of the searched families within one channel's 1984: DCT8,
+// the 16x8 pair, 16x16, the 32x16 pair, 32x32.
+__host__ __device__ constexpr int ZeroCounterOffset(int family) {
+  return family == 0 ? 0 : family == 4 ? 64 : family == 2 ? 192 : family == 6 ? 448 : 960;
+}
+__host__ __device__ constexpr int CompactFamily(int family) {
+  return family == 0 ? 0 : family == 4 ? 1 : family == 2 ? 2 : family == 6 ? 3 : 4;
+}
+__host__ __device__ constexpr int FullFamily(int compact) {
+  return compact == 0 ? 0 : compact == 1 ? 4 : compact == 2 ? 2 : compact == 3 ? 6 : 3;
+}
+
+// Work group (AC group g, chunk of its anchors): three waves, wave c counts
+// channel c.  A lane owns positions lane + 64 m of every family and keeps
+// their zero counts in registers (31 counters: 1, 2, 4, 8 and 16 per
+// family), so a transform costs its coalesced loads and no atomics; the
+// counters go to the frame's words once per wave.  The work group's threads
+// also count the chunk's blocks per order family and raw quant (LDS), and a
+// DCT8 transform whose sample decision is set counts into the sampled
+// counters too.
+constexpr int kStatChunks = 16;
+constexpr int kStatThreads = 3 * 64;
+
+__global__ __launch_bounds__(kStatThreads) void FrameStatisticsKernel(
+  const int32_t* __restrict__ quantized, size_t store,
+  const uint8_t* __restrict__ cells, int bw, int bh, int groups_w,
+  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ anchor_list,
+  const uint32_t* __restrict__ anchor_count, const int32_t* __restrict__ raw,
+  const uint8_t* __restrict__ samples, uint32_t* __restrict__ stats) {
+  __shared__ uint32_t raw_counts[kSearchedFamilies * 256];
+  const int g = blockIdx.x, chunk = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int c = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < kSearchedFamilies * 256; i += kStatThreads) raw_counts[i] = 0;
+  __syncthreads();
+  // zero counts of channel c over this chunk's transforms
+  uint32_t z8 = 0, z16x8[2] = {}, z16[4] = {}, z32x16[8] = {}, z32[16] = {}, sampled = 0;
+  uint32_t present = 0;
+  const int anchors = static_cast<int>(anchor_count[g]);
+  const int per = (anchors + kStatChunks - 1) / kStatChunks;
+  const int first = chunk * per, last = min(anchors, first + per);
+  const int32_t* plane = quantized + c * store;
+  for (int k = first; k < last; ++k) {
+    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
+    const size_t block = static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff);
+    const int st = __builtin_amdgcn_readfirstlane(cells[block] >> 1);
+    const int32_t* coef = plane + offsets[block];
+    const int family = OrderFamily(st);
+    present |= 1u << family;
+    if (family == 0) {
+      const uint32_t zero = coef[lane] == 0;
+      z8 += zero;
+      if (samples[block] != 0) sampled += zero;
+    } else if (family == 4) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) z16x8[m] += coef[lane + 64 * m] == 0;
+    } else if (family == 2) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) z16[m] += coef[lane + 64 * m] == 0;
+    } else if (family == 6) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) z32x16[m] += coef[lane + 64 * m] == 0;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) z32[m] += coef[lane + 64 * m] == 0;
+    }
+  }
+  uint32_t* zc = stats + c * kZeroCounters;
+  auto flush = [&](int family, const uint32_t* counts, int n) {
+    for (int m = 0; m < n; ++m) {
+      if (counts[m] != 0) atomicAdd(&zc[ZeroCounterOffset(family) + lane + 64 * m], counts[m]);
+    }
+  };
+  flush(0, &z8, 1);
+  flush(4, z16x8, 2);
+  flush(2, z16, 4);
+  flush(6, z32x16, 8);
+  flush(3, z32, 16);
+  if (sampled != 0) atomicAdd(&stats[kStatSampled + c * 64 + lane], sampled);
+  if (lane == 0 && present != 0) atomicOr(&stats[kStatPresent], present);
+  // blocks per order family and raw quant: this chunk's share of the
+  // group's blocks
+  const int x0 = (g % groups_w) * kGroupBlocks, y0 = (g / groups_w) * kGroupBlocks;
+  const int w = min(kGroupBlocks, bw - x0), h = min(kGroupBlocks, bh - y0);
+  const int count = w * h;
+  const int bper = (count + kStatChunks - 1) / kStatChunks;
+  for (int i = chunk * bper + tid; i < min(count, (chunk + 1) * bper); i += kStatThreads) {
+    const size_t block = static_cast<size_t>(y0 + i / w) * bw + x0 + i % w;
+    const int32_t q = raw[block];
+    if (q < 1 || q > 256) {
+      atomicOr(&stats[kStatError], kRawQuantError);
+    } else {
+      const int family = OrderFamily(cells[block] >> 1);
+      atomicAdd(&raw_counts[CompactFamily(family) * 256 + q - 1], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < kSearchedFamilies * 256; i += kStatThreads) {
+    if (raw_counts[i] != 0) {
+      atomicAdd(&stats[kStatRaw + FullFamily(i / 256) * 256 + i % 256], raw_counts[i]);
+    }
+  }
+}
+
 // ---- host ------------------------------------------------------------------
 
 // ComputeSimpleNaturalCoefficientOrder (ac_group.cpp:381-487): the zig-zag
@@ -333,6 +449,34 @@ size_t NaturalOrder(int strategy, uint32_t* order) {
   return tail == total ? total : 0;
 }
 
+// The effort-7 sample decisions (UseCoefficientOrderSample, src/codestream/
+// coefficient_order.cpp:115-126, drawn as :626-650 does): one xorshift128+
+// draw per block, AC groups in raster order and blocks in raster order
+// within each group; a block's transform is counted when the high half of
+// the draw's sum is at most half the 32-bit range.  The reference draws per
+// transform of a DCT8-only frame, where every block is one transform, which
+// is the only case that reads the decisions.
+std::vector<uint8_t> SampleDecisions(const AqLayout& l) {
+  std::vector<uint8_t> decisions(static_cast<size_t>(l.bw) * l.bh);
+  uint64_t first = 0x94D049BB133111EBull, second = 0xBF58476D1CE4E5B9ull;
+  for (int g = 0; g < l.groups; ++g) {
+    const int x0 = (g % l.groups_w) * kGroupBlocks, y0 = (g / l.groups_w) * kGroupBlocks;
+    const int x1 = std::min(x0 + kGroupBlocks, l.bw), y1 = std::min(y0 + kGroupBlocks, l.bh);
+    for (int y = y0; y < y1; ++y) {
+      for (int x = x0; x < x1; ++x) {
+        const uint64_t sum = first + second;
+        uint64_t mixed = first ^ (first << 23);
+        mixed ^= second ^ (mixed >> 18) ^ (second >> 5);
+        first = second;
+        second = mixed;
+        decisions[static_cast<size_t>(y) * l.bw + x] =
+          static_cast<uint8_t>((sum >> 32) <= (0xffffffffull >> 1));
+      }
+    }
+  }
+  return decisions;
+}
+
 bool ValidOrder(int strategy, const uint32_t* order) {
   const int cw = CoveredW(strategy), ch = CoveredH(strategy);
   const int n = cw * ch * 64;
@@ -363,6 +507,12 @@ struct GJXLAmdTokenizer {
   gjxl_amd::ContextModel model{};
   void* buffers[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
   size_t sizes[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
+  // pinned staging of the order table and its inverse; staged_copy marks
+  // the end of the last copy out of it
+  uint32_t* staged_orders = nullptr;
+  uint16_t* staged_inverse = nullptr;
+  hipEvent_t staged_copy = nullptr;
+  bool staged_pending = false;
 };
 
 namespace {
@@ -377,25 +527,37 @@ GJXLResult ValidateTokenizer(GJXLAmdTokenizer* t) {
   return gjxl_amd::ValidateDevice(t->device);
 }
 
-// Uploads the orders of every stage and channel with their inverses.
+// Uploads the orders of every stage and channel with their inverses through
+// the pinned staging buffers, without waiting: the copies are ordered before
+// every later launch on the stream, and the next upload waits for them
+// before it overwrites the staging.
 GJXLResult UploadOrders(GJXLAmdTokenizer* t, const std::vector<uint32_t>& table) {
   using namespace gjxl_amd;
-  std::vector<uint16_t> inverse(table.size());
+  if (t->staged_pending) {
+    GJXL_AMD_TRY(Check(hipEventSynchronize(t->staged_copy), "hipEventSynchronize"));
+    t->staged_pending = false;
+  }
+  std::memcpy(t->staged_orders, table.data(), 4 * table.size());
   for (int s = 0; s < kStrategyStages; ++s) {
     const int n = CoveredW(StageStrategy(s)) * CoveredH(StageStrategy(s)) * 64;
     for (int c = 0; c < 3; ++c) {
       const size_t base = 3 * static_cast<size_t>(StageOffset(s)) + c * n;
-      for (int i = 0; i < n; ++i) inverse[base + table[base + i]] = static_cast<uint16_t>(i);
+      for (int i = 0; i < n; ++i) {
+        t->staged_inverse[base + table[base + i]] = static_cast<uint16_t>(i);
+      }
     }
   }
   hipStream_t st = t->device->stream;
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_ORDERS], table.data(),
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_ORDERS], t->staged_orders,
                                     4 * table.size(), hipMemcpyHostToDevice, st),
                      "hipMemcpyAsync(orders)"));
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_INVERSE_ORDERS], inverse.data(),
-                                    2 * inverse.size(), hipMemcpyHostToDevice, st),
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_INVERSE_ORDERS],
+                                    t->staged_inverse, 2 * table.size(),
+                                    hipMemcpyHostToDevice, st),
                      "hipMemcpyAsync(inverse orders)"));
-  return Check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  GJXL_AMD_TRY(Check(hipEventRecord(t->staged_copy, st), "hipEventRecord"));
+  t->staged_pending = true;
+  return GJXL_OK;
 }
 
 }  // namespace
@@ -529,14 +691,33 @@ GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device, uint32_t width, uint
   z[GJXL_AMD_TOKENS_ORDERS] = 4 * 3 * kOrderStride;
   z[GJXL_AMD_TOKENS_INVERSE_ORDERS] = 2 * 3 * kOrderStride;
   z[GJXL_AMD_TOKENS_STATUS] = 4 * 4;
+  z[GJXL_AMD_TOKENS_STATISTICS] = 4 * kStatWords;
+  z[GJXL_AMD_TOKENS_SAMPLE_DECISIONS] = blocks;
   for (int i = 0; i < GJXL_AMD_TOKENS_BUFFER_COUNT; ++i) {
-    const GJXLResult r = Check(hipMalloc(&t->buffers[i], z[i]), "hipMalloc(tokenizer)");
+    const GJXLResult r = gjxl_amd::DeviceAlloc(device, z[i], &t->buffers[i]);
     if (r != GJXL_OK) {
       gjxl_amd_tokenizer_destroy(t);
       return r;
     }
   }
-  GJXLResult r = gjxl_amd_tokenizer_set_orders(t, 0, nullptr);
+  GJXLResult r = Check(hipHostMalloc(reinterpret_cast<void**>(&t->staged_orders),
+                                     4 * 3 * kOrderStride, hipHostMallocDefault),
+                       "hipHostMalloc(order staging)");
+  if (r == GJXL_OK) {
+    r = Check(hipHostMalloc(reinterpret_cast<void**>(&t->staged_inverse),
+                            2 * 3 * kOrderStride, hipHostMallocDefault),
+              "hipHostMalloc(order staging)");
+  }
+  if (r == GJXL_OK) r = Check(hipEventCreate(&t->staged_copy), "hipEventCreate");
+  if (r == GJXL_OK) r = gjxl_amd_tokenizer_set_orders(t, 0, nullptr);
+  if (r == GJXL_OK) {
+    const std::vector<uint8_t> decisions = SampleDecisions(t->layout);
+    r = Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_SAMPLE_DECISIONS], decisions.data(),
+                             decisions.size(), hipMemcpyHostToDevice, device->stream),
+              "hipMemcpyAsync(sample decisions)");
+    // the pageable source is gone when this returns
+    if (r == GJXL_OK) r = Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+  }
   if (r == GJXL_OK) {
     // DefaultSimpleBlockContextMap: src/codestream/simple_ac_context.h:29-33
     uint8_t map[3 * kOrderFamilies];
@@ -561,8 +742,11 @@ void gjxl_amd_tokenizer_destroy(GJXLAmdTokenizer* t) {
     (void)hipStreamSynchronize(t->device->stream);
   }
   for (void*& b : t->buffers) {
-    if (b != nullptr) (void)hipFree(b);
+    if (t->device != nullptr) gjxl_amd::DeviceFree(t->device, b);
   }
+  if (t->staged_copy != nullptr) (void)hipEventDestroy(t->staged_copy);
+  if (t->staged_orders != nullptr) (void)hipHostFree(t->staged_orders);
+  if (t->staged_inverse != nullptr) (void)hipHostFree(t->staged_inverse);
   delete t;
 }
 
@@ -666,5 +850,75 @@ GJXLResult gjxl_amd_tokenizer_buffer(GJXLAmdTokenizer* t, int32_t id, void** poi
   }
   *pointer = t->buffers[id];
   if (bytes != nullptr) *bytes = t->sizes[id];
+  return GJXL_OK;
+}
+
+namespace gjxl_amd {
+
+GJXLResult FrameStatisticsEnqueue(GJXLAmdTokenizer* t, const int32_t* quantized,
+                                  const uint8_t* cells, const int32_t* raw_quant) {
+  GJXL_AMD_TRY(ValidateTokenizer(t));
+  if (quantized == nullptr || cells == nullptr || raw_quant == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "frame statistics arguments are null");
+  }
+  hipStream_t s = t->device->stream;
+  const AqLayout& l = t->layout;
+  uint32_t* stats = Buf<uint32_t>(t, GJXL_AMD_TOKENS_STATISTICS);
+  uint32_t* offsets = Buf<uint32_t>(t, GJXL_AMD_TOKENS_OFFSETS);
+  uint32_t* anchors = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_LIST);
+  uint32_t* counts = Buf<uint32_t>(t, GJXL_AMD_TOKENS_ANCHOR_COUNT);
+  hipLaunchKernelGGL(FillKernel, dim3(CeilDiv(kStatWords, 256)), dim3(256), 0, s, stats,
+                     size_t{kStatWords}, 0u);
+  GJXL_AMD_TRY(CheckLaunch("FillKernel"));
+  // error word bit 0: the grid does not tile its groups (no transform of
+  // such a group is counted)
+  GJXL_AMD_TRY(LaunchAnchorLayout(s, cells, l.bw * 8, l.bh * 8, offsets, anchors, counts,
+                                  stats + kStatError, nullptr, 0, nullptr));
+  hipLaunchKernelGGL(FrameStatisticsKernel, dim3(l.groups, kStatChunks), dim3(kStatThreads), 0, s,
+                     quantized, l.store, cells, l.bw, l.bh, l.groups_w, offsets, anchors,
+                     counts, raw_quant, Buf<uint8_t>(t, GJXL_AMD_TOKENS_SAMPLE_DECISIONS),
+                     stats);
+  return CheckLaunch("FrameStatisticsKernel");
+}
+
+GJXLResult FrameStatisticsResult(GJXLAmdTokenizer* t, const uint32_t* words,
+                                 GJXLAmdFrameStatistics* out) {
+  if (words[kStatError] & kGridError) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "strategy grid does not tile its AC groups");
+  }
+  if (words[kStatError] & kRawQuantError) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Block-context raw quantization is out of range");
+  }
+  out->block_width = static_cast<uint32_t>(t->layout.bw);
+  out->block_height = static_cast<uint32_t>(t->layout.bh);
+  out->present_mask = static_cast<uint16_t>(words[kStatPresent]);
+  std::memcpy(out->zero_counts, words, sizeof(out->zero_counts));
+  std::memcpy(out->sampled_dct8_zero_counts, words + kStatSampled,
+              sizeof(out->sampled_dct8_zero_counts));
+  std::memcpy(out->raw_quant_counts, words + kStatRaw, sizeof(out->raw_quant_counts));
+  return GJXL_OK;
+}
+
+size_t FrameStatisticsWords() { return kStatWords; }
+
+}  // namespace gjxl_amd
+
+GJXLResult gjxl_amd_tokenizer_frame_statistics(GJXLAmdTokenizer* t, const int32_t* quantized,
+                                               const uint8_t* cells, const int32_t* raw_quant,
+                                               GJXLAmdFrameStatistics* statistics) {
+  using namespace gjxl_amd;
+  if (statistics == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "frame statistics output is null");
+  }
+  GJXL_AMD_TRY(FrameStatisticsEnqueue(t, quantized, cells, raw_quant));
+  std::vector<uint32_t> words(kStatWords);
+  hipStream_t s = t->device->stream;
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(words.data(), t->buffers[GJXL_AMD_TOKENS_STATISTICS],
+                                    4 * words.size(), hipMemcpyDeviceToHost, s),
+                     "hipMemcpyAsync(frame statistics)"));
+  GJXL_AMD_TRY(Check(hipStreamSynchronize(s), "hipStreamSynchronize"));
+  GJXLAmdFrameStatistics result;
+  GJXL_AMD_TRY(FrameStatisticsResult(t, words.data(), &result));
+  *statistics = result;
   return GJXL_OK;
 }

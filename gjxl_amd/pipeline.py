@@ -17,8 +17,8 @@ from typing import Callable
 
 import numpy as np
 
-from . import (AqSession, Butteraugli, Device, DeviceArray, Tokenizer, library, pad8,
-               reduce_distance_map)
+from . import (AqSession, Butteraugli, Device, DeviceArray, Tokenizer, entropy_inputs,
+               library, pad8, reduce_distance_map)
 
 
 def run_frontend(dev: Device, srgb: np.ndarray, intensity_target: float = 255.0,
@@ -44,8 +44,9 @@ class PlainPath:
     run() takes the packed sRGB8 pixels already on the device and leaves the
     strategy cells, the final quant field, the block distances and the last
     evaluation's frame in the session (d1.0 e7: two AQ updates, three
-    evaluations), and that frame's AC tokens in the tokenizer (natural
-    orders, default block-context map)."""
+    evaluations), and that frame's AC tokens in the tokenizer, with the
+    coefficient orders and block-context map that effort 7 chooses from the
+    frame's statistics."""
 
     def __init__(self, dev: Device, width: int, height: int,
                  target: float = 1.0, iterations: int = 2):
@@ -122,6 +123,13 @@ class PlainPath:
             mark("aq_reduce_update")
         # the reference fails the loop on an invalid block distance
         self.session.check()
+        # the effort-7 coefficient orders and block-context map, chosen on
+        # the host from the final frame's statistics (waits for them)
+        self.statistics = self.tokenizer.frame_statistics(self.quantized, self.cells,
+                                                          self.raw_quant)
+        self.entropy_inputs = entropy_inputs(self.statistics, 7)
+        self.tokenizer.set_entropy_inputs(self.entropy_inputs)
+        mark("entropy_inputs")
         # AC tokens of the last evaluation's frame (waits for the count)
         self.token_count = self.tokenizer.tokenize(self.quantized, self.cells,
                                                    self.raw_quant)

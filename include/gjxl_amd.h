@@ -285,6 +285,27 @@ GJXL_AMD_API GJXLResult gjxl_amd_aq_policy(GJXLAmdAqSession* session,
 GJXL_AMD_API GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* session,
                                              const float* quant_field);
 
+/* ---------------------------------------------------------------------- */
+/* Chosen precision form (not the plain path): the forward DCT as batched   */
+/* matrix products on MFMA in binary16 with split hi + lo operands and      */
+/* binary32 accumulation (hi*hi + hi*lo + lo*hi), in place of the binary64 */
+/* bases of ForwardDctCpu (src/codec/dct.cpp:135-182).  No counterpart in   */
+/* the reference, whose GPU DCTs are binary64 (src/gpu/cuda/                */
+/* cuda_exact_arithmetic_test.cpp:161-186).                                 */
+/* ---------------------------------------------------------------------- */
+/* count blocks of width x height floats (row-major, contiguous; sides 8,
+ * 16, 32 or 64) into count blocks of coefficients in the storage order of
+ * src/core/ac_strategy.h:95-108 (the longer side contiguous).  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_split_dct_blocks(GJXLAmdDevice* device,
+                                                  uint32_t width, uint32_t height,
+                                                  const float* blocks, size_t count,
+                                                  float* coefficients);
+/* Rewrites GJXL_AMD_AQ_COEFFICIENTS of a prepared session with the split
+ * form of every transform's forward DCT (from the Gaborish-inverted image
+ * the session was prepared with); later evaluations code those
+ * coefficients.  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_aq_split_forward_dct(GJXLAmdAqSession* session);
+
 /* Device pointer and size in bytes of one session buffer. */
 GJXL_AMD_API GJXLResult gjxl_amd_aq_buffer(GJXLAmdAqSession* session,
                                            int32_t id, void** pointer,
@@ -404,6 +425,9 @@ enum {
   GJXL_AMD_TOKENS_ORDERS,           /* u32 3 x 2624: scan -> coefficient */
   GJXL_AMD_TOKENS_INVERSE_ORDERS,   /* u16 3 x 2624: coefficient -> scan */
   GJXL_AMD_TOKENS_STATUS,           /* u32[4]: total tokens, error bits */
+  GJXL_AMD_TOKENS_STATISTICS,       /* u32: frame statistics (GJXLAmdFrame-
+                                       Statistics order) + present mask + error */
+  GJXL_AMD_TOKENS_SAMPLE_DECISIONS, /* u8 per block: effort-7 DCT8 sample */
   GJXL_AMD_TOKENS_BUFFER_COUNT
 };
 
@@ -457,6 +481,84 @@ GJXL_AMD_API GJXLResult gjxl_amd_tokenize(GJXLAmdTokenizer* tokenizer,
 GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_buffer(GJXLAmdTokenizer* tokenizer,
                                                   int32_t id, void** pointer,
                                                   size_t* bytes);
+
+/* ---------------------------------------------------------------------- */
+/* f1/f2: the CPU-chosen inputs of AC tokenization, from frame statistics   */
+/* gathered on the device.  The reference counts, on the CPU, the zero      */
+/* coefficients of every order family, channel and position                  */
+/* (CountGroupZeros, src/codestream/coefficient_order.cpp:223-316; at        */
+/* effort 7 over a xorshift128+ sample of the transforms when the frame is   */
+/* DCT8 only, :626-650) and the blocks per order family and raw quant       */
+/* (CountRawQuantAndOrders, src/codestream/block_context_map.cpp:67-103),   */
+/* then sorts the orders (:736-811) and clusters the block contexts         */
+/* (BuildAdaptiveMap, :126-200, via ComputeSimpleBlockContextMapForEncoder, */
+/* :435-476).  Here the counting is one kernel over the quantized store and */
+/* the sorting and clustering run on the host from 38 KB of counts.         */
+/* ---------------------------------------------------------------------- */
+enum {
+  GJXL_AMD_ORDER_FAMILIES = 13,     /* kSimpleCoefficientOrderCount */
+  GJXL_AMD_ZERO_COUNTS = 1984,      /* counters per channel, see below */
+  GJXL_AMD_MAX_QF_THRESHOLDS = 15,
+  GJXL_AMD_MAX_BLOCK_CONTEXTS = 16
+};
+
+/* zero_counts[c] (c = X, Y, B): the families of the searched strategies at
+ * offsets DCT8 0 (64 positions), DCT16x8 and 8x16 64 (128), DCT16x16 192
+ * (256), DCT32x16 and 16x32 448 (512), DCT32x32 960 (1024); position k is
+ * the coefficient index in the transform's storage layout.
+ * sampled_dct8_zero_counts: the DCT8 counts over the sampled transforms
+ * only (meaningful when DCT8 is the only family present).
+ * raw_quant_counts[f][q - 1]: blocks of order family f with raw quant q. */
+typedef struct {
+  uint32_t block_width, block_height;
+  uint16_t present_mask;            /* order families with anchors */
+  uint32_t zero_counts[3][GJXL_AMD_ZERO_COUNTS];
+  uint32_t sampled_dct8_zero_counts[3][64];
+  uint32_t raw_quant_counts[GJXL_AMD_ORDER_FAMILIES][256];
+} GJXLAmdFrameStatistics;
+
+/* SimpleCoefficientOrders (src/codestream/coefficient_order.h:32-41) and
+ * SimpleBlockContextMap (src/codestream/block_context_map.h): for each
+ * family f in used_order_mask, orders[f][c] holds its first n entries
+ * (scan position -> coefficient index, n = the family's size); the map is
+ * 3 * 13 * (qf_threshold_count + 1) labels. */
+typedef struct {
+  uint16_t used_order_mask;
+  uint32_t orders[GJXL_AMD_ORDER_FAMILIES][3][1024];
+  uint32_t qf_threshold_count;
+  uint32_t qf_thresholds[GJXL_AMD_MAX_QF_THRESHOLDS];
+  uint32_t num_contexts;
+  uint8_t context_map[3 * GJXL_AMD_ORDER_FAMILIES * (GJXL_AMD_MAX_QF_THRESHOLDS + 1)];
+} GJXLAmdEntropyInputs;
+
+/* The statistics of a quantized frame (inputs as gjxl_amd_tokenize).
+ * Waits for the stream.  Raw quant outside [1, 256] is
+ * GJXL_ERROR_INVALID_ARGUMENT, as CountRawQuantAndOrders has it. */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_frame_statistics(
+  GJXLAmdTokenizer* tokenizer, const int32_t* quantized, const uint8_t* cells,
+  const int32_t* raw_quant, GJXLAmdFrameStatistics* statistics);
+
+/* Host only.  The orders of ComputeSimpleCoefficientOrdersForEncoder with
+ * the behaviour of the effort (effort 7: kEffort7Dct8Sampled, otherwise
+ * kFull; src/codestream/workflow.cpp:2066-2072) and the map of
+ * ComputeSimpleBlockContextMapForEncoder.  out is written only on success. */
+GJXL_AMD_API GJXLResult gjxl_amd_entropy_inputs_from_statistics(
+  const GJXLAmdFrameStatistics* statistics, int32_t effort,
+  GJXLAmdEntropyInputs* out);
+
+/* gjxl_amd_tokenizer_set_orders and gjxl_amd_tokenizer_set_context_map
+ * with the inputs above. */
+GJXL_AMD_API GJXLResult gjxl_amd_tokenizer_set_entropy_inputs(
+  GJXLAmdTokenizer* tokenizer, const GJXLAmdEntropyInputs* inputs);
+
+/* Host only.  TokenizeSimpleCoefficientOrders (src/codestream/
+ * coefficient_order.cpp:876-919): per used family and channel, the Lehmer
+ * code of the order relative to the natural order, as {context, value}
+ * tokens.  *count receives the number of tokens; capacity is checked
+ * first (at most 3 * 1984 + 15 tokens). */
+GJXL_AMD_API GJXLResult gjxl_amd_coefficient_order_tokens(
+  const GJXLAmdEntropyInputs* inputs, uint32_t* contexts, uint32_t* values,
+  size_t capacity, size_t* count);
 
 /* ---------------------------------------------------------------------- */
 /* The whole plain path as one call.  Replaces, for one image, the stage     */
@@ -531,6 +633,8 @@ typedef struct {
   float quant_dc;
   int32_t evaluations;
   float scores[8];                  /* Butteraugli score per evaluation */
+  /* host: the coefficient orders and block-context map of the tokens */
+  const GJXLAmdEntropyInputs* entropy_inputs;
 } GJXLAmdPlainFrame;
 
 /* Stage ids of gjxl_amd_plain_path_stage_times. */
@@ -549,6 +653,7 @@ enum {
   GJXL_AMD_STAGE_BUTTERAUGLI_COMPARE,
   GJXL_AMD_STAGE_AQ_REDUCE_UPDATE,
   GJXL_AMD_STAGE_AC_TOKENIZE,
+  GJXL_AMD_STAGE_ENTROPY_INPUTS,
   GJXL_AMD_STAGE_TOKEN_READBACK,
   GJXL_AMD_STAGE_COUNT
 };
@@ -578,6 +683,64 @@ GJXL_AMD_API GJXLResult gjxl_amd_plain_path_stage_times(
 /* Pinned host memory (hipHostMalloc) for inputs and token outputs. */
 GJXL_AMD_API GJXLResult gjxl_amd_host_alloc(size_t bytes, void** out);
 GJXL_AMD_API void gjxl_amd_host_free(void* pointer);
+
+/* ---------------------------------------------------------------------- */
+/* f4: batch driver.  Replaces VarDctBatchEncoder (src/codestream/          */
+/* batch_workflow.h:47-109) for the plain path: a persistent,               */
+/* bounded-concurrency driver for independent images of any extent.  Each   */
+/* of max_in_flight workers is one host thread with its own device handle   */
+/* (its own stream) on one GPU and keeps its plain path between images of   */
+/* the same extent and options; images of different workers run on the     */
+/* GPU at once.  One encode call runs every request and keeps request       */
+/* order in the results; an image's failure is reported in its result and   */
+/* does not stop the others.  Results are published when every image is    */
+/* done; invalid driver or output arguments leave the results unchanged.    */
+/* Concurrent encode calls on one driver are serialized.  shutdown closes   */
+/* the driver: it drains the active call and joins the workers; later      */
+/* calls return GJXL_ERROR_UNAVAILABLE.                                     */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdBatch GJXLAmdBatch;
+
+typedef struct {
+  GJXLAmdImage image;               /* host pixels (device_resident == 0) */
+  GJXLAmdPlainPathOptions options;  /* struct_size 0: the defaults */
+} GJXLAmdBatchRequest;
+
+/* One image's outcome (VarDctBatchEncodingResult, batch_workflow.h:40-45).
+ * On success the token stream, its group offsets and the entropy inputs are
+ * library-owned host memory until gjxl_amd_batch_results_free.  Timing is
+ * wall time on the calling side (VarDctBatchSchedulingTiming, :22-38):
+ * queue = encode call to the worker taking the image, service = worker
+ * start to result, ready = queue + service. */
+typedef struct {
+  GJXLResult status;
+  char message[256];                /* the failure's diagnostic */
+  uint32_t width, height, groups;
+  uint64_t token_count;
+  uint32_t* token_values;
+  uint16_t* token_contexts;
+  uint32_t* token_group_offsets;    /* groups + 1 */
+  GJXLAmdEntropyInputs* entropy_inputs;
+  int32_t evaluations;
+  float scores[8];
+  uint64_t queue_nanoseconds, service_nanoseconds, ready_nanoseconds;
+} GJXLAmdBatchResult;
+
+GJXL_AMD_API GJXLResult gjxl_amd_batch_create(int device_ordinal,
+                                              uint32_t max_in_flight,
+                                              GJXLAmdBatch** out);
+GJXL_AMD_API uint32_t gjxl_amd_batch_max_in_flight(const GJXLAmdBatch* batch);
+/* results: count entries, zeroed (or freed) by the caller beforehand. */
+GJXL_AMD_API GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* batch,
+                                              const GJXLAmdBatchRequest* requests,
+                                              size_t count,
+                                              GJXLAmdBatchResult* results);
+/* Frees the host memory of count results and zeroes them. */
+GJXL_AMD_API void gjxl_amd_batch_results_free(GJXLAmdBatchResult* results,
+                                              size_t count);
+GJXL_AMD_API void gjxl_amd_batch_shutdown(GJXLAmdBatch* batch);
+/* Shuts down, then frees the driver; no call may be in progress. */
+GJXL_AMD_API void gjxl_amd_batch_destroy(GJXLAmdBatch* batch);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -640,3 +640,119 @@ def tokenize_frame(quantized: np.ndarray, cells: np.ndarray, raw: np.ndarray,
     if histogram:
         out["histogram"] = hist.reshape(nctx * 495, 128)
     return out
+
+
+# ---- f1/f2: effort-7 tokenization inputs -----------------------------------
+
+_E_DECLARED = False
+
+
+def _declare_entropy_inputs():
+    global _E_DECLARED
+    if _E_DECLARED:
+        return
+    _declare_tokenize()
+    lib = oracle()
+    P, SZ, I = _P, _SZ, _I
+    sig = {
+        "oracle_order_sample_decisions": ([SZ, SZ, P], None),
+        "oracle_frame_statistics": ([P, P, SZ, SZ, P, P, P, P, P, P], I),
+        "oracle_coefficient_orders": ([P, P, ctypes.c_uint16, SZ, SZ, I, P, P], I),
+        "oracle_block_context_map": ([P, SZ, P, ctypes.POINTER(ctypes.c_size_t), P,
+                                      ctypes.POINTER(ctypes.c_uint32)], I),
+        "oracle_order_tokens": ([ctypes.c_uint16, P, P, P, SZ,
+                                 ctypes.POINTER(ctypes.c_size_t)], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _E_DECLARED = True
+
+
+def order_sample_decisions(bw: int, bh: int) -> np.ndarray:
+    _declare_entropy_inputs()
+    out = np.zeros((bh, bw), np.uint8)
+    oracle().oracle_order_sample_decisions(bw, bh, ptr(out))
+    return out
+
+
+def frame_statistics(quantized: np.ndarray, cells: np.ndarray, raw: np.ndarray,
+                     decisions: np.ndarray | None = None) -> dict:
+    _declare_entropy_inputs()
+    bh, bw = cells.shape
+    quantized = np.ascontiguousarray(quantized, np.int32)
+    cells = np.ascontiguousarray(cells, np.uint8)
+    raw = np.ascontiguousarray(raw, np.int32)
+    if decisions is None:
+        decisions = order_sample_decisions(bw, bh)
+    zeros = np.zeros((3, 1984), np.uint32)
+    sampled = np.zeros((3, 64), np.uint32)
+    raw_counts = np.zeros((13, 256), np.uint32)
+    present = ctypes.c_uint16()
+    rc = oracle().oracle_frame_statistics(
+        ptr(quantized), ptr(cells), bw, bh, ptr(raw), ptr(decisions), ptr(zeros),
+        ptr(sampled), ctypes.byref(present), ptr(raw_counts))
+    if rc:
+        raise ValueError(f"oracle_frame_statistics failed with {rc}")
+    return {"zero_counts": zeros, "sampled": sampled, "raw_counts": raw_counts,
+            "present_mask": int(present.value), "bw": bw, "bh": bh}
+
+
+def coefficient_orders(stats: dict, effort7_sampled: bool = True) -> tuple[int, np.ndarray]:
+    """(used_order_mask, 7-stage order table)."""
+    _declare_entropy_inputs()
+    orders = np.zeros(ORDER_TABLE_SIZE, np.uint32)
+    used = ctypes.c_uint16()
+    rc = oracle().oracle_coefficient_orders(
+        ptr(stats["zero_counts"]), ptr(stats["sampled"]), stats["present_mask"],
+        stats["bw"], stats["bh"], int(effort7_sampled), ctypes.byref(used), ptr(orders))
+    if rc:
+        raise ValueError(f"oracle_coefficient_orders failed with {rc}")
+    return int(used.value), orders
+
+
+def block_context_map(raw_counts: np.ndarray, blocks: int) -> dict:
+    _declare_entropy_inputs()
+    raw_counts = np.ascontiguousarray(raw_counts, np.uint32)
+    thr = np.zeros(15, np.uint32)
+    nthr = ctypes.c_size_t()
+    cmap = np.zeros(3 * 13 * 16, np.uint8)
+    nctx = ctypes.c_uint32()
+    rc = oracle().oracle_block_context_map(ptr(raw_counts), blocks, ptr(thr),
+                                           ctypes.byref(nthr), ptr(cmap), ctypes.byref(nctx))
+    if rc:
+        raise ValueError(f"oracle_block_context_map failed with {rc}")
+    n = int(nthr.value)
+    return {"thresholds": thr[:n].copy(), "context_map": cmap[:3 * 13 * (n + 1)].copy(),
+            "num_contexts": int(nctx.value)}
+
+
+def order_tokens(used_mask: int, orders: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    _declare_entropy_inputs()
+    cap = 3 * 1984 + 15
+    ctx = np.zeros(cap, np.uint32)
+    val = np.zeros(cap, np.uint32)
+    n = ctypes.c_size_t()
+    orders = np.ascontiguousarray(orders, np.uint32)
+    rc = oracle().oracle_order_tokens(used_mask, ptr(orders), ptr(ctx), ptr(val), cap,
+                                      ctypes.byref(n))
+    if rc:
+        raise ValueError(f"oracle_order_tokens failed with {rc}")
+    return ctx[:n.value].copy(), val[:n.value].copy()
+
+
+def effort7_tokens(quantized: np.ndarray, cells: np.ndarray, raw: np.ndarray) -> dict:
+    """The frame's tokens with the coefficient orders and block-context map
+    that effort 7 chooses from its statistics (the reference's serializer,
+    src/codestream/vardct/encoder.cpp:1985-2010), plus those inputs."""
+    stats = frame_statistics(quantized, cells, raw)
+    used, orders = coefficient_orders(stats, True)
+    bh, bw = cells.shape
+    model = block_context_map(stats["raw_counts"], bw * bh)
+    tokens = tokenize_frame(quantized, cells, raw, orders=orders,
+                            thresholds=model["thresholds"], context_map=model["context_map"],
+                            num_contexts=model["num_contexts"])
+    tokens.update({"used_order_mask": used, "orders": orders, "statistics": stats, **model})
+    return tokens
+

@@ -256,6 +256,40 @@ int oracle_tokenize_frame(const int32_t* quantized, const uint8_t* cells,
                           uint16_t* contexts, size_t capacity,
                           uint32_t* group_offsets, uint32_t* histogram);
 
+/* ---- f1/f2: effort-7 tokenization inputs (entropy_inputs.c) ---------- */
+/* UseCoefficientOrderSample draws (coefficient_order.cpp:115-126,626-650),
+ * one per block, groups in raster order.  Pinned: the restatement in
+ * tests/coefficient_order_test.cpp:47-77 (seeds, threshold). */
+void oracle_order_sample_decisions(size_t bw, size_t bh, uint8_t* decisions);
+/* CountGroupZeros (coefficient_order.cpp:223-316) and
+ * CountRawQuantAndOrders (block_context_map.cpp:67-103); layouts in
+ * entropy_inputs.c.  Pinned: brute-force recount in the tests. */
+int oracle_frame_statistics(const int32_t* quantized, const uint8_t* cells,
+                            size_t bw, size_t bh, const int32_t* raw,
+                            const uint8_t* decisions /* nullable */,
+                            uint32_t* zero_counts /* 3*1984 */,
+                            uint32_t* sampled /* 3*64 */, uint16_t* present_mask,
+                            uint32_t* raw_counts /* 13*256 */);
+/* ComputeCoefficientOrdersWithCounts (coefficient_order.cpp:736-811) into
+ * the 7-stage order table.  Pinned: tests/coefficient_order_test.cpp:32-118
+ * (stable sort by uint64(zeros / sqrt(n) + 0.1f) after the LLF prefix). */
+int oracle_coefficient_orders(const uint32_t* zero_counts, const uint32_t* sampled,
+                              uint16_t present_mask, size_t bw, size_t bh,
+                              int effort7_sampled, uint16_t* used_mask,
+                              uint32_t* orders /* 3*2624 */);
+/* ComputeSimpleBlockContextMapForEncoder (block_context_map.cpp:435-476).
+ * Pinned: default map (tests/block_context_map_test.cpp:15-39) and the
+ * invariants of tests/test_oracle_entropy_inputs.py; parity unpinned beyond
+ * them for the merge order (no known answer in the reference's tests). */
+int oracle_block_context_map(const uint32_t* raw_counts, size_t block_count,
+                             uint32_t* thresholds, size_t* nthr, uint8_t* map,
+                             uint32_t* num_contexts);
+/* TokenizeSimpleCoefficientOrders (coefficient_order.cpp:876-919).
+ * Pinned: decode round trip of the Lehmer code in the tests. */
+int oracle_order_tokens(uint16_t used_mask, const uint32_t* orders,
+                        uint32_t* contexts, uint32_t* values, size_t capacity,
+                        size_t* count);
+
 /* ---- fast math: src/util/fast_math.h -------------------------------- */
 float oracle_fast_log2(float value);
 float oracle_fast_pow2(float value);

@@ -1,8 +1,9 @@
 """Writes tests/golden/plain_path_hashes.json: digests of every stage output
 of the oracle's plain path (XYB, Gaborish inverse, initial quant field,
 initial CfL, candidate costs, strategy grid, AQ setup, the three
-Butteraugli-driven evaluations of d1.0 e7 and the AC tokens of the last
-one) on the two seeded workloads, 512x512 ("basic") and 4032x3024
+Butteraugli-driven evaluations of d1.0 e7, the frame statistics, the
+effort-7 coefficient orders and block-context map, and the AC tokens of the
+last evaluation with them) on the two seeded workloads, 512x512 ("basic") and 4032x3024
 ("textured").
 
 The oracle is the plain-C restatement in oracle/ (each function pinned
@@ -51,7 +52,9 @@ def plain_path(name: str) -> dict:
     setup = binding.aq_setup(fe, w, h)
     loop = binding.aq_loop(fe, setup, fe["linear"], w, h)
     ev = loop["evaluation"]
-    tokens = binding.tokenize_frame(ev["quantized"], fe["cells"], ev["raw"])
+    tokens = binding.effort7_tokens(ev["quantized"], fe["cells"], ev["raw"])
+    order_ctx, order_val = binding.order_tokens(tokens["used_order_mask"], tokens["orders"])
+    stats = tokens["statistics"]
     seconds = time.time() - t0
     out = {
         "width": w, "height": h,
@@ -86,6 +89,17 @@ def plain_path(name: str) -> dict:
         "token_values": digest(tokens["values"]),
         "token_contexts": digest(tokens["contexts"]),
         "token_group_offsets": digest(tokens["group_offsets"]),
+        "zero_counts": digest(stats["zero_counts"]),
+        "sampled_zero_counts": digest(stats["sampled"]),
+        "raw_quant_counts": digest(stats["raw_counts"]),
+        "present_mask": stats["present_mask"],
+        "used_order_mask": tokens["used_order_mask"],
+        "orders": digest(tokens["orders"]),
+        "qf_thresholds": [int(v) for v in tokens["thresholds"]],
+        "context_map": digest(tokens["context_map"]),
+        "num_contexts": tokens["num_contexts"],
+        "order_token_count": int(order_val.size),
+        "order_tokens": digest(np.stack([order_ctx, order_val])),
         "strategy_counts": np.bincount(
             (np.ascontiguousarray(fe["cells"])[fe["cells"] & 1 == 1] >> 1).astype(np.int64),
             minlength=16).tolist(),

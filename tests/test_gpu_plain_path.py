@@ -84,6 +84,7 @@ def test_plain_path_matches_oracle_digests(device, name):
         "token_contexts": digest(tokens["contexts"]),
         "token_group_offsets": digest(tokens["group_offsets"]),
     }
+    got.update(entropy_digests(path.statistics.arrays(), path.entropy_inputs))
     path.close()
     # stage order: the first differing entry names where the paths part
     order = ["linear", "opsin", "preprocessed", "initial_quant_field", "pixel_mask",
@@ -91,15 +92,39 @@ def test_plain_path_matches_oracle_digests(device, name):
              "x_scale", "b_scale", "lower", "upper", "quant_dc", "fields",
              "block_distances", "score_bits", "final_field", "raw_quant", "y_to_x",
              "y_to_b", "quantized", "dc_quantized", "reconstructed_linear",
-             "distance_map", "token_count", "token_values", "token_contexts",
-             "token_group_offsets"]
+             "distance_map", *ENTROPY_KEYS, "token_count", "token_values",
+             "token_contexts", "token_group_offsets"]
     differ = [k for k in order if got[k] != want[k]]
     assert not differ, f"{name}: differs from the oracle at {differ}"
 
 
+ENTROPY_KEYS = ("zero_counts", "sampled_zero_counts", "raw_quant_counts", "present_mask",
+                "used_order_mask", "orders", "qf_thresholds", "context_map", "num_contexts",
+                "order_token_count", "order_tokens")
 FRAME_KEYS = ("cells", "final_field", "raw_quant", "y_to_x", "y_to_b", "quantized",
-              "dc_quantized", "token_count", "token_values", "token_contexts",
-              "token_group_offsets", "score_bits")
+              "dc_quantized", "used_order_mask", "orders", "qf_thresholds", "context_map",
+              "num_contexts", "order_token_count", "order_tokens", "token_count",
+              "token_values", "token_contexts", "token_group_offsets", "score_bits")
+
+
+def entropy_digests(stats, inputs):
+    """Digests of the frame statistics (None: not available) and of the
+    effort-7 orders, block-context map and order tokens, as the oracle's
+    table has them."""
+    thresholds, cmap, nctx = inputs.context_model()
+    order_ctx, order_val = gjxl_amd.order_tokens(inputs)
+    out = {"used_order_mask": int(inputs.used_order_mask),
+           "orders": digest(inputs.order_table()),
+           "qf_thresholds": [int(v) for v in thresholds],
+           "context_map": digest(cmap), "num_contexts": nctx,
+           "order_token_count": int(order_val.size),
+           "order_tokens": digest(np.stack([order_ctx, order_val]))}
+    if stats is not None:
+        out.update({"zero_counts": digest(stats["zero_counts"]),
+                    "sampled_zero_counts": digest(stats["sampled"]),
+                    "raw_quant_counts": digest(stats["raw_counts"]),
+                    "present_mask": stats["present_mask"]})
+    return out
 
 
 @pytest.mark.parametrize("name", ["512x512", "4032x3024"])
@@ -130,10 +155,12 @@ def test_native_plain_path_matches_oracle_digests(device, name):
             "token_group_offsets": digest(tokens["group_offsets"]),
             "score_bits": [int(np.float32(v).view(np.uint32)) for v in path.scores()],
         }
+        got.update(entropy_digests(None, path.entropy_inputs()))
         differ = [k for k in FRAME_KEYS if got[k] != want[k]]
         assert not differ, f"{name} run {run}: differs from the oracle at {differ}"
     stages = [s for s, _ in path.stage_times()]
     assert stages[0] == "upload" and stages[-1] == "token_readback"
+    assert "entropy_inputs" in stages
     assert stages.count("aq_evaluate") == 3
     path.close()
 
@@ -178,5 +205,5 @@ def test_cpp_caller_matches_oracle_digests(tmp_path):
         "token_group_offsets": digest(load("group_offsets", np.uint32)),
         "score_bits": [int(np.float32(v).view(np.uint32)) for v in report["scores"]],
     }
-    differ = [k for k in FRAME_KEYS if got[k] != want[k]]
+    differ = [k for k in FRAME_KEYS if k in got and got[k] != want[k]]
     assert not differ, f"C++ caller differs from the oracle at {differ}"

@@ -186,3 +186,42 @@ def test_plain_path_frame(device, size):
     _same(got, want)
     tok.close()
     path.close()
+
+
+@pytest.mark.parametrize("bw,bh,seed,weights", [
+    (40, 40, 1, None), (70, 33, 2, None), (64, 64, 3, [1, 0, 0, 0, 0, 0, 0, 0]),
+    (128, 96, 4, None), (504, 378, 5, None), (504, 378, 6, [1, 0, 0, 0, 0, 0, 0, 0])])
+def test_frame_statistics_and_effort7_tokens(device, bw, bh, seed, weights):
+    """The device's frame statistics equal the oracle's; the orders and map
+    chosen from them, and the tokens with them, equal the oracle's effort-7
+    tokens (504x378 blocks: the 4032x3024 grid, above the 8192-block raw-quant
+    split; DCT8-only grids take the sampled counts)."""
+    rng = np.random.default_rng(seed)
+    cells, q, raw = token_frames.frame(rng, bw, bh, density=0.2, weights=weights)
+    print(f"frame statistics {bw}x{bh} seed {seed}", flush=True)
+    tok = gjxl_amd.Tokenizer(device, bw * 8, bh * 8)
+    d_q, d_cells, d_raw = (device.array(np.ascontiguousarray(a)) for a in (q, cells, raw))
+    stats = tok.frame_statistics(d_q, d_cells, d_raw).arrays()
+    want = binding.effort7_tokens(q, cells, raw)
+    ws = want["statistics"]
+    assert stats["present_mask"] == ws["present_mask"]
+    assert np.array_equal(stats["zero_counts"], ws["zero_counts"])
+    assert np.array_equal(stats["sampled"], ws["sampled"])
+    assert np.array_equal(stats["raw_counts"], ws["raw_counts"])
+    inputs = gjxl_amd.entropy_inputs(tok.frame_statistics(d_q, d_cells, d_raw), 7)
+    assert inputs.used_order_mask == want["used_order_mask"]
+    assert np.array_equal(inputs.order_table(), want["orders"])
+    tok.set_entropy_inputs(inputs)
+    n = tok.tokenize(d_q, d_cells, d_raw)
+    _same(tok.tokens(n), want)
+    tok.close()
+
+
+def test_frame_statistics_reject_raw_quant_out_of_range(device):
+    rng = np.random.default_rng(7)
+    cells, q, raw = token_frames.frame(rng, 40, 40)
+    raw[3, 5] = 0
+    tok = gjxl_amd.Tokenizer(device, 320, 320)
+    with pytest.raises(gjxl_amd.GjxlError, match="raw quantization"):
+        tok.frame_statistics(device.array(q), device.array(cells), device.array(raw))
+    tok.close()
