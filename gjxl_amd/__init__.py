@@ -120,6 +120,7 @@ _SIGNATURES = {
     "gjxl_amd_plain_path_options_init": [_P, _SZ],
     "gjxl_amd_plain_path_create": [_P, _U32, _U32, _P, ctypes.POINTER(_P)],
     "gjxl_amd_plain_path_destroy": [_P],
+    "gjxl_amd_plain_path_reconfigure": [_P, _U32, _U32, _P],
     "gjxl_amd_plain_path_run": [_P, _P, _P, _P],
     "gjxl_amd_plain_path_stage_times": [_P, _P, _P, ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_int32)],
@@ -853,15 +854,40 @@ class NativePlainPath:
         self.handle = handle.value
         self._views = weakref.WeakSet()
         device._children.add(self)
-        bw, bh = pad8(width) // 8, pad8(height) // 8
-        self.bw, self.bh = bw, bh
-        self.groups = ((bw + 31) // 32) * ((bh + 31) // 32)
-        capacity = 3 * 65 * bw * bh
-        self.pixels = HostBuffer((height, width, 3), np.uint8)
+        self.options = opts
+        self.capacity_extent = (width, height)
+        self._extent(width, height)
+        capacity = 3 * 65 * self.bw * self.bh
+        self.pixels = HostBuffer((height * width * 3,), np.uint8)
         self.values = HostBuffer((capacity,), np.uint32)
         self.contexts = HostBuffer((capacity,), np.uint16)
         self.group_offsets = HostBuffer((self.groups + 1,), np.uint32)
         self.frame = PlainFrame()
+
+    def _extent(self, width: int, height: int) -> None:
+        self.width, self.height = width, height
+        bw, bh = pad8(width) // 8, pad8(height) // 8
+        self.bw, self.bh = bw, bh
+        self.groups = ((bw + 31) // 32) * ((bh + 31) // 32)
+
+    def reconfigure(self, width: int, height: int, distance: float | None = None,
+                    intensity_target: float | None = None) -> None:
+        """Re-targets the path to another extent no wider and no taller than
+        the creation extent (and optionally another distance or intensity
+        target) inside the buffers it holds; device views of the last run go
+        stale and are dropped."""
+        opts = PlainPathOptions()
+        ctypes.memmove(ctypes.byref(opts), ctypes.byref(self.options), ctypes.sizeof(opts))
+        if distance is not None:
+            opts.distance = distance
+        if intensity_target is not None:
+            opts.intensity_target = intensity_target
+        _check(library().gjxl_amd_plain_path_reconfigure(self.handle, width, height,
+                                                          ctypes.byref(opts)))
+        _drop_views(self)
+        self.options = opts
+        self.frame = PlainFrame()
+        self._extent(width, height)
 
     def close(self) -> None:
         _drop_views(self)
@@ -886,18 +912,20 @@ class NativePlainPath:
         (packed pixels already in device memory: no upload); returns the
         token count.  With read_tokens the stream lands in the pinned
         buffers."""
+        n = self.width * self.height * 3
         if resident is not None:
-            if resident.nbytes < self.width * self.height * 3:
+            if resident.nbytes < n:
                 raise ValueError("resident input is smaller than the image")
             img = Image(self.width, self.height, 3, 1, resident.ptr,
                         resident.nbytes, self.width * 3)
         else:
             if srgb is not None:
-                self.pixels.array[...] = srgb
-            img = Image(self.width, self.height, 3, 0, self.pixels.ptr,
-                        self.width * self.height * 3, self.width * 3)
+                if srgb.shape != (self.height, self.width, 3):
+                    raise ValueError("image extent differs from the plain path's")
+                self.pixels.array[:n] = np.ascontiguousarray(srgb, np.uint8).reshape(-1)
+            img = Image(self.width, self.height, 3, 0, self.pixels.ptr, n, self.width * 3)
         out = TokenOutput(self.values.ptr, self.contexts.ptr, self.group_offsets.ptr,
-                          self.values.shape[0], self.groups + 1)
+                          self.values.shape[0], self.group_offsets.shape[0])
         _check(library().gjxl_amd_plain_path_run(
             self.handle, ctypes.byref(img), ctypes.byref(out) if read_tokens else None,
             ctypes.byref(self.frame)))
@@ -907,7 +935,7 @@ class NativePlainPath:
         n = int(self.frame.token_count)
         return {"values": self.values.array[:n].copy(),
                 "contexts": self.contexts.array[:n].copy(),
-                "group_offsets": self.group_offsets.array.copy()}
+                "group_offsets": self.group_offsets.array[:self.groups + 1].copy()}
 
     def scores(self) -> list[float]:
         return [float(self.frame.scores[i]) for i in range(self.frame.evaluations)]

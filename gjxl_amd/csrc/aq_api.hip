@@ -28,7 +28,8 @@ struct GJXLAmdAqSession {
   uint32_t strategy_counts[gjxl_amd::kStrategyStages] = {};
   // owned device storage, indexed by GJXL_AMD_AQ_* ids
   void* buffers[GJXL_AMD_AQ_BUFFER_COUNT] = {};
-  size_t sizes[GJXL_AMD_AQ_BUFFER_COUNT] = {};
+  size_t sizes[GJXL_AMD_AQ_BUFFER_COUNT] = {};     // at the current extent
+  size_t capacity[GJXL_AMD_AQ_BUFFER_COUNT] = {};  // as allocated
 };
 
 using gjxl_amd::Check;
@@ -132,19 +133,11 @@ void SelectMatrixScales(const float stats[3], float target, int* xs, int* bs) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
-                                      uint32_t height,
-                                      GJXLAmdAqSession** out) {
-  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
-  if (out == nullptr || *out != nullptr || width == 0 || height == 0 ||
-      width > (1u << 16) || height > (1u << 16)) {
-    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session geometry is invalid");
-  }
-  auto* s = new (std::nothrow) GJXLAmdAqSession;
-  if (s == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "AQ session allocation failed");
-  s->device = device;
+// The extent fields of a session and the byte size of each of its buffers
+// at that extent (every size grows with the width and with the height).
+void SessionGeometry(GJXLAmdAqSession* s, uint32_t width, uint32_t height, size_t* z) {
   s->width = width;
   s->height = height;
   s->pw = gjxl_amd::PadTo8(width);
@@ -157,7 +150,6 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   const size_t groups = s->layout.groups;
   const size_t pixels = static_cast<size_t>(width) * height;
   const size_t padded = static_cast<size_t>(s->pw) * s->ph;
-  size_t* z = s->sizes;
   z[GJXL_AMD_AQ_OFFSETS] = 4 * n;
   z[GJXL_AMD_AQ_ANCHOR_LIST] = 4 * groups * gjxl_amd::kGroupAnchors;
   z[GJXL_AMD_AQ_ANCHOR_COUNT] = 4 * groups;
@@ -182,7 +174,64 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   z[GJXL_AMD_AQ_BLOCK_DISTANCE] = 4 * n;
   z[GJXL_AMD_AQ_SCORES] = 4 * 64;
   z[GJXL_AMD_AQ_STRATEGY_LISTS] = 4 * gjxl_amd::kStrategyStages * n;
+}
+
+// Zeroes the coefficient stores (their unused group tails must read zero)
+// and the statistics words on the device stream, then waits.
+GJXLResult ClearStores(GJXLAmdAqSession* s) {
+  GJXLResult r = GJXL_OK;
+  for (int id : {GJXL_AMD_AQ_COEFFICIENTS, GJXL_AMD_AQ_QUANTIZED, GJXL_AMD_AQ_STATS}) {
+    if (r != GJXL_OK) break;
+    hipLaunchKernelGGL(ZeroWordsKernel, dim3(1024), dim3(256), 0, s->device->stream,
+                       static_cast<uint32_t*>(s->buffers[id]), s->sizes[id] / 4);
+    r = gjxl_amd::CheckLaunch("ZeroWordsKernel");
+  }
+  if (r == GJXL_OK) r = Check(hipStreamSynchronize(s->device->stream), "hipStreamSynchronize");
+  return r;
+}
+
+}  // namespace
+
+namespace gjxl_amd {
+
+GJXLResult ResizeAqSession(GJXLAmdAqSession* s, uint32_t width, uint32_t height) {
+  if (width == 0 || height == 0 || width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session geometry is invalid");
+  }
+  GJXLAmdAqSession probe;
+  size_t need[GJXL_AMD_AQ_BUFFER_COUNT] = {};
+  SessionGeometry(&probe, width, height, need);
   for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
+    if (need[i] > s->capacity[i]) {
+      return Fail(GJXL_ERROR_INVALID_ARGUMENT, "extent exceeds the AQ session's capacity");
+    }
+  }
+  SessionGeometry(s, width, height, s->sizes);
+  s->prepared = false;
+  s->opsin = s->pre = nullptr;
+  s->cells = nullptr;
+  return ClearStores(s);
+}
+
+}  // namespace gjxl_amd
+
+extern "C" {
+
+GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
+                                      uint32_t height,
+                                      GJXLAmdAqSession** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || *out != nullptr || width == 0 || height == 0 ||
+      width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AQ session geometry is invalid");
+  }
+  auto* s = new (std::nothrow) GJXLAmdAqSession;
+  if (s == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "AQ session allocation failed");
+  s->device = device;
+  SessionGeometry(s, width, height, s->sizes);
+  size_t* z = s->sizes;
+  for (int i = 0; i < GJXL_AMD_AQ_BUFFER_COUNT; ++i) {
+    s->capacity[i] = z[i];
     const GJXLResult r = gjxl_amd::DeviceAlloc(device, z[i], &s->buffers[i]);
     if (r != GJXL_OK) {
       gjxl_amd_aq_session_destroy(s);
@@ -191,14 +240,7 @@ GJXLResult gjxl_amd_aq_session_create(GJXLAmdDevice* device, uint32_t width,
   }
   // the coefficient stores hold zeros in unused group tails (cleared on the
   // device stream, which every later stage of the session is ordered after)
-  GJXLResult r = GJXL_OK;
-  for (int id : {GJXL_AMD_AQ_COEFFICIENTS, GJXL_AMD_AQ_QUANTIZED, GJXL_AMD_AQ_STATS}) {
-    if (r != GJXL_OK) break;
-    hipLaunchKernelGGL(ZeroWordsKernel, dim3(1024), dim3(256), 0, device->stream,
-                       static_cast<uint32_t*>(s->buffers[id]), z[id] / 4);
-    r = gjxl_amd::CheckLaunch("ZeroWordsKernel");
-  }
-  if (r == GJXL_OK) r = Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
+  const GJXLResult r = ClearStores(s);
   if (r != GJXL_OK) {
     gjxl_amd_aq_session_destroy(s);
     return r;

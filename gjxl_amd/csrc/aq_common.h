@@ -33,6 +33,11 @@ struct StrategyLists {
   uint32_t counts[kStrategyStages];
 };
 
+// Re-targets the session to an extent whose buffers fit the ones it
+// allocated; the session must be prepared again.  Clears the coefficient
+// stores of the new extent and waits.
+GJXLResult ResizeAqSession(GJXLAmdAqSession* s, uint32_t width, uint32_t height);
+
 // The split-precision forward DCT of every transform of a frame into the
 // per-group coefficient store (split_dct.hip).
 GJXLResult LaunchSplitAnchorDct(GJXLAmdDevice* d, const float* opsin, int width, int height,

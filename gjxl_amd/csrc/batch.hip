@@ -6,9 +6,10 @@
 // encode calls on one driver are serialized, and shutdown drains the active
 // call before joining the workers.  Images of different workers overlap on
 // the GPU (one image's DC wavefront uses eight work groups; the others fill
-// the remaining CUs); a worker keeps its plain path, and so its device
-// buffers, between consecutive images of the same extent and options.
-// No kernels here.
+// the remaining CUs).  A worker keeps one plain path, and so its device
+// buffers, across images: it creates the path at the bounding extent of the
+// call's requests and re-targets it to each image's extent and options
+// inside those buffers.  No kernels here.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -38,6 +39,7 @@ struct Call {
   size_t count = 0;
   std::vector<GJXLAmdBatchResult>* staged = nullptr;
   Clock::time_point arrival;
+  uint32_t bound_width = 0, bound_height = 0;  // over the requests
   std::atomic<size_t> next{0};
   size_t done = 0;  // guarded by the driver's mutex
 };
@@ -45,7 +47,8 @@ struct Call {
 struct Worker {
   GJXLAmdDevice* device = nullptr;
   GJXLAmdPlainPath* path = nullptr;
-  uint32_t width = 0, height = 0;
+  uint32_t capacity_width = 0, capacity_height = 0;  // the path's creation extent
+  uint32_t width = 0, height = 0;                    // its current extent
   GJXLAmdPlainPathOptions options{};
   std::thread thread;
 };
@@ -70,8 +73,60 @@ bool SameOptions(const GJXLAmdPlainPathOptions& a, const GJXLAmdPlainPathOptions
          a.intensity_target == b.intensity_target;
 }
 
+void DropPath(Worker* w) {
+  gjxl_amd_plain_path_destroy(w->path);
+  w->path = nullptr;
+  w->capacity_width = w->capacity_height = w->width = w->height = 0;
+}
+
+// Points the worker's path at the image's extent and options: a reconfigure
+// inside the current buffers when they fit, else a new path at the call's
+// bounding extent (or at the image's own, when the bound is refused).
+GJXLResult PreparePath(Worker* w, const Call& call, uint32_t width, uint32_t height,
+                       const GJXLAmdPlainPathOptions& options) {
+  if (w->path != nullptr && width <= w->capacity_width && height <= w->capacity_height) {
+    if (width == w->width && height == w->height && SameOptions(w->options, options)) {
+      return GJXL_OK;
+    }
+    const GJXLResult r = gjxl_amd_plain_path_reconfigure(w->path, width, height, &options);
+    if (r == GJXL_OK) {
+      w->width = width;
+      w->height = height;
+      w->options = options;
+    } else if (r != GJXL_ERROR_INVALID_ARGUMENT && r != GJXL_ERROR_UNSUPPORTED) {
+      DropPath(w);  // refused arguments leave the path as it was; anything else may not
+    }
+    return r;
+  }
+  DropPath(w);
+  uint32_t cw = std::max(width, call.bound_width);
+  uint32_t ch = std::max(height, call.bound_height);
+  GJXLResult r = gjxl_amd_plain_path_create(w->device, cw, ch, &options, &w->path);
+  if (r == GJXL_OK && (cw != width || ch != height)) {
+    r = gjxl_amd_plain_path_reconfigure(w->path, width, height, &options);
+    if (r != GJXL_OK) DropPath(w);
+  }
+  if (r != GJXL_OK && (cw != width || ch != height)) {
+    w->path = nullptr;
+    cw = width;
+    ch = height;
+    r = gjxl_amd_plain_path_create(w->device, cw, ch, &options, &w->path);
+  }
+  if (r != GJXL_OK) {
+    w->path = nullptr;
+    return r;
+  }
+  w->capacity_width = cw;
+  w->capacity_height = ch;
+  w->width = width;
+  w->height = height;
+  w->options = options;
+  return GJXL_OK;
+}
+
 // One image on one worker into *r (which starts zeroed).
-void RunImage(Worker* w, const GJXLAmdBatchRequest& request, GJXLAmdBatchResult* r) {
+void RunImage(Worker* w, const Call& call, const GJXLAmdBatchRequest& request,
+              GJXLAmdBatchResult* r) {
   GJXLAmdPlainPathOptions options{};
   if (request.options.struct_size == 0) {
     (void)gjxl_amd_plain_path_options_init(&options, sizeof(options));
@@ -86,28 +141,13 @@ void RunImage(Worker* w, const GJXLAmdBatchRequest& request, GJXLAmdBatchResult*
   }
   r->width = image.width;
   r->height = image.height;
-  if (w->path != nullptr &&
-      (w->width != image.width || w->height != image.height || !SameOptions(w->options, options))) {
-    gjxl_amd_plain_path_destroy(w->path);
-    w->path = nullptr;
-  }
-  if (w->path == nullptr) {
-    const GJXLResult created =
-      gjxl_amd_plain_path_create(w->device, image.width, image.height, &options, &w->path);
-    if (created != GJXL_OK) {
-      w->path = nullptr;
-      return SetFailure(r, created);
-    }
-    w->width = image.width;
-    w->height = image.height;
-    w->options = options;
-  }
+  const GJXLResult prepared = PreparePath(w, call, image.width, image.height, options);
+  if (prepared != GJXL_OK) return SetFailure(r, prepared);
   GJXLAmdPlainFrame frame{};
   GJXLResult status = gjxl_amd_plain_path_run(w->path, &image, nullptr, &frame);
   if (status != GJXL_OK) {
     // a failed run may leave the path mid-sequence: start the next image afresh
-    gjxl_amd_plain_path_destroy(w->path);
-    w->path = nullptr;
+    DropPath(w);
     return SetFailure(r, status);
   }
   const size_t count = static_cast<size_t>(frame.token_count);
@@ -171,7 +211,7 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
       if (i >= call->count) break;
       GJXLAmdBatchResult& r = (*call->staged)[i];
       const Clock::time_point start = Clock::now();
-      RunImage(w, call->requests[i], &r);
+      RunImage(w, *call, call->requests[i], &r);
       const Clock::time_point end = Clock::now();
       r.queue_nanoseconds = Nanoseconds(start - call->arrival);
       r.service_nanoseconds = Nanoseconds(end - start);
@@ -185,8 +225,7 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
 void JoinAndRelease(GJXLAmdBatch* b) {
   for (auto& w : b->workers) {
     if (w->thread.joinable()) w->thread.join();
-    gjxl_amd_plain_path_destroy(w->path);
-    w->path = nullptr;
+    DropPath(w.get());
     gjxl_amd_device_destroy(w->device);
     w->device = nullptr;
   }
@@ -260,6 +299,15 @@ GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* b, const GJXLAmdBatchRequest* req
   call.count = count;
   call.staged = &staged;
   call.arrival = arrival;
+  for (size_t i = 0; i < count; ++i) {
+    const GJXLAmdImage& image = requests[i].image;
+    if (image.width == 0 || image.height == 0 || image.width > (1u << 16) ||
+        image.height > (1u << 16)) {
+      continue;  // refused in its own result
+    }
+    call.bound_width = std::max(call.bound_width, image.width);
+    call.bound_height = std::max(call.bound_height, image.height);
+  }
   {
     std::lock_guard<std::mutex> lock(b->mutex);
     b->call = &call;

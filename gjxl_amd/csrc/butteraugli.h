@@ -51,6 +51,7 @@ struct GJXLAmdButteraugli {
   float* map = nullptr;        // requested extent
   uint32_t* score_bits = nullptr;  // max of the map, as float bits
   void* allocation = nullptr;  // one block for everything above
+  size_t capacity_floats = 0;  // its size
 };
 
 namespace gjxl_amd {
@@ -60,6 +61,14 @@ namespace gjxl_amd {
 // it is not null, the score to b->score_bits.
 GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
                               float* map_out);
+
+// Extent fields for (width, height); returns the floats the allocation
+// needs at that extent (it grows with the width and with the height).
+size_t ButteraugliGeometry(GJXLAmdButteraugli* b, uint32_t width, uint32_t height);
+// Points the planes of the current extent into the allocation.
+void CarveButteraugli(GJXLAmdButteraugli* b);
+// Re-targets the object to an extent whose planes fit its allocation.
+GJXLResult ResizeButteraugli(GJXLAmdButteraugli* b, uint32_t width, uint32_t height);
 
 // a17: per-transform 1.2 * (mean d^16)^(1/16) broadcast over the covered
 // blocks.  map: width x height (unpadded); cells: (pw/8) x (ph/8).

@@ -1336,6 +1336,68 @@ GJXLResult LaunchReduceDistanceMap(hipStream_t s, const float* map, int width,
   return CheckLaunch("ReduceKernel");
 }
 
+size_t ButteraugliGeometry(GJXLAmdButteraugli* b, uint32_t width, uint32_t height) {
+  b->width = static_cast<int>(width);
+  b->height = static_cast<int>(height);
+  b->expanded = width < 8 || height < 8;
+  b->has_subscale = !b->expanded && width >= 15 && height >= 15;
+  b->xborder = width < 8 ? (8 - static_cast<int>(width)) / 2 : 0;
+  b->yborder = height < 8 ? (8 - static_cast<int>(height)) / 2 : 0;
+  const int mw = width < 8 ? 8 : static_cast<int>(width);
+  const int mh = height < 8 ? 8 : static_cast<int>(height);
+  b->scale[0].width = mw;
+  b->scale[0].height = mh;
+  b->scale[1].width = (static_cast<int>(width) + 1) / 2;
+  b->scale[1].height = (static_cast<int>(height) + 1) / 2;
+  size_t floats = kSigmaCount * 64 + static_cast<size_t>(width) * height + 64;
+  const int scales = b->has_subscale ? 2 : 1;
+  for (int k = 0; k < scales; ++k) {
+    auto& s = b->scale[k];
+    s.n = static_cast<size_t>(s.width) * s.height;
+    floats += kScalePlanes * s.n;
+  }
+  return floats;
+}
+
+void CarveButteraugli(GJXLAmdButteraugli* b) {
+  float* p = static_cast<float*>(b->allocation);
+  b->taps = p;
+  p += kSigmaCount * 64;
+  b->score_bits = reinterpret_cast<uint32_t*>(p);
+  p += 64;
+  b->map = p;
+  p += static_cast<size_t>(b->width) * b->height;
+  const int scales = b->has_subscale ? 2 : 1;
+  for (int k = 0; k < scales; ++k) {
+    auto& s = b->scale[k];
+    const size_t n = s.n;
+    s.input = p; p += 3 * n;
+    s.xyb = p; p += 3 * n;
+    s.ref_psycho = p; p += 10 * n;
+    s.dist_psycho = p; p += 10 * n;
+    s.ref_blurred = p; p += n;
+    s.mask_y = p; p += n;
+    s.mask_dc = p; p += n;
+    s.stages = p; p += 9 * n;
+    for (auto& t : s.tmp) { t = p; p += n; }
+  }
+}
+
+GJXLResult ResizeButteraugli(GJXLAmdButteraugli* b, uint32_t width, uint32_t height) {
+  if (width == 0 || height == 0 || width > 65535 || height > 65535) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli geometry is invalid");
+  }
+  GJXLAmdButteraugli probe;
+  if (ButteraugliGeometry(&probe, width, height) > b->capacity_floats) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "extent exceeds the Butteraugli capacity");
+  }
+  // the taps stay where they are: they lead the allocation at every extent
+  (void)ButteraugliGeometry(b, width, height);
+  CarveButteraugli(b);
+  b->prepared = false;
+  return GJXL_OK;
+}
+
 }  // namespace gjxl_amd
 
 using gjxl_amd::Check;
@@ -1358,51 +1420,15 @@ GJXLResult gjxl_amd_butteraugli_create(GJXLAmdDevice* device, uint32_t width,
   if (b == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "Butteraugli allocation failed");
   b->device = device;
   b->params = {hf_asymmetry, x_multiplier, intensity_target};
-  b->width = static_cast<int>(width);
-  b->height = static_cast<int>(height);
-  b->expanded = width < 8 || height < 8;
-  b->has_subscale = !b->expanded && width >= 15 && height >= 15;
-  b->xborder = width < 8 ? (8 - static_cast<int>(width)) / 2 : 0;
-  b->yborder = height < 8 ? (8 - static_cast<int>(height)) / 2 : 0;
-  const int mw = width < 8 ? 8 : static_cast<int>(width);
-  const int mh = height < 8 ? 8 : static_cast<int>(height);
-  b->scale[0].width = mw;
-  b->scale[0].height = mh;
-  b->scale[1].width = (static_cast<int>(width) + 1) / 2;
-  b->scale[1].height = (static_cast<int>(height) + 1) / 2;
-  size_t floats = gjxl_amd::kSigmaCount * 64 + static_cast<size_t>(width) * height + 64;
-  const int scales = b->has_subscale ? 2 : 1;
-  for (int k = 0; k < scales; ++k) {
-    auto& s = b->scale[k];
-    s.n = static_cast<size_t>(s.width) * s.height;
-    floats += gjxl_amd::kScalePlanes * s.n;
-  }
+  const size_t floats = gjxl_amd::ButteraugliGeometry(b, width, height);
   GJXLResult r = Check(hipSetDevice(device->ordinal), "hipSetDevice");
   if (r == GJXL_OK) r = gjxl_amd::DeviceAlloc(device, floats * sizeof(float), &b->allocation);
   if (r != GJXL_OK) {
     delete b;
     return r;
   }
-  float* p = static_cast<float*>(b->allocation);
-  b->taps = p;
-  p += gjxl_amd::kSigmaCount * 64;
-  b->score_bits = reinterpret_cast<uint32_t*>(p);
-  p += 64;
-  b->map = p;
-  p += static_cast<size_t>(width) * height;
-  for (int k = 0; k < scales; ++k) {
-    auto& s = b->scale[k];
-    const size_t n = s.n;
-    s.input = p; p += 3 * n;
-    s.xyb = p; p += 3 * n;
-    s.ref_psycho = p; p += 10 * n;
-    s.dist_psycho = p; p += 10 * n;
-    s.ref_blurred = p; p += n;
-    s.mask_y = p; p += n;
-    s.mask_dc = p; p += n;
-    s.stages = p; p += 9 * n;
-    for (auto& t : s.tmp) { t = p; p += n; }
-  }
+  b->capacity_floats = floats;
+  gjxl_amd::CarveButteraugli(b);
   float host[gjxl_amd::kSigmaCount * 64] = {};
   for (int k = 0; k < gjxl_amd::kSigmaCount; ++k) {
     gjxl_amd::HostTaps(gjxl_amd::kSigmas[k], host + k * 64);

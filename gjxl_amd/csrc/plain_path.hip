@@ -31,6 +31,12 @@ namespace {
 constexpr int kMaxMarks = 48;
 constexpr int kMaxEvaluations = 8;
 
+// The path's own device buffers
+enum PathBuffer {
+  kSrgb, kLinear, kOpsin, kPreprocessed, kQuantField, kStrategyMask, kPixelMask,
+  kIqfScratch, kYToX, kYToB, kCosts, kCells, kScoreWords, kPathBuffers
+};
+
 }  // namespace
 
 struct GJXLAmdPlainPath {
@@ -38,9 +44,10 @@ struct GJXLAmdPlainPath {
   uint32_t width = 0, height = 0, pw = 0, ph = 0, bw = 0, bh = 0, groups = 0;
   GJXLAmdPlainPathOptions options{};
   int32_t iterations = 2;
-  // device storage
+  // device storage: own[] as allocated (capacity[] bytes) and typed views
+  void* own[kPathBuffers] = {};
+  size_t capacity[kPathBuffers] = {};
   uint8_t* srgb = nullptr;
-  size_t srgb_bytes = 0;
   float* linear = nullptr;
   float* opsin = nullptr;
   float* preprocessed = nullptr;
@@ -88,9 +95,50 @@ GJXLResult Mark(GJXLAmdPlainPath* p, int32_t stage) {
   return Check(hipEventRecord(p->marks[p->mark_count++], p->device->stream), "hipEventRecord");
 }
 
-template <typename T>
-GJXLResult Alloc(GJXLAmdDevice* d, T** out, size_t bytes) {
-  return gjxl_amd::DeviceAlloc(d, bytes, reinterpret_cast<void**>(out));
+// The extent fields of a path and the byte size of each of its own buffers
+// at that extent (every size grows with the width and with the height).
+void PathGeometry(GJXLAmdPlainPath* p, uint32_t width, uint32_t height, size_t* z) {
+  p->width = width;
+  p->height = height;
+  p->pw = gjxl_amd::PadTo8(width);
+  p->ph = gjxl_amd::PadTo8(height);
+  p->bw = p->pw / 8;
+  p->bh = p->ph / 8;
+  p->groups = gjxl_amd::CeilDiv(p->bw, 32) * gjxl_amd::CeilDiv(p->bh, 32);
+  const size_t pixels = static_cast<size_t>(width) * height;
+  const size_t padded = static_cast<size_t>(p->pw) * p->ph;
+  const size_t blocks = static_cast<size_t>(p->bw) * p->bh;
+  const size_t tiles = static_cast<size_t>(gjxl_amd::CeilDiv(p->bw, 8)) *
+                       gjxl_amd::CeilDiv(p->bh, 8);
+  z[kSrgb] = 4 * pixels;  // room for RGBA rows without padding
+  z[kLinear] = 4 * 3 * pixels;
+  z[kOpsin] = 4 * 3 * padded;
+  z[kPreprocessed] = 4 * 3 * padded;
+  z[kQuantField] = 4 * blocks;
+  z[kStrategyMask] = 4 * blocks;
+  z[kPixelMask] = 4 * padded;
+  z[kIqfScratch] = gjxl_amd_initial_quant_field_scratch_bytes(p->pw, p->ph);
+  z[kYToX] = tiles;
+  z[kYToB] = tiles;
+  z[kCosts] = 4 * 7 * blocks;
+  z[kCells] = blocks;
+  z[kScoreWords] = 4 * kMaxEvaluations;
+}
+
+void PointViews(GJXLAmdPlainPath* p) {
+  p->srgb = static_cast<uint8_t*>(p->own[kSrgb]);
+  p->linear = static_cast<float*>(p->own[kLinear]);
+  p->opsin = static_cast<float*>(p->own[kOpsin]);
+  p->preprocessed = static_cast<float*>(p->own[kPreprocessed]);
+  p->quant_field = static_cast<float*>(p->own[kQuantField]);
+  p->strategy_mask = static_cast<float*>(p->own[kStrategyMask]);
+  p->pixel_mask = static_cast<float*>(p->own[kPixelMask]);
+  p->iqf_scratch = static_cast<float*>(p->own[kIqfScratch]);
+  p->y_to_x = static_cast<int8_t*>(p->own[kYToX]);
+  p->y_to_b = static_cast<int8_t*>(p->own[kYToB]);
+  p->costs = static_cast<float*>(p->own[kCosts]);
+  p->cells = static_cast<uint8_t*>(p->own[kCells]);
+  p->score_words = static_cast<uint32_t*>(p->own[kScoreWords]);
 }
 
 template <typename T>
@@ -323,34 +371,12 @@ GJXLResult gjxl_amd_plain_path_create(GJXLAmdDevice* device, uint32_t width, uin
   p->device = device;
   p->options = *options;
   p->iterations = 2;
-  p->width = width;
-  p->height = height;
-  p->pw = gjxl_amd::PadTo8(width);
-  p->ph = gjxl_amd::PadTo8(height);
-  p->bw = p->pw / 8;
-  p->bh = p->ph / 8;
-  p->groups = gjxl_amd::CeilDiv(p->bw, 32) * gjxl_amd::CeilDiv(p->bh, 32);
-  const size_t pixels = static_cast<size_t>(width) * height;
-  const size_t padded = static_cast<size_t>(p->pw) * p->ph;
-  const size_t blocks = static_cast<size_t>(p->bw) * p->bh;
-  const size_t tiles = static_cast<size_t>(gjxl_amd::CeilDiv(p->bw, 8)) *
-                       gjxl_amd::CeilDiv(p->bh, 8);
-  p->srgb_bytes = 4 * pixels;  // room for RGBA rows without padding
-  GJXLResult r = Alloc(device, &p->srgb, p->srgb_bytes);
-  if (r == GJXL_OK) r = Alloc(device, &p->linear, 4 * 3 * pixels);
-  if (r == GJXL_OK) r = Alloc(device, &p->opsin, 4 * 3 * padded);
-  if (r == GJXL_OK) r = Alloc(device, &p->preprocessed, 4 * 3 * padded);
-  if (r == GJXL_OK) r = Alloc(device, &p->quant_field, 4 * blocks);
-  if (r == GJXL_OK) r = Alloc(device, &p->strategy_mask, 4 * blocks);
-  if (r == GJXL_OK) r = Alloc(device, &p->pixel_mask, 4 * padded);
-  if (r == GJXL_OK) {
-    r = Alloc(device, &p->iqf_scratch, gjxl_amd_initial_quant_field_scratch_bytes(p->pw, p->ph));
+  PathGeometry(p, width, height, p->capacity);
+  GJXLResult r = GJXL_OK;
+  for (int i = 0; r == GJXL_OK && i < kPathBuffers; ++i) {
+    r = gjxl_amd::DeviceAlloc(device, p->capacity[i], &p->own[i]);
   }
-  if (r == GJXL_OK) r = Alloc(device, &p->y_to_x, tiles);
-  if (r == GJXL_OK) r = Alloc(device, &p->y_to_b, tiles);
-  if (r == GJXL_OK) r = Alloc(device, &p->costs, 4 * 7 * blocks);
-  if (r == GJXL_OK) r = Alloc(device, &p->cells, blocks);
-  if (r == GJXL_OK) r = Alloc(device, &p->score_words, 4 * kMaxEvaluations);
+  PointViews(p);
   if (r == GJXL_OK) {
     r = Check(hipHostMalloc(reinterpret_cast<void**>(&p->host_words), 4 * 32,
                             hipHostMallocDefault), "hipHostMalloc(plain path)");
@@ -399,13 +425,46 @@ void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* p) {
   if (p->host_statistics != nullptr) (void)hipHostFree(p->host_statistics);
   delete p->statistics;
   delete p->entropy_inputs;
-  void* buffers[] = {p->srgb, p->linear, p->opsin, p->preprocessed, p->quant_field,
-                     p->strategy_mask, p->pixel_mask, p->iqf_scratch, p->y_to_x, p->y_to_b,
-                     p->costs, p->cells, p->score_words};
-  for (void* b : buffers) {
+  for (void* b : p->own) {
     gjxl_amd::DeviceFree(p->device, b);
   }
   delete p;
+}
+
+GJXLResult gjxl_amd_plain_path_reconfigure(GJXLAmdPlainPath* p, uint32_t width,
+                                           uint32_t height,
+                                           const GJXLAmdPlainPathOptions* options) {
+  GJXL_AMD_TRY(Validate(p));
+  if (width == 0 || height == 0 || width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path geometry is invalid");
+  }
+  if (options != nullptr) GJXL_AMD_TRY(ValidateOptions(options));
+  GJXLAmdPlainPath probe;
+  size_t need[kPathBuffers] = {};
+  PathGeometry(&probe, width, height, need);
+  for (int i = 0; i < kPathBuffers; ++i) {
+    if (need[i] > p->capacity[i]) {
+      return Fail(GJXL_ERROR_INVALID_ARGUMENT, "extent exceeds the plain path's capacity");
+    }
+  }
+  GJXL_AMD_TRY(Check(hipSetDevice(p->device->ordinal), "hipSetDevice"));
+  // the last run's device views go stale from here on
+  p->ran = false;
+  p->token_count = 0;
+  // the parts check their own capacities; a refusal leaves every part at
+  // the old extent
+  GJXL_AMD_TRY(gjxl_amd::ResizeAqSession(p->session, width, height));
+  GJXLResult r = gjxl_amd::ResizeButteraugli(p->butteraugli, width, height);
+  if (r == GJXL_OK) r = gjxl_amd::ResizeTokenizer(p->tokenizer, width, height);
+  if (r != GJXL_OK) {
+    (void)gjxl_amd::ResizeAqSession(p->session, p->width, p->height);
+    (void)gjxl_amd::ResizeButteraugli(p->butteraugli, p->width, p->height);
+    (void)gjxl_amd::ResizeTokenizer(p->tokenizer, p->width, p->height);
+    return r;
+  }
+  PathGeometry(p, width, height, need);
+  if (options != nullptr) p->options = *options;
+  return GJXL_OK;
 }
 
 GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* image,

@@ -26,4 +26,8 @@ GJXLResult FrameStatisticsResult(GJXLAmdTokenizer* t, const uint32_t* words,
                                  GJXLAmdFrameStatistics* out);
 size_t FrameStatisticsWords();
 
+// Re-targets the tokenizer to an extent whose buffers fit the ones it
+// allocated; uploads that extent's sample decisions and waits.
+GJXLResult ResizeTokenizer(GJXLAmdTokenizer* t, uint32_t width, uint32_t height);
+
 }  // namespace gjxl_amd

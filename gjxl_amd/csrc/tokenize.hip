@@ -506,7 +506,8 @@ struct GJXLAmdTokenizer {
   gjxl_amd::AqLayout layout{};
   gjxl_amd::ContextModel model{};
   void* buffers[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
-  size_t sizes[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
+  size_t sizes[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};     // at the current extent
+  size_t capacity[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};  // as allocated
   // pinned staging of the order table and its inverse; staged_copy marks
   // the end of the last copy out of it
   uint32_t* staged_orders = nullptr;
@@ -558,6 +559,51 @@ GJXLResult UploadOrders(GJXLAmdTokenizer* t, const std::vector<uint32_t>& table)
   GJXL_AMD_TRY(Check(hipEventRecord(t->staged_copy, st), "hipEventRecord"));
   t->staged_pending = true;
   return GJXL_OK;
+}
+
+// The extent fields of a tokenizer and the byte size of each buffer at that
+// extent (every size grows with the width and with the height); false when
+// the frame exceeds the 32-bit token offsets.
+bool TokenizerGeometry(GJXLAmdTokenizer* t, uint32_t width, uint32_t height, size_t* z) {
+  using namespace gjxl_amd;
+  const AqLayout layout = MakeAqLayout(PadTo8(width), PadTo8(height));
+  const size_t blocks = static_cast<size_t>(layout.bw) * layout.bh;
+  const size_t groups = layout.groups;
+  // every block holds 64 coefficients per channel and every transform adds
+  // one count token: 3 * (64 * blocks + anchors) with anchors <= blocks
+  const size_t capacity = 3 * 65 * blocks;
+  if (capacity > 0xffffffffull) return false;  // token offsets and counts are 32-bit
+  t->width = width;
+  t->height = height;
+  t->layout = layout;
+  z[GJXL_AMD_TOKENS_VALUES] = 4 * capacity;
+  z[GJXL_AMD_TOKENS_CONTEXTS] = 2 * capacity;
+  z[GJXL_AMD_TOKENS_GROUP_OFFSETS] = 4 * (groups + 1);
+  z[GJXL_AMD_TOKENS_HISTOGRAM] =
+    4 * static_cast<size_t>(kMaxBlockContexts) * (kNonzeroBuckets + kZeroDensityContexts) * kSymbols;
+  z[GJXL_AMD_TOKENS_NONZERO_MAP] = 3 * blocks;
+  z[GJXL_AMD_TOKENS_TASKS] = 4 * groups * kGroupTasks;
+  z[GJXL_AMD_TOKENS_TASK_OFFSETS] = 4 * groups * kGroupTasks;
+  z[GJXL_AMD_TOKENS_GROUP_TOKENS] = 4 * groups;
+  z[GJXL_AMD_TOKENS_OFFSETS] = 4 * blocks;
+  z[GJXL_AMD_TOKENS_ANCHOR_LIST] = 4 * groups * kGroupAnchors;
+  z[GJXL_AMD_TOKENS_ANCHOR_COUNT] = 4 * groups;
+  z[GJXL_AMD_TOKENS_ORDERS] = 4 * 3 * kOrderStride;
+  z[GJXL_AMD_TOKENS_INVERSE_ORDERS] = 2 * 3 * kOrderStride;
+  z[GJXL_AMD_TOKENS_STATUS] = 4 * 4;
+  z[GJXL_AMD_TOKENS_STATISTICS] = 4 * kStatWords;
+  z[GJXL_AMD_TOKENS_SAMPLE_DECISIONS] = blocks;
+  return true;
+}
+
+GJXLResult UploadSampleDecisions(GJXLAmdTokenizer* t) {
+  const std::vector<uint8_t> decisions = gjxl_amd::SampleDecisions(t->layout);
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_SAMPLE_DECISIONS],
+                                    decisions.data(), decisions.size(),
+                                    hipMemcpyHostToDevice, t->device->stream),
+                     "hipMemcpyAsync(sample decisions)"));
+  // the pageable source is gone when this returns
+  return Check(hipStreamSynchronize(t->device->stream), "hipStreamSynchronize");
 }
 
 }  // namespace
@@ -661,39 +707,14 @@ GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device, uint32_t width, uint
   auto* t = new (std::nothrow) GJXLAmdTokenizer;
   if (t == nullptr) return Fail(GJXL_ERROR_OUT_OF_MEMORY, "tokenizer allocation failed");
   t->device = device;
-  t->width = width;
-  t->height = height;
-  t->layout = MakeAqLayout(PadTo8(width), PadTo8(height));
-  const size_t blocks = static_cast<size_t>(t->layout.bw) * t->layout.bh;
-  const size_t groups = t->layout.groups;
-  // every block holds 64 coefficients per channel and every transform adds
-  // one count token: 3 * (64 * blocks + anchors) with anchors <= blocks
-  const size_t capacity = 3 * 65 * blocks;
-  if (capacity > 0xffffffffull) {
-    // token offsets and counts are 32-bit words
+  if (!TokenizerGeometry(t, width, height, t->sizes)) {
     delete t;
     return Fail(GJXL_ERROR_UNSUPPORTED,
                 "frame too large for 32-bit token offsets (at most 22,025,473 blocks)");
   }
   size_t* z = t->sizes;
-  z[GJXL_AMD_TOKENS_VALUES] = 4 * capacity;
-  z[GJXL_AMD_TOKENS_CONTEXTS] = 2 * capacity;
-  z[GJXL_AMD_TOKENS_GROUP_OFFSETS] = 4 * (groups + 1);
-  z[GJXL_AMD_TOKENS_HISTOGRAM] =
-    4 * static_cast<size_t>(kMaxBlockContexts) * (kNonzeroBuckets + kZeroDensityContexts) * kSymbols;
-  z[GJXL_AMD_TOKENS_NONZERO_MAP] = 3 * blocks;
-  z[GJXL_AMD_TOKENS_TASKS] = 4 * groups * kGroupTasks;
-  z[GJXL_AMD_TOKENS_TASK_OFFSETS] = 4 * groups * kGroupTasks;
-  z[GJXL_AMD_TOKENS_GROUP_TOKENS] = 4 * groups;
-  z[GJXL_AMD_TOKENS_OFFSETS] = 4 * blocks;
-  z[GJXL_AMD_TOKENS_ANCHOR_LIST] = 4 * groups * kGroupAnchors;
-  z[GJXL_AMD_TOKENS_ANCHOR_COUNT] = 4 * groups;
-  z[GJXL_AMD_TOKENS_ORDERS] = 4 * 3 * kOrderStride;
-  z[GJXL_AMD_TOKENS_INVERSE_ORDERS] = 2 * 3 * kOrderStride;
-  z[GJXL_AMD_TOKENS_STATUS] = 4 * 4;
-  z[GJXL_AMD_TOKENS_STATISTICS] = 4 * kStatWords;
-  z[GJXL_AMD_TOKENS_SAMPLE_DECISIONS] = blocks;
   for (int i = 0; i < GJXL_AMD_TOKENS_BUFFER_COUNT; ++i) {
+    t->capacity[i] = z[i];
     const GJXLResult r = gjxl_amd::DeviceAlloc(device, z[i], &t->buffers[i]);
     if (r != GJXL_OK) {
       gjxl_amd_tokenizer_destroy(t);
@@ -710,14 +731,7 @@ GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device, uint32_t width, uint
   }
   if (r == GJXL_OK) r = Check(hipEventCreate(&t->staged_copy), "hipEventCreate");
   if (r == GJXL_OK) r = gjxl_amd_tokenizer_set_orders(t, 0, nullptr);
-  if (r == GJXL_OK) {
-    const std::vector<uint8_t> decisions = SampleDecisions(t->layout);
-    r = Check(hipMemcpyAsync(t->buffers[GJXL_AMD_TOKENS_SAMPLE_DECISIONS], decisions.data(),
-                             decisions.size(), hipMemcpyHostToDevice, device->stream),
-              "hipMemcpyAsync(sample decisions)");
-    // the pageable source is gone when this returns
-    if (r == GJXL_OK) r = Check(hipStreamSynchronize(device->stream), "hipStreamSynchronize");
-  }
+  if (r == GJXL_OK) r = UploadSampleDecisions(t);
   if (r == GJXL_OK) {
     // DefaultSimpleBlockContextMap: src/codestream/simple_ac_context.h:29-33
     uint8_t map[3 * kOrderFamilies];
@@ -900,6 +914,19 @@ GJXLResult FrameStatisticsResult(GJXLAmdTokenizer* t, const uint32_t* words,
 }
 
 size_t FrameStatisticsWords() { return kStatWords; }
+
+GJXLResult ResizeTokenizer(GJXLAmdTokenizer* t, uint32_t width, uint32_t height) {
+  if (width == 0 || height == 0 || width > (1u << 16) || height > (1u << 16)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "tokenizer geometry is invalid");
+  }
+  GJXLAmdTokenizer probe;
+  size_t need[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
+  bool fits = TokenizerGeometry(&probe, width, height, need);
+  for (int i = 0; fits && i < GJXL_AMD_TOKENS_BUFFER_COUNT; ++i) fits = need[i] <= t->capacity[i];
+  if (!fits) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "extent exceeds the tokenizer's capacity");
+  (void)TokenizerGeometry(t, width, height, t->sizes);
+  return UploadSampleDecisions(t);
+}
 
 }  // namespace gjxl_amd
 

@@ -666,6 +666,17 @@ GJXL_AMD_API GJXLResult gjxl_amd_plain_path_create(
   GJXLAmdDevice* device, uint32_t width, uint32_t height,
   const GJXLAmdPlainPathOptions* options, GJXLAmdPlainPath** out);
 GJXL_AMD_API void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* path);
+/* Re-targets the path to another extent (and, when options is not null,
+ * other options) inside the buffers allocated at creation: every extent
+ * whose width and height do not exceed the creation extent's fits.  No
+ * device allocation; the views of the last run's frame go stale.  A refused
+ * extent (GJXL_ERROR_INVALID_ARGUMENT) leaves the path as it was.  The
+ * reference builds its per-image state in each encode call
+ * (src/c_api/gjxl.cpp:470-637); this is the reuse its batch encoder
+ * (src/codestream/batch_workflow.h:47-109) leaves to the allocator. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_reconfigure(
+  GJXLAmdPlainPath* path, uint32_t width, uint32_t height,
+  const GJXLAmdPlainPathOptions* options);
 /* Runs the path on one image of the path's extent and returns when the
  * tokens are on the host (tokens may be null: they stay on the device).
  * frame may be null. */

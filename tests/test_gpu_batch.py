@@ -1,7 +1,8 @@
 """GPU: the batch driver (gjxl_amd_batch_*).
 
-Images of uneven extents (two share an extent, so a worker reuses its plain
-path) run on three workers at once.  Every result equals a lone run of the
+Images of uneven extents run on three workers at once; each worker creates
+its plain path at the bounding extent of the call and re-targets it to every
+image it takes.  Every result equals a lone run of the
 C++ plain path on the same image (tokens, group offsets, scores, orders and
 block-context map), the 512x512 result equals the oracle's digests, request
 order is kept, a bad request fails alone, and a shut-down driver refuses
