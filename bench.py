@@ -622,7 +622,8 @@ def main() -> int:
                 "span": "upload from pinned host memory included",
                 "steps": args.pcie_steps,
                 "ms_per_step": round(pcie_s * 1e3 / args.pcie_steps, 4),
-                "value": round(mp * args.pcie_steps / pcie_s, 3), "unit": "MP/s"},
+                "value": round(mp * args.pcie_steps / pcie_s, 3),
+                "unit": "MP/s"} if args.pcie_steps > 0 else None,
             "concurrent_images": concurrent,
             "batch_driver": batch,
             "split_precision_dct": split,

@@ -1018,24 +1018,32 @@ __global__ __launch_bounds__(kAqThreads) void CodeTransformKernel(
 // (x, y) is e(x, y-1) + e(x-1, y) because the update of (x-1, y) adds its
 // errors into the previous row's slot x; "north-west" is
 // e(x-1, y-1) + e(x-2, y); "north-east" e(x+1, y-1).  Every input of (x, y)
-// is therefore final at wavefront step t = x + 2y.  A work group walks the
-// wavefront of one DC group (<= 256 x 256 blocks): the thread that owns row y
-// keeps the row's last two samples in registers; row y reads the samples it
-// needs from rows y-1 and y-2 (x-1, x, x+1 of row y-1, x of row y-2,
-// produced 1 to 4 steps earlier) from an 8-slot ring in LDS that their
-// owners write each step.
+// is therefore final once (x+1, y-1) and (x-1, y) are.
+//
+// A work group walks the wavefront of one DC group (<= 256 x 256 blocks) for
+// two channels.  Each channel has two waves; lane l of wave h owns rows
+// 64h + l and 128 + 64h + l, so the row above a lane's row is the previous
+// lane's, except on lane 0.  Row y processes sample x at step
+// off(y) + x with off(y) = 2y + D * (y / 64) (+ D for B, below): sample
+// (x+1, y-1) is one step old when (x, y) needs it.  Within a wave, lane l
+// takes the outputs of lane l - 1's last sample by a lane shift of the
+// registers (DPP wave_shr:1); its north and north-west cells are the
+// north-east cells of the two previous steps, and north-north is the north
+// of its north.  Lane 0 takes them from a ring in LDS that lane 63 of the
+// other wave of its channel writes; the extra D steps of lag at every band
+// of 64 rows make that data at least D + 1 steps old, so a barrier every D
+// steps orders it.  B subtracts the quantized Y of the same sample, which
+// the Y wave of its row wrote into a per-row LDS ring D steps earlier.
 
 constexpr int kDcGroup = 256;
-constexpr int kRing = 8;
-
-struct DcRing {
-  int32_t q[kRing][kDcGroup];       // quantized value
-  int32_t err[kRing][kDcGroup];     // signed prediction error
-  uint32_t e[4][kRing][kDcGroup];   // sub-predictor errors
-};
+constexpr int kDcBatch = 8;            // steps per barrier (D)
+constexpr int kDcEdgeRing = 32;        // > 2D + 2 slots of a band edge
+constexpr int kDcYRing = 16;           // >= 2D slots of quantized Y per row
+constexpr int kDcRowThreads = kDcGroup / 2;
+constexpr int kDcWorkGroup = 2 * kDcRowThreads;
 
 // (1 << 24) / (i + 1) for the arguments the predictor forms: e >> shift
-// is below 64 and the final weight sum minus one below 127, so one table of
+// is below 64 and the final weight sum minus one below 32, so one table of
 // 128 entries covers both (filled per work group).
 constexpr int kReciprocals = 128;
 
@@ -1047,6 +1055,14 @@ __device__ __forceinline__ int BitWidth32(uint32_t v) {
   return 32 - __clz(static_cast<int>(v));
 }
 
+// What the row below reads of a sample (x, y): its quantized value,
+// prediction error and sub-predictor errors, and q(x, y - 1).
+struct DcCell {
+  int32_t q = 0, err = 0;
+  uint32_t e[4] = {0, 0, 0, 0};
+  int32_t q_up = 0;
+};
+
 // The predictor state a row keeps in registers: its own last two samples.
 struct DcRowState {
   int32_t q1 = 0, err1 = 0;
@@ -1054,62 +1070,52 @@ struct DcRowState {
 };
 
 // Bounds of the 32-bit evaluation: while every quantized value stays below
-// 2^18 and every stored prediction error below 2^22 in magnitude, each
-// sub-prediction is below 2^23, the weighted sum below 2^28 and the
-// prediction below 2^24, so 32-bit arithmetic gives the 64-bit results.
+// 2^18 and every stored prediction error below 2^21 in magnitude, each sum
+// of three errors fits the signed 24-bit multiplier, each sub-prediction is
+// below 2^23, the weighted sum below 2^28 and the prediction below 2^24, so
+// 32-bit arithmetic gives the 64-bit results.
 constexpr int32_t kDcNarrowQ = 1 << 18;
-constexpr int32_t kDcNarrowErr = 1 << 22;
+constexpr int32_t kDcNarrowErr = 1 << 21;
 
-// One sample (x, y) of one channel: weighted prediction from the ring and
-// the row state, quantisation of the residual, update of the ring slot and
-// the row state.  Returns the quantized DC value.  kWide evaluates the
-// predictor in 64-bit integers as the reference does; otherwise in 32-bit
-// integers, exact while the inputs keep the bounds above, and `overflow` is
-// set when this sample's outputs leave them (later samples would then read
-// out-of-bound inputs).
+// One sample (x, y) of one channel from its north-east, north and
+// north-west cells (the north-north value is n.q_up) and the row state:
+// weighted prediction, quantisation of the residual, update of the row
+// state.  Returns the sample's cell.  Row 0 reads all-zero north cells, so
+// the reference's zero errors and zero "up" value above the first row need
+// no selects; the row state's errors read zero at x = 0 and x = 1 because
+// every step off the row stores zero errors (every row starts at least two
+// steps after the thread's last sample).  The sample is evaluated on every
+// step; off the row (`active` false) its result is never read.  kWide
+// evaluates the predictor in 64-bit integers as the reference does;
+// otherwise in 32-bit integers, exact while the inputs keep the bounds
+// above, and `overflow` is set when an active sample's outputs leave them
+// (later samples would then read out-of-bound inputs).  Cells of absent
+// neighbours hold values that the selects discard.
 template <bool kWide>
-__device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
-                                            const uint32_t* reciprocal, int x, int y,
-                                            bool has_ne, float value, float inv,
-                                            bool& overflow) {
+__device__ __forceinline__ DcCell DcSample(const DcCell& ne, const DcCell& n, const DcCell& nw,
+                                           DcRowState& st, const uint32_t* reciprocal, int x,
+                                           int y, bool has_ne, bool active, float value,
+                                           float inv, bool& overflow) {
   using I = std::conditional_t<kWide, int64_t, int32_t>;
   using U = std::conditional_t<kWide, uint64_t, uint32_t>;
   constexpr uint8_t kCoeff[7] = {16, 10, 7, 7, 7, 0, 0};
   constexpr uint8_t kMaxWeights[4] = {13, 12, 12, 12};
-  const int s0 = x & (kRing - 1);
-  const int sm = (x - 1) & (kRing - 1), sp = (x + 1) & (kRing - 1);
-  // Every ring word the sample may need is read unconditionally (rows y - 1
-  // and y - 2 clamped to valid rows; slots of absent neighbours hold values
-  // that the selects below discard), so that the reads issue together.
-  const int ym1 = y > 0 ? y - 1 : 0, ym2 = y > 1 ? y - 2 : 0;
-  const int32_t q_n = ring.q[s0][ym1], q_nw = ring.q[sm][ym1];
-  const int32_t q_ne = ring.q[sp][ym1], q_nn = ring.q[s0][ym2];
-  const int32_t err_n = ring.err[s0][ym1], err_nw = ring.err[sm][ym1];
-  const int32_t err_ne = ring.err[sp][ym1];
-  uint32_t e_n[4], e_nw[4], e_ne[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    e_n[i] = ring.e[i][s0][ym1];
-    e_nw[i] = ring.e[i][sm][ym1];
-    e_ne[i] = ring.e[i][sp][ym1];
-  }
   // quantized neighbours
-  const I up = y ? q_n : 0;
+  const I up = n.q;
   const I W_ = x ? st.q1 : up;
   const I N_ = y ? up : W_;
-  const I NW_ = x && y ? q_nw : W_;
-  const I NE_ = y && has_ne ? q_ne : N_;
-  const I NN_ = y > 1 ? q_nn : N_;
+  const I NW_ = x && y ? nw.q : W_;
+  const I NE_ = y && has_ne ? ne.q : N_;
+  const I NN_ = y > 1 ? n.q_up : N_;
   // Predict (prediction.h:75-123)
   uint32_t weights[4];
   uint32_t weight_sum = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const uint32_t en_i = y ? e_n[i] : 0u;
-    const uint32_t north = en_i + (x ? st.e1[i] : 0u);
-    const uint32_t ne = has_ne ? (y ? e_ne[i] : 0u) : north;
-    const uint32_t nw = x ? (y ? e_nw[i] : 0u) + (x > 1 ? st.e2[i] : 0u) : north;
-    const uint32_t e = north + ne + nw;
+    const uint32_t north = n.e[i] + st.e1[i];
+    const uint32_t e_ne = has_ne ? ne.e[i] : north;
+    const uint32_t e_nw = x ? nw.e[i] + st.e2[i] : north;
+    const uint32_t e = north + e_ne + e_nw;
     int shift = BitWidthPlusOne(e) - 6;
     if (shift < 0) shift = 0;
     // k * r as k * (r - 1) + k: r - 1 < 2^24 fits the 24-bit multiplier
@@ -1124,17 +1130,29 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
     weights[i] >>= log_weight - 4;
     weight_sum += weights[i];
   }
-  const I ew = x ? st.err1 : 0;
-  const I en = y ? err_n : 0;
-  const I enw = x ? (y ? err_nw : 0) : en;
-  const I ene = has_ne ? (y ? err_ne : 0) : en;
+  const I ew = st.err1;
+  const I en = n.err;
+  const I enw = x ? nw.err : en;
+  const I ene = has_ne ? ne.err : en;
   const I n8 = N_ * 8, w8 = W_ * 8, ne8 = NE_ * 8, nw8 = NW_ * 8, nn8 = NN_ * 8;
+  // the reference's sums of products, with the three equal error
+  // coefficients of the last one factored (the same integer)
+  static_assert(kCoeff[2] == kCoeff[3] && kCoeff[3] == kCoeff[4], "factored coefficients");
+  I c1, c2, c3;
+  if constexpr (kWide) {
+    c1 = (en + ew + ene) * kCoeff[0];
+    c2 = (en + ew + enw) * kCoeff[1];
+    c3 = (enw + en + ene) * kCoeff[2];
+  } else {
+    c1 = __mul24(en + ew + ene, static_cast<int32_t>(kCoeff[0]));
+    c2 = __mul24(en + ew + enw, static_cast<int32_t>(kCoeff[1]));
+    c3 = __mul24(enw + en + ene, static_cast<int32_t>(kCoeff[2]));
+  }
   I pred[4];
   pred[0] = w8 + ne8 - n8;
-  pred[1] = n8 - (((en + ew + ene) * kCoeff[0]) >> 5);
-  pred[2] = w8 - (((en + ew + enw) * kCoeff[1]) >> 5);
-  pred[3] = n8 - ((enw * kCoeff[2] + en * kCoeff[3] + ene * kCoeff[4] +
-                   (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
+  pred[1] = n8 - (c1 >> 5);
+  pred[2] = w8 - (c2 >> 5);
+  pred[3] = n8 - ((c3 + (nn8 - n8) * kCoeff[5] + (nw8 - w8) * kCoeff[6]) >> 5);
   I sum = static_cast<I>((weight_sum >> 1) - 1);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -1145,8 +1163,10 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
       sum += __mul24(pred[i], static_cast<int32_t>(weights[i]));
     }
   }
+  // the table entry is below 2^24: a signed 32-bit operand
   I prediction = static_cast<I>(
-    (static_cast<int64_t>(sum) * static_cast<int64_t>(reciprocal[weight_sum - 1])) >> 24);
+    (static_cast<int64_t>(sum) *
+     static_cast<int64_t>(static_cast<int32_t>(reciprocal[weight_sum - 1]))) >> 24);
   if (((en ^ ew) | (en ^ enw)) <= 0) {
     I lo = w8 < ne8 ? w8 : ne8;
     lo = lo < n8 ? lo : n8;
@@ -1161,116 +1181,181 @@ __device__ __forceinline__ int32_t DcSample(DcRing& ring, DcRowState& st,
   if (residual > -0.62f && residual < 0.62f) residual = 0;
   float rounded = roundf(residual);
   if (rounded > 2 || rounded < -2) rounded = roundf(residual * 0.5f) * 2;
-  const int32_t ir = static_cast<int32_t>(roundf(rounded));
+  // rounded holds an integer
+  const int32_t ir = static_cast<int32_t>(rounded);
+  DcCell out;
   // the low 32 bits of predicted + ir
-  const int32_t value_q =
-    static_cast<int32_t>(static_cast<uint32_t>(predicted) + static_cast<uint32_t>(ir));
+  out.q = static_cast<int32_t>(static_cast<uint32_t>(predicted) + static_cast<uint32_t>(ir));
+  out.q_up = n.q;
   // Update (prediction.h:124-144); unsigned arithmetic wraps where a 32-bit
   // evaluation leaves its bounds (the sample is then flagged)
-  const I scaled = static_cast<I>(static_cast<U>(static_cast<I>(value_q)) << 3);
-  const int32_t err = static_cast<int32_t>(static_cast<U>(prediction) - static_cast<U>(scaled));
-  ring.q[s0][y] = value_q;
-  ring.err[s0][y] = err;
+  const I scaled = static_cast<I>(static_cast<U>(static_cast<I>(out.q)) << 3);
+  out.err = static_cast<int32_t>(static_cast<U>(prediction) - static_cast<U>(scaled));
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    I dd = static_cast<I>(static_cast<U>(pred[i]) - static_cast<U>(scaled));
-    const U magnitude = dd < 0 ? U(0) - static_cast<U>(dd) : static_cast<U>(dd);
-    const uint32_t ei = static_cast<uint32_t>((magnitude + 3) >> 3);
-    ring.e[i][s0][y] = ei;
+    if constexpr (kWide) {
+      I dd = static_cast<I>(static_cast<U>(pred[i]) - static_cast<U>(scaled));
+      const U magnitude = dd < 0 ? U(0) - static_cast<U>(dd) : static_cast<U>(dd);
+      out.e[i] = static_cast<uint32_t>((magnitude + 3) >> 3);
+    } else {
+      // |pred - scaled| + 3 on sign-flipped words (the order of signed words
+      // as unsigned ones); within the bounds the difference fits 32 bits
+      constexpr uint32_t kFlip = 0x80000000u;
+      out.e[i] = __usad(static_cast<uint32_t>(pred[i]) ^ kFlip,
+                        static_cast<uint32_t>(scaled) ^ kFlip, 3u) >> 3;
+    }
     st.e2[i] = st.e1[i];
-    st.e1[i] = ei;
+    st.e1[i] = active ? out.e[i] : 0u;
   }
-  st.q1 = value_q;
-  st.err1 = err;
+  st.q1 = out.q;
+  st.err1 = active ? out.err : 0;
   if (!kWide) {
-    overflow |= static_cast<uint32_t>(value_q) + kDcNarrowQ >= 2u * kDcNarrowQ ||
-                static_cast<uint32_t>(err) + kDcNarrowErr >= 2u * kDcNarrowErr;
+    overflow |= active && (static_cast<uint32_t>(out.q) + kDcNarrowQ >= 2u * kDcNarrowQ ||
+                           static_cast<uint32_t>(out.err) + kDcNarrowErr >= 2u * kDcNarrowErr);
   }
-  return value_q;
+  return out;
 }
+
+// Lane l receives lane l - 1's v, lane 0 receives edge.
+__device__ __forceinline__ int32_t FromLaneAbove(int32_t edge, int32_t v) {
+  constexpr int kWaveShr1 = 0x138;
+  return __builtin_amdgcn_update_dpp(edge, v, kWaveShr1, 0xf, 0xf, false);
+}
+
+__device__ __forceinline__ DcCell CellFromLaneAbove(const int32_t (&edge)[8], const DcCell& c) {
+  DcCell r;
+  r.q = FromLaneAbove(edge[0], c.q);
+  r.err = FromLaneAbove(edge[1], c.err);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r.e[i] = static_cast<uint32_t>(FromLaneAbove(edge[2 + i], static_cast<int32_t>(c.e[i])));
+  }
+  r.q_up = FromLaneAbove(edge[6], c.q_up);
+  return r;
+}
+
+struct DcShared {
+  uint32_t reciprocal[kReciprocals];
+  // [channel slot][writing wave][x % kDcEdgeRing]: lane 63's cells; slot
+  // kDcEdgeRing of every ring holds zeros (row 0's north)
+  int32_t edge[2][2][kDcEdgeRing + 1][8];
+  // quantized Y per row of the group, x % kDcYRing (B's work groups)
+  int32_t y_ring[kDcGroup][kDcYRing];
+};
 
 // grid: DC groups x 2.  Work group z = 0 quantises Y and X, z = 1 Y again
 // (without storing it) and B, which subtracts the sample's quantized Y
-// (times the Y step) from its DC first; Y produces that sample one step
-// earlier, so B runs one step behind (the reference quantizes B after Y's
-// whole plane).  Each channel has its own ring (2 x 48 KB of LDS, so two
-// such work groups never share a CU) and two waves: thread r owns rows r
-// and r + 128, whose samples fall in disjoint step ranges (row y is active
-// at steps 2y .. 2y + w - 1 and w <= 256), so each of the four waves of a
-// work group is busy on most steps and each SIMD runs one of them.  The steps
-// run in batches of kDcBatch; each thread reads its DC samples a batch ahead,
-// and the barrier between steps orders LDS only, so those reads stay in
-// flight.  A group is evaluated in 32-bit arithmetic first and again in
-// 64-bit arithmetic if any of its samples left the 32-bit bounds.
-constexpr int kDcBatch = 8;
-constexpr int kDcRowThreads = kDcGroup / 2;
-constexpr int kDcWorkGroup = 2 * kDcRowThreads;
-
+// (times the Y step) from its DC first and so runs D steps behind Y (the
+// reference quantizes B after Y's whole plane).  Each thread reads its DC
+// samples a batch of D steps ahead.  A group is evaluated in 32-bit
+// arithmetic first and again in 64-bit arithmetic if any of its samples
+// left the 32-bit bounds.
 struct DcWavefront {
   const float* src;      // this channel's DC plane
   int32_t* dst;          // this channel's quantized plane
-  size_t row_a, row_b;   // offsets of rows r and r + kDcRowThreads
-  int r, w, h, lag, slot, channel;
-  bool has_a, has_b, store;
+  uint32_t row_a, row_b;  // offsets of rows ya and yb
+  int ya, yb, off_a, off_b, w, h, slot, channel, lane, wave;
+  int lane0_off_a, lane0_off_b;   // off() of lane 0's two rows
+  bool store, write_y;
   float inv, step_y;
 
-  // the row and column of this thread's sample at step t; row < 0 if none
-  __device__ __forceinline__ void Locate(int t, int& row, int& x) const {
-    const int xa = t - lag - 2 * r;
-    const int xb = xa - 2 * kDcRowThreads;
-    if (has_a && xa >= 0 && xa < w) {
-      row = r;
-      x = xa;
-    } else if (has_b && xb >= 0 && xb < w) {
-      row = r + kDcRowThreads;
-      x = xb;
-    } else {
-      row = -1;
-      x = 0;
-    }
+  // the row this thread works on at step t (the second from two steps
+  // before its first sample) and its column there
+  __device__ __forceinline__ void Locate(int t, int& y, int& x) const {
+    const bool second = t >= off_b - 2;
+    y = second ? yb : ya;
+    x = t - (second ? off_b : off_a);
+  }
+  __device__ __forceinline__ bool Active(int y, int x) const {
+    return y < h && x >= 0 && x < w;
   }
 
   __device__ __forceinline__ void Fetch(float (&out)[kDcBatch], int t0) const {
 #pragma unroll
     for (int j = 0; j < kDcBatch; ++j) {
-      int row, x;
-      Locate(t0 + j, row, x);
-      out[j] = row < 0 ? 0.0f : src[(row == r ? row_a : row_b) + x];
+      int y, x;
+      Locate(t0 + j, y, x);
+      out[j] = Active(y, x) ? src[(y == ya ? row_a : row_b) + x] : 0.0f;
     }
   }
 
+  // The edge ring slot lane 0 reads at step t: the north-east column of
+  // its sample, x + 1, or the zero slot on row 0 (wave-uniform).
+  __device__ __forceinline__ int EdgeSlot(int t) const {
+    const bool second = t >= lane0_off_b - 2;
+    const int x = t - (second ? lane0_off_b : lane0_off_a);
+    const int slot = wave == 0 && !second ? kDcEdgeRing : (x + 1) & (kDcEdgeRing - 1);
+    return __builtin_amdgcn_readfirstlane(slot);
+  }
+
   // Walks the group's wavefront; returns whether a 32-bit evaluation left
-  // its bounds on this thread.
-  template <bool kWide>
-  __device__ __forceinline__ bool Run(DcRing (&ring)[2], const uint32_t* reciprocal) const {
+  // its bounds on this thread.  kB: this thread quantises B.  A batch of D
+  // steps is computed without branches (the scheduler interleaves the
+  // steps' independent work with the dependent chain); its stores and ring
+  // writes follow it, before the barrier, since every reader of those rings
+  // takes data at least D steps old.
+  template <bool kWide, bool kB>
+  __device__ __forceinline__ bool Run(DcShared& sh, int steps) const {
     float cur[kDcBatch], next[kDcBatch];
     Fetch(cur, 0);
     Fetch(next, kDcBatch);
     DcRowState st;
-    DcRing& own = ring[slot];
+    DcCell out, n, nw;  // this lane's last sample; north-east cells 1 and 2 steps ago
+    int32_t edge[8];
+    const int32_t(*ring_in)[8] = sh.edge[slot][wave ^ 1];
+    int32_t(*ring_out)[8] = sh.edge[slot][wave];
+    {
+      const int32_t* e = ring_in[EdgeSlot(0)];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) edge[k] = e[k];
+    }
     bool overflow = false;
-    const int steps = w + 2 * h + 1;
     for (int t0 = 0; t0 < steps; t0 += kDcBatch) {
+      DcCell cells[kDcBatch];
+#pragma unroll
+      for (int j = 0; j < kDcBatch; ++j) {
+        const int t = t0 + j;
+        const DcCell ne = CellFromLaneAbove(edge, out);
+        {
+          const int32_t* e = ring_in[EdgeSlot(t + 1)];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) edge[k] = e[k];
+        }
+        int y, x;
+        Locate(t, y, x);
+        const bool active = Active(y, x);
+        float value = cur[j];
+        if constexpr (kB) {
+          // (every row and slot index is inside the ring, also off the row)
+          const int32_t qy = sh.y_ring[y][x & (kDcYRing - 1)];
+          value = __builtin_fmaf(-static_cast<float>(qy), step_y, value);
+        }
+        out = DcSample<kWide>(ne, n, nw, st, sh.reciprocal, x, y, x + 1 < w, active, value,
+                              inv, overflow);
+        cells[j] = out;
+        nw = n;
+        n = ne;
+      }
 #pragma unroll
       for (int j = 0; j < kDcBatch; ++j) {
         int y, x;
         Locate(t0 + j, y, x);
-        if (y >= 0) {
-          float value = cur[j];
-          if (channel == 2) {
-            // Y's sample (x, y), written one step ago; its ring slot is
-            // reused only eight samples later
-            const int32_t qy = ring[0].q[x & (kRing - 1)][y];
-            value = __builtin_fmaf(-static_cast<float>(qy), step_y, value);
-          }
-          // the second row starts at x = 0, which reads no row state
-          const int32_t v =
-            DcSample<kWide>(own, st, reciprocal, x, y, x + 1 < w, value, inv, overflow);
-          if (store) dst[(y == r ? row_a : row_b) + x] = v;
+        const bool active = Active(y, x);
+        // Y's values for B: off-row steps write their slots too, which the
+        // ring's size keeps away from every slot B still reads
+        if (write_y) sh.y_ring[y][x & (kDcYRing - 1)] = cells[j].q;
+        if (active && store) dst[(y == ya ? row_a : row_b) + x] = cells[j].q;
+        if (active && lane == 63) {
+          int32_t* e = ring_out[x & (kDcEdgeRing - 1)];
+          e[0] = cells[j].q;
+          e[1] = cells[j].err;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) e[2 + i] = static_cast<int32_t>(cells[j].e[i]);
+          e[6] = cells[j].q_up;
         }
-        // the rings are the only state shared between rows
-        LdsBarrier();
       }
+      // orders the edge and Y rings between waves
+      LdsBarrier();
 #pragma unroll
       for (int j = 0; j < kDcBatch; ++j) cur[j] = next[j];
       Fetch(next, t0 + 2 * kDcBatch);
@@ -1279,13 +1364,19 @@ struct DcWavefront {
   }
 };
 
+__device__ __forceinline__ int DcOffset(int y, int lag) {
+  return 2 * y + kDcBatch * (y >> 6) + lag;
+}
+
 __global__ __launch_bounds__(kDcWorkGroup) void DcQuantKernel(
   const float* __restrict__ dc, int bw, int bh,
   const QuantizerState* __restrict__ qstate, int extra_precision,
   int32_t* __restrict__ quantized) {
-  __shared__ DcRing ring[2];  // [0]: Y, [1]: X or B
-  __shared__ uint32_t reciprocal[kReciprocals];
-  if (threadIdx.x < kReciprocals) reciprocal[threadIdx.x] = (1u << 24) / (threadIdx.x + 1);
+  __shared__ DcShared sh;
+  if (threadIdx.x < kReciprocals) sh.reciprocal[threadIdx.x] = (1u << 24) / (threadIdx.x + 1);
+  if (threadIdx.x < 2 * 2 * 8) {
+    sh.edge[threadIdx.x / 16][(threadIdx.x / 8) % 2][kDcEdgeRing][threadIdx.x % 8] = 0;
+  }
   __syncthreads();
   const int gx0 = blockIdx.x * kDcGroup, gy0 = blockIdx.y * kDcGroup;
   const size_t n = static_cast<size_t>(bw) * bh;
@@ -1293,28 +1384,47 @@ __global__ __launch_bounds__(kDcWorkGroup) void DcQuantKernel(
   DcWavefront f;
   f.w = min(kDcGroup, bw - gx0);
   f.h = min(kDcGroup, bh - gy0);
-  // waves 0-1: Y, waves 2-3: X (z = 0) or B (z = 1)
-  f.slot = threadIdx.x / kDcRowThreads;
+  // waves 0-1: Y, waves 2-3: X (z = 0) or B (z = 1); the wave's index
+  // (from the first lane) keeps every per-wave quantity scalar
+  const int wave_id = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x)) / 64;
+  f.slot = wave_id / 2;
   f.channel = f.slot == 0 ? 1 : with_b ? 2 : 0;
-  f.lag = f.channel == 2 ? 1 : 0;
+  const int lag = f.channel == 2 ? kDcBatch : 0;
   f.store = f.slot == 1 || !with_b;
+  f.write_y = f.slot == 0 && with_b;
   const QuantizerState q = *qstate;
   const float precision = static_cast<float>(1u << extra_precision);
   const float inverse_step = f.channel == 0 ? q.inverse_dc_steps[0]
                            : f.channel == 1 ? q.inverse_dc_steps[1] : q.inverse_dc_steps[2];
   f.inv = inverse_step * precision;
   f.step_y = q.dc_steps[1] / precision;
-  f.r = threadIdx.x % kDcRowThreads;
-  // rows r (first) and r + kDcRowThreads (second), clamped into the group
-  f.row_a = static_cast<size_t>(gy0 + min(f.r, f.h - 1)) * bw + gx0;
-  f.row_b = static_cast<size_t>(gy0 + min(f.r + kDcRowThreads, f.h - 1)) * bw + gx0;
-  f.has_a = f.r < f.h;
-  f.has_b = f.r + kDcRowThreads < f.h;
+  const int r = threadIdx.x % kDcRowThreads;
+  f.wave = wave_id % 2;
+  f.lane = r % 64;
+  f.ya = r;
+  f.yb = r + kDcRowThreads;
+  f.off_a = DcOffset(f.ya, lag);
+  f.off_b = DcOffset(f.yb, lag);
+  f.lane0_off_a = DcOffset(64 * f.wave, lag);
+  f.lane0_off_b = DcOffset(64 * f.wave + kDcRowThreads, lag);
+  // rows ya and yb, clamped into the group
+  // (n < 2^32: the extent is at most 8192 x 8192 blocks)
+  f.row_a = static_cast<uint32_t>(gy0 + min(f.ya, f.h - 1)) * bw + gx0;
+  f.row_b = static_cast<uint32_t>(gy0 + min(f.yb, f.h - 1)) * bw + gx0;
   f.src = dc + f.channel * n;
   f.dst = quantized + f.channel * n;
-  const bool overflow = f.Run<false>(ring, reciprocal);
+  // the last sample of the group, (w - 1, h - 1), of the later channel
+  const int steps = DcOffset(f.h - 1, with_b ? kDcBatch : 0) + f.w;
+  const bool overflow =
+    f.channel == 2 ? f.Run<false, true>(sh, steps) : f.Run<false, false>(sh, steps);
   // the rerun rewrites every sample the first walk stored
-  if (__syncthreads_or(overflow)) f.Run<true>(ring, reciprocal);
+  if (__syncthreads_or(overflow)) {
+    if (f.channel == 2) {
+      f.Run<true, true>(sh, steps);
+    } else {
+      f.Run<true, false>(sh, steps);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void DcReconstructKernel(
