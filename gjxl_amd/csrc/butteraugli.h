@@ -58,9 +58,10 @@ namespace gjxl_amd {
 
 // Enqueues a comparison of `distorted` (3 planes, requested extent, device)
 // against the prepared reference: the map goes to b->map and `map_out` when
-// it is not null, the score to b->score_bits.
+// it is not null, the score to b->score_bits.  keep_stages also leaves the
+// intermediate difference maps (the STAGES buffer) for inspection.
 GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
-                              float* map_out);
+                              float* map_out, bool keep_stages = false);
 
 // Extent fields for (width, height); returns the floats the allocation
 // needs at that extent (it grows with the width and with the height).

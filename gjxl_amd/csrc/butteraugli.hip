@@ -565,6 +565,15 @@ __device__ __forceinline__ float MaltaScaledDiff(float v0, float v1, const Malta
   return near || far ? corrected : s;
 }
 
+// Keeps one Malta window's responses computed before the next window's LDS
+// reads: an empty asm statement that takes the four values and orders
+// memory, so that the compiler neither sinks their arithmetic below it nor
+// hoists the next reads above it (with all windows live at once the kernel
+// would need over 400 registers).
+__device__ __forceinline__ void Settle(const float (&v)[4]) {
+  asm volatile("" : : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]) : "memory");
+}
+
 __device__ __forceinline__ float Sum4(float a, float b, float c, float d) {
   return (a + b) + (c + d);
 }
@@ -668,13 +677,20 @@ struct MaltaScales {
 //   reads of one stage issued before the previous stage's differences; each
 //   thread then evaluates four adjacent outputs of a row, for two rows, from
 //   9 x 12 register windows of 16-byte LDS reads, and stores the stage;
-// * after the last three stages, from their values in registers and the
-//   first three read back (the same thread stored them), the Malta
-//   accumulation in the reference's order (UHF, HF, MF), the asymmetric HF
-//   and symmetric MF L2 terms, the DC differences from LF, the Y
-//   blurred-activity term, masking and the final map (MaskPsychoImage /
-//   CombineChannelsToDiffmap): stages 7 and 8.
+// * the Malta accumulation in the reference's order (UHF, HF, MF per
+//   channel): the first three stages are UHF Y, HF Y and UHF X, and their
+//   partial sums ((0 + UHF Y) + HF Y, 0 + UHF X) go to planes 0 and 1 of
+//   `stages`, which the same thread reads back after the last three (HF X,
+//   MF X, MF Y);
+// * then the asymmetric HF and symmetric MF L2 terms, the DC differences
+//   from LF, the Y blurred-activity term, masking and the final map
+//   (MaskPsychoImage / CombineChannelsToDiffmap): stage 8.
+// kKeepStages stores the six Malta maps (stages 0-5) and the masked Y term
+// (stage 7) for inspection instead, and forms the partial sums from the
+// first three maps read back; without it only the partial sums and the
+// final map are written.
 // Three stages of LDS (34.6 KB) let four work groups share a CU.
+template <bool kKeepStages>
 __global__ __launch_bounds__(kThreads) void DifferencesKernel(
   const float* __restrict__ ps0, const float* __restrict__ ps1, int w, int h,
   MaltaScales ms, L2Weights lw, const float* __restrict__ ref_blurred,
@@ -739,9 +755,9 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       v1[j] = *reinterpret_cast<const float*>(p1 + f.offset[j]);
     }
   };
-  auto store = [&](int k, const FormIndex& f, const float (&v0)[kFormSlots],
+  auto store = [&](int k, int slot, const FormIndex& f, const float (&v0)[kFormSlots],
                    const float (&v1)[kFormSlots]) {
-    float* stage = &tile[k % 3][0][0];
+    float* stage = &tile[slot][0][0];
     const int lane = tid % 64;
 #pragma unroll
     for (int j = 0; j < kFormRows; ++j) {
@@ -757,17 +773,17 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       }
     }
   };
-  // stages first .. first + 2 into LDS
-  auto form = [&](int first) {
+  // stages k0, k1, k2 into LDS slots 0, 1, 2
+  auto form = [&](int k0, int k1, int k2) {
     FormIndex f;
     index(f);
     float a0[kFormSlots], a1[kFormSlots], b0[kFormSlots], b1[kFormSlots];
-    load(first, f, a0, a1);
-    load(first + 1, f, b0, b1);
-    store(first, f, a0, a1);
-    load(first + 2, f, a0, a1);
-    store(first + 1, f, b0, b1);
-    store(first + 2, f, a0, a1);
+    load(k0, f, a0, a1);
+    load(k1, f, b0, b1);
+    store(k0, 0, f, a0, a1);
+    load(k2, f, a0, a1);
+    store(k1, 1, f, b0, b1);
+    store(k2, 2, f, a0, a1);
   };
   const int q = tid % kMaltaQuads;
   const int x = x0 + 4 * q;
@@ -803,15 +819,15 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       }
     }
   };
-  // Malta of stage k (in LDS slot k % 3) for the four outputs of row ty
-  auto malta = [&](auto stage, int ty, float (&out)[4]) {
+  // Malta of stage k (in LDS slot `slot`) for the four outputs of row ty
+  auto malta = [&](auto stage, int slot, int ty, float (&out)[4]) {
     constexpr int k = decltype(stage)::value;
-    // win[r][c] = tile[k % 3][ty + r][4q + c]: rows y - 4 .. y + 4, columns
+    // win[r][c] = tile[slot][ty + r][4q + c]: rows y - 4 .. y + 4, columns
     // x - 4 .. x + 7
     float win[kMaltaWinH][kMaltaWinW];
 #pragma unroll
     for (int r = 0; r < kMaltaWinH; ++r) {
-      const float4* s4 = reinterpret_cast<const float4*>(&tile[k % 3][ty + r][4 * q]);
+      const float4* s4 = reinterpret_cast<const float4*>(&tile[slot][ty + r][4 * q]);
 #pragma unroll
       for (int c4 = 0; c4 < kMaltaWinW / 4; ++c4) {
         const float4 v = s4[c4];
@@ -829,7 +845,7 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       out[j] = MaltaResponse<(k < 4)>(P);
     }
   };
-  form(0);
+  form(4, 2, 5);
   __syncthreads();
   if (active) {
 #pragma unroll 1
@@ -839,16 +855,34 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       if (y >= h) break;
       const size_t base = static_cast<size_t>(y) * w + x;
       float sv[3][4];
-      malta(std::integral_constant<int, 0>{}, ty, sv[0]);
-      store4(stages, base, sv[0]);
-      malta(std::integral_constant<int, 1>{}, ty, sv[1]);
-      store4(stages + n, base, sv[1]);
-      malta(std::integral_constant<int, 2>{}, ty, sv[2]);
-      store4(stages + 2 * n, base, sv[2]);
+      // one Malta window live at a time: the scheduling barriers stand where
+      // the stores of the inspected stages keep the windows apart
+      malta(std::integral_constant<int, 4>{}, 0, ty, sv[0]);
+      if constexpr (kKeepStages) store4(stages + 4 * n, base, sv[0]);
+      Settle(sv[0]);
+      malta(std::integral_constant<int, 2>{}, 1, ty, sv[1]);
+      if constexpr (kKeepStages) store4(stages + 2 * n, base, sv[1]);
+      Settle(sv[1]);
+      malta(std::integral_constant<int, 5>{}, 2, ty, sv[2]);
+      if constexpr (kKeepStages) {
+        store4(stages + 5 * n, base, sv[2]);
+      } else {
+        float acc_y[4], acc_x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc_y[j] = 0.0f;
+          acc_y[j] += sv[0][j];
+          acc_y[j] += sv[1][j];
+          acc_x[j] = 0.0f;
+          acc_x[j] += sv[2][j];
+        }
+        store4(stages, base, acc_y);
+        store4(stages + n, base, acc_x);
+      }
     }
   }
   __syncthreads();
-  form(3);
+  form(3, 1, 0);
   __syncthreads();
   if (!active) return;
 #pragma unroll 1
@@ -858,15 +892,34 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
     if (y >= h) return;
     const size_t base = static_cast<size_t>(y) * w + x;
     float sv[6][4];
-    malta(std::integral_constant<int, 3>{}, ty, sv[3]);
-    store4(stages + 3 * n, base, sv[3]);
-    malta(std::integral_constant<int, 4>{}, ty, sv[4]);
-    store4(stages + 4 * n, base, sv[4]);
-    malta(std::integral_constant<int, 5>{}, ty, sv[5]);
-    store4(stages + 5 * n, base, sv[5]);
-    // stages 0-2 as this thread stored them
+    malta(std::integral_constant<int, 3>{}, 0, ty, sv[3]);
+    if constexpr (kKeepStages) store4(stages + 3 * n, base, sv[3]);
+    Settle(sv[3]);
+    malta(std::integral_constant<int, 1>{}, 1, ty, sv[1]);
+    if constexpr (kKeepStages) store4(stages + 1 * n, base, sv[1]);
+    Settle(sv[1]);
+    malta(std::integral_constant<int, 0>{}, 2, ty, sv[0]);
+    if constexpr (kKeepStages) store4(stages, base, sv[0]);
+    Settle(sv[0]);
+    // the first half's partial sums as this thread stored them (or its
+    // stages 4, 2, 5)
+    float acc_y[4], acc_x[4];
+    if constexpr (kKeepStages) {
+      load4(stages + 4 * n, base, sv[4]);
+      load4(stages + 2 * n, base, sv[2]);
+      load4(stages + 5 * n, base, sv[5]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) load4(stages + k * n, base, sv[k]);
+      for (int j = 0; j < 4; ++j) {
+        acc_y[j] = 0.0f;
+        acc_y[j] += sv[4][j];
+        acc_y[j] += sv[2][j];
+        acc_x[j] = 0.0f;
+        acc_x[j] += sv[5][j];
+      }
+    } else {
+      load4(stages, base, acc_y);
+      load4(stages + n, base, acc_x);
+    }
     // the per-pixel inputs of the L2 terms and the final map
     float p0[8][4], p1[8][4];  // planes LF 0-2, MF 3-5, HF 6-7
 #pragma unroll
@@ -883,12 +936,10 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       float a[3], dcv[3];
-      a[1] = 0.0f;
-      a[1] += sv[4][j];
-      a[1] += sv[2][j];
+      // Y: (UHF + HF) + MF; X: (UHF + HF) + MF, from 0 as the reference sums
+      a[1] = acc_y[j];
       a[1] += sv[0][j];
-      a[0] = 0.0f;
-      a[0] += sv[5][j];
+      a[0] = acc_x[j];
       a[0] += sv[3][j];
       a[0] += sv[1][j];
       a[2] = 0.0f;
@@ -925,7 +976,7 @@ __global__ __launch_bounds__(kThreads) void DifferencesKernel(
       const float masked_ac = ac0 * my[j] + ay * my[j] + a[2] * my[j];
       final_map[j] = sqrtf(masked_dc + masked_ac);
     }
-    store4(stages + 7 * n, base, masked_y);
+    if constexpr (kKeepStages) store4(stages + 7 * n, base, masked_y);
     store4(stages + 8 * n, base, final_map);
   }
 }
@@ -1241,7 +1292,7 @@ constexpr float kL2[9] = {400.0f, 1.50815703118f, 0.0f, 2150.0f, 10.6195433239f,
                           16.2176043152f, 29.2353797994f, 0.844626970982f,
                           0.703646627719f};
 
-GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
+GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s, bool keep_stages) {
   hipStream_t st = b->device->stream;
   const int w = s.width, h = s.height;
   const size_t n = s.n;
@@ -1282,9 +1333,15 @@ GJXLResult Differences(GJXLAmdButteraugli* b, ButteraugliScale& s) {
   const dim3 tiles(CeilDiv(static_cast<uint32_t>(w), kMaltaTileW),
                    CeilDiv(static_cast<uint32_t>(h), kMaltaTileH));
   const bool probed = ProbeBegin(b->device, GJXL_AMD_KERNEL_BUTTERAUGLI_DIFFERENCES);
-  hipLaunchKernelGGL(DifferencesKernel, tiles, dim3(kThreads), 0, st, s.ref_psycho,
-                     s.dist_psycho, w, h, ms, lw, s.ref_blurred, s.tmp[2], s.mask_y,
-                     s.mask_dc, b->params.x_multiplier, s.stages);
+  if (keep_stages) {
+    hipLaunchKernelGGL(DifferencesKernel<true>, tiles, dim3(kThreads), 0, st, s.ref_psycho,
+                       s.dist_psycho, w, h, ms, lw, s.ref_blurred, s.tmp[2], s.mask_y,
+                       s.mask_dc, b->params.x_multiplier, s.stages);
+  } else {
+    hipLaunchKernelGGL(DifferencesKernel<false>, tiles, dim3(kThreads), 0, st, s.ref_psycho,
+                       s.dist_psycho, w, h, ms, lw, s.ref_blurred, s.tmp[2], s.mask_y,
+                       s.mask_dc, b->params.x_multiplier, s.stages);
+  }
   ProbeEnd(b->device, probed);
   return CheckLaunch("DifferencesKernel");
 }
@@ -1294,7 +1351,7 @@ constexpr int kScalePlanes = 3 + 3 + 10 + 10 + 3 + 9 + 3;
 }  // namespace
 
 GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
-                              float* map_out) {
+                              float* map_out, bool keep_stages) {
   hipStream_t st = b->device->stream;
   ButteraugliScale& m = b->scale[0];
   const float* main_in = distorted;
@@ -1306,7 +1363,7 @@ GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
     main_in = m.input;
   }
   GJXL_AMD_TRY(Psycho(b, m, main_in, m.dist_psycho));
-  GJXL_AMD_TRY(Differences(b, m));
+  GJXL_AMD_TRY(Differences(b, m, keep_stages));
   const float* sub_map = nullptr;
   int sw = 0;
   if (b->has_subscale) {
@@ -1315,7 +1372,7 @@ GJXLResult ButteraugliCompare(GJXLAmdButteraugli* b, const float* distorted,
                        distorted, b->width, b->height, s.input);
     GJXL_AMD_TRY(CheckLaunch("SubsampleKernel"));
     GJXL_AMD_TRY(Psycho(b, s, s.input, s.dist_psycho));
-    GJXL_AMD_TRY(Differences(b, s));
+    GJXL_AMD_TRY(Differences(b, s, keep_stages));
     sub_map = s.stages + 8 * s.n;
     sw = s.width;
   }
@@ -1499,7 +1556,7 @@ GJXLResult gjxl_amd_butteraugli_compare(GJXLAmdButteraugli* b,
   if (!b->prepared) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "Butteraugli reference has not been prepared");
   }
-  return gjxl_amd::ButteraugliCompare(b, distorted_linear, distance_map);
+  return gjxl_amd::ButteraugliCompare(b, distorted_linear, distance_map, true);
 }
 
 GJXLResult gjxl_amd_butteraugli_score(GJXLAmdButteraugli* b, double* score) {

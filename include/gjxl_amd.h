@@ -337,7 +337,9 @@ GJXL_AMD_API void gjxl_amd_butteraugli_destroy(GJXLAmdButteraugli* b);
 GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_prepare(
   GJXLAmdButteraugli* b, const float* reference_linear);
 /* Enqueues one comparison; distance_map (device, width*height) may be null
- * (the map stays in the object's own buffer). */
+ * (the map stays in the object's own buffer).  Leaves every difference
+ * stage in the STAGES buffer; the AQ loop's comparisons (gjxl_amd_aq_run,
+ * the plain path) write only its final map (stage 8) and stage 6. */
 GJXL_AMD_API GJXLResult gjxl_amd_butteraugli_compare(
   GJXLAmdButteraugli* b, const float* distorted_linear, float* distance_map);
 /* Waits for the stream and returns the last comparison's score (the map's
