@@ -13,7 +13,9 @@
 // GPU, followed by the AcTokenizationProvider hand-off
 // (src/codestream/ac_tokenization_provider_internal.h:18-47), as PrepareWorkflow
 // (src/codestream/workflow.cpp:636-768) runs it for gjxl_encode
-// (src/c_api/gjxl.cpp:470-637) at distance d, effort 7.  The host waits three times:
+// (src/c_api/gjxl.cpp:470-637) at distance d, effort 7.  The Butteraugli
+// reference side runs on a second stream from the linear image on, beside
+// the stages up to the AQ loop.  The host waits three times:
 // once for the quantization-matrix statistics that choose the matrix scales
 // (src/codestream/workflow.cpp:1318-1484), once for the frame statistics and
 // once for the token count.
@@ -63,6 +65,12 @@ struct GJXLAmdPlainPath {
   GJXLAmdAqSession* session = nullptr;
   GJXLAmdButteraugli* butteraugli = nullptr;
   GJXLAmdTokenizer* tokenizer = nullptr;
+  // a second handle (its own stream) for the Butteraugli reference side,
+  // which needs only the linear image: it overlaps the stages before the
+  // AQ loop; linear_ready and reference_ready order the two streams
+  GJXLAmdDevice* side = nullptr;
+  hipEvent_t linear_ready = nullptr;
+  hipEvent_t reference_ready = nullptr;
   // pinned host words: token status (4), AQ error (1), scores (kMaxEvaluations)
   uint32_t* host_words = nullptr;
   // pinned host copy of the frame statistics; the entropy inputs chosen
@@ -188,6 +196,15 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
                                                   p->height, p->options.intensity_target,
                                                   p->linear, p->opsin));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_SRGB_TO_LINEAR_AND_OPSIN));
+  // the Butteraugli reference side on the side stream, from the linear image
+  GJXL_AMD_TRY(Check(hipEventRecord(p->linear_ready, d->stream), "hipEventRecord"));
+  GJXL_AMD_TRY(Check(hipStreamWaitEvent(p->side->stream, p->linear_ready, 0),
+                     "hipStreamWaitEvent"));
+  p->butteraugli->device = p->side;
+  const GJXLResult prepared = gjxl_amd_butteraugli_prepare(p->butteraugli, p->linear);
+  p->butteraugli->device = d;
+  GJXL_AMD_TRY(prepared);
+  GJXL_AMD_TRY(Check(hipEventRecord(p->reference_ready, p->side->stream), "hipEventRecord"));
   const float ones[3] = {1.0f, 1.0f, 1.0f};
   GJXL_AMD_TRY(gjxl_amd_gaborish_inverse(d, p->opsin, p->pw, p->ph, ones, p->preprocessed));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_GABORISH_INVERSE));
@@ -196,6 +213,12 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_QUANT_FIELD));
   GJXL_AMD_TRY(gjxl_amd_initial_cfl(d, p->preprocessed, p->pw, p->ph, p->y_to_x, p->y_to_b));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_CFL));
+  // what is left of the reference side once the device stream gets here:
+  // the side stream shares the GPU with the latency-bound stages before the
+  // candidate costs, which the costs themselves would not leave room for
+  GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
+                     "hipStreamWaitEvent"));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
   GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs(d, p->preprocessed, p->pw, p->ph, p->quant_field,
                                           p->pixel_mask, p->y_to_x, p->y_to_b, t, p->costs));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_COSTS));
@@ -204,8 +227,6 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(gjxl_amd_aq_prepare(p->session, p->opsin, p->preprocessed, p->cells,
                                    p->quant_field, t));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_PREPARE));
-  GJXL_AMD_TRY(gjxl_amd_butteraugli_prepare(p->butteraugli, p->linear));
-  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
   // RunAdaptiveQuantizationPolicyImpl (src/codec/adaptive_quantization.cpp:
   // 1600-1738): evaluate, compare, reduce, update
   hipStream_t st = d->stream;
@@ -400,6 +421,15 @@ GJXLResult gjxl_amd_plain_path_create(GJXLAmdDevice* device, uint32_t width, uin
   if (r == GJXL_OK) {
     r = gjxl_amd_butteraugli_create(device, width, height, 1.0f, 1.0f, 80.0f, &p->butteraugli);
   }
+  if (r == GJXL_OK) r = gjxl_amd_device_create(device->ordinal, &p->side);
+  if (r == GJXL_OK) {
+    r = Check(hipEventCreateWithFlags(&p->linear_ready, hipEventDisableTiming),
+              "hipEventCreateWithFlags");
+  }
+  if (r == GJXL_OK) {
+    r = Check(hipEventCreateWithFlags(&p->reference_ready, hipEventDisableTiming),
+              "hipEventCreateWithFlags");
+  }
   if (r == GJXL_OK) r = gjxl_amd_tokenizer_create(device, width, height, &p->tokenizer);
   if (r != GJXL_OK) {
     gjxl_amd_plain_path_destroy(p);
@@ -415,12 +445,16 @@ void gjxl_amd_plain_path_destroy(GJXLAmdPlainPath* p) {
     (void)hipSetDevice(p->device->ordinal);
     (void)hipStreamSynchronize(p->device->stream);
   }
+  if (p->side != nullptr) (void)hipStreamSynchronize(p->side->stream);
   gjxl_amd_tokenizer_destroy(p->tokenizer);
   gjxl_amd_butteraugli_destroy(p->butteraugli);
   gjxl_amd_aq_session_destroy(p->session);
   for (hipEvent_t e : p->marks) {
     if (e != nullptr) (void)hipEventDestroy(e);
   }
+  if (p->linear_ready != nullptr) (void)hipEventDestroy(p->linear_ready);
+  if (p->reference_ready != nullptr) (void)hipEventDestroy(p->reference_ready);
+  gjxl_amd_device_destroy(p->side);
   if (p->host_words != nullptr) (void)hipHostFree(p->host_words);
   if (p->host_statistics != nullptr) (void)hipHostFree(p->host_statistics);
   delete p->statistics;
