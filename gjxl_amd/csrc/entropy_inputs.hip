@@ -53,21 +53,62 @@ constexpr uint8_t kDefaultMap[3 * kFamilies] = {
   2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 3,
 };
 
+// The natural orders of the searched families, built once per process.
+struct NaturalOrders {
+  uint32_t order[5][1024];
+  GJXLResult status = GJXL_OK;
+  NaturalOrders() {
+    for (int i = 0; i < 5 && status == GJXL_OK; ++i) {
+      size_t n = 0;
+      status = gjxl_amd_natural_order(kSearched[i].strategy, order[i], 1024, &n);
+      if (status == GJXL_OK && n != static_cast<size_t>(kSearched[i].size)) {
+        status = Fail(GJXL_ERROR_INTERNAL, "natural order size disagrees with its family");
+      }
+    }
+  }
+};
+
+const NaturalOrders& Naturals() {
+  static const NaturalOrders orders;
+  return orders;
+}
+
+// Sorts items[0, n) by key[item], stably (least significant 11-bit digit
+// first), through tmp.  Keys are below 2^33.
+void StableSortByKey(uint32_t* items, size_t n, const uint64_t* key, uint32_t* tmp) {
+  uint64_t top = 0;
+  for (size_t i = 0; i < n; ++i) top = std::max(top, key[items[i]]);
+  uint32_t* from = items;
+  uint32_t* to = tmp;
+  for (int shift = 0; shift < 64 && (shift == 0 || (top >> shift) != 0); shift += 11) {
+    uint32_t start[2048] = {};
+    for (size_t i = 0; i < n; ++i) ++start[(key[from[i]] >> shift) & 2047];
+    uint32_t sum = 0;
+    for (uint32_t& b : start) {
+      const uint32_t c = b;
+      b = sum;
+      sum += c;
+    }
+    for (size_t i = 0; i < n; ++i) to[start[(key[from[i]] >> shift) & 2047]++] = from[i];
+    std::swap(from, to);
+  }
+  if (from != items) std::copy(from, from + n, items);
+}
+
 GJXLResult ComputeOrders(const GJXLAmdFrameStatistics& s, bool sampled_behaviour,
                          GJXLAmdEntropyInputs* out) {
   out->used_order_mask = 0;
   if (s.block_width < 5 && s.block_height < 5) return GJXL_OK;
+  const NaturalOrders& naturals = Naturals();
+  GJXL_AMD_TRY(naturals.status);
   const bool sample = sampled_behaviour && s.present_mask == 1;
-  std::vector<uint32_t> natural(1024), rank(1024);
-  std::vector<uint64_t> key(1024);
-  for (const SearchedFamily& f : kSearched) {
+  uint64_t key[1024];
+  uint32_t tmp[1024];
+  for (int fi = 0; fi < 5; ++fi) {
+    const SearchedFamily& f = kSearched[fi];
     if (!(s.present_mask & (1u << f.family))) continue;
-    size_t n = 0;
-    GJXL_AMD_TRY(gjxl_amd_natural_order(f.strategy, natural.data(), natural.size(), &n));
-    if (n != static_cast<size_t>(f.size)) {
-      return Fail(GJXL_ERROR_INTERNAL, "natural order size disagrees with its family");
-    }
-    for (size_t i = 0; i < n; ++i) rank[natural[i]] = static_cast<uint32_t>(i);
+    const size_t n = static_cast<size_t>(f.size);
+    const uint32_t* natural = naturals.order[fi];
     const float inverse_sqrt_size = 1.0f / std::sqrt(static_cast<float>(n));
     bool custom = false;
     for (int c = 0; c < 3; ++c) {
@@ -77,12 +118,12 @@ GJXLResult ComputeOrders(const GJXLAmdFrameStatistics& s, bool sampled_behaviour
         const float scaled = static_cast<float>(zeros[k]) * inverse_sqrt_size;
         key[k] = static_cast<uint64_t>(scaled + 0.1f);
       }
+      // ascending keys, equal keys in natural order: a stable sort of the
+      // natural sequence after the LLF prefix
       uint32_t* order = out->orders[f.family][c];
-      std::copy(natural.begin(), natural.begin() + n, order);
-      std::sort(order + f.llf, order + n, [&](uint32_t a, uint32_t b) {
-        return key[a] != key[b] ? key[a] < key[b] : rank[a] < rank[b];
-      });
-      custom |= !std::equal(order, order + n, natural.begin());
+      std::copy(natural, natural + n, order);
+      StableSortByKey(order + f.llf, n - f.llf, key, tmp);
+      custom |= !std::equal(order, order + n, natural);
     }
     if (custom) out->used_order_mask |= static_cast<uint16_t>(1u << f.family);
   }
@@ -216,12 +257,12 @@ GJXLResult gjxl_amd_entropy_inputs_from_statistics(const GJXLAmdFrameStatistics*
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "effort must be in [1, 10]");
   }
   GJXL_AMD_TRY(ValidateStatistics(*statistics));
-  std::vector<GJXLAmdEntropyInputs> staging(1);
-  GJXLAmdEntropyInputs& result = staging[0];
-  std::memset(&result, 0, sizeof(result));
-  GJXL_AMD_TRY(ComputeOrders(*statistics, effort == 7, &result));
-  ComputeMap(*statistics, &result);
-  std::memcpy(out, &result, sizeof(result));
+  // nothing below fails once the statistics and the natural orders are
+  // good, so out is written in place
+  GJXL_AMD_TRY(Naturals().status);
+  std::memset(out, 0, sizeof(*out));
+  GJXL_AMD_TRY(ComputeOrders(*statistics, effort == 7, out));
+  ComputeMap(*statistics, out);
   return GJXL_OK;
 }
 

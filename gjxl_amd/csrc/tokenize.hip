@@ -477,6 +477,21 @@ std::vector<uint8_t> SampleDecisions(const AqLayout& l) {
   return decisions;
 }
 
+// The natural order of every stage's strategy, built once per process
+// (empty on failure).
+const std::vector<uint32_t>& StageNaturalOrders() {
+  static const std::vector<uint32_t> table = [] {
+    std::vector<uint32_t> t(kOrderStride);
+    for (int s = 0; s < kStrategyStages; ++s) {
+      const int strategy = StageStrategy(s);
+      const size_t n = static_cast<size_t>(CoveredW(strategy)) * CoveredH(strategy) * 64;
+      if (NaturalOrder(strategy, t.data() + StageOffset(s)) != n) return std::vector<uint32_t>();
+    }
+    return t;
+  }();
+  return table;
+}
+
 bool ValidOrder(int strategy, const uint32_t* order) {
   const int cw = CoveredW(strategy), ch = CoveredH(strategy);
   const int n = cw * ch * 64;
@@ -774,6 +789,8 @@ GJXLResult gjxl_amd_tokenizer_set_orders(GJXLAmdTokenizer* t, uint16_t used_orde
   if (used_order_mask >> kOrderFamilies) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "coefficient-order mask names an absent family");
   }
+  const std::vector<uint32_t>& naturals = StageNaturalOrders();
+  if (naturals.empty()) return Fail(GJXL_ERROR_INTERNAL, "natural coefficient order is incomplete");
   std::vector<uint32_t> table(3 * kOrderStride);
   for (int s = 0; s < kStrategyStages; ++s) {
     const int strategy = StageStrategy(s);
@@ -788,8 +805,8 @@ GJXLResult gjxl_amd_tokenizer_set_orders(GJXLAmdTokenizer* t, uint16_t used_orde
                       "coefficient order is not a permutation with the LLF prefix");
         }
         std::memcpy(dst, src, 4 * n);
-      } else if (NaturalOrder(strategy, dst) != n) {
-        return Fail(GJXL_ERROR_INTERNAL, "natural coefficient order is incomplete");
+      } else {
+        std::memcpy(dst, naturals.data() + StageOffset(s), 4 * n);
       }
     }
   }
