@@ -317,9 +317,15 @@ __host__ __device__ constexpr int FullFamily(int compact) {
 // counters go to the frame's words once per wave.  The work group's threads
 // also count the chunk's blocks per order family and raw quant (LDS), and a
 // DCT8 transform whose sample decision is set counts into the sampled
-// counters too.
+// counters too.  The chunk's transforms are first listed in LDS by order
+// family (offset and sample decision, read by the first wave), so that each
+// wave walks them family by family with several transforms' loads in
+// flight.
 constexpr int kStatChunks = 16;
 constexpr int kStatThreads = 3 * 64;
+
+constexpr int kStatMaxChunk = (kGroupAnchors + kStatChunks - 1) / kStatChunks;
+static_assert(kStatMaxChunk <= 64, "one lane per transform of a chunk");
 
 __global__ __launch_bounds__(kStatThreads) void FrameStatisticsKernel(
   const int32_t* __restrict__ quantized, size_t store,
@@ -328,43 +334,85 @@ __global__ __launch_bounds__(kStatThreads) void FrameStatisticsKernel(
   const uint32_t* __restrict__ anchor_count, const int32_t* __restrict__ raw,
   const uint8_t* __restrict__ samples, uint32_t* __restrict__ stats) {
   __shared__ uint32_t raw_counts[kSearchedFamilies * 256];
+  __shared__ uint32_t list_offset[kStatMaxChunk];
+  __shared__ uint32_t list_sampled[kStatMaxChunk];
+  __shared__ int family_start[kSearchedFamilies + 1];
   const int g = blockIdx.x, chunk = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63;
   const int c = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int i = tid; i < kSearchedFamilies * 256; i += kStatThreads) raw_counts[i] = 0;
+  const int anchors = static_cast<int>(anchor_count[g]);
+  const int per = (anchors + kStatChunks - 1) / kStatChunks;
+  const int first = chunk * per, last = min(anchors, first + per);
+  if (c == 0) {
+    // lane k: transform first + k, bucketed by compact family (ballots in
+    // family order give each transform its slot)
+    const int k = first + lane;
+    const bool live = k < last;
+    int family = -1;
+    uint32_t offset = 0, sampled = 0;
+    if (live) {
+      const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
+      const size_t block = static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff);
+      family = CompactFamily(OrderFamily(cells[block] >> 1));
+      offset = offsets[block];
+      sampled = samples[block] != 0;
+    }
+    int base = 0;
+#pragma unroll
+    for (int f = 0; f < kSearchedFamilies; ++f) {
+      const uint64_t mask = __ballot(family == f);
+      if (family == f) {
+        const int slot = base + __popcll(mask & ((uint64_t{1} << lane) - 1));
+        list_offset[slot] = offset;
+        list_sampled[slot] = sampled;
+      }
+      if (lane == 0) family_start[f] = base;
+      base += __popcll(mask);
+    }
+    if (lane == 0) family_start[kSearchedFamilies] = base;
+  }
   __syncthreads();
   // zero counts of channel c over this chunk's transforms
   uint32_t z8 = 0, z16x8[2] = {}, z16[4] = {}, z32x16[8] = {}, z32[16] = {}, sampled = 0;
   uint32_t present = 0;
-  const int anchors = static_cast<int>(anchor_count[g]);
-  const int per = (anchors + kStatChunks - 1) / kStatChunks;
-  const int first = chunk * per, last = min(anchors, first + per);
   const int32_t* plane = quantized + c * store;
-  for (int k = first; k < last; ++k) {
-    const uint32_t packed = anchor_list[static_cast<size_t>(g) * kGroupAnchors + k];
-    const size_t block = static_cast<size_t>(packed >> 16) * bw + (packed & 0xffff);
-    const int st = __builtin_amdgcn_readfirstlane(cells[block] >> 1);
-    const int32_t* coef = plane + offsets[block];
-    const int family = OrderFamily(st);
-    present |= 1u << family;
-    if (family == 0) {
-      const uint32_t zero = coef[lane] == 0;
-      z8 += zero;
-      if (samples[block] != 0) sampled += zero;
-    } else if (family == 4) {
+  // family f's transforms, U at a time: the loads of a batch issue
+  // together, unconditionally (a batch's tail repeats the family's last
+  // transform and counts nothing)
+  auto walk = [&](int f, auto per_transform, auto unroll) {
+    constexpr int U = decltype(unroll)::value;
+    const int begin = family_start[f], end = family_start[f + 1];
+    if (end > begin) present |= 1u << FullFamily(f);
+    for (int k = begin; k < end; k += U) {
 #pragma unroll
-      for (int m = 0; m < 2; ++m) z16x8[m] += coef[lane + 64 * m] == 0;
-    } else if (family == 2) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) z16[m] += coef[lane + 64 * m] == 0;
-    } else if (family == 6) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) z32x16[m] += coef[lane + 64 * m] == 0;
-    } else {
-#pragma unroll
-      for (int m = 0; m < 16; ++m) z32[m] += coef[lane + 64 * m] == 0;
+      for (int u = 0; u < U; ++u) {
+        const int at = min(k + u, end - 1);
+        per_transform(plane + list_offset[at], at, k + u < end);
+      }
     }
-  }
+  };
+  walk(CompactFamily(0), [&](const int32_t* coef, int k, bool live) {
+    const uint32_t zero = (coef[lane] == 0) & live;
+    z8 += zero;
+    sampled += list_sampled[k] ? zero : 0u;
+  }, std::integral_constant<int, 8>{});
+  walk(CompactFamily(4), [&](const int32_t* coef, int, bool live) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) z16x8[m] += (coef[lane + 64 * m] == 0) & live;
+  }, std::integral_constant<int, 4>{});
+  walk(CompactFamily(2), [&](const int32_t* coef, int, bool live) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) z16[m] += (coef[lane + 64 * m] == 0) & live;
+  }, std::integral_constant<int, 4>{});
+  walk(CompactFamily(6), [&](const int32_t* coef, int, bool live) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) z32x16[m] += (coef[lane + 64 * m] == 0) & live;
+  }, std::integral_constant<int, 2>{});
+  walk(CompactFamily(3), [&](const int32_t* coef, int, bool live) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) z32[m] += (coef[lane + 64 * m] == 0) & live;
+  }, std::integral_constant<int, 1>{});
   uint32_t* zc = stats + c * kZeroCounters;
   auto flush = [&](int family, const uint32_t* counts, int n) {
     for (int m = 0; m < n; ++m) {
@@ -736,6 +784,7 @@ GJXLResult gjxl_amd_tokenizer_create(GJXLAmdDevice* device, uint32_t width, uint
       return r;
     }
   }
+
   GJXLResult r = Check(hipHostMalloc(reinterpret_cast<void**>(&t->staged_orders),
                                      4 * 3 * kOrderStride, hipHostMallocDefault),
                        "hipHostMalloc(order staging)");
@@ -773,6 +822,7 @@ void gjxl_amd_tokenizer_destroy(GJXLAmdTokenizer* t) {
   for (void*& b : t->buffers) {
     if (t->device != nullptr) gjxl_amd::DeviceFree(t->device, b);
   }
+
   if (t->staged_copy != nullptr) (void)hipEventDestroy(t->staged_copy);
   if (t->staged_orders != nullptr) (void)hipHostFree(t->staged_orders);
   if (t->staged_inverse != nullptr) (void)hipHostFree(t->staged_inverse);
@@ -940,6 +990,7 @@ GJXLResult ResizeTokenizer(GJXLAmdTokenizer* t, uint32_t width, uint32_t height)
   size_t need[GJXL_AMD_TOKENS_BUFFER_COUNT] = {};
   bool fits = TokenizerGeometry(&probe, width, height, need);
   for (int i = 0; fits && i < GJXL_AMD_TOKENS_BUFFER_COUNT; ++i) fits = need[i] <= t->capacity[i];
+
   if (!fits) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "extent exceeds the tokenizer's capacity");
   (void)TokenizerGeometry(t, width, height, t->sizes);
   return UploadSampleDecisions(t);
