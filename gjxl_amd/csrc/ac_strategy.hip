@@ -222,9 +222,9 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
           // n is a multiple of 64, so the lane's count is a whole number of
           // batches.  Batches go in groups of three through a ring of three
           // register sets, each read two batches ahead of its ordered adds
-          // (reads past the lane's count stay inside the term buffers); a
-          // batch past the chunk's end leaves acc unchanged by a select, as
-          // a branch would let the reads sink behind it.
+          // (reads past the lane's count stay inside the term buffers); the
+          // one or two batches after the last whole group are already in
+          // the first two sets.
           const int batches = min(kCflChunk, n - c * kCflChunk) / 4 / kCflSumBatch;
           const float4* src =
             reinterpret_cast<const float4*>(&terms[c & 1][sp][skind][slane][0]);
@@ -234,28 +234,29 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
 #pragma unroll
             for (int m = 0; m < kQuads; ++m) dst[m] = src[b * kQuads + m];
           };
-          auto add = [&](int b, const float4 (&v)[kQuads]) {
-            const bool live = b < batches;
+          auto add = [&](const float4 (&v)[kQuads]) {
 #pragma unroll
             for (int m = 0; m < kQuads; ++m) {
-              float t = acc + v[m].x;
-              t = t + v[m].y;
-              t = t + v[m].z;
-              t = t + v[m].w;
-              acc = live ? t : acc;
+              acc = acc + v[m].x;
+              acc = acc + v[m].y;
+              acc = acc + v[m].z;
+              acc = acc + v[m].w;
             }
           };
           load(0, r0);
           load(1, r1);
+          int b = 0;
 #pragma unroll 1
-          for (int b = 0; b < batches; b += 3) {
+          for (; b + 3 <= batches; b += 3) {
             load(b + 2, r2);
-            add(b, r0);
+            add(r0);
             load(b + 3, r0);
-            add(b + 1, r1);
+            add(r1);
             load(b + 4, r1);
-            add(b + 2, r2);
+            add(r2);
           }
+          if (b < batches) add(r0);
+          if (b + 1 < batches) add(r1);
         }
         if (c + 1 < kCflChunks && producer && c + 1 < chunks) produce(c + 1);
         __syncthreads();
