@@ -700,40 +700,62 @@ __device__ __forceinline__ int StageOf(int strategy) {
 
 constexpr int kSearchThreads = 64;
 
-// The tile's state and its candidate costs live in LDS; one thread of the
-// work group runs the search.
+// The tile's search state lives across the wave: lane i keeps block i's
+// cell (strategy << 1 | anchor), cost, priority and seven candidate costs in
+// registers.  Every lane runs the search with the same (uniform) values and
+// reads a block's state with readlane, writes it with a per-lane select: a few cycles
+// where an LDS round trip costs about a hundred, and the search is one long
+// chain of such accesses.
 struct TileSearch {
-  const float (*table)[64];  // [stage][block], the tile's candidate costs
   int bw, bh;
   int tbx, tby;  // tile origin in blocks
   int tw, th;
-  uint8_t* cells;  // [block]
-  float* cost;
-  uint8_t* prio;
+  int cell;      // this lane's block
+  float cost;
+  int prio;
+  float table[7];  // [stage]
 
-  __device__ uint8_t& Cell(int lx, int ly) { return cells[ly * 8 + lx]; }
-  __device__ float& Cost(int lx, int ly) { return cost[ly * 8 + lx]; }
-  __device__ uint8_t& Prio(int lx, int ly) { return prio[ly * 8 + lx]; }
-
-  __device__ float Table(int strategy, int lx, int ly) const {
-    return table[StageOf(strategy)][ly * 8 + lx];
+  __device__ __forceinline__ int Cell(int lx, int ly) const {
+    return __builtin_amdgcn_readlane(cell, ly * 8 + lx);
+  }
+  __device__ __forceinline__ float Cost(int lx, int ly) const {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cost), ly * 8 + lx));
+  }
+  __device__ __forceinline__ int Prio(int lx, int ly) const {
+    return __builtin_amdgcn_readlane(prio, ly * 8 + lx);
+  }
+  int lane;
+  // a uniform value into one lane: a compare and a select
+  __device__ __forceinline__ void SetCell(int lx, int ly, int v) {
+    cell = lane == ly * 8 + lx ? v : cell;
+  }
+  __device__ __forceinline__ void SetCost(int lx, int ly, float v) {
+    cost = lane == ly * 8 + lx ? v : cost;
+  }
+  __device__ __forceinline__ void SetPrio(int lx, int ly, int v) {
+    prio = lane == ly * 8 + lx ? v : prio;
+  }
+  // strategy is a constant at every call site, so the stage folds
+  __device__ __forceinline__ float Table(int strategy, int lx, int ly) const {
+    return __int_as_float(
+      __builtin_amdgcn_readlane(__float_as_int(table[StageOf(strategy)]), ly * 8 + lx));
   }
 
-  __device__ void Set(int lx, int ly, int strategy) {
+  __device__ __forceinline__ void Set(int lx, int ly, int strategy) {
     const int cw = CoveredW(strategy), ch = CoveredH(strategy);
     for (int dy = 0; dy < ch; ++dy)
       for (int dx = 0; dx < cw; ++dx)
-        Cell(lx + dx, ly + dy) = static_cast<uint8_t>((strategy << 1) | (dx == 0 && dy == 0));
+        SetCell(lx + dx, ly + dy, (strategy << 1) | (dx == 0 && dy == 0));
   }
 
   // src/codec/ac_strategy.cpp:172-191 (global block coordinates)
-  __device__ bool CrossesHorizontal(int sx, int y, int ex) {
+  __device__ __forceinline__ bool CrossesHorizontal(int sx, int y, int ex) const {
     if (sx >= bw || y >= bh || y % 8 == 0) return false;
     ex = min(ex, bw);
     const int limit = sx & ~7;
     while (sx != limit && !(Cell(sx - tbx, y - tby) & 1)) --sx;
     for (int x = sx; x < ex;) {
-      const uint8_t c = Cell(x - tbx, y - tby);
+      const int c = Cell(x - tbx, y - tby);
       if (!(c & 1)) return true;
       x += CoveredW(c >> 1);
     }
@@ -741,28 +763,28 @@ struct TileSearch {
   }
 
   // :193-213
-  __device__ bool CrossesVertical(int x, int sy, int ey) {
+  __device__ __forceinline__ bool CrossesVertical(int x, int sy, int ey) const {
     if (x >= bw || sy >= bh || x % 8 == 0) return false;
     ey = min(ey, bh);
     const int limit = sy & ~7;
     while (sy != limit && !(Cell(x - tbx, sy - tby) & 1)) --sy;
     for (int y = sy; y < ey;) {
-      const uint8_t c = Cell(x - tbx, y - tby);
+      const int c = Cell(x - tbx, y - tby);
       if (!(c & 1)) return true;
       y += CoveredH(c >> 1);
     }
     return false;
   }
 
-  __device__ void SetCostForTransform(int lx, int ly, int strategy, float c) {
+  __device__ __forceinline__ void SetCostForTransform(int lx, int ly, int strategy, float c) {
     const int cw = CoveredW(strategy), ch = CoveredH(strategy);
     for (int dy = 0; dy < ch; ++dy)
-      for (int dx = 0; dx < cw; ++dx) Cost(lx + dx, ly + dy) = 0.0f;
-    Cost(lx, ly) = c;
+      for (int dx = 0; dx < cw; ++dx) SetCost(lx + dx, ly + dy, 0.0f);
+    SetCost(lx, ly, c);
   }
 
   // :340-384
-  __device__ void TryMerge(int strategy, int lx, int ly, uint8_t priority) {
+  __device__ __forceinline__ void TryMerge(int strategy, int lx, int ly, int priority) {
     const int cw = CoveredW(strategy), ch = CoveredH(strategy);
     float current = 0.0f;
     for (int dy = 0; dy < ch; ++dy) {
@@ -775,16 +797,16 @@ struct TileSearch {
     if (candidate >= current) return;
     for (int dy = 0; dy < ch; ++dy) {
       for (int dx = 0; dx < cw; ++dx) {
-        Cost(lx + dx, ly + dy) = 0.0f;
-        Prio(lx + dx, ly + dy) = priority;
+        SetCost(lx + dx, ly + dy, 0.0f);
+        SetPrio(lx + dx, ly + dy, priority);
       }
     }
     Set(lx, ly, strategy);
-    Cost(lx, ly) = candidate;
+    SetCost(lx, ly, candidate);
   }
 
   // :404-589
-  __device__ void FirstLevelDivision(int blocks, int lx, int ly) {
+  __device__ __forceinline__ void FirstLevelDivision(int blocks, int lx, int ly) {
     const int half = blocks / 2;
     const int bx = tbx + lx, by = tby + ly;
     if (CrossesHorizontal(bx, by, bx + blocks) ||
@@ -832,30 +854,24 @@ struct TileSearch {
   }
 };
 
-// grid: one work group of 64 threads per tile.  Thread i loads block i's
-// seven candidate costs and initialises its state; thread 0 searches; thread
-// i stores block i's cell.
+// grid: one work group of one wave per tile.  Lane i loads block i's seven
+// candidate costs and initialises its state; the wave runs the search
+// (src/codec/ac_strategy.cpp:591-773 SearchTile); lane i stores block i's
+// cell.
 __global__ __launch_bounds__(kSearchThreads) void SearchKernel(
   const float* __restrict__ costs, int bw, int bh, float target,
   uint8_t* __restrict__ out_cells) {
-  __shared__ float table[7][64];
-  __shared__ uint8_t cells[64];
-  __shared__ float cost[64];
-  __shared__ uint8_t prio[64];
   const int tiles_w = (bw + 7) / 8;
   const int tile = blockIdx.x;
   TileSearch s;
-  s.table = table;
   s.bw = bw;
   s.bh = bh;
   s.tbx = (tile % tiles_w) * 8;
   s.tby = (tile / tiles_w) * 8;
   s.tw = min(8, bw - s.tbx);
   s.th = min(8, bh - s.tby);
-  s.cells = cells;
-  s.cost = cost;
-  s.prio = prio;
   const int i = threadIdx.x, lx = i % 8, ly = i / 8;
+  s.lane = i;
   const bool inside = lx < s.tw && ly < s.th;
   const size_t plane = static_cast<size_t>(bw) * bh;
   const size_t block = static_cast<size_t>(s.tby + (inside ? ly : 0)) * bw + s.tbx + (inside ? lx : 0);
@@ -863,52 +879,46 @@ __global__ __launch_bounds__(kSearchThreads) void SearchKernel(
 #pragma unroll
   for (int st = 0; st < 7; ++st) v[st] = costs[st * plane + block];
 #pragma unroll
-  for (int st = 0; st < 7; ++st) table[st][i] = inside ? v[st] : 0.0f;
-  cells[i] = static_cast<uint8_t>((kDct8 << 1) | 1);
-  cost[i] = 0.0f;
-  prio[i] = 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // src/codec/ac_strategy.cpp:591-773 SearchTile
-    const float baseline = 1.0f + -0.4f / (target + 1.4f);
-    for (int y = 0; y < s.th; ++y)
-      for (int x = 0; x < s.tw; ++x) s.Cost(x, y) = s.Table(kDct8, x, y) * baseline;
-    constexpr int kMergeStrategy[4] = {kDct16x8, kDct8x16, kDct16x32, kDct32x16};
-    constexpr uint8_t kMergePriority[4] = {2, 2, 4, 4};
-    for (int m = 0; m < 4; ++m) {
-      const int st = kMergeStrategy[m];
-      const int cw = CoveredW(st), ch = CoveredH(st);
-      for (int y = 0; y + ch <= s.th; y += ch) {
-        for (int x = 0; x + cw <= s.tw; x += cw) {
-          if (y + 3 < s.th && x + 3 < s.tw) {
-            if (st == kDct16x32) {
-              if ((y | x) % 4 == 0) s.FirstLevelDivision(4, x, y);
-              continue;
-            }
-            if (st == kDct32x16) continue;
+  for (int st = 0; st < 7; ++st) s.table[st] = inside ? v[st] : 0.0f;
+  s.cell = (kDct8 << 1) | 1;
+  s.prio = 0;
+  const float baseline = 1.0f + -0.4f / (target + 1.4f);
+  s.cost = inside ? s.table[StageOf(kDct8)] * baseline : 0.0f;
+  constexpr int kMergeStrategy[4] = {kDct16x8, kDct8x16, kDct16x32, kDct32x16};
+  constexpr int kMergePriority[4] = {2, 2, 4, 4};
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int st = kMergeStrategy[m];
+    const int cw = CoveredW(st), ch = CoveredH(st);
+    for (int y = 0; y + ch <= s.th; y += ch) {
+      for (int x = 0; x + cw <= s.tw; x += cw) {
+        if (y + 3 < s.th && x + 3 < s.tw) {
+          if (st == kDct16x32) {
+            if ((y | x) % 4 == 0) s.FirstLevelDivision(4, x, y);
+            continue;
           }
-          if ((st == kDct16x32 && y % 4 != 0) || (st == kDct32x16 && x % 4 != 0)) continue;
-          if (y + 1 < s.th && x + 1 < s.tw) {
-            if (st == kDct8x16) {
-              if ((y | x) % 2 == 0) s.FirstLevelDivision(2, x, y);
-              continue;
-            }
-            if (st == kDct16x8) continue;
-          }
-          if ((st == kDct8x16 && y % 2 == 1) || (st == kDct16x8 && x % 2 == 1)) continue;
-          s.TryMerge(st, x, y, kMergePriority[m]);
+          if (st == kDct32x16) continue;
         }
+        if ((st == kDct16x32 && y % 4 != 0) || (st == kDct32x16 && x % 4 != 0)) continue;
+        if (y + 1 < s.th && x + 1 < s.tw) {
+          if (st == kDct8x16) {
+            if ((y | x) % 2 == 0) s.FirstLevelDivision(2, x, y);
+            continue;
+          }
+          if (st == kDct16x8) continue;
+        }
+        if ((st == kDct8x16 && y % 2 == 1) || (st == kDct16x8 && x % 2 == 1)) continue;
+        s.TryMerge(st, x, y, kMergePriority[m]);
       }
     }
-    for (int y = 0; y + 1 < s.th; ++y)
-      for (int x = 0; x + 1 < s.tw; ++x)
-        if ((y | x) % 2 != 0) s.FirstLevelDivision(2, x, y);
-    for (int y = 0; y + 3 < s.th; y += 2)
-      for (int x = 0; x + 3 < s.tw; x += 2)
-        if ((y | x) % 4 != 0) s.FirstLevelDivision(4, x, y);
   }
-  __syncthreads();
-  if (inside) out_cells[block] = cells[i];
+  for (int y = 0; y + 1 < s.th; ++y)
+    for (int x = 0; x + 1 < s.tw; ++x)
+      if ((y | x) % 2 != 0) s.FirstLevelDivision(2, x, y);
+  for (int y = 0; y + 3 < s.th; y += 2)
+    for (int x = 0; x + 3 < s.tw; x += 2)
+      if ((y | x) % 4 != 0) s.FirstLevelDivision(4, x, y);
+  if (inside) out_cells[block] = static_cast<uint8_t>(s.cell);
 }
 
 template <int kStrategy>
