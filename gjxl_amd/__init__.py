@@ -84,6 +84,24 @@ _SIGNATURES = {
     "gjxl_amd_ac_strategy_costs":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P],
     "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
+    "gjxl_amd_ac_strategy_costs_pruned":
+        [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P, _P],
+    "gjxl_amd_proposal_parameter_count": [],
+    "gjxl_amd_proposal_create": [_P, _SZ, ctypes.POINTER(_P)],
+    "gjxl_amd_proposal_destroy": [_P],
+    "gjxl_amd_proposal_set_parameters": [_P, _P, _SZ],
+    "gjxl_amd_proposal_get_parameters": [_P, _P, _SZ],
+    "gjxl_amd_proposal_buffer": [_P, ctypes.c_int32, ctypes.POINTER(_P),
+                                 ctypes.POINTER(_SZ)],
+    "gjxl_amd_proposal_features": [_P, _P, _U32, _U32, _P, _P],
+    "gjxl_amd_proposal_forward": [_P, _P, _U32, _U32, _U32],
+    "gjxl_amd_proposal_backward": [_P, _P, _P, _F, _P],
+    "gjxl_amd_proposal_adam": [_P, _F, _F, _F, _F, _F],
+    "gjxl_amd_proposal_decide": [_P, _P, _U32, _U32, _F, _U32, _F, _P],
+    "gjxl_amd_proposal_warm_start": [_P, _P, _P, _SZ, _F, _F, _F, _P],
+    "gjxl_amd_proposal_options_init": [_P, _SZ],
+    "gjxl_amd_plain_path_set_proposal": [_P, _P, _P],
+    "gjxl_amd_plain_path_proposal_sample": [_P, _P, _P, _P],
     "gjxl_amd_aq_session_create": [_P, _U32, _U32, ctypes.POINTER(_P)],
     "gjxl_amd_aq_session_destroy": [_P],
     "gjxl_amd_aq_prepare": [_P, _P, _P, _P, _P, _F],
@@ -139,9 +157,9 @@ _VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
          "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy",
          "gjxl_amd_plain_path_destroy", "gjxl_amd_host_free",
          "gjxl_amd_batch_results_free", "gjxl_amd_batch_shutdown",
-         "gjxl_amd_batch_destroy"}
+         "gjxl_amd_batch_destroy", "gjxl_amd_proposal_destroy"}
 _STR = {"gjxl_amd_last_error"}
-_SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes"}
+_SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes", "gjxl_amd_proposal_parameter_count"}
 
 
 def exported_symbols() -> list[str]:
@@ -795,14 +813,14 @@ class PlainFrame(ctypes.Structure):
                 ("token_count", ctypes.c_uint64), ("x_qm_scale", ctypes.c_int32),
                 ("b_qm_scale", ctypes.c_int32), ("quant_dc", _F),
                 ("evaluations", ctypes.c_int32), ("scores", _F * 8),
-                ("entropy_inputs", _P)]
+                ("entropy_inputs", _P), ("proposal_allow", _P)]
 
 
 STAGE_NAMES = ("begin", "upload", "srgb_to_linear_and_opsin", "gaborish_inverse",
                "initial_quant_field", "initial_cfl", "ac_strategy_costs",
                "ac_strategy_search", "aq_prepare", "butteraugli_prepare", "aq_evaluate",
                "butteraugli_compare", "aq_reduce_update", "ac_tokenize", "entropy_inputs",
-               "token_readback")
+               "token_readback", "proposal")
 
 
 class HostBuffer:
@@ -956,10 +974,37 @@ class NativePlainPath:
                   "block_distance": ((bh, bw), np.float32),
                   "raw_quant": ((bh, bw), np.int32), "quantized": ((3, store), np.int32),
                   "dc_quantized": ((3, bh, bw), np.int32), "y_to_x": (tiles, np.int8),
-                  "y_to_b": (tiles, np.int8)}
+                  "y_to_b": (tiles, np.int8), "proposal_allow": ((bh, bw), np.uint8)}
         shape, dtype = shapes[name]
-        return DeviceArray(self.device, shape, dtype, pointer=getattr(self.frame, name),
-                           owner=self)
+        pointer = getattr(self.frame, name)
+        if not pointer:
+            raise RuntimeError(f"the last run has no {name} buffer")
+        return DeviceArray(self.device, shape, dtype, pointer=pointer, owner=self)
+
+    def set_proposal(self, net, **options) -> None:
+        """Attaches a proposal network (gjxl_amd.proposal.ProposalNet) for
+        learned runs, with GJXLAmdProposalOptions fields as keywords, or
+        detaches it (net None)."""
+        from .proposal import ProposalOptions
+
+        opts = ProposalOptions.defaults()
+        for k, v in options.items():
+            if not hasattr(opts, k):
+                raise TypeError(f"unknown proposal option {k!r}")
+            setattr(opts, k, v)
+        _check(library().gjxl_amd_plain_path_set_proposal(
+            self.handle, None if net is None else net.handle, ctypes.byref(opts)))
+        self.proposal = net
+
+    def proposal_sample(self, features: DeviceArray, labels: DeviceArray,
+                        targets: DeviceArray) -> None:
+        """Training data of the last run: features, strategy-code labels and
+        log(final / adjusted field) targets per block."""
+        n = self.bw * self.bh
+        if features.nbytes < 32 * n or labels.nbytes < n or targets.nbytes < 4 * n:
+            raise ValueError("proposal sample buffers are smaller than the block grid")
+        _check(library().gjxl_amd_plain_path_proposal_sample(
+            self.handle, features.ptr, labels.ptr, targets.ptr))
 
     def stage_times(self) -> list[tuple[str, float]]:
         stages = (ctypes.c_int32 * 64)()

@@ -301,6 +301,12 @@ __device__ __forceinline__ int CeilLog2Nonzero(int v) {
   return v <= 1 ? 0 : 32 - __clz(v - 1);
 }
 
+// cost plane (stage) of each searched strategy
+__host__ __device__ constexpr int StageOf(int strategy) {
+  return strategy == kDct8 ? 0 : strategy == kDct16x8 ? 1 : strategy == kDct8x16 ? 2
+       : strategy == kDct16x16 ? 3 : strategy == kDct32x16 ? 4 : strategy == kDct16x32 ? 5 : 6;
+}
+
 constexpr int kCostThreads = 512;
 constexpr int kCostSlots = 3072;
 
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
   const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
   const float* __restrict__ tables, const double* __restrict__ bases,
-  CostParams params, float* __restrict__ costs) {
+  CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs) {
   using S = CostShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
   constexpr int KC = 3 * K;     // candidate-channel planes
@@ -388,6 +394,26 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
     *by = ty * 8 + ly;
     return p < S::kPositions && lx + S::kCw <= tbw && ly + S::kCh <= tbh;
   };
+  // The learned form's pruning: a group none of whose candidates is allowed
+  // at its anchor writes +infinity for them (never merged) and does no work.
+  if (allow != nullptr) {
+    constexpr int kStage = StageOf(kStrategy);
+    bool wanted = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      int bx, by;
+      if (candidate(k, &bx, &by) && ((allow[static_cast<size_t>(by) * bw + bx] >> kStage) & 1)) {
+        wanted = true;
+      }
+    }
+    if (!wanted) {
+      int bx, by;
+      if (tid < K && candidate(tid, &bx, &by)) {
+        costs[static_cast<size_t>(by) * bw + bx] = __builtin_inff();
+      }
+      return;
+    }
+  }
 
   // Every global read of the work group is issued here, in the order the
   // results are needed (reads complete in issue order): the pixels, which
@@ -686,18 +712,6 @@ __global__ void FillNanKernel(float* __restrict__ p, size_t n) {
 constexpr int kStageStrategy[7] = {kDct8, kDct16x8, kDct8x16, kDct16x16,
                                    kDct32x16, kDct16x32, kDct32x32};
 
-__device__ __forceinline__ int StageOf(int strategy) {
-  switch (strategy) {
-    case kDct8: return 0;
-    case kDct16x8: return 1;
-    case kDct8x16: return 2;
-    case kDct16x16: return 3;
-    case kDct32x16: return 4;
-    case kDct16x32: return 5;
-    default: return 6;
-  }
-}
-
 constexpr int kSearchThreads = 64;
 
 // The tile's search state lives across the wave: lane i keeps block i's
@@ -925,14 +939,14 @@ template <int kStrategy>
 GJXLResult LaunchCost(GJXLAmdDevice* device, const float* opsin, int width,
                       int height, const float* qf, const float* mask,
                       const int8_t* yx, const int8_t* yb,
-                      const CostParams& params, float* stage_costs) {
+                      const CostParams& params, const uint8_t* allow, float* stage_costs) {
   using S = CostShape<kStrategy>;
   const int bw = width / 8, bh = height / 8;
   const dim3 grid(CeilDiv(S::kPositions, S::kCands), CeilDiv(bw, 8) * CeilDiv(bh, 8));
   const bool probed = ProbeBegin(device, GJXL_AMD_KERNEL_CANDIDATE_COST);
   hipLaunchKernelGGL(CandidateCostKernel<kStrategy>, grid, dim3(kCostThreads),
                      0, device->stream, opsin, width, height, qf, mask, yx, yb,
-                     device->quant_tables, device->dct_bases, params,
+                     device->quant_tables, device->dct_bases, params, allow,
                      stage_costs);
   ProbeEnd(device, probed);
   return CheckLaunch("CandidateCostKernel");
@@ -953,7 +967,8 @@ GJXLResult LaunchInitialCfl(GJXLAmdDevice* device, const float* opsin,
 GJXLResult LaunchCandidateCosts(GJXLAmdDevice* device, const float* opsin,
                                 int width, int height, const float* qf,
                                 const float* mask, const int8_t* yx,
-                                const int8_t* yb, float target, float* costs) {
+                                const int8_t* yb, float target, const uint8_t* allow,
+                                float* costs) {
   // src/codec/ac_strategy.cpp:884-891 (host libm, once per call)
   const float kBias = 0.13731742964354549f;
   const float ratio = (target + kBias) / (1.0f + kBias);
@@ -967,19 +982,20 @@ GJXLResult LaunchCandidateCosts(GJXLAmdDevice* device, const float* opsin,
   GJXL_AMD_TRY(CheckLaunch("FillNanKernel"));
   constexpr float kEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
   p.entropy_multiplier = kEntropy[0];
-  GJXL_AMD_TRY(LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 0 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 0 * stage));
   p.entropy_multiplier = kEntropy[1];
-  GJXL_AMD_TRY(LaunchCost<kDct16x8>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 1 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct16x8>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 1 * stage));
   p.entropy_multiplier = kEntropy[2];
-  GJXL_AMD_TRY(LaunchCost<kDct8x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 2 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct8x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 2 * stage));
   p.entropy_multiplier = kEntropy[3];
-  GJXL_AMD_TRY(LaunchCost<kDct16x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 3 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct16x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 3 * stage));
   p.entropy_multiplier = kEntropy[4];
-  GJXL_AMD_TRY(LaunchCost<kDct32x16>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 4 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct32x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 4 * stage));
   p.entropy_multiplier = kEntropy[5];
-  GJXL_AMD_TRY(LaunchCost<kDct16x32>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 5 * stage));
+  GJXL_AMD_TRY(LaunchCost<kDct16x32>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 5 * stage));
   p.entropy_multiplier = kEntropy[6];
-  return LaunchCost<kDct32x32>(device, opsin, width, height, qf, mask, yx, yb, p, costs + 6 * stage);
+  return LaunchCost<kDct32x32>(device, opsin, width, height, qf, mask, yx, yb, p, allow,
+                               costs + 6 * stage);
 }
 
 GJXLResult LaunchSearch(GJXLAmdDevice* device, const float* costs, int width,
@@ -1044,7 +1060,24 @@ GJXLResult gjxl_amd_ac_strategy_costs(GJXLAmdDevice* device,
   }
   return gjxl_amd::LaunchCandidateCosts(device, opsin, width, height,
                                         quant_field, pixel_mask, y_to_x,
-                                        y_to_b, butteraugli_target, costs);
+                                        y_to_b, butteraugli_target, nullptr, costs);
+}
+
+GJXLResult gjxl_amd_ac_strategy_costs_pruned(GJXLAmdDevice* device, const float* opsin,
+                                             uint32_t width, uint32_t height,
+                                             const float* quant_field,
+                                             const float* pixel_mask, const int8_t* y_to_x,
+                                             const int8_t* y_to_b, float butteraugli_target,
+                                             const uint8_t* allow, float* costs) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  GJXL_AMD_TRY(ValidateGeometry(width, height));
+  GJXL_AMD_TRY(ValidateTarget(butteraugli_target));
+  if (opsin == nullptr || quant_field == nullptr || pixel_mask == nullptr ||
+      y_to_x == nullptr || y_to_b == nullptr || costs == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "AC cost buffers must be non-null");
+  }
+  return gjxl_amd::LaunchCandidateCosts(device, opsin, width, height, quant_field, pixel_mask,
+                                        y_to_x, y_to_b, butteraugli_target, allow, costs);
 }
 
 GJXLResult gjxl_amd_ac_strategy_search(GJXLAmdDevice* device,

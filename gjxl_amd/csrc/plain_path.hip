@@ -26,6 +26,7 @@
 #include "aq_common.h"
 #include "butteraugli.h"
 #include "common.h"
+#include "proposal.h"
 #include "tokenize.h"
 
 namespace {
@@ -36,7 +37,7 @@ constexpr int kMaxEvaluations = 8;
 // The path's own device buffers
 enum PathBuffer {
   kSrgb, kLinear, kOpsin, kPreprocessed, kQuantField, kStrategyMask, kPixelMask,
-  kIqfScratch, kYToX, kYToB, kCosts, kCells, kScoreWords, kPathBuffers
+  kIqfScratch, kYToX, kYToB, kCosts, kCells, kScoreWords, kFeatures, kAllow, kPathBuffers
 };
 
 }  // namespace
@@ -62,6 +63,12 @@ struct GJXLAmdPlainPath {
   float* costs = nullptr;
   uint8_t* cells = nullptr;
   uint32_t* score_words = nullptr;
+  // the learned run's block features and allowed stages
+  uint16_t* features = nullptr;
+  uint8_t* allow = nullptr;
+  GJXLAmdProposal* proposal = nullptr;
+  GJXLAmdProposalOptions proposal_options{};
+  int32_t evaluations_run = 0;
   GJXLAmdAqSession* session = nullptr;
   GJXLAmdButteraugli* butteraugli = nullptr;
   GJXLAmdTokenizer* tokenizer = nullptr;
@@ -131,6 +138,8 @@ void PathGeometry(GJXLAmdPlainPath* p, uint32_t width, uint32_t height, size_t* 
   z[kCosts] = 4 * 7 * blocks;
   z[kCells] = blocks;
   z[kScoreWords] = 4 * kMaxEvaluations;
+  z[kFeatures] = 2 * GJXL_AMD_PROPOSAL_FEATURES * blocks;
+  z[kAllow] = blocks;
 }
 
 void PointViews(GJXLAmdPlainPath* p) {
@@ -147,6 +156,8 @@ void PointViews(GJXLAmdPlainPath* p) {
   p->costs = static_cast<float*>(p->own[kCosts]);
   p->cells = static_cast<uint8_t*>(p->own[kCells]);
   p->score_words = static_cast<uint32_t*>(p->own[kScoreWords]);
+  p->features = static_cast<uint16_t*>(p->own[kFeatures]);
+  p->allow = static_cast<uint8_t*>(p->own[kAllow]);
 }
 
 template <typename T>
@@ -211,6 +222,21 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(gjxl_amd_initial_quant_field(d, p->opsin, p->pw, p->ph, t, 1.0f, p->quant_field,
                                             p->strategy_mask, p->pixel_mask, p->iqf_scratch));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_QUANT_FIELD));
+  const GJXLAmdProposalOptions& po = p->proposal_options;
+  const float* head = nullptr;
+  if (p->proposal != nullptr) {
+    // the learned form: features, the network's forward and the allowed
+    // stages of every block
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalFeatures(d, p->preprocessed, p->pw, p->ph,
+                                                  p->quant_field, p->features));
+    GJXL_AMD_TRY(gjxl_amd_proposal_forward(p->proposal, p->features, 1, p->bh, p->bw));
+    void* hp = nullptr;
+    GJXL_AMD_TRY(gjxl_amd_proposal_buffer(p->proposal, GJXL_AMD_PROPOSAL_HEAD, &hp, nullptr));
+    head = static_cast<const float*>(hp);
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalDecide(d, head, p->bw, p->bh, po.threshold,
+                                                static_cast<int>(po.top_k), po.p_min, p->allow));
+    GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_PROPOSAL));
+  }
   GJXL_AMD_TRY(gjxl_amd_initial_cfl(d, p->preprocessed, p->pw, p->ph, p->y_to_x, p->y_to_b));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_CFL));
   // what is left of the reference side once the device stream gets here:
@@ -219,8 +245,9 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
                      "hipStreamWaitEvent"));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
-  GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs(d, p->preprocessed, p->pw, p->ph, p->quant_field,
-                                          p->pixel_mask, p->y_to_x, p->y_to_b, t, p->costs));
+  GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs_pruned(
+    d, p->preprocessed, p->pw, p->ph, p->quant_field, p->pixel_mask, p->y_to_x, p->y_to_b, t,
+    p->proposal != nullptr ? p->allow : nullptr, p->costs));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_COSTS));
   GJXL_AMD_TRY(gjxl_amd_ac_strategy_search(d, p->costs, p->pw, p->ph, t, p->cells));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_SEARCH));
@@ -234,9 +261,21 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   float* distance = AqBuffer<float>(p->session, GJXL_AMD_AQ_BLOCK_DISTANCE);
   const float* decoded = AqBuffer<float>(p->session, GJXL_AMD_AQ_LINEAR);
   const size_t n = static_cast<size_t>(p->bw) * p->bh;
-  GJXL_AMD_TRY(gjxl_amd_copy(d, field, AqBuffer<float>(p->session, GJXL_AMD_AQ_ADJUSTED_FIELD),
-                             4 * n));
-  for (int it = 0; it <= p->iterations; ++it) {
+  const float* adjusted = AqBuffer<float>(p->session, GJXL_AMD_AQ_ADJUSTED_FIELD);
+  // updates planned before the first evaluation: the plain path's, or the
+  // learned run's (whose judge may add more, up to the plain count)
+  int planned = p->iterations;
+  if (head != nullptr) {
+    float lower = 0.0f, upper = 0.0f;
+    GJXL_AMD_TRY(gjxl_amd_aq_policy(p->session, &lower, &upper, nullptr, nullptr, nullptr));
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalWarmStart(d, head, adjusted, n, po.field_limit, lower,
+                                                   upper, field));
+    planned = po.evaluations - 1;
+  } else {
+    GJXL_AMD_TRY(gjxl_amd_copy(d, field, adjusted, 4 * n));
+  }
+  int it = 0;
+  for (;; ++it) {
     GJXL_AMD_TRY(gjxl_amd_aq_evaluate(p->session, field));
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_EVALUATE));
     GJXL_AMD_TRY(gjxl_amd::ButteraugliCompare(p->butteraugli, decoded, nullptr));
@@ -246,11 +285,22 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
     GJXL_AMD_TRY(gjxl_amd::CheckLaunch("CopyScoreKernel"));
     GJXL_AMD_TRY(gjxl_amd::LaunchReduceDistanceMap(st, p->butteraugli->map, p->width, p->height,
                                                    p->cells, p->bw, p->bh, distance));
-    if (it < p->iterations) {
-      GJXL_AMD_TRY(gjxl_amd_aq_update(p->session, field, distance, it, t));
+    bool more = it < planned;
+    if (!more && head != nullptr && it < p->iterations) {
+      // the judge: the learned run goes on with the plain updates while its
+      // score is above judge_ratio * distance
+      GJXL_AMD_TRY(Check(hipMemcpyAsync(p->host_words + 8 + it, p->score_words + it, 4,
+                                        hipMemcpyDeviceToHost, st), "hipMemcpyAsync(score)"));
+      GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
+      float score = 0.0f;
+      std::memcpy(&score, p->host_words + 8 + it, 4);
+      more = !(score <= po.judge_ratio * t);
     }
+    if (more) GJXL_AMD_TRY(gjxl_amd_aq_update(p->session, field, distance, it, t));
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_REDUCE_UPDATE));
+    if (!more) break;
   }
+  p->evaluations_run = it + 1;
   // The CPU-chosen tokenization inputs of effort 7 (coefficient orders and
   // the adaptive block-context map) from the final frame's statistics, with
   // the AQ error flag and the scores in the same wait
@@ -267,7 +317,7 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
                      "hipMemcpyAsync(frame statistics)"));
   GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 4, stats + 7, 4, hipMemcpyDeviceToHost, st),
                      "hipMemcpyAsync(AQ error)"));
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * (p->iterations + 1),
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * p->evaluations_run,
                                     hipMemcpyDeviceToHost, st), "hipMemcpyAsync(scores)"));
   GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
   if (host[4] != 0) {
@@ -287,7 +337,7 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
                                     16, hipMemcpyDeviceToHost, st), "hipMemcpyAsync(status)"));
   GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
   GJXL_AMD_TRY(gjxl_amd::TokenizeResult(host, &p->token_count));
-  for (int it = 0; it <= p->iterations; ++it) std::memcpy(&p->scores[it], &host[8 + it], 4);
+  for (int e = 0; e < p->evaluations_run; ++e) std::memcpy(&p->scores[e], &host[8 + e], 4);
   return GJXL_OK;
 }
 
@@ -349,7 +399,8 @@ GJXLResult FillFrame(GJXLAmdPlainPath* p, GJXLAmdPlainFrame* f) {
   float lo = 0, hi = 0, qdc = 0;
   GJXL_AMD_TRY(gjxl_amd_aq_policy(p->session, &lo, &hi, &qdc, &r.x_qm_scale, &r.b_qm_scale));
   r.quant_dc = qdc;
-  r.evaluations = p->iterations + 1;
+  r.evaluations = p->evaluations_run;
+  r.proposal_allow = p->proposal != nullptr ? p->allow : nullptr;
   for (int i = 0; i < r.evaluations; ++i) r.scores[i] = p->scores[i];
   r.entropy_inputs = p->entropy_inputs;
   *f = r;
@@ -544,6 +595,56 @@ GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* imag
   GJXL_AMD_TRY(ReadTokens(p, tokens));
   p->ran = true;
   return FillFrame(p, frame);
+}
+
+GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* p, GJXLAmdProposal* net,
+                                            const GJXLAmdProposalOptions* options) {
+  GJXL_AMD_TRY(Validate(p));
+  if (net == nullptr) {
+    p->proposal = nullptr;
+    return GJXL_OK;
+  }
+  GJXLAmdProposalOptions o{};
+  if (options == nullptr) {
+    GJXL_AMD_TRY(gjxl_amd_proposal_options_init(&o, sizeof(o)));
+  } else {
+    if (options->struct_size < sizeof(GJXLAmdProposalOptions)) {
+      return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal options are too small");
+    }
+    o = *options;
+  }
+  if (!(o.threshold >= 0.0f && o.threshold <= 1.0f) || o.top_k > 7 ||
+      !std::isfinite(o.p_min) || o.evaluations < 1 || o.evaluations > p->iterations + 1 ||
+      !(o.field_limit >= 0.0f && o.field_limit <= 16.0f) || !(o.judge_ratio > 0.0f) ||
+      !std::isfinite(o.judge_ratio)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal options out of range");
+  }
+  void* head = nullptr;
+  size_t bytes = 0;
+  GJXL_AMD_TRY(gjxl_amd_proposal_buffer(net, GJXL_AMD_PROPOSAL_HEAD, &head, &bytes));
+  const size_t positions = bytes / (4 * GJXL_AMD_PROPOSAL_CHANNELS);
+  if (positions < p->capacity[kAllow]) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT,
+                "proposal network holds fewer positions than the path's blocks");
+  }
+  p->proposal = net;
+  p->proposal_options = o;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_plain_path_proposal_sample(GJXLAmdPlainPath* p, uint16_t* features,
+                                               uint8_t* labels, float* targets) {
+  GJXL_AMD_TRY(Validate(p));
+  if (!p->ran) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "the plain path has not completed a run");
+  if (features == nullptr || labels == nullptr || targets == nullptr) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal sample outputs must be non-null");
+  }
+  GJXL_AMD_TRY(gjxl_amd::LaunchProposalFeatures(p->device, p->preprocessed, p->pw, p->ph,
+                                                p->quant_field, features));
+  return gjxl_amd::LaunchProposalSample(
+    p->device, p->cells, AqBuffer<float>(p->session, GJXL_AMD_AQ_FIELD),
+    AqBuffer<float>(p->session, GJXL_AMD_AQ_ADJUSTED_FIELD),
+    static_cast<size_t>(p->bw) * p->bh, labels, targets);
 }
 
 GJXLResult gjxl_amd_plain_path_stage_times(GJXLAmdPlainPath* p, int32_t* stages,

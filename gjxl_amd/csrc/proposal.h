@@ -1,0 +1,20 @@
+// Internal launchers of the learned proposal (proposal.hip) and of the
+// pruned candidate costs (ac_strategy.hip), for the plain path's learned run.
+#pragma once
+
+#include "common.h"
+
+namespace gjxl_amd {
+
+GJXLResult LaunchProposalFeatures(GJXLAmdDevice* device, const float* opsin, int width,
+                                  int height, const float* quant_field, uint16_t* features);
+GJXLResult LaunchProposalDecide(GJXLAmdDevice* device, const float* head, int bw, int bh,
+                                float threshold, int top_k, float p_min, uint8_t* allow);
+GJXLResult LaunchProposalWarmStart(GJXLAmdDevice* device, const float* head,
+                                   const float* adjusted, size_t n, float limit, float lower,
+                                   float upper, float* field);
+GJXLResult LaunchProposalSample(GJXLAmdDevice* device, const uint8_t* cells, const float* field,
+                                const float* adjusted, size_t n, uint8_t* labels,
+                                float* targets);
+
+}  // namespace gjxl_amd

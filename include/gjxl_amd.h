@@ -189,6 +189,17 @@ GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_costs(
   GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
   const float* quant_field, const float* pixel_mask, const int8_t* y_to_x,
   const int8_t* y_to_b, float butteraugli_target, float* costs);
+/* The learned form's pruned costs (no counterpart in the reference): as
+ * gjxl_amd_ac_strategy_costs, but a work group whose candidates all have
+ * their stage's bit clear in allow ((w/8)*(h/8) bytes, bit s = stage s of
+ * the order above, from gjxl_amd_proposal_decide) is not evaluated and
+ * writes +infinity for them, which the greedy merge never selects.  A null
+ * allow evaluates every candidate. */
+GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_costs_pruned(
+  GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
+  const float* quant_field, const float* pixel_mask, const int8_t* y_to_x,
+  const int8_t* y_to_b, float butteraugli_target, const uint8_t* allow,
+  float* costs);
 
 /* ---------------------------------------------------------------------- */
 /* a10: greedy merge of the candidate costs into a strategy grid.  Replaces  */
@@ -637,6 +648,8 @@ typedef struct {
   float scores[8];                  /* Butteraugli score per evaluation */
   /* host: the coefficient orders and block-context map of the tokens */
   const GJXLAmdEntropyInputs* entropy_inputs;
+  /* the learned run's allowed stages per block (null in a plain run) */
+  const uint8_t* proposal_allow;
 } GJXLAmdPlainFrame;
 
 /* Stage ids of gjxl_amd_plain_path_stage_times. */
@@ -657,6 +670,7 @@ enum {
   GJXL_AMD_STAGE_AC_TOKENIZE,
   GJXL_AMD_STAGE_ENTROPY_INPUTS,
   GJXL_AMD_STAGE_TOKEN_READBACK,
+  GJXL_AMD_STAGE_PROPOSAL, /* the learned run's features, forward and decisions */
   GJXL_AMD_STAGE_COUNT
 };
 
@@ -754,6 +768,147 @@ GJXL_AMD_API void gjxl_amd_batch_results_free(GJXLAmdBatchResult* results,
 GJXL_AMD_API void gjxl_amd_batch_shutdown(GJXLAmdBatch* batch);
 /* Shuts down, then frees the driver; no call may be in progress. */
 GJXL_AMD_API void gjxl_amd_batch_destroy(GJXLAmdBatch* batch);
+
+/* ---------------------------------------------------------------------- */
+/* Chosen learned form (not the plain path; no counterpart in the           */
+/* reference): a proposal network that predicts, per 8x8 block, logits over */
+/* the format's 27 strategy codes and the log ratio of the Butteraugli      */
+/* loop's final quant field to the adjusted initial field, trained on       */
+/* labels of the plain search (src/codec/ac_strategy.cpp:1101               */
+/* FindAcStrategyGrid; the loop of src/codec/adaptive_quantization.cpp:     */
+/* 1600-1738).  Its confident tiles prune the candidate costs              */
+/* (gjxl_amd_ac_strategy_costs_pruned) and its field warm-starts the loop.  */
+/*                                                                          */
+/* Network: six 3x3 convolutions with zero padding over the block grid,     */
+/* 16 input features, 32 channels, ReLU after the first five; weights and   */
+/* activations bfloat16, products accumulated in binary32 on MFMA, biases   */
+/* binary32.  Output channels 0-26 are the logits, 27 the field ratio.      */
+/* Tensors are channel-last: features [n][h][w][16] and activations         */
+/* [n][h][w][32] of bfloat16 bits (uint16), the head [n][h][w][32] binary32. */
+/* Parameters: layer by layer, weights [9][32][inputs] (tap = 3 ky + kx     */
+/* reads row y + ky - 1, column x + kx - 1) then 32 biases, binary32.       */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdProposal GJXLAmdProposal;
+
+enum {
+  GJXL_AMD_PROPOSAL_FEATURES = 16,
+  GJXL_AMD_PROPOSAL_CHANNELS = 32,
+  GJXL_AMD_PROPOSAL_LAYERS = 6,
+  GJXL_AMD_PROPOSAL_CLASSES = 27,
+  GJXL_AMD_PROPOSAL_FIELD_CHANNEL = 27,
+  GJXL_AMD_PROPOSAL_IGNORE = 255 /* label of a block outside the loss */
+};
+
+/* Buffers of gjxl_amd_proposal_buffer. */
+enum {
+  GJXL_AMD_PROPOSAL_HEAD = 0,       /* [positions][32] binary32, last forward */
+  GJXL_AMD_PROPOSAL_PARAMETERS = 1, /* binary32 master parameters */
+  GJXL_AMD_PROPOSAL_GRADIENTS = 2,  /* binary32, of the last backward (summed loss) */
+  GJXL_AMD_PROPOSAL_ACTIVATIONS = 3 /* five [positions][32] bfloat16 planes */
+};
+
+/* Number of binary32 parameters (50880). */
+GJXL_AMD_API size_t gjxl_amd_proposal_parameter_count(void);
+/* A network on `device` for up to max_positions block positions per
+ * forward (n * h * w); parameters zero until set. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_create(GJXLAmdDevice* device,
+                                                 size_t max_positions,
+                                                 GJXLAmdProposal** out);
+GJXL_AMD_API void gjxl_amd_proposal_destroy(GJXLAmdProposal* net);
+/* Host parameters in, out (count = gjxl_amd_proposal_parameter_count());
+ * setting them resets the optimizer state.  Synchronous. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_set_parameters(GJXLAmdProposal* net,
+                                                         const float* host,
+                                                         size_t count);
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_get_parameters(GJXLAmdProposal* net,
+                                                         float* host, size_t count);
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_buffer(GJXLAmdProposal* net, int32_t id,
+                                                 void** pointer, size_t* bytes);
+/* Features of every 8x8 block of a padded XYB image (3 planes of
+ * width x height binary32, multiples of 8) and its initial quant field
+ * ((w/8)*(h/8)): features [h/8][w/8][16] bfloat16 -- the block mean of
+ * each channel (X times 16), log1p of the orthonormal-DCT8 energy of each
+ * channel in the bands u + v = 1-2, 3-5, 6-14 times 1e4 (X) or 1e2 (Y,
+ * B), the log of the field, three zeros.  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_features(GJXLAmdDevice* device,
+                                                   const float* opsin, uint32_t width,
+                                                   uint32_t height,
+                                                   const float* quant_field,
+                                                   uint16_t* features);
+/* Forward over n frames of h x w blocks (device features); the head and
+ * the activations stay in the network for the backward, which reads the
+ * features again: they must stay valid until it has run.  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_forward(GJXLAmdProposal* net,
+                                                  const uint16_t* features,
+                                                  uint32_t n, uint32_t h, uint32_t w);
+/* Loss and gradients of the last forward: labels (strategy code or
+ * GJXL_AMD_PROPOSAL_IGNORE) and field targets per position (device).  The
+ * loss is the sum over positions of the softmax cross-entropy plus
+ * qf_weight times the squared error of channel 27.  sums (host, may be
+ * null) receives the cross-entropy sum, the squared-error sum and the
+ * count of positions whose largest logit is the label (waits for the
+ * device then). */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_backward(GJXLAmdProposal* net,
+                                                   const uint8_t* labels,
+                                                   const float* targets,
+                                                   float qf_weight, double* sums);
+/* One Adam step on the gradients of the last backward times grad_scale
+ * (step counts from 1 after set_parameters).  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_adam(GJXLAmdProposal* net, float learning_rate,
+                                               float beta1, float beta2, float epsilon,
+                                               float grad_scale);
+/* Allowed stages per block from a head of bh x bw positions: per block a
+ * softmax over the seven candidate strategies' logits; stage s is allowed
+ * if it is DCT8, among the top_k probabilities (ties to the lower stage) or
+ * at least p_min.  A 64x64 tile whose every block has a largest
+ * probability of at least threshold keeps those bits; other tiles allow
+ * every stage.  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_decide(GJXLAmdDevice* device, const float* head,
+                                                 uint32_t block_width, uint32_t block_height,
+                                                 float threshold, uint32_t top_k,
+                                                 float p_min, uint8_t* allow);
+/* The loop's starting field: adjusted * exp(clamp(head channel 27, +-limit))
+ * clamped to [lower, upper].  Enqueued. */
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
+                                                     const float* head,
+                                                     const float* adjusted, size_t count,
+                                                     float limit, float lower, float upper,
+                                                     float* field);
+
+/* The learned run of a plain path.  With a network attached, a run adds
+ * the features and a forward after the initial quant field, prunes the
+ * candidate costs by the decisions, starts the Butteraugli loop from the
+ * warm-started field and runs `evaluations` evaluations (1-3); while the
+ * last score exceeds judge_ratio * distance and fewer than three ran, it
+ * goes on with the plain loop's updates.  The frame reports the
+ * evaluations run and the allowed stages. */
+typedef struct {
+  size_t struct_size;
+  float threshold;    /* tile confidence, default 0.8 */
+  uint32_t top_k;     /* default 2 */
+  float p_min;        /* default 0.1 */
+  int32_t evaluations;/* default 1 */
+  float field_limit;  /* default 2 */
+  float judge_ratio;  /* default 1.1 */
+} GJXLAmdProposalOptions;
+
+GJXL_AMD_API GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options,
+                                                       size_t struct_size);
+/* Attaches (net non-null; it must outlive the attachment and hold
+ * max_positions >= the path's block count) or detaches (null) the
+ * network.  Not while a run is in progress. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* path,
+                                                         GJXLAmdProposal* net,
+                                                         const GJXLAmdProposalOptions* options);
+/* Training data from the last run of a plain path (device outputs of
+ * (w/8)*(h/8) entries): the features of its Gaborish-inverted image and
+ * initial field, labels = the strategy code of each block's transform,
+ * targets = log(final field / adjusted initial field).  Enqueued on the
+ * path's device stream. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_proposal_sample(GJXLAmdPlainPath* path,
+                                                            uint16_t* features,
+                                                            uint8_t* labels,
+                                                            float* targets);
 
 #ifdef __cplusplus
 }  /* extern "C" */
