@@ -98,7 +98,7 @@ _SIGNATURES = {
     "gjxl_amd_proposal_backward": [_P, _P, _P, _F, _P],
     "gjxl_amd_proposal_adam": [_P, _F, _F, _F, _F, _F],
     "gjxl_amd_proposal_decide": [_P, _P, _U32, _U32, _F, _U32, _F, _P],
-    "gjxl_amd_proposal_warm_start": [_P, _P, _P, _SZ, _F, _F, _F, _P],
+    "gjxl_amd_proposal_warm_start": [_P, _P, _P, _SZ, _F, _F, _F, _F, _P],
     "gjxl_amd_proposal_options_init": [_P, _SZ],
     "gjxl_amd_plain_path_set_proposal": [_P, _P, _P],
     "gjxl_amd_plain_path_proposal_sample": [_P, _P, _P, _P],
@@ -813,7 +813,7 @@ class PlainFrame(ctypes.Structure):
                 ("token_count", ctypes.c_uint64), ("x_qm_scale", ctypes.c_int32),
                 ("b_qm_scale", ctypes.c_int32), ("quant_dc", _F),
                 ("evaluations", ctypes.c_int32), ("scores", _F * 8),
-                ("entropy_inputs", _P), ("proposal_allow", _P)]
+                ("entropy_inputs", _P), ("proposal_allow", _P), ("proposal_norm", _F)]
 
 
 STAGE_NAMES = ("begin", "upload", "srgb_to_linear_and_opsin", "gaborish_inverse",

@@ -69,6 +69,7 @@ struct GJXLAmdPlainPath {
   GJXLAmdProposal* proposal = nullptr;
   GJXLAmdProposalOptions proposal_options{};
   int32_t evaluations_run = 0;
+  float proposal_norm = 0.0f;
   GJXLAmdAqSession* session = nullptr;
   GJXLAmdButteraugli* butteraugli = nullptr;
   GJXLAmdTokenizer* tokenizer = nullptr;
@@ -137,7 +138,7 @@ void PathGeometry(GJXLAmdPlainPath* p, uint32_t width, uint32_t height, size_t* 
   z[kYToB] = tiles;
   z[kCosts] = 4 * 7 * blocks;
   z[kCells] = blocks;
-  z[kScoreWords] = 4 * kMaxEvaluations;
+  z[kScoreWords] = 4 * (kMaxEvaluations + 1);  // scores, then the judged norm
   z[kFeatures] = 2 * GJXL_AMD_PROPOSAL_FEATURES * blocks;
   z[kAllow] = blocks;
 }
@@ -268,8 +269,8 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   if (head != nullptr) {
     float lower = 0.0f, upper = 0.0f;
     GJXL_AMD_TRY(gjxl_amd_aq_policy(p->session, &lower, &upper, nullptr, nullptr, nullptr));
-    GJXL_AMD_TRY(gjxl_amd::LaunchProposalWarmStart(d, head, adjusted, n, po.field_limit, lower,
-                                                   upper, field));
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalWarmStart(d, head, adjusted, n, po.field_limit,
+                                                   po.field_offset, lower, upper, field));
     planned = po.evaluations - 1;
   } else {
     GJXL_AMD_TRY(gjxl_amd_copy(d, field, adjusted, 4 * n));
@@ -286,15 +287,16 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
     GJXL_AMD_TRY(gjxl_amd::LaunchReduceDistanceMap(st, p->butteraugli->map, p->width, p->height,
                                                    p->cells, p->bw, p->bh, distance));
     bool more = it < planned;
-    if (!more && head != nullptr && it < p->iterations) {
-      // the judge: the learned run goes on with the plain updates while its
-      // score is above judge_ratio * distance
-      GJXL_AMD_TRY(Check(hipMemcpyAsync(p->host_words + 8 + it, p->score_words + it, 4,
-                                        hipMemcpyDeviceToHost, st), "hipMemcpyAsync(score)"));
+    if (!more && head != nullptr) {
+      // the judge: the 3-norm of the block distances; the learned run goes
+      // on with the plain updates while it is above judge_ratio * distance
+      float* norm = reinterpret_cast<float*>(p->score_words + kMaxEvaluations);
+      GJXL_AMD_TRY(gjxl_amd::LaunchProposalNorm3(d, distance, n, norm));
+      GJXL_AMD_TRY(Check(hipMemcpyAsync(p->host_words + 24, norm, 4, hipMemcpyDeviceToHost, st),
+                         "hipMemcpyAsync(norm)"));
       GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
-      float score = 0.0f;
-      std::memcpy(&score, p->host_words + 8 + it, 4);
-      more = !(score <= po.judge_ratio * t);
+      std::memcpy(&p->proposal_norm, p->host_words + 24, 4);
+      more = it < p->iterations && !(p->proposal_norm <= po.judge_ratio * t);
     }
     if (more) GJXL_AMD_TRY(gjxl_amd_aq_update(p->session, field, distance, it, t));
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_REDUCE_UPDATE));
@@ -401,6 +403,7 @@ GJXLResult FillFrame(GJXLAmdPlainPath* p, GJXLAmdPlainFrame* f) {
   r.quant_dc = qdc;
   r.evaluations = p->evaluations_run;
   r.proposal_allow = p->proposal != nullptr ? p->allow : nullptr;
+  r.proposal_norm = p->proposal != nullptr ? p->proposal_norm : 0.0f;
   for (int i = 0; i < r.evaluations; ++i) r.scores[i] = p->scores[i];
   r.entropy_inputs = p->entropy_inputs;
   *f = r;
@@ -615,7 +618,8 @@ GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* p, GJXLAmdProposal
   }
   if (!(o.threshold >= 0.0f && o.threshold <= 1.0f) || o.top_k > 7 ||
       !std::isfinite(o.p_min) || o.evaluations < 1 || o.evaluations > p->iterations + 1 ||
-      !(o.field_limit >= 0.0f && o.field_limit <= 16.0f) || !(o.judge_ratio > 0.0f) ||
+      !(o.field_limit >= 0.0f && o.field_limit <= 16.0f) || !std::isfinite(o.field_offset) ||
+      !(o.judge_ratio > 0.0f) ||
       !std::isfinite(o.judge_ratio)) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal options out of range");
   }

@@ -11,8 +11,11 @@ GJXLResult LaunchProposalFeatures(GJXLAmdDevice* device, const float* opsin, int
 GJXLResult LaunchProposalDecide(GJXLAmdDevice* device, const float* head, int bw, int bh,
                                 float threshold, int top_k, float p_min, uint8_t* allow);
 GJXLResult LaunchProposalWarmStart(GJXLAmdDevice* device, const float* head,
-                                   const float* adjusted, size_t n, float limit, float lower,
-                                   float upper, float* field);
+                                   const float* adjusted, size_t n, float limit, float offset,
+                                   float lower, float upper, float* field);
+// (mean of d^3)^(1/3) of n block distances into *out (device).
+GJXLResult LaunchProposalNorm3(GJXLAmdDevice* device, const float* distances, size_t n,
+                               float* out);
 GJXLResult LaunchProposalSample(GJXLAmdDevice* device, const uint8_t* cells, const float* field,
                                 const float* adjusted, size_t n, uint8_t* labels,
                                 float* targets);

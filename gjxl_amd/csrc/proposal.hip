@@ -18,9 +18,10 @@
 //                      output gradient in bfloat16
 //   AdamKernel         Adam on the binary32 master parameters, repacking
 //                      the bfloat16 forward and gradient weights
-//   DecideKernel       allowed stages per block, one wave per 64x64 tile
+//   DecideKernel       allowed stages per block
 //   WarmStartKernel    the loop's starting field
 //   SampleKernel       labels and targets of a plain run
+//   Norm3Kernel        the 3-norm of the block distances (the judge)
 //
 // MFMA operands (16x16x16, 64 lanes): lane l holds A[l % 16][4 (l / 16) + i],
 // B[4 (l / 16) + i][l % 16] and D[4 (l / 16) + i][l % 16], i = 0..3.  In the
@@ -520,18 +521,18 @@ __global__ __launch_bounds__(64) void DecideKernel(const float* __restrict__ hea
     }
     if (rank < top_k || p[s] >= p_min) bits |= 1u << s;
   }
-  const bool unsure = inside && conf < threshold;
-  if (__ballot(unsure) != 0) bits = 0x7f;
+  if (conf < threshold) bits = 0x7f;  // an unsure block keeps the plain search
   if (inside) allow[at] = static_cast<uint8_t>(bits);
 }
 
 __global__ __launch_bounds__(256) void WarmStartKernel(const float* __restrict__ head,
                                                        const float* __restrict__ adjusted,
-                                                       size_t n, float limit, float lower,
-                                                       float upper, float* __restrict__ field) {
+                                                       size_t n, float limit, float offset,
+                                                       float lower, float upper,
+                                                       float* __restrict__ field) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float r = head[i * kChannels + kFieldChannel];
+  float r = head[i * kChannels + kFieldChannel] + offset;
   r = r < -limit ? -limit : (limit < r ? limit : r);
   const float f = adjusted[i] * expf(r);
   field[i] = f < lower ? lower : (upper < f ? upper : f);
@@ -548,7 +549,35 @@ __global__ __launch_bounds__(256) void SampleKernel(const uint8_t* __restrict__ 
   targets[i] = logf(field[i] / adjusted[i]);
 }
 
+// (mean of d^3)^(1/3) over n block distances, one work group: each thread
+// sums a fixed stride of the blocks in binary64, then a fixed tree.
+constexpr int kNormThreads = 1024;
+
+__global__ __launch_bounds__(kNormThreads) void Norm3Kernel(const float* __restrict__ d,
+                                                            size_t n, float* __restrict__ out) {
+  __shared__ double part[kNormThreads];
+  double s = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += kNormThreads) {
+    const double v = d[i];
+    s += v * v * v;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int half = kNormThreads / 2; half > 0; half /= 2) {
+    if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = static_cast<float>(cbrt(part[0] / static_cast<double>(n)));
+}
+
 }  // namespace
+
+GJXLResult LaunchProposalNorm3(GJXLAmdDevice* device, const float* distances, size_t n,
+                               float* out) {
+  hipLaunchKernelGGL(Norm3Kernel, dim3(1), dim3(kNormThreads), 0, device->stream, distances, n,
+                     out);
+  return CheckLaunch("Norm3Kernel");
+}
 
 GJXLResult LaunchProposalFeatures(GJXLAmdDevice* device, const float* opsin, int width,
                                   int height, const float* quant_field, uint16_t* features) {
@@ -568,10 +597,10 @@ GJXLResult LaunchProposalDecide(GJXLAmdDevice* device, const float* head, int bw
 }
 
 GJXLResult LaunchProposalWarmStart(GJXLAmdDevice* device, const float* head,
-                                   const float* adjusted, size_t n, float limit, float lower,
-                                   float upper, float* field) {
+                                   const float* adjusted, size_t n, float limit, float offset,
+                                   float lower, float upper, float* field) {
   hipLaunchKernelGGL(WarmStartKernel, dim3(CeilDiv(static_cast<uint32_t>(n), 256)), dim3(256),
-                     0, device->stream, head, adjusted, n, limit, lower, upper, field);
+                     0, device->stream, head, adjusted, n, limit, offset, lower, upper, field);
   return CheckLaunch("WarmStartKernel");
 }
 
@@ -926,14 +955,16 @@ GJXLResult gjxl_amd_proposal_decide(GJXLAmdDevice* device, const float* head,
 
 GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device, const float* head,
                                         const float* adjusted, size_t count, float limit,
-                                        float lower, float upper, float* field) {
+                                        float offset, float lower, float upper, float* field) {
   using namespace gjxl_amd;
   GJXL_AMD_TRY(ValidateDevice(device));
   if (head == nullptr || adjusted == nullptr || field == nullptr || count == 0 ||
-      count > (size_t{1} << 31) || !(limit >= 0.0f) || !(lower > 0.0f) || !(upper >= lower)) {
+      count > (size_t{1} << 31) || !(limit >= 0.0f) || !std::isfinite(offset) ||
+      !(lower > 0.0f) || !(upper >= lower)) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal warm start: buffers or bounds");
   }
-  return LaunchProposalWarmStart(device, head, adjusted, count, limit, lower, upper, field);
+  return LaunchProposalWarmStart(device, head, adjusted, count, limit, offset, lower, upper,
+                                 field);
 }
 
 GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options, size_t struct_size) {
@@ -942,12 +973,13 @@ GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options, size_
   }
   std::memset(options, 0, sizeof(*options));
   options->struct_size = sizeof(*options);
-  options->threshold = 0.8f;
+  options->threshold = 0.5f;
   options->top_k = 2;
   options->p_min = 0.1f;
-  options->evaluations = 1;
+  options->evaluations = 2;
   options->field_limit = 2.0f;
-  options->judge_ratio = 1.1f;
+  options->field_offset = 0.0f;
+  options->judge_ratio = 0.7f;
   return GJXL_OK;
 }
 

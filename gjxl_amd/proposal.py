@@ -51,7 +51,8 @@ _HEAD, _PARAMETERS, _GRADIENTS, _ACTIVATIONS = 0, 1, 2, 3
 
 class ProposalOptions(ctypes.Structure):
     _fields_ = [("struct_size", _SZ), ("threshold", _F), ("top_k", _U32), ("p_min", _F),
-                ("evaluations", ctypes.c_int32), ("field_limit", _F), ("judge_ratio", _F)]
+                ("evaluations", ctypes.c_int32), ("field_limit", _F), ("field_offset", _F),
+                ("judge_ratio", _F)]
 
     @classmethod
     def defaults(cls) -> "ProposalOptions":
@@ -200,10 +201,11 @@ def decide(device, head: DeviceArray, bw: int, bh: int, threshold: float, top_k:
 
 
 def warm_start(device, head: DeviceArray, adjusted: DeviceArray, count: int, limit: float,
-               lower: float, upper: float, out: DeviceArray | None = None) -> DeviceArray:
+               lower: float, upper: float, offset: float = 0.0,
+               out: DeviceArray | None = None) -> DeviceArray:
     out = out or device.empty((count,), np.float32)
     _check(library().gjxl_amd_proposal_warm_start(device.handle, head.ptr, adjusted.ptr,
-                                                  count, limit, lower, upper, out.ptr))
+                                                  count, limit, offset, lower, upper, out.ptr))
     return out
 
 

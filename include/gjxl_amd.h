@@ -648,8 +648,10 @@ typedef struct {
   float scores[8];                  /* Butteraugli score per evaluation */
   /* host: the coefficient orders and block-context map of the tokens */
   const GJXLAmdEntropyInputs* entropy_inputs;
-  /* the learned run's allowed stages per block (null in a plain run) */
+  /* the learned run's allowed stages per block (null in a plain run) and
+   * the 3-norm of its last evaluation's block distances (0 in a plain run) */
   const uint8_t* proposal_allow;
+  float proposal_norm;
 } GJXLAmdPlainFrame;
 
 /* Stage ids of gjxl_amd_plain_path_stage_times. */
@@ -860,36 +862,39 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_adam(GJXLAmdProposal* net, float learn
 /* Allowed stages per block from a head of bh x bw positions: per block a
  * softmax over the seven candidate strategies' logits; stage s is allowed
  * if it is DCT8, among the top_k probabilities (ties to the lower stage) or
- * at least p_min.  A 64x64 tile whose every block has a largest
- * probability of at least threshold keeps those bits; other tiles allow
- * every stage.  Enqueued. */
+ * at least p_min.  A block whose largest probability is below threshold
+ * allows every stage (the plain search).  Enqueued. */
 GJXL_AMD_API GJXLResult gjxl_amd_proposal_decide(GJXLAmdDevice* device, const float* head,
                                                  uint32_t block_width, uint32_t block_height,
                                                  float threshold, uint32_t top_k,
                                                  float p_min, uint8_t* allow);
-/* The loop's starting field: adjusted * exp(clamp(head channel 27, +-limit))
- * clamped to [lower, upper].  Enqueued. */
+/* The loop's starting field: adjusted * exp(clamp(head channel 27 +
+ * offset, +-limit)) clamped to [lower, upper].  Enqueued. */
 GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
                                                      const float* head,
                                                      const float* adjusted, size_t count,
-                                                     float limit, float lower, float upper,
-                                                     float* field);
+                                                     float limit, float offset, float lower,
+                                                     float upper, float* field);
 
 /* The learned run of a plain path.  With a network attached, a run adds
  * the features and a forward after the initial quant field, prunes the
  * candidate costs by the decisions, starts the Butteraugli loop from the
- * warm-started field and runs `evaluations` evaluations (1-3); while the
- * last score exceeds judge_ratio * distance and fewer than three ran, it
- * goes on with the plain loop's updates.  The frame reports the
- * evaluations run and the allowed stages. */
+ * warm-started field and runs `evaluations` evaluations (1-3).  The judge
+ * is the 3-norm of the last evaluation's block distances, (mean of d^3)^
+ * (1/3): while it exceeds judge_ratio * distance and fewer than three
+ * evaluations ran, the run goes on with the plain loop's updates (plain
+ * runs of the synthetic workloads end at 0.60-0.64 times the distance
+ * between distances 0.9 and 1.2).  The frame reports the evaluations run,
+ * the allowed stages and the judged norm. */
 typedef struct {
   size_t struct_size;
-  float threshold;    /* tile confidence, default 0.8 */
-  uint32_t top_k;     /* default 2 */
-  float p_min;        /* default 0.1 */
-  int32_t evaluations;/* default 1 */
-  float field_limit;  /* default 2 */
-  float judge_ratio;  /* default 1.1 */
+  float threshold;     /* block confidence, default 0.5 */
+  uint32_t top_k;      /* default 2 */
+  float p_min;         /* default 0.1 */
+  int32_t evaluations; /* default 2 */
+  float field_limit;   /* default 2 */
+  float field_offset;  /* log offset of the predicted field, default 0 */
+  float judge_ratio;   /* default 0.7 */
 } GJXLAmdProposalOptions;
 
 GJXL_AMD_API GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options,

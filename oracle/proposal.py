@@ -257,9 +257,8 @@ def decide(head: np.ndarray, threshold: float, top_k: int, p_min: float) -> np.n
 
     Per block: softmax over the seven candidate strategies' logits; stage s
     is allowed if s is DCT8, among the top_k probabilities (ties to the lower
-    stage) or at least p_min.  A 64x64 tile is confident when every block's
-    largest probability is at least threshold; blocks of other tiles allow
-    every stage (the plain search)."""
+    stage) or at least p_min.  A block whose largest probability is below
+    threshold allows every stage (the plain search)."""
     bh, bw, _ = head.shape
     z = head[..., list(STRATEGIES)].astype(np.float64)
     z = z - z.max(axis=-1, keepdims=True)
@@ -277,18 +276,14 @@ def decide(head: np.ndarray, threshold: float, top_k: int, p_min: float) -> np.n
     allow = np.zeros((bh, bw), np.uint8)
     for s in range(7):
         allow |= bits[..., s] << s
-    conf = p.max(axis=-1)
-    for ty in range(0, bh, 8):
-        for tx in range(0, bw, 8):
-            if conf[ty:ty + 8, tx:tx + 8].min() < threshold:
-                allow[ty:ty + 8, tx:tx + 8] = 0x7F
+    allow[p.max(axis=-1) < threshold] = 0x7F
     return allow
 
 
 def warm_start(head: np.ndarray, adjusted: np.ndarray, lower: float, upper: float,
-               limit: float = 2.0) -> np.ndarray:
-    """The loop's starting field: adjusted * exp(clamp(head[..., 27])),
-    clamped to the policy bounds (binary32)."""
-    r = np.clip(head[..., QF_CHANNEL].astype(np.float32), -limit, limit)
+               limit: float = 2.0, offset: float = 0.0) -> np.ndarray:
+    """The loop's starting field: adjusted * exp(clamp(head[..., 27] +
+    offset, +-limit)), clamped to the policy bounds (binary32)."""
+    r = np.clip(head[..., QF_CHANNEL].astype(np.float32) + np.float32(offset), -limit, limit)
     f = adjusted.astype(np.float32) * np.exp(r).astype(np.float32)
     return np.clip(f, np.float32(lower), np.float32(upper)).astype(np.float32)

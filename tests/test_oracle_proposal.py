@@ -117,17 +117,21 @@ def test_features_of_a_basis_pattern():
     assert np.all(f[..., 13:] == 0)
 
 
-def test_decide_tiles_and_ranks():
+def test_decide_blocks_and_ranks():
     head = np.full((16, 8, P.OUT), -5.0, np.float32)
-    # tile 0 (rows 0-7): confident DCT16x16 everywhere, DCT8 second
+    # rows 0-7: confident DCT16x16, DCT8 second
     head[:8, :, 4] = 6.0
     head[:8, :, 0] = 2.0
-    # tile 1 (rows 8-15): one block is unsure
+    # rows 8-15: confident DCT32x32 but one unsure block
     head[8:, :, 5] = 6.0
     head[12, 3, 5] = -5.0
     allow = P.decide(head, threshold=0.9, top_k=2, p_min=0.2)
     assert np.all(allow[:8] == (1 << 0) | (1 << 3))
-    assert np.all(allow[8:] == 0x7F)
+    unsure = np.zeros((8, 8), bool)
+    unsure[4, 3] = True
+    assert np.all(allow[8:][unsure] == 0x7F)
+    # DCT32x32 (stage 6) and the runner-up, where the tie ranks stage 0 first
+    assert np.all(allow[8:][~unsure] == (1 << 0) | (1 << 6))
     # equal probabilities (1/7 < p_min) rank to the lower stage: top 2 = stages 0, 1
     head2 = np.full((8, 8, P.OUT), 0.0, np.float32)
     allow2 = P.decide(head2, threshold=0.0, top_k=2, p_min=0.5)
@@ -141,3 +145,5 @@ def test_warm_start_bounds():
     f = P.warm_start(head, adj, 0.5, 4.0)
     assert f[0, 0] == 1.0 and f[0, 1] == 4.0 and f[1, 0] == np.float32(0.5)
     assert np.isclose(f[1, 1], 2.0, rtol=1e-6)
+    g = P.warm_start(head, adj, 0.1, 4.0, offset=np.log(1.5))
+    assert np.isclose(g[0, 0], 1.5, rtol=1e-6) and np.isclose(g[1, 1], 3.0, rtol=1e-6)
