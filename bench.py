@@ -434,6 +434,96 @@ def split_dct_figures(gjxl_amd, dev, srgb: np.ndarray, distances) -> dict:
     return out
 
 
+def _token_bits(tokens: dict) -> float:
+    """Entropy of the token stream given its contexts (bits): the size of an
+    ideal coder with static per-context statistics, a proxy for the ANS
+    stage, which this library does not run."""
+    v = tokens["values"].astype(np.uint64)
+    c = tokens["contexts"].astype(np.uint64)
+    _, cnt = np.unique((c << np.uint64(32)) | v, return_counts=True)
+    _, ccnt = np.unique(c, return_counts=True)
+    return float(-(cnt * np.log2(cnt)).sum() + (ccnt * np.log2(ccnt)).sum())
+
+
+def _rate_quality(path) -> tuple[float, float]:
+    """(3-norm of the last evaluation's block distances, token bits)."""
+    d = path.view("block_distance").numpy().astype(np.float64)
+    return float(np.mean(d ** 3) ** (1.0 / 3.0)), _token_bits(path.tokens())
+
+
+def learned_figures(gjxl_amd, dev, srgb: np.ndarray, steps: int) -> dict:
+    """The learned proposal (chosen form) against the plain path on the
+    bench image: MP/s with the input resident in HBM and the tokens read
+    back, the rate excess at equal quality against the plain path's own
+    rate-quality curve (distances 0.9-1.2; quality = 3-norm of the block
+    distances, rate = token entropy), with the trained network and with none
+    (zero parameters), grid agreement and the share of blocks left to the
+    plain search."""
+    from gjxl_amd import proposal as prop
+
+    h, w = srgb.shape[:2]
+    mp = w * h / 1e6
+    path = gjxl_amd.NativePlainPath(dev, w, h)
+    resident = dev.array(srgb)
+
+    def timed():
+        path.run(resident=resident)
+        dev.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            path.run(resident=resident)
+        return (time.perf_counter() - t0) / steps
+
+    curve = []
+    for dist in (0.9, 1.0, 1.1, 1.2):
+        path.reconfigure(w, h, distance=dist)
+        path.run(resident=resident)
+        curve.append(_rate_quality(path))
+    path.reconfigure(w, h, distance=1.0)
+    plain_s = timed()
+    plain_cells = path.view("cells").numpy()
+    qs = np.array([c[0] for c in curve])
+    lb = np.log(np.array([c[1] for c in curve]))
+
+    def excess(q, b):
+        return round(float(b / np.exp(np.interp(q, qs, lb)) - 1.0), 5)
+
+    out = {"parameters": "gjxl_amd/proposal_parameters.npz",
+           "training": {k: v for k, v in prop.load_metadata().items() if k != "history"},
+           "plain": {"value": round(mp / plain_s, 3), "ms_per_image": round(plain_s * 1e3, 3),
+                     "quality_3norm": round(curve[1][0], 5), "bits": round(curve[1][1])},
+           "runs": []}
+    trained = prop.load_parameters()
+    for name, params in (("trained", trained), ("none", np.zeros_like(trained))):
+        net = prop.ProposalNet(dev, path.bw * path.bh, params)
+        for opts in ({"evaluations": 1, "judge_ratio": 100.0},
+                     {"evaluations": 2, "judge_ratio": 100.0}, {}):
+            path.set_proposal(net, **opts)
+            seconds = timed()
+            q, b = _rate_quality(path)
+            cells = path.view("cells").numpy()
+            allow = path.view("proposal_allow").numpy()
+            stages = {}
+            for stage, ms in path.stage_times():
+                stages[stage] = stages.get(stage, 0.0) + ms
+            out["runs"].append({
+                "network": name, "options": opts or "defaults",
+                "evaluations": int(path.frame.evaluations),
+                "value": round(mp / seconds, 3), "ms_per_image": round(seconds * 1e3, 3),
+                "speedup_vs_plain": round(plain_s / seconds, 3),
+                "quality_3norm": round(q, 5), "rate_excess_at_equal_quality": excess(q, b),
+                "grid_agreement": round(float(((cells >> 1) == (plain_cells >> 1)).mean()), 4),
+                "plain_search_blocks": round(float((allow == 0x7F).mean()), 4),
+                "stage_ms": {k: round(v, 3) for k, v in stages.items()
+                             if k in ("proposal", "ac_strategy_costs", "aq_evaluate",
+                                      "butteraugli_compare")}})
+        path.set_proposal(None)
+        net.close()
+    path.close()
+    resident.free()
+    return out
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -454,6 +544,8 @@ def main() -> int:
     parser.add_argument("--batch-lanes", type=int, default=3)
     parser.add_argument("--split-distances", default="0.5,1,2,3",
                         help="distances of the split-precision DCT figures ('' to skip)")
+    parser.add_argument("--learned-steps", type=int, default=20,
+                        help="timed steps per learned-proposal configuration (0: skip)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--calibration-copy", action="store_true",
                         help="run a device copy of known size first (counter runs)")
@@ -576,6 +668,11 @@ def main() -> int:
         split = split_dct_figures(gjxl_amd, dev, srgb,
                                   [float(v) for v in args.split_distances.split(",")])
 
+    learned = None
+    if args.learned_steps > 0:
+        print("bench: learned proposal", file=sys.stderr, flush=True)
+        learned = learned_figures(gjxl_amd, dev, srgb, args.learned_steps)
+
     per_step = {k: float(np.sum(v)) / args.steps for k, v in stage_ms.items()}
     kind, units = probe_units(probe, w, h)
     launch_ms = float(np.mean(launches))
@@ -627,6 +724,7 @@ def main() -> int:
             "concurrent_images": concurrent,
             "batch_driver": batch,
             "split_precision_dct": split,
+            "learned_proposal": learned,
         },
         "roofline": roofline,
     }
