@@ -86,6 +86,11 @@ _SIGNATURES = {
     "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
     "gjxl_amd_ac_strategy_costs_pruned":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P, _P],
+    "gjxl_amd_cost_queue_create": [_P, _U32, ctypes.POINTER(_P)],
+    "gjxl_amd_cost_queue_destroy": [_P],
+    "gjxl_amd_cost_queue_run": [_P, _P, _U32],
+    "gjxl_amd_cost_queue_tune": [_P, _P, _U32, _U32],
+    "gjxl_amd_cost_queue_grids": [_P, _P, _P],
     "gjxl_amd_proposal_parameter_count": [],
     "gjxl_amd_proposal_create": [_P, _SZ, ctypes.POINTER(_P)],
     "gjxl_amd_proposal_destroy": [_P],
@@ -157,7 +162,8 @@ _VOID = {"gjxl_amd_device_destroy", "gjxl_amd_aq_session_destroy",
          "gjxl_amd_butteraugli_destroy", "gjxl_amd_tokenizer_destroy",
          "gjxl_amd_plain_path_destroy", "gjxl_amd_host_free",
          "gjxl_amd_batch_results_free", "gjxl_amd_batch_shutdown",
-         "gjxl_amd_batch_destroy", "gjxl_amd_proposal_destroy"}
+         "gjxl_amd_batch_destroy", "gjxl_amd_proposal_destroy",
+         "gjxl_amd_cost_queue_destroy"}
 _STR = {"gjxl_amd_last_error"}
 _SIZE = {"gjxl_amd_initial_quant_field_scratch_bytes", "gjxl_amd_proposal_parameter_count"}
 
@@ -784,6 +790,69 @@ def gpu_available() -> bool:
 
 
 from . import pipeline  # noqa: E402,F401  (stage sequencing over this API)
+
+
+# ---- the tile queue for batches of candidate costs (gjxl_amd_cost_queue_*) --------
+
+class CostFrame(ctypes.Structure):
+    _fields_ = [("opsin", _P), ("quant_field", _P), ("pixel_mask", _P), ("y_to_x", _P),
+                ("y_to_b", _P), ("allow", _P), ("costs", _P), ("width", _U32),
+                ("height", _U32), ("butteraugli_target", _F)]
+
+
+class CostQueue:
+    """Candidate costs of batches of frames through one captured graph of
+    persistent tile-queue kernels (gjxl_amd_cost_queue_*).  A frame is a dict
+    of DeviceArrays: opsin (Gaborish-inverted, 3 x h x w), quant_field,
+    pixel_mask, y_to_x, y_to_b, costs (7 x bh x bw), optionally allow, and
+    the target."""
+
+    def __init__(self, device: Device, max_frames: int):
+        self.device = device
+        handle = ctypes.c_void_p()
+        _check(library().gjxl_amd_cost_queue_create(device.handle, max_frames,
+                                                    ctypes.byref(handle)))
+        self.handle = handle.value
+        device._children.add(self)
+
+    def close(self) -> None:
+        if self.handle:
+            library().gjxl_amd_cost_queue_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _table(frames) -> ctypes.Array:
+        table = (CostFrame * len(frames))()
+        for i, f in enumerate(frames):
+            _, h, w = f["opsin"].shape
+            allow = f.get("allow")
+            table[i] = CostFrame(f["opsin"].ptr, f["quant_field"].ptr, f["pixel_mask"].ptr,
+                                 f["y_to_x"].ptr, f["y_to_b"].ptr,
+                                 allow.ptr if allow is not None else None, f["costs"].ptr,
+                                 w, h, f.get("target", 1.0))
+        return table
+
+    def run(self, frames) -> None:
+        table = self._table(frames)
+        _check(library().gjxl_amd_cost_queue_run(self.handle, table, len(frames)))
+        self._last = (table, frames)  # kept alive until the next call
+
+    def tune(self, frames, repetitions: int = 3) -> None:
+        table = self._table(frames)
+        _check(library().gjxl_amd_cost_queue_tune(self.handle, table, len(frames),
+                                                  repetitions))
+
+    def grids(self) -> tuple[list[int], list[float]]:
+        g = (ctypes.c_uint32 * 7)()
+        ms = (ctypes.c_float * 7)()
+        _check(library().gjxl_amd_cost_queue_grids(self.handle, g, ms))
+        return list(g), list(ms)
 
 
 # ---- the whole plain path (gjxl_amd_plain_path_*) ------------------------------

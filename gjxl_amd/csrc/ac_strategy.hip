@@ -12,7 +12,9 @@
 // CandidateCostKernel: one work group evaluates kCands candidates of one
 // strategy (3072 coefficient slots: 1x32x32, 2x32x16, 4x16x16, 8x16x8 or
 // 16x8x8), three channels each, entirely in LDS.
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <type_traits>
 
 #include "common.h"
@@ -326,18 +328,18 @@ struct CostShape {
   static constexpr int kPositions = kNx * kNy;
 };
 
-// One work group: kCands anchor positions of one 64x64 tile (grid y, after
-// the XCD-aware order), positions x * kCands + k of the tile's raster list
-// (grid x).  Only
-// positions whose footprint lies inside the tile are launched; the rest of
+// One work group's work: kCands anchor positions of one 64x64 tile
+// (group.y), positions group.x * kCands + k of the tile's raster list.  Only
+// positions whose footprint lies inside the tile are evaluated; the rest of
 // the cost plane is NaN (filled beforehand).
 template <int kStrategy>
-__global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
-  const float* __restrict__ opsin, int width, int height,
+__device__ __forceinline__ void CostWork(
+  const TileIndex group, const float* __restrict__ opsin, int width, int height,
   const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
   const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
   const float* __restrict__ tables, const double* __restrict__ bases,
-  CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs) {
+  CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs,
+  const int tid) {
   using S = CostShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
   constexpr int KC = 3 * K;     // candidate-channel planes
@@ -369,15 +371,11 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   __shared__ int valid[K];
   __shared__ int bxs[K], bys[K];
 
-  const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int kWaves = kCostThreads / 64;
   const int bw = width / 8, bh = height / 8;
   const int tiles_w = (bw + 7) / 8;
-  // the groups of one 64x64 tile (consecutive in dispatch order) share its
-  // pixels: the XCD-aware order keeps them on one L2
-  const TileIndex group = XcdTile();
   const int tx = group.y % tiles_w, ty = group.y / tiles_w;
   const int tbw = min(8, bw - tx * 8), tbh = min(8, bh - ty * 8);
   const size_t plane = static_cast<size_t>(width) * height;
@@ -702,6 +700,126 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   }
 }
 
+// One work group per (chunk, tile); the groups of one 64x64 tile
+// (consecutive in dispatch order) share its pixels, and the XCD-aware order
+// keeps them on one L2.
+template <int kStrategy>
+__global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
+  const float* __restrict__ opsin, int width, int height,
+  const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
+  const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
+  const float* __restrict__ tables, const double* __restrict__ bases,
+  CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs) {
+  CostWork<kStrategy>(XcdTile(), opsin, width, height, quant_field, pixel_mask, y_to_x, y_to_b,
+                      tables, bases, params, allow, costs, static_cast<int>(threadIdx.x));
+}
+
+// ---- the tile queue: candidate costs of a batch of frames -----------------
+//
+// Persistent work groups pull work items -- (frame, tile, chunk of anchor
+// positions) of one strategy -- from a counter in device memory until the
+// batch's items run out.  Frames are described in a table in device memory
+// (so one captured graph serves every batch); a plan kernel derives each
+// strategy's item offsets per frame and clears the counters.
+
+constexpr int kQueueMaxFrames = 256;
+
+struct QueueFrame {
+  const float* opsin;
+  const float* quant_field;
+  const float* pixel_mask;
+  const int8_t* y_to_x;
+  const int8_t* y_to_b;
+  const uint8_t* allow;
+  float* costs;
+  int32_t width, height;
+  CostParams params;  // entropy_multiplier set per strategy
+};
+
+struct QueueTable {
+  uint32_t count;
+  uint32_t counters[7];
+  uint32_t offsets[7][kQueueMaxFrames + 1];
+  QueueFrame frames[kQueueMaxFrames];
+};
+
+template <int kStrategy>
+constexpr uint32_t QueueChunks() {
+  return static_cast<uint32_t>((CostShape<kStrategy>::kPositions + CostShape<kStrategy>::kCands - 1) /
+                               CostShape<kStrategy>::kCands);
+}
+
+__constant__ uint32_t kQueueStageChunks[7] = {
+  QueueChunks<kDct8>(), QueueChunks<kDct16x8>(), QueueChunks<kDct8x16>(),
+  QueueChunks<kDct16x16>(), QueueChunks<kDct32x16>(), QueueChunks<kDct16x32>(),
+  QueueChunks<kDct32x32>()};
+
+// items per strategy: chunks x 64x64 tiles of each frame; counters cleared
+__global__ __launch_bounds__(64) void QueuePlanKernel(QueueTable* __restrict__ t) {
+  const int s = threadIdx.x;
+  if (s >= 7) return;
+  const uint32_t count = t->count;
+  uint32_t at = 0;
+  for (uint32_t f = 0; f < count; ++f) {
+    t->offsets[s][f] = at;
+    const int bw = t->frames[f].width / 8, bh = t->frames[f].height / 8;
+    at += kQueueStageChunks[s] * static_cast<uint32_t>(((bw + 7) / 8) * ((bh + 7) / 8));
+  }
+  t->offsets[s][count] = at;
+  t->counters[s] = 0;
+}
+
+// NaN over every frame's seven cost planes
+__global__ __launch_bounds__(256) void QueueFillKernel(const QueueTable* __restrict__ t) {
+  const uint32_t count = t->count;
+  const size_t step = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (uint32_t f = 0; f < count; ++f) {
+    const QueueFrame& q = t->frames[f];
+    const size_t n = 7 * static_cast<size_t>(q.width / 8) * (q.height / 8);
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+      q.costs[i] = __builtin_nanf("");
+    }
+  }
+}
+
+template <int kStrategy>
+__global__ __launch_bounds__(kCostThreads) void CostQueueKernel(
+  QueueTable* __restrict__ t, const float* __restrict__ tables,
+  const double* __restrict__ bases, float entropy_multiplier) {
+  constexpr int kStage = StageOf(kStrategy);
+  constexpr uint32_t kChunks = QueueChunks<kStrategy>();
+  __shared__ uint32_t next;
+  const uint32_t count = t->count;
+  const uint32_t total = t->offsets[kStage][count];
+  for (;;) {
+    __syncthreads();  // every lane has read `next` and the last item's LDS
+    if (threadIdx.x == 0) next = atomicAdd(&t->counters[kStage], 1u);
+    __syncthreads();
+    // uniform: the frame's fields become scalar loads
+    const uint32_t item = __builtin_amdgcn_readfirstlane(next);
+    if (item >= total) break;  // the counter only grows: every group ends here
+    // the frame: the last whose first item is at most `item`
+    int lo = 0, hi = static_cast<int>(count) - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (t->offsets[kStage][mid] <= item) lo = mid; else hi = mid - 1;
+    }
+    const QueueFrame& f = t->frames[lo];
+    const uint32_t local = item - t->offsets[kStage][lo];
+    const TileIndex g{static_cast<int>(local % kChunks), static_cast<int>(local / kChunks)};
+    CostParams params = f.params;
+    params.entropy_multiplier = entropy_multiplier;
+    const size_t stage = static_cast<size_t>(f.width / 8) * (f.height / 8);
+    // the thread index through a register move the compiler cannot hoist:
+    // the index arithmetic derived from it is redone per item instead of
+    // staying live across the whole loop (which took 160+ registers)
+    int tid;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(static_cast<int>(threadIdx.x)));
+    CostWork<kStrategy>(g, f.opsin, f.width, f.height, f.quant_field, f.pixel_mask, f.y_to_x,
+                        f.y_to_b, tables, bases, params, f.allow, f.costs + kStage * stage, tid);
+  }
+}
+
 __global__ void FillNanKernel(float* __restrict__ p, size_t n) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = __builtin_nanf("");
@@ -1008,7 +1126,72 @@ GJXLResult LaunchSearch(GJXLAmdDevice* device, const float* costs, int width,
   return CheckLaunch("SearchKernel");
 }
 
+// The cost parameters of a Butteraugli target
+// (src/codec/ac_strategy.cpp:884-891, host libm).
+CostParams TargetCostParams(float target) {
+  const float kBias = 0.13731742964354549f;
+  const float ratio = (target + kBias) / (1.0f + kBias);
+  CostParams p;
+  p.info_loss_multiplier = 1.2f * std::pow(ratio, 0.33677806662454718f);
+  p.zeros_multiplier = 9.3089059022677905f * std::pow(ratio, 0.50990926717963703f);
+  p.cost_delta = 10.833273317067883f * std::pow(ratio, 0.36702940662370243f);
+  p.entropy_multiplier = 1.0f;
+  return p;
+}
+
+constexpr float kStageEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
+
+__global__ void QueueResetKernel(QueueTable* __restrict__ t, int stage) {
+  if (threadIdx.x == 0) t->counters[stage] = 0;
+}
+
+GJXLResult LaunchQueueStage(GJXLAmdDevice* d, hipStream_t st, QueueTable* t, int stage,
+                            uint32_t grid) {
+  const dim3 g(grid), b(kCostThreads);
+  const float e = kStageEntropy[stage];
+  switch (stage) {
+    case 0: hipLaunchKernelGGL(CostQueueKernel<kDct8>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    case 1: hipLaunchKernelGGL(CostQueueKernel<kDct16x8>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    case 2: hipLaunchKernelGGL(CostQueueKernel<kDct8x16>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    case 3: hipLaunchKernelGGL(CostQueueKernel<kDct16x16>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    case 4: hipLaunchKernelGGL(CostQueueKernel<kDct32x16>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    case 5: hipLaunchKernelGGL(CostQueueKernel<kDct16x32>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+    default: hipLaunchKernelGGL(CostQueueKernel<kDct32x32>, g, b, 0, st, t, d->quant_tables, d->dct_bases, e); break;
+  }
+  return CheckLaunch("CostQueueKernel");
+}
+
+// plan, NaN fill, then the seven strategies' queues (kernels only: the
+// sequence is what the graph captures)
+GJXLResult LaunchQueue(GJXLAmdDevice* d, hipStream_t st, QueueTable* t, const uint32_t* grids) {
+  hipLaunchKernelGGL(QueuePlanKernel, dim3(1), dim3(64), 0, st, t);
+  GJXL_AMD_TRY(CheckLaunch("QueuePlanKernel"));
+  hipLaunchKernelGGL(QueueFillKernel, dim3(1024), dim3(256), 0, st, t);
+  GJXL_AMD_TRY(CheckLaunch("QueueFillKernel"));
+  // the largest transforms first: their items are the longest
+  constexpr int kOrder[7] = {6, 3, 4, 5, 1, 2, 0};
+  for (int i = 0; i < 7; ++i) GJXL_AMD_TRY(LaunchQueueStage(d, st, t, kOrder[i], grids[kOrder[i]]));
+  return GJXL_OK;
+}
+
 }  // namespace gjxl_amd
+
+// A tile queue for the candidate costs of batches of frames: its device
+// table, a pinned host copy, the persistent grid of each strategy and the
+// captured launch sequence.
+struct GJXLAmdCostQueue {
+  GJXLAmdDevice* device = nullptr;
+  uint32_t max_frames = 0;
+  gjxl_amd::QueueTable* table = nullptr;
+  gjxl_amd::QueueTable* host = nullptr;
+  hipEvent_t copied = nullptr;
+  bool copy_pending = false;
+  uint32_t grids[7] = {};
+  float tuned_ms[7] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 
 using gjxl_amd::Fail;
 
@@ -1030,6 +1213,83 @@ GJXLResult ValidateTarget(float target) {
   return GJXL_OK;
 }
 }  // namespace
+
+namespace gjxl_amd {
+namespace {
+
+void DropGraph(GJXLAmdCostQueue* q) {
+  if (q->exec != nullptr) (void)hipGraphExecDestroy(q->exec);
+  if (q->graph != nullptr) (void)hipGraphDestroy(q->graph);
+  q->exec = nullptr;
+  q->graph = nullptr;
+}
+
+GJXLResult CaptureQueue(GJXLAmdCostQueue* q) {
+  DropGraph(q);
+  hipStream_t st = q->device->stream;
+  GJXL_AMD_TRY(Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+                     "hipStreamBeginCapture"));
+  const GJXLResult launched = LaunchQueue(q->device, st, q->table, q->grids);
+  hipGraph_t graph = nullptr;
+  const GJXLResult ended = Check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+  GJXL_AMD_TRY(launched);
+  GJXL_AMD_TRY(ended);
+  q->graph = graph;
+  return Check(hipGraphInstantiate(&q->exec, q->graph, nullptr, nullptr, 0),
+               "hipGraphInstantiate");
+}
+
+// The batch into the host table (validated), then to the device table.
+GJXLResult UploadFrames(GJXLAmdCostQueue* q, const GJXLAmdCostFrame* frames, uint32_t count) {
+  if (frames == nullptr || count == 0 || count > q->max_frames) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue: 1 to max_frames frames");
+  }
+  uint64_t items = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const GJXLAmdCostFrame& f = frames[i];
+    if (f.opsin == nullptr || f.quant_field == nullptr || f.pixel_mask == nullptr ||
+        f.y_to_x == nullptr || f.y_to_b == nullptr || f.costs == nullptr) {
+      return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue: frame buffers must be non-null");
+    }
+    GJXL_AMD_TRY(ValidateGeometry(f.width, f.height));
+    if (f.width > 65536 || f.height > 65536) {
+      return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue: frames up to 65536 pixels a side");
+    }
+    GJXL_AMD_TRY(ValidateTarget(f.butteraugli_target));
+    items += 16ull * CeilDiv(f.width / 8, 8) * CeilDiv(f.height / 8, 8);
+  }
+  if (items >= (1ull << 31)) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue: batch too large");
+  if (q->copy_pending) {
+    GJXL_AMD_TRY(Check(hipEventSynchronize(q->copied), "hipEventSynchronize"));
+    q->copy_pending = false;
+  }
+  QueueTable* h = q->host;
+  h->count = count;
+  for (uint32_t i = 0; i < count; ++i) {
+    const GJXLAmdCostFrame& f = frames[i];
+    QueueFrame& o = h->frames[i];
+    o.opsin = f.opsin;
+    o.quant_field = f.quant_field;
+    o.pixel_mask = f.pixel_mask;
+    o.y_to_x = f.y_to_x;
+    o.y_to_b = f.y_to_b;
+    o.allow = f.allow;
+    o.costs = f.costs;
+    o.width = static_cast<int32_t>(f.width);
+    o.height = static_cast<int32_t>(f.height);
+    o.params = TargetCostParams(f.butteraugli_target);
+  }
+  hipStream_t st = q->device->stream;
+  const size_t bytes = offsetof(QueueTable, frames) + count * sizeof(QueueFrame);
+  GJXL_AMD_TRY(Check(hipMemcpyAsync(q->table, h, bytes, hipMemcpyHostToDevice, st),
+                     "hipMemcpyAsync(cost queue)"));
+  GJXL_AMD_TRY(Check(hipEventRecord(q->copied, st), "hipEventRecord"));
+  q->copy_pending = true;
+  return GJXL_OK;
+}
+
+}  // namespace
+}  // namespace gjxl_amd
 
 extern "C" {
 
@@ -1093,6 +1353,126 @@ GJXLResult gjxl_amd_ac_strategy_search(GJXLAmdDevice* device,
   }
   return gjxl_amd::LaunchSearch(device, costs, width, height,
                                 butteraugli_target, cells);
+}
+
+GJXLResult gjxl_amd_cost_queue_create(GJXLAmdDevice* device, uint32_t max_frames,
+                                      GJXLAmdCostQueue** out) {
+  GJXL_AMD_TRY(gjxl_amd::ValidateDevice(device));
+  if (out == nullptr || max_frames == 0 ||
+      max_frames > static_cast<uint32_t>(gjxl_amd::kQueueMaxFrames)) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue: 1 to 256 frames");
+  }
+  *out = nullptr;
+  auto* q = new GJXLAmdCostQueue();
+  q->device = device;
+  q->max_frames = max_frames;
+  int cus = 0;
+  GJXLResult r = gjxl_amd::Check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                                       device->ordinal),
+                                 "hipDeviceGetAttribute");
+  for (int i = 0; i < 7; ++i) q->grids[i] = 2 * static_cast<uint32_t>(cus > 0 ? cus : 256);
+  void* table = nullptr;
+  if (r == GJXL_OK) r = gjxl_amd::DeviceAlloc(device, sizeof(gjxl_amd::QueueTable), &table);
+  q->table = static_cast<gjxl_amd::QueueTable*>(table);
+  if (r == GJXL_OK) {
+    r = gjxl_amd::Check(hipHostMalloc(reinterpret_cast<void**>(&q->host),
+                                      sizeof(gjxl_amd::QueueTable), hipHostMallocDefault),
+                        "hipHostMalloc(cost queue)");
+  }
+  if (r == GJXL_OK) r = gjxl_amd::Check(hipEventCreateWithFlags(&q->copied, hipEventDisableTiming),
+                                        "hipEventCreate");
+  if (r == GJXL_OK) r = gjxl_amd::CaptureQueue(q);
+  if (r != GJXL_OK) {
+    gjxl_amd_cost_queue_destroy(q);
+    return r;
+  }
+  *out = q;
+  return GJXL_OK;
+}
+
+void gjxl_amd_cost_queue_destroy(GJXLAmdCostQueue* q) {
+  if (q == nullptr) return;
+  (void)hipStreamSynchronize(q->device->stream);
+  gjxl_amd::DropGraph(q);
+  if (q->copied != nullptr) (void)hipEventDestroy(q->copied);
+  if (q->host != nullptr) (void)hipHostFree(q->host);
+  gjxl_amd::DeviceFree(q->device, q->table);
+  delete q;
+}
+
+GJXLResult gjxl_amd_cost_queue_run(GJXLAmdCostQueue* q, const GJXLAmdCostFrame* frames,
+                                   uint32_t count) {
+  if (q == nullptr || q->exec == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue is null");
+  GJXL_AMD_TRY(gjxl_amd::UploadFrames(q, frames, count));
+  return gjxl_amd::Check(hipGraphLaunch(q->exec, q->device->stream), "hipGraphLaunch");
+}
+
+GJXLResult gjxl_amd_cost_queue_tune(GJXLAmdCostQueue* q, const GJXLAmdCostFrame* frames,
+                                    uint32_t count, uint32_t repetitions) {
+  using namespace gjxl_amd;
+  if (q == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue is null");
+  if (repetitions == 0 || repetitions > 100) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue tune: 1 to 100 repetitions");
+  }
+  GJXL_AMD_TRY(UploadFrames(q, frames, count));
+  GJXLAmdDevice* d = q->device;
+  hipStream_t st = d->stream;
+  hipLaunchKernelGGL(QueuePlanKernel, dim3(1), dim3(64), 0, st, q->table);
+  GJXL_AMD_TRY(CheckLaunch("QueuePlanKernel"));
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->ordinal);
+  uint64_t tiles = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    tiles += static_cast<uint64_t>(CeilDiv(frames[i].width / 8, 8)) * CeilDiv(frames[i].height / 8, 8);
+  }
+  hipEvent_t begin = nullptr, end = nullptr;
+  GJXL_AMD_TRY(Check(hipEventCreate(&begin), "hipEventCreate"));
+  GJXLResult r = Check(hipEventCreate(&end), "hipEventCreate");
+  constexpr uint32_t kChunks[7] = {QueueChunks<kDct8>(), QueueChunks<kDct16x8>(),
+                                   QueueChunks<kDct8x16>(), QueueChunks<kDct16x16>(),
+                                   QueueChunks<kDct32x16>(), QueueChunks<kDct16x32>(),
+                                   QueueChunks<kDct32x32>()};
+  for (int s = 0; r == GJXL_OK && s < 7; ++s) {
+    // persistent grids of one to four groups per CU, and one group per item
+    const uint64_t items = kChunks[s] * tiles;
+    const uint64_t variants[5] = {static_cast<uint64_t>(cus), 2ull * cus, 3ull * cus,
+                                  4ull * cus, items};
+    float best = 0.0f;
+    for (uint64_t grid : variants) {
+      grid = std::max<uint64_t>(1, std::min<uint64_t>(grid, items));
+      float total = 0.0f;
+      for (uint32_t k = 0; r == GJXL_OK && k <= repetitions; ++k) {
+        hipLaunchKernelGGL(QueueResetKernel, dim3(1), dim3(64), 0, st, q->table, s);
+        r = CheckLaunch("QueueResetKernel");
+        if (r == GJXL_OK) r = Check(hipEventRecord(begin, st), "hipEventRecord");
+        if (r == GJXL_OK) r = LaunchQueueStage(d, st, q->table, s, static_cast<uint32_t>(grid));
+        if (r == GJXL_OK) r = Check(hipEventRecord(end, st), "hipEventRecord");
+        if (r == GJXL_OK) r = Check(hipEventSynchronize(end), "hipEventSynchronize");
+        float ms = 0.0f;
+        if (r == GJXL_OK) r = Check(hipEventElapsedTime(&ms, begin, end), "hipEventElapsedTime");
+        if (k > 0) total += ms;  // the first run warms up
+      }
+      const float mean = total / static_cast<float>(repetitions);
+      if (r == GJXL_OK && (best == 0.0f || mean < best)) {
+        best = mean;
+        q->grids[s] = static_cast<uint32_t>(grid);
+        q->tuned_ms[s] = mean;
+      }
+    }
+  }
+  (void)hipEventDestroy(begin);
+  (void)hipEventDestroy(end);
+  GJXL_AMD_TRY(r);
+  return CaptureQueue(q);
+}
+
+GJXLResult gjxl_amd_cost_queue_grids(GJXLAmdCostQueue* q, uint32_t* grids, float* milliseconds) {
+  if (q == nullptr || grids == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost queue is null");
+  for (int i = 0; i < 7; ++i) {
+    grids[i] = q->grids[i];
+    if (milliseconds != nullptr) milliseconds[i] = q->tuned_ms[i];
+  }
+  return GJXL_OK;
 }
 
 }  // extern "C"

@@ -772,6 +772,58 @@ GJXL_AMD_API void gjxl_amd_batch_shutdown(GJXLAmdBatch* batch);
 GJXL_AMD_API void gjxl_amd_batch_destroy(GJXLAmdBatch* batch);
 
 /* ---------------------------------------------------------------------- */
+/* Chosen batching form (not the plain path; no counterpart in the          */
+/* reference, whose batching is host-level: src/codestream/batch_workflow.h */
+/* :47-109): the candidate costs of a batch of frames of uneven extents as  */
+/* persistent work groups pulling (frame, 64x64 tile, chunk of anchors)     */
+/* items of one strategy from a work queue in device memory, one captured   */
+/* hipGraph per queue (plan, NaN fill, seven strategies) replayed for every */
+/* batch, and an autotuner that times persistent grids per strategy on a    */
+/* batch and keeps the fastest.  Each frame's costs equal                   */
+/* gjxl_amd_ac_strategy_costs_pruned's for the same inputs.                 */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdCostQueue GJXLAmdCostQueue;
+
+/* One frame of a batch: the arguments of gjxl_amd_ac_strategy_costs_pruned
+ * (device pointers; allow may be null). */
+typedef struct {
+  const float* opsin;
+  const float* quant_field;
+  const float* pixel_mask;
+  const int8_t* y_to_x;
+  const int8_t* y_to_b;
+  const uint8_t* allow;
+  float* costs;
+  uint32_t width, height;  /* padded, multiples of 8, at most 65536 */
+  float butteraugli_target;
+} GJXLAmdCostFrame;
+
+/* A queue for batches of up to max_frames (at most 256) frames on the
+ * device's stream; its graph is captured here, with persistent grids of
+ * two groups per CU for every strategy until tuned. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_create(GJXLAmdDevice* device,
+                                                   uint32_t max_frames,
+                                                   GJXLAmdCostQueue** out);
+GJXL_AMD_API void gjxl_amd_cost_queue_destroy(GJXLAmdCostQueue* queue);
+/* Uploads the batch's table (waiting for the previous upload to finish
+ * reading the host copy) and launches the graph.  Enqueued: the buffers
+ * must stay valid until the stream reaches the end of the graph. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_run(GJXLAmdCostQueue* queue,
+                                                const GJXLAmdCostFrame* frames,
+                                                uint32_t count);
+/* Times each strategy's queue on this batch (repetitions runs per
+ * variant, after one warm-up) with persistent grids of one to four groups
+ * per CU and with one group per item, keeps the fastest grid per strategy
+ * and captures the graph again.  Synchronous; writes the frames' costs. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_tune(GJXLAmdCostQueue* queue,
+                                                 const GJXLAmdCostFrame* frames,
+                                                 uint32_t count, uint32_t repetitions);
+/* The grid of each strategy (7, stage order) and its tuned time in ms
+ * (0 before tuning; milliseconds may be null). */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_grids(GJXLAmdCostQueue* queue,
+                                                  uint32_t* grids, float* milliseconds);
+
+/* ---------------------------------------------------------------------- */
 /* Chosen learned form (not the plain path; no counterpart in the           */
 /* reference): a proposal network that predicts, per 8x8 block, logits over */
 /* the format's 27 strategy codes and the log ratio of the Butteraugli      */
