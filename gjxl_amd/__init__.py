@@ -154,6 +154,10 @@ _SIGNATURES = {
     "gjxl_amd_batch_create": [_I, _U32, ctypes.POINTER(_P)],
     "gjxl_amd_batch_max_in_flight": [_P],
     "gjxl_amd_batch_encode": [_P, _P, _SZ, _P],
+    "gjxl_amd_batch_set_tile_queue": [_P, ctypes.c_int32, _U32],
+    "gjxl_amd_plain_path_run_front": [_P, _P],
+    "gjxl_amd_plain_path_cost_frame": [_P, _P],
+    "gjxl_amd_plain_path_run_back": [_P, ctypes.c_int32, _P, _P],
     "gjxl_amd_batch_results_free": [_P, _SZ],
     "gjxl_amd_batch_shutdown": [_P],
     "gjxl_amd_batch_destroy": [_P],
@@ -1154,6 +1158,13 @@ class Batch:
         out = results if results is not None else (BatchResult * max(count, 1))()
         _check(library().gjxl_amd_batch_encode(self.handle, reqs, count, out))
         return out
+
+    def set_tile_queue(self, enable: bool = True, tune_repetitions: int = 0) -> None:
+        """Group mode: the candidate costs of each group of max_in_flight
+        images through one tile-queue graph (tuned on the next call's first
+        group when tune_repetitions > 0)."""
+        _check(library().gjxl_amd_batch_set_tile_queue(self.handle, int(enable),
+                                                       tune_repetitions))
 
     @staticmethod
     def free_results(results, count: int) -> None:

@@ -9,7 +9,14 @@
 // the remaining CUs).  A worker keeps one plain path, and so its device
 // buffers, across images: it creates the path at the bounding extent of the
 // call's requests and re-targets it to each image's extent and options
-// inside those buffers.  No kernels here.
+// inside those buffers.
+//
+// With the tile queue on (gjxl_amd_batch_set_tile_queue, the chosen batching
+// form), the workers take the call's images in groups of max_in_flight,
+// largest first: each runs its image's front half, then one worker launches
+// the queue's captured graph for the candidate costs of the whole group, and
+// each runs its back half once the costs are on its stream (an event).  No
+// kernels here.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -41,10 +48,13 @@ struct Call {
   Clock::time_point arrival;
   uint32_t bound_width = 0, bound_height = 0;  // over the requests
   std::atomic<size_t> next{0};
+  bool grouped = false;
+  std::vector<size_t> order;  // grouped: request indices, largest image first
   size_t done = 0;  // guarded by the driver's mutex
 };
 
 struct Worker {
+  size_t index = 0;
   GJXLAmdDevice* device = nullptr;
   GJXLAmdPlainPath* path = nullptr;
   uint32_t capacity_width = 0, capacity_height = 0;  // the path's creation extent
@@ -124,27 +134,36 @@ GJXLResult PreparePath(Worker* w, const Call& call, uint32_t width, uint32_t hei
   return GJXL_OK;
 }
 
-// One image on one worker into *r (which starts zeroed).
-void RunImage(Worker* w, const Call& call, const GJXLAmdBatchRequest& request,
-              GJXLAmdBatchResult* r) {
+// The options and path of one request; a refusal lands in *r (which starts
+// zeroed).
+GJXLResult PrepareImage(Worker* w, const Call& call, const GJXLAmdBatchRequest& request,
+                        GJXLAmdBatchResult* r) {
   GJXLAmdPlainPathOptions options{};
   if (request.options.struct_size == 0) {
     (void)gjxl_amd_plain_path_options_init(&options, sizeof(options));
   } else if (request.options.struct_size < sizeof(GJXLAmdPlainPathOptions)) {
-    return SetFailure(r, Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch request options are too small"));
+    const GJXLResult e = Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch request options are too small");
+    SetFailure(r, e);
+    return e;
   } else {
     options = request.options;
   }
   const GJXLAmdImage& image = request.image;
   if (image.device_resident) {
-    return SetFailure(r, Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch requests take host pixels"));
+    const GJXLResult e = Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch requests take host pixels");
+    SetFailure(r, e);
+    return e;
   }
   r->width = image.width;
   r->height = image.height;
   const GJXLResult prepared = PreparePath(w, call, image.width, image.height, options);
-  if (prepared != GJXL_OK) return SetFailure(r, prepared);
-  GJXLAmdPlainFrame frame{};
-  GJXLResult status = gjxl_amd_plain_path_run(w->path, &image, nullptr, &frame);
+  if (prepared != GJXL_OK) SetFailure(r, prepared);
+  return prepared;
+}
+
+// The outputs of a finished run (status) into *r.
+void CollectImage(Worker* w, GJXLResult status, const GJXLAmdPlainFrame& frame,
+                  GJXLAmdBatchResult* r) {
   if (status != GJXL_OK) {
     // a failed run may leave the path mid-sequence: start the next image afresh
     DropPath(w);
@@ -180,6 +199,15 @@ void RunImage(Worker* w, const Call& call, const GJXLAmdBatchRequest& request,
   r->status = GJXL_OK;
 }
 
+// One image on one worker into *r (which starts zeroed).
+void RunImage(Worker* w, const Call& call, const GJXLAmdBatchRequest& request,
+              GJXLAmdBatchResult* r) {
+  if (PrepareImage(w, call, request, r) != GJXL_OK) return;
+  GJXLAmdPlainFrame frame{};
+  const GJXLResult status = gjxl_amd_plain_path_run(w->path, &request.image, nullptr, &frame);
+  CollectImage(w, status, frame, r);
+}
+
 }  // namespace
 
 struct GJXLAmdBatch {
@@ -191,9 +219,106 @@ struct GJXLAmdBatch {
   Call* call = nullptr;
   uint64_t generation = 0;
   bool closed = false, stopping = false, joined = false;
+  // the tile queue (on worker 0's device) and the group hand-over
+  GJXLAmdCostQueue* queue = nullptr;
+  uint32_t tune_repetitions = 0;  // tune on the next group when non-zero
+  std::vector<GJXLAmdCostFrame> frames;
+  std::vector<char> front_ok;
+  std::vector<hipEvent_t> front_done;
+  hipEvent_t costs_done = nullptr;
+  bool queue_ok = false;
+  std::mutex group_mutex;
+  std::condition_variable group_cv;
+  size_t group_arrived = 0;
+  uint64_t group_phase = 0;
 };
 
 namespace {
+
+// Every worker arrives once per group step.
+void GroupBarrier(GJXLAmdBatch* b) {
+  std::unique_lock<std::mutex> lock(b->group_mutex);
+  const uint64_t phase = b->group_phase;
+  if (++b->group_arrived == b->workers.size()) {
+    b->group_arrived = 0;
+    ++b->group_phase;
+    b->group_cv.notify_all();
+  } else {
+    b->group_cv.wait(lock, [&] { return b->group_phase != phase; });
+  }
+}
+
+// Worker 0, between the barriers: the group's candidate costs on its
+// stream, after every front half; b->queue_ok says whether they ran.
+void LaunchGroupQueue(GJXLAmdBatch* b, Worker* leader) {
+  b->queue_ok = false;
+  std::vector<GJXLAmdCostFrame> frames;
+  for (size_t k = 0; k < b->workers.size(); ++k) {
+    if (!b->front_ok[k]) continue;
+    if (hipStreamWaitEvent(leader->device->stream, b->front_done[k], 0) != hipSuccess) return;
+    frames.push_back(b->frames[k]);
+  }
+  if (frames.empty()) return;
+  const uint32_t n = static_cast<uint32_t>(frames.size());
+  if (b->tune_repetitions != 0) {
+    const GJXLResult tuned = gjxl_amd_cost_queue_tune(b->queue, frames.data(), n,
+                                                      b->tune_repetitions);
+    b->tune_repetitions = 0;
+    if (tuned != GJXL_OK) return;
+  }
+  if (gjxl_amd_cost_queue_run(b->queue, frames.data(), n) != GJXL_OK) return;
+  b->queue_ok = hipEventRecord(b->costs_done, leader->device->stream) == hipSuccess;
+}
+
+// A grouped call: group g holds order[g * W .. g * W + W - 1]; worker j
+// takes slot j of every group.  A worker whose front half failed (or has no
+// image) still takes part in both barriers; one whose front half ran but
+// whose group's queue did not computes its own costs.
+void RunGroups(GJXLAmdBatch* b, Worker* w, Call* call) {
+  const size_t width = b->workers.size();
+  const size_t groups = (call->count + width - 1) / width;
+  for (size_t g = 0; g < groups; ++g) {
+    const size_t slot = g * width + w->index;
+    const bool has = slot < call->count;
+    const size_t i = has ? call->order[slot] : 0;
+    GJXLAmdBatchResult* r = has ? &(*call->staged)[i] : nullptr;
+    Clock::time_point start;
+    bool front = false;
+    if (has) {
+      start = Clock::now();
+      if (PrepareImage(w, *call, call->requests[i], r) == GJXL_OK) {
+        const GJXLResult status = gjxl_amd_plain_path_run_front(w->path, &call->requests[i].image);
+        if (status == GJXL_OK) {
+          front = gjxl_amd_plain_path_cost_frame(w->path, &b->frames[w->index]) == GJXL_OK &&
+                  hipEventRecord(b->front_done[w->index], w->device->stream) == hipSuccess;
+        } else {
+          DropPath(w);
+          SetFailure(r, status);
+        }
+      }
+    }
+    b->front_ok[w->index] = front;
+    GroupBarrier(b);
+    if (w->index == 0) LaunchGroupQueue(b, w);
+    GroupBarrier(b);
+    if (front) {
+      const bool ready = b->queue_ok &&
+                         hipStreamWaitEvent(w->device->stream, b->costs_done, 0) == hipSuccess;
+      GJXLAmdPlainFrame frame{};
+      const GJXLResult status = gjxl_amd_plain_path_run_back(w->path, ready ? 1 : 0, nullptr,
+                                                             &frame);
+      CollectImage(w, status, frame, r);
+    }
+    if (has) {
+      const Clock::time_point end = Clock::now();
+      r->queue_nanoseconds = Nanoseconds(start - call->arrival);
+      r->service_nanoseconds = Nanoseconds(end - start);
+      r->ready_nanoseconds = r->queue_nanoseconds + r->service_nanoseconds;
+      std::lock_guard<std::mutex> lock(b->mutex);
+      if (++call->done == call->count) b->finished.notify_all();
+    }
+  }
+}
 
 void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
   uint64_t seen = 0;
@@ -205,6 +330,10 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
       if (b->call == nullptr || b->generation == seen) return;  // stopping, nothing new
       call = b->call;
       seen = b->generation;
+    }
+    if (call->grouped) {
+      RunGroups(b, w, call);
+      continue;
     }
     for (;;) {
       const size_t i = call->next.fetch_add(1);
@@ -222,9 +351,23 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
   }
 }
 
+void DropQueue(GJXLAmdBatch* b) {
+  gjxl_amd_cost_queue_destroy(b->queue);
+  b->queue = nullptr;
+  for (hipEvent_t& e : b->front_done) {
+    if (e != nullptr) (void)hipEventDestroy(e);
+  }
+  b->front_done.clear();
+  if (b->costs_done != nullptr) (void)hipEventDestroy(b->costs_done);
+  b->costs_done = nullptr;
+}
+
 void JoinAndRelease(GJXLAmdBatch* b) {
   for (auto& w : b->workers) {
     if (w->thread.joinable()) w->thread.join();
+  }
+  DropQueue(b);
+  for (auto& w : b->workers) {
     DropPath(w.get());
     gjxl_amd_device_destroy(w->device);
     w->device = nullptr;
@@ -248,6 +391,7 @@ GJXLResult gjxl_amd_batch_create(int device_ordinal, uint32_t max_in_flight,
   try {
     for (uint32_t i = 0; r == GJXL_OK && i < max_in_flight; ++i) {
       auto w = std::make_unique<Worker>();
+      w->index = i;
       r = gjxl_amd_device_create(device_ordinal, &w->device);
       if (r == GJXL_OK) b->workers.push_back(std::move(w));
     }
@@ -299,6 +443,21 @@ GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* b, const GJXLAmdBatchRequest* req
   call.count = count;
   call.staged = &staged;
   call.arrival = arrival;
+  if (b->queue != nullptr) {
+    // groups of similar extents, largest first (results keep request order)
+    try {
+      call.order.resize(count);
+    } catch (...) {
+      return Fail(GJXL_ERROR_OUT_OF_MEMORY, "batch order staging failed");
+    }
+    for (size_t i = 0; i < count; ++i) call.order[i] = i;
+    auto pixels = [&](size_t i) {
+      return static_cast<uint64_t>(requests[i].image.width) * requests[i].image.height;
+    };
+    std::stable_sort(call.order.begin(), call.order.end(),
+                     [&](size_t a, size_t c) { return pixels(a) > pixels(c); });
+    call.grouped = true;
+  }
   for (size_t i = 0; i < count; ++i) {
     const GJXLAmdImage& image = requests[i].image;
     if (image.width == 0 || image.height == 0 || image.width > (1u << 16) ||
@@ -320,6 +479,45 @@ GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* b, const GJXLAmdBatchRequest* req
     b->call = nullptr;
   }
   std::memcpy(results, staged.data(), count * sizeof(GJXLAmdBatchResult));
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_batch_set_tile_queue(GJXLAmdBatch* b, int32_t enable,
+                                         uint32_t tune_repetitions) {
+  if (b == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "batch is null");
+  if (tune_repetitions > 100) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "tune repetitions must be at most 100");
+  }
+  std::lock_guard<std::mutex> serial(b->call_mutex);
+  {
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (b->closed) return Fail(GJXL_ERROR_UNAVAILABLE, "batch driver is shut down");
+  }
+  DropQueue(b);
+  if (!enable) return GJXL_OK;
+  const size_t n = b->workers.size();
+  GJXLResult r = gjxl_amd_cost_queue_create(b->workers[0]->device, static_cast<uint32_t>(n),
+                                            &b->queue);
+  try {
+    b->frames.assign(n, GJXLAmdCostFrame{});
+    b->front_ok.assign(n, 0);
+    b->front_done.assign(n, nullptr);
+  } catch (...) {
+    r = Fail(GJXL_ERROR_OUT_OF_MEMORY, "tile queue staging failed");
+  }
+  for (size_t i = 0; r == GJXL_OK && i < n; ++i) {
+    r = gjxl_amd::Check(hipEventCreateWithFlags(&b->front_done[i], hipEventDisableTiming),
+                        "hipEventCreate");
+  }
+  if (r == GJXL_OK) {
+    r = gjxl_amd::Check(hipEventCreateWithFlags(&b->costs_done, hipEventDisableTiming),
+                        "hipEventCreate");
+  }
+  if (r != GJXL_OK) {
+    DropQueue(b);
+    return r;
+  }
+  b->tune_repetitions = tune_repetitions;
   return GJXL_OK;
 }
 

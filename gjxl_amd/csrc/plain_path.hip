@@ -70,6 +70,8 @@ struct GJXLAmdPlainPath {
   GJXLAmdProposalOptions proposal_options{};
   int32_t evaluations_run = 0;
   float proposal_norm = 0.0f;
+  const float* proposal_head = nullptr;  // the learned run's head, for the back half
+  bool front_done = false;
   GJXLAmdAqSession* session = nullptr;
   GJXLAmdButteraugli* butteraugli = nullptr;
   GJXLAmdTokenizer* tokenizer = nullptr;
@@ -200,8 +202,12 @@ GJXLResult ValidateOptions(const GJXLAmdPlainPathOptions* o) {
 
 // The run after the upload (device-resident packed pixels in p->srgb or the
 // caller's device buffer).
-GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
-                       uint32_t bpp) {
+// The front half: XYB, the Butteraugli reference side (second stream),
+// Gaborish inverse, initial quant field, the learned proposal if attached,
+// initial CfL; enqueued, no host wait.  What the candidate costs read is
+// ready on the device stream at its end.
+GJXLResult RunFront(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
+                    uint32_t bpp) {
   GJXLAmdDevice* d = p->device;
   const float t = p->options.distance;
   GJXL_AMD_TRY(gjxl_amd_srgb8_to_linear_and_opsin(d, srgb, row_stride, bpp, p->width,
@@ -246,9 +252,23 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
                      "hipStreamWaitEvent"));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
-  GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs_pruned(
-    d, p->preprocessed, p->pw, p->ph, p->quant_field, p->pixel_mask, p->y_to_x, p->y_to_b, t,
-    p->proposal != nullptr ? p->allow : nullptr, p->costs));
+  p->proposal_head = head;
+  return GJXL_OK;
+}
+
+// The back half: the candidate costs (unless a caller has written them
+// into p->costs on this stream, costs_ready), the strategy search, AQ
+// preparation, the Butteraugli loop, the entropy inputs and the tokens.
+GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
+  GJXLAmdDevice* d = p->device;
+  const float t = p->options.distance;
+  const GJXLAmdProposalOptions& po = p->proposal_options;
+  const float* head = p->proposal_head;
+  if (!costs_ready) {
+    GJXL_AMD_TRY(gjxl_amd_ac_strategy_costs_pruned(
+      d, p->preprocessed, p->pw, p->ph, p->quant_field, p->pixel_mask, p->y_to_x, p->y_to_b, t,
+      p->proposal != nullptr ? p->allow : nullptr, p->costs));
+  }
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_COSTS));
   GJXL_AMD_TRY(gjxl_amd_ac_strategy_search(d, p->costs, p->pw, p->ph, t, p->cells));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AC_STRATEGY_SEARCH));
@@ -341,6 +361,12 @@ GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stri
   GJXL_AMD_TRY(gjxl_amd::TokenizeResult(host, &p->token_count));
   for (int e = 0; e < p->evaluations_run; ++e) std::memcpy(&p->scores[e], &host[8 + e], 4);
   return GJXL_OK;
+}
+
+GJXLResult RunResident(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
+                       uint32_t bpp) {
+  GJXL_AMD_TRY(RunFront(p, srgb, row_stride, bpp));
+  return RunBack(p, false);
 }
 
 GJXLResult ReadTokens(GJXLAmdPlainPath* p, const GJXLAmdTokenOutput* out) {
@@ -552,12 +578,19 @@ GJXLResult gjxl_amd_plain_path_reconfigure(GJXLAmdPlainPath* p, uint32_t width,
   }
   PathGeometry(p, width, height, need);
   if (options != nullptr) p->options = *options;
+  p->front_done = false;
+  p->ran = false;
   return GJXL_OK;
 }
 
-GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* image,
-                                   const GJXLAmdTokenOutput* tokens,
-                                   GJXLAmdPlainFrame* frame) {
+}  // extern "C"
+
+namespace {
+
+// Validates the image, starts the stage marks and uploads host pixels into
+// the path's device copy; *srgb, *stride: the resident pixels.
+GJXLResult BeginRun(GJXLAmdPlainPath* p, const GJXLAmdImage* image, const uint8_t** srgb_out,
+                    size_t* stride_out) {
   GJXL_AMD_TRY(Validate(p));
   if (image == nullptr || image->pixels == nullptr) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "plain path image is null");
@@ -576,6 +609,7 @@ GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* imag
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "pixel buffer is smaller than its rows");
   }
   p->ran = false;
+  p->front_done = false;
   p->mark_count = 0;
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BEGIN));
   const uint8_t* srgb = static_cast<const uint8_t*>(image->pixels);
@@ -594,7 +628,63 @@ GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* imag
     stride = row;
   }
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_UPLOAD));
-  GJXL_AMD_TRY(RunResident(p, srgb, stride, bpp));
+  *srgb_out = srgb;
+  *stride_out = stride;
+  return GJXL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* p, const GJXLAmdImage* image,
+                                   const GJXLAmdTokenOutput* tokens,
+                                   GJXLAmdPlainFrame* frame) {
+  const uint8_t* srgb = nullptr;
+  size_t stride = 0;
+  GJXL_AMD_TRY(BeginRun(p, image, &srgb, &stride));
+  GJXL_AMD_TRY(RunResident(p, srgb, stride, image->bytes_per_pixel));
+  GJXL_AMD_TRY(ReadTokens(p, tokens));
+  p->ran = true;
+  return FillFrame(p, frame);
+}
+
+GJXLResult gjxl_amd_plain_path_run_front(GJXLAmdPlainPath* p, const GJXLAmdImage* image) {
+  const uint8_t* srgb = nullptr;
+  size_t stride = 0;
+  GJXL_AMD_TRY(BeginRun(p, image, &srgb, &stride));
+  GJXL_AMD_TRY(RunFront(p, srgb, stride, image->bytes_per_pixel));
+  p->front_done = true;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_plain_path_cost_frame(GJXLAmdPlainPath* p, GJXLAmdCostFrame* frame) {
+  GJXL_AMD_TRY(Validate(p));
+  if (frame == nullptr) return Fail(GJXL_ERROR_INVALID_ARGUMENT, "cost frame is null");
+  GJXLAmdCostFrame f{};
+  f.opsin = p->preprocessed;
+  f.quant_field = p->quant_field;
+  f.pixel_mask = p->pixel_mask;
+  f.y_to_x = p->y_to_x;
+  f.y_to_b = p->y_to_b;
+  f.allow = p->proposal != nullptr ? p->allow : nullptr;
+  f.costs = p->costs;
+  f.width = p->pw;
+  f.height = p->ph;
+  f.butteraugli_target = p->options.distance;
+  *frame = f;
+  return GJXL_OK;
+}
+
+GJXLResult gjxl_amd_plain_path_run_back(GJXLAmdPlainPath* p, int32_t costs_ready,
+                                        const GJXLAmdTokenOutput* tokens,
+                                        GJXLAmdPlainFrame* frame) {
+  GJXL_AMD_TRY(Validate(p));
+  if (!p->front_done) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "the front half of this run has not been enqueued");
+  }
+  p->front_done = false;
+  GJXL_AMD_TRY(RunBack(p, costs_ready != 0));
   GJXL_AMD_TRY(ReadTokens(p, tokens));
   p->ran = true;
   return FillFrame(p, frame);

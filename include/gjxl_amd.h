@@ -574,6 +574,58 @@ GJXL_AMD_API GJXLResult gjxl_amd_coefficient_order_tokens(
   size_t capacity, size_t* count);
 
 /* ---------------------------------------------------------------------- */
+/* Chosen batching form (not the plain path; no counterpart in the          */
+/* reference, whose batching is host-level: src/codestream/batch_workflow.h */
+/* :47-109): the candidate costs of a batch of frames of uneven extents as  */
+/* persistent work groups pulling (frame, 64x64 tile, chunk of anchors)     */
+/* items of one strategy from a work queue in device memory, one captured   */
+/* hipGraph per queue (plan, NaN fill, seven strategies) replayed for every */
+/* batch, and an autotuner that times persistent grids per strategy on a    */
+/* batch and keeps the fastest.  Each frame's costs equal                   */
+/* gjxl_amd_ac_strategy_costs_pruned's for the same inputs.                 */
+/* ---------------------------------------------------------------------- */
+typedef struct GJXLAmdCostQueue GJXLAmdCostQueue;
+
+/* One frame of a batch: the arguments of gjxl_amd_ac_strategy_costs_pruned
+ * (device pointers; allow may be null). */
+typedef struct {
+  const float* opsin;
+  const float* quant_field;
+  const float* pixel_mask;
+  const int8_t* y_to_x;
+  const int8_t* y_to_b;
+  const uint8_t* allow;
+  float* costs;
+  uint32_t width, height;  /* padded, multiples of 8, at most 65536 */
+  float butteraugli_target;
+} GJXLAmdCostFrame;
+
+/* A queue for batches of up to max_frames (at most 256) frames on the
+ * device's stream; its graph is captured here, with persistent grids of
+ * two groups per CU for every strategy until tuned. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_create(GJXLAmdDevice* device,
+                                                   uint32_t max_frames,
+                                                   GJXLAmdCostQueue** out);
+GJXL_AMD_API void gjxl_amd_cost_queue_destroy(GJXLAmdCostQueue* queue);
+/* Uploads the batch's table (waiting for the previous upload to finish
+ * reading the host copy) and launches the graph.  Enqueued: the buffers
+ * must stay valid until the stream reaches the end of the graph. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_run(GJXLAmdCostQueue* queue,
+                                                const GJXLAmdCostFrame* frames,
+                                                uint32_t count);
+/* Times each strategy's queue on this batch (repetitions runs per
+ * variant, after one warm-up) with persistent grids of one to four groups
+ * per CU and with one group per item, keeps the fastest grid per strategy
+ * and captures the graph again.  Synchronous; writes the frames' costs. */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_tune(GJXLAmdCostQueue* queue,
+                                                 const GJXLAmdCostFrame* frames,
+                                                 uint32_t count, uint32_t repetitions);
+/* The grid of each strategy (7, stage order) and its tuned time in ms
+ * (0 before tuning; milliseconds may be null). */
+GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_grids(GJXLAmdCostQueue* queue,
+                                                  uint32_t* grids, float* milliseconds);
+
+/* ---------------------------------------------------------------------- */
 /* The whole plain path as one call.  Replaces, for one image, the stage     */
 /* sequence that gjxl_encode (src/c_api/gjxl.cpp:470-637) runs through       */
 /* PrepareWorkflow (src/codestream/workflow.cpp:636-768) and                 */
@@ -702,6 +754,21 @@ GJXL_AMD_API GJXLResult gjxl_amd_plain_path_run(GJXLAmdPlainPath* path,
                                                 const GJXLAmdImage* image,
                                                 const GJXLAmdTokenOutput* tokens,
                                                 GJXLAmdPlainFrame* frame);
+/* The run in two halves, for callers that batch the candidate costs of
+ * several paths (gjxl_amd_cost_queue_run): run_front uploads and enqueues
+ * every stage before the candidate costs (no host wait); cost_frame
+ * describes the path's cost inputs and planes; run_back, after the costs
+ * are on the path's device stream (costs_ready non-zero: written there by
+ * the caller, ordered before this call by an event the stream waits for;
+ * zero: computed here), runs the rest as gjxl_amd_plain_path_run does. */
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_run_front(GJXLAmdPlainPath* path,
+                                                      const GJXLAmdImage* image);
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_cost_frame(GJXLAmdPlainPath* path,
+                                                       GJXLAmdCostFrame* frame);
+GJXL_AMD_API GJXLResult gjxl_amd_plain_path_run_back(GJXLAmdPlainPath* path,
+                                                     int32_t costs_ready,
+                                                     const GJXLAmdTokenOutput* tokens,
+                                                     GJXLAmdPlainFrame* frame);
 /* Device time of each stage of the last completed run, from HIP events on
  * the device stream: stages[i] (GJXL_AMD_STAGE_*) took milliseconds[i];
  * *count receives the number of entries (evaluation stages repeat). */
@@ -767,61 +834,21 @@ GJXL_AMD_API GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* batch,
 /* Frees the host memory of count results and zeroes them. */
 GJXL_AMD_API void gjxl_amd_batch_results_free(GJXLAmdBatchResult* results,
                                               size_t count);
+/* The chosen batching form for this driver (no counterpart in the
+ * reference): with enable non-zero, later encode calls take their images in
+ * groups of max_in_flight, largest first; every worker runs its image's
+ * front half (gjxl_amd_plain_path_run_front), one tile-queue graph
+ * (gjxl_amd_cost_queue_*) computes the candidate costs of the whole group,
+ * and every worker runs its back half.  Results, their order and their
+ * values are those of the default mode.  tune_repetitions non-zero tunes
+ * the queue's grids on the next call's first group.  Not while an encode
+ * call is in progress. */
+GJXL_AMD_API GJXLResult gjxl_amd_batch_set_tile_queue(GJXLAmdBatch* batch, int32_t enable,
+                                                      uint32_t tune_repetitions);
 GJXL_AMD_API void gjxl_amd_batch_shutdown(GJXLAmdBatch* batch);
 /* Shuts down, then frees the driver; no call may be in progress. */
 GJXL_AMD_API void gjxl_amd_batch_destroy(GJXLAmdBatch* batch);
 
-/* ---------------------------------------------------------------------- */
-/* Chosen batching form (not the plain path; no counterpart in the          */
-/* reference, whose batching is host-level: src/codestream/batch_workflow.h */
-/* :47-109): the candidate costs of a batch of frames of uneven extents as  */
-/* persistent work groups pulling (frame, 64x64 tile, chunk of anchors)     */
-/* items of one strategy from a work queue in device memory, one captured   */
-/* hipGraph per queue (plan, NaN fill, seven strategies) replayed for every */
-/* batch, and an autotuner that times persistent grids per strategy on a    */
-/* batch and keeps the fastest.  Each frame's costs equal                   */
-/* gjxl_amd_ac_strategy_costs_pruned's for the same inputs.                 */
-/* ---------------------------------------------------------------------- */
-typedef struct GJXLAmdCostQueue GJXLAmdCostQueue;
-
-/* One frame of a batch: the arguments of gjxl_amd_ac_strategy_costs_pruned
- * (device pointers; allow may be null). */
-typedef struct {
-  const float* opsin;
-  const float* quant_field;
-  const float* pixel_mask;
-  const int8_t* y_to_x;
-  const int8_t* y_to_b;
-  const uint8_t* allow;
-  float* costs;
-  uint32_t width, height;  /* padded, multiples of 8, at most 65536 */
-  float butteraugli_target;
-} GJXLAmdCostFrame;
-
-/* A queue for batches of up to max_frames (at most 256) frames on the
- * device's stream; its graph is captured here, with persistent grids of
- * two groups per CU for every strategy until tuned. */
-GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_create(GJXLAmdDevice* device,
-                                                   uint32_t max_frames,
-                                                   GJXLAmdCostQueue** out);
-GJXL_AMD_API void gjxl_amd_cost_queue_destroy(GJXLAmdCostQueue* queue);
-/* Uploads the batch's table (waiting for the previous upload to finish
- * reading the host copy) and launches the graph.  Enqueued: the buffers
- * must stay valid until the stream reaches the end of the graph. */
-GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_run(GJXLAmdCostQueue* queue,
-                                                const GJXLAmdCostFrame* frames,
-                                                uint32_t count);
-/* Times each strategy's queue on this batch (repetitions runs per
- * variant, after one warm-up) with persistent grids of one to four groups
- * per CU and with one group per item, keeps the fastest grid per strategy
- * and captures the graph again.  Synchronous; writes the frames' costs. */
-GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_tune(GJXLAmdCostQueue* queue,
-                                                 const GJXLAmdCostFrame* frames,
-                                                 uint32_t count, uint32_t repetitions);
-/* The grid of each strategy (7, stage order) and its tuned time in ms
- * (0 before tuning; milliseconds may be null). */
-GJXL_AMD_API GJXLResult gjxl_amd_cost_queue_grids(GJXLAmdCostQueue* queue,
-                                                  uint32_t* grids, float* milliseconds);
 
 /* ---------------------------------------------------------------------- */
 /* Chosen learned form (not the plain path; no counterpart in the           */

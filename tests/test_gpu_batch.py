@@ -89,3 +89,36 @@ def test_batch_failure_is_per_image_and_shutdown_refuses():
         batch.encode(images[:1], results=sentinel)
     assert sentinel[0].width == 12345
     batch.close()
+
+
+def test_tile_queue_groups_match_default_mode(device):
+    """Group mode (gjxl_amd_batch_set_tile_queue): images in groups of three,
+    largest first, their candidate costs through one tile-queue graph per
+    group; every result equals the default mode's, in request order, on a
+    tuned call and on the call after it, and a bad request still fails
+    alone."""
+    images = _images() + [np.ascontiguousarray(synthetic.generate(520, 344, seed=8))]
+    batch = gjxl_amd.Batch(0, 3)
+    plain = batch.encode(images)
+    want = [(plain[i].tokens(), plain[i].scores_list()) for i in range(len(images))]
+    batch.free_results(plain, len(images))
+    batch.set_tile_queue(True, tune_repetitions=1)
+    for call in range(2):
+        print(f"tile-queue call {call}", flush=True)
+        got = batch.encode(images)
+        for i in range(len(images)):
+            r = got[i]
+            assert r.status == 0, r.message
+            assert (r.width, r.height) == images[i].shape[1::-1]
+            t = r.tokens()
+            for k in ("values", "contexts", "group_offsets"):
+                assert np.array_equal(t[k], want[i][0][k]), (call, i, k)
+            assert r.scores_list() == want[i][1]
+        batch.free_results(got, len(images))
+    bad = np.zeros((8, 8, 2), np.uint8)
+    got = batch.encode([images[3], bad, images[4]])
+    assert got[0].status == 0 and got[2].status == 0
+    assert got[1].status == 1 and b"RGB8" in got[1].message
+    batch.free_results(got, 3)
+    batch.set_tile_queue(False)
+    batch.close()
