@@ -22,6 +22,7 @@ image.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -302,28 +303,31 @@ def _concurrent(gjxl_amd, srgb, w, h, images: int, steps: int) -> dict:
             "unit": "MP/s", "digests": digests}
 
 
-def batch_workload(count: int, seed: int = 4) -> list[np.ndarray]:
+def batch_workload(count: int, seed: int = 4, largest: int = 4096) -> list[np.ndarray]:
     """`count` images whose width and height are drawn uniformly from 256 to
-    4096 in steps of 8, cropped at random offsets from one seeded 4096x4096
-    synthetic image ("textured", seed 2)."""
+    `largest` in steps of 8, cropped at random offsets from one seeded
+    4096x4096 synthetic image ("textured", seed 2)."""
     from gjxl_amd import synthetic
     rng = np.random.default_rng(seed)
     source = synthetic.generate(4096, 4096, seed=2, recipe="textured")
     out = []
     for _ in range(count):
-        w, h = (int(v) for v in rng.integers(32, 513, size=2) * 8)
+        w, h = (int(v) for v in rng.integers(32, largest // 8 + 1, size=2) * 8)
         x0, y0 = int(rng.integers(0, 4096 - w + 1)), int(rng.integers(0, 4096 - h + 1))
         out.append(np.ascontiguousarray(source[y0:y0 + h, x0:x0 + w]))
     return out
 
 
 def _batch(gjxl_amd, images: list[np.ndarray], lanes: int,
-           calls: int = 2) -> tuple[list[float], list[int]]:
+           calls: int = 2, tile_queue: bool = False) -> tuple[list[float], list[int]]:
     """Seconds of each of `calls` encode calls over the same images on one
     batch driver with `lanes` workers (the first call creates each worker's
     plain path at the bounding extent, later calls reuse it), and every
-    image's token count."""
+    image's token count.  tile_queue: group mode, tuned on the first call's
+    first group."""
     batch = gjxl_amd.Batch(0, lanes)
+    if tile_queue:
+        batch.set_tile_queue(True, tune_repetitions=2)
     seconds, counts = [], None
     for _ in range(calls):
         t0 = time.perf_counter()
@@ -343,6 +347,68 @@ def _batch(gjxl_amd, images: list[np.ndarray], lanes: int,
 
 
 MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def cost_queue_figures(gjxl_amd, dev, images: list[np.ndarray], reps: int = 3) -> dict:
+    """The candidate costs of a batch of uneven frames: per-frame launches
+    (gjxl_amd_ac_strategy_costs, one frame after another on one stream)
+    against one launch of the tuned tile-queue graph; device time from HIP
+    events, the costs compared bit for bit."""
+    lib = gjxl_amd.library()
+    frames, pixels = [], 0
+    for img in images:
+        h, w = img.shape[:2]
+        srgb = dev.array(img)
+        linear, opsin = dev.srgb8_to_linear_and_opsin(srgb, w, h)
+        pre = dev.gaborish_inverse(opsin)
+        iqf = dev.initial_quant_field(opsin, 1.0)
+        cfl = dev.initial_cfl(pre)
+        ph, pw = pre.shape[1:]
+        frames.append({"opsin": pre, "quant_field": iqf["quant_field"],
+                       "pixel_mask": iqf["pixel_mask"], "y_to_x": cfl["y_to_x"],
+                       "y_to_b": cfl["y_to_b"], "costs": dev.empty((7, ph // 8, pw // 8)),
+                       "target": 1.0})
+        iqf["strategy_mask"].free()
+        for a in (srgb, linear, opsin):
+            a.free()
+        pixels += w * h
+    dev.synchronize()
+
+    def eager():
+        for f in frames:
+            _, ph, pw = f["opsin"].shape
+            gjxl_amd._check(lib.gjxl_amd_ac_strategy_costs(
+                dev.handle, f["opsin"].ptr, pw, ph, f["quant_field"].ptr, f["pixel_mask"].ptr,
+                f["y_to_x"].ptr, f["y_to_b"].ptr, 1.0, f["costs"].ptr))
+
+    def timed(fn):
+        fn()
+        dev.synchronize()
+        dev.record(0)
+        for _ in range(reps):
+            fn()
+        dev.record(1)
+        dev.synchronize()
+        return dev.elapsed_ms(0, 1) / reps
+
+    eager_ms = timed(eager)
+    want = [hashlib.sha256(f["costs"].numpy().tobytes()).hexdigest() for f in frames]
+    queue = gjxl_amd.CostQueue(dev, len(frames))
+    queue.tune(frames, repetitions=2)
+    queue_ms = timed(lambda: queue.run(frames))
+    same = all(hashlib.sha256(f["costs"].numpy().tobytes()).hexdigest() == want[i]
+               for i, f in enumerate(frames))
+    grids, tuned = queue.grids()
+    queue.close()
+    for f in frames:
+        for a in f.values():
+            if hasattr(a, "free"):
+                a.free()
+    return {"frames": len(frames), "megapixels": round(pixels / 1e6, 2),
+            "per_frame_launches_ms": round(eager_ms, 3),
+            "tile_queue_graph_ms": round(queue_ms, 3),
+            "speedup": round(eager_ms / queue_ms, 3), "bit_exact": same,
+            "tuned_grids": grids, "tuned_stage_ms": [round(m, 4) for m in tuned]}
 
 
 def _dct_basis(n: int) -> np.ndarray:
@@ -646,7 +712,12 @@ def main() -> int:
         _batch(gjxl_amd, images[:2], args.batch_lanes, calls=1)  # warm-up
         alone_s, alone_counts = _batch(gjxl_amd, images, 1)
         lanes_s, lanes_counts = _batch(gjxl_amd, images, args.batch_lanes)
-        if alone_counts != lanes_counts:
+        queue_s, queue_counts = _batch(gjxl_amd, images, args.batch_lanes, tile_queue=True)
+        wide = max(args.batch_lanes, 6)
+        wide_lanes_s, wide_lanes_counts = _batch(gjxl_amd, images, wide)
+        wide_s, wide_counts = _batch(gjxl_amd, images, wide, tile_queue=True)
+        if not (alone_counts == lanes_counts == queue_counts == wide_counts ==
+                wide_lanes_counts):
             print("batch lanes disagree", file=sys.stderr)
             return 1
 
@@ -660,7 +731,13 @@ def main() -> int:
                  "note": "second encode call of the same images; the first also "
                          "creates each worker's plain path at the bounding extent",
                  "one_at_a_time": figure(alone_s),
-                 f"lanes_{args.batch_lanes}": figure(lanes_s)}
+                 f"lanes_{args.batch_lanes}": figure(lanes_s),
+                 f"tile_queue_lanes_{args.batch_lanes}": figure(queue_s),
+                 f"lanes_{wide}": figure(wide_lanes_s),
+                 f"tile_queue_lanes_{wide}": figure(wide_s),
+                 "candidate_costs": cost_queue_figures(gjxl_amd, dev, images),
+                 "candidate_costs_sides_256_to_1024": cost_queue_figures(
+                     gjxl_amd, dev, batch_workload(args.batch_images, seed=5, largest=1024))}
 
     split = None
     if args.split_distances:

@@ -371,6 +371,9 @@ __device__ __forceinline__ void CostWork(
   __shared__ int valid[K];
   __shared__ int bxs[K], bys[K];
 
+  // the thread index's range, which a caller's opaque copy hides
+  __builtin_assume(tid >= 0);
+  __builtin_assume(tid < kCostThreads);
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   constexpr int kWaves = kCostThreads / 64;
@@ -716,11 +719,15 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
 
 // ---- the tile queue: candidate costs of a batch of frames -----------------
 //
-// Persistent work groups pull work items -- (frame, tile, chunk of anchor
-// positions) of one strategy -- from a counter in device memory until the
-// batch's items run out.  Frames are described in a table in device memory
-// (so one captured graph serves every batch); a plan kernel derives each
-// strategy's item offsets per frame and clears the counters.
+// Persistent work groups pull work items -- (64x64 tile, chunk of anchor
+// positions) of one strategy -- from counters in device memory until the
+// batch's items run out.  The batch's tiles (frames in order) are split
+// over the eight XCDs by tile index mod 8, one counter each, with a tile's
+// chunks consecutive: a work group (dispatched round-robin over the XCDs)
+// drains its own XCD's queue, where the tile's pixels stay in that XCD's
+// L2, then helps the others.  Frames are described in a table in device
+// memory (so one captured graph serves every batch); a plan kernel derives
+// the tile offsets of the frames and clears the counters.
 
 constexpr int kQueueMaxFrames = 256;
 
@@ -736,10 +743,12 @@ struct QueueFrame {
   CostParams params;  // entropy_multiplier set per strategy
 };
 
+constexpr int kQueueXcds = 8;
+
 struct QueueTable {
   uint32_t count;
-  uint32_t counters[7];
-  uint32_t offsets[7][kQueueMaxFrames + 1];
+  uint32_t counters[7][kQueueXcds];
+  uint32_t tile_offsets[kQueueMaxFrames + 1];  // first 64x64 tile of each frame
   QueueFrame frames[kQueueMaxFrames];
 };
 
@@ -754,19 +763,18 @@ __constant__ uint32_t kQueueStageChunks[7] = {
   QueueChunks<kDct16x16>(), QueueChunks<kDct32x16>(), QueueChunks<kDct16x32>(),
   QueueChunks<kDct32x32>()};
 
-// items per strategy: chunks x 64x64 tiles of each frame; counters cleared
+// tile offsets of the frames; counters cleared
 __global__ __launch_bounds__(64) void QueuePlanKernel(QueueTable* __restrict__ t) {
-  const int s = threadIdx.x;
-  if (s >= 7) return;
+  if (threadIdx.x < 7 * kQueueXcds) (&t->counters[0][0])[threadIdx.x] = 0;
+  if (threadIdx.x != 0) return;
   const uint32_t count = t->count;
   uint32_t at = 0;
   for (uint32_t f = 0; f < count; ++f) {
-    t->offsets[s][f] = at;
+    t->tile_offsets[f] = at;
     const int bw = t->frames[f].width / 8, bh = t->frames[f].height / 8;
-    at += kQueueStageChunks[s] * static_cast<uint32_t>(((bw + 7) / 8) * ((bh + 7) / 8));
+    at += static_cast<uint32_t>(((bw + 7) / 8) * ((bh + 7) / 8));
   }
-  t->offsets[s][count] = at;
-  t->counters[s] = 0;
+  t->tile_offsets[count] = at;
 }
 
 // NaN over every frame's seven cost planes
@@ -782,41 +790,65 @@ __global__ __launch_bounds__(256) void QueueFillKernel(const QueueTable* __restr
   }
 }
 
+// Two groups per CU (four waves per SIMD), which the LDS allows.
 template <int kStrategy>
-__global__ __launch_bounds__(kCostThreads) void CostQueueKernel(
+__global__ __launch_bounds__(kCostThreads, 4) void CostQueueKernel(
   QueueTable* __restrict__ t, const float* __restrict__ tables,
   const double* __restrict__ bases, float entropy_multiplier) {
   constexpr int kStage = StageOf(kStrategy);
   constexpr uint32_t kChunks = QueueChunks<kStrategy>();
   __shared__ uint32_t next;
   const uint32_t count = t->count;
-  const uint32_t total = t->offsets[kStage][count];
-  for (;;) {
+  const uint32_t tiles = t->tile_offsets[count];
+  const int home = static_cast<int>(blockIdx.x % kQueueXcds);
+  for (int k = 0; k < kQueueXcds; ++k) {
+    const int x = (home + k) % kQueueXcds;
+    // XCD x's queue: tiles x, x + 8, ... with their chunks consecutive
+    const uint32_t length =
+      tiles > static_cast<uint32_t>(x) ? (tiles - x + kQueueXcds - 1) / kQueueXcds * kChunks : 0;
+    // thread 0 claims the next item while the group works on the current
+    // one: the counter's device-scope atomic (a few microseconds) overlaps
+    // the item's own loads instead of preceding them
+    uint32_t claimed = 0;
+    // the frame of the last item and where the next frame's tiles begin:
+    // tiles grow along one queue, so the frame only moves forward, and its
+    // fields stay in registers until it does (no chain of dependent loads
+    // before each item's pixel reads)
+    int frame = 0;
+    uint32_t frame_end = count > 1 ? t->tile_offsets[1] : tiles;
+    QueueFrame f = t->frames[0];
+    uint32_t first_tile = 0;
     __syncthreads();  // every lane has read `next` and the last item's LDS
-    if (threadIdx.x == 0) next = atomicAdd(&t->counters[kStage], 1u);
-    __syncthreads();
-    // uniform: the frame's fields become scalar loads
-    const uint32_t item = __builtin_amdgcn_readfirstlane(next);
-    if (item >= total) break;  // the counter only grows: every group ends here
-    // the frame: the last whose first item is at most `item`
-    int lo = 0, hi = static_cast<int>(count) - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) / 2;
-      if (t->offsets[kStage][mid] <= item) lo = mid; else hi = mid - 1;
+    if (threadIdx.x == 0) next = atomicAdd(&t->counters[kStage][x], 1u);
+    for (;;) {
+      __syncthreads();
+      // uniform: the frame's fields become scalar loads
+      const uint32_t q = __builtin_amdgcn_readfirstlane(next);
+      if (q >= length) break;  // the counter only grows: every group ends here
+      if (threadIdx.x == 0) claimed = atomicAdd(&t->counters[kStage][x], 1u);
+      const uint32_t tile = static_cast<uint32_t>(x) + kQueueXcds * (q / kChunks);
+      if (tile >= frame_end) {
+        do {
+          ++frame;
+          frame_end = frame + 1 < static_cast<int>(count) ? t->tile_offsets[frame + 1] : tiles;
+        } while (tile >= frame_end);
+        f = t->frames[frame];
+        first_tile = t->tile_offsets[frame];
+      }
+      const TileIndex g{static_cast<int>(q % kChunks), static_cast<int>(tile - first_tile)};
+      CostParams params = f.params;
+      params.entropy_multiplier = entropy_multiplier;
+      const size_t stage = static_cast<size_t>(f.width / 8) * (f.height / 8);
+      // the thread index through a register move the compiler cannot hoist:
+      // the index arithmetic derived from it is redone per item instead of
+      // staying live across the whole loop (which took 160+ registers)
+      int tid;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(static_cast<int>(threadIdx.x)));
+      CostWork<kStrategy>(g, f.opsin, f.width, f.height, f.quant_field, f.pixel_mask, f.y_to_x,
+                          f.y_to_b, tables, bases, params, f.allow, f.costs + kStage * stage, tid);
+      __syncthreads();  // every lane has read `next` and the item's LDS
+      if (threadIdx.x == 0) next = claimed;
     }
-    const QueueFrame& f = t->frames[lo];
-    const uint32_t local = item - t->offsets[kStage][lo];
-    const TileIndex g{static_cast<int>(local % kChunks), static_cast<int>(local / kChunks)};
-    CostParams params = f.params;
-    params.entropy_multiplier = entropy_multiplier;
-    const size_t stage = static_cast<size_t>(f.width / 8) * (f.height / 8);
-    // the thread index through a register move the compiler cannot hoist:
-    // the index arithmetic derived from it is redone per item instead of
-    // staying live across the whole loop (which took 160+ registers)
-    int tid;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(static_cast<int>(threadIdx.x)));
-    CostWork<kStrategy>(g, f.opsin, f.width, f.height, f.quant_field, f.pixel_mask, f.y_to_x,
-                        f.y_to_b, tables, bases, params, f.allow, f.costs + kStage * stage, tid);
   }
 }
 
@@ -1142,7 +1174,7 @@ CostParams TargetCostParams(float target) {
 constexpr float kStageEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
 
 __global__ void QueueResetKernel(QueueTable* __restrict__ t, int stage) {
-  if (threadIdx.x == 0) t->counters[stage] = 0;
+  if (threadIdx.x < kQueueXcds) t->counters[stage][threadIdx.x] = 0;
 }
 
 GJXLResult LaunchQueueStage(GJXLAmdDevice* d, hipStream_t st, QueueTable* t, int stage,
