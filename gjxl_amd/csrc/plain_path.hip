@@ -140,7 +140,8 @@ void PathGeometry(GJXLAmdPlainPath* p, uint32_t width, uint32_t height, size_t* 
   z[kYToB] = tiles;
   z[kCosts] = 4 * 7 * blocks;
   z[kCells] = blocks;
-  z[kScoreWords] = 4 * (kMaxEvaluations + 1);  // scores, then the judged norm
+  // scores, the judged norm, then (8-byte aligned) the norm's partial sums
+  z[kScoreWords] = 4 * (kMaxEvaluations + 2) + 8 * gjxl_amd::kProposalNormPartials;
   z[kFeatures] = 2 * GJXL_AMD_PROPOSAL_FEATURES * blocks;
   z[kAllow] = blocks;
 }
@@ -311,7 +312,8 @@ GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
       // the judge: the 3-norm of the block distances; the learned run goes
       // on with the plain updates while it is above judge_ratio * distance
       float* norm = reinterpret_cast<float*>(p->score_words + kMaxEvaluations);
-      GJXL_AMD_TRY(gjxl_amd::LaunchProposalNorm3(d, distance, n, norm));
+      double* partial = reinterpret_cast<double*>(p->score_words + kMaxEvaluations + 2);
+      GJXL_AMD_TRY(gjxl_amd::LaunchProposalNorm3(d, distance, n, partial, norm));
       GJXL_AMD_TRY(Check(hipMemcpyAsync(p->host_words + 24, norm, 4, hipMemcpyDeviceToHost, st),
                          "hipMemcpyAsync(norm)"));
       GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));

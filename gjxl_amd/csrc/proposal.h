@@ -13,9 +13,12 @@ GJXLResult LaunchProposalDecide(GJXLAmdDevice* device, const float* head, int bw
 GJXLResult LaunchProposalWarmStart(GJXLAmdDevice* device, const float* head,
                                    const float* adjusted, size_t n, float limit, float offset,
                                    float lower, float upper, float* field);
-// (mean of d^3)^(1/3) of n block distances into *out (device).
+// Partial sums of the norm's work groups (device doubles).
+constexpr int kProposalNormPartials = 256;
+// (mean of d^3)^(1/3) of n block distances into *out (device); partial
+// holds kProposalNormPartials doubles.
 GJXLResult LaunchProposalNorm3(GJXLAmdDevice* device, const float* distances, size_t n,
-                               float* out);
+                               double* partial, float* out);
 GJXLResult LaunchProposalSample(GJXLAmdDevice* device, const uint8_t* cells, const float* field,
                                 const float* adjusted, size_t n, uint8_t* labels,
                                 float* targets);

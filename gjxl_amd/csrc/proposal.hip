@@ -21,7 +21,7 @@
 //   DecideKernel       allowed stages per block
 //   WarmStartKernel    the loop's starting field
 //   SampleKernel       labels and targets of a plain run
-//   Norm3Kernel        the 3-norm of the block distances (the judge)
+//   Norm3*Kernel       the 3-norm of the block distances (the judge)
 //
 // MFMA operands (16x16x16, 64 lanes): lane l holds A[l % 16][4 (l / 16) + i],
 // B[4 (l / 16) + i][l % 16] and D[4 (l / 16) + i][l % 16], i = 0..3.  In the
@@ -549,34 +549,57 @@ __global__ __launch_bounds__(256) void SampleKernel(const uint8_t* __restrict__ 
   targets[i] = logf(field[i] / adjusted[i]);
 }
 
-// (mean of d^3)^(1/3) over n block distances, one work group: each thread
-// sums a fixed stride of the blocks in binary64, then a fixed tree.
-constexpr int kNormThreads = 1024;
+// (mean of d^3)^(1/3) over n block distances: kNormGroups work groups each
+// sum a contiguous range in binary64 (fixed strides, a fixed tree), then one
+// group adds the partial sums in order.
+constexpr int kNormThreads = 256;
+constexpr int kNormGroups = 256;
 
-__global__ __launch_bounds__(kNormThreads) void Norm3Kernel(const float* __restrict__ d,
-                                                            size_t n, float* __restrict__ out) {
-  __shared__ double part[kNormThreads];
-  double s = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += kNormThreads) {
-    const double v = d[i];
-    s += v * v * v;
-  }
+__device__ __forceinline__ double BlockSum(double s, double* part) {
   part[threadIdx.x] = s;
   __syncthreads();
   for (int half = kNormThreads / 2; half > 0; half /= 2) {
     if (threadIdx.x < half) part[threadIdx.x] += part[threadIdx.x + half];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = static_cast<float>(cbrt(part[0] / static_cast<double>(n)));
+  return part[0];
 }
+
+__global__ __launch_bounds__(kNormThreads) void Norm3PartialKernel(const float* __restrict__ d,
+                                                                   size_t n,
+                                                                   double* __restrict__ partial) {
+  __shared__ double part[kNormThreads];
+  const size_t per = (n + kNormGroups - 1) / kNormGroups;
+  const size_t begin = blockIdx.x * per, end = begin + per < n ? begin + per : n;
+  double s = 0.0;
+  for (size_t i = begin + threadIdx.x; i < end; i += kNormThreads) {
+    const double v = d[i];
+    s += v * v * v;
+  }
+  const double total = BlockSum(s, part);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kNormThreads) void Norm3FinalKernel(const double* __restrict__ partial,
+                                                                 size_t n, float* __restrict__ out) {
+  __shared__ double part[kNormThreads];
+  const double total = BlockSum(threadIdx.x < kNormGroups ? partial[threadIdx.x] : 0.0, part);
+  if (threadIdx.x == 0) *out = static_cast<float>(cbrt(total / static_cast<double>(n)));
+}
+
+static_assert(kNormGroups <= kNormThreads, "one final group sums the partials");
+static_assert(kNormGroups == kProposalNormPartials, "the caller's partial buffer");
 
 }  // namespace
 
 GJXLResult LaunchProposalNorm3(GJXLAmdDevice* device, const float* distances, size_t n,
-                               float* out) {
-  hipLaunchKernelGGL(Norm3Kernel, dim3(1), dim3(kNormThreads), 0, device->stream, distances, n,
-                     out);
-  return CheckLaunch("Norm3Kernel");
+                               double* partial, float* out) {
+  hipLaunchKernelGGL(Norm3PartialKernel, dim3(kNormGroups), dim3(kNormThreads), 0,
+                     device->stream, distances, n, partial);
+  GJXL_AMD_TRY(CheckLaunch("Norm3PartialKernel"));
+  hipLaunchKernelGGL(Norm3FinalKernel, dim3(1), dim3(kNormThreads), 0, device->stream, partial,
+                     n, out);
+  return CheckLaunch("Norm3FinalKernel");
 }
 
 GJXLResult LaunchProposalFeatures(GJXLAmdDevice* device, const float* opsin, int width,

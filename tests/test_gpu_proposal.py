@@ -267,6 +267,9 @@ def test_learned_run_defaults_and_judge(device):
     assert path.frame.evaluations in (2, 3)
     norm = path.frame.proposal_norm
     assert 0.0 < norm and (path.frame.evaluations == 3 or norm <= 0.7)
+    # the judged norm is the 3-norm of the last evaluation's block distances
+    d = path.view("block_distance").numpy().astype(np.float64)
+    assert norm == pytest.approx(np.mean(d ** 3) ** (1.0 / 3.0), rel=1e-6)
     path.set_proposal(net, judge_ratio=1e-3)
     path.run(img)
     assert path.frame.evaluations == 3
