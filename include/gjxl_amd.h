@@ -835,14 +835,15 @@ GJXL_AMD_API GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* batch,
 GJXL_AMD_API void gjxl_amd_batch_results_free(GJXLAmdBatchResult* results,
                                               size_t count);
 /* The chosen batching form for this driver (no counterpart in the
- * reference): with enable non-zero, later encode calls take their images in
- * groups of max_in_flight, largest first; every worker runs its image's
- * front half (gjxl_amd_plain_path_run_front), one tile-queue graph
- * (gjxl_amd_cost_queue_*) computes the candidate costs of the whole group,
- * and every worker runs its back half.  Results, their order and their
- * values are those of the default mode.  tune_repetitions non-zero tunes
- * the queue's grids on the next call's first group.  Not while an encode
- * call is in progress. */
+ * reference): with enable non-zero, later encode calls take their images
+ * largest first; every worker runs its image's front half
+ * (gjxl_amd_plain_path_run_front) and submits its cost inputs, and whenever
+ * no worker is still in a front half, one launch of a tile-queue graph
+ * (gjxl_amd_cost_queue_*, on a stream of its own) computes the candidate
+ * costs of every submitted image; each worker then runs its back half.
+ * Results, their order and their values are those of the default mode.
+ * tune_repetitions non-zero tunes the queue's grids at the next launch.
+ * Not while an encode call is in progress. */
 GJXL_AMD_API GJXLResult gjxl_amd_batch_set_tile_queue(GJXLAmdBatch* batch, int32_t enable,
                                                       uint32_t tune_repetitions);
 GJXL_AMD_API void gjxl_amd_batch_shutdown(GJXLAmdBatch* batch);
@@ -967,9 +968,9 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
  * the allowed stages and the judged norm. */
 typedef struct {
   size_t struct_size;
-  float threshold;     /* block confidence, default 0.5 */
+  float threshold;     /* block confidence, default 0.3 */
   uint32_t top_k;      /* default 2 */
-  float p_min;         /* default 0.1 */
+  float p_min;         /* default 0.2 */
   int32_t evaluations; /* default 2 */
   float field_limit;   /* default 2 */
   float field_offset;  /* log offset of the predicted field, default 0 */
