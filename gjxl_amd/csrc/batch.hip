@@ -12,10 +12,12 @@
 // inside those buffers.
 //
 // With the tile queue on (gjxl_amd_batch_set_tile_queue, the chosen batching
-// form), the workers take the call's images in groups of max_in_flight,
-// largest first: each runs its image's front half, then one worker launches
-// the queue's captured graph for the candidate costs of the whole group, and
-// each runs its back half once the costs are on its stream (an event).  No
+// form), the workers take the call's images largest first and submit each
+// image's cost inputs after its front half; whenever no worker is still in a
+// front half, the submitted frames' candidate costs go through one launch of
+// the queue's captured graph on the queue's own stream, and each worker runs
+// its back half once its costs are on its stream (an event).  A worker waits
+// only for the others' front halves, never for their back halves.  No
 // kernels here.
 #include <algorithm>
 #include <atomic>
@@ -219,104 +221,103 @@ struct GJXLAmdBatch {
   Call* call = nullptr;
   uint64_t generation = 0;
   bool closed = false, stopping = false, joined = false;
-  // the tile queue (on worker 0's device) and the group hand-over
+  // the tile queue on its own device handle (stream) and the hand-over:
+  // per worker its frame, the events of its front half and of its costs,
+  // and whether its submitted frame has been launched (and ran)
+  GJXLAmdDevice* queue_device = nullptr;
   GJXLAmdCostQueue* queue = nullptr;
-  uint32_t tune_repetitions = 0;  // tune on the next group when non-zero
+  uint32_t tune_repetitions = 0;  // tune at the next launch when non-zero
   std::vector<GJXLAmdCostFrame> frames;
-  std::vector<char> front_ok;
-  std::vector<hipEvent_t> front_done;
-  hipEvent_t costs_done = nullptr;
-  bool queue_ok = false;
-  std::mutex group_mutex;
+  std::vector<hipEvent_t> front_done, costs_done;
+  std::vector<char> launched, queue_ok;
+  std::mutex group_mutex;  // guards the hand-over state and the queue
   std::condition_variable group_cv;
-  size_t group_arrived = 0;
-  uint64_t group_phase = 0;
+  std::vector<size_t> pending;  // submitted workers, not yet launched
+  size_t in_front = 0;          // workers between taking an image and submitting
+  uint64_t launches = 0;
 };
 
 namespace {
 
-// Every worker arrives once per group step.
-void GroupBarrier(GJXLAmdBatch* b) {
-  std::unique_lock<std::mutex> lock(b->group_mutex);
-  const uint64_t phase = b->group_phase;
-  if (++b->group_arrived == b->workers.size()) {
-    b->group_arrived = 0;
-    ++b->group_phase;
-    b->group_cv.notify_all();
-  } else {
-    b->group_cv.wait(lock, [&] { return b->group_phase != phase; });
-  }
-}
-
-// Worker 0, between the barriers: the group's candidate costs on its
-// stream, after every front half; b->queue_ok says whether they ran.
-void LaunchGroupQueue(GJXLAmdBatch* b, Worker* leader) {
-  b->queue_ok = false;
+// Under group_mutex: the pending frames' candidate costs through one launch
+// of the queue's graph, after their front halves (events); each launched
+// worker's costs event is recorded after it.
+void LaunchPending(GJXLAmdBatch* b) {
+  hipStream_t st = b->queue_device->stream;
   std::vector<GJXLAmdCostFrame> frames;
-  for (size_t k = 0; k < b->workers.size(); ++k) {
-    if (!b->front_ok[k]) continue;
-    if (hipStreamWaitEvent(leader->device->stream, b->front_done[k], 0) != hipSuccess) return;
+  bool ok = true;
+  for (size_t k : b->pending) {
+    ok = ok && hipStreamWaitEvent(st, b->front_done[k], 0) == hipSuccess;
     frames.push_back(b->frames[k]);
   }
-  if (frames.empty()) return;
   const uint32_t n = static_cast<uint32_t>(frames.size());
-  if (b->tune_repetitions != 0) {
-    const GJXLResult tuned = gjxl_amd_cost_queue_tune(b->queue, frames.data(), n,
-                                                      b->tune_repetitions);
+  if (ok && b->tune_repetitions != 0) {
+    ok = gjxl_amd_cost_queue_tune(b->queue, frames.data(), n, b->tune_repetitions) == GJXL_OK;
     b->tune_repetitions = 0;
-    if (tuned != GJXL_OK) return;
   }
-  if (gjxl_amd_cost_queue_run(b->queue, frames.data(), n) != GJXL_OK) return;
-  b->queue_ok = hipEventRecord(b->costs_done, leader->device->stream) == hipSuccess;
+  ok = ok && gjxl_amd_cost_queue_run(b->queue, frames.data(), n) == GJXL_OK;
+  for (size_t k : b->pending) {
+    b->queue_ok[k] = ok && hipEventRecord(b->costs_done[k], st) == hipSuccess;
+    b->launched[k] = 1;
+  }
+  b->pending.clear();
+  ++b->launches;
+  b->group_cv.notify_all();
 }
 
-// A grouped call: group g holds order[g * W .. g * W + W - 1]; worker j
-// takes slot j of every group.  A worker whose front half failed (or has no
-// image) still takes part in both barriers; one whose front half ran but
-// whose group's queue did not computes its own costs.
-void RunGroups(GJXLAmdBatch* b, Worker* w, Call* call) {
-  const size_t width = b->workers.size();
-  const size_t groups = (call->count + width - 1) / width;
-  for (size_t g = 0; g < groups; ++g) {
-    const size_t slot = g * width + w->index;
-    const bool has = slot < call->count;
-    const size_t i = has ? call->order[slot] : 0;
-    GJXLAmdBatchResult* r = has ? &(*call->staged)[i] : nullptr;
-    Clock::time_point start;
+// A call in tile-queue mode: images taken in call->order (largest first).
+void RunQueued(GJXLAmdBatch* b, Worker* w, Call* call) {
+  const size_t j = w->index;
+  for (;;) {
+    size_t slot;
+    {
+      std::lock_guard<std::mutex> lock(b->group_mutex);
+      slot = call->next.fetch_add(1);
+      if (slot >= call->count) break;
+      ++b->in_front;
+    }
+    const size_t i = call->order[slot];
+    GJXLAmdBatchResult* r = &(*call->staged)[i];
+    const Clock::time_point start = Clock::now();
     bool front = false;
-    if (has) {
-      start = Clock::now();
-      if (PrepareImage(w, *call, call->requests[i], r) == GJXL_OK) {
-        const GJXLResult status = gjxl_amd_plain_path_run_front(w->path, &call->requests[i].image);
-        if (status == GJXL_OK) {
-          front = gjxl_amd_plain_path_cost_frame(w->path, &b->frames[w->index]) == GJXL_OK &&
-                  hipEventRecord(b->front_done[w->index], w->device->stream) == hipSuccess;
-        } else {
-          DropPath(w);
-          SetFailure(r, status);
-        }
+    if (PrepareImage(w, *call, call->requests[i], r) == GJXL_OK) {
+      const GJXLResult status = gjxl_amd_plain_path_run_front(w->path, &call->requests[i].image);
+      if (status == GJXL_OK) {
+        front = gjxl_amd_plain_path_cost_frame(w->path, &b->frames[j]) == GJXL_OK &&
+                hipEventRecord(b->front_done[j], w->device->stream) == hipSuccess;
+      } else {
+        DropPath(w);
+        SetFailure(r, status);
       }
     }
-    b->front_ok[w->index] = front;
-    GroupBarrier(b);
-    if (w->index == 0) LaunchGroupQueue(b, w);
-    GroupBarrier(b);
+    bool ready = false;
+    {
+      std::unique_lock<std::mutex> lock(b->group_mutex);
+      --b->in_front;
+      if (front) {
+        b->launched[j] = 0;
+        b->pending.push_back(j);
+      }
+      // the last worker out of its front half launches what is pending
+      if (!b->pending.empty() && b->in_front == 0) LaunchPending(b);
+      if (front) {
+        b->group_cv.wait(lock, [&] { return b->launched[j] != 0; });
+        ready = b->queue_ok[j] != 0;
+      }
+    }
     if (front) {
-      const bool ready = b->queue_ok &&
-                         hipStreamWaitEvent(w->device->stream, b->costs_done, 0) == hipSuccess;
+      ready = ready && hipStreamWaitEvent(w->device->stream, b->costs_done[j], 0) == hipSuccess;
       GJXLAmdPlainFrame frame{};
       const GJXLResult status = gjxl_amd_plain_path_run_back(w->path, ready ? 1 : 0, nullptr,
                                                              &frame);
       CollectImage(w, status, frame, r);
     }
-    if (has) {
-      const Clock::time_point end = Clock::now();
-      r->queue_nanoseconds = Nanoseconds(start - call->arrival);
-      r->service_nanoseconds = Nanoseconds(end - start);
-      r->ready_nanoseconds = r->queue_nanoseconds + r->service_nanoseconds;
-      std::lock_guard<std::mutex> lock(b->mutex);
-      if (++call->done == call->count) b->finished.notify_all();
-    }
+    const Clock::time_point end = Clock::now();
+    r->queue_nanoseconds = Nanoseconds(start - call->arrival);
+    r->service_nanoseconds = Nanoseconds(end - start);
+    r->ready_nanoseconds = r->queue_nanoseconds + r->service_nanoseconds;
+    std::lock_guard<std::mutex> lock(b->mutex);
+    if (++call->done == call->count) b->finished.notify_all();
   }
 }
 
@@ -332,7 +333,7 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
       seen = b->generation;
     }
     if (call->grouped) {
-      RunGroups(b, w, call);
+      RunQueued(b, w, call);
       continue;
     }
     for (;;) {
@@ -354,12 +355,14 @@ void WorkerLoop(GJXLAmdBatch* b, Worker* w) {
 void DropQueue(GJXLAmdBatch* b) {
   gjxl_amd_cost_queue_destroy(b->queue);
   b->queue = nullptr;
-  for (hipEvent_t& e : b->front_done) {
-    if (e != nullptr) (void)hipEventDestroy(e);
+  for (auto* events : {&b->front_done, &b->costs_done}) {
+    for (hipEvent_t& e : *events) {
+      if (e != nullptr) (void)hipEventDestroy(e);
+    }
+    events->clear();
   }
-  b->front_done.clear();
-  if (b->costs_done != nullptr) (void)hipEventDestroy(b->costs_done);
-  b->costs_done = nullptr;
+  gjxl_amd_device_destroy(b->queue_device);
+  b->queue_device = nullptr;
 }
 
 void JoinAndRelease(GJXLAmdBatch* b) {
@@ -444,7 +447,7 @@ GJXLResult gjxl_amd_batch_encode(GJXLAmdBatch* b, const GJXLAmdBatchRequest* req
   call.staged = &staged;
   call.arrival = arrival;
   if (b->queue != nullptr) {
-    // groups of similar extents, largest first (results keep request order)
+    // largest first (results keep request order)
     try {
       call.order.resize(count);
     } catch (...) {
@@ -496,22 +499,28 @@ GJXLResult gjxl_amd_batch_set_tile_queue(GJXLAmdBatch* b, int32_t enable,
   DropQueue(b);
   if (!enable) return GJXL_OK;
   const size_t n = b->workers.size();
-  GJXLResult r = gjxl_amd_cost_queue_create(b->workers[0]->device, static_cast<uint32_t>(n),
-                                            &b->queue);
+  GJXLResult r = gjxl_amd_device_create(b->ordinal, &b->queue_device);
+  if (r == GJXL_OK) {
+    r = gjxl_amd_cost_queue_create(b->queue_device, static_cast<uint32_t>(n), &b->queue);
+  }
   try {
     b->frames.assign(n, GJXLAmdCostFrame{});
-    b->front_ok.assign(n, 0);
     b->front_done.assign(n, nullptr);
+    b->costs_done.assign(n, nullptr);
+    b->launched.assign(n, 0);
+    b->queue_ok.assign(n, 0);
+    b->pending.clear();
+    b->pending.reserve(n);
   } catch (...) {
-    r = Fail(GJXL_ERROR_OUT_OF_MEMORY, "tile queue staging failed");
+    if (r == GJXL_OK) r = Fail(GJXL_ERROR_OUT_OF_MEMORY, "tile queue staging failed");
   }
   for (size_t i = 0; r == GJXL_OK && i < n; ++i) {
     r = gjxl_amd::Check(hipEventCreateWithFlags(&b->front_done[i], hipEventDisableTiming),
                         "hipEventCreate");
-  }
-  if (r == GJXL_OK) {
-    r = gjxl_amd::Check(hipEventCreateWithFlags(&b->costs_done, hipEventDisableTiming),
-                        "hipEventCreate");
+    if (r == GJXL_OK) {
+      r = gjxl_amd::Check(hipEventCreateWithFlags(&b->costs_done[i], hipEventDisableTiming),
+                          "hipEventCreate");
+    }
   }
   if (r != GJXL_OK) {
     DropQueue(b);

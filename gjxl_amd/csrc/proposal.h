@@ -23,4 +23,10 @@ GJXLResult LaunchProposalSample(GJXLAmdDevice* device, const uint8_t* cells, con
                                 const float* adjusted, size_t n, uint8_t* labels,
                                 float* targets);
 
+// The forward of gjxl_amd_proposal_forward on `stream` (the network's
+// buffers are shared: one forward at a time per network); *head (may be
+// null) receives the head's device address.
+GJXLResult ProposalForward(GJXLAmdProposal* net, hipStream_t stream, const uint16_t* features,
+                           uint32_t n, uint32_t h, uint32_t w, const float** head);
+
 }  // namespace gjxl_amd

@@ -680,6 +680,47 @@ size_t LossBlocks(size_t positions) {
 
 }  // namespace
 
+namespace gjxl_amd {
+
+GJXLResult ProposalForward(GJXLAmdProposal* net, hipStream_t st, const uint16_t* features,
+                           uint32_t n, uint32_t h, uint32_t w, const float** head) {
+  const size_t positions = static_cast<size_t>(n) * h * w;
+  if (features == nullptr || positions == 0 || positions > net->max_positions) {
+    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal forward: no input or too many positions");
+  }
+  const size_t plane = kChannels * net->max_positions;
+  const uint32_t tiles = static_cast<uint32_t>(
+    std::min<size_t>(static_cast<size_t>(n) * h * ((w + 15) / 16), 1u << 30));
+  const dim3 grid(std::min<uint32_t>(CeilDiv(tiles, kConvWaves), 2048));
+  const dim3 block(64 * kConvWaves);
+  const int ni = static_cast<int>(n), hi = static_cast<int>(h), wi = static_cast<int>(w);
+  hipLaunchKernelGGL((ConvKernel<kFeatures, true, true, false, false>), grid, block, 0, st,
+                     features, nullptr, net->wf + WeightOffset(0),
+                     net->params + ParamOffset(0) + LayerWeights(0), ni, hi, wi, net->act,
+                     nullptr);
+  GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
+  for (int L = 1; L < kLayers - 1; ++L) {
+    hipLaunchKernelGGL((ConvKernel<kChannels, true, true, false, false>), grid, block, 0, st,
+                       net->act + (L - 1) * plane, nullptr, net->wf + WeightOffset(L),
+                       net->params + ParamOffset(L) + LayerWeights(L), ni, hi, wi,
+                       net->act + L * plane, nullptr);
+    GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
+  }
+  hipLaunchKernelGGL((ConvKernel<kChannels, true, false, false, true>), grid, block, 0, st,
+                     net->act + (kLayers - 2) * plane, nullptr, net->wf + WeightOffset(kLayers - 1),
+                     net->params + ParamOffset(kLayers - 1) + LayerWeights(kLayers - 1), ni, hi,
+                     wi, nullptr, net->head);
+  GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
+  net->features = features;
+  net->n = n;
+  net->h = h;
+  net->w = w;
+  if (head != nullptr) *head = net->head;
+  return GJXL_OK;
+}
+
+}  // namespace gjxl_amd
+
 extern "C" {
 
 size_t gjxl_amd_proposal_parameter_count(void) { return gjxl_amd::kParams; }
@@ -826,41 +867,8 @@ GJXLResult gjxl_amd_proposal_features(GJXLAmdDevice* device, const float* opsin,
 
 GJXLResult gjxl_amd_proposal_forward(GJXLAmdProposal* net, const uint16_t* features,
                                      uint32_t n, uint32_t h, uint32_t w) {
-  using namespace gjxl_amd;
   GJXL_AMD_TRY(ValidateNet(net));
-  const size_t positions = static_cast<size_t>(n) * h * w;
-  if (features == nullptr || positions == 0 || positions > net->max_positions) {
-    return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal forward: no input or too many positions");
-  }
-  hipStream_t st = net->device->stream;
-  const size_t plane = kChannels * net->max_positions;
-  const uint32_t tiles = static_cast<uint32_t>(
-    std::min<size_t>(static_cast<size_t>(n) * h * ((w + 15) / 16), 1u << 30));
-  const dim3 grid(std::min<uint32_t>(CeilDiv(tiles, kConvWaves), 2048));
-  const dim3 block(64 * kConvWaves);
-  const int ni = static_cast<int>(n), hi = static_cast<int>(h), wi = static_cast<int>(w);
-  hipLaunchKernelGGL((ConvKernel<kFeatures, true, true, false, false>), grid, block, 0, st,
-                     features, nullptr, net->wf + WeightOffset(0),
-                     net->params + ParamOffset(0) + LayerWeights(0), ni, hi, wi, net->act,
-                     nullptr);
-  GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
-  for (int L = 1; L < kLayers - 1; ++L) {
-    hipLaunchKernelGGL((ConvKernel<kChannels, true, true, false, false>), grid, block, 0, st,
-                       net->act + (L - 1) * plane, nullptr, net->wf + WeightOffset(L),
-                       net->params + ParamOffset(L) + LayerWeights(L), ni, hi, wi,
-                       net->act + L * plane, nullptr);
-    GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
-  }
-  hipLaunchKernelGGL((ConvKernel<kChannels, true, false, false, true>), grid, block, 0, st,
-                     net->act + (kLayers - 2) * plane, nullptr, net->wf + WeightOffset(kLayers - 1),
-                     net->params + ParamOffset(kLayers - 1) + LayerWeights(kLayers - 1), ni, hi,
-                     wi, nullptr, net->head);
-  GJXL_AMD_TRY(CheckLaunch("ConvKernel"));
-  net->features = features;
-  net->n = n;
-  net->h = h;
-  net->w = w;
-  return GJXL_OK;
+  return gjxl_amd::ProposalForward(net, net->device->stream, features, n, h, w, nullptr);
 }
 
 GJXLResult gjxl_amd_proposal_backward(GJXLAmdProposal* net, const uint8_t* labels,
