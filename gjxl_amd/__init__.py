@@ -86,6 +86,8 @@ _SIGNATURES = {
     "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
     "gjxl_amd_ac_strategy_costs_pruned":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P, _P],
+    "gjxl_amd_rate_control_options_init": [_P, _SZ],
+    "gjxl_amd_rate_control_search": [_P, _P, _P, _P],
     "gjxl_amd_cost_queue_create": [_P, _U32, ctypes.POINTER(_P)],
     "gjxl_amd_cost_queue_destroy": [_P],
     "gjxl_amd_cost_queue_run": [_P, _P, _U32],

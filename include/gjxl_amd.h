@@ -574,6 +574,62 @@ GJXL_AMD_API GJXLResult gjxl_amd_coefficient_order_tokens(
   size_t capacity, size_t* count);
 
 /* ---------------------------------------------------------------------- */
+/* f3: rate control.  Replaces SearchTargetSize (src/codestream/rate_control */
+/* .cpp:246-337, src/codestream/rate_control_internal.h:35-86), which       */
+/* gjxl's workflow drives for a target file size (src/codestream/workflow.  */
+/* cpp:1683-1835): a bounded search over the Butteraugli target (distance)  */
+/* by complete encodes, the lower and upper targets first, then midpoints   */
+/* of the widest remaining interval.  The caller's evaluator encodes at one */
+/* target and reports the size; the search keeps the best candidate by the  */
+/* reference's selection rules.  Host code; nothing is enqueued.            */
+/* ---------------------------------------------------------------------- */
+enum {
+  GJXL_AMD_RATE_LARGEST_AT_OR_BELOW = 0,  /* TargetSizeSelectionPolicy */
+  GJXL_AMD_RATE_CLOSEST_ABSOLUTE = 1
+};
+
+typedef struct {
+  size_t struct_size;
+  uint64_t target_bytes;     /* > 0 */
+  uint64_t tolerance_bytes;
+  uint32_t maximum_attempts; /* 1-64, default 12 */
+  int32_t selection;         /* default largest at or below */
+  float minimum_target;      /* default 0.01 */
+  float maximum_target;      /* default 10 */
+} GJXLAmdRateControlOptions;
+
+/* One encode's outcome.  final_score breaks size ties (lower wins) only
+ * when score_evaluated is non-zero and resident is zero: the score of a
+ * resident GPU loop is a diagnostic in the reference. */
+typedef struct {
+  uint64_t encoded_bytes;    /* > 0 */
+  int32_t score_evaluated;
+  int32_t resident;
+  double final_score;        /* finite when evaluated */
+} GJXLAmdRateCandidate;
+
+/* Encodes at butteraugli_target into *out.  A failure is local to that
+ * candidate, except GJXL_ERROR_RESOURCE_PLAN_EXCEEDED, which ends the
+ * search with that code. */
+typedef GJXLResult (*GJXLAmdRateEvaluator)(void* user, float butteraugli_target,
+                                           GJXLAmdRateCandidate* out);
+
+typedef struct {
+  float selected_target;
+  GJXLAmdRateCandidate selected;
+  uint32_t attempts, failed_attempts;
+  int32_t target_size_met, search_exhausted;
+} GJXLAmdRateControlResult;
+
+GJXL_AMD_API GJXLResult gjxl_amd_rate_control_options_init(GJXLAmdRateControlOptions* options,
+                                                           size_t struct_size);
+/* On failure *result is left unchanged. */
+GJXL_AMD_API GJXLResult gjxl_amd_rate_control_search(const GJXLAmdRateControlOptions* options,
+                                                     GJXLAmdRateEvaluator evaluator,
+                                                     void* user,
+                                                     GJXLAmdRateControlResult* result);
+
+/* ---------------------------------------------------------------------- */
 /* Chosen batching form (not the plain path; no counterpart in the          */
 /* reference, whose batching is host-level: src/codestream/batch_workflow.h */
 /* :47-109): the candidate costs of a batch of frames of uneven extents as  */
