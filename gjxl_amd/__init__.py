@@ -86,6 +86,7 @@ _SIGNATURES = {
     "gjxl_amd_ac_strategy_search": [_P, _P, _U32, _U32, _F, _P],
     "gjxl_amd_ac_strategy_costs_pruned":
         [_P, _P, _U32, _U32, _P, _P, _P, _P, _F, _P, _P],
+    "gjxl_amd_sparse_ac": [_P, _P, _U32, _U32, _P, _P, _P, _P],
     "gjxl_amd_rate_control_options_init": [_P, _SZ],
     "gjxl_amd_rate_control_search": [_P, _P, _P, _P],
     "gjxl_amd_cost_queue_create": [_P, _U32, ctypes.POINTER(_P)],
@@ -796,6 +797,27 @@ def gpu_available() -> bool:
 
 
 from . import pipeline  # noqa: E402,F401  (stage sequencing over this API)
+
+
+# ---- f2: sparse AC read-back (gjxl_amd_sparse_ac) ----------------------------------
+
+def sparse_ac(device: Device, source: DeviceArray, count: int | None = None):
+    """The sparse form of `count` values of `source` (int8, int16 or int32):
+    (masks uint64, offsets uint32, values, total) read back to the host."""
+    width = source.dtype.itemsize
+    count = source.nbytes // width if count is None else count
+    words = (count + 63) // 64
+    masks = device.empty((max(words, 1),), np.uint64)
+    offsets = device.empty((max(words, 1),), np.uint32)
+    values = device.empty((max(count, 1),), source.dtype)
+    total = device.empty((1,), np.uint32)
+    _check(library().gjxl_amd_sparse_ac(device.handle, source.ptr, count, width, masks.ptr,
+                                        offsets.ptr, values.ptr, total.ptr))
+    n = int(total.numpy()[0])
+    out = (masks.numpy()[:words], offsets.numpy()[:words], values.numpy()[:n], n)
+    for a in (masks, offsets, values, total):
+        a.free()
+    return out
 
 
 # ---- the tile queue for batches of candidate costs (gjxl_amd_cost_queue_*) --------

@@ -574,6 +574,23 @@ GJXL_AMD_API GJXLResult gjxl_amd_coefficient_order_tokens(
   size_t capacity, size_t* count);
 
 /* ---------------------------------------------------------------------- */
+/* f2: sparse read-back of quantized AC coefficients.  Replaces             */
+/* LaunchCudaSparseAc (src/gpu/cuda/cuda_sparse_ac_kernels.h:11-17) in the  */
+/* format SparseCoefficientSpan reads (src/codec/sparse_coefficients.h:     */
+/* 90-107): for count values of `width` bytes (1, 2 or 4, signed) at       */
+/* source, masks[ceil(count / 64)] hold the nonzero positions of each 64   */
+/* values, offsets[] the absolute payload offset of each word's first      */
+/* nonzero, values[] the nonzero payloads (coefficient order within a      */
+/* word; ranges of different work groups in any order), *total their       */
+/* count.  values needs room for count payloads.  Unlike the reference,    */
+/* total is cleared here (by a kernel).  Enqueued; source and values must  */
+/* not overlap.                                                             */
+/* ---------------------------------------------------------------------- */
+GJXL_AMD_API GJXLResult gjxl_amd_sparse_ac(GJXLAmdDevice* device, const void* source,
+                                           uint32_t count, uint32_t width, uint64_t* masks,
+                                           uint32_t* offsets, void* values, uint32_t* total);
+
+/* ---------------------------------------------------------------------- */
 /* f3: rate control.  Replaces SearchTargetSize (src/codestream/rate_control */
 /* .cpp:246-337, src/codestream/rate_control_internal.h:35-86), which       */
 /* gjxl's workflow drives for a target file size (src/codestream/workflow.  */
