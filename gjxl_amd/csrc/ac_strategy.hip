@@ -329,17 +329,21 @@ struct CostShape {
 };
 
 // One work group's work: kCands anchor positions of one 64x64 tile
-// (group.y), positions group.x * kCands + k of the tile's raster list.  Only
-// positions whose footprint lies inside the tile are evaluated; the rest of
-// the cost plane is NaN (filled beforehand).
-template <int kStrategy>
+// (group.y), positions group.x * kCands + k of the tile's raster list, or
+// (kListed) entries group.x * kCands + k of a list of anchors (x | y << 16,
+// *list_count of them) from any tiles.  Only positions whose footprint lies
+// inside their tile are evaluated; the rest of the cost plane is NaN
+// (filled beforehand).  With allow, a candidate whose stage bit is clear at
+// its anchor costs +infinity, which the greedy merge never selects.
+template <int kStrategy, bool kListed = false>
 __device__ __forceinline__ void CostWork(
   const TileIndex group, const float* __restrict__ opsin, int width, int height,
   const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
   const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
   const float* __restrict__ tables, const double* __restrict__ bases,
   CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs,
-  const int tid) {
+  const int tid, const uint32_t* __restrict__ list = nullptr,
+  const uint32_t* __restrict__ list_count = nullptr) {
   using S = CostShape<kStrategy>;
   constexpr int W = S::kW, H = S::kH, N = S::kN, K = S::kCands;
   constexpr int KC = 3 * K;     // candidate-channel planes
@@ -370,6 +374,7 @@ __device__ __forceinline__ void CostWork(
   __shared__ float cfl[K][3];
   __shared__ int valid[K];
   __shared__ int bxs[K], bys[K];
+  __shared__ uint32_t listed[kListed ? K : 1];
 
   // the thread index's range, which a caller's opaque copy hides
   __builtin_assume(tid >= 0);
@@ -387,8 +392,24 @@ __device__ __forceinline__ void CostWork(
   const double* fvT = bases + BasisOffsetT(H);        // [y][v]
   const double* ih = bases + BasisOffset(W, false);   // [u][x]
   const double* iv = bases + BasisOffset(H, false);   // [v][y]
+  constexpr uint32_t kNoEntry = 0xffffffffu;
+  if constexpr (kListed) {
+    const uint32_t count = *list_count;
+    if (static_cast<uint32_t>(group.x) * K >= count) return;  // past the list
+    if (tid < K) {
+      const uint32_t at = static_cast<uint32_t>(group.x) * K + tid;
+      listed[tid] = at < count ? list[at] : kNoEntry;
+    }
+    __syncthreads();
+  }
   // candidate k of this work group: anchor block and whether it is launched
   auto candidate = [&](int k, int* bx, int* by) {
+    if constexpr (kListed) {
+      const uint32_t e = listed[k];
+      *bx = e == kNoEntry ? 0 : static_cast<int>(e & 0xffffu);
+      *by = e == kNoEntry ? 0 : static_cast<int>(e >> 16);
+      return e != kNoEntry;
+    }
     const int p = group.x * K + k;
     const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
     *bx = tx * 8 + lx;
@@ -397,7 +418,8 @@ __device__ __forceinline__ void CostWork(
   };
   // The learned form's pruning: a group none of whose candidates is allowed
   // at its anchor writes +infinity for them (never merged) and does no work.
-  if (allow != nullptr) {
+  // (Listed groups hold allowed candidates only.)
+  if (!kListed && allow != nullptr) {
     constexpr int kStage = StageOf(kStrategy);
     bool wanted = false;
 #pragma unroll
@@ -466,8 +488,12 @@ __device__ __forceinline__ void CostWork(
   // multipliers (ComputeQuantNormUnchecked, src/codec/ac_strategy.cpp:103-137)
   constexpr int kBlocks = S::kCw * S::kCh;
   float qv[kBlocks];
-  const int tile = ty * tiles_w + tx;
-  const int8_t yx = y_to_x[tile], yb = y_to_b[tile];
+  int8_t yx = 0, yb = 0;
+  if constexpr (!kListed) {
+    const int tile = ty * tiles_w + tx;
+    yx = y_to_x[tile];
+    yb = y_to_b[tile];
+  }
   bool ok = false;
   if (tid < K) {
     int bx, by;
@@ -475,6 +501,12 @@ __device__ __forceinline__ void CostWork(
     valid[tid] = ok;
     bxs[tid] = bx;
     bys[tid] = by;
+    if constexpr (kListed) {
+      // each listed candidate's own tile
+      const int tile = ok ? (by / 8) * tiles_w + bx / 8 : 0;
+      yx = y_to_x[tile];
+      yb = y_to_b[tile];
+    }
     const float* q = quant_field + static_cast<size_t>(ok ? by : 0) * bw + (ok ? bx : 0);
 #pragma unroll
     for (int dy = 0; dy < S::kCh; ++dy) {
@@ -698,8 +730,12 @@ __device__ __forceinline__ void CostWork(
     // std::pow(float, float) as the reference's libm evaluates it
     const float root8 = GlibcPowf(normalized, 1.0f / 8.0f);
     const float loss_cost = root8 * static_cast<float>(N) / qn[k];
-    costs[static_cast<size_t>(bys[k]) * bw + bxs[k]] =
-      entropy * params.entropy_multiplier + params.info_loss_multiplier * loss_cost;
+    const size_t at = static_cast<size_t>(bys[k]) * bw + bxs[k];
+    const bool allowed = kListed || allow == nullptr ||
+                         ((allow[at] >> StageOf(kStrategy)) & 1) != 0;
+    costs[at] = allowed ? entropy * params.entropy_multiplier +
+                            params.info_loss_multiplier * loss_cost
+                        : __builtin_inff();
   }
 }
 
@@ -715,6 +751,52 @@ __global__ __launch_bounds__(kCostThreads) void CandidateCostKernel(
   CostParams params, const uint8_t* __restrict__ allow, float* __restrict__ costs) {
   CostWork<kStrategy>(XcdTile(), opsin, width, height, quant_field, pixel_mask, y_to_x, y_to_b,
                       tables, bases, params, allow, costs, static_cast<int>(threadIdx.x));
+}
+
+// The allowed candidates of one strategy as a list of anchors (x | y << 16):
+// one thread per (tile, raster position); a fitting candidate whose stage
+// bit is clear at its anchor costs +infinity here.  Entries are appended in
+// wave order through one counter (any order: each candidate has its own
+// cost entry).
+template <int kStrategy>
+__global__ __launch_bounds__(256) void CandidateListKernel(
+  const uint8_t* __restrict__ allow, int bw, int bh, uint32_t* __restrict__ list,
+  uint32_t* __restrict__ count, float* __restrict__ costs) {
+  using S = CostShape<kStrategy>;
+  const int tiles_w = (bw + 7) / 8, tiles = tiles_w * ((bh + 7) / 8);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = i / S::kPositions, p = i % S::kPositions;
+  const int tx = tile % tiles_w, ty = tile / tiles_w;
+  const int lx = (p % S::kNx) * S::kStep, ly = (p / S::kNx) * S::kStep;
+  const int bx = tx * 8 + lx, by = ty * 8 + ly;
+  const bool fits = tile < tiles && lx + S::kCw <= min(8, bw - tx * 8) &&
+                    ly + S::kCh <= min(8, bh - ty * 8);
+  const size_t at = fits ? static_cast<size_t>(by) * bw + bx : 0;
+  const bool allowed = fits && ((allow[at] >> StageOf(kStrategy)) & 1) != 0;
+  if (fits && !allowed) costs[at] = __builtin_inff();
+  const uint64_t mask = __ballot(allowed);
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == 0 && mask != 0) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, 0);
+  if (allowed) {
+    const uint64_t below = lane == 0 ? 0 : mask & (~uint64_t{0} >> (64 - lane));
+    list[base + __popcll(below)] = static_cast<uint32_t>(bx) | (static_cast<uint32_t>(by) << 16);
+  }
+}
+
+// Listed candidates: group x takes entries x * kCands ... of the list.
+template <int kStrategy>
+__global__ __launch_bounds__(kCostThreads) void CandidateCostListKernel(
+  const float* __restrict__ opsin, int width, int height,
+  const float* __restrict__ quant_field, const float* __restrict__ pixel_mask,
+  const int8_t* __restrict__ y_to_x, const int8_t* __restrict__ y_to_b,
+  const float* __restrict__ tables, const double* __restrict__ bases,
+  CostParams params, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+  float* __restrict__ costs) {
+  CostWork<kStrategy, true>(TileIndex{static_cast<int>(blockIdx.x), 0}, opsin, width, height,
+                            quant_field, pixel_mask, y_to_x, y_to_b, tables, bases, params,
+                            nullptr, costs, static_cast<int>(threadIdx.x), list, count);
 }
 
 // ---- the tile queue: candidate costs of a batch of frames -----------------
@@ -1102,6 +1184,42 @@ GJXLResult LaunchCost(GJXLAmdDevice* device, const float* opsin, int width,
   return CheckLaunch("CandidateCostKernel");
 }
 
+// Entries of a strategy's candidate list: every raster position of every
+// tile.
+template <int kStrategy>
+size_t ListCapacity(int width, int height) {
+  return static_cast<size_t>(CeilDiv(width / 8, 8)) * CeilDiv(height / 8, 8) *
+         CostShape<kStrategy>::kPositions;
+}
+
+// The pruned costs of one strategy by list: the allowed anchors are listed
+// (CandidateListKernel, which also writes +infinity for the others), then a
+// grid as large as the list can be takes them kCands at a time; the groups
+// past the list's count exit at once.  *count is zero beforehand.
+template <int kStrategy>
+GJXLResult LaunchListedCost(GJXLAmdDevice* device, const float* opsin, int width, int height,
+                            const float* qf, const float* mask, const int8_t* yx,
+                            const int8_t* yb, const CostParams& params, const uint8_t* allow,
+                            uint32_t* list, uint32_t* count, float* stage_costs) {
+  using S = CostShape<kStrategy>;
+  const size_t capacity = ListCapacity<kStrategy>(width, height);
+  hipLaunchKernelGGL(CandidateListKernel<kStrategy>,
+                     dim3(CeilDiv(static_cast<uint32_t>(capacity), 256u)), dim3(256), 0,
+                     device->stream, allow, width / 8, height / 8, list, count, stage_costs);
+  GJXL_AMD_TRY(CheckLaunch("CandidateListKernel"));
+  const dim3 grid(CeilDiv(static_cast<uint32_t>(capacity), static_cast<uint32_t>(S::kCands)));
+  const bool probed = ProbeBegin(device, GJXL_AMD_KERNEL_CANDIDATE_COST);
+  hipLaunchKernelGGL(CandidateCostListKernel<kStrategy>, grid, dim3(kCostThreads), 0,
+                     device->stream, opsin, width, height, qf, mask, yx, yb,
+                     device->quant_tables, device->dct_bases, params, list, count, stage_costs);
+  ProbeEnd(device, probed);
+  return CheckLaunch("CandidateCostListKernel");
+}
+
+__global__ void ClearCountsKernel(uint32_t* __restrict__ counts, int n) {
+  if (static_cast<int>(threadIdx.x) < n) counts[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 GJXLResult LaunchInitialCfl(GJXLAmdDevice* device, const float* opsin,
@@ -1131,6 +1249,45 @@ GJXLResult LaunchCandidateCosts(GJXLAmdDevice* device, const float* opsin,
                      dim3(256), 0, device->stream, costs, 7 * stage);
   GJXL_AMD_TRY(CheckLaunch("FillNanKernel"));
   constexpr float kEntropy[7] = {1.0f, 1.21f, 1.21f, 1.34f, 1.49f, 1.49f, 1.48f};
+  const size_t capacities[6] = {
+    ListCapacity<kDct16x8>(width, height),  ListCapacity<kDct8x16>(width, height),
+    ListCapacity<kDct16x16>(width, height), ListCapacity<kDct32x16>(width, height),
+    ListCapacity<kDct16x32>(width, height), ListCapacity<kDct32x32>(width, height)};
+  size_t words = 8;
+  for (size_t c : capacities) words += c;
+  // List entries hold 16-bit block coordinates and the list grids 32-bit
+  // thread counts; larger frames (beyond any device's memory) take the
+  // pruned tile form below, which gives the same costs.
+  const bool listed = width / 8 <= 0xffff && height / 8 <= 0xffff && words < (size_t{1} << 31);
+  if (allow != nullptr && listed) {
+    // Pruned: DCT8 (allowed everywhere by the proposal's fallback) by tile,
+    // the larger transforms by list.  One stream-ordered scratch holds the
+    // six lists and their counts.
+    void* scratch = nullptr;
+    GJXL_AMD_TRY(DeviceAlloc(device, words * sizeof(uint32_t), &scratch));
+    uint32_t* counts = static_cast<uint32_t*>(scratch);
+    uint32_t* lists[6];
+    lists[0] = counts + 8;
+    for (int i = 1; i < 6; ++i) lists[i] = lists[i - 1] + capacities[i - 1];
+    hipLaunchKernelGGL(ClearCountsKernel, dim3(1), dim3(64), 0, device->stream, counts, 8);
+    GJXLResult r = CheckLaunch("ClearCountsKernel");
+    p.entropy_multiplier = kEntropy[0];
+    if (r == GJXL_OK) r = LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs);
+    p.entropy_multiplier = kEntropy[1];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct16x8>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[0], counts + 0, costs + 1 * stage);
+    p.entropy_multiplier = kEntropy[2];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct8x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[1], counts + 1, costs + 2 * stage);
+    p.entropy_multiplier = kEntropy[3];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct16x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[2], counts + 2, costs + 3 * stage);
+    p.entropy_multiplier = kEntropy[4];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct32x16>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[3], counts + 3, costs + 4 * stage);
+    p.entropy_multiplier = kEntropy[5];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct16x32>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[4], counts + 4, costs + 5 * stage);
+    p.entropy_multiplier = kEntropy[6];
+    if (r == GJXL_OK) r = LaunchListedCost<kDct32x32>(device, opsin, width, height, qf, mask, yx, yb, p, allow, lists[5], counts + 5, costs + 6 * stage);
+    DeviceFree(device, scratch);
+    return r;
+  }
   p.entropy_multiplier = kEntropy[0];
   GJXL_AMD_TRY(LaunchCost<kDct8>(device, opsin, width, height, qf, mask, yx, yb, p, allow, costs + 0 * stage));
   p.entropy_multiplier = kEntropy[1];

@@ -190,11 +190,15 @@ GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_costs(
   const float* quant_field, const float* pixel_mask, const int8_t* y_to_x,
   const int8_t* y_to_b, float butteraugli_target, float* costs);
 /* The learned form's pruned costs (no counterpart in the reference): as
- * gjxl_amd_ac_strategy_costs, but a work group whose candidates all have
- * their stage's bit clear in allow ((w/8)*(h/8) bytes, bit s = stage s of
- * the order above, from gjxl_amd_proposal_decide) is not evaluated and
- * writes +infinity for them, which the greedy merge never selects.  A null
- * allow evaluates every candidate. */
+ * gjxl_amd_ac_strategy_costs, but a candidate whose stage's bit is clear in
+ * allow at its anchor ((w/8)*(h/8) bytes, bit s = stage s of the order
+ * above, from gjxl_amd_proposal_decide) costs +infinity, which the greedy
+ * merge never selects; allowed candidates cost what
+ * gjxl_amd_ac_strategy_costs gives them, bit for bit.  Only the allowed
+ * ones are evaluated: the larger transforms' allowed anchors are listed on
+ * the device and evaluated several to a work group.  Scratch for the lists
+ * (about 4 bytes per candidate position) comes from the device's pool in
+ * stream order.  A null allow evaluates every candidate. */
 GJXL_AMD_API GJXLResult gjxl_amd_ac_strategy_costs_pruned(
   GJXLAmdDevice* device, const float* opsin, uint32_t width, uint32_t height,
   const float* quant_field, const float* pixel_mask, const int8_t* y_to_x,
@@ -1043,7 +1047,7 @@ typedef struct {
   size_t struct_size;
   float threshold;     /* block confidence, default 0.3 */
   uint32_t top_k;      /* default 2 */
-  float p_min;         /* default 0.2 */
+  float p_min;         /* default 0.1 */
   int32_t evaluations; /* default 2 */
   float field_limit;   /* default 2 */
   float field_offset;  /* log offset of the predicted field, default 0 */

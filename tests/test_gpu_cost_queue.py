@@ -5,6 +5,8 @@ graph of persistent tile-queue kernels; every frame's seven cost planes
 equal gjxl_amd_ac_strategy_costs_pruned's bit for bit (NaN entries
 included), on three replays with other batches and eager work between them,
 and again after the autotuner has re-captured the graph with its grids.
+The pruned per-frame costs themselves agree with the full evaluation at
+every allowed candidate.
 """
 import ctypes
 
@@ -83,6 +85,34 @@ def test_cost_queue_matches_per_frame_costs(device):
     queue.run(frames)
     _same(device, frames, want)
     queue.close()
+
+
+@pytest.mark.parametrize("w,h,seed,allow_seed", [(392, 264, 9, 1), (1000, 776, 6, 2),
+                                                  (136, 72, 7, 3), (2048, 1536, 11, 4)])
+def test_pruned_costs_follow_allow_bits(device, w, h, seed, allow_seed):
+    """Pruned costs: an allowed candidate's cost equals the full evaluation's
+    bit for bit, a fitting candidate whose stage bit is clear costs +inf, a
+    position that does not fit stays NaN; all bits set gives the full planes."""
+    f = _frame(device, w, h, seed, allow_seed=allow_seed)
+    if allow_seed % 2 == 0:  # DCT8 pruned too (the proposal never does this)
+        rng = np.random.default_rng(allow_seed + 100)
+        f["allow"] = device.array(rng.integers(0, 128, f["allow"].shape).astype(np.uint8))
+    print(f"pruned costs {w}x{h} seed {seed} allow {allow_seed}", flush=True)
+    pruned = _eager(device, f)
+    bits = f["allow"].numpy()
+    f_all = dict(f)
+    del f_all["allow"]
+    full = _eager(device, f_all)
+    f_all["allow"] = device.array(np.full(bits.shape, 0x7F, np.uint8))
+    assert np.array_equal(_eager(device, f_all), full)
+    fp = full.view(np.float32)
+    inf = np.array(np.inf, np.float32).view(np.uint32)
+    for s in range(7):
+        allowed = ((bits >> s) & 1).astype(bool)
+        fits = ~np.isnan(fp[s])
+        assert np.array_equal(pruned[s][allowed], full[s][allowed]), s
+        assert (pruned[s][fits & ~allowed] == inf).all(), s
+        assert np.isnan(pruned[s].view(np.float32)[~fits]).all(), s
 
 
 def test_cost_queue_refuses_bad_batches(device):
