@@ -73,7 +73,7 @@ constexpr int kCflSumBatches = kCflLaneCount / kCflSumBatch;
 // the read-ahead past a lane's last batch (at most four batches) stays
 // inside the LDS array
 static_assert((2 * 2 * 3 * 4 - 1) * (kCflLaneCount + 4) +
-              (kCflSumBatches + 4) * kCflSumBatch <= 4 * 4096, "read-ahead in bounds");
+              (kCflSumBatches + 4) * kCflSumBatch <= 3 * 4096, "read-ahead in bounds");
 
 // One work group per 64x64 colour tile (src/codec/chroma_from_luma.cpp:
 // 434-529): DCT8 of each block and channel, LLF cleared, scaled by the DCT8
@@ -84,18 +84,18 @@ static_assert((2 * 2 * 3 * 4 - 1) * (kCflLaneCount + 4) +
 // by the producer threads from values held in registers, a chunk at a time,
 // and summed by one thread per lane of the reference's 4-lane sums, in
 // element order, one chunk behind.
-__global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
+// Three work groups share a CU (48 KB of LDS and at most 168 VGPRs each).
+__global__ __launch_bounds__(kCflThreads, 3) void InitialCflKernel(
   const float* __restrict__ opsin, int width, int height,
   const float* __restrict__ tables, const double* __restrict__ bases,
   int8_t* __restrict__ y_to_x, int8_t* __restrict__ y_to_b) {
-  // DCT phase: X, Y, B coefficients, then a_x, b_x, a_b, b_b, in values;
-  // Newton phase: two buffers of terms[problem][kind][element] in the same
-  // memory
-  __shared__ __align__(16) float lds[4 * 4096];
+  // DCT phase: X, Y, B coefficients in values; Newton phase: two buffers
+  // of terms[problem][kind][element] in the same memory
+  __shared__ __align__(16) float lds[3 * 4096];
   float (*values)[4096] = reinterpret_cast<float (*)[4096]>(lds);
   float (*terms)[2][3][4][kCflTermStride] =
     reinterpret_cast<float (*)[2][3][4][kCflTermStride]>(lds);
-  static_assert(2 * 2 * 3 * 4 * kCflTermStride <= 4 * 4096, "term buffers fit");
+  static_assert(2 * 2 * 3 * 4 * kCflTermStride <= 3 * 4096, "term buffers fit");
   static_assert(kCflProducers % 4 == 0, "a producer's elements share one lane");
   __shared__ float lane_sum[2][3][4];
   __shared__ float state_x[2];
@@ -119,31 +119,28 @@ __global__ __launch_bounds__(kCflThreads) void InitialCflKernel(
     for (int y = 0; y < 8; ++y)
 #pragma unroll
       for (int x = 0; x < 8; ++x) p[y * 8 + x] = src[static_cast<size_t>(y) * width + x];
-    double hz[64];
+    // one column u at a time: its eight horizontal sums, then its eight
+    // coefficients (the same sums in the same order as row-wise passes)
 #pragma unroll
-    for (int y = 0; y < 8; ++y) {
+    for (int u = 0; u < 8; ++u) {
+      double hz[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int y = 0; y < 8; ++y) {
         double sum = 0.0;
 #pragma unroll
         for (int x = 0; x < 8; ++x) sum += static_cast<double>(p[y * 8 + x]) * basis[u * 8 + x];
-        hz[y * 8 + u] = sum;
+        hz[y] = sum;
       }
-    }
-    float coef[64];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int v = 0; v < 8; ++v) {
         double sum = 0.0;
 #pragma unroll
-        for (int y = 0; y < 8; ++y) sum += basis[v * 8 + y] * hz[y * 8 + u];
-        coef[CoefficientIndex<8, 8>(v, u)] = static_cast<float>(sum);
+        for (int y = 0; y < 8; ++y) sum += basis[v * 8 + y] * hz[y];
+        // the LLF coefficient is cleared
+        values[c][k * 64 + CoefficientIndex<8, 8>(v, u)] =
+          v == 0 && u == 0 ? 0.0f : static_cast<float>(sum);
       }
     }
-    coef[0] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 64; ++i) values[c][k * 64 + i] = coef[i];
   }
   if (tid < 2) {
     state_x[tid] = 0.0f;
