@@ -166,12 +166,17 @@ def _stage_spans(names: list[str]) -> dict[str, list[tuple[int, int]]]:
     return spans
 
 
+# kernel families whose HBM bytes per launch the counter summary reports
+TRAFFIC_KERNELS = ("CandidateCostKernel", "DcQuantKernel", "DifferencesKernel")
+
+
 def pmc_summary(fetch_csv: str, write_csv: str) -> dict:
-    """HBM bytes per AQ evaluation and per Butteraugli compare from two
-    rocprofv3 --pmc runs of `bench.py --calibration-copy --steps 1 --warmup 1
-    --no-cpu-baseline` (FETCH_SIZE, then WRITE_SIZE; both in KiB per
-    dispatch).  Each counter is scaled by what it reads for the calibration
-    copy (CALIBRATION_BYTES read and written)."""
+    """HBM bytes per AQ evaluation, per Butteraugli compare and per launch of
+    each TRAFFIC_KERNELS family from two rocprofv3 --pmc runs of `bench.py
+    --calibration-copy --steps 1 --warmup 1 --no-cpu-baseline` (FETCH_SIZE,
+    then WRITE_SIZE; both in KiB per dispatch).  Each counter is scaled by
+    what it reads for the calibration copy (CALIBRATION_BYTES read and
+    written)."""
     out = {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = _pmc_dispatches(path, counter)
@@ -183,15 +188,20 @@ def pmc_summary(fetch_csv: str, write_csv: str) -> dict:
             per = [sum(values[a:b + 1]) for a, b in spans]
             entry = out.setdefault(stage, {"launches": len(per)})
             entry[counter] = round(float(np.mean(per)) * 1024.0 * factor)
+        for family in TRAFFIC_KERNELS:
+            per = [v for name, v in zip(names, values) if family in name]
+            entry = out.setdefault(family, {"launches": len(per)})
+            entry[counter] = round(float(np.mean(per)) * 1024.0 * factor) if per else 0
         out.setdefault("calibration_factor", {})[counter] = round(factor, 4)
-    for stage in ("aq_evaluate", "butteraugli_compare"):
+    for stage in ("aq_evaluate", "butteraugli_compare") + TRAFFIC_KERNELS:
         out[stage]["traffic_bytes_per_launch"] = (out[stage]["FETCH_SIZE"] +
                                                   out[stage]["WRITE_SIZE"])
     return out
 
 
 def committed_traffic(stage: str) -> int | None:
-    """The per-launch HBM traffic of `stage` recorded in TRAFFIC_PROFILE."""
+    """The per-launch HBM traffic of `stage` (a stage or a kernel family)
+    recorded in TRAFFIC_PROFILE."""
     import re
     try:
         text = TRAFFIC_PROFILE.read_text()
@@ -758,7 +768,8 @@ def main() -> int:
         roofline = {"kernel": probe, "bound": "fp64 (VALU, no MFMA)",
                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                    "traffic": None}
+                    "traffic": committed_traffic(probe) if args.workload == "4032x3024"
+                    else None}
     else:
         achieved = units / (launch_ms * 1e-3) / 1e9
         roofline = {"kernel": probe, "bound": "hbm", "achieved": round(achieved, 1),
