@@ -247,6 +247,12 @@ GJXLResult RunFront(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
   }
   GJXL_AMD_TRY(gjxl_amd_initial_cfl(d, p->preprocessed, p->pw, p->ph, p->y_to_x, p->y_to_b));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_CFL));
+  // what is left of the reference side once the device stream gets here:
+  // the side stream shares the GPU with the latency-bound stages before the
+  // candidate costs, which the costs themselves would not leave room for
+  GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
+                     "hipStreamWaitEvent"));
+  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
   p->proposal_head = head;
   return GJXL_OK;
 }
@@ -290,11 +296,6 @@ GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
   } else {
     GJXL_AMD_TRY(gjxl_amd_copy(d, field, adjusted, 4 * n));
   }
-  // what is left of the reference side (second stream) once the device
-  // stream gets here; the first compare reads it
-  GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
-                     "hipStreamWaitEvent"));
-  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
   int it = 0;
   for (;; ++it) {
     GJXL_AMD_TRY(gjxl_amd_aq_evaluate(p->session, field));
