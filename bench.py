@@ -534,7 +534,9 @@ def learned_figures(gjxl_amd, dev, srgb: np.ndarray, steps: int) -> dict:
     rate-quality curve (distances 0.9-1.2; quality = 3-norm of the block
     distances, rate = token entropy), with the trained network and with none
     (zero parameters), grid agreement and the share of blocks left to the
-    plain search."""
+    plain search; for the trained network also the unjudged runs (last pass
+    quantized without a comparison), whose quality is that of the same run
+    comparing its last pass (same tokens, checked)."""
     from gjxl_amd import proposal as prop
 
     h, w = srgb.shape[:2]
@@ -572,19 +574,34 @@ def learned_figures(gjxl_amd, dev, srgb: np.ndarray, steps: int) -> dict:
     trained = prop.load_parameters()
     for name, params in (("trained", trained), ("none", np.zeros_like(trained))):
         net = prop.ProposalNet(dev, path.bw * path.bh, params)
-        for opts in ({"evaluations": 1, "judge_ratio": 100.0},
-                     {"evaluations": 2, "judge_ratio": 100.0}, {}):
+        runs = [{"evaluations": 1, "judge_ratio": 100.0},
+                {"evaluations": 2, "judge_ratio": 100.0}, {}]
+        if name == "trained":
+            runs += [{"evaluations": 1, "judge_ratio": 100.0, "final_compare": 0},
+                     {"evaluations": 2, "judge_ratio": 100.0, "final_compare": 0}]
+        for opts in runs:
             path.set_proposal(net, **opts)
             seconds = timed()
-            q, b = _rate_quality(path)
-            cells = path.view("cells").numpy()
-            allow = path.view("proposal_allow").numpy()
+            evaluations = int(path.frame.evaluations)
             stages = {}
             for stage, ms in path.stage_times():
                 stages[stage] = stages.get(stage, 0.0) + ms
+            if opts.get("final_compare") == 0:
+                # the unjudged last pass compares nothing: its quality is that
+                # of the same run comparing the last pass, whose tokens (and
+                # so field and quantization) are the same
+                tokens = path.tokens()
+                path.set_proposal(net, **dict(opts, final_compare=1))
+                path.run(resident=resident)
+                same = all(np.array_equal(tokens[k], path.tokens()[k]) for k in tokens)
+                if not same:
+                    raise RuntimeError("the unjudged last pass changed the tokens")
+            q, b = _rate_quality(path)
+            cells = path.view("cells").numpy()
+            allow = path.view("proposal_allow").numpy()
             out["runs"].append({
                 "network": name, "options": opts or "defaults",
-                "evaluations": int(path.frame.evaluations),
+                "evaluations": evaluations,
                 "value": round(mp / seconds, 3), "ms_per_image": round(seconds * 1e3, 3),
                 "speedup_vs_plain": round(plain_s / seconds, 3),
                 "quality_3norm": round(q, 5), "rate_excess_at_equal_quality": excess(q, b),

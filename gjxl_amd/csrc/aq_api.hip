@@ -213,6 +213,32 @@ GJXLResult ResizeAqSession(GJXLAmdAqSession* s, uint32_t width, uint32_t height)
   return ClearStores(s);
 }
 
+// The quantizing part of an evaluation: quantizer, final CfL, coefficient
+// coding and DC (what tokenization reads); the session is validated and
+// prepared.
+GJXLResult AqQuantize(GJXLAmdAqSession* s, const float* quant_field) {
+  GJXLAmdDevice* d = s->device;
+  hipStream_t st = d->stream;
+  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
+  uint32_t* select = Buf<uint32_t>(s, GJXL_AMD_AQ_SELECT);
+  auto* q = Buf<QuantizerState>(s, GJXL_AMD_AQ_QUANTIZER);
+  GJXL_AMD_TRY(LaunchQuantizer(st, quant_field, n, s->quant_dc, select, select + 4, q,
+                               Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT)));
+  GJXL_AMD_TRY(LaunchFinalCfl(d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph,
+                              s->cells, Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
+                              Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT), q,
+                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X),
+                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B)));
+  GJXL_AMD_TRY(LaunchCodeCoefficients(
+    d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph, s->cells,
+    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
+    Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q,
+    s->x_mult, s->b_mult, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED),
+    Buf<float>(s, GJXL_AMD_AQ_DC), Buf<int32_t>(s, GJXL_AMD_AQ_DC_QUANTIZED),
+    Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED)));
+  return GJXL_OK;
+}
+
 }  // namespace gjxl_amd
 
 extern "C" {
@@ -369,23 +395,8 @@ GJXLResult gjxl_amd_aq_evaluate(GJXLAmdAqSession* s, const float* quant_field) {
   using namespace gjxl_amd;
   GJXLAmdDevice* d = s->device;
   hipStream_t st = d->stream;
-  const size_t n = static_cast<size_t>(s->layout.bw) * s->layout.bh;
-  uint32_t* select = Buf<uint32_t>(s, GJXL_AMD_AQ_SELECT);
   auto* q = Buf<QuantizerState>(s, GJXL_AMD_AQ_QUANTIZER);
-  GJXL_AMD_TRY(LaunchQuantizer(st, quant_field, n, s->quant_dc, select, select + 4, q,
-                               Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT)));
-  GJXL_AMD_TRY(LaunchFinalCfl(d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph,
-                              s->cells, Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS),
-                              Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT), q,
-                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X),
-                              Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B)));
-  GJXL_AMD_TRY(LaunchCodeCoefficients(
-    d, Buf<float>(s, GJXL_AMD_AQ_COEFFICIENTS), s->pw, s->ph, s->cells,
-    Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),
-    Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_X), Buf<int8_t>(s, GJXL_AMD_AQ_Y_TO_B), q,
-    s->x_mult, s->b_mult, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED),
-    Buf<float>(s, GJXL_AMD_AQ_DC), Buf<int32_t>(s, GJXL_AMD_AQ_DC_QUANTIZED),
-    Buf<float>(s, GJXL_AMD_AQ_DC_RECONSTRUCTED)));
+  GJXL_AMD_TRY(AqQuantize(s, quant_field));
   GJXL_AMD_TRY(LaunchReconstruct(
     d, Buf<int32_t>(s, GJXL_AMD_AQ_QUANTIZED), s->pw, s->ph, s->cells,
     Buf<uint32_t>(s, GJXL_AMD_AQ_OFFSETS), Lists(s), Buf<int32_t>(s, GJXL_AMD_AQ_RAW_QUANT),

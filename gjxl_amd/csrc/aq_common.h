@@ -37,6 +37,10 @@ struct StrategyLists {
 // allocated; the session must be prepared again.  Clears the coefficient
 // stores of the new extent and waits.
 GJXLResult ResizeAqSession(GJXLAmdAqSession* s, uint32_t width, uint32_t height);
+// The quantizing part of gjxl_amd_aq_evaluate (quantizer, final CfL,
+// coefficient coding, DC) without reconstruction and loop filters; the
+// session is validated and prepared.
+GJXLResult AqQuantize(GJXLAmdAqSession* s, const float* quant_field);
 
 // The split-precision forward DCT of every transform of a frame into the
 // per-group coefficient store (split_dct.hip).

@@ -216,21 +216,27 @@ GJXLResult RunFront(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
                                                   p->linear, p->opsin));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_SRGB_TO_LINEAR_AND_OPSIN));
   // the Butteraugli reference side on the side stream, from the linear image
-  GJXL_AMD_TRY(Check(hipEventRecord(p->linear_ready, d->stream), "hipEventRecord"));
-  GJXL_AMD_TRY(Check(hipStreamWaitEvent(p->side->stream, p->linear_ready, 0),
-                     "hipStreamWaitEvent"));
-  p->butteraugli->device = p->side;
-  const GJXLResult prepared = gjxl_amd_butteraugli_prepare(p->butteraugli, p->linear);
-  p->butteraugli->device = d;
-  GJXL_AMD_TRY(prepared);
-  GJXL_AMD_TRY(Check(hipEventRecord(p->reference_ready, p->side->stream), "hipEventRecord"));
+  // (not needed by an unjudged learned run of one pass, which compares
+  // nothing)
+  const GJXLAmdProposalOptions& po = p->proposal_options;
+  const bool compares =
+    p->proposal == nullptr || po.final_compare != 0 || po.evaluations > 1;
+  if (compares) {
+    GJXL_AMD_TRY(Check(hipEventRecord(p->linear_ready, d->stream), "hipEventRecord"));
+    GJXL_AMD_TRY(Check(hipStreamWaitEvent(p->side->stream, p->linear_ready, 0),
+                       "hipStreamWaitEvent"));
+    p->butteraugli->device = p->side;
+    const GJXLResult prepared = gjxl_amd_butteraugli_prepare(p->butteraugli, p->linear);
+    p->butteraugli->device = d;
+    GJXL_AMD_TRY(prepared);
+    GJXL_AMD_TRY(Check(hipEventRecord(p->reference_ready, p->side->stream), "hipEventRecord"));
+  }
   const float ones[3] = {1.0f, 1.0f, 1.0f};
   GJXL_AMD_TRY(gjxl_amd_gaborish_inverse(d, p->opsin, p->pw, p->ph, ones, p->preprocessed));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_GABORISH_INVERSE));
   GJXL_AMD_TRY(gjxl_amd_initial_quant_field(d, p->opsin, p->pw, p->ph, t, 1.0f, p->quant_field,
                                             p->strategy_mask, p->pixel_mask, p->iqf_scratch));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_QUANT_FIELD));
-  const GJXLAmdProposalOptions& po = p->proposal_options;
   const float* head = nullptr;
   if (p->proposal != nullptr) {
     // the learned form: features, the network's forward and the allowed
@@ -250,9 +256,11 @@ GJXLResult RunFront(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
   // what is left of the reference side once the device stream gets here:
   // the side stream shares the GPU with the latency-bound stages before the
   // candidate costs, which the costs themselves would not leave room for
-  GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
-                     "hipStreamWaitEvent"));
-  GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
+  if (compares) {
+    GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
+                       "hipStreamWaitEvent"));
+    GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_BUTTERAUGLI_PREPARE));
+  }
   p->proposal_head = head;
   return GJXL_OK;
 }
@@ -296,8 +304,17 @@ GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
   } else {
     GJXL_AMD_TRY(gjxl_amd_copy(d, field, adjusted, 4 * n));
   }
+  // the unjudged learned run: its planned updates, then a last pass that
+  // only quantizes (what the tokens read), without a comparison
+  const bool unjudged = head != nullptr && po.final_compare == 0;
+  if (unjudged) p->proposal_norm = std::nanf("");
   int it = 0;
   for (;; ++it) {
+    if (unjudged && it == planned) {
+      GJXL_AMD_TRY(gjxl_amd::AqQuantize(p->session, field));
+      GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_EVALUATE));
+      break;
+    }
     GJXL_AMD_TRY(gjxl_amd_aq_evaluate(p->session, field));
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_EVALUATE));
     GJXL_AMD_TRY(gjxl_amd::ButteraugliCompare(p->butteraugli, decoded, nullptr));
@@ -324,7 +341,8 @@ GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_AQ_REDUCE_UPDATE));
     if (!more) break;
   }
-  p->evaluations_run = it + 1;
+  // evaluations with a comparison (the unjudged last pass has none)
+  p->evaluations_run = unjudged ? it : it + 1;
   // The CPU-chosen tokenization inputs of effort 7 (coefficient orders and
   // the adaptive block-context map) from the final frame's statistics, with
   // the AQ error flag and the scores in the same wait
@@ -341,8 +359,10 @@ GJXLResult RunBack(GJXLAmdPlainPath* p, bool costs_ready) {
                      "hipMemcpyAsync(frame statistics)"));
   GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 4, stats + 7, 4, hipMemcpyDeviceToHost, st),
                      "hipMemcpyAsync(AQ error)"));
-  GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * p->evaluations_run,
-                                    hipMemcpyDeviceToHost, st), "hipMemcpyAsync(scores)"));
+  if (p->evaluations_run > 0) {
+    GJXL_AMD_TRY(Check(hipMemcpyAsync(host + 8, p->score_words, 4 * p->evaluations_run,
+                                      hipMemcpyDeviceToHost, st), "hipMemcpyAsync(scores)"));
+  }
   GJXL_AMD_TRY(Check(hipStreamSynchronize(st), "hipStreamSynchronize"));
   if (host[4] != 0) {
     return Fail(GJXL_ERROR_INTERNAL, "Adaptive quantization produced an invalid block distance");
@@ -712,7 +732,7 @@ GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* p, GJXLAmdProposal
       !std::isfinite(o.p_min) || o.evaluations < 1 || o.evaluations > p->iterations + 1 ||
       !(o.field_limit >= 0.0f && o.field_limit <= 16.0f) || !std::isfinite(o.field_offset) ||
       !(o.judge_ratio > 0.0f) ||
-      !std::isfinite(o.judge_ratio)) {
+      !std::isfinite(o.judge_ratio) || (o.final_compare != 0 && o.final_compare != 1)) {
     return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal options out of range");
   }
   void* head = nullptr;

@@ -1007,6 +1007,7 @@ GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options, size_
   options->threshold = 0.3f;
   options->top_k = 2;
   options->p_min = 0.1f;
+  options->final_compare = 1;
   options->evaluations = 2;
   options->field_limit = 2.0f;
   options->field_offset = 0.0f;

@@ -52,7 +52,7 @@ _HEAD, _PARAMETERS, _GRADIENTS, _ACTIVATIONS = 0, 1, 2, 3
 class ProposalOptions(ctypes.Structure):
     _fields_ = [("struct_size", _SZ), ("threshold", _F), ("top_k", _U32), ("p_min", _F),
                 ("evaluations", ctypes.c_int32), ("field_limit", _F), ("field_offset", _F),
-                ("judge_ratio", _F)]
+                ("judge_ratio", _F), ("final_compare", ctypes.c_int32)]
 
     @classmethod
     def defaults(cls) -> "ProposalOptions":

@@ -1042,16 +1042,24 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
  * evaluations ran, the run goes on with the plain loop's updates (plain
  * runs of the synthetic workloads end at 0.60-0.64 times the distance
  * between distances 0.9 and 1.2).  The frame reports the evaluations run,
- * the allowed stages and the judged norm. */
+ * the allowed stages and the judged norm.  With final_compare zero the run
+ * is unjudged: evaluations - 1 evaluations with their updates, then a last
+ * pass that quantizes the updated field (the coefficients, DC and raw
+ * quantization the tokens read) without reconstruction, comparison or
+ * judge; the tokens equal those of a run that compares that pass too.  The
+ * frame then reports the comparisons run as its evaluations (none for
+ * evaluations 1, which also skips the Butteraugli reference side) and a
+ * NaN judged norm. */
 typedef struct {
   size_t struct_size;
-  float threshold;     /* block confidence, default 0.3 */
-  uint32_t top_k;      /* default 2 */
-  float p_min;         /* default 0.1 */
-  int32_t evaluations; /* default 2 */
-  float field_limit;   /* default 2 */
-  float field_offset;  /* log offset of the predicted field, default 0 */
-  float judge_ratio;   /* default 0.7 */
+  float threshold;       /* block confidence, default 0.3 */
+  uint32_t top_k;        /* default 2 */
+  float p_min;           /* default 0.1 */
+  int32_t evaluations;   /* default 2 */
+  float field_limit;     /* default 2 */
+  float field_offset;    /* log offset of the predicted field, default 0 */
+  float judge_ratio;     /* default 0.7 */
+  int32_t final_compare; /* 1 (default) or 0: the unjudged last pass above */
 } GJXLAmdProposalOptions;
 
 GJXL_AMD_API GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* options,

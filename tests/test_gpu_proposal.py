@@ -278,9 +278,44 @@ def test_learned_run_defaults_and_judge(device):
     small = prop.ProposalNet(device, 16, prop.load_parameters())
     with pytest.raises(gjxl_amd.GjxlError, match="fewer positions"):
         path.set_proposal(small)
+    with pytest.raises(gjxl_amd.GjxlError, match="INVALID_ARGUMENT"):
+        path.set_proposal(net, final_compare=2)
     path.set_proposal(None)
     path.run(img)
     assert path.frame.evaluations == 3 and not path.frame.proposal_allow
     small.close()
+    net.close()
+    path.close()
+
+
+def _same_tokens(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in ("values", "contexts", "group_offsets"))
+
+
+def test_unjudged_final_pass_keeps_the_tokens(device):
+    """final_compare 0: the last pass quantizes without reconstruction or a
+    comparison, and the tokens equal those of the same run comparing that
+    pass too (two passes, then one, which also skips the reference side);
+    a plain run afterwards still gives the plain tokens."""
+    w, h = 512, 512
+    img = gjxl_amd.synthetic.generate(**gjxl_amd.synthetic.WORKLOADS["512x512"])
+    path = gjxl_amd.NativePlainPath(device, w, h)
+    path.run(img)
+    plain = path.tokens()
+    net = prop.ProposalNet(device, path.bw * path.bh, prop.load_parameters())
+    for evaluations in (2, 1):
+        print(f"unjudged final pass: {evaluations} passes", flush=True)
+        path.set_proposal(net, evaluations=evaluations, judge_ratio=100.0)
+        path.run(img)
+        assert path.frame.evaluations == evaluations
+        compared = path.tokens()
+        path.set_proposal(net, evaluations=evaluations, judge_ratio=100.0, final_compare=0)
+        assert path.run(img) > 0
+        assert path.frame.evaluations == evaluations - 1
+        assert np.isnan(path.frame.proposal_norm)
+        assert _same_tokens(path.tokens(), compared)
+    path.set_proposal(None)
+    path.run(img)
+    assert _same_tokens(path.tokens(), plain)
     net.close()
     path.close()
