@@ -19,7 +19,9 @@
 // once for the quantization-matrix statistics that choose the matrix scales
 // (src/codestream/workflow.cpp:1318-1484), once for the frame statistics and
 // once for the token count.
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <new>
 
@@ -720,13 +722,16 @@ GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* p, GJXLAmdProposal
     return GJXL_OK;
   }
   GJXLAmdProposalOptions o{};
-  if (options == nullptr) {
-    GJXL_AMD_TRY(gjxl_amd_proposal_options_init(&o, sizeof(o)));
-  } else {
-    if (options->struct_size < sizeof(GJXLAmdProposalOptions)) {
+  GJXL_AMD_TRY(gjxl_amd_proposal_options_init(&o, sizeof(o)));
+  if (options != nullptr) {
+    // a caller built before final_compare passes the shorter struct: its
+    // fields over the defaults
+    constexpr size_t kFirstSize = offsetof(GJXLAmdProposalOptions, final_compare);
+    if (options->struct_size < kFirstSize) {
       return Fail(GJXL_ERROR_INVALID_ARGUMENT, "proposal options are too small");
     }
-    o = *options;
+    std::memcpy(&o, options, std::min(options->struct_size, sizeof(o)));
+    o.struct_size = sizeof(o);
   }
   if (!(o.threshold >= 0.0f && o.threshold <= 1.0f) || o.top_k > 7 ||
       !std::isfinite(o.p_min) || o.evaluations < 1 || o.evaluations > p->iterations + 1 ||

@@ -1049,7 +1049,8 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
  * judge; the tokens equal those of a run that compares that pass too.  The
  * frame then reports the comparisons run as its evaluations (none for
  * evaluations 1, which also skips the Butteraugli reference side) and a
- * NaN judged norm. */
+ * NaN judged norm.  set_proposal also takes a struct that ends before
+ * final_compare (struct_size), with final_compare at its default. */
 typedef struct {
   size_t struct_size;
   float threshold;       /* block confidence, default 0.3 */

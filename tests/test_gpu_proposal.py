@@ -14,6 +14,8 @@ on):
   the threshold, p_min or another stage's probability;
 * warm start: relative 1e-6.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -314,6 +316,20 @@ def test_unjudged_final_pass_keeps_the_tokens(device):
         assert path.frame.evaluations == evaluations - 1
         assert np.isnan(path.frame.proposal_norm)
         assert _same_tokens(path.tokens(), compared)
+    # a caller's struct from before final_compare: its fields over the
+    # defaults (a compared last pass); a shorter one is refused
+    opts = prop.ProposalOptions.defaults()
+    opts.evaluations, opts.judge_ratio, opts.final_compare = 2, 100.0, 0
+    opts.struct_size = prop.ProposalOptions.final_compare.offset
+    lib = gjxl_amd.library()
+    gjxl_amd._check(lib.gjxl_amd_plain_path_set_proposal(path.handle, net.handle,
+                                                         ctypes.byref(opts)))
+    path.run(img)
+    assert path.frame.evaluations == 2
+    opts.struct_size = prop.ProposalOptions.judge_ratio.offset
+    with pytest.raises(gjxl_amd.GjxlError, match="too small"):
+        gjxl_amd._check(lib.gjxl_amd_plain_path_set_proposal(path.handle, net.handle,
+                                                             ctypes.byref(opts)))
     path.set_proposal(None)
     path.run(img)
     assert _same_tokens(path.tokens(), plain)
