@@ -1048,8 +1048,9 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_warm_start(GJXLAmdDevice* device,
  * quantization the tokens read) without reconstruction, comparison or
  * judge; the tokens equal those of a run that compares that pass too.  The
  * frame then reports the comparisons run as its evaluations (none for
- * evaluations 1, which also skips the Butteraugli reference side) and a
- * NaN judged norm.  set_proposal also takes a struct that ends before
+ * evaluations 1, which also skips the Butteraugli reference side), a NaN
+ * judged norm, and the block distances and reconstruction of its last
+ * comparison (with none, those of an earlier run).  set_proposal also takes a struct that ends before
  * final_compare (struct_size), with final_compare at its default. */
 typedef struct {
   size_t struct_size;
