@@ -242,19 +242,33 @@ GJXLResult RunFront(GJXLAmdPlainPath* p, const uint8_t* srgb, size_t row_stride,
   const float* head = nullptr;
   if (p->proposal != nullptr) {
     // the learned form: features, the network's forward and the allowed
-    // stages of every block
-    GJXL_AMD_TRY(gjxl_amd::LaunchProposalFeatures(d, p->preprocessed, p->pw, p->ph,
+    // stages of every block, on this path's streams whatever handle the
+    // network was created on; with no Butteraugli reference side to run,
+    // on the side stream beside the CfL
+    GJXLAmdDevice* pd = compares ? d : p->side;
+    if (!compares) {
+      GJXL_AMD_TRY(Check(hipEventRecord(p->linear_ready, d->stream), "hipEventRecord"));
+      GJXL_AMD_TRY(Check(hipStreamWaitEvent(pd->stream, p->linear_ready, 0),
+                         "hipStreamWaitEvent"));
+    }
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalFeatures(pd, p->preprocessed, p->pw, p->ph,
                                                   p->quant_field, p->features));
-    GJXL_AMD_TRY(gjxl_amd_proposal_forward(p->proposal, p->features, 1, p->bh, p->bw));
-    void* hp = nullptr;
-    GJXL_AMD_TRY(gjxl_amd_proposal_buffer(p->proposal, GJXL_AMD_PROPOSAL_HEAD, &hp, nullptr));
-    head = static_cast<const float*>(hp);
-    GJXL_AMD_TRY(gjxl_amd::LaunchProposalDecide(d, head, p->bw, p->bh, po.threshold,
+    GJXL_AMD_TRY(gjxl_amd::ProposalForward(p->proposal, pd->stream, p->features, 1, p->bh,
+                                           p->bw, &head));
+    GJXL_AMD_TRY(gjxl_amd::LaunchProposalDecide(pd, head, p->bw, p->bh, po.threshold,
                                                 static_cast<int>(po.top_k), po.p_min, p->allow));
+    if (!compares) {
+      GJXL_AMD_TRY(Check(hipEventRecord(p->reference_ready, pd->stream), "hipEventRecord"));
+    }
     GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_PROPOSAL));
   }
   GJXL_AMD_TRY(gjxl_amd_initial_cfl(d, p->preprocessed, p->pw, p->ph, p->y_to_x, p->y_to_b));
   GJXL_AMD_TRY(Mark(p, GJXL_AMD_STAGE_INITIAL_CFL));
+  if (!compares) {
+    // the decisions, which the candidate costs read
+    GJXL_AMD_TRY(Check(hipStreamWaitEvent(d->stream, p->reference_ready, 0),
+                       "hipStreamWaitEvent"));
+  }
   // what is left of the reference side once the device stream gets here:
   // the side stream shares the GPU with the latency-bound stages before the
   // candidate costs, which the costs themselves would not leave room for
