@@ -1068,7 +1068,10 @@ GJXL_AMD_API GJXLResult gjxl_amd_proposal_options_init(GJXLAmdProposalOptions* o
                                                        size_t struct_size);
 /* Attaches (net non-null; it must outlive the attachment and hold
  * max_positions >= the path's block count) or detaches (null) the
- * network.  Not while a run is in progress. */
+ * network.  Not while a run is in progress.  A run uses the network on
+ * the path's own streams, whatever handle created it: work enqueued on the
+ * network's handle (gjxl_amd_proposal_adam and the like) must be complete
+ * before the run (gjxl_amd_proposal_set_parameters waits for its own). */
 GJXL_AMD_API GJXLResult gjxl_amd_plain_path_set_proposal(GJXLAmdPlainPath* path,
                                                          GJXLAmdProposal* net,
                                                          const GJXLAmdProposalOptions* options);

@@ -290,6 +290,32 @@ def test_learned_run_defaults_and_judge(device):
     path.close()
 
 
+def test_network_on_another_handle(device):
+    """A network created on another device handle (its own stream) runs on
+    the path's streams: the learned run's tokens equal those with the
+    network on the path's own handle."""
+    w, h = 512, 512
+    img = gjxl_amd.synthetic.generate(**gjxl_amd.synthetic.WORKLOADS["512x512"])
+    path = gjxl_amd.NativePlainPath(device, w, h)
+    params = prop.load_parameters()
+    own = prop.ProposalNet(device, path.bw * path.bh, params)
+    path.set_proposal(own, evaluations=2, judge_ratio=100.0)
+    path.run(img)
+    want = path.tokens()
+    other_device = gjxl_amd.Device(0)
+    other = prop.ProposalNet(other_device, path.bw * path.bh, params)
+    for final_compare in (1, 0):
+        print(f"network on another handle, final_compare {final_compare}", flush=True)
+        path.set_proposal(other, evaluations=2, judge_ratio=100.0, final_compare=final_compare)
+        path.run(img)
+        assert _same_tokens(path.tokens(), want)
+    path.set_proposal(None)
+    other.close()
+    other_device.close()
+    own.close()
+    path.close()
+
+
 def _same_tokens(a, b):
     return all(np.array_equal(a[k], b[k]) for k in ("values", "contexts", "group_offsets"))
 
