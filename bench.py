@@ -653,9 +653,9 @@ def main() -> int:
         return 0
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (images and tiles are independent; see DESIGN.md for how "
-              "the path would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU "
+              "path (images and tiles are independent, so it would shard by "
+              "image, or by tile for one large image).", file=sys.stderr)
         return 2
 
     import gjxl_amd
